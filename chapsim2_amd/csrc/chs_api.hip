@@ -1,0 +1,313 @@
+// extern "C" entry points of libchapsim_hip (declared in include/chapsim_hip.h).
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "chs_flow.h"
+#include "chs_handle.h"
+#include "chs_poisson.h"
+
+namespace chs {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+std::string DevBuf::ensure(size_t b) {
+  if (b <= bytes) return "";
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) return std::string("device allocation: ") + hipGetErrorString(e);
+  bytes = b;
+  return "";
+}
+
+Handle::~Handle() {
+  flow.reset();
+  poisson.reset();
+  plans64.clear();
+  plans32.clear();
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+template <typename T>
+static std::map<PlanKey, std::unique_ptr<DevPlan<T>>>& plan_map(Handle* h);
+template <>
+std::map<PlanKey, std::unique_ptr<DevPlan<double>>>& plan_map<double>(Handle* h) { return h->plans64; }
+template <>
+std::map<PlanKey, std::unique_ptr<DevPlan<float>>>& plan_map<float>(Handle* h) { return h->plans32; }
+
+template <typename T>
+std::string Handle::plan(int dir, int op, int iacc, const int ibc[2], bool has_fbc, int n_in, int n_out,
+                         const DevPlan<T>** out) {
+  PlanKey key{dir, op, iacc, ibc[0], ibc[1], has_fbc ? 1 : 0, n_in, n_out};
+  auto& mp = plan_map<T>(this);
+  auto it = mp.find(key);
+  if (it != mp.end()) {
+    *out = it->second.get();
+    return "";
+  }
+  LinePlan lp;
+  std::string e = make_line_plan(mesh.line_geom(dir), Op(op), iacc, ibc, has_fbc, n_in, n_out, &lp);
+  if (!e.empty()) return e;
+  std::unique_ptr<DevPlan<T>> dp(new DevPlan<T>());
+  e = upload_plan<T>(lp, dp.get());
+  if (!e.empty()) return e;
+  *out = dp.get();
+  mp[key] = std::move(dp);
+  return "";
+}
+template std::string Handle::plan<double>(int, int, int, const int[2], bool, int, int, const DevPlan<double>**);
+template std::string Handle::plan<float>(int, int, int, const int[2], bool, int, int, const DevPlan<float>**);
+
+}  // namespace chs
+
+using chs::set_error;
+
+static int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+#define API_CHECK_HANDLE(h) \
+  if (!(h)) return fail(CHS_EINVAL, "null handle")
+
+extern "C" {
+
+int chs_abi_version(void) { return CHS_ABI_VERSION; }
+
+const char* chs_last_error(void) { return chs::g_err.c_str(); }
+
+int chs_create(const chs_desc_t* desc, chs_handle_t** out) {
+  if (!desc || !out) return fail(CHS_EINVAL, "null argument");
+  *out = nullptr;
+  for (int d = 0; d < 3; ++d)
+    if (desc->nc[d] < 8) return fail(CHS_EINVAL, "each direction needs at least 8 cells");
+  if (desc->iacc < CHS_CD2 || desc->iacc > CHS_CP6) return fail(CHS_EINVAL, "invalid accuracy");
+  if (desc->istret != 0 && desc->istret != 2 && desc->istret != 3 && desc->istret != 4)
+    return fail(CHS_EINVAL, "invalid istret");
+  if (desc->precision != CHS_PREC_FP64 && desc->precision != CHS_PREC_MIXED)
+    return fail(CHS_EINVAL, "invalid precision");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CHS_EHIP, "no HIP device");
+  chs_handle_t* H = new chs_handle_t();
+  chs::Handle& h = H->h;
+  h.desc = *desc;
+  (void)hipGetDevice(&h.device);
+  if (hipStreamCreate(&h.stream) != hipSuccess) {
+    delete H;
+    return fail(CHS_EHIP, "stream creation failed");
+  }
+  h.mesh.build(desc->nc, desc->periodic_y != 0, desc->lxx, desc->lyb, desc->lyt, desc->lzz, desc->istret,
+               desc->rstret);
+  if (!h.mesh.periodic[1] && desc->poisson != CHS_POISSON_FISHPACK) {
+    delete H;
+    return fail(CHS_EUNSUP, "walls in y need the FFT(x,z)+TDMA(y) Poisson path");
+  }
+  std::string e;
+  if (desc->poisson == CHS_POISSON_FISHPACK) {
+    e = chs::Poisson::create(h.mesh, 2, &h.poisson);
+    if (!e.empty()) {
+      delete H;
+      return fail(CHS_EUNSUP, e);
+    }
+    e = chs::Flow::create(&h, &h.flow);
+    if (!e.empty()) {
+      delete H;
+      return fail(CHS_ENOMEM, e);
+    }
+  }
+  *out = H;
+  return CHS_OK;
+}
+
+int chs_destroy(chs_handle_t* h) {
+  if (!h) return CHS_OK;
+  (void)hipStreamSynchronize(h->h.stream);
+  delete h;
+  return CHS_OK;
+}
+
+int chs_sync(chs_handle_t* h) {
+  API_CHECK_HANDLE(h);
+  hipError_t e = hipStreamSynchronize(h->h.stream);
+  if (e != hipSuccess) return fail(CHS_EHIP, hipGetErrorString(e));
+  return CHS_OK;
+}
+
+int chs_op3d(chs_handle_t* H, int dir, int op, int iacc, const int ibc[2], const double* fi, const int shp_in[3],
+             double* fo, const int shp_out[3], const double* fbc, int mem) {
+  API_CHECK_HANDLE(H);
+  chs::Handle& h = H->h;
+  if (dir < 0 || dir > 2 || op < 0 || op > 5 || !ibc || !fi || !fo || !shp_in || !shp_out)
+    return fail(CHS_EINVAL, "invalid argument");
+  for (int d = 0; d < 3; ++d)
+    if (shp_in[d] <= 0 || shp_out[d] <= 0) return fail(CHS_EINVAL, "non-positive extent");
+  const chs::DevPlan<double>* P = nullptr;
+  std::string e = h.plan<double>(dir, op, iacc, ibc, fbc != nullptr, shp_in[dir], shp_out[dir], &P);
+  if (!e.empty()) return fail(CHS_EINVAL, e);
+  chs::Ext3 ein{{shp_in[0], shp_in[1], shp_in[2]}}, eout{{shp_out[0], shp_out[1], shp_out[2]}};
+  const size_t nin = (size_t)shp_in[0] * shp_in[1] * shp_in[2];
+  const size_t nout = (size_t)shp_out[0] * shp_out[1] * shp_out[2];
+  const size_t nfbc = (dir == 0) ? 4ul * shp_in[1] * shp_in[2]
+                      : (dir == 1) ? (size_t)shp_in[0] * 4 * shp_in[2]
+                                   : (size_t)shp_in[0] * shp_in[1] * 4;
+  const double *din = fi, *dfbc = fbc;
+  double* dout = fo;
+  if ((e = h.io[2].ensure(sizeof(double) * nout)).size()) return fail(CHS_ENOMEM, e);
+  double* scratch = h.io[2].as<double>();
+  if (mem == CHS_MEM_HOST) {
+    if ((e = h.io[0].ensure(sizeof(double) * nin)).size()) return fail(CHS_ENOMEM, e);
+    if ((e = h.io[1].ensure(sizeof(double) * nout)).size()) return fail(CHS_ENOMEM, e);
+    if ((e = h.io[3].ensure(sizeof(double) * nfbc)).size()) return fail(CHS_ENOMEM, e);
+    if (hipMemcpyAsync(h.io[0].p, fi, sizeof(double) * nin, hipMemcpyHostToDevice, h.stream) != hipSuccess)
+      return fail(CHS_EHIP, "copy in");
+    if (fbc && hipMemcpyAsync(h.io[3].p, fbc, sizeof(double) * nfbc, hipMemcpyHostToDevice, h.stream) != hipSuccess)
+      return fail(CHS_EHIP, "copy fbc");
+    din = h.io[0].as<double>();
+    dout = h.io[1].as<double>();
+    dfbc = fbc ? h.io[3].as<double>() : nullptr;
+  } else if (mem != CHS_MEM_DEVICE) {
+    return fail(CHS_EINVAL, "invalid mem kind");
+  }
+  chs::LineIn<double> in;
+  in.A = din;
+  chs::LineOut<double> o;
+  o.out = dout;
+  e = chs::launch_line_op<double>(*P, ein, eout, in, dfbc, o, scratch, h.stream);
+  if (!e.empty()) return fail(CHS_EINVAL, e);
+  if (mem == CHS_MEM_HOST &&
+      hipMemcpyAsync(fo, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, h.stream) != hipSuccess)
+    return fail(CHS_EHIP, "copy out");
+  hipError_t he = hipStreamSynchronize(h.stream);
+  if (he != hipSuccess) return fail(CHS_EHIP, hipGetErrorString(he));
+  return CHS_OK;
+}
+
+int chs_poisson(chs_handle_t* H, double* rhs, int mem) {
+  API_CHECK_HANDLE(H);
+  chs::Handle& h = H->h;
+  if (!h.poisson) return fail(CHS_EUNSUP, "no Poisson solver for this configuration");
+  const size_t n = (size_t)h.mesh.nc[0] * h.mesh.nc[1] * h.mesh.nc[2];
+  double* d = rhs;
+  std::string e;
+  if (mem == CHS_MEM_HOST) {
+    if ((e = h.io[0].ensure(sizeof(double) * n)).size()) return fail(CHS_ENOMEM, e);
+    if (hipMemcpyAsync(h.io[0].p, rhs, sizeof(double) * n, hipMemcpyHostToDevice, h.stream) != hipSuccess)
+      return fail(CHS_EHIP, "copy in");
+    d = h.io[0].as<double>();
+  }
+  e = h.poisson->solve(d, h.stream);
+  if (!e.empty()) return fail(CHS_EHIP, e);
+  if (mem == CHS_MEM_HOST &&
+      hipMemcpyAsync(rhs, d, sizeof(double) * n, hipMemcpyDeviceToHost, h.stream) != hipSuccess)
+    return fail(CHS_EHIP, "copy out");
+  hipError_t he = hipStreamSynchronize(h.stream);
+  if (he != hipSuccess) return fail(CHS_EHIP, hipGetErrorString(he));
+  return CHS_OK;
+}
+
+#define FLOW_OR_FAIL(H)                                                         \
+  API_CHECK_HANDLE(H);                                                          \
+  if (!(H)->h.flow) return fail(CHS_EUNSUP, "no flow solver for this configuration"); \
+  chs::Flow& F = *(H)->h.flow
+
+#define RET(expr)                                          \
+  do {                                                     \
+    std::string e_ = (expr);                               \
+    if (!e_.empty()) return fail(CHS_EHIP, e_);            \
+  } while (0)
+
+int chs_set_state(chs_handle_t* H, const double* qx, const double* qy, const double* qz, const double* pr,
+                  int mem) {
+  FLOW_OR_FAIL(H);
+  if (!qx || !qy || !qz || !pr) return fail(CHS_EINVAL, "null field");
+  RET(F.set_state(qx, qy, qz, pr, mem));
+  return CHS_OK;
+}
+
+int chs_get_state(chs_handle_t* H, double* qx, double* qy, double* qz, double* pr, int mem) {
+  FLOW_OR_FAIL(H);
+  if (!qx || !qy || !qz || !pr) return fail(CHS_EINVAL, "null field");
+  RET(F.get_state(qx, qy, qz, pr, mem));
+  return CHS_OK;
+}
+
+int chs_update_re(chs_handle_t* H, int iter) {
+  FLOW_OR_FAIL(H);
+  F.update_re(iter);
+  return CHS_OK;
+}
+
+int chs_substep(chs_handle_t* H, int isub) {
+  FLOW_OR_FAIL(H);
+  if (isub < 0 || isub > 3) return fail(CHS_EINVAL, "isub must be 0..3");
+  RET(F.substep(isub));
+  hipError_t he = hipStreamSynchronize(H->h.stream);
+  if (he != hipSuccess) return fail(CHS_EHIP, hipGetErrorString(he));
+  return CHS_OK;
+}
+
+int chs_run(chs_handle_t* H, int iter0, int nsteps) {
+  FLOW_OR_FAIL(H);
+  for (int it = iter0 + 1; it <= iter0 + nsteps; ++it) {
+    F.update_re(it);
+    for (int isub = 1; isub <= 3; ++isub) RET(F.substep(isub));
+  }
+  hipError_t he = hipStreamSynchronize(H->h.stream);
+  if (he != hipSuccess) return fail(CHS_EHIP, hipGetErrorString(he));
+  return CHS_OK;
+}
+
+int chs_momentum_rhs(chs_handle_t* H, int isub, double* mx, double* my, double* mz, int mem) {
+  FLOW_OR_FAIL(H);
+  if (isub < 0 || isub > 3) return fail(CHS_EINVAL, "isub must be 0..3");
+  RET(F.momentum_rhs(isub));
+  RET(F.copy_rhs(mx, my, mz, mem));
+  return CHS_OK;
+}
+
+int chs_mass_check(chs_handle_t* H, double out[3]) {
+  FLOW_OR_FAIL(H);
+  RET(F.mass_check(out));
+  return CHS_OK;
+}
+
+int chs_divergence(chs_handle_t* H, double* div, int mem) {
+  FLOW_OR_FAIL(H);
+  RET(F.divergence(F.div.p));
+  hipMemcpyKind k = (mem == CHS_MEM_HOST) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  if (hipMemcpyAsync(div, F.div.p, sizeof(double) * F.div.size(), k, H->h.stream) != hipSuccess)
+    return fail(CHS_EHIP, "copy out");
+  hipError_t he = hipStreamSynchronize(H->h.stream);
+  if (he != hipSuccess) return fail(CHS_EHIP, hipGetErrorString(he));
+  return CHS_OK;
+}
+
+}  // extern "C"
+
+#include "../../include/chapsim_hip_inspect.h"
+
+extern "C" {
+
+double chs_inspect_coeff(int fam, int is_rhs, int row, int k, int bc, int acc) {
+  if (fam < 0 || fam > 5 || row < 1 || row > 5 || k < 1 || k > (is_rhs ? 6 : 3) || bc < 0 || bc > 6 || acc < 1 ||
+      acc > 4)
+    return NAN;
+  return chs::coeff(chs::Fam(fam), is_rhs != 0, row, k, bc, acc);
+}
+
+int chs_inspect_op1d(const chs_desc_t* desc, int dir, int op, int iacc, const int ibc[2], const double* fi, int n_in,
+                     double* fo, int n_out, const double* fbc) {
+  if (!desc || !ibc || !fi || !fo || dir < 0 || dir > 2 || op < 0 || op > 5) return fail(CHS_EINVAL, "invalid argument");
+  chs::Mesh m;
+  m.build(desc->nc, desc->periodic_y != 0, desc->lxx, desc->lyb, desc->lyt, desc->lzz, desc->istret, desc->rstret);
+  chs::LinePlan p;
+  std::string e = chs::make_line_plan(m.line_geom(dir), chs::Op(op), iacc, ibc, fbc != nullptr, n_in, n_out, &p);
+  if (!e.empty()) return fail(CHS_EINVAL, e);
+  chs::apply_plan_host(p, fi, fbc, fo);
+  return CHS_OK;
+}
+
+}  // extern "C"

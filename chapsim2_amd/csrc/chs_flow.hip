@@ -1,0 +1,550 @@
+// Device-resident momentum RHS, RK3 fractional step, projection and
+// boundary enforcement (Cartesian, constant properties, x and z periodic).
+//
+// Sequence of one sub-step follows Solve_momentum_eq
+// (src/eq_momentum2.f90:1799-1932):
+//   Compute_momentum_rhs (:101-1571) -> q += rhs -> enforce_velo_from_fbc
+//   -> solve_poisson (:1663-1718) -> p += phi -> Correct_massflux (:1577-1660)
+//   -> enforce_velo_from_fbc.
+// Elementwise products of the RHS are fused into the loads of the operator
+// that consumes them; each directional contribution accumulates in place in
+// its own array, and the three are summed in the reference's order by the
+// fractional-step kernel (Calculate_momentum_fractional_step, :35-71).
+#include "chs_flow.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace chs {
+
+#define FCHK(x)                                                             \
+  do {                                                                      \
+    hipError_t e_ = (x);                                                    \
+    if (e_ != hipSuccess) return std::string(#x ": ") + hipGetErrorString(e_); \
+  } while (0)
+#define SCHK(x)                      \
+  do {                               \
+    std::string s_ = (x);            \
+    if (!s_.empty()) return s_;      \
+  } while (0)
+
+static const double ONE_THIRD = 1.0 / 3.0;
+enum { VU = 0, VV = 1, VW = 2, VP = 3, VT = 4 };
+
+// ---------------------------------------------------------------------------
+// elementwise kernels
+// ---------------------------------------------------------------------------
+__global__ void k_div3(double* __restrict__ d, const double* __restrict__ a, const double* __restrict__ b,
+                       const double* __restrict__ c, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double v = 0.0;
+    v = v + a[i];
+    v = v + b[i];
+    v = v + c[i];
+    d[i] = v;
+  }
+}
+
+// rhs = dt * (tg*((ax + ay) + az) + tz*rhs0 + ta*pfc); rhs0 = (ax + ay) + az.
+// Optionally accumulates the x-pencil volume integral of rhs per block.
+__global__ void k_combine(double* __restrict__ ax, const double* __restrict__ ay, const double* __restrict__ az,
+                          const double* __restrict__ pfc, double* __restrict__ r0, long n, double tg, double tz,
+                          double ta, double dt, int n1, int n2, const double* __restrict__ dyw, double dxdz_x,
+                          double dxdz_z, double* __restrict__ partial) {
+  __shared__ double red[256];
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double w = 0.0;
+  if (i < n) {
+    double cur = ax[i];
+    cur = cur + ay[i];
+    cur = cur + az[i];
+    double tot = tg * cur + tz * r0[i];
+    r0[i] = cur;
+    tot = tot + ta * pfc[i];
+    double rv = dt * tot;
+    ax[i] = rv;
+    if (partial) {
+      int j = (int)((i / n1) % n2);
+      w = rv * dyw[j] * dxdz_x * dxdz_z;
+    }
+  }
+  if (partial) {
+    red[threadIdx.x] = w;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  }
+}
+
+// sum the block partials in a fixed order; bulk = sum / vol
+__global__ void k_finish_sum(const double* __restrict__ partial, int np, double vol, double* __restrict__ bulk) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) s = s + partial[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bulk[0] = red[0] / vol;
+}
+
+// q = q + (rhs - bulk); y walls of qy set from fby (enforce_velo_from_fbc)
+__global__ void k_update(double* __restrict__ q, const double* __restrict__ rhs, const double* __restrict__ bulk,
+                         long n, int n1, int n2, const double* __restrict__ fbw) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double r = rhs[i];
+  if (bulk) r = r - bulk[0];
+  double v = q[i] + r;
+  if (fbw) {
+    long i1 = i % n1, rest = i / n1;
+    long j = rest % n2, k = rest / n2;
+    if (j == 0) v = fbw[i1 + (long)n1 * (0 + 4 * k)];
+    else if (j == n2 - 1) v = fbw[i1 + (long)n1 * (1 + 4 * k)];
+  }
+  q[i] = v;
+}
+
+__global__ void k_sub_bulk(double* __restrict__ o, const double* __restrict__ a, const double* __restrict__ bulk,
+                           long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = a[i] - bulk[0];
+}
+
+__global__ void k_add(double* __restrict__ a, const double* __restrict__ b, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = a[i] + b[i];
+}
+
+__global__ void k_enforce_walls(double* __restrict__ q, int n1, int n2, int n3, const double* __restrict__ fbw) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)n1 * n3) return;
+  long i = t % n1, k = t / n1;
+  q[i + (long)n1 * (0 + (long)n2 * k)] = fbw[i + (long)n1 * (0 + 4 * k)];
+  q[i + (long)n1 * ((n2 - 1) + (long)n2 * k)] = fbw[i + (long)n1 * (1 + 4 * k)];
+}
+
+// planes (n1,2,n3) <- a(:,0,:), a(:,n2-1,:)
+__global__ void k_extract_walls(double* __restrict__ pl, const double* __restrict__ a, int n1, int n2, int n3) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)n1 * n3) return;
+  long i = t % n1, k = t / n1;
+  pl[i + (long)n1 * (0 + 2 * k)] = a[i + (long)n1 * (0 + (long)n2 * k)];
+  pl[i + (long)n1 * (1 + 2 * k)] = a[i + (long)n1 * ((n2 - 1) + (long)n2 * k)];
+}
+
+// div and visc wall values (nx,4,nz) from the three wall-plane derivatives
+__global__ void k_wall_fbc(double* __restrict__ fdiv, double* __restrict__ fvisc, const double* __restrict__ dxw,
+                           const double* __restrict__ dyw, const double* __restrict__ dzw, int n1, int n3) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)n1 * n3) return;
+  long i = t % n1, k = t / n1;
+  for (int s = 0; s < 2; ++s) {
+    long p = i + (long)n1 * (s + 2 * k);
+    double dv = dxw[p] + dyw[p] + dzw[p];
+    double vs = (dyw[p] - ONE_THIRD * dv) * 2.0 * 1.0;
+    fdiv[i + (long)n1 * (s + 4 * k)] = dv;
+    fdiv[i + (long)n1 * (s + 2 + 4 * k)] = dv;
+    fvisc[i + (long)n1 * (s + 4 * k)] = vs;
+    fvisc[i + (long)n1 * (s + 2 + 4 * k)] = vs;
+  }
+}
+
+__global__ void k_neg_prod(double* __restrict__ o, const double* __restrict__ a, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = -a[i] * a[i];
+}
+__global__ void k_neg(double* __restrict__ o, const double* __restrict__ a, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = -a[i];
+}
+
+// max |div| over x windows [lo, hi] (0-based inclusive), per block partials
+__global__ void k_maxabs_xwin(const double* __restrict__ d, int n1, long n, int lo, int hi,
+                              double* __restrict__ partial) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x = (int)(i % n1);
+    if (x >= lo && x <= hi) m = fmax(m, fabs(d[i]));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+static inline dim3 grid1(long n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
+
+// ---------------------------------------------------------------------------
+// bc tables (src/bc_general.f90:137-245, 455-757), Cartesian subset
+// ---------------------------------------------------------------------------
+static void bc_symm(const int ibc[2], const int* jbc, int mbc[2][3]) {
+  for (int i = 0; i < 2; ++i) {
+    mbc[i][0] = mbc[i][1] = mbc[i][2] = ibc[i];
+    if (jbc) {
+      const int a = ibc[i], b = jbc[i];
+      if (a == BC_SYMMETRIC && b == BC_SYMMETRIC) mbc[i][2] = BC_SYMMETRIC;
+      else if (a == BC_SYMMETRIC && b == BC_ASYMMETRIC) mbc[i][2] = BC_ASYMMETRIC;
+      else if (a == BC_ASYMMETRIC && b == BC_SYMMETRIC) mbc[i][2] = BC_ASYMMETRIC;
+      else if (a == BC_ASYMMETRIC && b == BC_ASYMMETRIC) mbc[i][2] = BC_SYMMETRIC;
+      else if (a != b) {
+        if (a == BC_DIRICHLET) mbc[i][0] = mbc[i][1] = mbc[i][2] = a;
+        if (b == BC_DIRICHLET) mbc[i][0] = mbc[i][1] = mbc[i][2] = b;
+      }
+    } else {
+      if (ibc[i] == BC_SYMMETRIC) mbc[i][1] = BC_ASYMMETRIC;
+      else if (ibc[i] == BC_ASYMMETRIC) { mbc[i][1] = BC_SYMMETRIC; mbc[i][2] = BC_SYMMETRIC; }
+    }
+  }
+}
+static void prod_bc(const int a[2], const int b[2], int o[2]) {
+  int m[2][3];
+  bc_symm(a, b, m);
+  o[0] = m[0][2]; o[1] = m[1][2];
+}
+static void tau_bc(const int ftp[2], const int q[2], int o[2]) {
+  int m[2][3], g[2];
+  bc_symm(q, nullptr, m);
+  g[0] = m[0][1]; g[1] = m[1][1];
+  bc_symm(ftp, g, m);
+  o[0] = m[0][2]; o[1] = m[1][2];
+}
+
+// ---------------------------------------------------------------------------
+Flow::~Flow() {
+  for (void* p : owned) (void)hipFree(p);
+}
+
+std::string Flow::alloc(Field<double>& f, int e1, int e2, int e3) {
+  f.e = Ext3{{e1, e2, e3}};
+  FCHK(hipMalloc(&f.p, sizeof(double) * (size_t)f.size()));
+  FCHK(hipMemset(f.p, 0, sizeof(double) * (size_t)f.size()));
+  owned.push_back(f.p);
+  return "";
+}
+
+std::string Flow::create(Handle* H, std::unique_ptr<Flow>* out) {
+  std::unique_ptr<Flow> F(new Flow());
+  Flow& f = *F;
+  f.H = H;
+  const Mesh& m = H->mesh;
+  const chs_desc_t& d = H->desc;
+  f.nx = m.nc[0]; f.ny = m.nc[1]; f.nz = m.nc[2];
+  f.px = m.np[0]; f.py = m.np[1]; f.pz = m.np[2];
+  f.wall_y = !m.periodic[1];
+  f.dt = d.dt;
+  f.sigma2p = 1.0;
+  f.tg[0] = 1.0; f.tg[1] = 8.0 / 15.0; f.tg[2] = 5.0 / 12.0; f.tg[3] = 3.0 * 0.25;
+  f.tz[0] = 0.0; f.tz[1] = 0.0; f.tz[2] = -17.0 / 60.0; f.tz[3] = -5.0 / 12.0;
+  for (int i = 0; i < 4; ++i) f.ta[i] = f.tg[i] + f.tz[i];
+  for (int dd = 0; dd < 3; ++dd)
+    for (int v = 0; v < 5; ++v)
+      for (int s = 0; s < 2; ++s) f.ibc[dd][v][s] = BC_PERIODIC;
+  if (f.wall_y)
+    for (int s = 0; s < 2; ++s) {
+      for (int v = 0; v < 3; ++v) f.ibc[1][v][s] = BC_DIRICHLET;
+      f.ibc[1][VT][s] = BC_DIRICHLET;
+      f.ibc[1][VP][s] = BC_NEUMANN;
+    }
+  int ftp[3][2];
+  for (int dd = 0; dd < 3; ++dd)
+    for (int s = 0; s < 2; ++s) ftp[dd][s] = (f.ibc[dd][VT][s] == BC_NEUMANN) ? BC_DIRICHLET : f.ibc[dd][VT][s];
+  for (int c = 0; c < 3; ++c) {
+    // convection: product of the advecting (diagonal) velocity and component c
+    prod_bc(f.ibc[0][VU], f.ibc[0][c], f.mcov[c][0]);
+    prod_bc(f.ibc[1][VV], f.ibc[1][c], f.mcov[c][1]);
+    prod_bc(f.ibc[2][VW], f.ibc[2][c], f.mcov[c][2]);
+    for (int dd = 0; dd < 3; ++dd) tau_bc(ftp[dd], f.ibc[dd][c], f.mtau[c][dd]);
+  }
+  f.update_re(0);
+  // volume weights (Get_volumetric_average_3d_for_var_xcx)
+  f.dyw.assign(f.ny, m.h[1]);
+  if (m.stretched)
+    for (int j = 0; j < f.ny; ++j) f.dyw[j] = m.h[1] / m.ymc[j];
+  {
+    double vol = 0.0;
+    const double dx = m.h[0], dz = m.h[2];
+    for (int j = 0; j < f.ny; ++j)
+      for (int k = 0; k < f.nz; ++k)
+        for (int i = 0; i < f.px; ++i) vol = vol + f.dyw[j] * dx * dz;
+    f.vol = vol;
+  }
+  const int nx = f.nx, ny = f.ny, nz = f.nz, px = f.px, py = f.py, pz = f.pz;
+  SCHK(f.alloc(f.qx, px, ny, nz)); SCHK(f.alloc(f.qy, nx, py, nz)); SCHK(f.alloc(f.qz, nx, ny, pz));
+  SCHK(f.alloc(f.pr, nx, ny, nz)); SCHK(f.alloc(f.pcor, nx, ny, nz));
+  SCHK(f.alloc(f.mx0, px, ny, nz)); SCHK(f.alloc(f.my0, nx, py, nz)); SCHK(f.alloc(f.mz0, nx, ny, pz));
+  SCHK(f.alloc(f.qxix, nx, ny, nz)); SCHK(f.alloc(f.qxdx, nx, ny, nz)); SCHK(f.alloc(f.qxdy, px, py, nz));
+  SCHK(f.alloc(f.qxiy, px, py, nz)); SCHK(f.alloc(f.qxiz, px, ny, pz)); SCHK(f.alloc(f.qxdz, px, ny, pz));
+  SCHK(f.alloc(f.qydx, px, py, nz)); SCHK(f.alloc(f.qyix, px, py, nz)); SCHK(f.alloc(f.qydy, nx, ny, nz));
+  SCHK(f.alloc(f.qyiy, nx, ny, nz)); SCHK(f.alloc(f.qydz, nx, py, pz)); SCHK(f.alloc(f.qyiz, nx, py, pz));
+  SCHK(f.alloc(f.qzdx, px, ny, pz)); SCHK(f.alloc(f.qzix, px, ny, pz)); SCHK(f.alloc(f.qziy, nx, py, pz));
+  SCHK(f.alloc(f.qzdy, nx, py, pz)); SCHK(f.alloc(f.qziz, nx, ny, nz)); SCHK(f.alloc(f.qzdz, nx, ny, nz));
+  SCHK(f.alloc(f.div, nx, ny, nz));
+  const int cext[3][3] = {{px, ny, nz}, {nx, py, nz}, {nx, ny, pz}};
+  for (int c = 0; c < 3; ++c)
+    for (int q = 0; q < 4; ++q) SCHK(f.alloc(f.acc[c][q], cext[c][0], cext[c][1], cext[c][2]));
+  long big = 0;
+  for (auto* fl : {&f.qx, &f.qy, &f.qz, &f.qxdy, &f.qydz, &f.qxiz}) big = std::max(big, fl->size());
+  f.scratch_n = big;
+  FCHK(hipMalloc(&f.scratch, sizeof(double) * (size_t)big));
+  f.owned.push_back(f.scratch);
+  // wall values (nx,4,nz) for u, v, w, p; constant walls as initialise_fbc_flow_given
+  const long nf = (long)nx * 4 * nz;
+  auto wall = [&](double** p, double v0, double v1) -> std::string {
+    std::vector<double> h(nf);
+    for (long t = 0; t < nf; ++t) h[t] = ((t / nx) % 4) % 2 == 0 ? v0 : v1;
+    FCHK(hipMalloc(p, sizeof(double) * nf));
+    FCHK(hipMemcpy(*p, h.data(), sizeof(double) * nf, hipMemcpyHostToDevice));
+    f.owned.push_back(*p);
+    return "";
+  };
+  SCHK(wall(&f.fby_u, d.uwall[0][0], d.uwall[1][0]));
+  SCHK(wall(&f.fby_v, d.uwall[0][1], d.uwall[1][1]));
+  SCHK(wall(&f.fby_w, d.uwall[0][2], d.uwall[1][2]));
+  SCHK(wall(&f.fby_p, 0.0, 0.0));
+  SCHK(wall(&f.fby_pn, -0.0, -0.0));
+  SCHK(wall(&f.fby_conv, 0.0, 0.0));
+  SCHK(wall(&f.fby_visc, 0.0, 0.0));
+  SCHK(wall(&f.fby_div, 0.0, 0.0));
+  for (double** p : {&f.plane_a, &f.plane_b, &f.plane_c, &f.plane_d}) {
+    FCHK(hipMalloc(p, sizeof(double) * (size_t)std::max(px, nx) * 2 * std::max(pz, nz)));
+    f.owned.push_back(*p);
+  }
+  {
+    long nblk = (f.qx.size() + 255) / 256;
+    FCHK(hipMalloc(&f.partial, sizeof(double) * (size_t)(nblk + 8)));
+    f.owned.push_back(f.partial);
+    FCHK(hipMalloc(&f.bulk, sizeof(double) * 8));
+    f.owned.push_back(f.bulk);
+    FCHK(hipMalloc(&f.dyw_dev, sizeof(double) * (size_t)ny));
+    FCHK(hipMemcpy(f.dyw_dev, f.dyw.data(), sizeof(double) * ny, hipMemcpyHostToDevice));
+    f.owned.push_back(f.dyw_dev);
+  }
+  // the conv-y fbc of the y momentum is -fbcy_qy * fbcy_qy
+  hipLaunchKernelGGL(k_neg_prod, grid1(nf), dim3(256), 0, H->stream, f.fby_conv, f.fby_v, nf);
+  hipLaunchKernelGGL(k_neg, grid1(nf), dim3(256), 0, H->stream, f.fby_pn, f.fby_p, nf);
+  FCHK(hipStreamSynchronize(H->stream));
+  *out = std::move(F);
+  return "";
+}
+
+void Flow::update_re(int iter) {
+  const chs_desc_t& d = H->desc;
+  rre = (iter < d.init_re_to) ? 1.0 / d.reninit : 1.0 / d.ren;
+}
+
+std::string Flow::op(int dir, int opk, const int ibc2[2], const Field<double>& A, Field<double>& out,
+                     const double* fbc, double sA, const Field<double>* B, double cB, const Field<double>* Cm,
+                     double sOut, int accumulate) {
+  const DevPlan<double>* P = nullptr;
+  SCHK(H->plan<double>(dir, opk, H->desc.iacc, ibc2, fbc != nullptr, A.e.n[dir], out.e.n[dir], &P));
+  LineIn<double> in;
+  in.A = A.p;
+  in.B = B ? B->p : nullptr;
+  in.Cm = Cm ? Cm->p : nullptr;
+  in.sA = sA;
+  in.cB = cB;
+  LineOut<double> o;
+  o.out = out.p;
+  o.sOut = sOut;
+  o.accumulate = accumulate;
+  if (out.size() > scratch_n) return "flow: scratch too small";
+  return launch_line_op<double>(*P, A.e, out.e, in, fbc, o, scratch, H->stream);
+}
+
+std::string Flow::set_state(const double* hx, const double* hy, const double* hz, const double* hp, int mem) {
+  hipMemcpyKind k = (mem == CHS_MEM_HOST) ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  FCHK(hipMemcpyAsync(qx.p, hx, sizeof(double) * qx.size(), k, H->stream));
+  FCHK(hipMemcpyAsync(qy.p, hy, sizeof(double) * qy.size(), k, H->stream));
+  FCHK(hipMemcpyAsync(qz.p, hz, sizeof(double) * qz.size(), k, H->stream));
+  FCHK(hipMemcpyAsync(pr.p, hp, sizeof(double) * pr.size(), k, H->stream));
+  FCHK(hipMemsetAsync(pcor.p, 0, sizeof(double) * pcor.size(), H->stream));
+  FCHK(hipStreamSynchronize(H->stream));
+  return "";
+}
+
+std::string Flow::get_state(double* hx, double* hy, double* hz, double* hp, int mem) {
+  hipMemcpyKind k = (mem == CHS_MEM_HOST) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  FCHK(hipMemcpyAsync(hx, qx.p, sizeof(double) * qx.size(), k, H->stream));
+  FCHK(hipMemcpyAsync(hy, qy.p, sizeof(double) * qy.size(), k, H->stream));
+  FCHK(hipMemcpyAsync(hz, qz.p, sizeof(double) * qz.size(), k, H->stream));
+  FCHK(hipMemcpyAsync(hp, pr.p, sizeof(double) * pr.size(), k, H->stream));
+  FCHK(hipStreamSynchronize(H->stream));
+  return "";
+}
+
+std::string Flow::copy_rhs(double* mx, double* my, double* mz, int mem) {
+  hipMemcpyKind k = (mem == CHS_MEM_HOST) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  if (H->desc.idriven == 1) {
+    // the x right-hand side after the constant-flux correction (:1547-1551)
+    hipLaunchKernelGGL(k_sub_bulk, grid1(acc[0][0].size()), dim3(256), 0, H->stream, scratch, acc[0][0].p, bulk,
+                       acc[0][0].size());
+    FCHK(hipMemcpyAsync(mx, scratch, sizeof(double) * acc[0][0].size(), k, H->stream));
+  } else {
+    FCHK(hipMemcpyAsync(mx, acc[0][0].p, sizeof(double) * acc[0][0].size(), k, H->stream));
+  }
+  FCHK(hipMemcpyAsync(my, acc[1][0].p, sizeof(double) * acc[1][0].size(), k, H->stream));
+  FCHK(hipMemcpyAsync(mz, acc[2][0].p, sizeof(double) * acc[2][0].size(), k, H->stream));
+  FCHK(hipStreamSynchronize(H->stream));
+  return "";
+}
+
+// Compute_momentum_rhs + Calculate_momentum_fractional_step + forcing.
+// Result: dt-scaled rhs in acc[c][0]; bulk (if forced) in this->bulk.
+std::string Flow::momentum_rhs(int isub) {
+  hipStream_t s = H->stream;
+  const int(*B)[5][2] = ibc;
+  const double* fu = wall_y ? fby_u : nullptr;
+  const double* fv = wall_y ? fby_v : nullptr;
+  const double* fw = wall_y ? fby_w : nullptr;
+  // ---- first-level operators, src/eq_momentum2.f90:347-506 ----
+  SCHK(op(0, MIDP_P2C, B[0][VU], qx, qxix, nullptr));
+  SCHK(op(0, D1_P2C, B[0][VU], qx, qxdx, nullptr));
+  SCHK(op(1, D1_C2P, B[1][VU], qx, qxdy, fu));
+  SCHK(op(1, MIDP_C2P, B[1][VU], qx, qxiy, fu));
+  SCHK(op(2, MIDP_C2P, B[2][VU], qx, qxiz, nullptr));
+  SCHK(op(2, D1_C2P, B[2][VU], qx, qxdz, nullptr));
+  SCHK(op(0, D1_C2P, B[0][VV], qy, qydx, nullptr));
+  SCHK(op(0, MIDP_C2P, B[0][VV], qy, qyix, nullptr));
+  SCHK(op(1, D1_P2C, B[1][VV], qy, qydy, nullptr));
+  SCHK(op(1, MIDP_P2C, B[1][VV], qy, qyiy, fv));
+  SCHK(op(2, D1_C2P, B[2][VV], qy, qydz, nullptr));
+  SCHK(op(2, MIDP_C2P, B[2][VV], qy, qyiz, nullptr));
+  SCHK(op(0, D1_C2P, B[0][VW], qz, qzdx, nullptr));
+  SCHK(op(0, MIDP_C2P, B[0][VW], qz, qzix, nullptr));
+  SCHK(op(1, MIDP_C2P, B[1][VW], qz, qziy, fw));
+  SCHK(op(1, D1_C2P, B[1][VW], qz, qzdy, fw));
+  SCHK(op(2, MIDP_P2C, B[2][VW], qz, qziz, nullptr));
+  SCHK(op(2, D1_P2C, B[2][VW], qz, qzdz, nullptr));
+  hipLaunchKernelGGL(k_div3, grid1(div.size()), dim3(256), 0, s, div.p, qxdx.p, qydy.p, qzdz.p, div.size());
+  // ---- y-wall values of the divergence and of the y-normal stress ----
+  if (wall_y) {
+    const long nf2 = (long)nx * nz;
+    // qxdx at the walls: x-derivative of the wall planes of qxiy
+    hipLaunchKernelGGL(k_extract_walls, grid1((long)px * nz), dim3(256), 0, s, plane_a, qxiy.p, px, py, nz);
+    Field<double> pa{plane_a, Ext3{{px, 2, nz}}}, pb{plane_b, Ext3{{nx, 2, nz}}};
+    SCHK(op(0, D1_P2C, B[0][VU], pa, pb, nullptr));
+    // qydy at the walls: y-derivative C2P of qyiy (Dirichlet rows are decoupled)
+    {
+      const DevPlan<double>* P = nullptr;
+      SCHK(H->plan<double>(1, D1_C2P, H->desc.iacc, B[1][VV], true, ny, py, &P));
+      SCHK(launch_wall_rows<double>(*P, qyiy.e, qyiy.p, fby_v, plane_c, s));
+    }
+    // qzdz at the walls: z-derivative of the wall planes of qziy
+    hipLaunchKernelGGL(k_extract_walls, grid1((long)nx * pz), dim3(256), 0, s, plane_a, qziy.p, nx, py, pz);
+    Field<double> pa2{plane_a, Ext3{{nx, 2, pz}}}, pd{plane_d, Ext3{{nx, 2, nz}}};
+    SCHK(op(2, D1_P2C, B[2][VW], pa2, pd, nullptr));
+    hipLaunchKernelGGL(k_wall_fbc, grid1(nf2), dim3(256), 0, s, fby_div, fby_visc, plane_b, plane_c, plane_d, nx,
+                       nz);
+  }
+  const double r = rre;
+  // ---- x-momentum, :798-948 ----
+  Field<double>* X = acc[0];
+  SCHK(op(0, D1_C2P, mcov[0][0], qxix, X[0], nullptr, -1.0, nullptr, 1.0, &qxix));
+  SCHK(op(1, D1_P2C, mcov[0][1], qyix, X[1], nullptr, -1.0, nullptr, 1.0, &qxiy));
+  SCHK(op(2, D1_P2C, mcov[0][2], qzix, X[2], nullptr, -1.0, nullptr, 1.0, &qxiz));
+  SCHK(op(0, D1_C2P, B[0][VP], pr, X[3], nullptr, -1.0));
+  SCHK(op(0, D1_C2P, mtau[0][0], qxdx, X[0], nullptr, 2.0 * 1.0, &div, -ONE_THIRD, nullptr, r, 1));
+  SCHK(op(1, D1_P2C, mtau[0][1], qxdy, X[1], nullptr, 1.0, &qydx, 1.0, nullptr, r, 1));
+  SCHK(op(2, D1_P2C, mtau[0][2], qxdz, X[2], nullptr, 1.0, &qzdx, 1.0, nullptr, r, 1));
+  // ---- y-momentum, :960-1235 ----
+  Field<double>* Y = acc[1];
+  SCHK(op(0, D1_P2C, mcov[1][0], qxiy, Y[0], nullptr, -1.0, nullptr, 1.0, &qyix));
+  SCHK(op(1, D1_C2P, mcov[1][1], qyiy, Y[1], wall_y ? fby_conv : nullptr, -1.0, nullptr, 1.0, &qyiy));
+  SCHK(op(2, D1_P2C, mcov[1][2], qyiz, Y[2], nullptr, -1.0, nullptr, 1.0, &qziy));
+  SCHK(op(1, D1_C2P, B[1][VP], pr, Y[3], wall_y ? fby_pn : nullptr, -1.0));
+  SCHK(op(0, D1_P2C, mtau[1][0], qxdy, Y[0], nullptr, 1.0, &qydx, 1.0, nullptr, r, 1));
+  SCHK(op(1, D1_C2P, mtau[1][1], qydy, Y[1], wall_y ? fby_visc : nullptr, 2.0, &div, -ONE_THIRD, nullptr, r, 1));
+  SCHK(op(2, D1_P2C, mtau[1][2], qzdy, Y[2], nullptr, 1.0, &qydz, 1.0, nullptr, r, 1));
+  // ---- z-momentum, :1248-1480 ----
+  Field<double>* Z = acc[2];
+  SCHK(op(0, D1_P2C, mcov[2][0], qxiz, Z[0], nullptr, -1.0, nullptr, 1.0, &qzix));
+  SCHK(op(1, D1_P2C, mcov[2][1], qyiz, Z[1], nullptr, -1.0, nullptr, 1.0, &qziy));
+  SCHK(op(2, D1_C2P, mcov[2][2], qziz, Z[2], nullptr, -1.0, nullptr, 1.0, &qziz));
+  SCHK(op(2, D1_C2P, B[2][VP], pr, Z[3], nullptr, -1.0));
+  SCHK(op(0, D1_P2C, mtau[2][0], qxdz, Z[0], nullptr, 1.0, &qzdx, 1.0, nullptr, r, 1));
+  SCHK(op(1, D1_P2C, mtau[2][1], qzdy, Z[1], nullptr, 1.0, &qydz, 1.0, nullptr, r, 1));
+  SCHK(op(2, D1_C2P, mtau[2][2], qzdz, Z[2], nullptr, 2.0, &div, -ONE_THIRD, nullptr, r, 1));
+  // ---- fractional step and forcing ----
+  const Mesh& m = H->mesh;
+  const bool forced = (H->desc.idriven == 1);
+  Field<double>* rhs0[3] = {&mx0, &my0, &mz0};
+  for (int c = 0; c < 3; ++c) {
+    long n = acc[c][0].size();
+    double* part = (c == 0 && forced) ? partial : nullptr;
+    hipLaunchKernelGGL(k_combine, grid1(n), dim3(256), 0, s, acc[c][0].p, acc[c][1].p, acc[c][2].p, acc[c][3].p,
+                       rhs0[c]->p, n, tg[isub], tz[isub], ta[isub], dt, acc[c][0].e.n[0], acc[c][0].e.n[1],
+                       dyw_dev, m.h[0], m.h[2], part);
+  }
+  if (forced) {
+    int nb = (int)((acc[0][0].size() + 255) / 256);
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, s, partial, nb, vol, bulk);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("momentum_rhs: ") + hipGetErrorString(e);
+  return "";
+}
+
+std::string Flow::divergence(double* d) {
+  Field<double> D{d, Ext3{{nx, ny, nz}}};
+  SCHK(op(0, D1_P2C, ibc[0][VU], qx, D, nullptr));
+  SCHK(op(1, D1_P2C, ibc[1][VV], qy, D, nullptr, 1.0, nullptr, 1.0, nullptr, 1.0, 1));
+  SCHK(op(2, D1_P2C, ibc[2][VW], qz, D, nullptr, 1.0, nullptr, 1.0, nullptr, 1.0, 1));
+  return "";
+}
+
+std::string Flow::substep(int isub) {
+  hipStream_t s = H->stream;
+  SCHK(momentum_rhs(isub));
+  const bool forced = (H->desc.idriven == 1);
+  hipLaunchKernelGGL(k_update, grid1(qx.size()), dim3(256), 0, s, qx.p, acc[0][0].p, forced ? bulk : nullptr,
+                     qx.size(), px, ny, (const double*)nullptr);
+  hipLaunchKernelGGL(k_update, grid1(qy.size()), dim3(256), 0, s, qy.p, acc[1][0].p, (const double*)nullptr,
+                     qy.size(), nx, py, wall_y ? fby_v : nullptr);
+  hipLaunchKernelGGL(k_update, grid1(qz.size()), dim3(256), 0, s, qz.p, acc[2][0].p, (const double*)nullptr,
+                     qz.size(), nx, ny, (const double*)nullptr);
+  // Poisson RHS: div(q) / (alpha * sigma2 * dt), then the solve
+  SCHK(divergence(pcor.p));
+  const double coeff = 1.0 / (ta[isub] * sigma2p * dt);
+  SCHK(H->poisson->solve(pcor.p, s, coeff));
+  hipLaunchKernelGGL(k_add, grid1(pr.size()), dim3(256), 0, s, pr.p, pcor.p, pr.size());
+  // Correct_massflux
+  const double sc = dt * ta[isub] * sigma2p;
+  SCHK(op(0, D1_C2P, ibc[0][VP], pcor, qx, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1));
+  SCHK(op(1, D1_C2P, ibc[1][VP], pcor, qy, wall_y ? fby_p : nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1));
+  SCHK(op(2, D1_C2P, ibc[2][VP], pcor, qz, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1));
+  if (wall_y)
+    hipLaunchKernelGGL(k_enforce_walls, grid1((long)nx * nz), dim3(256), 0, s, qy.p, nx, py, nz, fby_v);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("substep: ") + hipGetErrorString(e);
+  return "";
+}
+
+std::string Flow::mass_check(double out[3]) {
+  SCHK(divergence(div.p));
+  const int lo[3] = {0, 4, nx - 6}, hi[3] = {3, nx - 5, nx - 1};
+  const int nb = 512;
+  std::vector<double> h(nb);
+  for (int w = 0; w < 3; ++w) {
+    hipLaunchKernelGGL(k_maxabs_xwin, dim3(nb), dim3(256), 0, H->stream, div.p, nx, div.size(), lo[w], hi[w],
+                       partial);
+    FCHK(hipMemcpyAsync(h.data(), partial, sizeof(double) * nb, hipMemcpyDeviceToHost, H->stream));
+    FCHK(hipStreamSynchronize(H->stream));
+    double mx = 0.0;
+    for (double v : h) mx = std::max(mx, v);
+    out[w] = mx;
+  }
+  return "";
+}
+
+}  // namespace chs

@@ -1,0 +1,76 @@
+// Device-side line operator: applies a planned compact operator (explicit
+// RHS stencil + pre-factored tridiagonal solve) to every line of a 3-D array
+// in one direction, with a fused elementwise input transform and a fused
+// accumulating output.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+
+#include "chs_plan.h"
+
+namespace chs {
+
+template <typename T>
+struct DevRow {
+  int nterm;
+  int idx[kMaxRowTerms];
+  T w[kMaxRowTerms];
+  T u[4];
+};
+
+// Device copy of a LinePlan in precision T.
+template <typename T>
+struct DevPlan {
+  LinePlan host;
+  DevRow<T>* rows = nullptr;  // 4 rows
+  T* a = nullptr;             // n_out each
+  T* c = nullptr;
+  T* d = nullptr;
+  T* pw = nullptr;
+  T* x1 = nullptr;
+  T* metric = nullptr;
+  ~DevPlan();
+};
+
+template <typename T>
+std::string upload_plan(const LinePlan& p, DevPlan<T>* dp);
+
+// Input transform f = sA * (A + cB * B) [* Cm]; B and Cm optional.
+template <typename T>
+struct LineIn {
+  const T* A = nullptr;
+  const T* B = nullptr;
+  const T* Cm = nullptr;
+  T sA = T(1);
+  T cB = T(1);
+};
+
+// Output out = (accumulate ? out : 0) + value * sOut
+template <typename T>
+struct LineOut {
+  T* out = nullptr;
+  T sOut = T(1);
+  int accumulate = 0;
+};
+
+// Extents of the input / output arrays (Fortran order, x fastest).
+struct Ext3 {
+  int n[3];
+};
+
+template <typename T>
+std::string launch_line_op(const DevPlan<T>& P, const Ext3& ein, const Ext3& eout, const LineIn<T>& in,
+                           const T* fbc, const LineOut<T>& out, T* scratch, hipStream_t s);
+
+// Evaluate only the first and last output rows of a y-direction plan on every
+// (x, z) line, writing planes of extents (e1, 2, e3).  Valid when the plan's
+// end rows are decoupled from the line (explicit rows, or LHS rows (0,1,0)),
+// as for Dirichlet walls with compact schemes.
+template <typename T>
+std::string launch_wall_rows(const DevPlan<T>& P, const Ext3& ein, const T* in, const T* fbc, T* planes,
+                             hipStream_t s);
+
+}  // namespace chs

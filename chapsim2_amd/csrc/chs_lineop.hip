@@ -1,0 +1,314 @@
+// Compact line operators on gfx950.
+//
+// One thread owns one line.  For y- and z-lines consecutive threads own
+// consecutive x positions, so every load and store of a wavefront is a
+// contiguous 64-element segment.  The explicit RHS (src/basics_operations2.f90
+// :1980-2719) is evaluated on the fly with a 5-point register window; the four
+// boundary rows use the affine recipes built on the host.  The forward Thomas
+// sweep (src/basics_algorithms.f90:36-66) writes its intermediate values to a
+// scratch array laid out like the output; the backward sweep reads them back
+// (the round trip is served by L2/MALL) and writes the final value through the
+// fused output transform.  Periodic lines use the Sherman-Morrison form of
+// Solve_TDMA_cyclic (:68-90) with the RHS-independent correction vector
+// precomputed, and the basic solution's first entry accumulated during the
+// forward sweep so the correction is applied in the same backward pass.
+#include "chs_lineop.h"
+
+#include <vector>
+
+namespace chs {
+
+#define CHS_CHECK(x)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (x);                                                    \
+    if (e_ != hipSuccess) return std::string(#x ": ") + hipGetErrorString(e_); \
+  } while (0)
+
+template <typename T>
+DevPlan<T>::~DevPlan() {
+  for (void* p : {(void*)rows, (void*)a, (void*)c, (void*)d, (void*)pw, (void*)x1, (void*)metric})
+    if (p) (void)hipFree(p);
+}
+
+template <typename T>
+static std::string up(const std::vector<double>& v, T** dst) {
+  if (v.empty()) { *dst = nullptr; return ""; }
+  std::vector<T> t(v.begin(), v.end());
+  CHS_CHECK(hipMalloc(dst, sizeof(T) * t.size()));
+  CHS_CHECK(hipMemcpy(*dst, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice));
+  return "";
+}
+
+template <typename T>
+std::string upload_plan(const LinePlan& p, DevPlan<T>* dp) {
+  dp->host = p;
+  DevRow<T> r[4];
+  for (int q = 0; q < 4; ++q) {
+    r[q].nterm = p.rows[q].nterm;
+    for (int t = 0; t < kMaxRowTerms; ++t) {
+      r[q].idx[t] = p.rows[q].idx[t];
+      r[q].w[t] = T(p.rows[q].w[t]);
+    }
+    for (int m = 0; m < 4; ++m) r[q].u[m] = T(p.rows[q].u[m]);
+  }
+  CHS_CHECK(hipMalloc(&dp->rows, sizeof(r)));
+  CHS_CHECK(hipMemcpy(dp->rows, r, sizeof(r), hipMemcpyHostToDevice));
+  std::string e;
+  if (!(e = up(p.a, &dp->a)).empty()) return e;
+  if (!(e = up(p.c, &dp->c)).empty()) return e;
+  if (!(e = up(p.d, &dp->d)).empty()) return e;
+  if (!(e = up(p.pw, &dp->pw)).empty()) return e;
+  if (!(e = up(p.x1, &dp->x1)).empty()) return e;
+  if (!(e = up(p.metric, &dp->metric)).empty()) return e;
+  return "";
+}
+
+template <typename T>
+struct KArgs {
+  int n_in, n_out;
+  int e1, e2;      // input extents 1,2
+  int o1, o2;      // output extents 1,2
+  long nlines;
+  const T* A; const T* B; const T* Cm; T sA, cB;
+  const T* fbc;
+  T c1, c2, dd, sgn;
+  const DevRow<T>* rows;
+  int solve; T b0;
+  const T* a; const T* c; const T* d; const T* pw; const T* x1;
+  T an, cn, den;
+  const T* metric; int neu_lo, neu_hi;
+  T* out; T sOut; int accumulate;
+  T* scratch;
+};
+
+// bulk offsets (p1,p2,p3,p4) per operator kind, relative to the output row
+template <int OPK> struct Offs;
+template <> struct Offs<MIDP_C2P> { static constexpr int p1 = 0, p2 = -1, p3 = 1, p4 = -2; };
+template <> struct Offs<MIDP_P2C> { static constexpr int p1 = 0, p2 = 1, p3 = -1, p4 = 2; };
+template <> struct Offs<D1_C2C> { static constexpr int p1 = 1, p2 = -1, p3 = 2, p4 = -2; };
+template <> struct Offs<D1_P2P> { static constexpr int p1 = 1, p2 = -1, p3 = 2, p4 = -2; };
+template <> struct Offs<D1_C2P> { static constexpr int p1 = 0, p2 = -1, p3 = 1, p4 = -2; };
+template <> struct Offs<D1_P2C> { static constexpr int p1 = 1, p2 = 0, p3 = 2, p4 = -1; };
+
+template <typename T, int DIR, int OPK>
+__global__ void __launch_bounds__(64) k_line(KArgs<T> P) {
+  const long L = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (L >= P.nlines) return;
+  long ib, is, ob, os, fb, fs;
+  if (DIR == 0) {
+    long j = L % P.e2, k = L / P.e2;
+    ib = (j + (long)P.e2 * k) * P.e1; is = 1;
+    ob = (j + (long)P.o2 * k) * P.o1; os = 1;
+    fb = 4 * (j + (long)P.e2 * k); fs = 1;
+  } else if (DIR == 1) {
+    long i = L % P.e1, k = L / P.e1;
+    ib = i + (long)P.e1 * P.e2 * k; is = P.e1;
+    ob = i + (long)P.o1 * P.o2 * k; os = P.o1;
+    fb = i + (long)P.e1 * 4 * k; fs = P.e1;
+  } else {
+    long i = L % P.e1, j = L / P.e1;
+    ib = i + (long)P.e1 * j; is = (long)P.e1 * P.e2;
+    ob = i + (long)P.o1 * j; os = (long)P.o1 * P.o2;
+    fb = i + (long)P.e1 * j; fs = (long)P.e1 * P.e2;
+  }
+  const int n_in = P.n_in, n_out = P.n_out;
+  auto F = [&](int k) -> T {
+    const long o = ib + is * k;
+    T t = P.A[o];
+    if (P.B) t = t + P.cB * P.B[o];
+    t = P.sA * t;
+    if (P.Cm) t = t * P.Cm[o];
+    return t;
+  };
+  T fb4[4] = {T(0), T(0), T(0), T(0)};
+  if (P.fbc)
+    for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[fb + fs * m];
+  auto special = [&](int q) -> T {
+    const DevRow<T>& R = P.rows[q];
+    T v = T(0);
+    for (int t = 0; t < R.nterm; ++t) v = v + R.w[t] * F(R.idx[t]);
+    if (P.fbc)
+      for (int m = 0; m < 4; ++m) v = v + R.u[m] * fb4[m];
+    return v * P.dd;
+  };
+  auto emit = [&](int r, T val) {
+    if (P.metric) val = val * P.metric[r];
+    if (r == 0 && P.neu_lo) val = fb4[0];
+    if (r == n_out - 1 && P.neu_hi) val = fb4[1];
+    const long o = ob + os * r;
+    T prev = P.accumulate ? P.out[o] : T(0);
+    P.out[o] = prev + val * P.sOut;
+  };
+  using O = Offs<OPK>;
+  const int m = (P.solve == 2) ? n_out - 1 : n_out;
+  T* sc = P.scratch;
+  T y = T(0), S = T(0), rn = T(0);
+  T w0, w1, w2, w3, w4;  // f[r-2 .. r+2]
+  auto clampi = [&](int k) { return k < 0 ? 0 : (k >= n_in ? n_in - 1 : k); };
+  w0 = F(0); w1 = F(clampi(1)); w2 = F(clampi(2)); w3 = F(clampi(3)); w4 = F(clampi(4));
+  auto win = [&](int off) -> T {
+    switch (off) {
+      case -2: return w0;
+      case -1: return w1;
+      case 0: return w2;
+      case 1: return w3;
+      default: return w4;
+    }
+  };
+  for (int r = 0; r < n_out; ++r) {
+    T rv;
+    if (r < 2) {
+      rv = special(r);
+    } else if (r >= n_out - 2) {
+      rv = special(r - n_out + 4);
+    } else {
+      rv = (P.c1 * (win(O::p1) + P.sgn * win(O::p2)) + P.c2 * (win(O::p3) + P.sgn * win(O::p4))) * P.dd;
+      w0 = w1; w1 = w2; w2 = w3; w3 = w4;
+      w4 = F(clampi(r + 3));
+    }
+    if (P.solve == 0) {
+      emit(r, rv);
+    } else if (r < m) {
+      y = (r == 0) ? rv / P.b0 : (rv - P.a[r] * y) * P.d[r];
+      if (P.solve == 2) S = S + P.pw[r] * y;
+      if (r < m - 1) sc[ob + os * r] = y;
+    } else {
+      rn = rv;
+    }
+  }
+  if (P.solve == 0) return;
+  T xn = T(0);
+  if (P.solve == 2) {
+    xn = (rn - P.cn * S - P.an * y) / P.den;
+    emit(n_out - 1, xn);
+  }
+  T xnext = y;
+  emit(m - 1, (P.solve == 2) ? xnext + P.x1[m - 1] * xn : xnext);
+  for (int r = m - 2; r >= 0; --r) {
+    const T xb = sc[ob + os * r] - P.c[r] * xnext;
+    xnext = xb;
+    emit(r, (P.solve == 2) ? xb + P.x1[r] * xn : xb);
+  }
+}
+
+template <typename T, int DIR>
+static void dispatch_op(int op, dim3 g, dim3 b, hipStream_t s, const KArgs<T>& a) {
+  switch (op) {
+    case MIDP_C2P: hipLaunchKernelGGL((k_line<T, DIR, MIDP_C2P>), g, b, 0, s, a); break;
+    case MIDP_P2C: hipLaunchKernelGGL((k_line<T, DIR, MIDP_P2C>), g, b, 0, s, a); break;
+    case D1_C2C: hipLaunchKernelGGL((k_line<T, DIR, D1_C2C>), g, b, 0, s, a); break;
+    case D1_P2P: hipLaunchKernelGGL((k_line<T, DIR, D1_P2P>), g, b, 0, s, a); break;
+    case D1_C2P: hipLaunchKernelGGL((k_line<T, DIR, D1_C2P>), g, b, 0, s, a); break;
+    case D1_P2C: hipLaunchKernelGGL((k_line<T, DIR, D1_P2C>), g, b, 0, s, a); break;
+  }
+}
+
+template <typename T>
+std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eout, const LineIn<T>& in, const T* fbc,
+                           const LineOut<T>& out, T* scratch, hipStream_t s) {
+  const LinePlan& p = D.host;
+  const int dir = p.dir;
+  if (ein.n[dir] != p.n_in || eout.n[dir] != p.n_out) return "line op: extent along the line differs from plan";
+  for (int k = 0; k < 3; ++k)
+    if (k != dir && ein.n[k] != eout.n[k]) return "line op: cross extents of input and output differ";
+  if (p.has_fbc != (fbc != nullptr)) return "line op: fbc presence differs from plan";
+  if (p.solve && !scratch) return "line op: scratch required";
+  KArgs<T> a;
+  a.n_in = p.n_in; a.n_out = p.n_out;
+  a.e1 = ein.n[0]; a.e2 = ein.n[1];
+  a.o1 = eout.n[0]; a.o2 = eout.n[1];
+  a.nlines = (dir == 0) ? (long)ein.n[1] * ein.n[2] : (dir == 1) ? (long)ein.n[0] * ein.n[2] : (long)ein.n[0] * ein.n[1];
+  a.A = in.A; a.B = in.B; a.Cm = in.Cm; a.sA = in.sA; a.cB = in.cB;
+  a.fbc = fbc;
+  a.c1 = T(p.c1); a.c2 = T(p.c2); a.dd = T(p.dd); a.sgn = T(p.sgn);
+  a.rows = D.rows;
+  a.solve = p.solve; a.b0 = T(p.b0);
+  a.a = D.a; a.c = D.c; a.d = D.d; a.pw = D.pw; a.x1 = D.x1;
+  a.an = T(p.cyc_an); a.cn = T(p.cyc_cn); a.den = T(p.cyc_den);
+  a.metric = D.metric; a.neu_lo = p.neu_lo; a.neu_hi = p.neu_hi;
+  a.out = out.out; a.sOut = out.sOut; a.accumulate = out.accumulate;
+  a.scratch = scratch;
+  if (a.nlines <= 0) return "";
+  dim3 b(64), g((unsigned)((a.nlines + 63) / 64));
+  if (dir == 0) dispatch_op<T, 0>(p.op, g, b, s, a);
+  else if (dir == 1) dispatch_op<T, 1>(p.op, g, b, s, a);
+  else dispatch_op<T, 2>(p.op, g, b, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("line op launch: ") + hipGetErrorString(e);
+  return "";
+}
+
+// First and last output rows of a y-direction plan on every (x, z) line.
+template <typename T>
+__global__ void __launch_bounds__(64) k_wall_rows(KArgs<T> P, int e3, T* __restrict__ planes) {
+  const long L = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (L >= (long)P.e1 * e3) return;
+  const long i = L % P.e1, k = L / P.e1;
+  const long ib = i + (long)P.e1 * P.e2 * k, is = P.e1;
+  const long fb = i + (long)P.e1 * 4 * k, fs = P.e1;
+  auto F = [&](int q) -> T { return P.A[ib + is * q]; };
+  T fb4[4] = {T(0), T(0), T(0), T(0)};
+  if (P.fbc)
+    for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[fb + fs * m];
+  for (int side = 0; side < 2; ++side) {
+    const int q = side ? 3 : 0;
+    const int r = side ? P.n_out - 1 : 0;
+    const DevRow<T>& R = P.rows[q];
+    T v = T(0);
+    for (int t = 0; t < R.nterm; ++t) v = v + R.w[t] * F(R.idx[t]);
+    if (P.fbc)
+      for (int m = 0; m < 4; ++m) v = v + R.u[m] * fb4[m];
+    v = v * P.dd;
+    if (P.solve) v = side ? v * P.d[r] : v / P.b0;  // decoupled end rows
+    if (P.metric) v = v * P.metric[r];
+    if (r == 0 && P.neu_lo) v = fb4[0];
+    if (r == P.n_out - 1 && P.neu_hi) v = fb4[1];
+    planes[i + (long)P.e1 * (side + 2 * k)] = v;
+  }
+}
+
+template <typename T>
+std::string launch_wall_rows(const DevPlan<T>& D, const Ext3& ein, const T* in, const T* fbc, T* planes,
+                             hipStream_t s) {
+  const LinePlan& p = D.host;
+  if (p.dir != 1) return "wall rows: y plans only";
+  if (ein.n[1] != p.n_in) return "wall rows: extent differs from plan";
+  if (p.has_fbc != (fbc != nullptr)) return "wall rows: fbc presence differs from plan";
+  if (p.solve == 2) return "wall rows: periodic line";
+  if (p.solve == 1) {
+    const int n = p.n_out;
+    if (!(p.b0 == 1.0 && p.c[0] == 0.0 && p.a[n - 1] == 0.0 && p.d[n - 1] == 1.0))
+      return "wall rows: end rows are coupled to the line";
+  }
+  KArgs<T> a{};
+  a.n_in = p.n_in; a.n_out = p.n_out;
+  a.e1 = ein.n[0]; a.e2 = ein.n[1];
+  a.A = in;
+  a.fbc = fbc;
+  a.dd = T(p.dd);
+  a.rows = D.rows;
+  a.solve = p.solve; a.b0 = T(p.b0);
+  a.d = D.d;
+  a.metric = D.metric; a.neu_lo = p.neu_lo; a.neu_hi = p.neu_hi;
+  const long n = (long)ein.n[0] * ein.n[2];
+  if (n <= 0) return "";
+  hipLaunchKernelGGL(k_wall_rows<T>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, a, ein.n[2], planes);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("wall rows launch: ") + hipGetErrorString(e);
+  return "";
+}
+
+template struct DevPlan<double>;
+template struct DevPlan<float>;
+template std::string upload_plan<double>(const LinePlan&, DevPlan<double>*);
+template std::string upload_plan<float>(const LinePlan&, DevPlan<float>*);
+template std::string launch_line_op<double>(const DevPlan<double>&, const Ext3&, const Ext3&, const LineIn<double>&,
+                                            const double*, const LineOut<double>&, double*, hipStream_t);
+template std::string launch_wall_rows<double>(const DevPlan<double>&, const Ext3&, const double*, const double*,
+                                              double*, hipStream_t);
+template std::string launch_wall_rows<float>(const DevPlan<float>&, const Ext3&, const float*, const float*, float*,
+                                             hipStream_t);
+template std::string launch_line_op<float>(const DevPlan<float>&, const Ext3&, const Ext3&, const LineIn<float>&,
+                                           const float*, const LineOut<float>&, float*, hipStream_t);
+
+}  // namespace chs

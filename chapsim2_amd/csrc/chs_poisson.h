@@ -1,0 +1,45 @@
+// Pressure Poisson solver: real FFT in x, complex FFT in z, tridiagonal
+// solve in y per (kx, kz) mode, and the inverse transforms.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "chs_geom.h"
+
+namespace chs {
+
+class Poisson {
+ public:
+  // kind: 2 = FFT(x,z)+TDMA(y) with the 2nd-order y operator of
+  // src/poisson_fishpack.f90:213-350 (y walls or periodic y is rejected).
+  static std::string create(const Mesh& m, int kind, std::unique_ptr<Poisson>* out);
+  ~Poisson();
+  // In-place solve on a device array of (nx, ny, nz), Fortran order.
+  // `in_scale` multiplies the right-hand side before the forward transform.
+  std::string solve(double* rhs, hipStream_t s, double in_scale = 1.0);
+  std::string solve(float* rhs, hipStream_t s, float in_scale = 1.0f);
+  int nx = 0, ny = 0, nz = 0, nxh = 0;
+
+ private:
+  template <typename T> std::string solve_t(T* rhs, hipStream_t s, T in_scale);
+  std::vector<int> rad_x, rad_z;  // radix plans of nx/2 and nz
+  // device tables
+  double* tw_x64 = nullptr; float* tw_x32 = nullptr;   // twiddles of length nx/2 (cos, sin pairs)
+  double* tw_z64 = nullptr; float* tw_z32 = nullptr;   // twiddles of length nz
+  double* rot64 = nullptr; float* rot32 = nullptr;     // exp(-2 pi i k / nx), k = 0..nx/2
+  double* ya64 = nullptr; double* yc64 = nullptr;      // y tridiagonal off-diagonals (ny)
+  float* ya32 = nullptr; float* yc32 = nullptr;
+  double* fd64 = nullptr; double* fz64 = nullptr;      // per-mode factors (nxh, nz, ny)
+  float* fd32 = nullptr; float* fz32 = nullptr;
+  void* work = nullptr;                                // complex work (nxh, ny, nz)
+  size_t work_bytes = 0;
+};
+
+// Radix plan of n over {4, 2, 3, 5}; empty if n has another prime factor.
+std::vector<int> radix_plan(int n);
+
+}  // namespace chs

@@ -1,0 +1,446 @@
+// FFT(x,z) + TDMA(y) pressure Poisson solve on gfx950.
+//
+// Follows src/poisson_fishpack.f90:354-490 (fishpack_fft_simple): forward real
+// FFT along x, forward FFT along z, a tridiagonal solve along y per wavenumber
+// pair with eigenvalues -4/h^2 sin^2(pi m/N) (fishpack_root_1D, :60-157) and the
+// 2nd-order y operator of fishpack_fft_init (:213-350), then the inverse
+// transforms and the 1/(nx*nz) scaling.  FFTPACK's half-complex real-to-real
+// layout is replaced by the equivalent complex spectrum (kx = 0..nx/2, all kz):
+// each FFTPACK coefficient pair shares one eigenvalue, so the solves coincide.
+//
+// Transforms are LDS-resident Stockham FFTs (radix 4/2/3/5), a workgroup per
+// tile of lines; the x transform packs the real line into nx/2 complex points.
+// The per-mode tridiagonal factors (TRID0's D and pivots, including its
+// singular-pivot rule |Z| <= 1e-10 => last unknown 0) are computed once on the
+// host in TRID0's order and read by the y kernel.
+#include "chs_poisson.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace chs {
+
+#define PCHK(x)                                                             \
+  do {                                                                      \
+    hipError_t e_ = (x);                                                    \
+    if (e_ != hipSuccess) return std::string(#x ": ") + hipGetErrorString(e_); \
+  } while (0)
+
+std::vector<int> radix_plan(int n) {
+  std::vector<int> r;
+  while (n % 4 == 0) { r.push_back(4); n /= 4; }
+  while (n % 2 == 0) { r.push_back(2); n /= 2; }
+  while (n % 3 == 0) { r.push_back(3); n /= 3; }
+  while (n % 5 == 0) { r.push_back(5); n /= 5; }
+  if (n != 1) r.clear();
+  return r;
+}
+
+template <typename T>
+struct cx {
+  T re, im;
+};
+template <typename T>
+__device__ inline cx<T> cadd(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ inline cx<T> csub(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T>
+__device__ inline cx<T> cmul(cx<T> a, cx<T> b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+template <typename T>
+__device__ inline cx<T> cscale(cx<T> a, T s) { return {a.re * s, a.im * s}; }
+// multiply by s*i (s = +-1)
+template <typename T>
+__device__ inline cx<T> cmuli(cx<T> a, T s) { return {-s * a.im, s * a.re}; }
+
+constexpr int kTileC = 4096;  // complex points per workgroup tile
+constexpr int kThreads = 256;
+
+template <typename T, int R>
+__device__ inline void small_dft(cx<T>* v, T s) {
+  if constexpr (R == 2) {
+    cx<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    cx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    cx<T> t2 = cadd(v[1], v[3]), t3 = cmuli(csub(v[1], v[3]), s);
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  } else if constexpr (R == 3) {
+    const T h = T(0.5), q = T(0.86602540378443864676);
+    cx<T> t = cadd(v[1], v[2]), u = cmuli(csub(v[1], v[2]), s * q);
+    cx<T> m = csub(v[0], cscale(t, h));
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(m, u);
+    v[2] = csub(m, u);
+  } else {  // R == 5
+    const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+    const T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);
+    cx<T> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    cx<T> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    cx<T> m1 = cadd(v[0], cadd(cscale(a1, c1), cscale(a2, c2)));
+    cx<T> m2 = cadd(v[0], cadd(cscale(a1, c2), cscale(a2, c1)));
+    cx<T> n1 = cmuli(cadd(cscale(b1, s1), cscale(b2, s2)), s);
+    cx<T> n2 = cmuli(csub(cscale(b1, s2), cscale(b2, s1)), s);
+    v[0] = cadd(v[0], cadd(a1, a2));
+    v[1] = cadd(m1, n1);
+    v[4] = csub(m1, n1);
+    v[2] = cadd(m2, n2);
+    v[3] = csub(m2, n2);
+  }
+}
+
+// One Stockham stage over `nl` lines of Nc points held in LDS (in place: all
+// inputs are read into registers before the barrier, outputs written after).
+template <typename T, int R>
+__device__ void stage(cx<T>* buf, int Nc, int nl, int Ns, const cx<T>* tw, T s) {
+  constexpr int QMAX = (16 + R - 1) / R;
+  const int nb = Nc / R;
+  const int total = nl * nb;
+  const int tstep = Nc / (Ns * R);
+  cx<T> v[QMAX][R];
+  int bidx[QMAX];
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    int b = threadIdx.x + q * kThreads;
+    bidx[q] = b;
+    if (b < total) {
+      int line = b / nb, j = b % nb;
+      int k = j % Ns;
+      const cx<T>* src = buf + line * Nc + j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        cx<T> x = src[r * nb];
+        if (r > 0 && k > 0) {
+          cx<T> w = tw[r * k * tstep];
+          w.im = s < T(0) ? w.im : -w.im;
+          x = cmul(x, w);
+        }
+        v[q][r] = x;
+      }
+      small_dft<T, R>(v[q], s);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    int b = bidx[q];
+    if (b < total) {
+      int line = b / nb, j = b % nb;
+      int k = j % Ns;
+      cx<T>* dst = buf + line * Nc + (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[r * Ns] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+struct Radices {
+  int n;
+  int r[12];
+};
+
+template <typename T>
+__device__ void fft_tile(cx<T>* buf, int Nc, int nl, const Radices& rp, const cx<T>* tw, T s) {
+  int Ns = 1;
+  for (int q = 0; q < rp.n; ++q) {
+    switch (rp.r[q]) {
+      case 2: stage<T, 2>(buf, Nc, nl, Ns, tw, s); break;
+      case 3: stage<T, 3>(buf, Nc, nl, Ns, tw, s); break;
+      case 4: stage<T, 4>(buf, Nc, nl, Ns, tw, s); break;
+      default: stage<T, 5>(buf, Nc, nl, Ns, tw, s); break;
+    }
+    Ns *= rp.r[q];
+  }
+}
+
+// x forward: real lines (nx) -> complex spectrum kx = 0..nx/2 into W.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_fft_x_fwd(const T* __restrict__ in, cx<T>* __restrict__ W, int nx,
+                                                       long nlines, Radices rp, const cx<T>* tw, const cx<T>* rot,
+                                                       T in_scale) {
+  extern __shared__ unsigned char smem[];
+  cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+  const int N = nx / 2, nxh = N + 1;
+  const int lpb = kTileC / N;
+  const long L0 = (long)blockIdx.x * lpb;
+  const int nl = (int)((nlines - L0) < lpb ? (nlines - L0) : lpb);
+  for (int t = threadIdx.x; t < nl * N; t += kThreads) {
+    int line = t / N, n = t % N;
+    const T* src = in + (L0 + line) * nx;
+    buf[t] = {src[2 * n] * in_scale, src[2 * n + 1] * in_scale};
+  }
+  __syncthreads();
+  fft_tile<T>(buf, N, nl, rp, tw, T(-1));
+  for (int t = threadIdx.x; t < nl * nxh; t += kThreads) {
+    int line = t / nxh, k = t % nxh;
+    cx<T> zk = buf[line * N + (k % N)];
+    cx<T> zn = buf[line * N + ((N - k) % N)];
+    zn.im = -zn.im;  // conj(Z[N-k])
+    cx<T> e = cscale(cadd(zk, zn), T(0.5));
+    cx<T> d = csub(zk, zn);
+    cx<T> o = {d.im * T(0.5), -d.re * T(0.5)};  // (Z - conj)/(2i)
+    cx<T> x = cadd(e, cmul(rot[k], o));
+    W[(L0 + line) * nxh + k] = x;
+  }
+}
+
+// x inverse: complex spectrum kx = 0..nx/2 -> real lines, times `scale`.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_fft_x_inv(const cx<T>* __restrict__ W, T* __restrict__ out, int nx,
+                                                       long nlines, Radices rp, const cx<T>* tw, const cx<T>* rot,
+                                                       T scale) {
+  extern __shared__ unsigned char smem[];
+  cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+  const int N = nx / 2, nxh = N + 1;
+  const int lpb = kTileC / N;
+  const long L0 = (long)blockIdx.x * lpb;
+  const int nl = (int)((nlines - L0) < lpb ? (nlines - L0) : lpb);
+  for (int t = threadIdx.x; t < nl * N; t += kThreads) {
+    int line = t / N, k = t % N;
+    const cx<T>* src = W + (L0 + line) * nxh;
+    cx<T> xk = src[k], xn = src[N - k];
+    xn.im = -xn.im;  // conj(X[N-k])
+    cx<T> e = cadd(xk, xn);
+    cx<T> r = rot[k];
+    r.im = -r.im;  // exp(+2 pi i k / nx)
+    cx<T> o = cmul(csub(xk, xn), r);
+    buf[t] = {e.re - o.im, e.im + o.re};  // e + i*o
+  }
+  __syncthreads();
+  fft_tile<T>(buf, N, nl, rp, tw, T(1));
+  for (int t = threadIdx.x; t < nl * N; t += kThreads) {
+    int line = t / N, n = t % N;
+    T* dst = out + (L0 + line) * nx;
+    cx<T> z = buf[t];
+    dst[2 * n] = z.re * scale;
+    dst[2 * n + 1] = z.im * scale;
+  }
+}
+
+// z transform (either sign) in place on W(nxh, ny, nz): tiles of consecutive kx.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_fft_z(cx<T>* __restrict__ W, int nxh, int ny, int nz, Radices rp,
+                                                   const cx<T>* tw, T s) {
+  extern __shared__ unsigned char smem[];
+  cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+  const int lpb = kTileC / nz;
+  const int ntile = (nxh + lpb - 1) / lpb;
+  const int tile = blockIdx.x % ntile, j = blockIdx.x / ntile;
+  const int kx0 = tile * lpb;
+  const int nl = (nxh - kx0) < lpb ? (nxh - kx0) : lpb;
+  const long zs = (long)nxh * ny;
+  for (int t = threadIdx.x; t < nl * nz; t += kThreads) {
+    int kz = t / nl, line = t % nl;
+    buf[line * nz + kz] = W[kx0 + line + (long)nxh * j + zs * kz];
+  }
+  __syncthreads();
+  fft_tile<T>(buf, nz, nl, rp, tw, s);
+  for (int t = threadIdx.x; t < nl * nz; t += kThreads) {
+    int kz = t / nl, line = t % nl;
+    W[kx0 + line + (long)nxh * j + zs * kz] = buf[line * nz + kz];
+  }
+}
+
+// y tridiagonal per (kx, kz) with host-built TRID0 factors.
+template <typename T>
+__global__ void __launch_bounds__(64) k_ytdma(cx<T>* __restrict__ W, int nxh, int ny, int nz, const double* __restrict__ ya,
+                                             const T* __restrict__ fd, const T* __restrict__ fz) {
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nm = (long)nxh * nz;
+  if (m >= nm) return;
+  const long kx = m % nxh, kz = m / nxh;
+  const long js = nxh;                 // stride along j in W
+  const long base = kx + (long)nxh * ny * kz;
+  const long fs = nm;                  // stride along j in the factor tables
+  cx<T> t = W[base];
+  T z = fz[m];
+  t = cscale(t, z);
+  W[base] = t;
+  for (int j = 1; j < ny - 1; ++j) {
+    const T aj = T(ya[j]);
+    cx<T> v = W[base + js * j];
+    z = fz[m + fs * j];
+    t = cscale(csub(v, cscale(t, aj)), z);
+    W[base + js * j] = t;
+  }
+  {
+    const int j = ny - 1;
+    const T aj = T(ya[j]);
+    const T zl = fz[m + fs * j];  // the pivot itself for the last row
+    cx<T> v = W[base + js * j];
+    if (fabs((double)zl) > 1.0e-10) {
+      v = csub(v, cscale(t, aj));
+      t = {v.re / zl, v.im / zl};
+    } else {
+      t = {T(0), T(0)};
+    }
+    W[base + js * j] = t;
+  }
+  for (int j = ny - 2; j >= 0; --j) {
+    cx<T> v = W[base + js * j];
+    const T dj = fd[m + fs * j];
+    t = csub(v, cscale(t, dj));
+    W[base + js * j] = t;
+  }
+}
+
+Poisson::~Poisson() {
+  for (void* p : {(void*)tw_x64, (void*)tw_x32, (void*)tw_z64, (void*)tw_z32, (void*)rot64, (void*)rot32,
+                  (void*)ya64, (void*)yc64, (void*)ya32, (void*)yc32, (void*)fd64, (void*)fz64, (void*)fd32,
+                  (void*)fz32, work})
+    if (p) (void)hipFree(p);
+}
+
+template <typename T>
+static std::string upv(const std::vector<double>& v, T** d) {
+  std::vector<T> t(v.begin(), v.end());
+  PCHK(hipMalloc(d, sizeof(T) * t.size()));
+  PCHK(hipMemcpy(*d, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice));
+  return "";
+}
+
+std::string Poisson::create(const Mesh& m, int kind, std::unique_ptr<Poisson>* out) {
+  if (kind != 2) return "poisson: only the FFT(x,z)+TDMA(y) path is built";
+  std::unique_ptr<Poisson> p(new Poisson());
+  p->nx = m.nc[0]; p->ny = m.nc[1]; p->nz = m.nc[2];
+  if (p->nx % 2) return "poisson: nx must be even";
+  p->nxh = p->nx / 2 + 1;
+  p->rad_x = radix_plan(p->nx / 2);
+  p->rad_z = radix_plan(p->nz);
+  if (p->rad_x.empty() || p->rad_z.empty()) return "poisson: nx/2 and nz must be 2,3,5-smooth";
+  if (p->rad_x.size() > 12 || p->rad_z.size() > 12) return "poisson: transform too long";
+  if (p->nx / 2 > kTileC || p->nz > kTileC) return "poisson: transform too long for one tile";
+  if (p->ny < 3) return "poisson: ny too small";
+  const double pi = 2.0 * std::asin(1.0);
+  // twiddles exp(-2 pi i m / Nc)
+  auto twid = [&](int Nc) {
+    std::vector<double> t(2 * Nc);
+    for (int q = 0; q < Nc; ++q) {
+      double a = 2.0 * pi * double(q) / double(Nc);
+      t[2 * q] = std::cos(a);
+      t[2 * q + 1] = -std::sin(a);
+    }
+    return t;
+  };
+  std::vector<double> twx = twid(p->nx / 2), twz = twid(p->nz), rot(2 * p->nxh);
+  for (int k = 0; k < p->nxh; ++k) {
+    double a = 2.0 * pi * double(k) / double(p->nx);
+    rot[2 * k] = std::cos(a);
+    rot[2 * k + 1] = -std::sin(a);
+  }
+  std::string e;
+  if (!(e = upv(twx, &p->tw_x64)).empty() || !(e = upv(twx, &p->tw_x32)).empty()) return e;
+  if (!(e = upv(twz, &p->tw_z64)).empty() || !(e = upv(twz, &p->tw_z32)).empty()) return e;
+  if (!(e = upv(rot, &p->rot64)).empty() || !(e = upv(rot, &p->rot32)).empty()) return e;
+  // eigenvalues (fishpack_root_1D, LP = 1): 0, -4/h^2 sin^2(pi m/N) pairs, -4/h^2
+  auto roots = [&](int L, double C1) {
+    std::vector<double> rt(L, 0.0);
+    double dx = pi / (2.0 * double(L));
+    rt[0] = 0.0;
+    rt[L - 1] = -4.0 * C1;
+    for (int i = 3; i <= L; i += 2) {
+      double s = std::sin(double(i - 1) * dx);
+      rt[i - 2] = -4.0 * C1 * (s * s);
+      rt[i - 1] = rt[i - 2];
+    }
+    return rt;
+  };
+  std::vector<double> xrt = roots(p->nx, m.h2r[0]), zrt = roots(p->nz, m.h2r[2]);
+  // complex mode k of a length-L transform -> FFTPACK root index
+  auto mode_root = [](const std::vector<double>& rt, int L, int k) {
+    int mm = k <= L / 2 ? k : L - k;
+    if (mm == 0) return rt[0];
+    if (2 * mm == L) return rt[L - 1];
+    return rt[2 * mm - 1];
+  };
+  // y operator (fishpack_fft_init, :312-323; Cartesian: rp = rc = 1)
+  const int ny = p->ny;
+  std::vector<double> a(ny), b(ny), c(ny);
+  for (int j = 0; j < ny; ++j) {
+    a[j] = m.h2r[1] * m.ymc[j] * m.ymp[j] * 1.0 * 1.0;
+    c[j] = m.h2r[1] * m.ymc[j] * m.ymp[j + 1] * 1.0 * 1.0;
+  }
+  for (int j = 0; j < ny; ++j) b[j] = -(a[j] + c[j]);
+  if (!m.periodic[1]) {
+    b[0] = b[0] + a[0];
+    a[0] = 0.0;
+    b[ny - 1] = b[ny - 1] + c[ny - 1];
+    c[ny - 1] = 0.0;
+  } else {
+    return "poisson: periodic y needs the 3-D FFT path";
+  }
+  if (!(e = upv(a, &p->ya64)).empty() || !(e = upv(c, &p->yc64)).empty()) return e;
+  // TRID0 factors per mode: fz[j] = 1/(bb_j - a_j d_{j-1}) for j < ny-1 and the
+  // raw last pivot at j = ny-1; fd[j] = c_j * z_j.
+  const long nm = (long)p->nxh * p->nz;
+  std::vector<double> fd((size_t)nm * ny), fz((size_t)nm * ny);
+  std::vector<double> bb(ny), d(ny);
+  for (int kz = 0; kz < p->nz; ++kz)
+    for (int kx = 0; kx < p->nxh; ++kx) {
+      const double xr = mode_root(xrt, p->nx, kx), zr = mode_root(zrt, p->nz, kz);
+      for (int j = 0; j < ny; ++j) bb[j] = b[j] + xr / (1.0 * 1.0) + zr;
+      const long mi = kx + (long)p->nxh * kz;
+      double z = 1.0 / bb[0];
+      d[0] = c[0] * z;
+      fz[mi] = z;
+      fd[mi] = d[0];
+      for (int j = 1; j < ny - 1; ++j) {
+        z = 1.0 / (bb[j] - a[j] * d[j - 1]);
+        d[j] = c[j] * z;
+        fz[mi + nm * j] = z;
+        fd[mi + nm * j] = d[j];
+      }
+      z = bb[ny - 1] - a[ny - 1] * d[ny - 2];
+      fz[mi + nm * (ny - 1)] = z;
+      fd[mi + nm * (ny - 1)] = 0.0;
+    }
+  if (!(e = upv(fd, &p->fd64)).empty() || !(e = upv(fz, &p->fz64)).empty()) return e;
+  if (!(e = upv(fd, &p->fd32)).empty() || !(e = upv(fz, &p->fz32)).empty()) return e;
+  p->work_bytes = sizeof(double) * 2 * (size_t)p->nxh * ny * p->nz;
+  PCHK(hipMalloc(&p->work, p->work_bytes));
+  *out = std::move(p);
+  return "";
+}
+
+template <typename T>
+std::string Poisson::solve_t(T* rhs, hipStream_t s, T in_scale) {
+  Radices rx{}, rz{};
+  rx.n = (int)rad_x.size();
+  for (int q = 0; q < rx.n; ++q) rx.r[q] = rad_x[q];
+  rz.n = (int)rad_z.size();
+  for (int q = 0; q < rz.n; ++q) rz.r[q] = rad_z[q];
+  const bool dbl = sizeof(T) == 8;
+  const cx<T>* twx = reinterpret_cast<const cx<T>*>(dbl ? (void*)tw_x64 : (void*)tw_x32);
+  const cx<T>* twz = reinterpret_cast<const cx<T>*>(dbl ? (void*)tw_z64 : (void*)tw_z32);
+  const cx<T>* rot = reinterpret_cast<const cx<T>*>(dbl ? (void*)rot64 : (void*)rot32);
+  const T* fd = reinterpret_cast<const T*>(dbl ? (void*)fd64 : (void*)fd32);
+  const T* fz = reinterpret_cast<const T*>(dbl ? (void*)fz64 : (void*)fz32);
+  cx<T>* W = reinterpret_cast<cx<T>*>(work);
+  const long nlx = (long)ny * nz;
+  const int N = nx / 2;
+  const int lpbx = kTileC / N;
+  const size_t shx = sizeof(cx<T>) * (size_t)lpbx * N;
+  dim3 gx((unsigned)((nlx + lpbx - 1) / lpbx));
+  hipLaunchKernelGGL(k_fft_x_fwd<T>, gx, dim3(kThreads), shx, s, rhs, W, nx, nlx, rx, twx, rot, in_scale);
+  const int lpbz = kTileC / nz;
+  const size_t shz = sizeof(cx<T>) * (size_t)lpbz * nz;
+  dim3 gz((unsigned)(((nxh + lpbz - 1) / lpbz) * ny));
+  hipLaunchKernelGGL(k_fft_z<T>, gz, dim3(kThreads), shz, s, W, nxh, ny, nz, rz, twz, T(-1));
+  const long nm = (long)nxh * nz;
+  hipLaunchKernelGGL(k_ytdma<T>, dim3((unsigned)((nm + 63) / 64)), dim3(64), 0, s, W, nxh, ny, nz, ya64, fd, fz);
+  hipLaunchKernelGGL(k_fft_z<T>, gz, dim3(kThreads), shz, s, W, nxh, ny, nz, rz, twz, T(1));
+  const T scale = T(1.0 / (double(nx) * double(nz)));
+  hipLaunchKernelGGL(k_fft_x_inv<T>, gx, dim3(kThreads), shx, s, W, rhs, nx, nlx, rx, twx, rot, scale);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("poisson launch: ") + hipGetErrorString(e);
+  return "";
+}
+
+std::string Poisson::solve(double* rhs, hipStream_t s, double in_scale) { return solve_t<double>(rhs, s, in_scale); }
+std::string Poisson::solve(float* rhs, hipStream_t s, float in_scale) { return solve_t<float>(rhs, s, in_scale); }
+
+}  // namespace chs

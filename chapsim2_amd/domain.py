@@ -1,0 +1,114 @@
+"""Domain / solver handle: the host-side counterpart of the reference's
+``t_domain`` + ``t_flow`` set-up (src/geometry.f90:410-593,
+src/bc_general.f90:137-757, src/input_general.f90:842-888), backed by a
+``chs_handle`` of libchapsim_hip.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+# boundary kinds (src/modules.f90:223-234) and accuracies (:248-251)
+IBC_INTERIOR, IBC_PERIODIC, IBC_SYMMETRIC, IBC_ASYMMETRIC, IBC_DIRICHLET, IBC_NEUMANN, IBC_INTRPL = range(7)
+IACCU_CD2, IACCU_CD4, IACCU_CP4, IACCU_CP6 = 1, 2, 3, 4
+FFT_FISHPACK_2DFFT, FFT_2DECOMP_3DFFT = 2, 3
+PREC_FP64, PREC_MIXED = 0, 1
+
+
+def make_desc(nc, lxx, lyb, lyt, lzz, periodic_y=False, istret=0, rstret=0.0, iacc=IACCU_CP6,
+              dt=1e-3, ren=1.0, reninit=None, init_re_to=0, idriven=0, poisson=FFT_FISHPACK_2DFFT,
+              uwall=((0.0, 0.0, 0.0), (0.0, 0.0, 0.0)), precision=PREC_FP64, projection=0,
+              div_tol=1e-10, max_refine=50):
+    d = _lib.ChsDesc()
+    for k in range(3):
+        d.nc[k] = int(nc[k])
+    d.periodic_y = int(bool(periodic_y))
+    d.lxx, d.lyb, d.lyt, d.lzz = float(lxx), float(lyb), float(lyt), float(lzz)
+    d.istret, d.rstret = int(istret), float(rstret)
+    d.iacc = int(iacc)
+    d.dt = float(dt)
+    d.ren = float(ren)
+    d.reninit = float(ren if reninit is None else reninit)
+    d.init_re_to = int(init_re_to)
+    d.idriven = int(idriven)
+    d.poisson = int(poisson)
+    for s in range(2):
+        for v in range(3):
+            d.uwall[s][v] = float(uwall[s][v])
+    d.precision = int(precision)
+    d.projection = int(projection)
+    d.div_tol = float(div_tol)
+    d.max_refine = int(max_refine)
+    return d
+
+
+class Domain:
+    """One channel (y walls) or periodic box on the GPU."""
+
+    def __init__(self, nc, lxx, lyb, lyt, lzz, **kw):
+        self.lib = _lib.load()
+        self.desc = make_desc(nc, lxx, lyb, lyt, lzz, **kw)
+        self.nc = tuple(int(n) for n in nc)
+        py = self.nc[1] if kw.get("periodic_y", False) else self.nc[1] + 1
+        self.np = (self.nc[0], py, self.nc[2])
+        nx, ny, nz = self.nc
+        self.shapes = {
+            "qx": (nx, ny, nz), "qy": (nx, py, nz), "qz": (nx, ny, nz), "pr": (nx, ny, nz),
+            "pcc": (nx, ny, nz), "cpc": (nx, py, nz), "ccp": (nx, ny, nz), "ccc": (nx, ny, nz),
+            "ppc": (nx, py, nz), "pcp": (nx, ny, nz), "cpp": (nx, py, nz),
+        }
+        h = C.c_void_p()
+        _lib.check(self.lib.chs_create(C.byref(self.desc), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.chs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state ---------------------------------------------------------
+    def set_state(self, qx, qy, qz, pr):
+        arrs = [np.asfortranarray(a, dtype=np.float64) for a in (qx, qy, qz, pr)]
+        for a, k in zip(arrs, ("qx", "qy", "qz", "pr")):
+            if a.shape != self.shapes[k]:
+                raise ValueError(f"{k} has shape {a.shape}, expected {self.shapes[k]}")
+        _lib.check(self.lib.chs_set_state(self.h, *[a.ctypes.data for a in arrs], _lib.MEM_HOST))
+
+    def get_state(self):
+        out = [np.zeros(self.shapes[k], order="F") for k in ("qx", "qy", "qz", "pr")]
+        _lib.check(self.lib.chs_get_state(self.h, *[a.ctypes.data for a in out], _lib.MEM_HOST))
+        return out
+
+    def update_re(self, it):
+        _lib.check(self.lib.chs_update_re(self.h, int(it)))
+
+    def substep(self, isub):
+        _lib.check(self.lib.chs_substep(self.h, int(isub)))
+
+    def run(self, iter0, nsteps):
+        _lib.check(self.lib.chs_run(self.h, int(iter0), int(nsteps)))
+
+    def sync(self):
+        _lib.check(self.lib.chs_sync(self.h))
+
+    def momentum_rhs(self, isub):
+        out = [np.zeros(self.shapes[k], order="F") for k in ("qx", "qy", "qz")]
+        _lib.check(self.lib.chs_momentum_rhs(self.h, int(isub), *[a.ctypes.data for a in out], _lib.MEM_HOST))
+        return out
+
+    def mass_check(self):
+        out = np.zeros(3)
+        _lib.check(self.lib.chs_mass_check(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def divergence(self):
+        out = np.zeros(self.shapes["ccc"], order="F")
+        _lib.check(self.lib.chs_divergence(self.h, out.ctypes.data, _lib.MEM_HOST))
+        return out

@@ -1,0 +1,138 @@
+/*
+ * chapsim_hip.h -- C ABI of the MI355X (gfx950) implementation of the
+ * CHAPSim2 per-timestep hot path.
+ *
+ * The reference (weiwangstfc/CHAPSim2) is a Fortran + MPI code with no plugin
+ * API; its hot path is a set of Fortran module procedures.  Each entry point
+ * below names the reference procedure it replaces (file:line in the CHAPSim2
+ * tree).  A Fortran host binds these through iso_c_binding (see
+ * chapsim2_amd/fortran/chapsim_hip.f90).
+ *
+ * Conventions
+ *  - All arrays are Fortran order (x fastest), FP64, with the staggered
+ *    extents of the reference: qx (np1,nc2,nc3), qy (nc1,np2,nc3),
+ *    qz (nc1,nc2,np3), p (nc1,nc2,nc3); np = nc for a periodic direction,
+ *    nc+1 otherwise (src/geometry.f90:434-446).
+ *  - Boundary values fbc use the reference's per-direction layouts
+ *    (4,n2,n3) / (n1,4,n3) / (n1,n2,4) (src/basics_operations2.f90:3952-4790).
+ *  - `mem` = CHS_MEM_HOST for host pointers (copied in and out) or
+ *    CHS_MEM_DEVICE for device pointers on the handle's GPU.
+ *  - Every function returns 0 on success or a negative CHS_E* code; the
+ *    message of the last error of the calling thread is chs_last_error().
+ *    The library never aborts and never prints.
+ *  - One host thread per handle; calls are synchronous.
+ */
+#ifndef CHAPSIM_HIP_H
+#define CHAPSIM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHS_ABI_VERSION 1
+
+enum {
+  CHS_OK = 0,
+  CHS_EINVAL = -1,   /* bad argument, shape or boundary kind */
+  CHS_ENOMEM = -2,   /* device allocation failed */
+  CHS_EHIP = -3,     /* HIP runtime error */
+  CHS_ENAN = -4,     /* non-finite value detected */
+  CHS_EUNSUP = -5    /* configuration outside the supported set */
+};
+
+enum { CHS_MEM_HOST = 0, CHS_MEM_DEVICE = 1 };
+
+/* boundary kinds, src/modules.f90:223-234 */
+enum { CHS_BC_INTERIOR = 0, CHS_BC_PERIODIC = 1, CHS_BC_SYMMETRIC = 2, CHS_BC_ASYMMETRIC = 3,
+       CHS_BC_DIRICHLET = 4, CHS_BC_NEUMANN = 5, CHS_BC_INTRPL = 6 };
+/* accuracies, src/modules.f90:248-251 */
+enum { CHS_CD2 = 1, CHS_CD4 = 2, CHS_CP4 = 3, CHS_CP6 = 4 };
+/* operator kinds of chs_op3d (the 18 Get_* procedures are kind x direction) */
+enum { CHS_MIDP_C2P = 0, CHS_MIDP_P2C = 1, CHS_1DER_C2C = 2, CHS_1DER_P2P = 3,
+       CHS_1DER_C2P = 4, CHS_1DER_P2C = 5 };
+/* precision of the step-level solver */
+enum { CHS_PREC_FP64 = 0, CHS_PREC_MIXED = 1 };
+/* Poisson path, src/modules.f90:169-173 */
+enum { CHS_POISSON_FISHPACK = 2, CHS_POISSON_3DFFT = 3 };
+
+/* Domain/solver description (bind(C) on the Fortran side).  Replaces the
+ * t_domain/t_flow set-up of src/geometry.f90:410-593, src/bc_general.f90
+ * :137-757 and src/input_general.f90:842-888. */
+typedef struct chs_desc {
+  int nc[3];            /* cells in x, y, z */
+  int periodic_y;       /* 1: periodic y (x and z are always periodic) */
+  double lxx, lyb, lyt, lzz;
+  int istret;           /* 0 uniform; 2 two-sided, 3 bottom, 4 top tanh */
+  double rstret;        /* tanh beta = 20 * rstret (src/geometry.f90:237) */
+  int iacc;             /* CHS_CD2 .. CHS_CP6 */
+  double dt;
+  double ren, reninit;  /* Reynolds numbers (src/tools_solver.f90:30-47) */
+  int init_re_to;
+  int idriven;          /* 0 none, 1 constant x mass flux */
+  int poisson;          /* CHS_POISSON_* */
+  double uwall[2][3];   /* y-wall velocities (bottom, top) x (u, v, w) */
+  int precision;        /* CHS_PREC_* */
+  int projection;       /* 0 single projection (reference); 1 refined to div_tol */
+  double div_tol;       /* target max|div u| for projection = 1 */
+  int max_refine;       /* cap on refinement sweeps for projection = 1 */
+} chs_desc_t;
+
+typedef struct chs_handle chs_handle_t;
+
+/* Library / handle lifecycle.  Replaces Prepare_LHS_coeffs_for_operations
+ * (src/basics_operations2.f90:1600-1800), initialise_fft
+ * (src/poisson_interface.f90:102-116) and Allocate_flow_variables. */
+int chs_abi_version(void);
+const char *chs_last_error(void);
+int chs_create(const chs_desc_t *desc, chs_handle_t **out);
+int chs_destroy(chs_handle_t *h);
+int chs_sync(chs_handle_t *h);
+
+/* One compact operator on a 3-D array.  Replaces the 18
+ * Get_{x,y,z}_{midp_C2P,midp_P2C,1der_C2C,1der_P2P,1der_C2P,1der_P2C}_3D
+ * (fi3d, fo3d, dm, iacc, ibc, fbc2d) of src/basics_operations2.f90:270-312,
+ * bodies :3952-4790.  dir = 0,1,2 for x,y,z; fbc may be NULL ("absent"). */
+int chs_op3d(chs_handle_t *h, int dir, int op, int iacc, const int ibc[2], const double *fi,
+             const int shp_in[3], double *fo, const int shp_out[3], const double *fbc, int mem);
+
+/* Pressure Poisson solve in place on a (nc1,nc2,nc3) array.  Replaces
+ * solve_fft_poisson (src/poisson_interface.f90:119-149) ->
+ * fishpack_fft_simple (src/poisson_fishpack.f90:354-490) or poisson_000
+ * (src/poisson_1stderivcomp.f90:1016-1195). */
+int chs_poisson(chs_handle_t *h, double *rhs, int mem);
+
+/* Device-resident state exchange (the reference keeps qx, qy, qz, pres in
+ * t_flow; restart files src/io_restart.f90:112-171 hold the same arrays). */
+int chs_set_state(chs_handle_t *h, const double *qx, const double *qy, const double *qz,
+                  const double *pr, int mem);
+int chs_get_state(chs_handle_t *h, double *qx, double *qy, double *qz, double *pr, int mem);
+
+/* Set 1/Re for iteration iter.  Replaces Update_Re (src/tools_solver.f90:30-47). */
+int chs_update_re(chs_handle_t *h, int iter);
+
+/* One RK3 sub-step (isub = 1..3) or the initial Euler step (isub = 0).
+ * Replaces Solve_momentum_eq(fl, dm, isub) (src/eq_momentum2.f90:1799-1932). */
+int chs_substep(chs_handle_t *h, int isub);
+
+/* nsteps full time steps starting at iteration iter0+1 (Update_Re, three
+ * sub-steps).  Replaces the RK3 loop of Solve_eqs_iteration
+ * (src/chapsim.f90:243-303). */
+int chs_run(chs_handle_t *h, int iter0, int nsteps);
+
+/* Momentum right-hand sides of sub-step isub after the fractional-step
+ * combination and forcing (the mx_rhs/my_rhs/mz_rhs of
+ * Compute_momentum_rhs, src/eq_momentum2.f90:101-1571).  Advances rhs0. */
+int chs_momentum_rhs(chs_handle_t *h, int isub, double *mx, double *my, double *mz, int mem);
+
+/* max|div u| over the inlet, bulk and outlet x windows.  Replaces
+ * Check_element_mass_conservation (src/eq_continuity.f90:337-397). */
+int chs_mass_check(chs_handle_t *h, double out[3]);
+
+/* Divergence of the current velocity, (nc1,nc2,nc3).  Replaces
+ * Get_divergence_flow (src/eq_continuity.f90:91-161). */
+int chs_divergence(chs_handle_t *h, double *div, int mem);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

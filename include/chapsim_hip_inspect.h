@@ -1,0 +1,33 @@
+/*
+ * chapsim_hip_inspect.h -- host-only inspection entry points of
+ * libchapsim_hip.  They touch no GPU: they expose the operator planner (the
+ * scheme tables of src/basics_operations2.f90:361-1509 and the per-line plans
+ * the kernels execute) so a build without a GPU can be checked.
+ *
+ * These have no counterpart in the reference; they are for test harnesses
+ * and for maintainers verifying a port of the tables.
+ */
+#ifndef CHAPSIM_HIP_INSPECT_H
+#define CHAPSIM_HIP_INSPECT_H
+
+#include "chapsim_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scheme table entry: family 0..5 (d1 C2C, d1 P2P, d1 C2P, d1 P2C, midp C2P,
+ * midp P2C), is_rhs 0/1, row 1..5, k 1..3 (LHS) or 1..6 (RHS), bc 0..6,
+ * acc 1..4. */
+double chs_inspect_coeff(int fam, int is_rhs, int row, int k, int bc, int acc);
+
+/* Apply the planned operator to one line on the host (same arithmetic as the
+ * GPU kernel).  The mesh is built from desc; fbc may be NULL.  Returns 0 or a
+ * negative CHS_E* code (message in chs_last_error()). */
+int chs_inspect_op1d(const chs_desc_t *desc, int dir, int op, int iacc, const int ibc[2],
+                     const double *fi, int n_in, double *fo, int n_out, const double *fbc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,904 @@
+/*
+ * Oracle part 2: mesh, boundary tables, FFTPACK-convention transforms, the
+ * FISHPACK FFT(x,z)+TDMA(y) Poisson solve, the 3-D FFT Poisson solve for the
+ * triply periodic case, and the momentum RHS / RK3 / projection sub-step.
+ * TEST INFRASTRUCTURE ONLY.
+ *
+ * Follows:
+ *   mesh        src/geometry.f90:151-269, 410-593
+ *   bc tables   src/bc_general.f90:137-245, 455-757
+ *   RFFTF/RFFTB lib/fishpack4.1/fftpack.f:129-240 (transform definitions)
+ *   Poisson     src/poisson_fishpack.f90:20-57 (TRID0), 60-157, 213-350, 354-490
+ *   3-D Poisson src/poisson_1stderivcomp.f90:1016-1195 (poisson_000), 2343-2541
+ *   momentum    src/eq_momentum2.f90:35-71, 101-1571, 1577-1718, 1799-1932
+ *   divergence  src/eq_continuity.f90:91-161, 337-397
+ *   bulk sums   src/tools_general.f90:1202-1268
+ *   Re          src/tools_solver.f90:30-47
+ *   RK3         src/input_general.f90:842-888
+ *
+ * Pinning (tests/test_oracle_*.py):
+ *   mesh      - closed form of the tanh map and its derivative.
+ *   RFFTF/B   - numpy.fft on random data (the FFTPACK definitions).
+ *   Poisson   - the assembled 2nd-order operator applied to the solution
+ *               reproduces the RHS (dense check on small grids) and the
+ *               singular-pivot gauge sets the last (0,0) unknown to zero.
+ *   sub-step  - the CD2 configuration projects to machine-zero divergence
+ *               (discretely consistent pair); Taylor-Green decay against
+ *               the analytic exp(-2 nu t) law; Poiseuille is a steady state
+ *               of the wall-bounded CD2 discretisation.  The per-term
+ *               ordering inside Compute_momentum_rhs beyond those checks is
+ *               "parity unpinned" (no running reference exists here).
+ */
+#include "chs_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXP 1.0e20
+static const double ONE_THIRD = 1.0 / 3.0;
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+int orc_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* ---------------------------------------------------------------------- */
+/* mesh: src/geometry.f90:151-269 (tanh), 410-593                          */
+/* ---------------------------------------------------------------------- */
+static void map_tanh(const orc_dom *dm, int is_node, int n, double *y,
+                     double *mp) {
+  double eta_shift, eta_delta;
+  if (is_node) {
+    eta_shift = 0.0;
+    eta_delta = 1.0 / (double)(n - 1);
+  } else {
+    eta_shift = 1.0 / ((double)n) * 0.5;
+    eta_delta = 1.0 / (double)n;
+  }
+  double gamma = 1.0, delta = 0.0, ymin = 0.0, ymax = 0.0;
+  if (dm->istret == 2) { gamma = 0.5; delta = 0.5; ymin = -1.0; ymax = 1.0; }
+  else if (dm->istret == 3) { gamma = -1.0; delta = 1.0; ymin = 0.0; ymax = 1.0; }
+  else if (dm->istret == 4) { gamma = 1.0; delta = 0.0; ymin = 0.0; ymax = 1.0; }
+  double beta = dm->rstret * 20.0;
+  double mm = tanh(beta * gamma);
+  double eta1 = 0.0 + eta_shift;
+  for (int j = 0; j < n; ++j) {
+    double eta = (j == 0) ? eta1 : eta1 + (double)j * eta_delta;
+    double yy = tanh(beta * (eta - delta)) / mm;
+    double m = 1.0 - yy * yy * mm * mm;
+    if (m < 1.0e-20 && m > -1.0e-20)
+      m = 1.0;
+    else
+      m = mm / beta / m;
+    double ff = (dm->lyt - dm->lyb) / (ymax - ymin);
+    y[j] = (yy - ymin) * ff + dm->lyb;
+    mp[j] = m / ff;
+  }
+}
+
+int orc_dom_init(orc_dom *dm, const int nc[3], const int periodic[3],
+                 double lxx, double lyb, double lyt, double lzz, int istret,
+                 double rstret) {
+  memset(dm, 0, sizeof(*dm));
+  for (int d = 0; d < 3; ++d) {
+    dm->nc[d] = nc[d];
+    dm->periodic[d] = periodic[d];
+    dm->npgeo[d] = nc[d] + 1;
+    dm->np[d] = periodic[d] ? nc[d] : nc[d] + 1;
+  }
+  dm->lxx = lxx; dm->lyb = lyb; dm->lyt = lyt; dm->lzz = lzz;
+  dm->istret = istret; dm->rstret = rstret;
+  dm->stretching = (istret != 0);
+  dm->h[0] = lxx / (double)nc[0];
+  dm->h[2] = lzz / (double)nc[2];
+  dm->h[1] = (lyt - lyb) / (double)nc[1];
+  int npg = dm->npgeo[1], ncy = nc[1];
+  dm->yp = (double *)calloc((size_t)npg, sizeof(double));
+  dm->yc = (double *)calloc((size_t)ncy, sizeof(double));
+  dm->ymp = (double *)malloc(sizeof(double) * (size_t)npg);
+  dm->ymc = (double *)malloc(sizeof(double) * (size_t)ncy);
+  for (int j = 0; j < npg; ++j) dm->ymp[j] = 1.0;
+  for (int j = 0; j < ncy; ++j) dm->ymc[j] = 1.0;
+  if (dm->stretching) {
+    dm->h[1] = 1.0 / (double)ncy;
+    map_tanh(dm, 1, npg, dm->yp, dm->ymp);
+    map_tanh(dm, 0, ncy, dm->yc, dm->ymc);
+  } else {
+    for (int j = 0; j < npg; ++j) dm->yp[j] = (double)j * dm->h[1] + lyb;
+    for (int j = 0; j < ncy; ++j)
+      dm->yc[j] = (double)j * dm->h[1] + dm->h[1] * 0.5 + lyb;
+  }
+  for (int d = 0; d < 3; ++d) {
+    dm->h2r[d] = 1.0 / (dm->h[d] * dm->h[d]);
+    dm->h1r[d] = 1.0 / dm->h[d];
+  }
+  return 0;
+}
+
+void orc_dom_free(orc_dom *dm) {
+  free(dm->yp); free(dm->yc); free(dm->ymp); free(dm->ymc);
+  dm->yp = dm->yc = dm->ymp = dm->ymc = NULL;
+}
+
+/* ---------------------------------------------------------------------- */
+/* complex FFT (mixed radix 2/3/5, O(n^2) DFT for other factors)           */
+/* ---------------------------------------------------------------------- */
+typedef struct { double re, im; } cpx;
+
+static void fft_rec(cpx *x, int n, int stride, cpx *out, int sign, cpx *tmp) {
+  if (n == 1) { out[0] = x[0]; return; }
+  int p = (n % 4 == 0) ? 2 : (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : (n % 5 == 0) ? 5 : n;
+  int m = n / p;
+  if (p == n) {
+    for (int k = 0; k < n; ++k) {
+      double sr = 0.0, si = 0.0;
+      for (int j = 0; j < n; ++j) {
+        double ang = sign * 2.0 * M_PI * (double)((long)j * k % n) / (double)n;
+        double c = cos(ang), s = sin(ang);
+        cpx v = x[(long)j * stride];
+        sr += v.re * c - v.im * s;
+        si += v.re * s + v.im * c;
+      }
+      out[k].re = sr; out[k].im = si;
+    }
+    return;
+  }
+  for (int r = 0; r < p; ++r) fft_rec(x + (long)r * stride, m, stride * p, tmp + (long)r * m, sign, out);
+  for (int k = 0; k < m; ++k) {
+    for (int q = 0; q < p; ++q) {
+      double sr = 0.0, si = 0.0;
+      for (int r = 0; r < p; ++r) {
+        long e = ((long)r * (k + (long)q * m)) % n;
+        double ang = sign * 2.0 * M_PI * (double)e / (double)n;
+        double c = cos(ang), s = sin(ang);
+        cpx v = tmp[(long)r * m + k];
+        sr += v.re * c - v.im * s;
+        si += v.re * s + v.im * c;
+      }
+      out[k + (long)q * m].re = sr;
+      out[k + (long)q * m].im = si;
+    }
+  }
+}
+
+static void cfft(cpx *x, int n, int sign) {
+  cpx *out = (cpx *)malloc(sizeof(cpx) * (size_t)n);
+  cpx *tmp = (cpx *)malloc(sizeof(cpx) * (size_t)n);
+  fft_rec(x, n, 1, out, sign, tmp);
+  memcpy(x, out, sizeof(cpx) * (size_t)n);
+  free(out);
+  free(tmp);
+}
+
+/* RFFTF: r(1)=sum, r(2k)=sum r cos, r(2k+1)=-sum r sin, r(n)=alternating */
+void orc_rfftf(int n, double *r) {
+  cpx *x = (cpx *)malloc(sizeof(cpx) * (size_t)n);
+  for (int j = 0; j < n; ++j) { x[j].re = r[j]; x[j].im = 0.0; }
+  cfft(x, n, -1);
+  r[0] = x[0].re;
+  for (int k = 1; 2 * k < n; ++k) {
+    r[2 * k - 1] = x[k].re;
+    r[2 * k] = x[k].im;
+  }
+  if (n % 2 == 0) r[n - 1] = x[n / 2].re;
+  free(x);
+}
+
+void orc_rfftb(int n, double *r) {
+  cpx *x = (cpx *)malloc(sizeof(cpx) * (size_t)n);
+  x[0].re = r[0]; x[0].im = 0.0;
+  for (int k = 1; 2 * k < n; ++k) {
+    x[k].re = r[2 * k - 1]; x[k].im = r[2 * k];
+    x[n - k].re = r[2 * k - 1]; x[n - k].im = -r[2 * k];
+  }
+  if (n % 2 == 0) { x[n / 2].re = r[n - 1]; x[n / 2].im = 0.0; }
+  cfft(x, n, +1);
+  for (int j = 0; j < n; ++j) r[j] = x[j].re;
+  free(x);
+}
+
+/* ---------------------------------------------------------------------- */
+/* flow state                                                             */
+/* ---------------------------------------------------------------------- */
+enum { VU = 0, VV = 1, VW = 2, VP = 3, VT = 4 };
+enum { BINT = 0, BPER = 1, BSYM = 2, BASY = 3, BDIR = 4, BNEU = 5 };
+
+struct orc_flow {
+  orc_flow_cfg cfg;
+  orc_dom dm;
+  int ibc[3][5][2]; /* [dir][var][side] calculation BCs */
+  int ftp[3][2];
+  double fconst[3][5][2];
+  int mcov[3][3][2]; /* [mom component][dir][side] */
+  int mtau[3][3][2];
+  int freq[3];
+  double rre, dt, sigma2p;
+  double tg[4], tz[4], ta[4];
+  int nx, ny, nz, px, py, pz;
+  double *qx, *qy, *qz, *pres, *pcor, *mx0, *my0, *mz0;
+  double *fbx[4], *fby[4], *fbz[4]; /* constant-bc arrays for u,v,w,p */
+  /* Poisson */
+  double *xrt, *zrt, *pa, *pb, *pc;
+  double *kx2, *ky2, *kz2; /* modified wavenumbers for the 3-D path */
+};
+
+/* build_bc_symm_operation, src/bc_general.f90:455-510; returns JBC_PROD */
+static void bc_symm(const int ibc[2], const int *jbc, int mbc[2][3]) {
+  for (int i = 0; i < 2; ++i) {
+    mbc[i][0] = mbc[i][1] = mbc[i][2] = ibc[i];
+    if (jbc) {
+      if (ibc[i] == BSYM && jbc[i] == BSYM) mbc[i][2] = BSYM;
+      else if (ibc[i] == BSYM && jbc[i] == BASY) mbc[i][2] = BASY;
+      else if (ibc[i] == BASY && jbc[i] == BSYM) mbc[i][2] = BASY;
+      else if (ibc[i] == BASY && jbc[i] == BASY) mbc[i][2] = BSYM;
+      else if (ibc[i] != jbc[i]) {
+        if (ibc[i] == BDIR) mbc[i][0] = mbc[i][1] = mbc[i][2] = ibc[i];
+        if (jbc[i] == BDIR) mbc[i][0] = mbc[i][1] = mbc[i][2] = jbc[i];
+      } else {
+        mbc[i][0] = mbc[i][1] = mbc[i][2] = ibc[i];
+      }
+    } else {
+      if (ibc[i] == BSYM) { mbc[i][1] = BASY; }
+      else if (ibc[i] == BASY) { mbc[i][1] = BSYM; mbc[i][2] = BSYM; }
+    }
+  }
+}
+
+/* conv: prod(ibc_a, ibc_b); tau: prod(ftp, grad(ibc_q)) */
+static void bc_prod(const int a[2], const int b[2], int out[2]) {
+  int m[2][3];
+  bc_symm(a, b, m);
+  out[0] = m[0][2]; out[1] = m[1][2];
+}
+static void bc_tau(const int ftp[2], const int q[2], int out[2]) {
+  int m[2][3], g[2];
+  bc_symm(q, NULL, m);
+  g[0] = m[0][1]; g[1] = m[1][1];
+  bc_symm(ftp, g, m);
+  out[0] = m[0][2]; out[1] = m[1][2];
+}
+
+static double *fbc_const_array(int dir, int n1, int n2, int n3,
+                               const double v[2]) {
+  /* layouts (4,n2,n3) / (n1,4,n3) / (n1,n2,4) */
+  long sz = (dir == 0) ? 4L * n2 * n3 : (dir == 1) ? (long)n1 * 4 * n3 : (long)n1 * n2 * 4;
+  double *a = (double *)malloc(sizeof(double) * (size_t)sz);
+  for (long t = 0; t < sz; ++t) {
+    long m;
+    if (dir == 0) m = t % 4;
+    else if (dir == 1) m = (t / n1) % 4;
+    else m = t / ((long)n1 * n2);
+    a[t] = v[m % 2];
+  }
+  return a;
+}
+
+orc_flow *orc_flow_create(const orc_flow_cfg *cfg) {
+  orc_flow *f = (orc_flow *)calloc(1, sizeof(orc_flow));
+  f->cfg = *cfg;
+  int per[3] = {1, cfg->wall_y ? 0 : 1, 1};
+  orc_dom_init(&f->dm, cfg->nc, per, cfg->lxx, cfg->lyb, cfg->lyt, cfg->lzz,
+               cfg->istret, cfg->rstret);
+  for (int d = 0; d < 3; ++d)
+    for (int v = 0; v < 5; ++v)
+      for (int s = 0; s < 2; ++s) {
+        f->ibc[d][v][s] = BPER;
+        f->fconst[d][v][s] = 0.0;
+      }
+  if (cfg->wall_y) {
+    for (int s = 0; s < 2; ++s) {
+      for (int v = 0; v < 3; ++v) {
+        f->ibc[1][v][s] = BDIR;
+        f->fconst[1][v][s] = cfg->uwall[s][v];
+      }
+      f->ibc[1][VT][s] = BDIR;
+      /* config_calc_basic_ibc: pressure Neumann where qy is Dirichlet */
+      f->ibc[1][VP][s] = BNEU;
+      f->fconst[1][VP][s] = 0.0;
+    }
+  }
+  for (int d = 0; d < 3; ++d)
+    for (int s = 0; s < 2; ++s)
+      f->ftp[d][s] = (f->ibc[d][VT][s] == BNEU) ? BDIR : f->ibc[d][VT][s];
+  /* config_calc_eqs_ibc, src/bc_general.f90:512-757 (Cartesian) */
+  {
+    int (*B)[5][2] = f->ibc;
+    bc_prod(B[0][VU], B[0][VU], f->mcov[0][0]);
+    bc_prod(B[1][VV], B[1][VU], f->mcov[0][1]);
+    bc_prod(B[2][VW], B[2][VU], f->mcov[0][2]);
+    bc_tau(f->ftp[0], B[0][VU], f->mtau[0][0]);
+    bc_tau(f->ftp[1], B[1][VU], f->mtau[0][1]);
+    bc_tau(f->ftp[2], B[2][VU], f->mtau[0][2]);
+    bc_prod(B[0][VU], B[0][VV], f->mcov[1][0]);
+    bc_prod(B[1][VV], B[1][VV], f->mcov[1][1]);
+    bc_prod(B[2][VW], B[2][VV], f->mcov[1][2]);
+    bc_tau(f->ftp[0], B[0][VV], f->mtau[1][0]);
+    bc_tau(f->ftp[1], B[1][VV], f->mtau[1][1]);
+    bc_tau(f->ftp[2], B[2][VV], f->mtau[1][2]);
+    bc_prod(B[0][VU], B[0][VW], f->mcov[2][0]);
+    bc_prod(B[1][VV], B[1][VW], f->mcov[2][1]);
+    bc_prod(B[2][VW], B[2][VW], f->mcov[2][2]);
+    bc_tau(f->ftp[0], B[0][VW], f->mtau[2][0]);
+    bc_tau(f->ftp[1], B[1][VW], f->mtau[2][1]);
+    bc_tau(f->ftp[2], B[2][VW], f->mtau[2][2]);
+    for (int d = 0; d < 3; ++d) {
+      f->freq[d] = 0;
+      for (int v = 0; v < 3; ++v)
+        for (int s = 0; s < 2; ++s)
+          if (B[d][v][s] == BDIR || B[d][v][s] == BINT) f->freq[d] = 1;
+    }
+  }
+  f->sigma2p = 1.0;
+  f->dt = cfg->dt;
+  f->tg[0] = 1.0; f->tg[1] = 8.0 / 15.0; f->tg[2] = 5.0 / 12.0; f->tg[3] = 3.0 * 0.25;
+  f->tz[0] = 0.0; f->tz[1] = 0.0; f->tz[2] = -17.0 / 60.0; f->tz[3] = -5.0 / 12.0;
+  for (int i = 0; i < 4; ++i) f->ta[i] = f->tg[i] + f->tz[i];
+  const orc_dom *dm = &f->dm;
+  f->nx = dm->nc[0]; f->ny = dm->nc[1]; f->nz = dm->nc[2];
+  f->px = dm->np[0]; f->py = dm->np[1]; f->pz = dm->np[2];
+  long npcc = (long)f->px * f->ny * f->nz, ncpc = (long)f->nx * f->py * f->nz;
+  long nccp = (long)f->nx * f->ny * f->pz, nccc = (long)f->nx * f->ny * f->nz;
+  f->qx = (double *)calloc((size_t)npcc, sizeof(double));
+  f->qy = (double *)calloc((size_t)ncpc, sizeof(double));
+  f->qz = (double *)calloc((size_t)nccp, sizeof(double));
+  f->pres = (double *)calloc((size_t)nccc, sizeof(double));
+  f->pcor = (double *)calloc((size_t)nccc, sizeof(double));
+  f->mx0 = (double *)calloc((size_t)npcc, sizeof(double));
+  f->my0 = (double *)calloc((size_t)ncpc, sizeof(double));
+  f->mz0 = (double *)calloc((size_t)nccp, sizeof(double));
+  /* constant fbc arrays (initialise_fbc_flow_given): u on pcc, v on cpc,
+   * w on ccp, p on ccc */
+  const int sh[4][3] = {{f->px, f->ny, f->nz}, {f->nx, f->py, f->nz},
+                        {f->nx, f->ny, f->pz}, {f->nx, f->ny, f->nz}};
+  for (int v = 0; v < 4; ++v) {
+    f->fbx[v] = fbc_const_array(0, sh[v][0], sh[v][1], sh[v][2], f->fconst[0][v]);
+    f->fby[v] = fbc_const_array(1, sh[v][0], sh[v][1], sh[v][2], f->fconst[1][v]);
+    f->fbz[v] = fbc_const_array(2, sh[v][0], sh[v][1], sh[v][2], f->fconst[2][v]);
+  }
+  /* Poisson setup */
+  if (cfg->poisson == 2) {
+    /* fishpack_fft_init, src/poisson_fishpack.f90:213-350 (x, z periodic) */
+    int nx = f->nx, ny = f->ny, nz = f->nz;
+    f->xrt = (double *)calloc((size_t)nx, sizeof(double));
+    f->zrt = (double *)calloc((size_t)nz, sizeof(double));
+    for (int pass = 0; pass < 2; ++pass) {
+      int L = pass ? nz : nx;
+      double C1 = pass ? dm->h2r[2] : dm->h2r[0];
+      double *rt = pass ? f->zrt : f->xrt;
+      double pi = 2.0 * asin(1.0);
+      double scal = (double)L;
+      double dx = pi / (2.0 * scal);
+      rt[0] = 0.0;
+      rt[L - 1] = -4.0 * C1;
+      for (int i = 3; i <= L; i += 2) {
+        double s = sin((double)(i - 1) * dx);
+        rt[i - 2] = -4.0 * C1 * (s * s);
+        rt[i - 1] = rt[i - 2];
+      }
+    }
+    f->pa = (double *)calloc((size_t)ny, sizeof(double));
+    f->pb = (double *)calloc((size_t)ny, sizeof(double));
+    f->pc = (double *)calloc((size_t)ny, sizeof(double));
+    for (int j = 0; j < ny; ++j) {
+      f->pa[j] = dm->h2r[1] * dm->ymc[j] * dm->ymp[j] * 1.0 * 1.0;
+      f->pc[j] = dm->h2r[1] * dm->ymc[j] * dm->ymp[j + 1] * 1.0 * 1.0;
+    }
+    for (int j = 0; j < ny; ++j) f->pb[j] = -(f->pa[j] + f->pc[j]);
+    if (!dm->periodic[1]) {
+      f->pb[0] = f->pb[0] + f->pa[0];
+      f->pa[0] = 0.0;
+      f->pb[ny - 1] = f->pb[ny - 1] + f->pc[ny - 1];
+      f->pc[ny - 1] = 0.0;
+    }
+  }
+  orc_flow_update_re(f, 0);
+  return f;
+}
+
+void orc_flow_destroy(orc_flow *f) {
+  if (!f) return;
+  free(f->qx); free(f->qy); free(f->qz); free(f->pres); free(f->pcor);
+  free(f->mx0); free(f->my0); free(f->mz0);
+  for (int v = 0; v < 4; ++v) { free(f->fbx[v]); free(f->fby[v]); free(f->fbz[v]); }
+  free(f->xrt); free(f->zrt); free(f->pa); free(f->pb); free(f->pc);
+  free(f->kx2); free(f->ky2); free(f->kz2);
+  orc_dom_free(&f->dm);
+  free(f);
+}
+
+const orc_dom *orc_flow_dom(const orc_flow *f) { return &f->dm; }
+
+void orc_flow_set_state(orc_flow *f, const double *qx, const double *qy,
+                        const double *qz, const double *pr) {
+  memcpy(f->qx, qx, sizeof(double) * (size_t)f->px * f->ny * f->nz);
+  memcpy(f->qy, qy, sizeof(double) * (size_t)f->nx * f->py * f->nz);
+  memcpy(f->qz, qz, sizeof(double) * (size_t)f->nx * f->ny * f->pz);
+  memcpy(f->pres, pr, sizeof(double) * (size_t)f->nx * f->ny * f->nz);
+  memset(f->pcor, 0, sizeof(double) * (size_t)f->nx * f->ny * f->nz);
+}
+
+void orc_flow_get_state(const orc_flow *f, double *qx, double *qy, double *qz,
+                        double *pr) {
+  memcpy(qx, f->qx, sizeof(double) * (size_t)f->px * f->ny * f->nz);
+  memcpy(qy, f->qy, sizeof(double) * (size_t)f->nx * f->py * f->nz);
+  memcpy(qz, f->qz, sizeof(double) * (size_t)f->nx * f->ny * f->pz);
+  memcpy(pr, f->pres, sizeof(double) * (size_t)f->nx * f->ny * f->nz);
+}
+
+void orc_flow_get_rhs0(const orc_flow *f, double *mx, double *my, double *mz) {
+  memcpy(mx, f->mx0, sizeof(double) * (size_t)f->px * f->ny * f->nz);
+  memcpy(my, f->my0, sizeof(double) * (size_t)f->nx * f->py * f->nz);
+  memcpy(mz, f->mz0, sizeof(double) * (size_t)f->nx * f->ny * f->pz);
+}
+
+/* Update_Re, src/tools_solver.f90:30-47 */
+void orc_flow_update_re(orc_flow *f, int iter) {
+  if (iter < f->cfg.init_re_to)
+    f->rre = 1.0 / f->cfg.reninit;
+  else
+    f->rre = 1.0 / f->cfg.ren;
+}
+
+/* ---------------------------------------------------------------------- */
+/* helpers                                                                */
+/* ---------------------------------------------------------------------- */
+static double *arr(long n) { return (double *)calloc((size_t)n, sizeof(double)); }
+
+static void op(orc_flow *f, int dir, int kind, const int ibc[2],
+               const double *in, int a1, int a2, int a3, double *out, int b1,
+               int b2, int b3, const double *fbc) {
+  int si[3] = {a1, a2, a3}, so[3] = {b1, b2, b3};
+  orc_op3d(&f->dm, dir, kind, f->cfg.iacc, ibc, in, si, out, so, fbc);
+}
+
+/* volumetric average, src/tools_general.f90:1202-1268 (x-pencil arrays) */
+double orc_vol_average_x(orc_flow *f, const double *var, int n1) {
+  const orc_dom *dm = &f->dm;
+  double vol = 0.0, fo = 0.0;
+  double dx = dm->h[0], dy = dm->h[1], dz = dm->h[2];
+  for (int j = 0; j < f->ny; ++j) {
+    if (dm->stretching) dy = dm->h[1] / dm->ymc[j];
+    for (int k = 0; k < f->nz; ++k)
+      for (int i = 0; i < n1; ++i) {
+        fo = fo + var[i + (long)n1 * (j + (long)f->ny * k)] * dy * dx * dz;
+        vol = vol + dy * dx * dz;
+      }
+  }
+  return fo / vol;
+}
+
+/* enforce_velo_from_fbc, src/bc_dirichlet.f90:461-515 (y walls) */
+static void enforce_velo(orc_flow *f) {
+  if (f->ibc[1][VV][0] == BDIR || f->ibc[1][VV][1] == BDIR) {
+    for (int k = 0; k < f->nz; ++k)
+      for (int i = 0; i < f->nx; ++i) {
+        if (f->ibc[1][VV][0] == BDIR)
+          f->qy[i + (long)f->nx * (0 + (long)f->py * k)] = f->fby[VV][i + (long)f->nx * (0 + 4L * k)];
+        if (f->ibc[1][VV][1] == BDIR)
+          f->qy[i + (long)f->nx * ((f->py - 1) + (long)f->py * k)] = f->fby[VV][i + (long)f->nx * (1 + 4L * k)];
+      }
+  }
+}
+
+/* Get_divergence_flow, src/eq_continuity.f90:91-161 */
+void orc_divergence(orc_flow *f, double *div) {
+  int nx = f->nx, ny = f->ny, nz = f->nz;
+  long n = (long)nx * ny * nz;
+  double *d0 = arr(n);
+  for (long t = 0; t < n; ++t) div[t] = 0.0;
+  op(f, 0, ORC_1DER_P2C, f->ibc[0][VU], f->qx, f->px, ny, nz, d0, nx, ny, nz, f->fbx[VU]);
+  for (long t = 0; t < n; ++t) div[t] = div[t] + d0[t];
+  op(f, 1, ORC_1DER_P2C, f->ibc[1][VV], f->qy, nx, f->py, nz, d0, nx, ny, nz, NULL);
+  for (long t = 0; t < n; ++t) div[t] = div[t] + d0[t];
+  op(f, 2, ORC_1DER_P2C, f->ibc[2][VW], f->qz, nx, ny, f->pz, d0, nx, ny, nz, NULL);
+  for (long t = 0; t < n; ++t) div[t] = div[t] + d0[t];
+  free(d0);
+}
+
+/* Check_element_mass_conservation windows, src/eq_continuity.f90:386-389 */
+void orc_mass_check(orc_flow *f, double out[3]) {
+  int nx = f->nx, ny = f->ny, nz = f->nz;
+  double *div = arr((long)nx * ny * nz);
+  orc_divergence(f, div);
+  int lo[3] = {1, 5, nx - 5}, hi[3] = {4, nx - 4, nx};
+  for (int w = 0; w < 3; ++w) {
+    double m = 0.0;
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = lo[w]; i <= hi[w]; ++i) {
+          double v = fabs(div[(i - 1) + (long)nx * (j + (long)ny * k)]);
+          if (v > m) m = v;
+        }
+    out[w] = m;
+  }
+  free(div);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Poisson: FISHPACK path, src/poisson_fishpack.f90:354-490                */
+/* ---------------------------------------------------------------------- */
+static void trid0(int ng, double *t, const double *a, const double *bb,
+                  const double *c, double *d) {
+  int nr = ng, mm1 = nr - 1;
+  double z = 1.0 / bb[0];
+  d[0] = c[0] * z;
+  t[0] = t[0] * z;
+  for (int i = 1; i < mm1; ++i) {
+    z = 1.0 / (bb[i] - a[i] * d[i - 1]);
+    d[i] = c[i] * z;
+    t[i] = (t[i] - a[i] * t[i - 1]) * z;
+  }
+  z = bb[nr - 1] - a[nr - 1] * d[mm1 - 1];
+  if (fabs(z) > 1.0e-10)
+    t[nr - 1] = (t[nr - 1] - a[nr - 1] * t[mm1 - 1]) / z;
+  else
+    t[nr - 1] = 0.0;
+  for (int ip = 1; ip <= mm1; ++ip) {
+    int i = nr - ip - 1;
+    t[i] = t[i] - d[i] * t[i + 1];
+  }
+}
+
+static void poisson_fishpack(orc_flow *f, double *rhs) {
+  int nx = f->nx, ny = f->ny, nz = f->nz;
+#pragma omp parallel
+  {
+    int nmax = nx > nz ? nx : nz;
+    nmax = nmax > ny ? nmax : ny;
+    double *t = (double *)malloc(sizeof(double) * (size_t)nmax);
+    double *bb = (double *)malloc(sizeof(double) * (size_t)ny);
+    double *d = (double *)malloc(sizeof(double) * (size_t)ny);
+#pragma omp for schedule(static)
+    for (long L = 0; L < (long)ny * nz; ++L) {
+      double *row = rhs + L * nx;
+      orc_rfftf(nx, row);
+    }
+#pragma omp for schedule(static)
+    for (long L = 0; L < (long)nx * ny; ++L) {
+      long i = L % nx, j = L / nx;
+      for (int k = 0; k < nz; ++k) t[k] = rhs[i + (long)nx * (j + (long)ny * k)];
+      orc_rfftf(nz, t);
+      for (int k = 0; k < nz; ++k) rhs[i + (long)nx * (j + (long)ny * k)] = t[k];
+    }
+#pragma omp for schedule(static)
+    for (long L = 0; L < (long)nx * nz; ++L) {
+      long i = L % nx, k = L / nx;
+      for (int j = 0; j < ny; ++j) {
+        bb[j] = f->pb[j] + f->xrt[i] / (1.0 * 1.0) + f->zrt[k];
+        t[j] = rhs[i + (long)nx * (j + (long)ny * k)];
+      }
+      trid0(ny, t, f->pa, bb, f->pc, d);
+      for (int j = 0; j < ny; ++j) rhs[i + (long)nx * (j + (long)ny * k)] = t[j];
+    }
+#pragma omp for schedule(static)
+    for (long L = 0; L < (long)nx * ny; ++L) {
+      long i = L % nx, j = L / nx;
+      for (int k = 0; k < nz; ++k) t[k] = rhs[i + (long)nx * (j + (long)ny * k)];
+      orc_rfftb(nz, t);
+      for (int k = 0; k < nz; ++k) rhs[i + (long)nx * (j + (long)ny * k)] = t[k];
+    }
+#pragma omp for schedule(static)
+    for (long L = 0; L < (long)ny * nz; ++L) orc_rfftb(nx, rhs + L * nx);
+    free(t);
+    free(bb);
+    free(d);
+  }
+  double sx = (double)nx, sz = (double)nz;
+  long n = (long)nx * ny * nz;
+  for (long t = 0; t < n; ++t) rhs[t] = rhs[t] / sx / sz;
+}
+
+void orc_poisson_solve(orc_flow *f, double *rhs) {
+  if (f->cfg.poisson == 2) poisson_fishpack(f, rhs);
+}
+
+/* ---------------------------------------------------------------------- */
+/* momentum RHS, src/eq_momentum2.f90:101-1571 (Cartesian, flow only)      */
+/* ---------------------------------------------------------------------- */
+static double *fill_maxp(long n) {
+  double *a = (double *)malloc(sizeof(double) * (size_t)n);
+  for (long t = 0; t < n; ++t) a[t] = MAXP;
+  return a;
+}
+
+void orc_momentum_rhs(orc_flow *f, int isub, double *mx, double *my,
+                      double *mz) {
+  int nx = f->nx, ny = f->ny, nz = f->nz, px = f->px, py = f->py, pz = f->pz;
+  long Npcc = (long)px * ny * nz, Ncpc = (long)nx * py * nz, Nccp = (long)nx * ny * pz;
+  long Nccc = (long)nx * ny * nz, Nppc = (long)px * py * nz, Npcp = (long)px * ny * pz;
+  long Ncpp = (long)nx * py * pz;
+  int (*B)[5][2] = f->ibc;
+  const double rre = f->rre;
+  (void)isub;
+  /* first-level operators */
+  double *qxix = arr(Nccc), *qxdx = arr(Nccc), *qxdy = arr(Nppc), *qxiy = arr(Nppc);
+  double *qxiz = arr(Npcp), *qxdz = arr(Npcp);
+  op(f, 0, ORC_MIDP_P2C, B[0][VU], f->qx, px, ny, nz, qxix, nx, ny, nz, f->fbx[VU]);
+  op(f, 0, ORC_1DER_P2C, B[0][VU], f->qx, px, ny, nz, qxdx, nx, ny, nz, f->fbx[VU]);
+  op(f, 1, ORC_1DER_C2P, B[1][VU], f->qx, px, ny, nz, qxdy, px, py, nz, f->fby[VU]);
+  op(f, 1, ORC_MIDP_C2P, B[1][VU], f->qx, px, ny, nz, qxiy, px, py, nz, f->fby[VU]);
+  op(f, 2, ORC_MIDP_C2P, B[2][VU], f->qx, px, ny, nz, qxiz, px, ny, pz, f->fbz[VU]);
+  op(f, 2, ORC_1DER_C2P, B[2][VU], f->qx, px, ny, nz, qxdz, px, ny, pz, f->fbz[VU]);
+  double *qxdx_cpc = NULL;
+  if (f->freq[1]) {
+    qxdx_cpc = arr(Ncpc);
+    op(f, 0, ORC_1DER_P2C, B[0][VU], qxiy, px, py, nz, qxdx_cpc, nx, py, nz, NULL);
+  }
+  double *qydx = arr(Nppc), *qyix = arr(Nppc), *qydy = arr(Nccc), *qyiy = arr(Nccc);
+  double *qydz = arr(Ncpp), *qyiz = arr(Ncpp);
+  op(f, 0, ORC_1DER_C2P, B[0][VV], f->qy, nx, py, nz, qydx, px, py, nz, f->fbx[VV]);
+  op(f, 0, ORC_MIDP_C2P, B[0][VV], f->qy, nx, py, nz, qyix, px, py, nz, f->fbx[VV]);
+  op(f, 1, ORC_1DER_P2C, B[1][VV], f->qy, nx, py, nz, qydy, nx, ny, nz, NULL);
+  op(f, 1, ORC_MIDP_P2C, B[1][VV], f->qy, nx, py, nz, qyiy, nx, ny, nz, f->fby[VV]);
+  op(f, 2, ORC_1DER_C2P, B[2][VV], f->qy, nx, py, nz, qydz, nx, py, pz, f->fbz[VV]);
+  op(f, 2, ORC_MIDP_C2P, B[2][VV], f->qy, nx, py, nz, qyiz, nx, py, pz, f->fbz[VV]);
+  double *qydy_cpc = NULL;
+  if (f->freq[1]) {
+    double *t = arr(Nccc);
+    qydy_cpc = arr(Ncpc);
+    op(f, 1, ORC_MIDP_P2C, B[1][VV], f->qy, nx, py, nz, t, nx, ny, nz, f->fby[VV]);
+    op(f, 1, ORC_1DER_C2P, B[1][VV], t, nx, ny, nz, qydy_cpc, nx, py, nz, f->fby[VV]);
+    free(t);
+  }
+  double *qzdx = arr(Npcp), *qzix = arr(Npcp), *qziy = arr(Ncpp), *qzdy = arr(Ncpp);
+  double *qziz = arr(Nccc), *qzdz = arr(Nccc);
+  op(f, 0, ORC_1DER_C2P, B[0][VW], f->qz, nx, ny, pz, qzdx, px, ny, pz, f->fbx[VW]);
+  op(f, 0, ORC_MIDP_C2P, B[0][VW], f->qz, nx, ny, pz, qzix, px, ny, pz, f->fbx[VW]);
+  op(f, 1, ORC_MIDP_C2P, B[1][VW], f->qz, nx, ny, pz, qziy, nx, py, pz, f->fby[VW]);
+  op(f, 1, ORC_1DER_C2P, B[1][VW], f->qz, nx, ny, pz, qzdy, nx, py, pz, f->fby[VW]);
+  op(f, 2, ORC_MIDP_P2C, B[2][VW], f->qz, nx, ny, pz, qziz, nx, ny, nz, f->fbz[VW]);
+  op(f, 2, ORC_1DER_P2C, B[2][VW], f->qz, nx, ny, pz, qzdz, nx, ny, nz, NULL);
+  double *qzdz_cpc = NULL;
+  if (f->freq[1]) {
+    double *t = arr(Ncpp);
+    qzdz_cpc = arr(Ncpc);
+    op(f, 1, ORC_MIDP_C2P, B[1][VW], f->qz, nx, ny, pz, t, nx, py, pz, f->fby[VW]);
+    op(f, 2, ORC_1DER_P2C, B[2][VW], t, nx, py, pz, qzdz_cpc, nx, py, nz, NULL);
+    free(t);
+  }
+  /* divergence */
+  double *div = arr(Nccc);
+  for (long t = 0; t < Nccc; ++t) div[t] = 0.0;
+  for (long t = 0; t < Nccc; ++t) div[t] = div[t] + qxdx[t];
+  for (long t = 0; t < Nccc; ++t) div[t] = div[t] + qydy[t];
+  for (long t = 0; t < Nccc; ++t) div[t] = div[t] + qzdz[t];
+  long nfy = (long)nx * 4 * nz;
+  double *fbdiv_y = NULL;
+  if (f->freq[1]) {
+    fbdiv_y = arr(nfy);
+    for (int k = 0; k < nz; ++k)
+      for (int i = 0; i < nx; ++i)
+        for (int s = 0; s < 2; ++s) {
+          long j = s ? py - 1 : 0;
+          long src = i + (long)nx * (j + (long)py * k);
+          double v = qxdx_cpc[src] + qydy_cpc[src] + qzdz_cpc[src];
+          fbdiv_y[i + (long)nx * (s + 4L * k)] = v;
+          fbdiv_y[i + (long)nx * (s + 2 + 4L * k)] = v;
+        }
+  }
+  double *maxp_x = fill_maxp(4L * (ny > py ? ny : py) * (nz > pz ? nz : pz));
+  double *maxp_y = fill_maxp((long)(nx > px ? nx : px) * 4 * (nz > pz ? nz : pz));
+  double *maxp_z = fill_maxp((long)(nx > px ? nx : px) * (ny > py ? ny : py) * 4);
+  double *tc = arr(Nccc), *tppc = arr(Nppc), *tpcp = arr(Npcp), *tcpp = arr(Ncpp);
+  double *r1 = arr(Npcc > Ncpc ? (Npcc > Nccp ? Npcc : Nccp) : (Ncpc > Nccp ? Ncpc : Nccp));
+  double *acc_y = arr(Npcc > Ncpc ? (Npcc > Nccp ? Npcc : Nccp) : (Ncpc > Nccp ? Ncpc : Nccp));
+  double *acc_z = arr(Npcc > Ncpc ? (Npcc > Nccp ? Npcc : Nccp) : (Ncpc > Nccp ? Ncpc : Nccp));
+  double *pfc = arr(Npcc > Ncpc ? (Npcc > Nccp ? Npcc : Nccp) : (Ncpc > Nccp ? Ncpc : Nccp));
+  double *mpres = arr(Nccc);
+  for (long t = 0; t < Nccc; ++t) mpres[t] = -f->pres[t];
+  double *nfbx = arr(4L * ny * nz), *nfby = arr((long)nx * 4 * nz), *nfbz = arr((long)nx * ny * 4);
+  for (long t = 0; t < 4L * ny * nz; ++t) nfbx[t] = -f->fbx[VP][t];
+  for (long t = 0; t < (long)nx * 4 * nz; ++t) nfby[t] = -f->fby[VP][t];
+  for (long t = 0; t < (long)nx * ny * 4; ++t) nfbz[t] = -f->fbz[VP][t];
+
+  /* ---------------- x-momentum ---------------- */
+  for (long t = 0; t < Nccc; ++t) tc[t] = -qxix[t] * qxix[t];
+  op(f, 0, ORC_1DER_C2P, f->mcov[0][0], tc, nx, ny, nz, r1, px, ny, nz, maxp_x);
+  for (long t = 0; t < Npcc; ++t) mx[t] = 0.0 + r1[t];
+  for (long t = 0; t < Nppc; ++t) tppc[t] = -qyix[t] * qxiy[t];
+  op(f, 1, ORC_1DER_P2C, f->mcov[0][1], tppc, px, py, nz, r1, px, ny, nz, NULL);
+  for (long t = 0; t < Npcc; ++t) acc_y[t] = 0.0 + r1[t];
+  for (long t = 0; t < Npcp; ++t) tpcp[t] = -qzix[t] * qxiz[t];
+  op(f, 2, ORC_1DER_P2C, f->mcov[0][2], tpcp, px, ny, pz, r1, px, ny, nz, NULL);
+  for (long t = 0; t < Npcc; ++t) acc_z[t] = 0.0 + r1[t];
+  op(f, 0, ORC_1DER_C2P, B[0][VP], mpres, nx, ny, nz, pfc, px, ny, nz, nfbx);
+  for (long t = 0; t < Nccc; ++t) tc[t] = (2.0 * 1.0) * (qxdx[t] - ONE_THIRD * div[t]);
+  op(f, 0, ORC_1DER_C2P, f->mtau[0][0], tc, nx, ny, nz, r1, px, ny, nz, maxp_x);
+  for (long t = 0; t < Npcc; ++t) mx[t] = mx[t] + r1[t] * rre;
+  for (long t = 0; t < Nppc; ++t) tppc[t] = (qxdy[t] + qydx[t]) * 1.0;
+  op(f, 1, ORC_1DER_P2C, f->mtau[0][1], tppc, px, py, nz, r1, px, ny, nz, NULL);
+  for (long t = 0; t < Npcc; ++t) acc_y[t] = acc_y[t] + r1[t] * rre;
+  for (long t = 0; t < Npcp; ++t) tpcp[t] = (qxdz[t] + qzdx[t]) * 1.0;
+  op(f, 2, ORC_1DER_P2C, f->mtau[0][2], tpcp, px, ny, pz, r1, px, ny, nz, NULL);
+  for (long t = 0; t < Npcc; ++t) acc_z[t] = acc_z[t] + r1[t] * rre;
+  for (long t = 0; t < Npcc; ++t) mx[t] = mx[t] + acc_y[t];
+  for (long t = 0; t < Npcc; ++t) mx[t] = mx[t] + acc_z[t];
+  /* Calculate_momentum_fractional_step */
+  for (long t = 0; t < Npcc; ++t) {
+    double cur = mx[t], last = f->mx0[t];
+    double tot = f->tg[isub] * cur + f->tz[isub] * last;
+    f->mx0[t] = cur;
+    tot = tot + f->ta[isub] * pfc[t];
+    mx[t] = f->dt * tot;
+  }
+
+  /* ---------------- y-momentum ---------------- */
+  for (long t = 0; t < Nppc; ++t) tppc[t] = -qxiy[t] * qyix[t];
+  op(f, 0, ORC_1DER_P2C, f->mcov[1][0], tppc, px, py, nz, r1, nx, py, nz, NULL);
+  for (long t = 0; t < Ncpc; ++t) my[t] = 0.0 + r1[t];
+  double *fbc_y = maxp_y, *fbc_own = NULL;
+  if (f->freq[1]) {
+    fbc_own = arr(nfy);
+    for (long t = 0; t < nfy; ++t) fbc_own[t] = -f->fby[VV][t] * f->fby[VV][t];
+    fbc_y = fbc_own;
+  }
+  for (long t = 0; t < Nccc; ++t) tc[t] = -qyiy[t] * qyiy[t];
+  op(f, 1, ORC_1DER_C2P, f->mcov[1][1], tc, nx, ny, nz, r1, nx, py, nz, fbc_y);
+  for (long t = 0; t < Ncpc; ++t) acc_y[t] = 0.0 + r1[t];
+  for (long t = 0; t < Ncpp; ++t) tcpp[t] = -qyiz[t] * qziy[t];
+  op(f, 2, ORC_1DER_P2C, f->mcov[1][2], tcpp, nx, py, pz, r1, nx, py, nz, NULL);
+  for (long t = 0; t < Ncpc; ++t) acc_z[t] = 0.0 + r1[t];
+  {
+    double *pf = arr(Ncpc);
+    op(f, 1, ORC_1DER_C2P, B[1][VP], mpres, nx, ny, nz, pf, nx, py, nz, nfby);
+    for (long t = 0; t < Ncpc; ++t) pfc[t] = 0.0 + pf[t];
+    free(pf);
+  }
+  for (long t = 0; t < Nppc; ++t) tppc[t] = (qxdy[t] + qydx[t]) * 1.0;
+  op(f, 0, ORC_1DER_P2C, f->mtau[1][0], tppc, px, py, nz, r1, nx, py, nz, NULL);
+  for (long t = 0; t < Ncpc; ++t) my[t] = my[t] + r1[t] * rre;
+  fbc_y = maxp_y;
+  if (f->freq[1]) {
+    for (int k = 0; k < nz; ++k)
+      for (int i = 0; i < nx; ++i)
+        for (int s = 0; s < 2; ++s) {
+          long j = s ? py - 1 : 0;
+          double v = qydy_cpc[i + (long)nx * (j + (long)py * k)];
+          fbc_own[i + (long)nx * (s + 4L * k)] = v;
+          fbc_own[i + (long)nx * (s + 2 + 4L * k)] = v;
+        }
+    for (long t = 0; t < nfy; ++t)
+      fbc_own[t] = (fbc_own[t] - ONE_THIRD * fbdiv_y[t]) * 2.0 * 1.0;
+    fbc_y = fbc_own;
+  }
+  for (long t = 0; t < Nccc; ++t) tc[t] = (qydy[t] - ONE_THIRD * div[t]) * 2.0 * 1.0;
+  op(f, 1, ORC_1DER_C2P, f->mtau[1][1], tc, nx, ny, nz, r1, nx, py, nz, fbc_y);
+  for (long t = 0; t < Ncpc; ++t) acc_y[t] = acc_y[t] + r1[t] * rre;
+  for (long t = 0; t < Ncpp; ++t) tcpp[t] = (qzdy[t] + qydz[t] - 0.0) * 1.0;
+  op(f, 2, ORC_1DER_P2C, f->mtau[1][2], tcpp, nx, py, pz, r1, nx, py, nz, NULL);
+  for (long t = 0; t < Ncpc; ++t) acc_z[t] = acc_z[t] + r1[t] * rre;
+  for (long t = 0; t < Ncpc; ++t) my[t] = my[t] + acc_y[t];
+  for (long t = 0; t < Ncpc; ++t) my[t] = my[t] + acc_z[t];
+  for (long t = 0; t < Ncpc; ++t) {
+    double cur = my[t], last = f->my0[t];
+    double tot = f->tg[isub] * cur + f->tz[isub] * last;
+    f->my0[t] = cur;
+    tot = tot + f->ta[isub] * pfc[t];
+    my[t] = f->dt * tot;
+  }
+
+  /* ---------------- z-momentum ---------------- */
+  for (long t = 0; t < Npcp; ++t) tpcp[t] = -qxiz[t] * qzix[t];
+  op(f, 0, ORC_1DER_P2C, f->mcov[2][0], tpcp, px, ny, pz, r1, nx, ny, pz, NULL);
+  for (long t = 0; t < Nccp; ++t) mz[t] = 0.0 + r1[t];
+  for (long t = 0; t < Ncpp; ++t) tcpp[t] = -qyiz[t] * qziy[t];
+  op(f, 1, ORC_1DER_P2C, f->mcov[2][1], tcpp, nx, py, pz, r1, nx, ny, pz, NULL);
+  for (long t = 0; t < Nccp; ++t) acc_y[t] = 0.0 + r1[t];
+  for (long t = 0; t < Nccc; ++t) tc[t] = -qziz[t] * qziz[t];
+  op(f, 2, ORC_1DER_C2P, f->mcov[2][2], tc, nx, ny, nz, r1, nx, ny, pz, maxp_z);
+  for (long t = 0; t < Nccp; ++t) acc_z[t] = 0.0 + r1[t];
+  {
+    double *pf = arr(Nccp);
+    op(f, 2, ORC_1DER_C2P, B[2][VP], mpres, nx, ny, nz, pf, nx, ny, pz, nfbz);
+    for (long t = 0; t < Nccp; ++t) pfc[t] = 0.0 + pf[t];
+    free(pf);
+  }
+  for (long t = 0; t < Npcp; ++t) tpcp[t] = (qxdz[t] + qzdx[t]) * 1.0;
+  op(f, 0, ORC_1DER_P2C, f->mtau[2][0], tpcp, px, ny, pz, r1, nx, ny, pz, NULL);
+  for (long t = 0; t < Nccp; ++t) mz[t] = mz[t] + r1[t] * rre;
+  for (long t = 0; t < Ncpp; ++t) tcpp[t] = (qzdy[t] + qydz[t]) * 1.0;
+  op(f, 1, ORC_1DER_P2C, f->mtau[2][1], tcpp, nx, py, pz, r1, nx, ny, pz, NULL);
+  for (long t = 0; t < Nccp; ++t) acc_y[t] = acc_y[t] + r1[t] * rre;
+  for (long t = 0; t < Nccc; ++t) tc[t] = ((qzdz[t] - ONE_THIRD * div[t]) + 0.0) * 2.0 * 1.0;
+  op(f, 2, ORC_1DER_C2P, f->mtau[2][2], tc, nx, ny, nz, r1, nx, ny, pz, maxp_z);
+  for (long t = 0; t < Nccp; ++t) acc_z[t] = acc_z[t] + r1[t] * rre;
+  for (long t = 0; t < Nccp; ++t) mz[t] = mz[t] + acc_y[t];
+  for (long t = 0; t < Nccp; ++t) mz[t] = mz[t] + acc_z[t];
+  for (long t = 0; t < Nccp; ++t) {
+    double cur = mz[t], last = f->mz0[t];
+    double tot = f->tg[isub] * cur + f->tz[isub] * last;
+    f->mz0[t] = cur;
+    tot = tot + f->ta[isub] * pfc[t];
+    mz[t] = f->dt * tot;
+  }
+  /* constant-flux forcing (after the fractional step), :1547-1562 */
+  if (f->cfg.idriven == 1) {
+    double bulk = orc_vol_average_x(f, mx, px);
+    for (long t = 0; t < Npcc; ++t) mx[t] = mx[t] - bulk;
+  }
+  free(qxix); free(qxdx); free(qxdy); free(qxiy); free(qxiz); free(qxdz);
+  free(qydx); free(qyix); free(qydy); free(qyiy); free(qydz); free(qyiz);
+  free(qzdx); free(qzix); free(qziy); free(qzdy); free(qziz); free(qzdz);
+  free(qxdx_cpc); free(qydy_cpc); free(qzdz_cpc); free(div); free(fbdiv_y);
+  free(maxp_x); free(maxp_y); free(maxp_z); free(tc); free(tppc); free(tpcp);
+  free(tcpp); free(r1); free(acc_y); free(acc_z); free(pfc); free(mpres);
+  free(nfbx); free(nfby); free(nfbz); free(fbc_own);
+}
+
+/* the projection half of Solve_momentum_eq (:1866-1921): Poisson RHS,
+ * solve, pressure update, velocity correction, wall enforcement */
+void orc_project(orc_flow *f, int isub) {
+  int nx = f->nx, ny = f->ny, nz = f->nz, px = f->px, py = f->py, pz = f->pz;
+  long Npcc = (long)px * ny * nz, Ncpc = (long)nx * py * nz, Nccp = (long)nx * ny * pz;
+  long Nccc = (long)nx * ny * nz;
+  double *mx = arr(Npcc), *my = arr(Ncpc), *mz = arr(Nccp);
+  double *div = arr(Nccc);
+  for (long t = 0; t < Nccc; ++t) f->pcor[t] = 0.0;
+  orc_divergence(f, div);
+  for (long t = 0; t < Nccc; ++t) f->pcor[t] = f->pcor[t] + div[t];
+  double coeff = 1.0 / (f->ta[isub] * f->sigma2p * f->dt);
+  for (long t = 0; t < Nccc; ++t) f->pcor[t] = f->pcor[t] * coeff;
+  orc_poisson_solve(f, f->pcor);
+  for (long t = 0; t < Nccc; ++t) f->pres[t] = f->pres[t] + f->pcor[t];
+  double s = f->dt * f->ta[isub] * f->sigma2p;
+  op(f, 0, ORC_1DER_C2P, f->ibc[0][VP], f->pcor, nx, ny, nz, mx, px, ny, nz, f->fbx[VP]);
+  for (long t = 0; t < Npcc; ++t) f->qx[t] = f->qx[t] - s * mx[t];
+  op(f, 1, ORC_1DER_C2P, f->ibc[1][VP], f->pcor, nx, ny, nz, my, nx, py, nz, f->fby[VP]);
+  for (long t = 0; t < Ncpc; ++t) f->qy[t] = f->qy[t] - s * my[t];
+  op(f, 2, ORC_1DER_C2P, f->ibc[2][VP], f->pcor, nx, ny, nz, mz, nx, ny, pz, f->fbz[VP]);
+  for (long t = 0; t < Nccp; ++t) f->qz[t] = f->qz[t] - s * mz[t];
+  enforce_velo(f);
+  free(mx); free(my); free(mz); free(div);
+}
+
+/* Solve_momentum_eq, src/eq_momentum2.f90:1799-1932 */
+void orc_solve_momentum_eq(orc_flow *f, int isub) {
+  int nx = f->nx, ny = f->ny, nz = f->nz, px = f->px, py = f->py, pz = f->pz;
+  long Npcc = (long)px * ny * nz, Ncpc = (long)nx * py * nz, Nccp = (long)nx * ny * pz;
+  long Nccc = (long)nx * ny * nz;
+  double *mx = arr(Npcc), *my = arr(Ncpc), *mz = arr(Nccp);
+  orc_momentum_rhs(f, isub, mx, my, mz);
+  for (long t = 0; t < Npcc; ++t) f->qx[t] = f->qx[t] + mx[t];
+  for (long t = 0; t < Ncpc; ++t) f->qy[t] = f->qy[t] + my[t];
+  for (long t = 0; t < Nccp; ++t) f->qz[t] = f->qz[t] + mz[t];
+  enforce_velo(f);
+  /* solve_poisson, :1663-1718 */
+  double *div = arr(Nccc);
+  for (long t = 0; t < Nccc; ++t) f->pcor[t] = 0.0;
+  orc_divergence(f, div);
+  for (long t = 0; t < Nccc; ++t) f->pcor[t] = f->pcor[t] + div[t];
+  double coeff = 1.0 / (f->ta[isub] * f->sigma2p * f->dt);
+  for (long t = 0; t < Nccc; ++t) f->pcor[t] = f->pcor[t] * coeff;
+  orc_poisson_solve(f, f->pcor);
+  for (long t = 0; t < Nccc; ++t) f->pres[t] = f->pres[t] + f->pcor[t];
+  /* Correct_massflux, :1577-1660 */
+  double s = f->dt * f->ta[isub] * f->sigma2p;
+  op(f, 0, ORC_1DER_C2P, f->ibc[0][VP], f->pcor, nx, ny, nz, mx, px, ny, nz, f->fbx[VP]);
+  for (long t = 0; t < Npcc; ++t) f->qx[t] = f->qx[t] - s * mx[t];
+  op(f, 1, ORC_1DER_C2P, f->ibc[1][VP], f->pcor, nx, ny, nz, my, nx, py, nz, f->fby[VP]);
+  for (long t = 0; t < Ncpc; ++t) f->qy[t] = f->qy[t] - s * my[t];
+  op(f, 2, ORC_1DER_C2P, f->ibc[2][VP], f->pcor, nx, ny, nz, mz, nx, ny, pz, f->fbz[VP]);
+  for (long t = 0; t < Nccp; ++t) f->qz[t] = f->qz[t] - s * mz[t];
+  enforce_velo(f);
+  free(mx); free(my); free(mz); free(div);
+}
+
+/* one time step of Solve_eqs_iteration, src/chapsim.f90:243-303 */
+void orc_flow_step(orc_flow *f, int iter) {
+  orc_flow_update_re(f, iter);
+  for (int isub = 1; isub <= 3; ++isub) orc_solve_momentum_eq(f, isub);
+}
