@@ -23,7 +23,23 @@ std::string DevBuf::ensure(size_t b) {
   return "";
 }
 
+bool Handle::prof_pair(hipEvent_t* a, hipEvent_t* b) {
+  if (prof_used + 2 > prof_ev.size()) {
+    if (prof_ev.size() >= 16384) return false;
+    for (int i = 0; i < 256; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return false;
+      prof_ev.push_back(e);
+    }
+  }
+  *a = prof_ev[prof_used];
+  *b = prof_ev[prof_used + 1];
+  prof_used += 2;
+  return true;
+}
+
 Handle::~Handle() {
+  for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
   flow.reset();
   poisson.reset();
   plans64.clear();
@@ -307,6 +323,40 @@ int chs_inspect_op1d(const chs_desc_t* desc, int dir, int op, int iacc, const in
   std::string e = chs::make_line_plan(m.line_geom(dir), chs::Op(op), iacc, ibc, fbc != nullptr, n_in, n_out, &p);
   if (!e.empty()) return fail(CHS_EINVAL, e);
   chs::apply_plan_host(p, fi, fbc, fo);
+  return CHS_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int chs_profile_begin(chs_handle_t* H, int kernel_class) {
+  API_CHECK_HANDLE(H);
+  if (kernel_class < 0 || kernel_class > 2) return fail(CHS_EINVAL, "kernel class must be 0..2");
+  chs::Handle& h = H->h;
+  (void)hipStreamSynchronize(h.stream);
+  h.prof_class = kernel_class;
+  h.prof_used = 0;
+  h.prof_bytes = 0.0;
+  return CHS_OK;
+}
+
+int chs_profile_end(chs_handle_t* H, double* ms_total, long long* launches, double* alg_bytes) {
+  API_CHECK_HANDLE(H);
+  chs::Handle& h = H->h;
+  hipError_t e = hipStreamSynchronize(h.stream);
+  if (e != hipSuccess) return fail(CHS_EHIP, hipGetErrorString(e));
+  double tot = 0.0;
+  for (size_t i = 0; i + 1 < h.prof_used; i += 2) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, h.prof_ev[i], h.prof_ev[i + 1]) != hipSuccess) return fail(CHS_EHIP, "event time");
+    tot += ms;
+  }
+  if (ms_total) *ms_total = tot;
+  if (launches) *launches = (long long)(h.prof_used / 2);
+  if (alg_bytes) *alg_bytes = h.prof_bytes;
+  h.prof_class = -1;
+  h.prof_used = 0;
   return CHS_OK;
 }
 

@@ -359,7 +359,16 @@ std::string Flow::op(int dir, int opk, const int ibc2[2], const Field<double>& A
   o.sOut = sOut;
   o.accumulate = accumulate;
   if (out.size() > scratch_n) return "flow: scratch too small";
-  return launch_line_op<double>(*P, A.e, out.e, in, fbc, o, scratch, H->stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const bool timed = (H->prof_class == dir) && H->prof_pair(&e0, &e1);
+  if (timed) {
+    (void)hipEventRecord(e0, H->stream);
+    const int nin_arrays = 1 + (B ? 1 : 0) + (Cm ? 1 : 0);
+    H->prof_bytes += 8.0 * ((double)A.size() * nin_arrays + (double)out.size() * (1 + (accumulate ? 1 : 0)));
+  }
+  std::string r = launch_line_op<double>(*P, A.e, out.e, in, fbc, o, scratch, H->stream);
+  if (timed) (void)hipEventRecord(e1, H->stream);
+  return r;
 }
 
 std::string Flow::set_state(const double* hx, const double* hy, const double* hz, const double* hp, int mem) {

@@ -8,6 +8,7 @@
 #include <memory>
 #include <string>
 #include <tuple>
+#include <vector>
 
 #include "../../include/chapsim_hip.h"
 #include "chs_geom.h"
@@ -41,7 +42,15 @@ struct Handle {
   DevBuf io[4];
   std::unique_ptr<Poisson> poisson;
   std::unique_ptr<Flow> flow;
+  // launch timing of one kernel class (0/1/2: x/y/z line operators) with
+  // events on the handle's stream, and the algorithmic bytes of those launches
+  int prof_class = -1;
+  std::vector<hipEvent_t> prof_ev;
+  size_t prof_used = 0;
+  double prof_bytes = 0.0;
   ~Handle();
+  // event pair for the next timed launch (nullptr pair when the pool is full)
+  bool prof_pair(hipEvent_t* a, hipEvent_t* b);
 
   template <typename T>
   std::string plan(int dir, int op, int iacc, const int ibc[2], bool has_fbc, int n_in, int n_out,

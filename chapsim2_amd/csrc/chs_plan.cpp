@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <utility>
 
 namespace chs {
 
@@ -624,6 +625,94 @@ std::string make_line_plan(const LineGeom& g, Op op, int acc, const int ibc0[2],
       p.solve = 1;
     }
   }
+  // partitioned form of the same tridiagonal system
+  if (p.solve) {
+    const int n = n_out;
+    std::vector<double> a(n), b(n), c(n);
+    for (int i = 0; i < n; ++i) {
+      int row = (i == 0) ? 1 : (i == 1) ? 2 : (i == n - 2) ? 4 : (i == n - 1) ? 5 : 3;
+      int bc = (i < 2) ? p.ibc[0] : (i >= n - 2) ? p.ibc[1] : BC_PERIODIC;
+      a[i] = coeff(fam, false, row, 1, bc, acc);
+      b[i] = coeff(fam, false, row, 2, bc, acc);
+      c[i] = coeff(fam, false, row, 3, bc, acc);
+    }
+    if (!per) {
+      a[0] = 0.0;      // no coupling outside the line
+      c[n - 1] = 0.0;
+    }
+    p.a_raw = a; p.b_raw = b; p.c_raw = c;
+    int P = (n + kChunk - 1) / kChunk;
+    if (P < 1) P = 1;
+    if (P > kMaxChunks) return "line too long for the partitioned solve";
+    p.part_P = P;
+    p.part_start.resize(P + 1);
+    for (int q = 0; q <= P; ++q) p.part_start[q] = (int)((long)q * n / P);
+    for (int q = 0; q < P; ++q)
+      if (p.part_start[q + 1] - p.part_start[q] < 2) return "chunk shorter than two rows";
+    p.ld.assign(n, 0.0); p.lc.assign(n, 0.0); p.sv.assign(n, 0.0); p.sw.assign(n, 0.0);
+    for (int q = 0; q < P; ++q) {
+      const int s0 = p.part_start[q], e0 = p.part_start[q + 1];
+      // local factors
+      for (int i = s0; i < e0; ++i) {
+        if (i == s0) {
+          p.ld[i] = 1.0 / b[i];
+          p.lc[i] = c[i] * p.ld[i];
+        } else {
+          p.ld[i] = 1.0 / (b[i] - a[i] * p.lc[i - 1]);
+          p.lc[i] = c[i] * p.ld[i];
+        }
+      }
+      auto local_solve = [&](std::vector<double>& x) {
+        for (int i = s0; i < e0; ++i) x[i] = (i == s0) ? x[i] * p.ld[i] : (x[i] - a[i] * x[i - 1]) * p.ld[i];
+        for (int i = e0 - 2; i >= s0; --i) x[i] = x[i] - p.lc[i] * x[i + 1];
+      };
+      std::vector<double> v(n, 0.0), w(n, 0.0);
+      v[s0] = a[s0];
+      w[e0 - 1] = c[e0 - 1];
+      local_solve(v);
+      local_solve(w);
+      for (int i = s0; i < e0; ++i) { p.sv[i] = v[i]; p.sw[i] = w[i]; }
+    }
+    // reduced system on (F_q, L_q): F_q + sv[s]L_{q-1} + sw[s]F_{q+1} = yF_q, same for L_q
+    const int M = 2 * P;
+    std::vector<double> R((size_t)M * M, 0.0);
+    for (int q = 0; q < P; ++q) {
+      const int s0 = p.part_start[q], e0 = p.part_start[q + 1];
+      const int qm = (q + P - 1) % P, qp = (q + 1) % P;
+      for (int t = 0; t < 2; ++t) {
+        const int row = 2 * q + t, i = t ? e0 - 1 : s0;
+        R[(size_t)row * M + row] += 1.0;
+        if (per || q > 0) R[(size_t)row * M + 2 * qm + 1] += p.sv[i];
+        if (per || q < P - 1) R[(size_t)row * M + 2 * qp] += p.sw[i];
+      }
+    }
+    // invert R by Gauss-Jordan with partial pivoting
+    std::vector<double> I((size_t)M * M, 0.0);
+    for (int i = 0; i < M; ++i) I[(size_t)i * M + i] = 1.0;
+    for (int col = 0; col < M; ++col) {
+      int piv = col;
+      for (int r = col + 1; r < M; ++r)
+        if (std::fabs(R[(size_t)r * M + col]) > std::fabs(R[(size_t)piv * M + col])) piv = r;
+      if (std::fabs(R[(size_t)piv * M + col]) < 1e-300) return "singular reduced system";
+      if (piv != col)
+        for (int k = 0; k < M; ++k) {
+          std::swap(R[(size_t)piv * M + k], R[(size_t)col * M + k]);
+          std::swap(I[(size_t)piv * M + k], I[(size_t)col * M + k]);
+        }
+      const double inv = 1.0 / R[(size_t)col * M + col];
+      for (int k = 0; k < M; ++k) { R[(size_t)col * M + k] *= inv; I[(size_t)col * M + k] *= inv; }
+      for (int r = 0; r < M; ++r) {
+        if (r == col) continue;
+        const double f = R[(size_t)r * M + col];
+        if (f == 0.0) continue;
+        for (int k = 0; k < M; ++k) {
+          R[(size_t)r * M + k] -= f * R[(size_t)col * M + k];
+          I[(size_t)r * M + k] -= f * I[(size_t)col * M + k];
+        }
+      }
+    }
+    p.rinv = I;
+  }
   // y metric and Neumann overwrite
   if (g.dir == 1 && is_derivative(op)) {
     if (g.stretched) {
@@ -649,42 +738,42 @@ void apply_plan_host(const LinePlan& p, const double* f, const double* fbc, doub
       for (int m = 0; m < 4; ++m) v = v + R.u[m] * fbc[m];
     return v * p.dd;
   };
-  auto emit = [&](int r, double val) {
+  std::vector<double> y(n_out);
+  for (int r = 0; r < n_out; ++r) {
+    if (r < 2) y[r] = special(r);
+    else if (r >= n_out - 2) y[r] = special(r - n_out + 4);
+    else
+      y[r] = (p.c1 * (f[clampi(r + p.p1)] + p.sgn * f[clampi(r + p.p2)]) +
+              p.c2 * (f[clampi(r + p.p3)] + p.sgn * f[clampi(r + p.p4)])) * p.dd;
+  }
+  if (p.solve) {
+    const int P = p.part_P, M = 2 * P;
+    std::vector<double> Y(M);
+    for (int q = 0; q < P; ++q) {
+      const int s0 = p.part_start[q], e0 = p.part_start[q + 1];
+      for (int i = s0; i < e0; ++i) y[i] = (i == s0) ? y[i] * p.ld[i] : (y[i] - p.a_raw[i] * y[i - 1]) * p.ld[i];
+      for (int i = e0 - 2; i >= s0; --i) y[i] = y[i] - p.lc[i] * y[i + 1];
+      Y[2 * q] = y[s0];
+      Y[2 * q + 1] = y[e0 - 1];
+    }
+    std::vector<double> U(M, 0.0);
+    for (int r = 0; r < M; ++r) {
+      double acc = 0.0;
+      for (int k = 0; k < M; ++k) acc = acc + p.rinv[(size_t)r * M + k] * Y[k];
+      U[r] = acc;
+    }
+    for (int q = 0; q < P; ++q) {
+      const int s0 = p.part_start[q], e0 = p.part_start[q + 1];
+      const double Lm = U[2 * ((q + P - 1) % P) + 1], Fp = U[2 * ((q + 1) % P)];
+      for (int i = s0; i < e0; ++i) y[i] = y[i] - p.sv[i] * Lm - p.sw[i] * Fp;
+    }
+  }
+  for (int r = 0; r < n_out; ++r) {
+    double val = y[r];
     if (!p.metric.empty()) val = val * p.metric[r];
     if (r == 0 && p.neu_lo) val = fbc[0];
     if (r == n_out - 1 && p.neu_hi) val = fbc[1];
     out[r] = val;
-  };
-  std::vector<double> rhs(n_out);
-  for (int r = 0; r < n_out; ++r) {
-    if (r < 2) rhs[r] = special(r);
-    else if (r >= n_out - 2) rhs[r] = special(r - n_out + 4);
-    else
-      rhs[r] = (p.c1 * (f[clampi(r + p.p1)] + p.sgn * f[clampi(r + p.p2)]) +
-                p.c2 * (f[clampi(r + p.p3)] + p.sgn * f[clampi(r + p.p4)])) * p.dd;
-  }
-  if (p.solve == 0) {
-    for (int r = 0; r < n_out; ++r) emit(r, rhs[r]);
-    return;
-  }
-  const int m = (p.solve == 2) ? n_out - 1 : n_out;
-  std::vector<double> y(n_out);
-  double S = 0.0;
-  for (int r = 0; r < m; ++r) {
-    y[r] = (r == 0) ? rhs[r] / p.b0 : (rhs[r] - p.a[r] * y[r - 1]) * p.d[r];
-    if (p.solve == 2) S = S + p.pw[r] * y[r];
-  }
-  double xn = 0.0;
-  if (p.solve == 2) {
-    xn = (rhs[n_out - 1] - p.cyc_cn * S - p.cyc_an * y[m - 1]) / p.cyc_den;
-    emit(n_out - 1, xn);
-  }
-  double xnext = y[m - 1];
-  emit(m - 1, (p.solve == 2) ? xnext + p.x1[m - 1] * xn : xnext);
-  for (int r = m - 2; r >= 0; --r) {
-    double xb = y[r] - p.c[r] * xnext;
-    xnext = xb;
-    emit(r, (p.solve == 2) ? xb + p.x1[r] * xn : xb);
   }
 }
 

@@ -38,6 +38,8 @@ struct LineGeom {
 };
 
 constexpr int kMaxRowTerms = 12;
+constexpr int kChunk = 16;      // rows per thread in the partitioned solve
+constexpr int kMaxChunks = 16;  // chunks (waves) per line
 
 // One boundary row: rhs = (sum_t w[t] * f[idx[t]] + sum_m u[m] * fbc[m]) * dd
 struct RowRecipe {
@@ -66,6 +68,15 @@ struct LinePlan {
   std::vector<double> pw;      // cyclic: weights giving the basic solution's first entry
   std::vector<double> x1;      // cyclic: RHS-independent correction vector
   double cyc_an = 0, cyc_cn = 0, cyc_den = 1.0;
+  // Partitioned solve used by the GPU kernels: the line is split into P chunks
+  // of at most kChunk rows; each chunk is solved locally (ld/lc local Thomas
+  // factors) and corrected with precomputed spike vectors sv (coupling to the
+  // previous chunk's last unknown) and sw (to the next chunk's first unknown),
+  // whose values follow from the 2P x 2P reduced system (inverse rinv).
+  int part_P = 1;
+  std::vector<int> part_start;     // P+1 chunk boundaries
+  std::vector<double> a_raw, b_raw, c_raw;
+  std::vector<double> ld, lc, sv, sw, rinv;
   // post-processing (y derivatives): metric factor per output row, Neumann overwrite
   std::vector<double> metric;
   bool neu_lo = false, neu_hi = false;

@@ -27,6 +27,14 @@ double chs_inspect_coeff(int fam, int is_rhs, int row, int k, int bc, int acc);
 int chs_inspect_op1d(const chs_desc_t *desc, int dir, int op, int iacc, const int ibc[2],
                      const double *fi, int n_in, double *fo, int n_out, const double *fbc);
 
+/* Launch timing of one kernel class during subsequent solver calls, with HIP
+ * events on the handle's stream: class 0/1/2 = x/y/z line operators of the
+ * momentum right-hand side and projection.  chs_profile_end returns the summed
+ * event time, the number of timed launches and their algorithmic bytes
+ * (arrays read plus arrays written, 8 bytes per element). */
+int chs_profile_begin(chs_handle_t *h, int kernel_class);
+int chs_profile_end(chs_handle_t *h, double *ms_total, long long *launches, double *alg_bytes);
+
 #ifdef __cplusplus
 }
 #endif
