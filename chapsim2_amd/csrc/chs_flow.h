@@ -11,6 +11,10 @@
 
 namespace chs {
 
+// block partials used by the max reductions (the partial buffer holds at
+// least this many entries)
+constexpr int kMaxPartials = 512;
+
 // A device array with its Fortran extents.
 template <typename T>
 struct Field {
@@ -56,6 +60,7 @@ class Flow {
   double *fby_conv = nullptr, *fby_visc = nullptr, *fby_div = nullptr;
   double *plane_a = nullptr, *plane_b = nullptr, *plane_c = nullptr, *plane_d = nullptr;
   double *partial = nullptr, *bulk = nullptr;
+  long npartial = 0;  // entries of partial (at least kMaxPartials)
   double* scratch = nullptr;
   long scratch_n = 0;
   double* dyw_dev = nullptr;

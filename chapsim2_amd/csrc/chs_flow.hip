@@ -165,6 +165,9 @@ __global__ void k_neg(double* __restrict__ o, const double* __restrict__ a, long
   if (i < n) o[i] = -a[i];
 }
 
+// maximum that propagates NaN (fmax would drop it and hide a blow-up)
+__device__ __forceinline__ double nanmax(double a, double b) { return (b > a || b != b) ? b : a; }
+
 // max |div| over x windows [lo, hi] (0-based inclusive), per block partials
 __global__ void k_maxabs_xwin(const double* __restrict__ d, int n1, long n, int lo, int hi,
                               double* __restrict__ partial) {
@@ -172,12 +175,12 @@ __global__ void k_maxabs_xwin(const double* __restrict__ d, int n1, long n, int 
   double m = 0.0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     int x = (int)(i % n1);
-    if (x >= lo && x <= hi) m = fmax(m, fabs(d[i]));
+    if (x >= lo && x <= hi) m = nanmax(m, fabs(d[i]));
   }
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    if ((int)threadIdx.x < s) red[threadIdx.x] = nanmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
@@ -321,7 +324,8 @@ std::string Flow::create(Handle* H, std::unique_ptr<Flow>* out) {
     f.owned.push_back(*p);
   }
   {
-    long nblk = (f.qx.size() + 255) / 256;
+    long nblk = std::max<long>((f.qx.size() + 255) / 256, kMaxPartials);
+    f.npartial = nblk;
     FCHK(hipMalloc(&f.partial, sizeof(double) * (size_t)(nblk + 8)));
     f.owned.push_back(f.partial);
     FCHK(hipMalloc(&f.bulk, sizeof(double) * 8));
@@ -542,7 +546,7 @@ std::string Flow::substep(int isub) {
 std::string Flow::mass_check(double out[3]) {
   SCHK(divergence(div.p));
   const int lo[3] = {0, 4, nx - 6}, hi[3] = {3, nx - 5, nx - 1};
-  const int nb = 512;
+  const int nb = kMaxPartials;
   std::vector<double> h(nb);
   for (int w = 0; w < 3; ++w) {
     hipLaunchKernelGGL(k_maxabs_xwin, dim3(nb), dim3(256), 0, H->stream, div.p, nx, div.size(), lo[w], hi[w],
@@ -550,7 +554,7 @@ std::string Flow::mass_check(double out[3]) {
     FCHK(hipMemcpyAsync(h.data(), partial, sizeof(double) * nb, hipMemcpyDeviceToHost, H->stream));
     FCHK(hipStreamSynchronize(H->stream));
     double mx = 0.0;
-    for (double v : h) mx = std::max(mx, v);
+    for (double v : h) mx = (v > mx || v != v) ? v : mx;
     out[w] = mx;
   }
   return "";

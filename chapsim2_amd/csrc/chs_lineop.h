@@ -32,6 +32,13 @@ struct DevPlan {
   T* pw = nullptr;
   T* x1 = nullptr;
   T* metric = nullptr;
+  // partitioned solve (LinePlan::part_P chunks per line)
+  T* ld = nullptr;
+  T* lc = nullptr;
+  T* ar = nullptr;
+  T* sv = nullptr;
+  T* sw = nullptr;
+  T* rinv = nullptr;
   ~DevPlan();
 };
 

@@ -1,6 +1,26 @@
 // Compact line operators on gfx950.
 //
-// One thread owns one line.  For y- and z-lines consecutive threads own
+// Partitioned kernels (k_line_yz, k_line_x; lines of up to kChunk*kMaxChunks
+// rows).  A line of n rows is cut into P = ceil(n/kChunk) chunks and one
+// thread owns one chunk, so a launch carries P times more threads than lines
+// (a 256x128x256 grid has only 32768-65536 lines per direction, which is too
+// few for 256 CUs).  Each thread evaluates the explicit RHS of its rows from a
+// register window, runs the chunk-local Thomas sweeps with host-factored
+// coefficients, publishes its first and last values through LDS, and corrects
+// its rows with the spike vectors and two rows of the inverse of the 2P x 2P
+// reduced system (the same algebra for wall-bounded and periodic lines; a
+// periodic line's corner entries close the reduced system cyclically).
+//  - y and z lines: a workgroup is 64 neighbouring lines x P chunks; wave q of
+//    the workgroup holds chunk q of 64 lines, so every load and store of a
+//    wavefront is a contiguous 64-element segment along x and the plan's
+//    per-row coefficients are wave-uniform.
+//  - x lines (contiguous in memory): a workgroup stages 256/P whole lines in
+//    LDS with coalesced loads (applying the input transform), threads work on
+//    their chunks from LDS, and the results go back through LDS to coalesced
+//    stores.  Row k of a line sits at LDS offset k + k/16 so that the 16 chunks
+//    of a line fall into distinct banks.
+//
+// Single-thread-per-line kernel (k_line; longer lines).  For y- and z-lines consecutive threads own
 // consecutive x positions, so every load and store of a wavefront is a
 // contiguous 64-element segment.  The explicit RHS (src/basics_operations2.f90
 // :1980-2719) is evaluated on the fly with a 5-point register window; the four
@@ -26,7 +46,8 @@ namespace chs {
 
 template <typename T>
 DevPlan<T>::~DevPlan() {
-  for (void* p : {(void*)rows, (void*)a, (void*)c, (void*)d, (void*)pw, (void*)x1, (void*)metric})
+  for (void* p : {(void*)rows, (void*)a, (void*)c, (void*)d, (void*)pw, (void*)x1, (void*)metric, (void*)ld,
+                  (void*)lc, (void*)ar, (void*)sv, (void*)sw, (void*)rinv})
     if (p) (void)hipFree(p);
 }
 
@@ -60,8 +81,16 @@ std::string upload_plan(const LinePlan& p, DevPlan<T>* dp) {
   if (!(e = up(p.pw, &dp->pw)).empty()) return e;
   if (!(e = up(p.x1, &dp->x1)).empty()) return e;
   if (!(e = up(p.metric, &dp->metric)).empty()) return e;
+  if (!(e = up(p.ld, &dp->ld)).empty()) return e;
+  if (!(e = up(p.lc, &dp->lc)).empty()) return e;
+  if (!(e = up(p.a_raw, &dp->ar)).empty()) return e;
+  if (!(e = up(p.sv, &dp->sv)).empty()) return e;
+  if (!(e = up(p.sw, &dp->sw)).empty()) return e;
+  if (!(e = up(p.rinv, &dp->rinv)).empty()) return e;
   return "";
 }
+
+static inline int lds_pos_host(int k) { return k + (k >> 4); }
 
 template <typename T>
 struct KArgs {
@@ -79,6 +108,9 @@ struct KArgs {
   const T* metric; int neu_lo, neu_hi;
   T* out; T sOut; int accumulate;
   T* scratch;
+  // partitioned form
+  int P, per, ls, nl;
+  const T* ld; const T* lc; const T* ar; const T* sv; const T* sw; const T* rinv;
 };
 
 // bulk offsets (p1,p2,p3,p4) per operator kind, relative to the output row
@@ -191,6 +223,247 @@ __global__ void __launch_bounds__(64) k_line(KArgs<T> P) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// partitioned kernels
+
+// Chunk rows [s0, s0+len) of one line: explicit RHS into y[].  FW(t) returns
+// the transformed input at line index s0-2+t from the fast window source,
+// FA(k) at an arbitrary line index k (boundary recipes).
+template <typename T, int OPK, class FW, class FA>
+__device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int s0, int len, FW&& Fw, FA&& Fa,
+                                          const T (&fb4)[4], T (&y)[kChunk]) {
+  using O = Offs<OPK>;
+  T f[kChunk + 4];
+#pragma unroll
+  for (int t = 0; t < kChunk + 4; ++t) f[t] = (t < len + 4) ? Fw(t) : T(0);
+#pragma unroll
+  for (int i = 0; i < kChunk; ++i)
+    y[i] = (P.c1 * (f[i + 2 + O::p1] + P.sgn * f[i + 2 + O::p2]) +
+            P.c2 * (f[i + 2 + O::p3] + P.sgn * f[i + 2 + O::p4])) * P.dd;
+  auto special = [&](int rq) -> T {
+    const DevRow<T>& R = P.rows[rq];
+    T v = T(0);
+    for (int t = 0; t < R.nterm; ++t) v = v + R.w[t] * Fa(R.idx[t]);
+    if (P.fbc)
+      for (int m = 0; m < 4; ++m) v = v + R.u[m] * fb4[m];
+    return v * P.dd;
+  };
+  if (q == 0) {
+    y[0] = special(0);
+    y[1] = special(1);
+  }
+  if (q == P.P - 1) {
+    const T v2 = special(2), v3 = special(3);
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) {
+      if (i == len - 2) y[i] = v2;
+      if (i == len - 1) y[i] = v3;
+    }
+  }
+}
+
+// Chunk-local Thomas sweeps; returns the chunk's first and last values.
+template <typename T>
+__device__ __forceinline__ void chunk_solve(const KArgs<T>& P, int s0, int len, T (&y)[kChunk], T& yf, T& yl) {
+#pragma unroll
+  for (int i = 0; i < kChunk; ++i)
+    if (i < len) y[i] = (i == 0) ? y[i] * P.ld[s0] : (y[i] - P.ar[s0 + i] * y[i - 1]) * P.ld[s0 + i];
+#pragma unroll
+  for (int i = kChunk - 2; i >= 0; --i)
+    if (i < len - 1) y[i] = y[i] - P.lc[s0 + i] * y[i + 1];
+  yf = y[0];
+  yl = y[0];
+#pragma unroll
+  for (int i = 1; i < kChunk; ++i)
+    if (i == len - 1) yl = y[i];
+}
+
+// Reduced-system values coupling chunk q to its neighbours; ex(k) reads the
+// published first/last value k (0..2P-1) of the thread's line.
+template <typename T, class EX>
+__device__ __forceinline__ void chunk_couple(const KArgs<T>& P, int q, int s0, int len, EX&& ex, T (&y)[kChunk]) {
+  const int Pn = P.P, M = 2 * Pn;
+  const int qm = (q + Pn - 1) % Pn, qp = (q + 1) % Pn;
+  const T* rL = P.rinv + (long)(2 * qm + 1) * M;
+  const T* rF = P.rinv + (long)(2 * qp) * M;
+  T uL = T(0), uF = T(0);
+  for (int k = 0; k < M; ++k) {
+    const T v = ex(k);
+    uL = uL + rL[k] * v;
+    uF = uF + rF[k] * v;
+  }
+  if (!P.per && q == 0) uL = T(0);
+  if (!P.per && q == Pn - 1) uF = T(0);
+#pragma unroll
+  for (int i = 0; i < kChunk; ++i)
+    if (i < len) y[i] = y[i] - P.sv[s0 + i] * uL - P.sw[s0 + i] * uF;
+}
+
+template <typename T>
+__device__ __forceinline__ T chunk_post(const KArgs<T>& P, int r, T val, const T (&fb4)[4]) {
+  if (P.metric) val = val * P.metric[r];
+  if (r == 0 && P.neu_lo) val = fb4[0];
+  if (r == P.n_out - 1 && P.neu_hi) val = fb4[1];
+  return val;
+}
+
+template <typename T, int DIR, int OPK>
+__global__ void __launch_bounds__(64 * kMaxChunks) k_line_yz(KArgs<T> P) {
+  __shared__ T ex[2 * kMaxChunks * 64];
+  const int lane = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long Lr = (long)blockIdx.x * 64 + lane;
+  const bool live = Lr < P.nlines;
+  const long L = live ? Lr : P.nlines - 1;
+  long ib, is, ob, os, fb, fs;
+  if (DIR == 1) {
+    const long i = L % P.e1, k = L / P.e1;
+    ib = i + (long)P.e1 * P.e2 * k; is = P.e1;
+    ob = i + (long)P.o1 * P.o2 * k; os = P.o1;
+    fb = i + (long)P.e1 * 4 * k; fs = P.e1;
+  } else {
+    const long i = L % P.e1, j = L / P.e1;
+    ib = i + (long)P.e1 * j; is = (long)P.e1 * P.e2;
+    ob = i + (long)P.o1 * j; os = (long)P.o1 * P.o2;
+    fb = i + (long)P.e1 * j; fs = (long)P.e1 * P.e2;
+  }
+  const int n_in = P.n_in, n_out = P.n_out, Pn = P.P;
+  const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
+  auto Fa = [&](int k) -> T {
+    const long o = ib + is * k;
+    T t = P.A[o];
+    if (P.B) t = t + P.cB * P.B[o];
+    t = P.sA * t;
+    if (P.Cm) t = t * P.Cm[o];
+    return t;
+  };
+  auto Fw = [&](int t) -> T {
+    int k = s0 - 2 + t;
+    k = k < 0 ? 0 : (k >= n_in ? n_in - 1 : k);
+    return Fa(k);
+  };
+  T fb4[4] = {T(0), T(0), T(0), T(0)};
+  if (P.fbc && (q == 0 || q == Pn - 1))
+    for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[fb + fs * m];
+  T y[kChunk];
+  chunk_rhs<T, OPK>(P, q, s0, len, Fw, Fa, fb4, y);
+  if (P.solve) {
+    T yf, yl;
+    chunk_solve<T>(P, s0, len, y, yf, yl);
+    ex[lane + 64 * (2 * q)] = yf;
+    ex[lane + 64 * (2 * q + 1)] = yl;
+    __syncthreads();
+    chunk_couple<T>(P, q, s0, len, [&](int k) { return ex[lane + 64 * k]; }, y);
+  }
+  if (!live) return;
+#pragma unroll
+  for (int i = 0; i < kChunk; ++i) {
+    if (i < len) {
+      const int r = s0 + i;
+      const T val = chunk_post<T>(P, r, y[i], fb4);
+      const long o = ob + os * r;
+      const T prev = P.accumulate ? P.out[o] : T(0);
+      P.out[o] = prev + val * P.sOut;
+    }
+  }
+}
+
+__device__ __forceinline__ int lds_pos(int k) { return k + (k >> 4); }
+
+template <typename T, int OPK>
+__global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* tile = reinterpret_cast<T*>(smem_raw);
+  const int NL = P.nl, LS = P.ls, Pn = P.P;
+  T* exb = tile + (long)NL * LS;
+  const int tid = threadIdx.x;
+  const int nthr = NL * Pn;
+  const int ll = tid / Pn, q = tid - ll * Pn;
+  const long L0 = (long)blockIdx.x * NL;
+  const int n_in = P.n_in, n_out = P.n_out;
+  // stage NL lines (input transform applied)
+  for (int e = tid; e < NL * n_in; e += nthr) {
+    const int l = e / n_in, k = e - l * n_in;
+    const long L = L0 + l;
+    T v = T(0);
+    if (L < P.nlines) {
+      const long j = L % P.e2, kk = L / P.e2;
+      const long o = (j + (long)P.e2 * kk) * P.e1 + k;
+      v = P.A[o];
+      if (P.B) v = v + P.cB * P.B[o];
+      v = P.sA * v;
+      if (P.Cm) v = v * P.Cm[o];
+    }
+    tile[(long)l * LS + lds_pos(k)] = v;
+  }
+  __syncthreads();
+  const long L = L0 + ll;
+  const bool live = L < P.nlines;
+  const T* row = tile + (long)ll * LS;
+  const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
+  auto Fa = [&](int k) -> T { return row[lds_pos(k)]; };
+  auto Fw = [&](int t) -> T {
+    int k = s0 - 2 + t;
+    k = k < 0 ? 0 : (k >= n_in ? n_in - 1 : k);
+    return row[lds_pos(k)];
+  };
+  T fb4[4] = {T(0), T(0), T(0), T(0)};
+  if (P.fbc && live && (q == 0 || q == Pn - 1)) {
+    const long j = L % P.e2, kk = L / P.e2;
+    for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * (j + (long)P.e2 * kk) + m];
+  }
+  T y[kChunk];
+  chunk_rhs<T, OPK>(P, q, s0, len, Fw, Fa, fb4, y);
+  if (P.solve) {
+    T yf, yl;
+    chunk_solve<T>(P, s0, len, y, yf, yl);
+    T* ex = exb + (long)ll * 2 * Pn;
+    ex[2 * q] = yf;
+    ex[2 * q + 1] = yl;
+    __syncthreads();
+    chunk_couple<T>(P, q, s0, len, [&](int k) { return ex[k]; }, y);
+  }
+  __syncthreads();  // every thread has finished reading its input rows
+  T* orow = tile + (long)ll * LS;
+#pragma unroll
+  for (int i = 0; i < kChunk; ++i)
+    if (i < len) orow[lds_pos(s0 + i)] = chunk_post<T>(P, s0 + i, y[i], fb4);
+  __syncthreads();
+  for (int e = tid; e < NL * n_out; e += nthr) {
+    const int l = e / n_out, k = e - l * n_out;
+    const long Lw = L0 + l;
+    if (Lw >= P.nlines) continue;
+    const long j = Lw % P.e2, kk = Lw / P.e2;
+    const long o = (j + (long)P.o2 * kk) * P.o1 + k;
+    const T val = tile[(long)l * LS + lds_pos(k)];
+    const T prev = P.accumulate ? P.out[o] : T(0);
+    P.out[o] = prev + val * P.sOut;
+  }
+}
+
+template <typename T, int DIR>
+static void dispatch_part(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
+  if (DIR == 0) {
+    switch (op) {
+      case MIDP_C2P: hipLaunchKernelGGL((k_line_x<T, MIDP_C2P>), g, b, shm, s, a); break;
+      case MIDP_P2C: hipLaunchKernelGGL((k_line_x<T, MIDP_P2C>), g, b, shm, s, a); break;
+      case D1_C2C: hipLaunchKernelGGL((k_line_x<T, D1_C2C>), g, b, shm, s, a); break;
+      case D1_P2P: hipLaunchKernelGGL((k_line_x<T, D1_P2P>), g, b, shm, s, a); break;
+      case D1_C2P: hipLaunchKernelGGL((k_line_x<T, D1_C2P>), g, b, shm, s, a); break;
+      case D1_P2C: hipLaunchKernelGGL((k_line_x<T, D1_P2C>), g, b, shm, s, a); break;
+    }
+  } else {
+    switch (op) {
+      case MIDP_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_C2P>), g, b, 0, s, a); break;
+      case MIDP_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_P2C>), g, b, 0, s, a); break;
+      case D1_C2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2C>), g, b, 0, s, a); break;
+      case D1_P2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2P>), g, b, 0, s, a); break;
+      case D1_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2P>), g, b, 0, s, a); break;
+      case D1_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2C>), g, b, 0, s, a); break;
+    }
+  }
+}
+
 template <typename T, int DIR>
 static void dispatch_op(int op, dim3 g, dim3 b, hipStream_t s, const KArgs<T>& a) {
   switch (op) {
@@ -212,8 +485,7 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
   for (int k = 0; k < 3; ++k)
     if (k != dir && ein.n[k] != eout.n[k]) return "line op: cross extents of input and output differ";
   if (p.has_fbc != (fbc != nullptr)) return "line op: fbc presence differs from plan";
-  if (p.solve && !scratch) return "line op: scratch required";
-  KArgs<T> a;
+  KArgs<T> a{};
   a.n_in = p.n_in; a.n_out = p.n_out;
   a.e1 = ein.n[0]; a.e2 = ein.n[1];
   a.o1 = eout.n[0]; a.o2 = eout.n[1];
@@ -229,6 +501,34 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
   a.out = out.out; a.sOut = out.sOut; a.accumulate = out.accumulate;
   a.scratch = scratch;
   if (a.nlines <= 0) return "";
+  if (p.part_P <= kMaxChunks && p.n_out >= 4) {
+    a.P = p.part_P;
+    a.per = (p.solve == 2);
+    a.ld = D.ld; a.lc = D.lc; a.ar = D.ar; a.sv = D.sv; a.sw = D.sw; a.rinv = D.rinv;
+    if (dir == 0) {
+      const int nmax = p.n_in > p.n_out ? p.n_in : p.n_out;
+      int ls = lds_pos_host(nmax) + 1;
+      ls += ((8 - ls) % 32 + 32) % 32;  // row stride = 8 (mod 32) doubles: neighbouring lines in other banks
+      a.ls = ls;
+      // lines per workgroup: at most 256 threads and 48 KiB of LDS
+      int nl = 256 / p.part_P;
+      const int cap = (int)((48 * 1024) / (sizeof(T) * (size_t)(ls + 2 * p.part_P)));
+      if (nl > cap) nl = cap;
+      if (nl < 1) return "line op: line does not fit in LDS";
+      a.nl = nl;
+      const size_t shm = sizeof(T) * ((size_t)a.nl * ls + (size_t)a.nl * 2 * p.part_P);
+      dim3 g((unsigned)((a.nlines + a.nl - 1) / a.nl)), b((unsigned)(a.nl * p.part_P));
+      dispatch_part<T, 0>(p.op, g, b, shm, s, a);
+    } else {
+      dim3 g((unsigned)((a.nlines + 63) / 64)), b((unsigned)(64 * p.part_P));
+      if (dir == 1) dispatch_part<T, 1>(p.op, g, b, 0, s, a);
+      else dispatch_part<T, 2>(p.op, g, b, 0, s, a);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return std::string("line op launch: ") + hipGetErrorString(e);
+    return "";
+  }
+  if (p.solve && !scratch) return "line op: scratch required";
   dim3 b(64), g((unsigned)((a.nlines + 63) / 64));
   if (dir == 0) dispatch_op<T, 0>(p.op, g, b, s, a);
   else if (dir == 1) dispatch_op<T, 1>(p.op, g, b, s, a);

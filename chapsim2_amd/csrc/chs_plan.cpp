@@ -625,6 +625,17 @@ std::string make_line_plan(const LineGeom& g, Op op, int acc, const int ibc0[2],
       p.solve = 1;
     }
   }
+  // chunk boundaries of the partitioned evaluation (all plans)
+  {
+    const int n = n_out;
+    int P = (n + kChunk - 1) / kChunk;
+    if (P < 1) P = 1;
+    p.part_P = P;
+    p.part_start.resize(P + 1);
+    for (int q = 0; q <= P; ++q) p.part_start[q] = (int)((long)q * n / P);
+    for (int q = 0; q < P; ++q)
+      if (p.part_start[q + 1] - p.part_start[q] < 2) return "chunk shorter than two rows";
+  }
   // partitioned form of the same tridiagonal system
   if (p.solve) {
     const int n = n_out;
@@ -641,14 +652,7 @@ std::string make_line_plan(const LineGeom& g, Op op, int acc, const int ibc0[2],
       c[n - 1] = 0.0;
     }
     p.a_raw = a; p.b_raw = b; p.c_raw = c;
-    int P = (n + kChunk - 1) / kChunk;
-    if (P < 1) P = 1;
-    if (P > kMaxChunks) return "line too long for the partitioned solve";
-    p.part_P = P;
-    p.part_start.resize(P + 1);
-    for (int q = 0; q <= P; ++q) p.part_start[q] = (int)((long)q * n / P);
-    for (int q = 0; q < P; ++q)
-      if (p.part_start[q + 1] - p.part_start[q] < 2) return "chunk shorter than two rows";
+    const int P = p.part_P;
     p.ld.assign(n, 0.0); p.lc.assign(n, 0.0); p.sv.assign(n, 0.0); p.sw.assign(n, 0.0);
     for (int q = 0; q < P; ++q) {
       const int s0 = p.part_start[q], e0 = p.part_start[q + 1];

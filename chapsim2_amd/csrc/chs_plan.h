@@ -73,6 +73,8 @@ struct LinePlan {
   // factors) and corrected with precomputed spike vectors sv (coupling to the
   // previous chunk's last unknown) and sw (to the next chunk's first unknown),
   // whose values follow from the 2P x 2P reduced system (inverse rinv).
+  // Chunk q holds rows [q*n_out/P, (q+1)*n_out/P) with P = ceil(n_out/kChunk);
+  // the GPU kernels use this form when P <= kMaxChunks.
   int part_P = 1;
   std::vector<int> part_start;     // P+1 chunk boundaries
   std::vector<double> a_raw, b_raw, c_raw;

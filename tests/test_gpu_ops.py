@@ -107,3 +107,47 @@ def test_ops_polynomial_exact_on_gpu(chan):
     lhs = outs[2]
     rhs = 2.0 * outs[0] - 3.0 * outs[1]
     assert np.abs(lhs - rhs).max() / np.abs(rhs).max() < 1e-12
+
+
+NC_LONG = (250, 130, 256)
+
+
+def test_ops_long_lines(po):
+    """Line lengths 130-257 (9 to 16 chunks per line in the partitioned
+    kernels, uneven chunk lengths for 130/131/250) against the oracle."""
+    from chapsim2_amd import operations as ops
+    from chapsim2_amd.domain import Domain
+
+    dm = Domain(NC_LONG, LX, -1.0, 1.0, LZ, istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1)
+    odm = po.Dom(list(NC_LONG), [1, 0, 1], LX, -1.0, 1.0, LZ, 2, 0.075)
+    rng = np.random.default_rng(17)
+    nchecked = 0
+    try:
+        for d in (0, 1, 2):
+            for op in range(6):
+                for acc in (1, 4):
+                    bcs = [((1, 1), False)] if d != 1 else [((4, 4), True), ((5, 5), True), ((2, 3), False)]
+                    for bc, fb in bcs:
+                        per = bc[0] == 1
+                        ext_in = list(NC_LONG)
+                        ext_out = list(NC_LONG)
+                        ext_in[d] = _n(NC_LONG[d], per, INP[op])
+                        ext_out[d] = _n(NC_LONG[d], per, OUT[op])
+                        fi = np.asfortranarray(rng.standard_normal(ext_in))
+                        fshape = list(ext_in)
+                        fshape[d] = 4
+                        fbc = np.asfortranarray(rng.standard_normal(fshape)) if fb else None
+                        try:
+                            ref = odm.op3d(d, op, acc, list(bc), fi, ext_out, fbc)
+                        except ValueError:
+                            continue
+                        if not np.all(np.isfinite(ref)) or np.abs(ref).max() > 1e15:
+                            continue
+                        out = np.zeros(ext_out, order="F")
+                        ops.op3d(dm, d, op, fi, out, acc, list(bc), fbc)
+                        err = np.abs(out - ref).max() / max(1.0, np.abs(ref).max())
+                        assert err < RTOL, (d, op, acc, bc, fb, err)
+                        nchecked += 1
+    finally:
+        dm.close()
+    assert nchecked > 40
