@@ -17,7 +17,7 @@ EXPORTS = [
     "chs_abi_version", "chs_last_error", "chs_create", "chs_destroy", "chs_sync", "chs_op3d",
     "chs_poisson", "chs_set_state", "chs_get_state", "chs_update_re", "chs_substep", "chs_run",
     "chs_momentum_rhs", "chs_mass_check", "chs_divergence",
-    "chs_inspect_coeff", "chs_inspect_op1d", "chs_profile_begin", "chs_profile_end",
+    "chs_inspect_coeff", "chs_inspect_op1d", "chs_inspect_projection", "chs_profile_begin", "chs_profile_end",
 ]
 
 
@@ -83,6 +83,7 @@ def load():
     L.chs_inspect_coeff.argtypes = [C.c_int] * 6
     L.chs_inspect_op1d.argtypes = [C.POINTER(ChsDesc), C.c_int, C.c_int, C.c_int, ip, dp, C.c_int, dp,
                                    C.c_int, dp]
+    L.chs_inspect_projection.argtypes = [C.POINTER(ChsDesc), dp, dp, dp, dp, dp]
     L.chs_profile_begin.argtypes = [vp, C.c_int]
     L.chs_profile_end.argtypes = [vp, dp, C.POINTER(C.c_longlong), dp]
     for name in EXPORTS:

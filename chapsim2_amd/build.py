@@ -20,6 +20,7 @@ ARCH = os.environ.get("CHS_OFFLOAD_ARCH", "gfx950")
 SOURCES = [
     "chs_plan.cpp",
     "chs_geom.cpp",
+    "chs_exact.cpp",
     "chs_lineop.hip",
     "chs_poisson.hip",
     "chs_flow.hip",
@@ -28,7 +29,9 @@ SOURCES = [
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
-# Packed FP32 instructions are disabled for all device code.
+# Packed FP32 instructions (v_pk_*_f32) are disabled for all device code: the
+# compiler forms them from plain float code and some of their forms have given
+# silently wrong results on MI355X.
 DEVICE_FLAGS = [
     f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -5,6 +5,7 @@
 
 #include "chs_flow.h"
 #include "chs_handle.h"
+#include "chs_exact.h"
 #include "chs_poisson.h"
 
 namespace chs {
@@ -105,6 +106,10 @@ int chs_create(const chs_desc_t* desc, chs_handle_t** out) {
     return fail(CHS_EINVAL, "invalid istret");
   if (desc->precision != CHS_PREC_FP64 && desc->precision != CHS_PREC_MIXED)
     return fail(CHS_EINVAL, "invalid precision");
+  if (desc->projection != CHS_PROJ_REFERENCE && desc->projection != CHS_PROJ_EXACT)
+    return fail(CHS_EINVAL, "invalid projection");
+  if (desc->projection == CHS_PROJ_EXACT && desc->periodic_y)
+    return fail(CHS_EUNSUP, "the exact projection is built for walls in y");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CHS_EHIP, "no HIP device");
   chs_handle_t* H = new chs_handle_t();
@@ -132,6 +137,15 @@ int chs_create(const chs_desc_t* desc, chs_handle_t** out) {
     if (!e.empty()) {
       delete H;
       return fail(CHS_ENOMEM, e);
+    }
+    if (desc->projection == CHS_PROJ_EXACT) {
+      chs::ExactOps o;
+      e = chs::build_exact_ops(h.mesh, desc->iacc, h.flow->ibc, h.flow->wall_y, &o);
+      if (e.empty()) e = h.poisson->setup_exact(o);
+      if (!e.empty()) {
+        delete H;
+        return fail(CHS_EUNSUP, e);
+      }
     }
   }
   *out = H;
@@ -323,6 +337,23 @@ int chs_inspect_op1d(const chs_desc_t* desc, int dir, int op, int iacc, const in
   std::string e = chs::make_line_plan(m.line_geom(dir), chs::Op(op), iacc, ibc, fbc != nullptr, n_in, n_out, &p);
   if (!e.empty()) return fail(CHS_EINVAL, e);
   chs::apply_plan_host(p, fi, fbc, fo);
+  return CHS_OK;
+}
+
+int chs_inspect_projection(const chs_desc_t* desc, double* ly, double* lam, double* lx, double* lz, double* resid) {
+  if (!desc) return fail(CHS_EINVAL, "invalid argument");
+  chs::Mesh m;
+  m.build(desc->nc, desc->periodic_y != 0, desc->lxx, desc->lyb, desc->lyt, desc->lzz, desc->istret, desc->rstret);
+  int ibc[3][5][2];
+  chs::flow_bc_kinds(!m.periodic[1], ibc);
+  chs::ExactOps o;
+  std::string e = chs::build_exact_ops(m, desc->iacc, ibc, !m.periodic[1], &o);
+  if (!e.empty()) return fail(CHS_EINVAL, e);
+  if (ly) std::copy(o.Ly.begin(), o.Ly.end(), ly);
+  if (lam) std::copy(o.lam.begin(), o.lam.end(), lam);
+  if (lx) std::copy(o.lx.begin(), o.lx.end(), lx);
+  if (lz) std::copy(o.lz.begin(), o.lz.end(), lz);
+  if (resid) *resid = o.eig_resid;
   return CHS_OK;
 }
 

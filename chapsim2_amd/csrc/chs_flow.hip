@@ -11,6 +11,7 @@
 // its own array, and the three are summed in the reference's order by the
 // fractional-step kernel (Calculate_momentum_fractional_step, :35-71).
 #include "chs_flow.h"
+#include "chs_exact.h"
 
 #include <cmath>
 #include <cstring>
@@ -250,15 +251,7 @@ std::string Flow::create(Handle* H, std::unique_ptr<Flow>* out) {
   f.tg[0] = 1.0; f.tg[1] = 8.0 / 15.0; f.tg[2] = 5.0 / 12.0; f.tg[3] = 3.0 * 0.25;
   f.tz[0] = 0.0; f.tz[1] = 0.0; f.tz[2] = -17.0 / 60.0; f.tz[3] = -5.0 / 12.0;
   for (int i = 0; i < 4; ++i) f.ta[i] = f.tg[i] + f.tz[i];
-  for (int dd = 0; dd < 3; ++dd)
-    for (int v = 0; v < 5; ++v)
-      for (int s = 0; s < 2; ++s) f.ibc[dd][v][s] = BC_PERIODIC;
-  if (f.wall_y)
-    for (int s = 0; s < 2; ++s) {
-      for (int v = 0; v < 3; ++v) f.ibc[1][v][s] = BC_DIRICHLET;
-      f.ibc[1][VT][s] = BC_DIRICHLET;
-      f.ibc[1][VP][s] = BC_NEUMANN;
-    }
+  flow_bc_kinds(f.wall_y, f.ibc);
   int ftp[3][2];
   for (int dd = 0; dd < 3; ++dd)
     for (int s = 0; s < 2; ++s) ftp[dd][s] = (f.ibc[dd][VT][s] == BC_NEUMANN) ? BC_DIRICHLET : f.ibc[dd][VT][s];
@@ -529,7 +522,8 @@ std::string Flow::substep(int isub) {
   // Poisson RHS: div(q) / (alpha * sigma2 * dt), then the solve
   SCHK(divergence(pcor.p));
   const double coeff = 1.0 / (ta[isub] * sigma2p * dt);
-  SCHK(H->poisson->solve(pcor.p, s, coeff));
+  if (H->desc.projection == CHS_PROJ_EXACT) SCHK(H->poisson->solve_exact(pcor.p, s, coeff));
+  else SCHK(H->poisson->solve(pcor.p, s, coeff));
   hipLaunchKernelGGL(k_add, grid1(pr.size()), dim3(256), 0, s, pr.p, pcor.p, pr.size());
   // Correct_massflux
   const double sc = dt * ta[isub] * sigma2p;

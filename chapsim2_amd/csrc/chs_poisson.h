@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "chs_exact.h"
 #include "chs_geom.h"
 
 namespace chs {
@@ -22,6 +23,13 @@ class Poisson {
   // `in_scale` multiplies the right-hand side before the forward transform.
   std::string solve(double* rhs, hipStream_t s, double in_scale = 1.0);
   std::string solve(float* rhs, hipStream_t s, float in_scale = 1.0f);
+  // Exact discrete projection operator (see chs_exact.h): upload the Fourier
+  // eigenvalues and the eigen-decomposition of the wall-normal block; then
+  // solve_exact inverts D G with FFT(x,z) and two dense products along y.
+  std::string setup_exact(const ExactOps& o);
+  std::string solve_exact(double* rhs, hipStream_t s, double in_scale = 1.0);
+  std::string solve_exact(float* rhs, hipStream_t s, float in_scale = 1.0f);
+  bool has_exact() const { return V64 != nullptr; }
   int nx = 0, ny = 0, nz = 0, nxh = 0;
 
  private:
@@ -37,6 +45,12 @@ class Poisson {
   float* fd32 = nullptr; float* fz32 = nullptr;
   void* work = nullptr;                                // complex work (nxh, ny, nz)
   size_t work_bytes = 0;
+  template <typename T> std::string solve_exact_t(T* rhs, hipStream_t s, T in_scale);
+  // exact operator tables
+  double *V64 = nullptr, *Vi64 = nullptr, *lam64 = nullptr, *lx64 = nullptr, *lz64 = nullptr;
+  float *V32 = nullptr, *Vi32 = nullptr, *lam32 = nullptr, *lx32 = nullptr, *lz32 = nullptr;
+  int null_idx = -1;
+  void* work2 = nullptr;                               // second complex work array
 };
 
 // Radix plan of n over {4, 2, 3, 5}; empty if n has another prime factor.

@@ -288,10 +288,87 @@ __global__ void __launch_bounds__(64) k_ytdma(cx<T>* __restrict__ W, int nxh, in
   }
 }
 
+// Wall-normal mixing of the spectrum by a dense ny x ny matrix:
+//   out(i, g) = s(i, g) * sum_j M(i, j) in(j, g)
+// for every real column g = (kz, c) of the complex spectrum (c = 2 kx + re/im),
+// with s = 1/(lam_i + lx(kx) + lz(kz)) when lam is given (0 for the constant
+// mode of (kx, kz) = (0, 0)) and s = 1 otherwise.  A workgroup computes a
+// 128-row x 32-column tile; the j loop runs over 16-wide slabs of M and of the
+// input staged in LDS, and each thread accumulates a 4 x 4 block.
+constexpr int kMixR = 128, kMixC = 32, kMixK = 16;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_ymix(const T* __restrict__ M, const T* __restrict__ in, T* __restrict__ out,
+                                              int ny, int ncr, long ncol, const T* __restrict__ lam,
+                                              const T* __restrict__ lx, const T* __restrict__ lz, int null_idx) {
+  __shared__ T As[kMixK][kMixR];
+  __shared__ T Bs[kMixK][kMixC];
+  const int tid = threadIdx.x;
+  const int tx = tid & 7, ty = tid >> 3;
+  const int r0 = blockIdx.y * kMixR;
+  const long g0 = (long)blockIdx.x * kMixC;
+  T acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+  for (int k0 = 0; k0 < ny; k0 += kMixK) {
+    for (int e = tid; e < kMixR * kMixK; e += 256) {
+      const int r = e / kMixK, k = e - r * kMixK;
+      const int gi = r0 + r, gk = k0 + k;
+      As[k][r] = (gi < ny && gk < ny) ? M[(long)gi * ny + gk] : T(0);
+    }
+    for (int e = tid; e < kMixK * kMixC; e += 256) {
+      const int k = e / kMixC, c = e - k * kMixC;
+      const long g = g0 + c;
+      const int gk = k0 + k;
+      T v = T(0);
+      if (g < ncol && gk < ny) {
+        const long kz = g / ncr, cc = g - kz * ncr;
+        v = in[cc + (long)ncr * (gk + (long)ny * kz)];
+      }
+      Bs[k][c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kMixK; ++k) {
+      T a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[q] = As[k][ty * 4 + q];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b[q] = Bs[k][tx * 4 + q];
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q] = acc[p][q] + a[p] * b[q];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int i = r0 + ty * 4 + p;
+    if (i >= ny) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long g = g0 + tx * 4 + q;
+      if (g >= ncol) continue;
+      const long kz = g / ncr, cc = g - kz * ncr;
+      const long kx = cc >> 1;
+      T v = acc[p][q];
+      if (lam) {
+        if (i == null_idx && kx == 0 && kz == 0) v = T(0);
+        else v = v / (lam[i] + lx[kx] + lz[kz]);
+      }
+      out[cc + (long)ncr * (i + (long)ny * kz)] = v;
+    }
+  }
+}
+
 Poisson::~Poisson() {
   for (void* p : {(void*)tw_x64, (void*)tw_x32, (void*)tw_z64, (void*)tw_z32, (void*)rot64, (void*)rot32,
                   (void*)ya64, (void*)yc64, (void*)ya32, (void*)yc32, (void*)fd64, (void*)fz64, (void*)fd32,
-                  (void*)fz32, work})
+                  (void*)fz32, work, (void*)V64, (void*)Vi64, (void*)lam64, (void*)lx64, (void*)lz64,
+                  (void*)V32, (void*)Vi32, (void*)lam32, (void*)lx32, (void*)lz32, work2})
     if (p) (void)hipFree(p);
 }
 
@@ -438,6 +515,71 @@ std::string Poisson::solve_t(T* rhs, hipStream_t s, T in_scale) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("poisson launch: ") + hipGetErrorString(e);
   return "";
+}
+
+std::string Poisson::setup_exact(const ExactOps& o) {
+  if (o.nx != nx || o.ny != ny || o.nz != nz || o.nxh != nxh) return "poisson: exact operators of another grid";
+  if ((int)o.lx.size() != nxh || (int)o.lz.size() != nz || (int)o.lam.size() != ny) return "poisson: bad exact tables";
+  std::string e;
+  if (!(e = upv(o.V, &V64)).empty() || !(e = upv(o.V, &V32)).empty()) return e;
+  if (!(e = upv(o.Vi, &Vi64)).empty() || !(e = upv(o.Vi, &Vi32)).empty()) return e;
+  if (!(e = upv(o.lam, &lam64)).empty() || !(e = upv(o.lam, &lam32)).empty()) return e;
+  if (!(e = upv(o.lx, &lx64)).empty() || !(e = upv(o.lx, &lx32)).empty()) return e;
+  if (!(e = upv(o.lz, &lz64)).empty() || !(e = upv(o.lz, &lz32)).empty()) return e;
+  null_idx = o.null_idx;
+  PCHK(hipMalloc(&work2, work_bytes));
+  return "";
+}
+
+template <typename T>
+std::string Poisson::solve_exact_t(T* rhs, hipStream_t s, T in_scale) {
+  if (!V64) return "poisson: exact operators not set up";
+  Radices rx{}, rz{};
+  rx.n = (int)rad_x.size();
+  for (int q = 0; q < rx.n; ++q) rx.r[q] = rad_x[q];
+  rz.n = (int)rad_z.size();
+  for (int q = 0; q < rz.n; ++q) rz.r[q] = rad_z[q];
+  const bool dbl = sizeof(T) == 8;
+  const cx<T>* twx = reinterpret_cast<const cx<T>*>(dbl ? (void*)tw_x64 : (void*)tw_x32);
+  const cx<T>* twz = reinterpret_cast<const cx<T>*>(dbl ? (void*)tw_z64 : (void*)tw_z32);
+  const cx<T>* rot = reinterpret_cast<const cx<T>*>(dbl ? (void*)rot64 : (void*)rot32);
+  const T* V = reinterpret_cast<const T*>(dbl ? (void*)V64 : (void*)V32);
+  const T* Vi = reinterpret_cast<const T*>(dbl ? (void*)Vi64 : (void*)Vi32);
+  const T* lam = reinterpret_cast<const T*>(dbl ? (void*)lam64 : (void*)lam32);
+  const T* lxv = reinterpret_cast<const T*>(dbl ? (void*)lx64 : (void*)lx32);
+  const T* lzv = reinterpret_cast<const T*>(dbl ? (void*)lz64 : (void*)lz32);
+  cx<T>* W = reinterpret_cast<cx<T>*>(work);
+  T* Wr = reinterpret_cast<T*>(work);
+  T* W2r = reinterpret_cast<T*>(work2);
+  const long nlx = (long)ny * nz;
+  const int N = nx / 2;
+  const int lpbx = kTileC / N;
+  const size_t shx = sizeof(cx<T>) * (size_t)lpbx * N;
+  dim3 gx((unsigned)((nlx + lpbx - 1) / lpbx));
+  hipLaunchKernelGGL(k_fft_x_fwd<T>, gx, dim3(kThreads), shx, s, rhs, W, nx, nlx, rx, twx, rot, in_scale);
+  const int lpbz = kTileC / nz;
+  const size_t shz = sizeof(cx<T>) * (size_t)lpbz * nz;
+  dim3 gz((unsigned)(((nxh + lpbz - 1) / lpbz) * ny));
+  hipLaunchKernelGGL(k_fft_z<T>, gz, dim3(kThreads), shz, s, W, nxh, ny, nz, rz, twz, T(-1));
+  const int ncr = 2 * nxh;
+  const long ncol = (long)ncr * nz;
+  dim3 gm((unsigned)((ncol + kMixC - 1) / kMixC), (unsigned)((ny + kMixR - 1) / kMixR));
+  hipLaunchKernelGGL(k_ymix<T>, gm, dim3(256), 0, s, Vi, Wr, W2r, ny, ncr, ncol, lam, lxv, lzv, null_idx);
+  hipLaunchKernelGGL(k_ymix<T>, gm, dim3(256), 0, s, V, W2r, Wr, ny, ncr, ncol, (const T*)nullptr,
+                     (const T*)nullptr, (const T*)nullptr, -1);
+  hipLaunchKernelGGL(k_fft_z<T>, gz, dim3(kThreads), shz, s, W, nxh, ny, nz, rz, twz, T(1));
+  const T scale = T(1.0 / (double(nx) * double(nz)));
+  hipLaunchKernelGGL(k_fft_x_inv<T>, gx, dim3(kThreads), shx, s, W, rhs, nx, nlx, rx, twx, rot, scale);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("poisson launch: ") + hipGetErrorString(e);
+  return "";
+}
+
+std::string Poisson::solve_exact(double* rhs, hipStream_t s, double in_scale) {
+  return solve_exact_t<double>(rhs, s, in_scale);
+}
+std::string Poisson::solve_exact(float* rhs, hipStream_t s, float in_scale) {
+  return solve_exact_t<float>(rhs, s, in_scale);
 }
 
 std::string Poisson::solve(double* rhs, hipStream_t s, double in_scale) { return solve_t<double>(rhs, s, in_scale); }

@@ -14,6 +14,7 @@ IBC_INTERIOR, IBC_PERIODIC, IBC_SYMMETRIC, IBC_ASYMMETRIC, IBC_DIRICHLET, IBC_NE
 IACCU_CD2, IACCU_CD4, IACCU_CP4, IACCU_CP6 = 1, 2, 3, 4
 FFT_FISHPACK_2DFFT, FFT_2DECOMP_3DFFT = 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
+PROJ_REFERENCE, PROJ_EXACT = 0, 1
 
 
 def make_desc(nc, lxx, lyb, lyt, lzz, periodic_y=False, istret=0, rstret=0.0, iacc=IACCU_CP6,

@@ -54,6 +54,14 @@ enum { CHS_MIDP_C2P = 0, CHS_MIDP_P2C = 1, CHS_1DER_C2C = 2, CHS_1DER_P2P = 3,
 enum { CHS_PREC_FP64 = 0, CHS_PREC_MIXED = 1 };
 /* Poisson path, src/modules.f90:169-173 */
 enum { CHS_POISSON_FISHPACK = 2, CHS_POISSON_3DFFT = 3 };
+/* pressure correction of a sub-step:
+ *  REFERENCE - one solve with the 2nd-order operator of the FISHPACK path
+ *              (src/poisson_fishpack.f90), as the reference does;
+ *  EXACT     - inverts the composed operator D*G of the configured compact
+ *              schemes (FFT in x and z, eigen-basis of the wall-normal block),
+ *              so one correction leaves max|div u| at round-off.  No
+ *              counterpart in the reference. */
+enum { CHS_PROJ_REFERENCE = 0, CHS_PROJ_EXACT = 1 };
 
 /* Domain/solver description (bind(C) on the Fortran side).  Replaces the
  * t_domain/t_flow set-up of src/geometry.f90:410-593, src/bc_general.f90
@@ -72,7 +80,7 @@ typedef struct chs_desc {
   int poisson;          /* CHS_POISSON_* */
   double uwall[2][3];   /* y-wall velocities (bottom, top) x (u, v, w) */
   int precision;        /* CHS_PREC_* */
-  int projection;       /* 0 single projection (reference); 1 refined to div_tol */
+  int projection;       /* CHS_PROJ_* */
   double div_tol;       /* target max|div u| for projection = 1 */
   int max_refine;       /* cap on refinement sweeps for projection = 1 */
 } chs_desc_t;

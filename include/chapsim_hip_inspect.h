@@ -27,6 +27,14 @@ double chs_inspect_coeff(int fam, int is_rhs, int row, int k, int bc, int acc);
 int chs_inspect_op1d(const chs_desc_t *desc, int dir, int op, int iacc, const int ibc[2],
                      const double *fi, int n_in, double *fo, int n_out, const double *fbc);
 
+/* Operators of the exact projection for the channel described by desc (host
+ * only): ly = D_y G_y (ny*ny, row-major) with the wall faces held, its
+ * eigenvalues lam (ny, ascending), the Fourier eigenvalues lx (nx/2+1) and lz
+ * (nz) of the periodic D G blocks, and the decomposition residual
+ * max|V^-1 Ly V - diag(lam)| / max|Ly|.  Any output pointer may be NULL. */
+int chs_inspect_projection(const chs_desc_t *desc, double *ly, double *lam, double *lx, double *lz,
+                           double *resid);
+
 /* Launch timing of one kernel class during subsequent solver calls, with HIP
  * events on the handle's stream: class 0/1/2 = x/y/z line operators of the
  * momentum right-hand side and projection.  chs_profile_end returns the summed

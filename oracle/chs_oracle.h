@@ -92,6 +92,7 @@ typedef struct orc_flow_cfg {
   int init_re_to;
   int idriven;         /* 0 none, 1 x mass flux */
   int poisson;         /* 2 = FISHPACK 2-D FFT + y TDMA; 3 = 3-D FFT (triply periodic) */
+  int projection;      /* 0 reference correction; 1 exact D*G inverse (y walls only) */
 } orc_flow_cfg;
 
 orc_flow *orc_flow_create(const orc_flow_cfg *cfg);

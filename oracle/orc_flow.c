@@ -234,6 +234,9 @@ struct orc_flow {
   /* Poisson */
   double *xrt, *zrt, *pa, *pb, *pc;
   double *kx2, *ky2, *kz2; /* modified wavenumbers for the 3-D path */
+  /* exact projection: wall-normal block D_y G_y (row-major), eigenvalues of
+   * the periodic blocks per FFTPACK coefficient, left null vector of ly */
+  double *ly, *lxe, *lze, *lnull;
 };
 
 /* build_bc_symm_operation, src/bc_general.f90:455-510; returns JBC_PROD */
@@ -285,6 +288,97 @@ static double *fbc_const_array(int dir, int n1, int n2, int n3,
     a[t] = v[m % 2];
   }
   return a;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Exact projection (no counterpart in the reference).  The composed        */
+/* operator L = D_x G_x + D_y G_y + D_z G_z of the divergence (D1_P2C of     */
+/* the velocity, src/eq_continuity.f90:91-161) and of the correction        */
+/* gradient (D1_C2P of the pressure increment, src/eq_momentum2.f90         */
+/* :1577-1660, wall faces then reset to the wall velocity) is built by       */
+/* applying the oracle's own line operators to unit vectors: a circulant     */
+/* first column in x and z (diagonal in the FFTPACK real Fourier basis),     */
+/* a dense ny x ny block in y.  Each real Fourier coefficient line is solved */
+/* by LU with partial pivoting of ly + (lx + lz) I; the (0,0) line replaces  */
+/* its last equation by phi(ny-1) = 0 and is then shifted so that the left  */
+/* null vector of ly is orthogonal to it (the gauge).                       */
+/* ---------------------------------------------------------------------- */
+static int dense_solve(int n, double *M, double *x) {
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(M[(size_t)i * n + k]) > fabs(M[(size_t)p * n + k])) p = i;
+    if (M[(size_t)p * n + k] == 0.0) return -1;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) {
+        double t = M[(size_t)p * n + j];
+        M[(size_t)p * n + j] = M[(size_t)k * n + j];
+        M[(size_t)k * n + j] = t;
+      }
+      double t = x[p]; x[p] = x[k]; x[k] = t;
+    }
+    for (int i = k + 1; i < n; ++i) {
+      double fct = M[(size_t)i * n + k] / M[(size_t)k * n + k];
+      if (fct == 0.0) continue;
+      for (int j = k + 1; j < n; ++j) M[(size_t)i * n + j] -= fct * M[(size_t)k * n + j];
+      x[i] -= fct * x[k];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double sum = x[i];
+    for (int j = i + 1; j < n; ++j) sum -= M[(size_t)i * n + j] * x[j];
+    x[i] = sum / M[(size_t)i * n + i];
+  }
+  return 0;
+}
+
+static void exact_setup(orc_flow *f) {
+  const orc_dom *dm = &f->dm;
+  int nx = f->nx, ny = f->ny, nz = f->nz, acc = f->cfg.iacc;
+  double pi = 2.0 * asin(1.0);
+  f->ly = (double *)calloc((size_t)ny * ny, sizeof(double));
+  f->lnull = (double *)calloc((size_t)ny, sizeof(double));
+  double *e = (double *)calloc((size_t)(ny + 1), sizeof(double));
+  double *g = (double *)calloc((size_t)(ny + 1), sizeof(double));
+  double *col = (double *)calloc((size_t)(ny + 1), sizeof(double));
+  double fb0[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < ny; ++k) {
+    for (int j = 0; j < ny; ++j) e[j] = (j == k) ? 1.0 : 0.0;
+    orc_op1d(dm, 1, ORC_1DER_C2P, acc, f->ibc[1][VP], e, ny, g, ny + 1, fb0);
+    g[0] = 0.0;
+    g[ny] = 0.0;
+    orc_op1d(dm, 1, ORC_1DER_P2C, acc, f->ibc[1][VV], g, ny + 1, col, ny, NULL);
+    for (int i = 0; i < ny; ++i) f->ly[(size_t)i * ny + k] = col[i];
+  }
+  /* left null vector: ly^T l = 0 with l(ny-1) = 1 */
+  double *M = (double *)malloc(sizeof(double) * (size_t)ny * ny);
+  for (int i = 0; i < ny; ++i)
+    for (int j = 0; j < ny; ++j) M[(size_t)i * ny + j] = f->ly[(size_t)j * ny + i];
+  for (int j = 0; j < ny; ++j) {
+    M[(size_t)(ny - 1) * ny + j] = (j == ny - 1) ? 1.0 : 0.0;
+    f->lnull[j] = (j == ny - 1) ? 1.0 : 0.0;
+  }
+  dense_solve(ny, M, f->lnull);
+  free(M);
+  for (int pass = 0; pass < 2; ++pass) {
+    int dir = pass ? 2 : 0, n = pass ? nz : nx;
+    double *e0 = (double *)calloc((size_t)n, sizeof(double));
+    double *t = (double *)calloc((size_t)n, sizeof(double));
+    double *c = (double *)calloc((size_t)n, sizeof(double));
+    double *lv = (double *)calloc((size_t)n, sizeof(double));
+    e0[0] = 1.0;
+    orc_op1d(dm, dir, ORC_1DER_C2P, acc, f->ibc[dir][VP], e0, n, t, n, NULL);
+    orc_op1d(dm, dir, ORC_1DER_P2C, acc, f->ibc[dir][dir], t, n, c, n, NULL);
+    for (int i = 0; i < n; ++i) {
+      int m = (i + 1) / 2; /* FFTPACK coefficient i holds wavenumber (i+1)/2 */
+      double sum = 0.0;
+      for (int j = 0; j < n; ++j) sum += c[j] * cos(2.0 * pi * (double)(((long)j * m) % n) / (double)n);
+      lv[i] = sum;
+    }
+    if (pass) f->lze = lv; else f->lxe = lv;
+    free(e0); free(t); free(c);
+  }
+  free(e); free(g); free(col);
 }
 
 orc_flow *orc_flow_create(const orc_flow_cfg *cfg) {
@@ -405,6 +499,7 @@ orc_flow *orc_flow_create(const orc_flow_cfg *cfg) {
       f->pc[ny - 1] = 0.0;
     }
   }
+  if (cfg->projection == 1) exact_setup(f);
   orc_flow_update_re(f, 0);
   return f;
 }
@@ -416,6 +511,7 @@ void orc_flow_destroy(orc_flow *f) {
   for (int v = 0; v < 4; ++v) { free(f->fbx[v]); free(f->fby[v]); free(f->fbz[v]); }
   free(f->xrt); free(f->zrt); free(f->pa); free(f->pb); free(f->pc);
   free(f->kx2); free(f->ky2); free(f->kz2);
+  free(f->ly); free(f->lxe); free(f->lze); free(f->lnull);
   orc_dom_free(&f->dm);
   free(f);
 }
@@ -602,8 +698,63 @@ static void poisson_fishpack(orc_flow *f, double *rhs) {
   for (long t = 0; t < n; ++t) rhs[t] = rhs[t] / sx / sz;
 }
 
+static void poisson_exact(orc_flow *f, double *rhs) {
+  int nx = f->nx, ny = f->ny, nz = f->nz;
+#pragma omp parallel
+  {
+    int nmax = nx > nz ? nx : nz;
+    nmax = nmax > ny ? nmax : ny;
+    double *t = (double *)malloc(sizeof(double) * (size_t)nmax);
+    double *M = (double *)malloc(sizeof(double) * (size_t)ny * ny);
+#pragma omp for schedule(static)
+    for (long L = 0; L < (long)ny * nz; ++L) orc_rfftf(nx, rhs + L * nx);
+#pragma omp for schedule(static)
+    for (long L = 0; L < (long)nx * ny; ++L) {
+      long i = L % nx, j = L / nx;
+      for (int k = 0; k < nz; ++k) t[k] = rhs[i + (long)nx * (j + (long)ny * k)];
+      orc_rfftf(nz, t);
+      for (int k = 0; k < nz; ++k) rhs[i + (long)nx * (j + (long)ny * k)] = t[k];
+    }
+#pragma omp for schedule(dynamic, 4)
+    for (long L = 0; L < (long)nx * nz; ++L) {
+      long i = L % nx, k = L / nx;
+      double lam = f->lxe[i] + f->lze[k];
+      for (int a = 0; a < ny; ++a)
+        for (int b = 0; b < ny; ++b) M[(size_t)a * ny + b] = f->ly[(size_t)a * ny + b] + (a == b ? lam : 0.0);
+      for (int j = 0; j < ny; ++j) t[j] = rhs[i + (long)nx * (j + (long)ny * k)];
+      int gauge = (i == 0 && k == 0);
+      if (gauge) {
+        for (int b = 0; b < ny; ++b) M[(size_t)(ny - 1) * ny + b] = (b == ny - 1) ? 1.0 : 0.0;
+        t[ny - 1] = 0.0;
+      }
+      dense_solve(ny, M, t);
+      if (gauge) {
+        double lp = 0.0, l1 = 0.0;
+        for (int j = 0; j < ny; ++j) { lp += f->lnull[j] * t[j]; l1 += f->lnull[j]; }
+        for (int j = 0; j < ny; ++j) t[j] -= lp / l1;
+      }
+      for (int j = 0; j < ny; ++j) rhs[i + (long)nx * (j + (long)ny * k)] = t[j];
+    }
+#pragma omp for schedule(static)
+    for (long L = 0; L < (long)nx * ny; ++L) {
+      long i = L % nx, j = L / nx;
+      for (int k = 0; k < nz; ++k) t[k] = rhs[i + (long)nx * (j + (long)ny * k)];
+      orc_rfftb(nz, t);
+      for (int k = 0; k < nz; ++k) rhs[i + (long)nx * (j + (long)ny * k)] = t[k];
+    }
+#pragma omp for schedule(static)
+    for (long L = 0; L < (long)ny * nz; ++L) orc_rfftb(nx, rhs + L * nx);
+    free(t);
+    free(M);
+  }
+  double sx = (double)nx, sz = (double)nz;
+  long n = (long)nx * ny * nz;
+  for (long q = 0; q < n; ++q) rhs[q] = rhs[q] / sx / sz;
+}
+
 void orc_poisson_solve(orc_flow *f, double *rhs) {
-  if (f->cfg.poisson == 2) poisson_fishpack(f, rhs);
+  if (f->cfg.projection == 1) poisson_exact(f, rhs);
+  else if (f->cfg.poisson == 2) poisson_fishpack(f, rhs);
 }
 
 /* ---------------------------------------------------------------------- */

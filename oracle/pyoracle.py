@@ -38,6 +38,7 @@ class OrcFlowCfg(C.Structure):
         ("wall_y", C.c_int), ("uwall", (C.c_double * 3) * 2),
         ("iacc", C.c_int), ("dt", C.c_double), ("ren", C.c_double), ("reninit", C.c_double),
         ("init_re_to", C.c_int), ("idriven", C.c_int), ("poisson", C.c_int),
+        ("projection", C.c_int),
     ]
 
 
@@ -220,7 +221,7 @@ class Flow:
 
     def __init__(self, nc, lxx, lyb, lyt, lzz, istret=0, rstret=0.0, wall_y=True,
                  iacc=CP6, dt=1e-3, ren=1.0, reninit=None, init_re_to=0, idriven=0,
-                 poisson=2, uwall=((0.0, 0.0, 0.0), (0.0, 0.0, 0.0))):
+                 poisson=2, uwall=((0.0, 0.0, 0.0), (0.0, 0.0, 0.0)), projection=0):
         cfg = OrcFlowCfg()
         for d in range(3):
             cfg.nc[d] = nc[d]
@@ -233,6 +234,7 @@ class Flow:
         cfg.iacc, cfg.dt, cfg.ren = iacc, dt, ren
         cfg.reninit = ren if reninit is None else reninit
         cfg.init_re_to, cfg.idriven, cfg.poisson = init_re_to, idriven, poisson
+        cfg.projection = int(projection)
         self.cfg = cfg
         self.h = lib().orc_flow_create(C.byref(cfg))
         self.dom = Dom.from_ptr(lib().orc_flow_dom(self.h), self)

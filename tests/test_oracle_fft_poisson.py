@@ -1,0 +1,142 @@
+"""Pins the oracle's FFTPACK-convention transforms
+(lib/fishpack4.1/fftpack.f:129-240), its FISHPACK Poisson solve
+(src/poisson_fishpack.f90:20-57, 213-490) and its exact projection."""
+import numpy as np
+import pytest
+
+
+def _fftpack_forward(r):
+    """RFFTF definition: r[0] = sum, then (Re, Im) pairs of sum r e^{-i...}
+    for wavenumbers 1..n/2 (the last one real for even n)."""
+    n = len(r)
+    F = np.fft.rfft(r)
+    out = np.empty(n)
+    out[0] = F[0].real
+    for k in range(1, (n - 1) // 2 + 1):
+        out[2 * k - 1] = F[k].real
+        out[2 * k] = F[k].imag
+    if n % 2 == 0:
+        out[n - 1] = F[n // 2].real
+    return out
+
+
+@pytest.mark.parametrize("n", [8, 12, 30, 64, 96])
+def test_rfft_against_numpy(po, n):
+    rng = np.random.default_rng(n)
+    r = rng.standard_normal(n)
+    f = po.rfftf(r)
+    assert np.abs(f - _fftpack_forward(r)).max() < 1e-12 * n
+    assert np.abs(po.rfftb(f) - n * r).max() < 1e-12 * n
+
+
+def _fishpack_operator(f):
+    """Dense 2nd-order operator of fishpack_fft_init (x, z periodic; y walls
+    with Neumann closure), built from the mesh alone."""
+    dm = f.dom
+    nx, ny, nz = dm.nc
+    hx, hz = dm.h[0], dm.h[2]
+    def per(n, h):
+        D = -2.0 * np.eye(n) + np.eye(n, k=1) + np.eye(n, k=-1)
+        D[0, -1] = D[-1, 0] = 1.0
+        return D / h**2
+    ymc, ymp = dm.ymc, dm.ymp
+    hy = dm.h[1]
+    Y = np.zeros((ny, ny))
+    for j in range(ny):
+        a = ymc[j] * ymp[j] / hy**2
+        c = ymc[j] * ymp[j + 1] / hy**2
+        if j > 0:
+            Y[j, j - 1] += a
+            Y[j, j] -= a
+        if j < ny - 1:
+            Y[j, j + 1] += c
+            Y[j, j] -= c
+    Ix, Iy, Iz = np.eye(nx), np.eye(ny), np.eye(nz)
+    # Fortran order (x fastest): index = i + nx*(j + ny*k)
+    return (np.kron(Iz, np.kron(Iy, per(nx, hx))) + np.kron(Iz, np.kron(Y, Ix))
+            + np.kron(per(nz, hz), np.kron(Iy, Ix)))
+
+
+def test_fishpack_solves_2nd_order_operator(po):
+    nc = [8, 10, 6]
+    f = po.Flow(nc, 2.0, -1.0, 1.0, 1.5, istret=2, rstret=0.075, iacc=1, poisson=2)
+    A = _fishpack_operator(f)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(A.shape[0])
+    rhs = A @ x  # compatible right-hand side
+    phi = f.poisson(np.asfortranarray(rhs.reshape(nc, order="F"))).reshape(-1, order="F")
+    assert np.abs(A @ phi - rhs).max() < 1e-10 * np.abs(rhs).max()
+    # gauge of TRID0's singular pivot: the solution differs from x by a constant
+    d = phi - x
+    assert np.ptp(d) < 1e-10 * np.abs(x).max()
+
+
+@pytest.mark.parametrize("iacc", [1, 2, 3, 4])
+def test_exact_projection_zero_divergence(po, iacc):
+    """One exact projection leaves the discrete divergence at round-off, and
+    a second one changes nothing."""
+    from chapsim2_amd.initial import channel_fields
+
+    nc = (24, 20, 12)
+    lx, lz = 2 * np.pi, np.pi
+    cfg = dict(istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1, iacc=iacc)
+    fields = channel_fields(nc, lx, -1.0, 1.0, lz, 2, 0.075, noise=0.1, seed=3)
+    f = po.Flow(list(nc), lx, -1.0, 1.0, lz, wall_y=True, poisson=2, projection=1, **cfg)
+    f.set_state(*fields)
+    f.update_re(0)
+    d0 = np.abs(f.divergence()).max()
+    f.project(1)
+    d1 = np.abs(f.divergence()).max()
+    q1 = f.get_state()
+    f.project(1)
+    q2 = f.get_state()
+    assert d0 > 1e-2 and d1 < 1e-13 * max(1.0, d0)
+    for a, b in zip(q1[:3], q2[:3]):
+        assert np.abs(a - b).max() < 1e-13
+
+
+def test_exact_projection_equals_fishpack_for_cd2(po):
+    """With 2nd-order central schemes the composed D*G is FISHPACK's
+    operator, so the exact projection and the reference projection give the
+    same velocities (an independent pin of both solves)."""
+    from chapsim2_amd.initial import channel_fields
+
+    nc = (24, 20, 12)
+    lx, lz = 2 * np.pi, np.pi
+    cfg = dict(istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1, iacc=1)
+    fields = channel_fields(nc, lx, -1.0, 1.0, lz, 2, 0.075, noise=0.1, seed=4)
+    out = []
+    for proj in (0, 1):
+        f = po.Flow(list(nc), lx, -1.0, 1.0, lz, wall_y=True, poisson=2, projection=proj, **cfg)
+        f.set_state(*fields)
+        f.update_re(0)
+        f.project(1)
+        out.append(f.get_state())
+    for a, b in zip(out[0][:3], out[1][:3]):
+        assert np.abs(a - b).max() < 1e-12 * max(1.0, np.abs(a).max())
+    # pressures agree up to the gauge constant
+    dp = out[0][3] - out[1][3]
+    assert np.ptp(dp) < 1e-9 * max(1.0, np.abs(out[0][3]).max())
+
+
+def test_reference_projection_growth_with_cp6(po):
+    """The reference scheme (2nd-order Poisson, CP6 D and G) does not remove
+    the divergence: repeated projections contract it only slowly, and RK3
+    steps amplify it.  The exact projection does not have this problem."""
+    from chapsim2_amd.initial import channel_fields
+
+    nc = (32, 24, 16)
+    lx, lz = 4 * np.pi, 4 * np.pi / 3
+    cfg = dict(istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1, iacc=4)
+    fields = channel_fields(nc, lx, -1.0, 1.0, lz, 2, 0.075, noise=0.1, seed=7)
+    divs = {}
+    for proj in (0, 1):
+        f = po.Flow(list(nc), lx, -1.0, 1.0, lz, wall_y=True, poisson=2, projection=proj, **cfg)
+        f.set_state(*fields)
+        f.update_re(0)
+        f.substep(0)
+        for it in range(1, 4):
+            f.step(it)
+        divs[proj] = np.abs(f.divergence()).max()
+    assert divs[0] > 1.0
+    assert divs[1] < 1e-13
