@@ -1,0 +1,215 @@
+!> Fortran binding of libchapsim_hip (include/chapsim_hip.h) for the
+!> reference solver.  The C ABI entry points are declared with bind(C);
+!> the 18 Get_*_3D wrappers keep the reference's argument order
+!> (src/basics_operations2.f90:270-312) with the domain argument replaced
+!> by the library handle, and map a non-zero status to an error stop as
+!> Print_error_msg does (src/tools_general.f90:10-20).
+module chapsim_hip
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: CHS_OK = 0
+  integer(c_int), parameter, public :: CHS_MEM_HOST = 0, CHS_MEM_DEVICE = 1
+  integer(c_int), parameter, public :: CHS_PROJ_REFERENCE = 0, CHS_PROJ_EXACT = 1
+  integer(c_int), parameter, public :: CHS_PREC_FP64 = 0, CHS_PREC_MIXED = 1
+  integer(c_int), parameter, public :: CHS_POISSON_FISHPACK = 2
+  ! operator kinds of chs_op3d
+  integer(c_int), parameter, public :: CHS_MIDP_C2P = 0, CHS_MIDP_P2C = 1, CHS_1DER_C2C = 2, &
+                                       CHS_1DER_P2P = 3, CHS_1DER_C2P = 4, CHS_1DER_P2C = 5
+
+  !> chs_desc_t; uwall(component, side)
+  type, bind(C), public :: chs_desc_t
+    integer(c_int) :: nc(3)
+    integer(c_int) :: periodic_y
+    real(c_double) :: lxx, lyb, lyt, lzz
+    integer(c_int) :: istret
+    real(c_double) :: rstret
+    integer(c_int) :: iacc
+    real(c_double) :: dt
+    real(c_double) :: ren, reninit
+    integer(c_int) :: init_re_to
+    integer(c_int) :: idriven
+    integer(c_int) :: poisson
+    real(c_double) :: uwall(3, 2)
+    integer(c_int) :: precision
+    integer(c_int) :: projection
+    real(c_double) :: div_tol
+    integer(c_int) :: max_refine
+  end type chs_desc_t
+
+  public :: chs_abi_version, chs_create, chs_destroy, chs_sync, chs_op3d, chs_poisson
+  public :: chs_set_state, chs_get_state, chs_update_re, chs_substep, chs_run, chs_mass_check
+  public :: chs_divergence, chs_inspect_coeff, chs_inspect_op1d, chs_error_message, chs_check
+  public :: Get_x_midp_C2P_3D, Get_x_midp_P2C_3D, Get_x_1der_C2C_3D, Get_x_1der_P2P_3D, &
+            Get_x_1der_C2P_3D, Get_x_1der_P2C_3D, Get_y_midp_C2P_3D, Get_y_midp_P2C_3D, &
+            Get_y_1der_C2C_3D, Get_y_1der_P2P_3D, Get_y_1der_C2P_3D, Get_y_1der_P2C_3D, &
+            Get_z_midp_C2P_3D, Get_z_midp_P2C_3D, Get_z_1der_C2C_3D, Get_z_1der_P2P_3D, &
+            Get_z_1der_C2P_3D, Get_z_1der_P2C_3D
+
+  interface
+    integer(c_int) function chs_abi_version() bind(C, name="chs_abi_version")
+      import :: c_int
+    end function
+    type(c_ptr) function chs_last_error() bind(C, name="chs_last_error")
+      import :: c_ptr
+    end function
+    integer(c_int) function chs_create(desc, h) bind(C, name="chs_create")
+      import :: c_int, c_ptr, chs_desc_t
+      type(chs_desc_t), intent(in) :: desc
+      type(c_ptr), intent(out) :: h
+    end function
+    integer(c_int) function chs_destroy(h) bind(C, name="chs_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function chs_sync(h) bind(C, name="chs_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function chs_op3d(h, dir, op, iacc, ibc, fi, shp_in, fo, shp_out, fbc, mem) &
+        bind(C, name="chs_op3d")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: dir, op, iacc, mem
+      integer(c_int), intent(in) :: ibc(2), shp_in(3), shp_out(3)
+      real(c_double), intent(in) :: fi(*)
+      real(c_double), intent(inout) :: fo(*)
+      type(c_ptr), value :: fbc
+    end function
+    integer(c_int) function chs_poisson(h, rhs, mem) bind(C, name="chs_poisson")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(inout) :: rhs(*)
+      integer(c_int), value :: mem
+    end function
+    integer(c_int) function chs_set_state(h, qx, qy, qz, pr, mem) bind(C, name="chs_set_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: qx(*), qy(*), qz(*), pr(*)
+      integer(c_int), value :: mem
+    end function
+    integer(c_int) function chs_get_state(h, qx, qy, qz, pr, mem) bind(C, name="chs_get_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(inout) :: qx(*), qy(*), qz(*), pr(*)
+      integer(c_int), value :: mem
+    end function
+    integer(c_int) function chs_update_re(h, iter) bind(C, name="chs_update_re")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: iter
+    end function
+    integer(c_int) function chs_substep(h, isub) bind(C, name="chs_substep")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: isub
+    end function
+    integer(c_int) function chs_run(h, iter0, nsteps) bind(C, name="chs_run")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: iter0, nsteps
+    end function
+    integer(c_int) function chs_mass_check(h, out) bind(C, name="chs_mass_check")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out(3)
+    end function
+    integer(c_int) function chs_divergence(h, div, mem) bind(C, name="chs_divergence")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(inout) :: div(*)
+      integer(c_int), value :: mem
+    end function
+    real(c_double) function chs_inspect_coeff(fam, is_rhs, row, k, bc, acc) bind(C, name="chs_inspect_coeff")
+      import :: c_int, c_double
+      integer(c_int), value :: fam, is_rhs, row, k, bc, acc
+    end function
+    integer(c_int) function chs_inspect_op1d(desc, dir, op, iacc, ibc, fi, n_in, fo, n_out, fbc) &
+        bind(C, name="chs_inspect_op1d")
+      import :: c_int, c_double, c_ptr, chs_desc_t
+      type(chs_desc_t), intent(in) :: desc
+      integer(c_int), value :: dir, op, iacc, n_in, n_out
+      integer(c_int), intent(in) :: ibc(2)
+      real(c_double), intent(in) :: fi(*)
+      real(c_double), intent(inout) :: fo(*)
+      type(c_ptr), value :: fbc
+    end function
+  end interface
+
+contains
+
+  !> Last error message of the calling thread.
+  function chs_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    integer :: n
+    p = chs_last_error()
+    msg = ''
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, s, [4096])
+    n = 0
+    do while (n < 4096)
+      if (s(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    allocate (character(len=n) :: msg)
+    msg = transfer(s(1:n), msg)
+  end function
+
+  !> Stop with the library's message when status is not CHS_OK.
+  subroutine chs_check(status, where)
+    integer(c_int), intent(in) :: status
+    character(len=*), intent(in) :: where
+    if (status /= CHS_OK) then
+      write (*, '(A)') 'chapsim_hip: '//where//': '//chs_error_message()
+      error stop 1
+    end if
+  end subroutine
+
+  subroutine op3d(h, dir, op, fi3d, fo3d, iacc, ibc, fbc2d)
+    type(c_ptr), intent(in) :: h
+    integer(c_int), intent(in) :: dir, op, iacc, ibc(2)
+    real(c_double), intent(in), target, contiguous :: fi3d(:, :, :)
+    real(c_double), intent(inout), target, contiguous :: fo3d(:, :, :)
+    real(c_double), intent(in), target, contiguous, optional :: fbc2d(:, :, :)
+    integer(c_int) :: si(3), so(3)
+    type(c_ptr) :: pf
+    si = int(shape(fi3d), c_int)
+    so = int(shape(fo3d), c_int)
+    pf = c_null_ptr
+    if (present(fbc2d)) pf = c_loc(fbc2d)
+    call chs_check(chs_op3d(h, dir, op, iacc, ibc, fi3d, si, fo3d, so, pf, CHS_MEM_HOST), 'chs_op3d')
+  end subroutine
+
+#define CHS_WRAP(NAME, DIR, OP) \
+  subroutine NAME(fi3d, fo3d, h, iacc, ibc, fbc2d); \
+    real(c_double), intent(in), contiguous :: fi3d(:, :, :); \
+    real(c_double), intent(inout), contiguous :: fo3d(:, :, :); \
+    type(c_ptr), intent(in) :: h; \
+    integer, intent(in) :: iacc, ibc(2); \
+    real(c_double), intent(in), contiguous, optional :: fbc2d(:, :, :); \
+    call op3d(h, DIR, OP, fi3d, fo3d, int(iacc, c_int), int(ibc, c_int), fbc2d); \
+  end subroutine
+
+  CHS_WRAP(Get_x_midp_C2P_3D, 0_c_int, CHS_MIDP_C2P)
+  CHS_WRAP(Get_x_midp_P2C_3D, 0_c_int, CHS_MIDP_P2C)
+  CHS_WRAP(Get_x_1der_C2C_3D, 0_c_int, CHS_1DER_C2C)
+  CHS_WRAP(Get_x_1der_P2P_3D, 0_c_int, CHS_1DER_P2P)
+  CHS_WRAP(Get_x_1der_C2P_3D, 0_c_int, CHS_1DER_C2P)
+  CHS_WRAP(Get_x_1der_P2C_3D, 0_c_int, CHS_1DER_P2C)
+  CHS_WRAP(Get_y_midp_C2P_3D, 1_c_int, CHS_MIDP_C2P)
+  CHS_WRAP(Get_y_midp_P2C_3D, 1_c_int, CHS_MIDP_P2C)
+  CHS_WRAP(Get_y_1der_C2C_3D, 1_c_int, CHS_1DER_C2C)
+  CHS_WRAP(Get_y_1der_P2P_3D, 1_c_int, CHS_1DER_P2P)
+  CHS_WRAP(Get_y_1der_C2P_3D, 1_c_int, CHS_1DER_C2P)
+  CHS_WRAP(Get_y_1der_P2C_3D, 1_c_int, CHS_1DER_P2C)
+  CHS_WRAP(Get_z_midp_C2P_3D, 2_c_int, CHS_MIDP_C2P)
+  CHS_WRAP(Get_z_midp_P2C_3D, 2_c_int, CHS_MIDP_P2C)
+  CHS_WRAP(Get_z_1der_C2C_3D, 2_c_int, CHS_1DER_C2C)
+  CHS_WRAP(Get_z_1der_P2P_3D, 2_c_int, CHS_1DER_P2P)
+  CHS_WRAP(Get_z_1der_C2P_3D, 2_c_int, CHS_1DER_C2P)
+  CHS_WRAP(Get_z_1der_P2C_3D, 2_c_int, CHS_1DER_P2C)
+
+end module chapsim_hip
