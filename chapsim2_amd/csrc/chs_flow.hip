@@ -81,11 +81,22 @@ __global__ void k_combine(double* __restrict__ ax, const double* __restrict__ ay
   }
 }
 
-// sum the block partials in a fixed order; bulk = sum / vol
-__global__ void k_finish_sum(const double* __restrict__ partial, int np, double vol, double* __restrict__ bulk) {
-  __shared__ double red[256];
+// sum the block partials in a fixed order; bulk = sum / vol.  One block of
+// 1024 threads; each thread first sums its strided share with 8 loads in
+// flight.
+__global__ void __launch_bounds__(1024) k_finish_sum(const double* __restrict__ partial, int np, double vol,
+                                                     double* __restrict__ bulk) {
+  __shared__ double red[1024];
   double s = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) s = s + partial[i];
+  int i = threadIdx.x;
+  for (; i + 7 * 1024 < np; i += 8 * 1024) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = partial[i + u * 1024];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s = s + v[u];
+  }
+  for (; i < np; i += 1024) s = s + partial[i];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int k = blockDim.x / 2; k > 0; k >>= 1) {
@@ -494,7 +505,7 @@ std::string Flow::momentum_rhs(int isub) {
   }
   if (forced) {
     int nb = (int)((acc[0][0].size() + 255) / 256);
-    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, s, partial, nb, vol, bulk);
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, s, partial, nb, vol, bulk);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("momentum_rhs: ") + hipGetErrorString(e);

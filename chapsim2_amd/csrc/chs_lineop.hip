@@ -312,17 +312,17 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_yz(KArgs<T> P) {
   __shared__ T ex[2 * kMaxChunks * 64];
   const int lane = threadIdx.x & 63;
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const long Lr = (long)blockIdx.x * 64 + lane;
+  const int Lr = blockIdx.x * 64 + lane;  // nlines < 2^31 (checked at launch)
   const bool live = Lr < P.nlines;
-  const long L = live ? Lr : P.nlines - 1;
+  const int L = live ? Lr : (int)P.nlines - 1;
   long ib, is, ob, os, fb, fs;
   if (DIR == 1) {
-    const long i = L % P.e1, k = L / P.e1;
+    const int i = L % P.e1, k = L / P.e1;
     ib = i + (long)P.e1 * P.e2 * k; is = P.e1;
     ob = i + (long)P.o1 * P.o2 * k; os = P.o1;
     fb = i + (long)P.e1 * 4 * k; fs = P.e1;
   } else {
-    const long i = L % P.e1, j = L / P.e1;
+    const int i = L % P.e1, j = L / P.e1;
     ib = i + (long)P.e1 * j; is = (long)P.e1 * P.e2;
     ob = i + (long)P.o1 * j; os = (long)P.o1 * P.o2;
     fb = i + (long)P.e1 * j; fs = (long)P.e1 * P.e2;
@@ -382,13 +382,13 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
   const long L0 = (long)blockIdx.x * NL;
   const int n_in = P.n_in, n_out = P.n_out;
   // stage NL lines (input transform applied)
+  // (an x line L starts at L * e1 in the input and at L * o1 in the output)
   for (int e = tid; e < NL * n_in; e += nthr) {
     const int l = e / n_in, k = e - l * n_in;
     const long L = L0 + l;
     T v = T(0);
     if (L < P.nlines) {
-      const long j = L % P.e2, kk = L / P.e2;
-      const long o = (j + (long)P.e2 * kk) * P.e1 + k;
+      const long o = L * P.e1 + k;
       v = P.A[o];
       if (P.B) v = v + P.cB * P.B[o];
       v = P.sA * v;
@@ -408,10 +408,8 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
     return row[lds_pos(k)];
   };
   T fb4[4] = {T(0), T(0), T(0), T(0)};
-  if (P.fbc && live && (q == 0 || q == Pn - 1)) {
-    const long j = L % P.e2, kk = L / P.e2;
-    for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * (j + (long)P.e2 * kk) + m];
-  }
+  if (P.fbc && live && (q == 0 || q == Pn - 1))
+    for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
   T y[kChunk];
   chunk_rhs<T, OPK>(P, q, s0, len, Fw, Fa, fb4, y);
   if (P.solve) {
@@ -433,8 +431,7 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
     const int l = e / n_out, k = e - l * n_out;
     const long Lw = L0 + l;
     if (Lw >= P.nlines) continue;
-    const long j = Lw % P.e2, kk = Lw / P.e2;
-    const long o = (j + (long)P.o2 * kk) * P.o1 + k;
+    const long o = Lw * P.o1 + k;
     const T val = tile[(long)l * LS + lds_pos(k)];
     const T prev = P.accumulate ? P.out[o] : T(0);
     P.out[o] = prev + val * P.sOut;
@@ -501,6 +498,7 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
   a.out = out.out; a.sOut = out.sOut; a.accumulate = out.accumulate;
   a.scratch = scratch;
   if (a.nlines <= 0) return "";
+  if (a.nlines >= (1L << 31) - 64) return "line op: too many lines";
   if (p.part_P <= kMaxChunks && p.n_out >= 4) {
     a.P = p.part_P;
     a.per = (p.solve == 2);
