@@ -42,7 +42,25 @@ def parse():
                     help="line-operator direction whose launches are timed (0 x, 1 y, 2 z)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nc", type=int, nargs=3, default=list(NC))
+    ap.add_argument("--ops", action="store_true",
+                    help="time each of the 18 compact operators (CP6, channel boundary kinds) instead")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (calibrates HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """A plain copy of known size: 5 torch copies of `mib` MiB, the
+    reference point for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    n = mib * 1024 * 1024 // 8
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8, "copies": 5}))
 
 
 def cpu_baseline(nc, fields, cfg):
@@ -73,12 +91,69 @@ def cpu_baseline(nc, fields, cfg):
                       f"reference algorithm (FP64, single 2nd-order projection), {dt:.1f} s"}
 
 
+def ops_bench(args):
+    """Each compact operator on device-resident arrays of the channel grid:
+    average kernel time from HIP events and the algorithmic bandwidth
+    (one array read, one written)."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from chapsim2_amd import _lib
+    from chapsim2_amd.domain import Domain
+
+    nc = tuple(args.nc)
+    dm = Domain(nc, 4 * np.pi, -1.0, 1.0, 4 * np.pi / 3, istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1)
+    L = dm.lib
+    names = ["midp_C2P", "midp_P2C", "1der_C2C", "1der_P2P", "1der_C2P", "1der_P2C"]
+    inp = {0: "c", 1: "p", 2: "c", 3: "p", 4: "c", 5: "p"}
+    out = {0: "p", 1: "c", 2: "c", 3: "p", 4: "p", 5: "c"}
+    res = []
+    for d in range(3):
+        for op in range(6):
+            ei, eo = list(nc), list(nc)
+            if d == 1:
+                ei[1] = nc[1] + (inp[op] == "p")
+                eo[1] = nc[1] + (out[op] == "p")
+                ibc = [5, 5] if inp[op] == "c" else [4, 4]
+            else:
+                ibc = [1, 1]
+            fi = torch.rand(ei[2], ei[1], ei[0], dtype=torch.float64, device="cuda")
+            fo = torch.zeros(eo[2], eo[1], eo[0], dtype=torch.float64, device="cuda")
+            fshape = list(ei)
+            fshape[d] = 4
+            fbc = torch.zeros(fshape[2], fshape[1], fshape[0], dtype=torch.float64, device="cuda") if d == 1 else None
+            torch.cuda.synchronize()
+            args_ = (dm.h, d, op, 4, _lib.ivec(ibc), C.c_void_p(fi.data_ptr()), _lib.ivec(ei),
+                     C.c_void_p(fo.data_ptr()), _lib.ivec(eo), C.c_void_p(fbc.data_ptr()) if fbc is not None else None,
+                     _lib.MEM_DEVICE)
+            for _ in range(3):
+                _lib.check(L.chs_op3d(*args_))
+            _lib.check(L.chs_profile_begin(dm.h, d))
+            for _ in range(args.steps):
+                _lib.check(L.chs_op3d(*args_))
+            ms, nl, byt = C.c_double(), C.c_longlong(), C.c_double()
+            _lib.check(L.chs_profile_end(dm.h, C.byref(ms), C.byref(nl), C.byref(byt)))
+            t = ms.value / nl.value
+            res.append({"op": f"Get_{'xyz'[d]}_{names[op]}_3D", "us": round(t * 1e3, 2),
+                        "GBps": round(byt.value / nl.value / (t * 1e-3) / 1e9, 1)})
+    dm.close()
+    print(json.dumps({"ops_bench": res, "grid": list(nc), "launches_each": args.steps}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs the whole domain on one GPU; "
                                    "multi-GPU runs are not part of it (--gpus 1)"}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+    if args.ops:
+        ops_bench(args)
+        return
     import numpy as np
     import torch
 

@@ -205,7 +205,14 @@ int chs_op3d(chs_handle_t* H, int dir, int op, int iacc, const int ibc[2], const
   in.A = din;
   chs::LineOut<double> o;
   o.out = dout;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const bool timed = (h.prof_class == dir) && h.prof_pair(&e0, &e1);
+  if (timed) {
+    (void)hipEventRecord(e0, h.stream);
+    h.prof_bytes += 8.0 * ((double)nin + (double)nout);
+  }
   e = chs::launch_line_op<double>(*P, ein, eout, in, dfbc, o, scratch, h.stream);
+  if (timed) (void)hipEventRecord(e1, h.stream);
   if (!e.empty()) return fail(CHS_EINVAL, e);
   if (mem == CHS_MEM_HOST &&
       hipMemcpyAsync(fo, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, h.stream) != hipSuccess)

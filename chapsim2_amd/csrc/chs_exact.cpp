@@ -341,6 +341,35 @@ std::string build_exact_ops(const Mesh& m, int acc, const int ibc[3][5][2], bool
     apply_plan_host(D, gf.data(), nullptr, col.data());
     for (int i = 0; i < ny; ++i) o.Ly[(size_t)i * ny + k] = col[i];
   }
+  // mirror symmetry
+  double lmx = 0.0, asym = 0.0;
+  for (int i = 0; i < ny; ++i)
+    for (int k = 0; k < ny; ++k) {
+      lmx = std::max(lmx, std::fabs(o.Ly[(size_t)i * ny + k]));
+      asym = std::max(asym, std::fabs(o.Ly[(size_t)i * ny + k] - o.Ly[(size_t)(ny - 1 - i) * ny + (ny - 1 - k)]));
+    }
+  if (ny % 2 == 0 && asym <= 1e-12 * lmx) {
+    const int h = ny / 2;
+    std::vector<double> Le((size_t)h * h), Lo((size_t)h * h);
+    for (int j = 0; j < h; ++j)
+      for (int k = 0; k < h; ++k) {
+        const double a = o.Ly[(size_t)j * ny + k], b = o.Ly[(size_t)j * ny + (ny - 1 - k)];
+        Le[(size_t)j * h + k] = a + b;
+        Lo[(size_t)j * h + k] = a - b;
+      }
+    double re = 0.0, ro = 0.0;
+    if (!(e = eig_real(h, Le, &o.lame, &o.Ve, &o.Vie, &re)).empty()) return e;
+    if (!(e = eig_real(h, Lo, &o.lamo, &o.Vo, &o.Vio, &ro)).empty()) return e;
+    o.null_even = 0;
+    for (int i = 1; i < h; ++i)
+      if (std::fabs(o.lame[i]) < std::fabs(o.lame[o.null_even])) o.null_even = i;
+    double lm = 0.0;
+    for (double l : o.lame) lm = std::max(lm, std::fabs(l));
+    for (double l : o.lamo) lm = std::max(lm, std::fabs(l));
+    double omin = 1e300;
+    for (double l : o.lamo) omin = std::min(omin, std::fabs(l));
+    if (std::fabs(o.lame[o.null_even]) <= 1e-9 * lm && omin > 1e-9 * lm) o.sym = true;
+  }
   if (!(e = eig_real(ny, o.Ly, &o.lam, &o.V, &o.Vi, &o.eig_resid)).empty()) return e;
   o.null_idx = 0;
   for (int i = 1; i < ny; ++i)

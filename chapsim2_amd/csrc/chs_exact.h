@@ -37,6 +37,14 @@ struct ExactOps {
   std::vector<double> Vi;   // inverse of V
   int null_idx = -1;        // eigenvalue index of the constant mode
   double eig_resid = 0.0;   // max |Vi Ly V - diag(lam)| / max |Ly|
+  // Mirror-symmetric channel (Ly commutes with the reflection j -> ny-1-j,
+  // ny even): Ly splits into an even and an odd block of size ny/2 acting on
+  // r_e(j) = r(j) + r(ny-1-j) and r_o(j) = r(j) - r(ny-1-j); each block is
+  // diagonalised on its own (half the work per solve; the near-degenerate
+  // even/odd pairs of wall modes fall into different blocks).
+  bool sym = false;
+  std::vector<double> Ve, Vie, lame, Vo, Vio, lamo;  // (ny/2)^2, (ny/2)
+  int null_even = -1;       // constant mode in the even block
 };
 
 // A = V diag(lam) V^-1 for a real matrix with real simple eigenvalues

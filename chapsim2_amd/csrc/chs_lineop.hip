@@ -438,6 +438,99 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
   }
 }
 
+// x lines with a wave-uniform chunk: a workgroup stages 64 whole lines in LDS
+// (row stride odd, so the 64 lanes reading one column of their own lines hit
+// distinct banks); wave q of the workgroup processes chunk q of all 64 lines,
+// which keeps the plan's per-row coefficients in scalar registers.
+template <typename T, int OPK>
+__global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* tile = reinterpret_cast<T*>(smem_raw);
+  const int LS = P.ls, Pn = P.P;
+  T* exb = tile + (long)64 * LS;
+  const int tid = threadIdx.x, nthr = 64 * Pn;
+  const int lane = tid & 63;
+  const int q = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const long L0 = (long)blockIdx.x * 64;
+  const int n_in = P.n_in, n_out = P.n_out;
+  for (int e = tid; e < 64 * n_in; e += nthr) {
+    const int l = e / n_in, k = e - l * n_in;
+    const long L = L0 + l;
+    T v = T(0);
+    if (L < P.nlines) {
+      const long o = L * P.e1 + k;
+      v = P.A[o];
+      if (P.B) v = v + P.cB * P.B[o];
+      v = P.sA * v;
+      if (P.Cm) v = v * P.Cm[o];
+    }
+    tile[l * LS + k] = v;
+  }
+  __syncthreads();
+  const long L = L0 + lane;
+  const bool live = L < P.nlines;
+  T* row = tile + lane * LS;
+  const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
+  auto Fa = [&](int k) -> T { return row[k]; };
+  auto Fw = [&](int t) -> T {
+    int k = s0 - 2 + t;
+    k = k < 0 ? 0 : (k >= n_in ? n_in - 1 : k);
+    return row[k];
+  };
+  T fb4[4] = {T(0), T(0), T(0), T(0)};
+  if (P.fbc && live && (q == 0 || q == Pn - 1))
+    for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
+  T y[kChunk];
+  chunk_rhs<T, OPK>(P, q, s0, len, Fw, Fa, fb4, y);
+  if (P.solve) {
+    T yf, yl;
+    chunk_solve<T>(P, s0, len, y, yf, yl);
+    exb[lane + 64 * (2 * q)] = yf;
+    exb[lane + 64 * (2 * q + 1)] = yl;
+    __syncthreads();
+    chunk_couple<T>(P, q, s0, len, [&](int k) { return exb[lane + 64 * k]; }, y);
+  }
+  __syncthreads();  // every wave has finished reading its input rows
+#pragma unroll
+  for (int i = 0; i < kChunk; ++i)
+    if (i < len) row[s0 + i] = chunk_post<T>(P, s0 + i, y[i], fb4);
+  __syncthreads();
+  for (int e = tid; e < 64 * n_out; e += nthr) {
+    const int l = e / n_out, k = e - l * n_out;
+    const long Lw = L0 + l;
+    if (Lw >= P.nlines) continue;
+    const long o = Lw * P.o1 + k;
+    const T val = tile[l * LS + k];
+    const T prev = P.accumulate ? P.out[o] : T(0);
+    P.out[o] = prev + val * P.sOut;
+  }
+}
+
+template <typename T>
+static void dispatch_xu(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
+  switch (op) {
+    case MIDP_C2P: hipLaunchKernelGGL((k_line_xu<T, MIDP_C2P>), g, b, shm, s, a); break;
+    case MIDP_P2C: hipLaunchKernelGGL((k_line_xu<T, MIDP_P2C>), g, b, shm, s, a); break;
+    case D1_C2C: hipLaunchKernelGGL((k_line_xu<T, D1_C2C>), g, b, shm, s, a); break;
+    case D1_P2P: hipLaunchKernelGGL((k_line_xu<T, D1_P2P>), g, b, shm, s, a); break;
+    case D1_C2P: hipLaunchKernelGGL((k_line_xu<T, D1_C2P>), g, b, shm, s, a); break;
+    case D1_P2C: hipLaunchKernelGGL((k_line_xu<T, D1_P2C>), g, b, shm, s, a); break;
+  }
+}
+
+// LDS bytes a workgroup may use, queried once per device.
+static size_t max_lds_per_block() {
+  static size_t cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+  if (!cached[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) v = 65536;
+    cached[dev] = (size_t)v;
+  }
+  return cached[dev];
+}
+
 template <typename T, int DIR>
 static void dispatch_part(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
   if (DIR == 0) {
@@ -503,8 +596,14 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
     a.P = p.part_P;
     a.per = (p.solve == 2);
     a.ld = D.ld; a.lc = D.lc; a.ar = D.ar; a.sv = D.sv; a.sw = D.sw; a.rinv = D.rinv;
-    if (dir == 0) {
-      const int nmax = p.n_in > p.n_out ? p.n_in : p.n_out;
+    const int nmax = p.n_in > p.n_out ? p.n_in : p.n_out;
+    const int lsu = nmax | 1;  // odd row stride
+    const size_t shm_u = sizeof(T) * ((size_t)64 * lsu + (size_t)64 * 2 * p.part_P);
+    if (dir == 0 && shm_u <= max_lds_per_block()) {
+      a.ls = lsu;
+      dim3 g((unsigned)((a.nlines + 63) / 64)), b((unsigned)(64 * p.part_P));
+      dispatch_xu<T>(p.op, g, b, shm_u, s, a);
+    } else if (dir == 0) {
       int ls = lds_pos_host(nmax) + 1;
       ls += ((8 - ls) % 32 + 32) % 32;  // row stride = 8 (mod 32) doubles: neighbouring lines in other banks
       a.ls = ls;

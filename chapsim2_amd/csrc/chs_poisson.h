@@ -50,6 +50,10 @@ class Poisson {
   double *V64 = nullptr, *Vi64 = nullptr, *lam64 = nullptr, *lx64 = nullptr, *lz64 = nullptr;
   float *V32 = nullptr, *Vi32 = nullptr, *lam32 = nullptr, *lx32 = nullptr, *lz32 = nullptr;
   int null_idx = -1;
+  // even / odd blocks of a mirror-symmetric channel (null when not symmetric)
+  double *Ve64 = nullptr, *Vo64 = nullptr, *Vie64 = nullptr, *Vio64 = nullptr, *lame64 = nullptr, *lamo64 = nullptr;
+  float *Ve32 = nullptr, *Vo32 = nullptr, *Vie32 = nullptr, *Vio32 = nullptr, *lame32 = nullptr, *lamo32 = nullptr;
+  int null_even = -1;
   void* work2 = nullptr;                               // second complex work array
 };
 

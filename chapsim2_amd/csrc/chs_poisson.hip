@@ -301,7 +301,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_ymix(const T* __restrict__ M, const T* __restrict__ in, T* __restrict__ out,
                                               int ny, int ncr, long ncol, const T* __restrict__ lam,
                                               const T* __restrict__ lx, const T* __restrict__ lz, int null_idx) {
-  __shared__ T As[kMixK][kMixR];
+  __shared__ T As[kMixK][kMixR + 1];  // +1: the transposed stores of M hit distinct banks
   __shared__ T Bs[kMixK][kMixC];
   const int tid = threadIdx.x;
   const int tx = tid & 7, ty = tid >> 3;
@@ -364,11 +364,126 @@ __global__ void __launch_bounds__(256) k_ymix(const T* __restrict__ M, const T* 
   }
 }
 
+// Mirror-symmetric form of the wall-normal mixing (ny even, h = ny/2).
+// STAGE 1: fold each column into r_e(j) = r(j) + r(ny-1-j) and
+//          r_o(j) = r(j) - r(ny-1-j), multiply by the block inverses
+//          Vie / Vio and scale by 1/(lam + lx + lz); the result is stored as
+//          rows [0, h) even, [h, ny) odd.
+// STAGE 2: multiply the even / odd rows by Ve / Vo and unfold:
+//          phi(j) = (e(j) + o(j))/2, phi(ny-1-j) = (e(j) - o(j))/2.
+// A workgroup covers 64 rows of each half (row tile t) and 32 columns; each
+// thread accumulates a 4 x 4 block of one half.
+constexpr int kSymR = 64;
+
+template <typename T, int STAGE>
+__global__ void __launch_bounds__(256) k_ymix_sym(const T* __restrict__ Me, const T* __restrict__ Mo,
+                                                  const T* __restrict__ in, T* __restrict__ out, int ny, int ncr,
+                                                  long ncol, const T* __restrict__ lame, const T* __restrict__ lamo,
+                                                  const T* __restrict__ lx, const T* __restrict__ lz, int null_even) {
+  constexpr int AS = kMixK * (2 * kSymR + 1);
+  constexpr int BS = 2 * kMixK * kMixC;
+  constexpr int CS = 2 * kSymR * (kMixC + 1);
+  constexpr int SM = (AS + BS > CS) ? AS + BS : CS;
+  __shared__ T smem[SM];
+  T* As = smem;        // [k][2*kSymR + 1]
+  T* Bs = smem + AS;   // [half][k][kMixC]
+  const int h = ny / 2;
+  const int tid = threadIdx.x;
+  const int tx = tid & 7, ty = tid >> 3;
+  const int hf = ty >> 4;            // 0 even, 1 odd
+  const int rl = (ty & 15) * 4;      // first local row of this thread
+  const int j0 = blockIdx.y * kSymR;  // first row of the tile within a half
+  const long g0 = (long)blockIdx.x * kMixC;
+  T acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+  for (int k0 = 0; k0 < h; k0 += kMixK) {
+    for (int e = tid; e < 2 * kSymR * kMixK; e += 256) {
+      const int r = e / kMixK, k = e - r * kMixK;
+      const int half = r / kSymR, gi = j0 + (r - half * kSymR), gk = k0 + k;
+      const T* M = half ? Mo : Me;
+      As[k * (2 * kSymR + 1) + r] = (gi < h && gk < h) ? M[(long)gi * h + gk] : T(0);
+    }
+    for (int e = tid; e < 2 * kMixK * kMixC; e += 256) {
+      const int half = e / (kMixK * kMixC), rem = e - half * (kMixK * kMixC);
+      const int k = rem / kMixC, c = rem - k * kMixC;
+      const long g = g0 + c;
+      const int gk = k0 + k;
+      T v = T(0);
+      if (g < ncol && gk < h) {
+        const long kz = g / ncr, cc = g - kz * ncr;
+        const long base = cc + (long)ncr * (long)ny * kz;
+        if (STAGE == 1) {
+          const T a = in[base + (long)ncr * gk], b = in[base + (long)ncr * (ny - 1 - gk)];
+          v = half ? a - b : a + b;
+        } else {
+          v = in[base + (long)ncr * (half * h + gk)];
+        }
+      }
+      Bs[(half * kMixK + k) * kMixC + c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kMixK; ++k) {
+      T a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[q] = As[k * (2 * kSymR + 1) + hf * kSymR + rl + q];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b[q] = Bs[(hf * kMixK + k) * kMixC + tx * 4 + q];
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q] = acc[p][q] + a[p] * b[q];
+    }
+    __syncthreads();
+  }
+  if (STAGE == 1) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int i = j0 + rl + p;
+      if (i >= h) continue;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const long g = g0 + tx * 4 + q;
+        if (g >= ncol) continue;
+        const long kz = g / ncr, cc = g - kz * ncr;
+        const long kx = cc >> 1;
+        T v = acc[p][q];
+        if (hf == 0 && i == null_even && kx == 0 && kz == 0) v = T(0);
+        else v = v / ((hf ? lamo[i] : lame[i]) + lx[kx] + lz[kz]);
+        out[cc + (long)ncr * ((long)(hf * h + i) + (long)ny * kz)] = v;
+      }
+    }
+  } else {
+    T* Cs = smem;  // [half][row][kMixC + 1]; the K loop ended with a barrier
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Cs[(hf * kSymR + rl + p) * (kMixC + 1) + tx * 4 + q] = acc[p][q];
+    __syncthreads();
+    for (int e = tid; e < kSymR * kMixC; e += 256) {
+      const int r = e / kMixC, c = e - r * kMixC;
+      const int j = j0 + r;
+      const long g = g0 + c;
+      if (j >= h || g >= ncol) continue;
+      const long kz = g / ncr, cc = g - kz * ncr;
+      const T ev = Cs[r * (kMixC + 1) + c], od = Cs[(kSymR + r) * (kMixC + 1) + c];
+      const long base = cc + (long)ncr * (long)ny * kz;
+      out[base + (long)ncr * j] = T(0.5) * (ev + od);
+      out[base + (long)ncr * (ny - 1 - j)] = T(0.5) * (ev - od);
+    }
+  }
+}
+
 Poisson::~Poisson() {
   for (void* p : {(void*)tw_x64, (void*)tw_x32, (void*)tw_z64, (void*)tw_z32, (void*)rot64, (void*)rot32,
                   (void*)ya64, (void*)yc64, (void*)ya32, (void*)yc32, (void*)fd64, (void*)fz64, (void*)fd32,
                   (void*)fz32, work, (void*)V64, (void*)Vi64, (void*)lam64, (void*)lx64, (void*)lz64,
-                  (void*)V32, (void*)Vi32, (void*)lam32, (void*)lx32, (void*)lz32, work2})
+                  (void*)V32, (void*)Vi32, (void*)lam32, (void*)lx32, (void*)lz32, work2, (void*)Ve64,
+                  (void*)Vo64, (void*)Vie64, (void*)Vio64, (void*)lame64, (void*)lamo64, (void*)Ve32, (void*)Vo32,
+                  (void*)Vie32, (void*)Vio32, (void*)lame32, (void*)lamo32})
     if (p) (void)hipFree(p);
 }
 
@@ -527,6 +642,15 @@ std::string Poisson::setup_exact(const ExactOps& o) {
   if (!(e = upv(o.lx, &lx64)).empty() || !(e = upv(o.lx, &lx32)).empty()) return e;
   if (!(e = upv(o.lz, &lz64)).empty() || !(e = upv(o.lz, &lz32)).empty()) return e;
   null_idx = o.null_idx;
+  if (o.sym) {
+    if (!(e = upv(o.Ve, &Ve64)).empty() || !(e = upv(o.Ve, &Ve32)).empty()) return e;
+    if (!(e = upv(o.Vo, &Vo64)).empty() || !(e = upv(o.Vo, &Vo32)).empty()) return e;
+    if (!(e = upv(o.Vie, &Vie64)).empty() || !(e = upv(o.Vie, &Vie32)).empty()) return e;
+    if (!(e = upv(o.Vio, &Vio64)).empty() || !(e = upv(o.Vio, &Vio32)).empty()) return e;
+    if (!(e = upv(o.lame, &lame64)).empty() || !(e = upv(o.lame, &lame32)).empty()) return e;
+    if (!(e = upv(o.lamo, &lamo64)).empty() || !(e = upv(o.lamo, &lamo32)).empty()) return e;
+    null_even = o.null_even;
+  }
   PCHK(hipMalloc(&work2, work_bytes));
   return "";
 }
@@ -563,10 +687,20 @@ std::string Poisson::solve_exact_t(T* rhs, hipStream_t s, T in_scale) {
   hipLaunchKernelGGL(k_fft_z<T>, gz, dim3(kThreads), shz, s, W, nxh, ny, nz, rz, twz, T(-1));
   const int ncr = 2 * nxh;
   const long ncol = (long)ncr * nz;
-  dim3 gm((unsigned)((ncol + kMixC - 1) / kMixC), (unsigned)((ny + kMixR - 1) / kMixR));
-  hipLaunchKernelGGL(k_ymix<T>, gm, dim3(256), 0, s, Vi, Wr, W2r, ny, ncr, ncol, lam, lxv, lzv, null_idx);
-  hipLaunchKernelGGL(k_ymix<T>, gm, dim3(256), 0, s, V, W2r, Wr, ny, ncr, ncol, (const T*)nullptr,
-                     (const T*)nullptr, (const T*)nullptr, -1);
+  if (Ve64) {
+    auto pick = [&](double* d, float* f) { return reinterpret_cast<const T*>(dbl ? (void*)d : (void*)f); };
+    const int h = ny / 2;
+    dim3 gs((unsigned)((ncol + kMixC - 1) / kMixC), (unsigned)((h + kSymR - 1) / kSymR));
+    hipLaunchKernelGGL((k_ymix_sym<T, 1>), gs, dim3(256), 0, s, pick(Vie64, Vie32), pick(Vio64, Vio32), Wr, W2r,
+                       ny, ncr, ncol, pick(lame64, lame32), pick(lamo64, lamo32), lxv, lzv, null_even);
+    hipLaunchKernelGGL((k_ymix_sym<T, 2>), gs, dim3(256), 0, s, pick(Ve64, Ve32), pick(Vo64, Vo32), W2r, Wr, ny,
+                       ncr, ncol, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, -1);
+  } else {
+    dim3 gm((unsigned)((ncol + kMixC - 1) / kMixC), (unsigned)((ny + kMixR - 1) / kMixR));
+    hipLaunchKernelGGL(k_ymix<T>, gm, dim3(256), 0, s, Vi, Wr, W2r, ny, ncr, ncol, lam, lxv, lzv, null_idx);
+    hipLaunchKernelGGL(k_ymix<T>, gm, dim3(256), 0, s, V, W2r, Wr, ny, ncr, ncol, (const T*)nullptr,
+                       (const T*)nullptr, (const T*)nullptr, -1);
+  }
   hipLaunchKernelGGL(k_fft_z<T>, gz, dim3(kThreads), shz, s, W, nxh, ny, nz, rz, twz, T(1));
   const T scale = T(1.0 / (double(nx) * double(nz)));
   hipLaunchKernelGGL(k_fft_x_inv<T>, gx, dim3(kThreads), shx, s, W, rhs, nx, nlx, rx, twx, rot, scale);

@@ -35,9 +35,9 @@ int chs_inspect_op1d(const chs_desc_t *desc, int dir, int op, int iacc, const in
 int chs_inspect_projection(const chs_desc_t *desc, double *ly, double *lam, double *lx, double *lz,
                            double *resid);
 
-/* Launch timing of one kernel class during subsequent solver calls, with HIP
- * events on the handle's stream: class 0/1/2 = x/y/z line operators of the
- * momentum right-hand side and projection.  chs_profile_end returns the summed
+/* Launch timing of one kernel class during subsequent solver and chs_op3d
+ * calls, with HIP events on the handle's stream: class 0/1/2 = x/y/z line
+ * operators.  chs_profile_end returns the summed
  * event time, the number of timed launches and their algorithmic bytes
  * (arrays read plus arrays written, 8 bytes per element). */
 int chs_profile_begin(chs_handle_t *h, int kernel_class);

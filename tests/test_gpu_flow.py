@@ -66,11 +66,14 @@ def test_steps_vs_oracle(pair):
     np.testing.assert_allclose(dm.mass_check(), of.mass_check(), rtol=1e-6, atol=1e-13)
 
 
-@pytest.mark.parametrize("nc,iacc", [((32, 24, 16), 4), ((48, 40, 24), 2), ((40, 32, 30), 1)])
+@pytest.mark.parametrize("nc,iacc", [((32, 24, 16), 4), ((48, 40, 24), 2), ((40, 32, 30), 1),
+                                     ((32, 25, 16), 4), ((32, 144, 16), 4)])
 def test_exact_projection_steps_vs_oracle(po, nc, iacc):
     """Exact projection (inverse of the composed D*G): GPU eigen-basis solve
     against the oracle's per-mode dense LU; after every step max|div u| is at
-    round-off (the reference's single 2nd-order projection grows it instead)."""
+    round-off (the reference's single 2nd-order projection grows it instead).
+    Even ny takes the mirror-symmetric even/odd path (ny = 144: two row
+    tiles per half), odd ny the general one."""
     from chapsim2_amd.domain import PROJ_EXACT, Domain
     from chapsim2_amd.initial import channel_fields
 
