@@ -126,9 +126,13 @@ int chs_create(const chs_desc_t* desc, chs_handle_t** out) {
     delete H;
     return fail(CHS_EUNSUP, "walls in y need the FFT(x,z)+TDMA(y) Poisson path");
   }
+  if (h.mesh.periodic[1] && desc->poisson != CHS_POISSON_3DFFT) {
+    delete H;
+    return fail(CHS_EUNSUP, "periodic y needs the 3-D FFT Poisson path");
+  }
   std::string e;
-  if (desc->poisson == CHS_POISSON_FISHPACK) {
-    e = chs::Poisson::create(h.mesh, 2, &h.poisson);
+  if (desc->poisson == CHS_POISSON_FISHPACK || desc->poisson == CHS_POISSON_3DFFT) {
+    e = chs::Poisson::create(h.mesh, desc->poisson, desc->iacc, &h.poisson);
     if (!e.empty()) {
       delete H;
       return fail(CHS_EUNSUP, e);

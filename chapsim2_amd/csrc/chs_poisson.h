@@ -17,7 +17,10 @@ class Poisson {
  public:
   // kind: 2 = FFT(x,z)+TDMA(y) with the 2nd-order y operator of
   // src/poisson_fishpack.f90:213-350 (y walls or periodic y is rejected).
-  static std::string create(const Mesh& m, int kind, std::unique_ptr<Poisson>* out);
+  // kind 3 = triply periodic 3-D FFT path of poisson_000
+  // (src/poisson_1stderivcomp.f90:1016-1195) with the modified wavenumbers of
+  // the iacc staggered scheme.
+  static std::string create(const Mesh& m, int kind, int iacc, std::unique_ptr<Poisson>* out);
   ~Poisson();
   // In-place solve on a device array of (nx, ny, nz), Fortran order.
   // `in_scale` multiplies the right-hand side before the forward transform.
@@ -31,10 +34,15 @@ class Poisson {
   std::string solve_exact(float* rhs, hipStream_t s, float in_scale = 1.0f);
   bool has_exact() const { return V64 != nullptr; }
   int nx = 0, ny = 0, nz = 0, nxh = 0;
+  int kind = 2;
 
  private:
+  static std::string create_3d(const Mesh& m, int iacc, std::unique_ptr<Poisson>* out);
   template <typename T> std::string solve_t(T* rhs, hipStream_t s, T in_scale);
-  std::vector<int> rad_x, rad_z;  // radix plans of nx/2 and nz
+  template <typename T> std::string solve_3d_t(T* rhs, hipStream_t s, T in_scale);
+  std::vector<int> rad_x, rad_y, rad_z;  // radix plans of nx/2, ny and nz
+  double* tw_y64 = nullptr; float* tw_y32 = nullptr;   // twiddles of length ny (3-D path)
+  double* ly64 = nullptr; float* ly32 = nullptr;       // 3-D path: ky'^2 (lx/lz hold kx'^2, kz'^2)
   // device tables
   double* tw_x64 = nullptr; float* tw_x32 = nullptr;   // twiddles of length nx/2 (cos, sin pairs)
   double* tw_z64 = nullptr; float* tw_z32 = nullptr;   // twiddles of length nz

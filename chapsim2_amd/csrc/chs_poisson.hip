@@ -18,6 +18,8 @@
 #include <cmath>
 #include <cstring>
 
+#include "chs_plan.h"
+
 namespace chs {
 
 #define PCHK(x)                                                             \
@@ -221,28 +223,56 @@ __global__ void __launch_bounds__(kThreads) k_fft_x_inv(const cx<T>* __restrict_
   }
 }
 
-// z transform (either sign) in place on W(nxh, ny, nz): tiles of consecutive kx.
+// Complex transform (either sign) in place along one strided dimension of the
+// spectrum W(nxh, ny, nz): lines of n points at stride ls, for every kx (tiles
+// of consecutive kx, so loads and stores are contiguous along kx) and every
+// value of the remaining index (count no, stride os).  z: n = nz,
+// ls = nxh*ny, os = nxh; y: n = ny, ls = nxh, os = nxh*ny.
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_fft_z(cx<T>* __restrict__ W, int nxh, int ny, int nz, Radices rp,
-                                                   const cx<T>* tw, T s) {
+__global__ void __launch_bounds__(kThreads) k_fft_s(cx<T>* __restrict__ W, int nxh, int n, long ls, int no, long os,
+                                                   Radices rp, const cx<T>* tw, T s) {
   extern __shared__ unsigned char smem[];
   cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
-  const int lpb = kTileC / nz;
+  const int lpb = kTileC / n;
   const int ntile = (nxh + lpb - 1) / lpb;
-  const int tile = blockIdx.x % ntile, j = blockIdx.x / ntile;
+  const int tile = blockIdx.x % ntile, o = blockIdx.x / ntile;
+  if (o >= no) return;
   const int kx0 = tile * lpb;
   const int nl = (nxh - kx0) < lpb ? (nxh - kx0) : lpb;
-  const long zs = (long)nxh * ny;
-  for (int t = threadIdx.x; t < nl * nz; t += kThreads) {
-    int kz = t / nl, line = t % nl;
-    buf[line * nz + kz] = W[kx0 + line + (long)nxh * j + zs * kz];
+  const long base = kx0 + os * o;
+  for (int t = threadIdx.x; t < nl * n; t += kThreads) {
+    int m = t / nl, line = t % nl;
+    buf[line * n + m] = W[base + line + ls * m];
   }
   __syncthreads();
-  fft_tile<T>(buf, nz, nl, rp, tw, s);
-  for (int t = threadIdx.x; t < nl * nz; t += kThreads) {
-    int kz = t / nl, line = t % nl;
-    W[kx0 + line + (long)nxh * j + zs * kz] = buf[line * nz + kz];
+  fft_tile<T>(buf, n, nl, rp, tw, s);
+  for (int t = threadIdx.x; t < nl * n; t += kThreads) {
+    int m = t / nl, line = t % nl;
+    W[base + line + ls * m] = buf[line * n + m];
   }
+}
+
+// Triply periodic solve in spectral space: phi = -rhs / (kx'^2 + ky'^2 + kz'^2)
+// with the compact schemes' modified wavenumbers; the mean mode is set to 0.
+template <typename T>
+__global__ void k_div_k2(cx<T>* __restrict__ W, int nxh, int ny, int nz, const T* __restrict__ k2x,
+                         const T* __restrict__ k2y, const T* __restrict__ k2z) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)nxh * ny * nz;
+  if (i >= n) return;
+  const int kx = (int)(i % nxh);
+  const long r = i / nxh;
+  const int ky = (int)(r % ny), kz = (int)(r / ny);
+  const T kk = k2x[kx] + k2y[ky] + k2z[kz];
+  cx<T> v = W[i];
+  if (kk < T(1e-16)) {
+    v.re = T(0);
+    v.im = T(0);
+  } else {
+    v.re = v.re / (-kk);
+    v.im = v.im / (-kk);
+  }
+  W[i] = v;
 }
 
 // y tridiagonal per (kx, kz) with host-built TRID0 factors.
@@ -483,7 +513,8 @@ Poisson::~Poisson() {
                   (void*)fz32, work, (void*)V64, (void*)Vi64, (void*)lam64, (void*)lx64, (void*)lz64,
                   (void*)V32, (void*)Vi32, (void*)lam32, (void*)lx32, (void*)lz32, work2, (void*)Ve64,
                   (void*)Vo64, (void*)Vie64, (void*)Vio64, (void*)lame64, (void*)lamo64, (void*)Ve32, (void*)Vo32,
-                  (void*)Vie32, (void*)Vio32, (void*)lame32, (void*)lamo32})
+                  (void*)Vie32, (void*)Vio32, (void*)lame32, (void*)lamo32, (void*)tw_y64, (void*)tw_y32,
+                  (void*)ly64, (void*)ly32})
     if (p) (void)hipFree(p);
 }
 
@@ -495,9 +526,11 @@ static std::string upv(const std::vector<double>& v, T** d) {
   return "";
 }
 
-std::string Poisson::create(const Mesh& m, int kind, std::unique_ptr<Poisson>* out) {
-  if (kind != 2) return "poisson: only the FFT(x,z)+TDMA(y) path is built";
+std::string Poisson::create(const Mesh& m, int kind, int iacc, std::unique_ptr<Poisson>* out) {
+  if (kind != 2 && kind != 3) return "poisson: unknown kind";
+  if (kind == 3) return create_3d(m, iacc, out);
   std::unique_ptr<Poisson> p(new Poisson());
+  p->kind = 2;
   p->nx = m.nc[0]; p->ny = m.nc[1]; p->nz = m.nc[2];
   if (p->nx % 2) return "poisson: nx must be even";
   p->nxh = p->nx / 2 + 1;
@@ -598,8 +631,128 @@ std::string Poisson::create(const Mesh& m, int kind, std::unique_ptr<Poisson>* o
   return "";
 }
 
+// Triply periodic path (poisson_000, src/poisson_1stderivcomp.f90:1016-1195,
+// wavenumbers of waves :2405-2480 with the coefficients of :190-200): real FFT
+// in x, complex FFTs in y and z, division by minus the sum of the squared
+// modified wavenumbers of the staggered first derivative
+//   k'(w) = n/L * (2a sin(w/2) + 2b sin(3w/2)) / (1 + 2 alpha cos w),
+// the mean mode set to zero, inverse transforms and 1/(nx ny nz).  The
+// reference's half-cell shift factors (abxyz) are applied before and undone
+// after a real scalar division, so they cancel and are not formed here.
+std::string Poisson::create_3d(const Mesh& m, int iacc, std::unique_ptr<Poisson>* out) {
+  if (!m.periodic[0] || !m.periodic[1] || !m.periodic[2]) return "poisson: the 3-D FFT path needs periodic x, y, z";
+  if (m.stretched) return "poisson: the 3-D FFT path needs a uniform y mesh";
+  std::unique_ptr<Poisson> p(new Poisson());
+  p->kind = 3;
+  p->nx = m.nc[0]; p->ny = m.nc[1]; p->nz = m.nc[2];
+  if (p->nx % 2) return "poisson: nx must be even";
+  p->nxh = p->nx / 2 + 1;
+  p->rad_x = radix_plan(p->nx / 2);
+  p->rad_y = radix_plan(p->ny);
+  p->rad_z = radix_plan(p->nz);
+  if (p->rad_x.empty() || p->rad_y.empty() || p->rad_z.empty()) return "poisson: nx/2, ny, nz must be 2,3,5-smooth";
+  if (p->rad_x.size() > 12 || p->rad_y.size() > 12 || p->rad_z.size() > 12) return "poisson: transform too long";
+  if (p->nx / 2 > kTileC || p->ny > kTileC || p->nz > kTileC) return "poisson: transform too long for one tile";
+  const double pi = 2.0 * std::asin(1.0);
+  auto twid = [&](int Nc) {
+    std::vector<double> t(2 * Nc);
+    for (int q = 0; q < Nc; ++q) {
+      double a = 2.0 * pi * double(q) / double(Nc);
+      t[2 * q] = std::cos(a);
+      t[2 * q + 1] = -std::sin(a);
+    }
+    return t;
+  };
+  std::vector<double> twx = twid(p->nx / 2), twy = twid(p->ny), twz = twid(p->nz), rot(2 * p->nxh);
+  for (int k = 0; k < p->nxh; ++k) {
+    double a = 2.0 * pi * double(k) / double(p->nx);
+    rot[2 * k] = std::cos(a);
+    rot[2 * k + 1] = -std::sin(a);
+  }
+  // modified wavenumbers squared per complex mode index
+  const double al = coeff(F_D1C2P, false, 3, 1, BC_PERIODIC, iacc);
+  const double ar = coeff(F_D1C2P, true, 3, 1, BC_PERIODIC, iacc);
+  const double br = coeff(F_D1C2P, true, 3, 2, BC_PERIODIC, iacc);
+  const double len[3] = {m.lxx, m.lyt - m.lyb, m.lzz};
+  std::vector<double> k2[3];
+  for (int d = 0; d < 3; ++d) {
+    const int n = m.nc[d];
+    const double h = len[d] / n;
+    const double ac = ar / h, bc = br / h;
+    std::vector<double> half(n / 2 + 1);
+    for (int i = 0; i <= n / 2; ++i) {
+      const double w = 2.0 * pi * double(i) / double(n);
+      double wp = ac * 2.0 * h * std::sin(w * 0.5) + bc * 2.0 * h * std::sin(1.5 * w);
+      wp = wp / (1.0 + 2.0 * al * std::cos(w));
+      const double kk = double(n) * wp / len[d];
+      half[i] = kk * kk;
+    }
+    const int nk = (d == 0) ? p->nxh : n;
+    k2[d].resize(nk);
+    for (int i = 0; i < nk; ++i) k2[d][i] = half[i <= n / 2 ? i : n - i];
+  }
+  std::string e;
+  if (!(e = upv(twx, &p->tw_x64)).empty() || !(e = upv(twx, &p->tw_x32)).empty()) return e;
+  if (!(e = upv(twy, &p->tw_y64)).empty() || !(e = upv(twy, &p->tw_y32)).empty()) return e;
+  if (!(e = upv(twz, &p->tw_z64)).empty() || !(e = upv(twz, &p->tw_z32)).empty()) return e;
+  if (!(e = upv(rot, &p->rot64)).empty() || !(e = upv(rot, &p->rot32)).empty()) return e;
+  if (!(e = upv(k2[0], &p->lx64)).empty() || !(e = upv(k2[0], &p->lx32)).empty()) return e;
+  if (!(e = upv(k2[1], &p->ly64)).empty() || !(e = upv(k2[1], &p->ly32)).empty()) return e;
+  if (!(e = upv(k2[2], &p->lz64)).empty() || !(e = upv(k2[2], &p->lz32)).empty()) return e;
+  p->work_bytes = sizeof(double) * 2 * (size_t)p->nxh * p->ny * p->nz;
+  PCHK(hipMalloc(&p->work, p->work_bytes));
+  *out = std::move(p);
+  return "";
+}
+
+template <typename T>
+std::string Poisson::solve_3d_t(T* rhs, hipStream_t s, T in_scale) {
+  Radices rx{}, ry{}, rz{};
+  rx.n = (int)rad_x.size();
+  for (int q = 0; q < rx.n; ++q) rx.r[q] = rad_x[q];
+  ry.n = (int)rad_y.size();
+  for (int q = 0; q < ry.n; ++q) ry.r[q] = rad_y[q];
+  rz.n = (int)rad_z.size();
+  for (int q = 0; q < rz.n; ++q) rz.r[q] = rad_z[q];
+  const bool dbl = sizeof(T) == 8;
+  auto pk = [&](void* d, void* f) { return dbl ? d : f; };
+  const cx<T>* twx = reinterpret_cast<const cx<T>*>(pk(tw_x64, tw_x32));
+  const cx<T>* twy = reinterpret_cast<const cx<T>*>(pk(tw_y64, tw_y32));
+  const cx<T>* twz = reinterpret_cast<const cx<T>*>(pk(tw_z64, tw_z32));
+  const cx<T>* rot = reinterpret_cast<const cx<T>*>(pk(rot64, rot32));
+  const T* k2x = reinterpret_cast<const T*>(pk(lx64, lx32));
+  const T* k2y = reinterpret_cast<const T*>(pk(ly64, ly32));
+  const T* k2z = reinterpret_cast<const T*>(pk(lz64, lz32));
+  cx<T>* W = reinterpret_cast<cx<T>*>(work);
+  const long nlx = (long)ny * nz;
+  const int N = nx / 2;
+  const int lpbx = kTileC / N;
+  const size_t shx = sizeof(cx<T>) * (size_t)lpbx * N;
+  dim3 gx((unsigned)((nlx + lpbx - 1) / lpbx));
+  hipLaunchKernelGGL(k_fft_x_fwd<T>, gx, dim3(kThreads), shx, s, rhs, W, nx, nlx, rx, twx, rot, in_scale);
+  const int lpby = kTileC / ny, lpbz = kTileC / nz;
+  const size_t shy = sizeof(cx<T>) * (size_t)lpby * ny, shz = sizeof(cx<T>) * (size_t)lpbz * nz;
+  dim3 gy((unsigned)(((nxh + lpby - 1) / lpby) * nz)), gz((unsigned)(((nxh + lpbz - 1) / lpbz) * ny));
+  hipLaunchKernelGGL(k_fft_s<T>, gy, dim3(kThreads), shy, s, W, nxh, ny, (long)nxh, nz, (long)nxh * ny, ry, twy,
+                     T(-1));
+  hipLaunchKernelGGL(k_fft_s<T>, gz, dim3(kThreads), shz, s, W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz,
+                     T(-1));
+  const long nm = (long)nxh * ny * nz;
+  hipLaunchKernelGGL(k_div_k2<T>, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s, W, nxh, ny, nz, k2x, k2y, k2z);
+  hipLaunchKernelGGL(k_fft_s<T>, gz, dim3(kThreads), shz, s, W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz,
+                     T(1));
+  hipLaunchKernelGGL(k_fft_s<T>, gy, dim3(kThreads), shy, s, W, nxh, ny, (long)nxh, nz, (long)nxh * ny, ry, twy,
+                     T(1));
+  const T scale = T(1.0 / (double(nx) * double(ny) * double(nz)));
+  hipLaunchKernelGGL(k_fft_x_inv<T>, gx, dim3(kThreads), shx, s, W, rhs, nx, nlx, rx, twx, rot, scale);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("poisson launch: ") + hipGetErrorString(e);
+  return "";
+}
+
 template <typename T>
 std::string Poisson::solve_t(T* rhs, hipStream_t s, T in_scale) {
+  if (kind == 3) return solve_3d_t<T>(rhs, s, in_scale);
   Radices rx{}, rz{};
   rx.n = (int)rad_x.size();
   for (int q = 0; q < rx.n; ++q) rx.r[q] = rad_x[q];
@@ -621,10 +774,12 @@ std::string Poisson::solve_t(T* rhs, hipStream_t s, T in_scale) {
   const int lpbz = kTileC / nz;
   const size_t shz = sizeof(cx<T>) * (size_t)lpbz * nz;
   dim3 gz((unsigned)(((nxh + lpbz - 1) / lpbz) * ny));
-  hipLaunchKernelGGL(k_fft_z<T>, gz, dim3(kThreads), shz, s, W, nxh, ny, nz, rz, twz, T(-1));
+  hipLaunchKernelGGL(k_fft_s<T>, gz, dim3(kThreads), shz, s, W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz,
+                     T(-1));
   const long nm = (long)nxh * nz;
   hipLaunchKernelGGL(k_ytdma<T>, dim3((unsigned)((nm + 63) / 64)), dim3(64), 0, s, W, nxh, ny, nz, ya64, fd, fz);
-  hipLaunchKernelGGL(k_fft_z<T>, gz, dim3(kThreads), shz, s, W, nxh, ny, nz, rz, twz, T(1));
+  hipLaunchKernelGGL(k_fft_s<T>, gz, dim3(kThreads), shz, s, W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz,
+                     T(1));
   const T scale = T(1.0 / (double(nx) * double(nz)));
   hipLaunchKernelGGL(k_fft_x_inv<T>, gx, dim3(kThreads), shx, s, W, rhs, nx, nlx, rx, twx, rot, scale);
   hipError_t e = hipGetLastError();
@@ -684,7 +839,8 @@ std::string Poisson::solve_exact_t(T* rhs, hipStream_t s, T in_scale) {
   const int lpbz = kTileC / nz;
   const size_t shz = sizeof(cx<T>) * (size_t)lpbz * nz;
   dim3 gz((unsigned)(((nxh + lpbz - 1) / lpbz) * ny));
-  hipLaunchKernelGGL(k_fft_z<T>, gz, dim3(kThreads), shz, s, W, nxh, ny, nz, rz, twz, T(-1));
+  hipLaunchKernelGGL(k_fft_s<T>, gz, dim3(kThreads), shz, s, W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz,
+                     T(-1));
   const int ncr = 2 * nxh;
   const long ncol = (long)ncr * nz;
   if (Ve64) {
@@ -701,7 +857,8 @@ std::string Poisson::solve_exact_t(T* rhs, hipStream_t s, T in_scale) {
     hipLaunchKernelGGL(k_ymix<T>, gm, dim3(256), 0, s, V, W2r, Wr, ny, ncr, ncol, (const T*)nullptr,
                        (const T*)nullptr, (const T*)nullptr, -1);
   }
-  hipLaunchKernelGGL(k_fft_z<T>, gz, dim3(kThreads), shz, s, W, nxh, ny, nz, rz, twz, T(1));
+  hipLaunchKernelGGL(k_fft_s<T>, gz, dim3(kThreads), shz, s, W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz,
+                     T(1));
   const T scale = T(1.0 / (double(nx) * double(nz)));
   hipLaunchKernelGGL(k_fft_x_inv<T>, gx, dim3(kThreads), shx, s, W, rhs, nx, nlx, rx, twx, rot, scale);
   hipError_t e = hipGetLastError();

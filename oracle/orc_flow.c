@@ -499,6 +499,31 @@ orc_flow *orc_flow_create(const orc_flow_cfg *cfg) {
       f->pc[ny - 1] = 0.0;
     }
   }
+  if (cfg->poisson == 3) {
+    /* waves, src/poisson_1stderivcomp.f90:2405-2480 (bcx = bcy = bcz = 0,
+     * istret = 0), coefficients of :190-200 */
+    double pi = 2.0 * asin(1.0);
+    double alc = orc_coeff(ORC_FAM_D1C2P, 0, 3, 1, BPER, cfg->iacc);
+    double ar = orc_coeff(ORC_FAM_D1C2P, 1, 3, 1, BPER, cfg->iacc);
+    double br = orc_coeff(ORC_FAM_D1C2P, 1, 3, 2, BPER, cfg->iacc);
+    double len[3] = {dm->lxx, dm->lyt - dm->lyb, dm->lzz};
+    double **k2[3] = {&f->kx2, &f->ky2, &f->kz2};
+    for (int d = 0; d < 3; ++d) {
+      int n = dm->nc[d];
+      double h = dm->h[d];
+      double ac = ar / h, bc = br / h;
+      double *t = (double *)calloc((size_t)n, sizeof(double));
+      for (int i = 1; i <= n / 2 + 1; ++i) {
+        double w = 2.0 * pi * (double)(i - 1) / (double)n;
+        double wp = ac * 2.0 * h * sin(w * 0.5) + bc * 2.0 * h * sin(1.5 * w);
+        wp = wp / (1.0 + 2.0 * alc * cos(w));
+        double kk = (double)n * wp / len[d];
+        t[i - 1] = kk * kk;
+      }
+      for (int i = n / 2 + 2; i <= n; ++i) t[i - 1] = t[n - i + 1];
+      *k2[d] = t;
+    }
+  }
   if (cfg->projection == 1) exact_setup(f);
   orc_flow_update_re(f, 0);
   return f;
@@ -752,9 +777,62 @@ static void poisson_exact(orc_flow *f, double *rhs) {
   for (long q = 0; q < n; ++q) rhs[q] = rhs[q] / sx / sz;
 }
 
+/* poisson_000, src/poisson_1stderivcomp.f90:1016-1195, triply periodic:
+ * forward 3-D transform scaled by 1/(nx ny nz); each coefficient divided by
+ * -(kx2 + ky2 + kz2) (real and imaginary parts alike, kxyz = (1,1) k2);
+ * zero where k2 < 1e-16; inverse transform.  The abxyz half-cell rotations
+ * multiply by exp(-i theta) before and exp(+i theta) after the real
+ * division, and the sign flips for the upper half of the modes are applied
+ * twice, so both cancel. */
+static void poisson_000(orc_flow *f, double *rhs) {
+  int nx = f->nx, ny = f->ny, nz = f->nz;
+  long n = (long)nx * ny * nz;
+  cpx *c = (cpx *)malloc(sizeof(cpx) * (size_t)n);
+  for (long t = 0; t < n; ++t) { c[t].re = rhs[t]; c[t].im = 0.0; }
+  for (int pass = 0; pass < 2; ++pass) {
+    int sign = pass ? 1 : -1;
+    if (pass) {
+      double sc = 1.0 / (double)nx / (double)ny / (double)nz;
+      for (long t = 0; t < n; ++t) {
+        long i = t % nx, j = (t / nx) % ny, k = t / ((long)nx * ny);
+        double kk = f->kx2[i] + f->ky2[j] + f->kz2[k];
+        double re = c[t].re * sc, im = c[t].im * sc;
+        if (kk < 1.0e-16) { re = 0.0; im = 0.0; }
+        else { re = re / (-kk); im = im / (-kk); }
+        c[t].re = re; c[t].im = im;
+      }
+    }
+#pragma omp parallel
+    {
+      int nmax = nx > ny ? nx : ny;
+      nmax = nmax > nz ? nmax : nz;
+      cpx *line = (cpx *)malloc(sizeof(cpx) * (size_t)nmax);
+#pragma omp for schedule(static)
+      for (long L = 0; L < (long)ny * nz; ++L) cfft(c + L * nx, nx, sign);
+#pragma omp for schedule(static)
+      for (long L = 0; L < (long)nx * nz; ++L) {
+        long i = L % nx, k = L / nx;
+        for (int j = 0; j < ny; ++j) line[j] = c[i + (long)nx * (j + (long)ny * k)];
+        cfft(line, ny, sign);
+        for (int j = 0; j < ny; ++j) c[i + (long)nx * (j + (long)ny * k)] = line[j];
+      }
+#pragma omp for schedule(static)
+      for (long L = 0; L < (long)nx * ny; ++L) {
+        for (int k = 0; k < nz; ++k) line[k] = c[L + (long)nx * ny * k];
+        cfft(line, nz, sign);
+        for (int k = 0; k < nz; ++k) c[L + (long)nx * ny * k] = line[k];
+      }
+      free(line);
+    }
+  }
+  for (long t = 0; t < n; ++t) rhs[t] = c[t].re;
+  free(c);
+}
+
 void orc_poisson_solve(orc_flow *f, double *rhs) {
   if (f->cfg.projection == 1) poisson_exact(f, rhs);
   else if (f->cfg.poisson == 2) poisson_fishpack(f, rhs);
+  else if (f->cfg.poisson == 3) poisson_000(f, rhs);
 }
 
 /* ---------------------------------------------------------------------- */

@@ -140,3 +140,56 @@ def test_reference_projection_growth_with_cp6(po):
         divs[proj] = np.abs(f.divergence()).max()
     assert divs[0] > 1.0
     assert divs[1] < 1e-13
+
+
+def test_poisson_000_single_mode(po):
+    """Triply periodic path: a Fourier mode is divided by minus the squared
+    modified wavenumbers of the staggered CP6 derivative (published
+    coefficients alpha = 9/62, a = 63/62, b = 17/186 per unit spacing)."""
+    nc = [16, 12, 10]
+    L = 2 * np.pi
+    f = po.Flow(nc, L, -np.pi, np.pi, L, istret=0, wall_y=False, iacc=4, poisson=3)
+    h = [L / n for n in nc]
+    xc = (np.arange(nc[0]) + 0.5) * h[0]
+    yc = -np.pi + (np.arange(nc[1]) + 0.5) * h[1]
+    zc = (np.arange(nc[2]) + 0.5) * h[2]
+    X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
+    m = (3, 2, 4)
+    rhs = np.cos(m[0] * X) * np.cos(m[1] * Y) * np.sin(m[2] * Z)
+
+    def kp(mm, n):
+        w = 2 * np.pi * mm / n
+        al, a, b = 9 / 62, 63 / 62, 17 / 186
+        return n / L * (2 * a * np.sin(w / 2) + 2 * b * np.sin(1.5 * w)) / (1 + 2 * al * np.cos(w))
+
+    k2 = sum(kp(mm, n) ** 2 for mm, n in zip(m, nc))
+    phi = f.poisson(np.asfortranarray(rhs))
+    assert np.abs(phi + rhs / k2).max() < 1e-13 * np.abs(rhs / k2).max()
+    # the mean is removed
+    phi0 = f.poisson(np.asfortranarray(np.ones(nc)))
+    assert np.abs(phi0).max() < 1e-14
+
+
+def test_tgv_early_viscous_decay(po):
+    """Taylor-Green vortex (src/initialisation.f90:892-975): for t << 1 the
+    kinetic energy decays at the viscous rate 2 nu |k|^2 = 6/Re."""
+    from chapsim2_amd.initial import tgv_fields
+
+    nc = (32, 32, 32)
+    L = 2 * np.pi
+    f = po.Flow(list(nc), L, -np.pi, np.pi, L, istret=0, wall_y=False, iacc=4, poisson=3, dt=1e-3,
+                ren=1600.0, idriven=0)
+    f.set_state(*tgv_fields(nc, L, -np.pi, np.pi, L))
+    f.update_re(0)
+    f.substep(0)
+
+    def ke():
+        qx, qy, qz, _ = f.get_state()
+        return 0.5 * np.mean(qx**2 + qy**2 + qz**2)
+
+    e0 = ke()
+    for it in range(1, 11):
+        f.step(it)
+    rate = np.log(ke() / e0) / 0.01
+    assert abs(rate + 6.0 / 1600.0) < 1e-4 * 6.0 / 1600.0
+    assert np.abs(f.divergence()).max() < 1e-13
