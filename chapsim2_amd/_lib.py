@@ -9,7 +9,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libchapsim_hip.so")
 
-CHS_OK, CHS_EINVAL, CHS_ENOMEM, CHS_EHIP, CHS_ENAN, CHS_EUNSUP = 0, -1, -2, -3, -4, -5
+CHS_OK, CHS_EINVAL, CHS_ENOMEM, CHS_EHIP, CHS_ENAN, CHS_EUNSUP, CHS_ERANGE = 0, -1, -2, -3, -4, -5, -6
 MEM_HOST, MEM_DEVICE = 0, 1
 
 # every entry point of include/chapsim_hip.h and include/chapsim_hip_inspect.h
@@ -17,6 +17,7 @@ EXPORTS = [
     "chs_abi_version", "chs_last_error", "chs_create", "chs_destroy", "chs_sync", "chs_op3d",
     "chs_poisson", "chs_set_state", "chs_get_state", "chs_update_re", "chs_substep", "chs_run",
     "chs_momentum_rhs", "chs_mass_check", "chs_divergence",
+    "chs_props_create", "chs_props_destroy", "chs_props_info", "chs_props_from_rhoh",
     "chs_inspect_coeff", "chs_inspect_op1d", "chs_inspect_projection", "chs_profile_begin", "chs_profile_end",
 ]
 
@@ -81,6 +82,10 @@ def load():
     L.chs_divergence.argtypes = [vp, vp, C.c_int]
     L.chs_inspect_coeff.restype = C.c_double
     L.chs_inspect_coeff.argtypes = [C.c_int] * 6
+    L.chs_props_create.argtypes = [dp, C.c_int, C.c_double, C.POINTER(vp)]
+    L.chs_props_destroy.argtypes = [vp]
+    L.chs_props_info.argtypes = [vp, dp, dp, dp]
+    L.chs_props_from_rhoh.argtypes = [vp, vp, C.c_long] + [vp] * 7 + [C.c_int, C.c_int]
     L.chs_inspect_op1d.argtypes = [C.POINTER(ChsDesc), C.c_int, C.c_int, C.c_int, ip, dp, C.c_int, dp,
                                    C.c_int, dp]
     L.chs_inspect_projection.argtypes = [C.POINTER(ChsDesc), dp, dp, dp, dp, dp]

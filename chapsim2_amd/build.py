@@ -25,6 +25,7 @@ SOURCES = [
     "chs_poisson.hip",
     "chs_flow.hip",
     "chs_api.hip",
+    "chs_props.hip",
 ]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

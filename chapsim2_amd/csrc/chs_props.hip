@@ -1,0 +1,284 @@
+// Thermal properties from the tabulated equation of state, per cell.
+//
+// Host: the table build of buildup_property_relations_from_table
+// (src/input_thermo.f90:617-693): selection sort by T (:512-556), reference
+// state by bisection in T (:121-154), non-dimensionalisation, rho*h and its
+// range.  Device: one thread per cell runs ftp_refresh_thermal_properties_from_DH
+// (:393-446) -- clamp, bisection on rho*h, h, properties from h (:345-374),
+// h = rho*h / d, properties from h -- with the bisected columns (rho*h, h, d)
+// staged in LDS when they fit.  Floating-point contraction is off in the
+// kernel so every product and sum rounds as in the reference's expressions.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/chapsim_hip.h"
+
+namespace chs {
+void set_error(const std::string& msg);
+}
+
+struct chs_props {
+  int n = 0;
+  double dhmin = 0, dhmax = 0;
+  double ref[7] = {0, 0, 0, 0, 0, 0, 0};
+  std::vector<double> col[8];  // t, d, m, k, b, cp, h, rhoh (non-dimensional)
+  double* dev = nullptr;       // 8 columns of n, same order
+  int* flag = nullptr;         // out-of-range counter
+  void* io = nullptr;          // host-memory staging
+  size_t io_bytes = 0;
+  hipStream_t stream = nullptr;
+  ~chs_props() {
+    if (dev) (void)hipFree(dev);
+    if (flag) (void)hipFree(flag);
+    if (io) (void)hipFree(io);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+enum { CT = 0, CD, CM, CK, CB, CCP, CH, CRH };
+constexpr double kMinP = 1.0e-20;  // src/modules.f90:141
+constexpr int kPropsLds = 2048;    // rows staged in LDS (3 columns, 48 KB)
+
+int fail(int code, const std::string& msg) {
+  chs::set_error(msg);
+  return code;
+}
+
+void bisect_host(const std::vector<double>& c, double v, int* i1o, int* i2o) {
+  int i1 = 0, i2 = (int)c.size() - 1;
+  while (i2 - i1 > 1) {
+    const int im = i1 + (i2 - i1) / 2;
+    const double d1 = c[i1] - v, dm = c[im] - v;
+    if (d1 * dm > kMinP) i1 = im;
+    else i2 = im;
+  }
+  *i1o = i1;
+  *i2o = i2;
+}
+
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ void bisect_dev(const double* c, int n, double v, int* i1o, int* i2o) {
+  int i1 = 0, i2 = n - 1;
+  while (i2 - i1 > 1) {
+    const int im = i1 + (i2 - i1) / 2;
+    const double d1 = c[i1] - v, dm = c[im] - v;
+    if (d1 * dm > kMinP) i1 = im;
+    else i2 = im;
+  }
+  *i1o = i1;
+  *i2o = i2;
+}
+
+template <bool LDS>
+__global__ void __launch_bounds__(256) k_props_from_dh(const double* __restrict__ tab, int n, double dhmin,
+                                                       double dhmax, const double* __restrict__ rhoh_in, long N,
+                                                       double* __restrict__ oh, double* __restrict__ od,
+                                                       double* __restrict__ om, double* __restrict__ ok,
+                                                       double* __restrict__ ot, double* __restrict__ ob,
+                                                       double* __restrict__ ocp, int* __restrict__ flag) {
+  extern __shared__ double sh[];
+  const double* crh = tab + (long)CRH * n;
+  const double* ch = tab + (long)CH * n;
+  const double* cd = tab + (long)CD * n;
+  if (LDS) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      sh[i] = crh[i];
+      sh[n + i] = ch[i];
+      sh[2 * n + i] = cd[i];
+    }
+    __syncthreads();
+    crh = sh;
+    ch = sh + n;
+    cd = sh + 2 * n;
+  }
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double v = rhoh_in[c];
+  if (v < dhmin) {
+    v = dhmin;
+    atomicAdd(flag, 1);
+  } else if (v > dhmax) {
+    v = dhmax;
+    atomicAdd(flag, 1);
+  }
+  int i1, i2;
+  bisect_dev(crh, n, v, &i1, &i2);
+  double w1 = (crh[i2] - v) / (crh[i2] - crh[i1]);
+  double w2 = 1.0 - w1;
+  double h = w1 * ch[i1] + w2 * ch[i2];
+  // properties from h (first pass needs only d)
+  bisect_dev(ch, n, h, &i1, &i2);
+  w1 = (ch[i2] - h) / (ch[i2] - ch[i1]);
+  w2 = 1.0 - w1;
+  const double d1 = w1 * cd[i1] + w2 * cd[i2];
+  h = v / d1;
+  bisect_dev(ch, n, h, &i1, &i2);
+  w1 = (ch[i2] - h) / (ch[i2] - ch[i1]);
+  w2 = 1.0 - w1;
+  auto interp = [&](int col) { return w1 * tab[(long)col * n + i1] + w2 * tab[(long)col * n + i2]; };
+  if (oh) oh[c] = h;
+  if (od) od[c] = w1 * cd[i1] + w2 * cd[i2];
+  if (om) om[c] = interp(CM);
+  if (ok) ok[c] = interp(CK);
+  if (ot) ot[c] = interp(CT);
+  if (ob) ob[c] = interp(CB);
+  if (ocp) ocp[c] = interp(CCP);
+}
+
+#pragma clang fp contract(on)
+
+}  // namespace
+
+extern "C" {
+
+int chs_props_create(const double* table, int nrows, double ref_t0, chs_props_t** out) {
+  if (!table || !out || nrows < 2) return fail(CHS_EINVAL, "invalid argument");
+  *out = nullptr;
+  chs_props* p = new chs_props();
+  p->n = nrows;
+  for (auto& c : p->col) c.resize(nrows);
+  for (int i = 0; i < nrows; ++i) {
+    const double* r = table + 8 * (size_t)i;
+    p->col[CH][i] = r[1];
+    p->col[CT][i] = r[2];
+    p->col[CD][i] = r[3];
+    p->col[CM][i] = r[4];
+    p->col[CK][i] = r[5];
+    p->col[CCP][i] = r[6];
+    p->col[CB][i] = r[7];
+    p->col[CRH][i] = r[3] * r[1];
+  }
+  // selection sort by T (the first minimum wins, as minloc does)
+  for (int i = 0; i < nrows; ++i) {
+    int kmin = i;
+    for (int j = i + 1; j < nrows; ++j)
+      if (p->col[CT][j] < p->col[CT][kmin]) kmin = j;
+    if (kmin != i)
+      for (auto& c : p->col) std::swap(c[i], c[kmin]);
+  }
+  int i1, i2;
+  bisect_host(p->col[CT], ref_t0, &i1, &i2);
+  {
+    const double w1 = (p->col[CT][i2] - ref_t0) / (p->col[CT][i2] - p->col[CT][i1]), w2 = 1.0 - w1;
+    auto in = [&](int c) {
+      const double a = w1 * p->col[c][i1];
+      const double b = w2 * p->col[c][i2];
+      return a + b;
+    };
+    p->ref[0] = ref_t0;
+    p->ref[1] = in(CD);
+    p->ref[2] = in(CM);
+    p->ref[3] = in(CK);
+    p->ref[4] = in(CH);
+    p->ref[5] = in(CB);
+    p->ref[6] = in(CCP);
+  }
+  for (int i = 0; i < nrows; ++i) {
+    p->col[CT][i] = p->col[CT][i] / p->ref[0];
+    p->col[CD][i] = p->col[CD][i] / p->ref[1];
+    p->col[CM][i] = p->col[CM][i] / p->ref[2];
+    p->col[CK][i] = p->col[CK][i] / p->ref[3];
+    p->col[CB][i] = p->col[CB][i] / p->ref[5];
+    p->col[CCP][i] = p->col[CCP][i] / p->ref[6];
+    p->col[CH][i] = (p->col[CH][i] - p->ref[4]) / p->ref[0] / p->ref[6];
+    p->col[CRH][i] = p->col[CD][i] * p->col[CH][i];
+  }
+  int imin = 0, imax = 0;
+  for (int i = 1; i < nrows; ++i) {
+    if (p->col[CRH][i] < p->col[CRH][imin]) imin = i;
+    if (p->col[CRH][i] > p->col[CRH][imax]) imax = i;
+  }
+  p->dhmin = p->col[CRH][imin];
+  p->dhmax = p->col[CRH][imax];
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    delete p;
+    return fail(CHS_EHIP, "no HIP device");
+  }
+  std::vector<double> flat(8 * (size_t)nrows);
+  for (int c = 0; c < 8; ++c) std::memcpy(flat.data() + (size_t)c * nrows, p->col[c].data(), sizeof(double) * nrows);
+  if (hipStreamCreate(&p->stream) != hipSuccess || hipMalloc(&p->dev, sizeof(double) * flat.size()) != hipSuccess ||
+      hipMalloc(&p->flag, sizeof(int)) != hipSuccess ||
+      hipMemcpy(p->dev, flat.data(), sizeof(double) * flat.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    delete p;
+    return fail(CHS_ENOMEM, "device allocation failed");
+  }
+  *out = p;
+  return CHS_OK;
+}
+
+int chs_props_destroy(chs_props_t* p) {
+  if (!p) return CHS_OK;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  delete p;
+  return CHS_OK;
+}
+
+int chs_props_info(const chs_props_t* p, double* dhmin, double* dhmax, double ref[7]) {
+  if (!p) return fail(CHS_EINVAL, "null handle");
+  if (dhmin) *dhmin = p->dhmin;
+  if (dhmax) *dhmax = p->dhmax;
+  if (ref)
+    for (int i = 0; i < 7; ++i) ref[i] = p->ref[i];
+  return CHS_OK;
+}
+
+int chs_props_from_rhoh(chs_props_t* p, const double* rhoh, long n, double* h, double* d, double* m, double* k,
+                        double* t, double* b, double* cp, int mode, int mem) {
+  if (!p || !rhoh || n < 0) return fail(CHS_EINVAL, "invalid argument");
+  if (mode != CHS_PROPS_TABLE) return fail(CHS_EUNSUP, "unknown property mode");
+  if (mem != CHS_MEM_HOST && mem != CHS_MEM_DEVICE) return fail(CHS_EINVAL, "invalid mem kind");
+  if (n == 0) return CHS_OK;
+  double* outs[7] = {h, d, m, k, t, b, cp};
+  const double* din = rhoh;
+  double* dout[7] = {h, d, m, k, t, b, cp};
+  if (mem == CHS_MEM_HOST) {
+    int nout = 0;
+    for (double* o : outs) nout += o != nullptr;
+    const size_t need = sizeof(double) * (size_t)n * (1 + nout);
+    if (need > p->io_bytes) {
+      if (p->io) (void)hipFree(p->io);
+      p->io = nullptr;
+      p->io_bytes = 0;
+      if (hipMalloc(&p->io, need) != hipSuccess) return fail(CHS_ENOMEM, "staging allocation failed");
+      p->io_bytes = need;
+    }
+    double* base = static_cast<double*>(p->io);
+    if (hipMemcpyAsync(base, rhoh, sizeof(double) * n, hipMemcpyHostToDevice, p->stream) != hipSuccess)
+      return fail(CHS_EHIP, "copy in");
+    din = base;
+    int slot = 1;
+    for (int q = 0; q < 7; ++q) dout[q] = outs[q] ? base + (size_t)n * slot++ : nullptr;
+  }
+  if (hipMemsetAsync(p->flag, 0, sizeof(int), p->stream) != hipSuccess) return fail(CHS_EHIP, "flag reset");
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  if (p->n <= kPropsLds) {
+    const size_t shm = sizeof(double) * 3 * (size_t)p->n;
+    hipLaunchKernelGGL(k_props_from_dh<true>, dim3(grid), dim3(256), shm, p->stream, p->dev, p->n, p->dhmin,
+                       p->dhmax, din, n, dout[0], dout[1], dout[2], dout[3], dout[4], dout[5], dout[6], p->flag);
+  } else {
+    hipLaunchKernelGGL(k_props_from_dh<false>, dim3(grid), dim3(256), 0, p->stream, p->dev, p->n, p->dhmin, p->dhmax,
+                       din, n, dout[0], dout[1], dout[2], dout[3], dout[4], dout[5], dout[6], p->flag);
+  }
+  if (hipGetLastError() != hipSuccess) return fail(CHS_EHIP, "property kernel launch");
+  if (mem == CHS_MEM_HOST)
+    for (int q = 0; q < 7; ++q)
+      if (outs[q] && hipMemcpyAsync(outs[q], dout[q], sizeof(double) * n, hipMemcpyDeviceToHost, p->stream) !=
+                         hipSuccess)
+        return fail(CHS_EHIP, "copy out");
+  int nbad = 0;
+  if (hipMemcpyAsync(&nbad, p->flag, sizeof(int), hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+      hipStreamSynchronize(p->stream) != hipSuccess)
+    return fail(CHS_EHIP, "synchronise");
+  if (nbad) return fail(CHS_ERANGE, "rho*h is out of range (" + std::to_string(nbad) + " cells clamped)");
+  return CHS_OK;
+}
+
+}  // extern "C"

@@ -37,7 +37,8 @@ enum {
   CHS_ENOMEM = -2,   /* device allocation failed */
   CHS_EHIP = -3,     /* HIP runtime error */
   CHS_ENAN = -4,     /* non-finite value detected */
-  CHS_EUNSUP = -5    /* configuration outside the supported set */
+  CHS_EUNSUP = -5,   /* configuration outside the supported set */
+  CHS_ERANGE = -6    /* input outside the tabulated range (values were clamped) */
 };
 
 enum { CHS_MEM_HOST = 0, CHS_MEM_DEVICE = 1 };
@@ -139,6 +140,34 @@ int chs_mass_check(chs_handle_t *h, double out[3]);
 /* Divergence of the current velocity, (nc1,nc2,nc3).  Replaces
  * Get_divergence_flow (src/eq_continuity.f90:91-161). */
 int chs_divergence(chs_handle_t *h, double *div, int mem);
+
+/* ---- thermal properties from a tabulated equation of state ------------- */
+
+typedef struct chs_props chs_props_t;
+enum { CHS_PROPS_TABLE = 0 };
+
+/* Build the non-dimensional property table from nrows dimensional rows of
+ * (P, H, T, D, M, K, CP, B) (the columns of the reference's NIST tables):
+ * sort by T, reference state by interpolation at ref_t0, scaling, rho*h and
+ * its range.  Replaces buildup_property_relations_from_table
+ * (src/input_thermo.f90:617-693) with ftplist_sort_t_small2big (:512-556)
+ * and ftp_get_thermal_properties_dimensional_from_T (:121-154). */
+int chs_props_create(const double *table, int nrows, double ref_t0, chs_props_t **out);
+int chs_props_destroy(chs_props_t *p);
+/* dhmin, dhmax (non-dimensional rho*h range) and the dimensional reference
+ * state ref = (T0, d, m, k, h, b, cp); any pointer may be NULL. */
+int chs_props_info(const chs_props_t *p, double *dhmin, double *dhmax, double ref[7]);
+
+/* Per-cell properties from rho*h: clamp to [dhmin, dhmax], bisection on
+ * rho*h, h by linear interpolation, properties from h, h = rho*h / d,
+ * properties from h again.  Replaces ftp_refresh_thermal_properties_from_DH
+ * (src/input_thermo.f90:393-446, with ftp_refresh_thermal_properties_from_H
+ * :345-374) applied to every cell as Update_thermal_properties does
+ * (src/eq_energy.f90:14-47).  Any output may be NULL.  Returns CHS_ERANGE
+ * (after writing the clamped results) when an input was out of range, where
+ * the reference stops with an error. */
+int chs_props_from_rhoh(chs_props_t *p, const double *rhoh, long n, double *h, double *d, double *m, double *k,
+                        double *t, double *b, double *cp, int mode, int mem);
 
 #ifdef __cplusplus
 }

@@ -116,6 +116,20 @@ double orc_vol_average_x(orc_flow *f, const double *var, int nx);
 void orc_set_threads(int n);
 int orc_get_threads(void);
 
+/* ---- thermal-property table (src/input_thermo.f90:121-446, 512-693) ---- */
+#define ORC_PROPS_MAX 16384
+typedef struct orc_props {
+  int n;
+  double t[ORC_PROPS_MAX], d[ORC_PROPS_MAX], m[ORC_PROPS_MAX], k[ORC_PROPS_MAX];
+  double b[ORC_PROPS_MAX], cp[ORC_PROPS_MAX], h[ORC_PROPS_MAX], rhoh[ORC_PROPS_MAX];
+  double ref[7]; /* T0, d, m, k, h, b, cp of the reference state (dimensional) */
+  double dhmin, dhmax;
+} orc_props;
+/* raw: nlist rows of (P, H, T, D, M, K, CP, B), dimensional */
+int orc_props_build(const double *raw, int nlist, double ref_t0, orc_props *p);
+void orc_props_from_h(const orc_props *p, double h, double out[6]);
+int orc_props_from_dh(const orc_props *p, double rhoh, double out[7]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,9 +42,20 @@ class OrcFlowCfg(C.Structure):
     ]
 
 
+def _stale():
+    if not os.path.exists(LIB_PATH):
+        return True
+    t = os.path.getmtime(LIB_PATH)
+    srcs = [f for f in os.listdir(HERE) if f.endswith((".c", ".h")) or f == "Makefile"]
+    return any(os.path.getmtime(os.path.join(HERE, f)) > t for f in srcs)
+
+
 def build(force=False):
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.check_call(["make", "-s", "-C", HERE])
+    """Build liboracle.so when it is missing or older than its sources (the
+    sources are absent on a machine that only received the built library)."""
+    have_src = os.path.exists(os.path.join(HERE, "orc_ops.c"))
+    if force or (have_src and _stale()) or not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-s", "-B" if force else "-s", "-C", HERE])
 
 
 _lib = None
@@ -302,3 +313,47 @@ def set_threads(n):
 
 def get_threads():
     return lib().orc_get_threads()
+
+
+class OrcProps(C.Structure):
+    _MAX = 16384
+    _fields_ = [("n", C.c_int)] + [(nm, C.c_double * 16384) for nm in ("t", "d", "m", "k", "b", "cp", "h", "rhoh")] + [
+        ("ref", C.c_double * 7), ("dhmin", C.c_double), ("dhmax", C.c_double)]
+
+
+class Props:
+    """Thermal-property table of the oracle (src/input_thermo.f90:121-693)."""
+
+    def __init__(self, raw, ref_t0):
+        raw = np.ascontiguousarray(raw, dtype=np.float64)
+        self.s = OrcProps()
+        rc = lib().orc_props_build(_p(raw), raw.shape[0], C.c_double(ref_t0), C.byref(self.s))
+        if rc != 0:
+            raise ValueError("bad table")
+
+    @property
+    def dhmin(self):
+        return self.s.dhmin
+
+    @property
+    def dhmax(self):
+        return self.s.dhmax
+
+    def column(self, name):
+        return np.array(getattr(self.s, name)[: self.s.n])
+
+    def from_h(self, h):
+        out = (C.c_double * 6)()
+        lib().orc_props_from_h(C.byref(self.s), C.c_double(h), out)
+        return np.array(out[:])
+
+    def from_dh(self, rhoh):
+        """(h, d, m, k, t, b, cp), clamped flag"""
+        rhoh = np.atleast_1d(np.asarray(rhoh, dtype=np.float64))
+        res = np.zeros((len(rhoh), 7))
+        flags = np.zeros(len(rhoh), dtype=bool)
+        out = (C.c_double * 7)()
+        for i, v in enumerate(rhoh):
+            flags[i] = lib().orc_props_from_dh(C.byref(self.s), C.c_double(v), out) != 0
+            res[i] = out[:]
+        return res, flags
