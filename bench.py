@@ -7,7 +7,7 @@ flux, Re_b = 2800, dt = 5e-4; initial field laminar Poiseuille + 10 % seeded
 uniform noise.  One step = one RK3 time step (three sub-steps: momentum RHS,
 fractional step, Poisson solve, projection).
 
-    python bench.py [--steps K] [--warmup W] [--form fp64c|fp64r]
+    python bench.py [--steps K] [--warmup W] [--form fp64c|mixed|fp64r]
 
 Prints one JSON line.  `value` is Mpts*RK3-steps/s with all fields resident
 in HBM; `roofline` reports the achieved HBM bandwidth of the dominant kernel
@@ -35,8 +35,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--form", default="fp64c", choices=["fp64c", "fp64r"],
+    ap.add_argument("--form", default="fp64c", choices=["fp64c", "mixed", "fp64r"],
                     help="fp64c: FP64 with the exact projection (max|div u| at round-off); "
+                         "mixed: FP32 momentum right-hand side, FP64 state, divergence and exact projection; "
                          "fp64r: FP64 with the reference's single 2nd-order projection")
     ap.add_argument("--profile-class", type=int, default=0,
                     help="line-operator direction whose launches are timed (0 x, 1 y, 2 z)")
@@ -167,8 +168,11 @@ def main():
     fields = channel_fields(nc, lx, -1.0, 1.0, lz, 2, 0.075, noise=0.1, seed=20261202)
     from chapsim2_amd.domain import PROJ_EXACT, PROJ_REFERENCE
 
-    proj = PROJ_EXACT if args.form == "fp64c" else PROJ_REFERENCE
-    dm = Domain(nc, lx, -1.0, 1.0, lz, projection=proj, **cfg)
+    from chapsim2_amd.domain import PREC_FP64, PREC_MIXED
+
+    proj = PROJ_REFERENCE if args.form == "fp64r" else PROJ_EXACT
+    prec = PREC_MIXED if args.form == "mixed" else PREC_FP64
+    dm = Domain(nc, lx, -1.0, 1.0, lz, projection=proj, precision=prec, **cfg)
     dm.set_state(*fields)
     dm.update_re(0)
     dm.substep(0)  # initial Euler step + projection (src/chapsim.f90:135)
@@ -206,7 +210,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if args.form == "mixed" else "f64",
         "data": "synthetic",
         "config": {
             "workload": f"periodic channel {nc[0]}x{nc[1]}x{nc[2]}, 4pi x 2 x 4pi/3, tanh beta 1.5, CP6, "

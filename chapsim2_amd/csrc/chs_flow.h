@@ -23,6 +23,29 @@ struct Field {
   long size() const { return (long)e.n[0] * e.n[1] * e.n[2]; }
 };
 
+// Everything the momentum right-hand side reads and writes, in the precision
+// T it is computed in.  For T = double the velocity / pressure fields alias
+// the solver state; for T = float (mixed form) they are rounded copies made
+// at the start of each sub-step.
+template <typename T>
+struct RhsWork {
+  Field<T> qx, qy, qz, pr;
+  Field<T> mx0, my0, mz0;  // previous sub-step's right-hand side (rhs0)
+  // first-level intermediates
+  Field<T> qxix, qxdx, qxdy, qxiy, qxiz, qxdz;
+  Field<T> qydx, qyix, qydy, qyiy, qydz, qyiz;
+  Field<T> qzdx, qzix, qziy, qzdy, qziz, qzdz, div;
+  // accumulators per momentum component: [comp][x-part, y-part, z-part, pfc]
+  Field<T> acc[3][4];
+  // wall data (nx,4,nz) and wall-plane scratch
+  T *fby_u = nullptr, *fby_v = nullptr, *fby_w = nullptr, *fby_p = nullptr, *fby_pn = nullptr;
+  T *fby_conv = nullptr, *fby_visc = nullptr, *fby_div = nullptr;
+  T *plane_a = nullptr, *plane_b = nullptr, *plane_c = nullptr, *plane_d = nullptr;
+  T* scratch = nullptr;
+  long scratch_n = 0;
+  bool ready = false;
+};
+
 class Flow {
  public:
   static std::string create(Handle* H, std::unique_ptr<Flow>* out);
@@ -40,6 +63,7 @@ class Flow {
   Handle* H = nullptr;
   int nx, ny, nz, px, py, pz;
   bool wall_y = true;
+  bool mixed = false;   // momentum right-hand side in FP32 (CHS_PREC_MIXED)
   double rre = 1.0, dt = 0.0, sigma2p = 1.0;
   double tg[4], tz[4], ta[4];
   int ibc[3][5][2];     // [dir][u,v,w,p,T][side]
@@ -47,29 +71,23 @@ class Flow {
   double vol = 1.0;     // sum of the cell volumes (x-pencil qx cells)
   std::vector<double> dyw;  // per-j volume weight dy (size ny)
 
-  // state
-  Field<double> qx, qy, qz, pr, pcor, mx0, my0, mz0;
-  // first-level intermediates
-  Field<double> qxix, qxdx, qxdy, qxiy, qxiz, qxdz;
-  Field<double> qydx, qyix, qydy, qyiy, qydz, qyiz;
-  Field<double> qzdx, qzix, qziy, qzdy, qziz, qzdz, div;
-  // accumulators per momentum component: [comp][x-part, y-part, z-part, pfc]
-  Field<double> acc[3][4];
-  // wall planes (nx,4,nz) and plane scratch
-  double *fby_u = nullptr, *fby_v = nullptr, *fby_w = nullptr, *fby_p = nullptr, *fby_pn = nullptr;
-  double *fby_conv = nullptr, *fby_visc = nullptr, *fby_div = nullptr;
-  double *plane_a = nullptr, *plane_b = nullptr, *plane_c = nullptr, *plane_d = nullptr;
+  // FP64 state, pressure increment and divergence
+  Field<double> qx, qy, qz, pr, pcor, div;
+  RhsWork<double> w64;
+  RhsWork<float> w32;
   double *partial = nullptr, *bulk = nullptr;
   long npartial = 0;  // entries of partial (at least kMaxPartials)
-  double* scratch = nullptr;
-  long scratch_n = 0;
   double* dyw_dev = nullptr;
 
  private:
-  std::string op(int dir, int opk, const int ibc2[2], const Field<double>& A, Field<double>& out,
-                 const double* fbc, double sA = 1.0, const Field<double>* B = nullptr, double cB = 1.0,
-                 const Field<double>* Cm = nullptr, double sOut = 1.0, int accumulate = 0);
-  std::string alloc(Field<double>& f, int e1, int e2, int e3);
+  template <typename T>
+  std::string op(RhsWork<T>& w, int dir, int opk, const int ibc2[2], const Field<T>& A, Field<T>& out,
+                 const T* fbc, T sA = T(1), const Field<T>* B = nullptr, T cB = T(1), const Field<T>* Cm = nullptr,
+                 T sOut = T(1), int accumulate = 0);
+  template <typename T> std::string momentum_rhs_t(RhsWork<T>& w, int isub);
+  template <typename T> std::string alloc_work(RhsWork<T>& w, bool alias_state);
+  template <typename T> std::string alloc(Field<T>& f, int e1, int e2, int e3);
+  template <typename T> std::string update_state(RhsWork<T>& w);
   std::vector<void*> owned;
 };
 

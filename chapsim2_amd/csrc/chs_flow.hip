@@ -10,6 +10,12 @@
 // that consumes them; each directional contribution accumulates in place in
 // its own array, and the three are summed in the reference's order by the
 // fractional-step kernel (Calculate_momentum_fractional_step, :35-71).
+//
+// Precision forms: FP64 everywhere, or mixed (CHS_PREC_MIXED): the momentum
+// right-hand side (44 line operators, products, fractional-step combination)
+// runs in FP32 on rounded copies of the state, while the state update, the
+// bulk-flux sum, the divergence, the pressure solve and the velocity
+// correction stay FP64.
 #include "chs_flow.h"
 #include "chs_exact.h"
 
@@ -35,11 +41,12 @@ enum { VU = 0, VV = 1, VW = 2, VP = 3, VT = 4 };
 // ---------------------------------------------------------------------------
 // elementwise kernels
 // ---------------------------------------------------------------------------
-__global__ void k_div3(double* __restrict__ d, const double* __restrict__ a, const double* __restrict__ b,
-                       const double* __restrict__ c, long n) {
+template <typename T>
+__global__ void k_div3(T* __restrict__ d, const T* __restrict__ a, const T* __restrict__ b, const T* __restrict__ c,
+                       long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    double v = 0.0;
+    T v = T(0);
     v = v + a[i];
     v = v + b[i];
     v = v + c[i];
@@ -48,26 +55,27 @@ __global__ void k_div3(double* __restrict__ d, const double* __restrict__ a, con
 }
 
 // rhs = dt * (tg*((ax + ay) + az) + tz*rhs0 + ta*pfc); rhs0 = (ax + ay) + az.
-// Optionally accumulates the x-pencil volume integral of rhs per block.
-__global__ void k_combine(double* __restrict__ ax, const double* __restrict__ ay, const double* __restrict__ az,
-                          const double* __restrict__ pfc, double* __restrict__ r0, long n, double tg, double tz,
-                          double ta, double dt, int n1, int n2, const double* __restrict__ dyw, double dxdz_x,
-                          double dxdz_z, double* __restrict__ partial) {
+// Optionally accumulates the x-pencil volume integral of rhs per block (FP64).
+template <typename T>
+__global__ void k_combine(T* __restrict__ ax, const T* __restrict__ ay, const T* __restrict__ az,
+                          const T* __restrict__ pfc, T* __restrict__ r0, long n, T tg, T tz, T ta, T dt, int n1,
+                          int n2, const double* __restrict__ dyw, double dxdz_x, double dxdz_z,
+                          double* __restrict__ partial) {
   __shared__ double red[256];
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double w = 0.0;
   if (i < n) {
-    double cur = ax[i];
+    T cur = ax[i];
     cur = cur + ay[i];
     cur = cur + az[i];
-    double tot = tg * cur + tz * r0[i];
+    T tot = tg * cur + tz * r0[i];
     r0[i] = cur;
     tot = tot + ta * pfc[i];
-    double rv = dt * tot;
+    T rv = dt * tot;
     ax[i] = rv;
     if (partial) {
       int j = (int)((i / n1) % n2);
-      w = rv * dyw[j] * dxdz_x * dxdz_z;
+      w = (double)rv * dyw[j] * dxdz_x * dxdz_z;
     }
   }
   if (partial) {
@@ -106,12 +114,13 @@ __global__ void __launch_bounds__(1024) k_finish_sum(const double* __restrict__ 
   if (threadIdx.x == 0) bulk[0] = red[0] / vol;
 }
 
-// q = q + (rhs - bulk); y walls of qy set from fby (enforce_velo_from_fbc)
-__global__ void k_update(double* __restrict__ q, const double* __restrict__ rhs, const double* __restrict__ bulk,
-                         long n, int n1, int n2, const double* __restrict__ fbw) {
+// q = q + (rhs - bulk) in FP64; y walls of qy set from fby (enforce_velo_from_fbc)
+template <typename T>
+__global__ void k_update(double* __restrict__ q, const T* __restrict__ rhs, const double* __restrict__ bulk, long n,
+                         int n1, int n2, const double* __restrict__ fbw) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double r = rhs[i];
+  double r = (double)rhs[i];
   if (bulk) r = r - bulk[0];
   double v = q[i] + r;
   if (fbw) {
@@ -123,10 +132,16 @@ __global__ void k_update(double* __restrict__ q, const double* __restrict__ rhs,
   q[i] = v;
 }
 
-__global__ void k_sub_bulk(double* __restrict__ o, const double* __restrict__ a, const double* __restrict__ bulk,
-                           long n) {
+template <typename T>
+__global__ void k_sub_bulk(double* __restrict__ o, const T* __restrict__ a, const double* __restrict__ bulk, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) o[i] = a[i] - bulk[0];
+  if (i < n) o[i] = (double)a[i] - (bulk ? bulk[0] : 0.0);
+}
+
+template <typename TO, typename TI>
+__global__ void k_cvt(TO* __restrict__ o, const TI* __restrict__ a, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = (TO)a[i];
 }
 
 __global__ void k_add(double* __restrict__ a, const double* __restrict__ b, long n) {
@@ -143,7 +158,8 @@ __global__ void k_enforce_walls(double* __restrict__ q, int n1, int n2, int n3, 
 }
 
 // planes (n1,2,n3) <- a(:,0,:), a(:,n2-1,:)
-__global__ void k_extract_walls(double* __restrict__ pl, const double* __restrict__ a, int n1, int n2, int n3) {
+template <typename T>
+__global__ void k_extract_walls(T* __restrict__ pl, const T* __restrict__ a, int n1, int n2, int n3) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)n1 * n3) return;
   long i = t % n1, k = t / n1;
@@ -152,15 +168,16 @@ __global__ void k_extract_walls(double* __restrict__ pl, const double* __restric
 }
 
 // div and visc wall values (nx,4,nz) from the three wall-plane derivatives
-__global__ void k_wall_fbc(double* __restrict__ fdiv, double* __restrict__ fvisc, const double* __restrict__ dxw,
-                           const double* __restrict__ dyw, const double* __restrict__ dzw, int n1, int n3) {
+template <typename T>
+__global__ void k_wall_fbc(T* __restrict__ fdiv, T* __restrict__ fvisc, const T* __restrict__ dxw,
+                           const T* __restrict__ dyw, const T* __restrict__ dzw, int n1, int n3) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)n1 * n3) return;
   long i = t % n1, k = t / n1;
   for (int s = 0; s < 2; ++s) {
     long p = i + (long)n1 * (s + 2 * k);
-    double dv = dxw[p] + dyw[p] + dzw[p];
-    double vs = (dyw[p] - ONE_THIRD * dv) * 2.0 * 1.0;
+    T dv = dxw[p] + dyw[p] + dzw[p];
+    T vs = (dyw[p] - T(ONE_THIRD) * dv) * T(2) * T(1);
     fdiv[i + (long)n1 * (s + 4 * k)] = dv;
     fdiv[i + (long)n1 * (s + 2 + 4 * k)] = dv;
     fvisc[i + (long)n1 * (s + 4 * k)] = vs;
@@ -168,11 +185,13 @@ __global__ void k_wall_fbc(double* __restrict__ fdiv, double* __restrict__ fvisc
   }
 }
 
-__global__ void k_neg_prod(double* __restrict__ o, const double* __restrict__ a, long n) {
+template <typename T>
+__global__ void k_neg_prod(T* __restrict__ o, const T* __restrict__ a, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) o[i] = -a[i] * a[i];
 }
-__global__ void k_neg(double* __restrict__ o, const double* __restrict__ a, long n) {
+template <typename T>
+__global__ void k_neg(T* __restrict__ o, const T* __restrict__ a, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) o[i] = -a[i];
 }
@@ -240,11 +259,73 @@ Flow::~Flow() {
   for (void* p : owned) (void)hipFree(p);
 }
 
-std::string Flow::alloc(Field<double>& f, int e1, int e2, int e3) {
+template <typename T>
+std::string Flow::alloc(Field<T>& f, int e1, int e2, int e3) {
   f.e = Ext3{{e1, e2, e3}};
-  FCHK(hipMalloc(&f.p, sizeof(double) * (size_t)f.size()));
-  FCHK(hipMemset(f.p, 0, sizeof(double) * (size_t)f.size()));
+  FCHK(hipMalloc(&f.p, sizeof(T) * (size_t)f.size()));
+  FCHK(hipMemset(f.p, 0, sizeof(T) * (size_t)f.size()));
   owned.push_back(f.p);
+  return "";
+}
+
+// Work arrays of the right-hand side in precision T; with alias_state the
+// velocity / pressure fields are the FP64 state itself.
+template <typename T>
+std::string Flow::alloc_work(RhsWork<T>& w, bool alias_state) {
+  const int nx_ = nx, ny_ = ny, nz_ = nz, px_ = px, py_ = py, pz_ = pz;
+  if (alias_state) {
+    if constexpr (sizeof(T) == sizeof(double)) {
+      w.qx = qx; w.qy = qy; w.qz = qz; w.pr = pr;
+    } else {
+      return "work: only FP64 work arrays may alias the state";
+    }
+  } else {
+    SCHK(alloc(w.qx, px_, ny_, nz_)); SCHK(alloc(w.qy, nx_, py_, nz_)); SCHK(alloc(w.qz, nx_, ny_, pz_));
+    SCHK(alloc(w.pr, nx_, ny_, nz_));
+  }
+  SCHK(alloc(w.mx0, px_, ny_, nz_)); SCHK(alloc(w.my0, nx_, py_, nz_)); SCHK(alloc(w.mz0, nx_, ny_, pz_));
+  SCHK(alloc(w.qxix, nx_, ny_, nz_)); SCHK(alloc(w.qxdx, nx_, ny_, nz_)); SCHK(alloc(w.qxdy, px_, py_, nz_));
+  SCHK(alloc(w.qxiy, px_, py_, nz_)); SCHK(alloc(w.qxiz, px_, ny_, pz_)); SCHK(alloc(w.qxdz, px_, ny_, pz_));
+  SCHK(alloc(w.qydx, px_, py_, nz_)); SCHK(alloc(w.qyix, px_, py_, nz_)); SCHK(alloc(w.qydy, nx_, ny_, nz_));
+  SCHK(alloc(w.qyiy, nx_, ny_, nz_)); SCHK(alloc(w.qydz, nx_, py_, pz_)); SCHK(alloc(w.qyiz, nx_, py_, pz_));
+  SCHK(alloc(w.qzdx, px_, ny_, pz_)); SCHK(alloc(w.qzix, px_, ny_, pz_)); SCHK(alloc(w.qziy, nx_, py_, pz_));
+  SCHK(alloc(w.qzdy, nx_, py_, pz_)); SCHK(alloc(w.qziz, nx_, ny_, nz_)); SCHK(alloc(w.qzdz, nx_, ny_, nz_));
+  SCHK(alloc(w.div, nx_, ny_, nz_));
+  const int cext[3][3] = {{px_, ny_, nz_}, {nx_, py_, nz_}, {nx_, ny_, pz_}};
+  for (int c = 0; c < 3; ++c)
+    for (int q = 0; q < 4; ++q) SCHK(alloc(w.acc[c][q], cext[c][0], cext[c][1], cext[c][2]));
+  long big = 0;
+  for (auto* fl : {&w.qx, &w.qy, &w.qz, &w.qxdy, &w.qydz, &w.qxiz}) big = std::max(big, fl->size());
+  w.scratch_n = big;
+  FCHK(hipMalloc(&w.scratch, sizeof(T) * (size_t)big));
+  owned.push_back(w.scratch);
+  // wall values (nx,4,nz) for u, v, w, p; constant walls as initialise_fbc_flow_given
+  const chs_desc_t& d = H->desc;
+  const long nf = (long)nx_ * 4 * nz_;
+  auto wall = [&](T** p, double v0, double v1) -> std::string {
+    std::vector<T> h(nf);
+    for (long t = 0; t < nf; ++t) h[t] = T(((t / nx_) % 4) % 2 == 0 ? v0 : v1);
+    FCHK(hipMalloc(p, sizeof(T) * nf));
+    FCHK(hipMemcpy(*p, h.data(), sizeof(T) * nf, hipMemcpyHostToDevice));
+    owned.push_back(*p);
+    return "";
+  };
+  SCHK(wall(&w.fby_u, d.uwall[0][0], d.uwall[1][0]));
+  SCHK(wall(&w.fby_v, d.uwall[0][1], d.uwall[1][1]));
+  SCHK(wall(&w.fby_w, d.uwall[0][2], d.uwall[1][2]));
+  SCHK(wall(&w.fby_p, 0.0, 0.0));
+  SCHK(wall(&w.fby_pn, -0.0, -0.0));
+  SCHK(wall(&w.fby_conv, 0.0, 0.0));
+  SCHK(wall(&w.fby_visc, 0.0, 0.0));
+  SCHK(wall(&w.fby_div, 0.0, 0.0));
+  for (T** p : {&w.plane_a, &w.plane_b, &w.plane_c, &w.plane_d}) {
+    FCHK(hipMalloc(p, sizeof(T) * (size_t)std::max(px_, nx_) * 2 * std::max(pz_, nz_)));
+    owned.push_back(*p);
+  }
+  // the conv-y fbc of the y momentum is -fbcy_qy * fbcy_qy
+  hipLaunchKernelGGL(k_neg_prod<T>, grid1(nf), dim3(256), 0, H->stream, w.fby_conv, w.fby_v, nf);
+  hipLaunchKernelGGL(k_neg<T>, grid1(nf), dim3(256), 0, H->stream, w.fby_pn, w.fby_p, nf);
+  w.ready = true;
   return "";
 }
 
@@ -257,6 +338,7 @@ std::string Flow::create(Handle* H, std::unique_ptr<Flow>* out) {
   f.nx = m.nc[0]; f.ny = m.nc[1]; f.nz = m.nc[2];
   f.px = m.np[0]; f.py = m.np[1]; f.pz = m.np[2];
   f.wall_y = !m.periodic[1];
+  f.mixed = (d.precision == CHS_PREC_MIXED);
   f.dt = d.dt;
   f.sigma2p = 1.0;
   f.tg[0] = 1.0; f.tg[1] = 8.0 / 15.0; f.tg[2] = 5.0 / 12.0; f.tg[3] = 3.0 * 0.25;
@@ -288,45 +370,10 @@ std::string Flow::create(Handle* H, std::unique_ptr<Flow>* out) {
   }
   const int nx = f.nx, ny = f.ny, nz = f.nz, px = f.px, py = f.py, pz = f.pz;
   SCHK(f.alloc(f.qx, px, ny, nz)); SCHK(f.alloc(f.qy, nx, py, nz)); SCHK(f.alloc(f.qz, nx, ny, pz));
-  SCHK(f.alloc(f.pr, nx, ny, nz)); SCHK(f.alloc(f.pcor, nx, ny, nz));
-  SCHK(f.alloc(f.mx0, px, ny, nz)); SCHK(f.alloc(f.my0, nx, py, nz)); SCHK(f.alloc(f.mz0, nx, ny, pz));
-  SCHK(f.alloc(f.qxix, nx, ny, nz)); SCHK(f.alloc(f.qxdx, nx, ny, nz)); SCHK(f.alloc(f.qxdy, px, py, nz));
-  SCHK(f.alloc(f.qxiy, px, py, nz)); SCHK(f.alloc(f.qxiz, px, ny, pz)); SCHK(f.alloc(f.qxdz, px, ny, pz));
-  SCHK(f.alloc(f.qydx, px, py, nz)); SCHK(f.alloc(f.qyix, px, py, nz)); SCHK(f.alloc(f.qydy, nx, ny, nz));
-  SCHK(f.alloc(f.qyiy, nx, ny, nz)); SCHK(f.alloc(f.qydz, nx, py, pz)); SCHK(f.alloc(f.qyiz, nx, py, pz));
-  SCHK(f.alloc(f.qzdx, px, ny, pz)); SCHK(f.alloc(f.qzix, px, ny, pz)); SCHK(f.alloc(f.qziy, nx, py, pz));
-  SCHK(f.alloc(f.qzdy, nx, py, pz)); SCHK(f.alloc(f.qziz, nx, ny, nz)); SCHK(f.alloc(f.qzdz, nx, ny, nz));
-  SCHK(f.alloc(f.div, nx, ny, nz));
-  const int cext[3][3] = {{px, ny, nz}, {nx, py, nz}, {nx, ny, pz}};
-  for (int c = 0; c < 3; ++c)
-    for (int q = 0; q < 4; ++q) SCHK(f.alloc(f.acc[c][q], cext[c][0], cext[c][1], cext[c][2]));
-  long big = 0;
-  for (auto* fl : {&f.qx, &f.qy, &f.qz, &f.qxdy, &f.qydz, &f.qxiz}) big = std::max(big, fl->size());
-  f.scratch_n = big;
-  FCHK(hipMalloc(&f.scratch, sizeof(double) * (size_t)big));
-  f.owned.push_back(f.scratch);
-  // wall values (nx,4,nz) for u, v, w, p; constant walls as initialise_fbc_flow_given
-  const long nf = (long)nx * 4 * nz;
-  auto wall = [&](double** p, double v0, double v1) -> std::string {
-    std::vector<double> h(nf);
-    for (long t = 0; t < nf; ++t) h[t] = ((t / nx) % 4) % 2 == 0 ? v0 : v1;
-    FCHK(hipMalloc(p, sizeof(double) * nf));
-    FCHK(hipMemcpy(*p, h.data(), sizeof(double) * nf, hipMemcpyHostToDevice));
-    f.owned.push_back(*p);
-    return "";
-  };
-  SCHK(wall(&f.fby_u, d.uwall[0][0], d.uwall[1][0]));
-  SCHK(wall(&f.fby_v, d.uwall[0][1], d.uwall[1][1]));
-  SCHK(wall(&f.fby_w, d.uwall[0][2], d.uwall[1][2]));
-  SCHK(wall(&f.fby_p, 0.0, 0.0));
-  SCHK(wall(&f.fby_pn, -0.0, -0.0));
-  SCHK(wall(&f.fby_conv, 0.0, 0.0));
-  SCHK(wall(&f.fby_visc, 0.0, 0.0));
-  SCHK(wall(&f.fby_div, 0.0, 0.0));
-  for (double** p : {&f.plane_a, &f.plane_b, &f.plane_c, &f.plane_d}) {
-    FCHK(hipMalloc(p, sizeof(double) * (size_t)std::max(px, nx) * 2 * std::max(pz, nz)));
-    f.owned.push_back(*p);
-  }
+  SCHK(f.alloc(f.pr, nx, ny, nz)); SCHK(f.alloc(f.pcor, nx, ny, nz)); SCHK(f.alloc(f.div, nx, ny, nz));
+  // the FP64 work set is always present (projection and its wall data use it)
+  SCHK(f.alloc_work(f.w64, true));
+  if (f.mixed) SCHK(f.alloc_work(f.w32, false));
   {
     long nblk = std::max<long>((f.qx.size() + 255) / 256, kMaxPartials);
     f.npartial = nblk;
@@ -338,9 +385,6 @@ std::string Flow::create(Handle* H, std::unique_ptr<Flow>* out) {
     FCHK(hipMemcpy(f.dyw_dev, f.dyw.data(), sizeof(double) * ny, hipMemcpyHostToDevice));
     f.owned.push_back(f.dyw_dev);
   }
-  // the conv-y fbc of the y momentum is -fbcy_qy * fbcy_qy
-  hipLaunchKernelGGL(k_neg_prod, grid1(nf), dim3(256), 0, H->stream, f.fby_conv, f.fby_v, nf);
-  hipLaunchKernelGGL(k_neg, grid1(nf), dim3(256), 0, H->stream, f.fby_pn, f.fby_p, nf);
   FCHK(hipStreamSynchronize(H->stream));
   *out = std::move(F);
   return "";
@@ -351,30 +395,31 @@ void Flow::update_re(int iter) {
   rre = (iter < d.init_re_to) ? 1.0 / d.reninit : 1.0 / d.ren;
 }
 
-std::string Flow::op(int dir, int opk, const int ibc2[2], const Field<double>& A, Field<double>& out,
-                     const double* fbc, double sA, const Field<double>* B, double cB, const Field<double>* Cm,
-                     double sOut, int accumulate) {
-  const DevPlan<double>* P = nullptr;
-  SCHK(H->plan<double>(dir, opk, H->desc.iacc, ibc2, fbc != nullptr, A.e.n[dir], out.e.n[dir], &P));
-  LineIn<double> in;
+template <typename T>
+std::string Flow::op(RhsWork<T>& w, int dir, int opk, const int ibc2[2], const Field<T>& A, Field<T>& out,
+                     const T* fbc, T sA, const Field<T>* B, T cB, const Field<T>* Cm, T sOut, int accumulate) {
+  const DevPlan<T>* P = nullptr;
+  SCHK(H->plan<T>(dir, opk, H->desc.iacc, ibc2, fbc != nullptr, A.e.n[dir], out.e.n[dir], &P));
+  LineIn<T> in;
   in.A = A.p;
   in.B = B ? B->p : nullptr;
   in.Cm = Cm ? Cm->p : nullptr;
   in.sA = sA;
   in.cB = cB;
-  LineOut<double> o;
+  LineOut<T> o;
   o.out = out.p;
   o.sOut = sOut;
   o.accumulate = accumulate;
-  if (out.size() > scratch_n) return "flow: scratch too small";
+  if (out.size() > w.scratch_n) return "flow: scratch too small";
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const bool timed = (H->prof_class == dir) && H->prof_pair(&e0, &e1);
   if (timed) {
     (void)hipEventRecord(e0, H->stream);
     const int nin_arrays = 1 + (B ? 1 : 0) + (Cm ? 1 : 0);
-    H->prof_bytes += 8.0 * ((double)A.size() * nin_arrays + (double)out.size() * (1 + (accumulate ? 1 : 0)));
+    H->prof_bytes += (double)sizeof(T) *
+                     ((double)A.size() * nin_arrays + (double)out.size() * (1 + (accumulate ? 1 : 0)));
   }
-  std::string r = launch_line_op<double>(*P, A.e, out.e, in, fbc, o, scratch, H->stream);
+  std::string r = launch_line_op<T>(*P, A.e, out.e, in, fbc, o, w.scratch, H->stream);
   if (timed) (void)hipEventRecord(e1, H->stream);
   return r;
 }
@@ -402,109 +447,116 @@ std::string Flow::get_state(double* hx, double* hy, double* hz, double* hp, int 
 
 std::string Flow::copy_rhs(double* mx, double* my, double* mz, int mem) {
   hipMemcpyKind k = (mem == CHS_MEM_HOST) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-  if (H->desc.idriven == 1) {
+  const bool forced = (H->desc.idriven == 1);
+  double* out[3] = {mx, my, mz};
+  double* tmp = w64.scratch;
+  for (int c = 0; c < 3; ++c) {
+    const long n = w64.acc[c][0].size();
     // the x right-hand side after the constant-flux correction (:1547-1551)
-    hipLaunchKernelGGL(k_sub_bulk, grid1(acc[0][0].size()), dim3(256), 0, H->stream, scratch, acc[0][0].p, bulk,
-                       acc[0][0].size());
-    FCHK(hipMemcpyAsync(mx, scratch, sizeof(double) * acc[0][0].size(), k, H->stream));
-  } else {
-    FCHK(hipMemcpyAsync(mx, acc[0][0].p, sizeof(double) * acc[0][0].size(), k, H->stream));
+    const double* b = (c == 0 && forced) ? bulk : nullptr;
+    if (mixed)
+      hipLaunchKernelGGL(k_sub_bulk<float>, grid1(n), dim3(256), 0, H->stream, tmp, w32.acc[c][0].p, b, n);
+    else
+      hipLaunchKernelGGL(k_sub_bulk<double>, grid1(n), dim3(256), 0, H->stream, tmp, w64.acc[c][0].p, b, n);
+    FCHK(hipMemcpyAsync(out[c], tmp, sizeof(double) * n, k, H->stream));
+    FCHK(hipStreamSynchronize(H->stream));
   }
-  FCHK(hipMemcpyAsync(my, acc[1][0].p, sizeof(double) * acc[1][0].size(), k, H->stream));
-  FCHK(hipMemcpyAsync(mz, acc[2][0].p, sizeof(double) * acc[2][0].size(), k, H->stream));
-  FCHK(hipStreamSynchronize(H->stream));
   return "";
 }
 
 // Compute_momentum_rhs + Calculate_momentum_fractional_step + forcing.
-// Result: dt-scaled rhs in acc[c][0]; bulk (if forced) in this->bulk.
-std::string Flow::momentum_rhs(int isub) {
+// Result: dt-scaled rhs in w.acc[c][0]; bulk (if forced) in this->bulk.
+template <typename T>
+std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub) {
   hipStream_t s = H->stream;
   const int(*B)[5][2] = ibc;
-  const double* fu = wall_y ? fby_u : nullptr;
-  const double* fv = wall_y ? fby_v : nullptr;
-  const double* fw = wall_y ? fby_w : nullptr;
+  const T* fu = wall_y ? w.fby_u : nullptr;
+  const T* fv = wall_y ? w.fby_v : nullptr;
+  const T* fw = wall_y ? w.fby_w : nullptr;
+  const T one = T(1), m1 = T(-1);
   // ---- first-level operators, src/eq_momentum2.f90:347-506 ----
-  SCHK(op(0, MIDP_P2C, B[0][VU], qx, qxix, nullptr));
-  SCHK(op(0, D1_P2C, B[0][VU], qx, qxdx, nullptr));
-  SCHK(op(1, D1_C2P, B[1][VU], qx, qxdy, fu));
-  SCHK(op(1, MIDP_C2P, B[1][VU], qx, qxiy, fu));
-  SCHK(op(2, MIDP_C2P, B[2][VU], qx, qxiz, nullptr));
-  SCHK(op(2, D1_C2P, B[2][VU], qx, qxdz, nullptr));
-  SCHK(op(0, D1_C2P, B[0][VV], qy, qydx, nullptr));
-  SCHK(op(0, MIDP_C2P, B[0][VV], qy, qyix, nullptr));
-  SCHK(op(1, D1_P2C, B[1][VV], qy, qydy, nullptr));
-  SCHK(op(1, MIDP_P2C, B[1][VV], qy, qyiy, fv));
-  SCHK(op(2, D1_C2P, B[2][VV], qy, qydz, nullptr));
-  SCHK(op(2, MIDP_C2P, B[2][VV], qy, qyiz, nullptr));
-  SCHK(op(0, D1_C2P, B[0][VW], qz, qzdx, nullptr));
-  SCHK(op(0, MIDP_C2P, B[0][VW], qz, qzix, nullptr));
-  SCHK(op(1, MIDP_C2P, B[1][VW], qz, qziy, fw));
-  SCHK(op(1, D1_C2P, B[1][VW], qz, qzdy, fw));
-  SCHK(op(2, MIDP_P2C, B[2][VW], qz, qziz, nullptr));
-  SCHK(op(2, D1_P2C, B[2][VW], qz, qzdz, nullptr));
-  hipLaunchKernelGGL(k_div3, grid1(div.size()), dim3(256), 0, s, div.p, qxdx.p, qydy.p, qzdz.p, div.size());
+  SCHK(op<T>(w, 0, MIDP_P2C, B[0][VU], w.qx, w.qxix, nullptr));
+  SCHK(op<T>(w, 0, D1_P2C, B[0][VU], w.qx, w.qxdx, nullptr));
+  SCHK(op<T>(w, 1, D1_C2P, B[1][VU], w.qx, w.qxdy, fu));
+  SCHK(op<T>(w, 1, MIDP_C2P, B[1][VU], w.qx, w.qxiy, fu));
+  SCHK(op<T>(w, 2, MIDP_C2P, B[2][VU], w.qx, w.qxiz, nullptr));
+  SCHK(op<T>(w, 2, D1_C2P, B[2][VU], w.qx, w.qxdz, nullptr));
+  SCHK(op<T>(w, 0, D1_C2P, B[0][VV], w.qy, w.qydx, nullptr));
+  SCHK(op<T>(w, 0, MIDP_C2P, B[0][VV], w.qy, w.qyix, nullptr));
+  SCHK(op<T>(w, 1, D1_P2C, B[1][VV], w.qy, w.qydy, nullptr));
+  SCHK(op<T>(w, 1, MIDP_P2C, B[1][VV], w.qy, w.qyiy, fv));
+  SCHK(op<T>(w, 2, D1_C2P, B[2][VV], w.qy, w.qydz, nullptr));
+  SCHK(op<T>(w, 2, MIDP_C2P, B[2][VV], w.qy, w.qyiz, nullptr));
+  SCHK(op<T>(w, 0, D1_C2P, B[0][VW], w.qz, w.qzdx, nullptr));
+  SCHK(op<T>(w, 0, MIDP_C2P, B[0][VW], w.qz, w.qzix, nullptr));
+  SCHK(op<T>(w, 1, MIDP_C2P, B[1][VW], w.qz, w.qziy, fw));
+  SCHK(op<T>(w, 1, D1_C2P, B[1][VW], w.qz, w.qzdy, fw));
+  SCHK(op<T>(w, 2, MIDP_P2C, B[2][VW], w.qz, w.qziz, nullptr));
+  SCHK(op<T>(w, 2, D1_P2C, B[2][VW], w.qz, w.qzdz, nullptr));
+  hipLaunchKernelGGL(k_div3<T>, grid1(w.div.size()), dim3(256), 0, s, w.div.p, w.qxdx.p, w.qydy.p, w.qzdz.p,
+                     w.div.size());
   // ---- y-wall values of the divergence and of the y-normal stress ----
   if (wall_y) {
     const long nf2 = (long)nx * nz;
     // qxdx at the walls: x-derivative of the wall planes of qxiy
-    hipLaunchKernelGGL(k_extract_walls, grid1((long)px * nz), dim3(256), 0, s, plane_a, qxiy.p, px, py, nz);
-    Field<double> pa{plane_a, Ext3{{px, 2, nz}}}, pb{plane_b, Ext3{{nx, 2, nz}}};
-    SCHK(op(0, D1_P2C, B[0][VU], pa, pb, nullptr));
+    hipLaunchKernelGGL(k_extract_walls<T>, grid1((long)px * nz), dim3(256), 0, s, w.plane_a, w.qxiy.p, px, py, nz);
+    Field<T> pa{w.plane_a, Ext3{{px, 2, nz}}}, pb{w.plane_b, Ext3{{nx, 2, nz}}};
+    SCHK(op<T>(w, 0, D1_P2C, B[0][VU], pa, pb, nullptr));
     // qydy at the walls: y-derivative C2P of qyiy (Dirichlet rows are decoupled)
     {
-      const DevPlan<double>* P = nullptr;
-      SCHK(H->plan<double>(1, D1_C2P, H->desc.iacc, B[1][VV], true, ny, py, &P));
-      SCHK(launch_wall_rows<double>(*P, qyiy.e, qyiy.p, fby_v, plane_c, s));
+      const DevPlan<T>* P = nullptr;
+      SCHK(H->plan<T>(1, D1_C2P, H->desc.iacc, B[1][VV], true, ny, py, &P));
+      SCHK(launch_wall_rows<T>(*P, w.qyiy.e, w.qyiy.p, w.fby_v, w.plane_c, s));
     }
     // qzdz at the walls: z-derivative of the wall planes of qziy
-    hipLaunchKernelGGL(k_extract_walls, grid1((long)nx * pz), dim3(256), 0, s, plane_a, qziy.p, nx, py, pz);
-    Field<double> pa2{plane_a, Ext3{{nx, 2, pz}}}, pd{plane_d, Ext3{{nx, 2, nz}}};
-    SCHK(op(2, D1_P2C, B[2][VW], pa2, pd, nullptr));
-    hipLaunchKernelGGL(k_wall_fbc, grid1(nf2), dim3(256), 0, s, fby_div, fby_visc, plane_b, plane_c, plane_d, nx,
-                       nz);
+    hipLaunchKernelGGL(k_extract_walls<T>, grid1((long)nx * pz), dim3(256), 0, s, w.plane_a, w.qziy.p, nx, py, pz);
+    Field<T> pa2{w.plane_a, Ext3{{nx, 2, pz}}}, pd{w.plane_d, Ext3{{nx, 2, nz}}};
+    SCHK(op<T>(w, 2, D1_P2C, B[2][VW], pa2, pd, nullptr));
+    hipLaunchKernelGGL(k_wall_fbc<T>, grid1(nf2), dim3(256), 0, s, w.fby_div, w.fby_visc, w.plane_b, w.plane_c,
+                       w.plane_d, nx, nz);
   }
-  const double r = rre;
+  const T r = T(rre), third = T(-ONE_THIRD);
   // ---- x-momentum, :798-948 ----
-  Field<double>* X = acc[0];
-  SCHK(op(0, D1_C2P, mcov[0][0], qxix, X[0], nullptr, -1.0, nullptr, 1.0, &qxix));
-  SCHK(op(1, D1_P2C, mcov[0][1], qyix, X[1], nullptr, -1.0, nullptr, 1.0, &qxiy));
-  SCHK(op(2, D1_P2C, mcov[0][2], qzix, X[2], nullptr, -1.0, nullptr, 1.0, &qxiz));
-  SCHK(op(0, D1_C2P, B[0][VP], pr, X[3], nullptr, -1.0));
-  SCHK(op(0, D1_C2P, mtau[0][0], qxdx, X[0], nullptr, 2.0 * 1.0, &div, -ONE_THIRD, nullptr, r, 1));
-  SCHK(op(1, D1_P2C, mtau[0][1], qxdy, X[1], nullptr, 1.0, &qydx, 1.0, nullptr, r, 1));
-  SCHK(op(2, D1_P2C, mtau[0][2], qxdz, X[2], nullptr, 1.0, &qzdx, 1.0, nullptr, r, 1));
+  Field<T>* X = w.acc[0];
+  SCHK(op<T>(w, 0, D1_C2P, mcov[0][0], w.qxix, X[0], nullptr, m1, nullptr, one, &w.qxix));
+  SCHK(op<T>(w, 1, D1_P2C, mcov[0][1], w.qyix, X[1], nullptr, m1, nullptr, one, &w.qxiy));
+  SCHK(op<T>(w, 2, D1_P2C, mcov[0][2], w.qzix, X[2], nullptr, m1, nullptr, one, &w.qxiz));
+  SCHK(op<T>(w, 0, D1_C2P, B[0][VP], w.pr, X[3], nullptr, m1));
+  SCHK(op<T>(w, 0, D1_C2P, mtau[0][0], w.qxdx, X[0], nullptr, T(2), &w.div, third, nullptr, r, 1));
+  SCHK(op<T>(w, 1, D1_P2C, mtau[0][1], w.qxdy, X[1], nullptr, one, &w.qydx, one, nullptr, r, 1));
+  SCHK(op<T>(w, 2, D1_P2C, mtau[0][2], w.qxdz, X[2], nullptr, one, &w.qzdx, one, nullptr, r, 1));
   // ---- y-momentum, :960-1235 ----
-  Field<double>* Y = acc[1];
-  SCHK(op(0, D1_P2C, mcov[1][0], qxiy, Y[0], nullptr, -1.0, nullptr, 1.0, &qyix));
-  SCHK(op(1, D1_C2P, mcov[1][1], qyiy, Y[1], wall_y ? fby_conv : nullptr, -1.0, nullptr, 1.0, &qyiy));
-  SCHK(op(2, D1_P2C, mcov[1][2], qyiz, Y[2], nullptr, -1.0, nullptr, 1.0, &qziy));
-  SCHK(op(1, D1_C2P, B[1][VP], pr, Y[3], wall_y ? fby_pn : nullptr, -1.0));
-  SCHK(op(0, D1_P2C, mtau[1][0], qxdy, Y[0], nullptr, 1.0, &qydx, 1.0, nullptr, r, 1));
-  SCHK(op(1, D1_C2P, mtau[1][1], qydy, Y[1], wall_y ? fby_visc : nullptr, 2.0, &div, -ONE_THIRD, nullptr, r, 1));
-  SCHK(op(2, D1_P2C, mtau[1][2], qzdy, Y[2], nullptr, 1.0, &qydz, 1.0, nullptr, r, 1));
+  Field<T>* Y = w.acc[1];
+  SCHK(op<T>(w, 0, D1_P2C, mcov[1][0], w.qxiy, Y[0], nullptr, m1, nullptr, one, &w.qyix));
+  SCHK(op<T>(w, 1, D1_C2P, mcov[1][1], w.qyiy, Y[1], wall_y ? w.fby_conv : nullptr, m1, nullptr, one, &w.qyiy));
+  SCHK(op<T>(w, 2, D1_P2C, mcov[1][2], w.qyiz, Y[2], nullptr, m1, nullptr, one, &w.qziy));
+  SCHK(op<T>(w, 1, D1_C2P, B[1][VP], w.pr, Y[3], wall_y ? w.fby_pn : nullptr, m1));
+  SCHK(op<T>(w, 0, D1_P2C, mtau[1][0], w.qxdy, Y[0], nullptr, one, &w.qydx, one, nullptr, r, 1));
+  SCHK(op<T>(w, 1, D1_C2P, mtau[1][1], w.qydy, Y[1], wall_y ? w.fby_visc : nullptr, T(2), &w.div, third, nullptr, r,
+             1));
+  SCHK(op<T>(w, 2, D1_P2C, mtau[1][2], w.qzdy, Y[2], nullptr, one, &w.qydz, one, nullptr, r, 1));
   // ---- z-momentum, :1248-1480 ----
-  Field<double>* Z = acc[2];
-  SCHK(op(0, D1_P2C, mcov[2][0], qxiz, Z[0], nullptr, -1.0, nullptr, 1.0, &qzix));
-  SCHK(op(1, D1_P2C, mcov[2][1], qyiz, Z[1], nullptr, -1.0, nullptr, 1.0, &qziy));
-  SCHK(op(2, D1_C2P, mcov[2][2], qziz, Z[2], nullptr, -1.0, nullptr, 1.0, &qziz));
-  SCHK(op(2, D1_C2P, B[2][VP], pr, Z[3], nullptr, -1.0));
-  SCHK(op(0, D1_P2C, mtau[2][0], qxdz, Z[0], nullptr, 1.0, &qzdx, 1.0, nullptr, r, 1));
-  SCHK(op(1, D1_P2C, mtau[2][1], qzdy, Z[1], nullptr, 1.0, &qydz, 1.0, nullptr, r, 1));
-  SCHK(op(2, D1_C2P, mtau[2][2], qzdz, Z[2], nullptr, 2.0, &div, -ONE_THIRD, nullptr, r, 1));
+  Field<T>* Z = w.acc[2];
+  SCHK(op<T>(w, 0, D1_P2C, mcov[2][0], w.qxiz, Z[0], nullptr, m1, nullptr, one, &w.qzix));
+  SCHK(op<T>(w, 1, D1_P2C, mcov[2][1], w.qyiz, Z[1], nullptr, m1, nullptr, one, &w.qziy));
+  SCHK(op<T>(w, 2, D1_C2P, mcov[2][2], w.qziz, Z[2], nullptr, m1, nullptr, one, &w.qziz));
+  SCHK(op<T>(w, 2, D1_C2P, B[2][VP], w.pr, Z[3], nullptr, m1));
+  SCHK(op<T>(w, 0, D1_P2C, mtau[2][0], w.qxdz, Z[0], nullptr, one, &w.qzdx, one, nullptr, r, 1));
+  SCHK(op<T>(w, 1, D1_P2C, mtau[2][1], w.qzdy, Z[1], nullptr, one, &w.qydz, one, nullptr, r, 1));
+  SCHK(op<T>(w, 2, D1_C2P, mtau[2][2], w.qzdz, Z[2], nullptr, T(2), &w.div, third, nullptr, r, 1));
   // ---- fractional step and forcing ----
   const Mesh& m = H->mesh;
   const bool forced = (H->desc.idriven == 1);
-  Field<double>* rhs0[3] = {&mx0, &my0, &mz0};
+  Field<T>* rhs0[3] = {&w.mx0, &w.my0, &w.mz0};
   for (int c = 0; c < 3; ++c) {
-    long n = acc[c][0].size();
+    long n = w.acc[c][0].size();
     double* part = (c == 0 && forced) ? partial : nullptr;
-    hipLaunchKernelGGL(k_combine, grid1(n), dim3(256), 0, s, acc[c][0].p, acc[c][1].p, acc[c][2].p, acc[c][3].p,
-                       rhs0[c]->p, n, tg[isub], tz[isub], ta[isub], dt, acc[c][0].e.n[0], acc[c][0].e.n[1],
-                       dyw_dev, m.h[0], m.h[2], part);
+    hipLaunchKernelGGL(k_combine<T>, grid1(n), dim3(256), 0, s, w.acc[c][0].p, w.acc[c][1].p, w.acc[c][2].p,
+                       w.acc[c][3].p, rhs0[c]->p, n, T(tg[isub]), T(tz[isub]), T(ta[isub]), T(dt),
+                       w.acc[c][0].e.n[0], w.acc[c][0].e.n[1], dyw_dev, m.h[0], m.h[2], part);
   }
   if (forced) {
-    int nb = (int)((acc[0][0].size() + 255) / 256);
+    int nb = (int)((w.acc[0][0].size() + 255) / 256);
     hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, s, partial, nb, vol, bulk);
   }
   hipError_t e = hipGetLastError();
@@ -512,24 +564,43 @@ std::string Flow::momentum_rhs(int isub) {
   return "";
 }
 
+std::string Flow::momentum_rhs(int isub) {
+  if (!mixed) return momentum_rhs_t<double>(w64, isub);
+  // rounded copies of the state for the FP32 right-hand side
+  hipStream_t s = H->stream;
+  hipLaunchKernelGGL((k_cvt<float, double>), grid1(qx.size()), dim3(256), 0, s, w32.qx.p, qx.p, qx.size());
+  hipLaunchKernelGGL((k_cvt<float, double>), grid1(qy.size()), dim3(256), 0, s, w32.qy.p, qy.p, qy.size());
+  hipLaunchKernelGGL((k_cvt<float, double>), grid1(qz.size()), dim3(256), 0, s, w32.qz.p, qz.p, qz.size());
+  hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr.size()), dim3(256), 0, s, w32.pr.p, pr.p, pr.size());
+  return momentum_rhs_t<float>(w32, isub);
+}
+
+template <typename T>
+std::string Flow::update_state(RhsWork<T>& w) {
+  hipStream_t s = H->stream;
+  const bool forced = (H->desc.idriven == 1);
+  hipLaunchKernelGGL(k_update<T>, grid1(qx.size()), dim3(256), 0, s, qx.p, w.acc[0][0].p, forced ? bulk : nullptr,
+                     qx.size(), px, ny, (const double*)nullptr);
+  hipLaunchKernelGGL(k_update<T>, grid1(qy.size()), dim3(256), 0, s, qy.p, w.acc[1][0].p, (const double*)nullptr,
+                     qy.size(), nx, py, wall_y ? w64.fby_v : nullptr);
+  hipLaunchKernelGGL(k_update<T>, grid1(qz.size()), dim3(256), 0, s, qz.p, w.acc[2][0].p, (const double*)nullptr,
+                     qz.size(), nx, ny, (const double*)nullptr);
+  return "";
+}
+
 std::string Flow::divergence(double* d) {
   Field<double> D{d, Ext3{{nx, ny, nz}}};
-  SCHK(op(0, D1_P2C, ibc[0][VU], qx, D, nullptr));
-  SCHK(op(1, D1_P2C, ibc[1][VV], qy, D, nullptr, 1.0, nullptr, 1.0, nullptr, 1.0, 1));
-  SCHK(op(2, D1_P2C, ibc[2][VW], qz, D, nullptr, 1.0, nullptr, 1.0, nullptr, 1.0, 1));
+  SCHK(op<double>(w64, 0, D1_P2C, ibc[0][VU], qx, D, nullptr));
+  SCHK(op<double>(w64, 1, D1_P2C, ibc[1][VV], qy, D, nullptr, 1.0, nullptr, 1.0, nullptr, 1.0, 1));
+  SCHK(op<double>(w64, 2, D1_P2C, ibc[2][VW], qz, D, nullptr, 1.0, nullptr, 1.0, nullptr, 1.0, 1));
   return "";
 }
 
 std::string Flow::substep(int isub) {
   hipStream_t s = H->stream;
   SCHK(momentum_rhs(isub));
-  const bool forced = (H->desc.idriven == 1);
-  hipLaunchKernelGGL(k_update, grid1(qx.size()), dim3(256), 0, s, qx.p, acc[0][0].p, forced ? bulk : nullptr,
-                     qx.size(), px, ny, (const double*)nullptr);
-  hipLaunchKernelGGL(k_update, grid1(qy.size()), dim3(256), 0, s, qy.p, acc[1][0].p, (const double*)nullptr,
-                     qy.size(), nx, py, wall_y ? fby_v : nullptr);
-  hipLaunchKernelGGL(k_update, grid1(qz.size()), dim3(256), 0, s, qz.p, acc[2][0].p, (const double*)nullptr,
-                     qz.size(), nx, ny, (const double*)nullptr);
+  if (mixed) SCHK(update_state<float>(w32));
+  else SCHK(update_state<double>(w64));
   // Poisson RHS: div(q) / (alpha * sigma2 * dt), then the solve
   SCHK(divergence(pcor.p));
   const double coeff = 1.0 / (ta[isub] * sigma2p * dt);
@@ -538,11 +609,12 @@ std::string Flow::substep(int isub) {
   hipLaunchKernelGGL(k_add, grid1(pr.size()), dim3(256), 0, s, pr.p, pcor.p, pr.size());
   // Correct_massflux
   const double sc = dt * ta[isub] * sigma2p;
-  SCHK(op(0, D1_C2P, ibc[0][VP], pcor, qx, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1));
-  SCHK(op(1, D1_C2P, ibc[1][VP], pcor, qy, wall_y ? fby_p : nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1));
-  SCHK(op(2, D1_C2P, ibc[2][VP], pcor, qz, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1));
+  SCHK(op<double>(w64, 0, D1_C2P, ibc[0][VP], pcor, qx, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1));
+  SCHK(op<double>(w64, 1, D1_C2P, ibc[1][VP], pcor, qy, wall_y ? w64.fby_p : nullptr, 1.0, nullptr, 1.0, nullptr,
+                  -sc, 1));
+  SCHK(op<double>(w64, 2, D1_C2P, ibc[2][VP], pcor, qz, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1));
   if (wall_y)
-    hipLaunchKernelGGL(k_enforce_walls, grid1((long)nx * nz), dim3(256), 0, s, qy.p, nx, py, nz, fby_v);
+    hipLaunchKernelGGL(k_enforce_walls, grid1((long)nx * nz), dim3(256), 0, s, qy.p, nx, py, nz, w64.fby_v);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("substep: ") + hipGetErrorString(e);
   return "";
