@@ -17,7 +17,10 @@ EXPORTS = [
     "chs_abi_version", "chs_last_error", "chs_create", "chs_destroy", "chs_sync", "chs_op3d",
     "chs_poisson", "chs_set_state", "chs_get_state", "chs_update_re", "chs_substep", "chs_run",
     "chs_momentum_rhs", "chs_mass_check", "chs_divergence",
-    "chs_props_create", "chs_props_destroy", "chs_props_info", "chs_props_from_rhoh",
+    "chs_batch_create", "chs_batch_destroy", "chs_batch_instance", "chs_batch_run",
+    "chs_props_create", "chs_props_destroy", "chs_props_info", "chs_props_from_rhoh", "chs_props_from_h",
+    "chs_mlp_create", "chs_mlp_destroy", "chs_mlp_info", "chs_mlp_get_params", "chs_mlp_set_params",
+    "chs_mlp_grad", "chs_mlp_train", "chs_mlp_eval",
     "chs_inspect_coeff", "chs_inspect_op1d", "chs_inspect_projection", "chs_profile_begin", "chs_profile_end",
 ]
 
@@ -82,17 +85,32 @@ def load():
     L.chs_divergence.argtypes = [vp, vp, C.c_int]
     L.chs_inspect_coeff.restype = C.c_double
     L.chs_inspect_coeff.argtypes = [C.c_int] * 6
+    L.chs_batch_create.argtypes = [C.POINTER(ChsDesc), C.c_int, C.POINTER(vp)]
+    L.chs_batch_destroy.argtypes = [vp]
+    L.chs_batch_instance.argtypes = [vp, C.c_int]
+    L.chs_batch_instance.restype = vp
+    L.chs_batch_run.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.chs_props_create.argtypes = [dp, C.c_int, C.c_double, C.POINTER(vp)]
     L.chs_props_destroy.argtypes = [vp]
     L.chs_props_info.argtypes = [vp, dp, dp, dp]
     L.chs_props_from_rhoh.argtypes = [vp, vp, C.c_long] + [vp] * 7 + [C.c_int, C.c_int]
+    L.chs_props_from_h.argtypes = [vp, vp, C.c_long] + [vp] * 6 + [C.c_int]
+    L.chs_mlp_create.argtypes = [vp, C.c_int, C.c_uint, C.POINTER(vp)]
+    L.chs_mlp_destroy.argtypes = [vp]
+    L.chs_mlp_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_float)]
+    L.chs_mlp_get_params.argtypes = [vp, vp, vp]
+    L.chs_mlp_set_params.argtypes = [vp, vp]
+    L.chs_mlp_grad.argtypes = [vp, vp, vp, C.c_int, vp, C.POINTER(C.c_float)]
+    L.chs_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float)]
+    L.chs_mlp_eval.argtypes = [vp, vp, C.c_long] + [vp] * 5 + [C.c_int]
     L.chs_inspect_op1d.argtypes = [C.POINTER(ChsDesc), C.c_int, C.c_int, C.c_int, ip, dp, C.c_int, dp,
                                    C.c_int, dp]
     L.chs_inspect_projection.argtypes = [C.POINTER(ChsDesc), dp, dp, dp, dp, dp]
     L.chs_profile_begin.argtypes = [vp, C.c_int]
     L.chs_profile_end.argtypes = [vp, dp, C.POINTER(C.c_longlong), dp]
     for name in EXPORTS:
-        getattr(L, name).restype = getattr(L, name).restype or C.c_int
+        if name != "chs_batch_instance":
+            getattr(L, name).restype = getattr(L, name).restype or C.c_int
     L.chs_last_error.restype = C.c_char_p
     L.chs_inspect_coeff.restype = C.c_double
     _lib = L

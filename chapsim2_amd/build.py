@@ -26,6 +26,7 @@ SOURCES = [
     "chs_flow.hip",
     "chs_api.hip",
     "chs_props.hip",
+    "chs_mlp.hip",
 ]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

@@ -301,6 +301,132 @@ int chs_run(chs_handle_t* H, int iter0, int nsteps) {
   return CHS_OK;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched channels: independent instances advanced together, each on its own
+// stream, optionally as one captured graph per RK3 step (fork from a main
+// stream to every instance stream, join back)
+struct chs_batch {
+  std::vector<chs_handle_t*> inst;
+  hipStream_t main = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<double> rre_captured;
+  bool warmed = false;
+  ~chs_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t e : join) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (main) (void)hipStreamDestroy(main);
+    for (chs_handle_t* h : inst) chs_destroy(h);
+  }
+};
+
+static std::string batch_capture(chs_batch* b) {
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  if (hipStreamBeginCapture(b->main, hipStreamCaptureModeRelaxed) != hipSuccess) return "begin capture";
+  std::string err;
+  if (hipEventRecord(b->fork, b->main) != hipSuccess) err = "fork record";
+  for (size_t i = 0; i < b->inst.size() && err.empty(); ++i) {
+    chs::Handle& h = b->inst[i]->h;
+    if (hipStreamWaitEvent(h.stream, b->fork, 0) != hipSuccess) { err = "fork wait"; break; }
+    for (int isub = 1; isub <= 3 && err.empty(); ++isub) err = h.flow->substep(isub);
+    if (err.empty() && hipEventRecord(b->join[i], h.stream) != hipSuccess) err = "join record";
+    if (err.empty() && hipStreamWaitEvent(b->main, b->join[i], 0) != hipSuccess) err = "join wait";
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(b->main, &g);
+  if (!err.empty()) {
+    if (g) (void)hipGraphDestroy(g);
+    return "capture: " + err;
+  }
+  if (ec != hipSuccess || !g) return std::string("end capture: ") + hipGetErrorString(ec);
+  b->graph = g;
+  if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) return "graph instantiate";
+  b->rre_captured.resize(b->inst.size());
+  for (size_t i = 0; i < b->inst.size(); ++i) b->rre_captured[i] = b->inst[i]->h.flow->rre;
+  return "";
+}
+
+extern "C" {
+
+int chs_batch_create(const chs_desc_t* descs, int n, chs_batch_t** out) {
+  if (!descs || !out || n < 1) return fail(CHS_EINVAL, "invalid argument");
+  *out = nullptr;
+  chs_batch* b = new chs_batch();
+  for (int i = 0; i < n; ++i) {
+    chs_handle_t* h = nullptr;
+    const int rc = chs_create(&descs[i], &h);
+    if (rc != CHS_OK) {
+      const std::string msg = chs::g_err;
+      delete b;
+      return fail(rc, "instance " + std::to_string(i) + ": " + msg);
+    }
+    b->inst.push_back(h);
+    if (!h->h.flow) {
+      delete b;
+      return fail(CHS_EUNSUP, "instance " + std::to_string(i) + ": no flow solver for this configuration");
+    }
+  }
+  b->join.resize(n, nullptr);
+  bool ok = hipStreamCreate(&b->main) == hipSuccess &&
+            hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; i < n && ok; ++i) ok = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    delete b;
+    return fail(CHS_EHIP, "stream / event creation failed");
+  }
+  *out = b;
+  return CHS_OK;
+}
+
+int chs_batch_destroy(chs_batch_t* b) {
+  delete b;
+  return CHS_OK;
+}
+
+chs_handle_t* chs_batch_instance(chs_batch_t* b, int i) {
+  if (!b || i < 0 || i >= (int)b->inst.size()) return nullptr;
+  return b->inst[i];
+}
+
+int chs_batch_run(chs_batch_t* b, int iter0, int nsteps, int use_graph) {
+  if (!b || nsteps < 0) return fail(CHS_EINVAL, "invalid argument");
+  for (chs_handle_t* h : b->inst)
+    if (h->h.prof_class >= 0 && use_graph) return fail(CHS_EINVAL, "launch profiling is active on an instance");
+  for (int it = iter0 + 1; it <= iter0 + nsteps; ++it) {
+    bool recapture = !b->exec;
+    for (size_t i = 0; i < b->inst.size(); ++i) {
+      chs::Flow& F = *b->inst[i]->h.flow;
+      F.update_re(it);
+      if (b->exec && F.rre != b->rre_captured[i]) recapture = true;
+    }
+    if (!use_graph || !b->warmed) {
+      // eager step: every instance on its own stream (plans are created here)
+      for (chs_handle_t* h : b->inst)
+        for (int isub = 1; isub <= 3; ++isub) RET(h->h.flow->substep(isub));
+      b->warmed = true;
+      continue;
+    }
+    if (recapture) {
+      for (chs_handle_t* h : b->inst)
+        if (hipStreamSynchronize(h->h.stream) != hipSuccess) return fail(CHS_EHIP, "synchronise");
+      const std::string e = batch_capture(b);
+      if (!e.empty()) return fail(CHS_EHIP, e);
+    }
+    if (hipGraphLaunch(b->exec, b->main) != hipSuccess) return fail(CHS_EHIP, "graph launch");
+  }
+  if (hipStreamSynchronize(b->main) != hipSuccess) return fail(CHS_EHIP, "synchronise");
+  for (chs_handle_t* h : b->inst)
+    if (hipStreamSynchronize(h->h.stream) != hipSuccess) return fail(CHS_EHIP, "synchronise");
+  return CHS_OK;
+}
+
 int chs_momentum_rhs(chs_handle_t* H, int isub, double* mx, double* my, double* mz, int mem) {
   FLOW_OR_FAIL(H);
   if (isub < 0 || isub > 3) return fail(CHS_EINVAL, "isub must be 0..3");

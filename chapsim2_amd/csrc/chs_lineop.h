@@ -39,6 +39,11 @@ struct DevPlan {
   T* sv = nullptr;
   T* sw = nullptr;
   T* rinv = nullptr;
+  // boundary rows 0, 1 (first chunk) and n-2, n-1 (last chunk) as dense
+  // weights over the chunk's register window of kChunk + 4 inputs
+  // (win_ok false when a recipe reaches outside that window)
+  T* swin = nullptr;
+  bool win_ok = false;
   ~DevPlan();
 };
 

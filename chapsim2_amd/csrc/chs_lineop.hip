@@ -47,7 +47,7 @@ namespace chs {
 template <typename T>
 DevPlan<T>::~DevPlan() {
   for (void* p : {(void*)rows, (void*)a, (void*)c, (void*)d, (void*)pw, (void*)x1, (void*)metric, (void*)ld,
-                  (void*)lc, (void*)ar, (void*)sv, (void*)sw, (void*)rinv})
+                  (void*)lc, (void*)ar, (void*)sv, (void*)sw, (void*)rinv, (void*)swin})
     if (p) (void)hipFree(p);
 }
 
@@ -87,6 +87,22 @@ std::string upload_plan(const LinePlan& p, DevPlan<T>* dp) {
   if (!(e = up(p.sv, &dp->sv)).empty()) return e;
   if (!(e = up(p.sw, &dp->sw)).empty()) return e;
   if (!(e = up(p.rinv, &dp->rinv)).empty()) return e;
+  // dense window weights of the four boundary rows
+  {
+    const int W = kChunk + 4;
+    std::vector<double> sw(4 * W, 0.0);
+    bool ok = p.part_P >= 1 && (int)p.part_start.size() == p.part_P + 1;
+    for (int q = 0; q < 4 && ok; ++q) {
+      const int s0 = (q < 2) ? 0 : p.part_start[p.part_P - 1];
+      for (int t = 0; t < p.rows[q].nterm; ++t) {
+        const int slot = p.rows[q].idx[t] - (s0 - 2);
+        if (slot < 0 || slot >= W) { ok = false; break; }
+        sw[q * W + slot] += p.rows[q].w[t];
+      }
+    }
+    dp->win_ok = ok;
+    if (ok && !(e = up(sw, &dp->swin)).empty()) return e;
+  }
   return "";
 }
 
@@ -111,6 +127,7 @@ struct KArgs {
   // partitioned form
   int P, per, ls, nl;
   const T* ld; const T* lc; const T* ar; const T* sv; const T* sw; const T* rinv;
+  const T* swin;  // 4 x (kChunk + 4) boundary-row window weights, or null
 };
 
 // bulk offsets (p1,p2,p3,p4) per operator kind, relative to the output row
@@ -226,16 +243,15 @@ __global__ void __launch_bounds__(64) k_line(KArgs<T> P) {
 // ---------------------------------------------------------------------------
 // partitioned kernels
 
-// Chunk rows [s0, s0+len) of one line: explicit RHS into y[].  FW(t) returns
-// the transformed input at line index s0-2+t from the fast window source,
-// FA(k) at an arbitrary line index k (boundary recipes).
-template <typename T, int OPK, class FW, class FA>
-__device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int s0, int len, FW&& Fw, FA&& Fa,
+// Chunk rows [s0, s0+len) of one line: explicit RHS into y[] from the
+// window f[t] = input at line index s0-2+t (clamped to the line).  The four
+// boundary rows use their dense window weights; FA(k) (input at an arbitrary
+// line index) is only used when a boundary recipe reaches outside the window.
+template <typename T, int OPK, class FA>
+__device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int len, const T (&f)[kChunk + 4], FA&& Fa,
                                           const T (&fb4)[4], T (&y)[kChunk]) {
   using O = Offs<OPK>;
-  T f[kChunk + 4];
-#pragma unroll
-  for (int t = 0; t < kChunk + 4; ++t) f[t] = (t < len + 4) ? Fw(t) : T(0);
+  constexpr int W = kChunk + 4;
 #pragma unroll
   for (int i = 0; i < kChunk; ++i)
     y[i] = (P.c1 * (f[i + 2 + O::p1] + P.sgn * f[i + 2 + O::p2]) +
@@ -243,7 +259,13 @@ __device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int s0, int 
   auto special = [&](int rq) -> T {
     const DevRow<T>& R = P.rows[rq];
     T v = T(0);
-    for (int t = 0; t < R.nterm; ++t) v = v + R.w[t] * Fa(R.idx[t]);
+    if (P.swin) {
+      const T* wv = P.swin + rq * W;
+#pragma unroll
+      for (int t = 0; t < W; ++t) v = v + wv[t] * f[t];
+    } else {
+      for (int t = 0; t < R.nterm; ++t) v = v + R.w[t] * Fa(R.idx[t]);
+    }
     if (P.fbc)
       for (int m = 0; m < 4; ++m) v = v + R.u[m] * fb4[m];
     return v * P.dd;
@@ -259,6 +281,40 @@ __device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int s0, int 
       if (i == len - 2) y[i] = v2;
       if (i == len - 1) y[i] = v3;
     }
+  }
+}
+
+// Window of one y/z line chunk from global memory: every load is issued
+// before any use (indices clamped into the line, so all addresses are
+// valid), then the input transform sA * (A + cB * B) [* Cm] is applied.
+template <typename T>
+__device__ __forceinline__ void load_window(const KArgs<T>& P, long ib, long is, int s0, T (&f)[kChunk + 4]) {
+  constexpr int W = kChunk + 4;
+  const int nmax = P.n_in - 1;
+  int o[W];  // element offsets (arrays hold fewer than 2^31 elements, checked at launch)
+#pragma unroll
+  for (int t = 0; t < W; ++t) {
+    int k = s0 - 2 + t;
+    k = k < 0 ? 0 : (k > nmax ? nmax : k);
+    o[t] = (int)(ib + is * k);
+  }
+#pragma unroll
+  for (int t = 0; t < W; ++t) f[t] = P.A[o[t]];
+  if (P.B) {
+    T g[W];
+#pragma unroll
+    for (int t = 0; t < W; ++t) g[t] = P.B[o[t]];
+#pragma unroll
+    for (int t = 0; t < W; ++t) f[t] = f[t] + P.cB * g[t];
+  }
+#pragma unroll
+  for (int t = 0; t < W; ++t) f[t] = P.sA * f[t];
+  if (P.Cm) {
+    T g[W];
+#pragma unroll
+    for (int t = 0; t < W; ++t) g[t] = P.Cm[o[t]];
+#pragma unroll
+    for (int t = 0; t < W; ++t) f[t] = f[t] * g[t];
   }
 }
 
@@ -337,16 +393,13 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_yz(KArgs<T> P) {
     if (P.Cm) t = t * P.Cm[o];
     return t;
   };
-  auto Fw = [&](int t) -> T {
-    int k = s0 - 2 + t;
-    k = k < 0 ? 0 : (k >= n_in ? n_in - 1 : k);
-    return Fa(k);
-  };
+  T f[kChunk + 4];
+  load_window<T>(P, ib, is, s0, f);
   T fb4[4] = {T(0), T(0), T(0), T(0)};
   if (P.fbc && (q == 0 || q == Pn - 1))
     for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[fb + fs * m];
   T y[kChunk];
-  chunk_rhs<T, OPK>(P, q, s0, len, Fw, Fa, fb4, y);
+  chunk_rhs<T, OPK>(P, q, len, f, Fa, fb4, y);
   if (P.solve) {
     T yf, yl;
     chunk_solve<T>(P, s0, len, y, yf, yl);
@@ -356,14 +409,22 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_yz(KArgs<T> P) {
     chunk_couple<T>(P, q, s0, len, [&](int k) { return ex[lane + 64 * k]; }, y);
   }
   if (!live) return;
+  T prev[kChunk];
+#pragma unroll
+  for (int i = 0; i < kChunk; ++i) prev[i] = T(0);
+  if (P.accumulate) {
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) {
+      const int r = (s0 + i < n_out) ? s0 + i : n_out - 1;
+      prev[i] = P.out[ob + os * r];
+    }
+  }
 #pragma unroll
   for (int i = 0; i < kChunk; ++i) {
     if (i < len) {
       const int r = s0 + i;
       const T val = chunk_post<T>(P, r, y[i], fb4);
-      const long o = ob + os * r;
-      const T prev = P.accumulate ? P.out[o] : T(0);
-      P.out[o] = prev + val * P.sOut;
+      P.out[ob + os * r] = prev[i] + val * P.sOut;
     }
   }
 }
@@ -402,16 +463,18 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
   const T* row = tile + (long)ll * LS;
   const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
   auto Fa = [&](int k) -> T { return row[lds_pos(k)]; };
-  auto Fw = [&](int t) -> T {
+  T f[kChunk + 4];
+#pragma unroll
+  for (int t = 0; t < kChunk + 4; ++t) {
     int k = s0 - 2 + t;
     k = k < 0 ? 0 : (k >= n_in ? n_in - 1 : k);
-    return row[lds_pos(k)];
-  };
+    f[t] = row[lds_pos(k)];
+  }
   T fb4[4] = {T(0), T(0), T(0), T(0)};
   if (P.fbc && live && (q == 0 || q == Pn - 1))
     for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
   T y[kChunk];
-  chunk_rhs<T, OPK>(P, q, s0, len, Fw, Fa, fb4, y);
+  chunk_rhs<T, OPK>(P, q, len, f, Fa, fb4, y);
   if (P.solve) {
     T yf, yl;
     chunk_solve<T>(P, s0, len, y, yf, yl);
@@ -453,18 +516,48 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
   const int q = __builtin_amdgcn_readfirstlane(tid >> 6);
   const long L0 = (long)blockIdx.x * 64;
   const int n_in = P.n_in, n_out = P.n_out;
-  for (int e = tid; e < 64 * n_in; e += nthr) {
-    const int l = e / n_in, k = e - l * n_in;
-    const long L = L0 + l;
-    T v = T(0);
-    if (L < P.nlines) {
-      const long o = L * P.e1 + k;
-      v = P.A[o];
-      if (P.B) v = v + P.cB * P.B[o];
-      v = P.sA * v;
-      if (P.Cm) v = v * P.Cm[o];
+  // staging: batches of UB elements per thread, all loads of a batch issued
+  // before any use (line indices clamped so every address is valid)
+  constexpr int UB = 8;
+  const long Lmax = P.nlines - 1;
+  const int tin = 64 * n_in;
+  for (int b0 = 0; b0 < tin; b0 += nthr * UB) {
+    long o[UB];
+    T v[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      int e = b0 + tid + u * nthr;
+      e = e < tin ? e : tin - 1;
+      const int l = e / n_in, k = e - l * n_in;
+      const long L = (L0 + l) < Lmax ? (L0 + l) : Lmax;
+      o[u] = L * P.e1 + k;
     }
-    tile[l * LS + k] = v;
+#pragma unroll
+    for (int u = 0; u < UB; ++u) v[u] = P.A[o[u]];
+    if (P.B) {
+      T g[UB];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) g[u] = P.B[o[u]];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) v[u] = v[u] + P.cB * g[u];
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) v[u] = P.sA * v[u];
+    if (P.Cm) {
+      T g[UB];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) g[u] = P.Cm[o[u]];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) v[u] = v[u] * g[u];
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const int e = b0 + tid + u * nthr;
+      if (e < tin) {
+        const int l = e / n_in, k = e - l * n_in;
+        tile[l * LS + k] = v[u];
+      }
+    }
   }
   __syncthreads();
   const long L = L0 + lane;
@@ -472,16 +565,18 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
   T* row = tile + lane * LS;
   const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
   auto Fa = [&](int k) -> T { return row[k]; };
-  auto Fw = [&](int t) -> T {
+  T f[kChunk + 4];
+#pragma unroll
+  for (int t = 0; t < kChunk + 4; ++t) {
     int k = s0 - 2 + t;
     k = k < 0 ? 0 : (k >= n_in ? n_in - 1 : k);
-    return row[k];
-  };
+    f[t] = row[k];
+  }
   T fb4[4] = {T(0), T(0), T(0), T(0)};
   if (P.fbc && live && (q == 0 || q == Pn - 1))
     for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
   T y[kChunk];
-  chunk_rhs<T, OPK>(P, q, s0, len, Fw, Fa, fb4, y);
+  chunk_rhs<T, OPK>(P, q, len, f, Fa, fb4, y);
   if (P.solve) {
     T yf, yl;
     chunk_solve<T>(P, s0, len, y, yf, yl);
@@ -495,14 +590,29 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
   for (int i = 0; i < kChunk; ++i)
     if (i < len) row[s0 + i] = chunk_post<T>(P, s0 + i, y[i], fb4);
   __syncthreads();
-  for (int e = tid; e < 64 * n_out; e += nthr) {
-    const int l = e / n_out, k = e - l * n_out;
-    const long Lw = L0 + l;
-    if (Lw >= P.nlines) continue;
-    const long o = Lw * P.o1 + k;
-    const T val = tile[l * LS + k];
-    const T prev = P.accumulate ? P.out[o] : T(0);
-    P.out[o] = prev + val * P.sOut;
+  const int tout = 64 * n_out;
+  for (int b0 = 0; b0 < tout; b0 += nthr * UB) {
+    long o[UB];
+    T val[UB], prev[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      int e = b0 + tid + u * nthr;
+      e = e < tout ? e : tout - 1;
+      const int l = e / n_out, k = e - l * n_out;
+      const long Lw = (L0 + l) < Lmax ? (L0 + l) : Lmax;
+      o[u] = Lw * P.o1 + k;
+      val[u] = tile[l * LS + k];
+      prev[u] = T(0);
+    }
+    if (P.accumulate) {
+#pragma unroll
+      for (int u = 0; u < UB; ++u) prev[u] = P.out[o[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const int e = b0 + tid + u * nthr;
+      if (e < tout && L0 + e / n_out <= Lmax) P.out[o[u]] = prev[u] + val[u] * P.sOut;
+    }
   }
 }
 
@@ -592,10 +702,13 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
   a.scratch = scratch;
   if (a.nlines <= 0) return "";
   if (a.nlines >= (1L << 31) - 64) return "line op: too many lines";
+  if ((long)ein.n[0] * ein.n[1] * ein.n[2] >= (1L << 31) || (long)eout.n[0] * eout.n[1] * eout.n[2] >= (1L << 31))
+    return "line op: arrays of 2^31 or more elements";
   if (p.part_P <= kMaxChunks && p.n_out >= 4) {
     a.P = p.part_P;
     a.per = (p.solve == 2);
     a.ld = D.ld; a.lc = D.lc; a.ar = D.ar; a.sv = D.sv; a.sw = D.sw; a.rinv = D.rinv;
+    a.swin = D.win_ok ? D.swin : nullptr;
     const int nmax = p.n_in > p.n_out ? p.n_in : p.n_out;
     const int lsu = nmax | 1;  // odd row stride
     const size_t shm_u = sizeof(T) * ((size_t)64 * lsu + (size_t)64 * 2 * p.part_P);

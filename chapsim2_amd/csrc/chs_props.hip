@@ -15,29 +15,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/chapsim_hip.h"
+#include "chs_props.h"
 
 namespace chs {
 void set_error(const std::string& msg);
 }
 
-struct chs_props {
-  int n = 0;
-  double dhmin = 0, dhmax = 0;
-  double ref[7] = {0, 0, 0, 0, 0, 0, 0};
-  std::vector<double> col[8];  // t, d, m, k, b, cp, h, rhoh (non-dimensional)
-  double* dev = nullptr;       // 8 columns of n, same order
-  int* flag = nullptr;         // out-of-range counter
-  void* io = nullptr;          // host-memory staging
-  size_t io_bytes = 0;
-  hipStream_t stream = nullptr;
-  ~chs_props() {
-    if (dev) (void)hipFree(dev);
-    if (flag) (void)hipFree(flag);
-    if (io) (void)hipFree(io);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
 
 namespace {
 
@@ -132,6 +115,37 @@ __global__ void __launch_bounds__(256) k_props_from_dh(const double* __restrict_
   if (ocp) ocp[c] = interp(CCP);
 }
 
+// ftp_refresh_thermal_properties_from_H per cell (no clamping, as the
+// reference: outside the table the end segment extrapolates)
+template <bool LDS>
+__global__ void __launch_bounds__(256) k_props_from_h(const double* __restrict__ tab, int n,
+                                                      const double* __restrict__ h_in, long N,
+                                                      double* __restrict__ od, double* __restrict__ om,
+                                                      double* __restrict__ ok, double* __restrict__ ot,
+                                                      double* __restrict__ ob, double* __restrict__ ocp) {
+  extern __shared__ double sh[];
+  const double* ch = tab + (long)CH * n;
+  if (LDS) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sh[i] = ch[i];
+    __syncthreads();
+    ch = sh;
+  }
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const double h = h_in[c];
+  int i1, i2;
+  bisect_dev(ch, n, h, &i1, &i2);
+  const double w1 = (ch[i2] - h) / (ch[i2] - ch[i1]);
+  const double w2 = 1.0 - w1;
+  auto interp = [&](int col) { return w1 * tab[(long)col * n + i1] + w2 * tab[(long)col * n + i2]; };
+  if (od) od[c] = interp(CD);
+  if (om) om[c] = interp(CM);
+  if (ok) ok[c] = interp(CK);
+  if (ot) ot[c] = interp(CT);
+  if (ob) ob[c] = interp(CB);
+  if (ocp) ocp[c] = interp(CCP);
+}
+
 #pragma clang fp contract(on)
 
 }  // namespace
@@ -163,9 +177,9 @@ int chs_props_create(const double* table, int nrows, double ref_t0, chs_props_t*
     if (kmin != i)
       for (auto& c : p->col) std::swap(c[i], c[kmin]);
   }
-  int i1, i2;
-  bisect_host(p->col[CT], ref_t0, &i1, &i2);
-  {
+  if (ref_t0 > 0.0) {
+    int i1, i2;
+    bisect_host(p->col[CT], ref_t0, &i1, &i2);
     const double w1 = (p->col[CT][i2] - ref_t0) / (p->col[CT][i2] - p->col[CT][i1]), w2 = 1.0 - w1;
     auto in = [&](int c) {
       const double a = w1 * p->col[c][i1];
@@ -179,17 +193,18 @@ int chs_props_create(const double* table, int nrows, double ref_t0, chs_props_t*
     p->ref[4] = in(CH);
     p->ref[5] = in(CB);
     p->ref[6] = in(CCP);
+    for (int i = 0; i < nrows; ++i) {
+      p->col[CT][i] = p->col[CT][i] / p->ref[0];
+      p->col[CD][i] = p->col[CD][i] / p->ref[1];
+      p->col[CM][i] = p->col[CM][i] / p->ref[2];
+      p->col[CK][i] = p->col[CK][i] / p->ref[3];
+      p->col[CB][i] = p->col[CB][i] / p->ref[5];
+      p->col[CCP][i] = p->col[CCP][i] / p->ref[6];
+      p->col[CH][i] = (p->col[CH][i] - p->ref[4]) / p->ref[0] / p->ref[6];
+      p->col[CRH][i] = p->col[CD][i] * p->col[CH][i];
+    }
   }
-  for (int i = 0; i < nrows; ++i) {
-    p->col[CT][i] = p->col[CT][i] / p->ref[0];
-    p->col[CD][i] = p->col[CD][i] / p->ref[1];
-    p->col[CM][i] = p->col[CM][i] / p->ref[2];
-    p->col[CK][i] = p->col[CK][i] / p->ref[3];
-    p->col[CB][i] = p->col[CB][i] / p->ref[5];
-    p->col[CCP][i] = p->col[CCP][i] / p->ref[6];
-    p->col[CH][i] = (p->col[CH][i] - p->ref[4]) / p->ref[0] / p->ref[6];
-    p->col[CRH][i] = p->col[CD][i] * p->col[CH][i];
-  }
+  // ref_t0 <= 0: the rows are already non-dimensional (ref stays zero)
   int imin = 0, imax = 0;
   for (int i = 1; i < nrows; ++i) {
     if (p->col[CRH][i] < p->col[CRH][imin]) imin = i;
@@ -278,6 +293,49 @@ int chs_props_from_rhoh(chs_props_t* p, const double* rhoh, long n, double* h, d
       hipStreamSynchronize(p->stream) != hipSuccess)
     return fail(CHS_EHIP, "synchronise");
   if (nbad) return fail(CHS_ERANGE, "rho*h is out of range (" + std::to_string(nbad) + " cells clamped)");
+  return CHS_OK;
+}
+
+int chs_props_from_h(chs_props_t* p, const double* h, long n, double* d, double* m, double* k, double* t,
+                     double* b, double* cp, int mem) {
+  if (!p || !h || n < 0) return fail(CHS_EINVAL, "invalid argument");
+  if (mem != CHS_MEM_HOST && mem != CHS_MEM_DEVICE) return fail(CHS_EINVAL, "invalid mem kind");
+  if (n == 0) return CHS_OK;
+  double* outs[6] = {d, m, k, t, b, cp};
+  const double* din = h;
+  double* dout[6] = {d, m, k, t, b, cp};
+  if (mem == CHS_MEM_HOST) {
+    int nout = 0;
+    for (double* o : outs) nout += o != nullptr;
+    const size_t need = sizeof(double) * (size_t)n * (1 + nout);
+    if (need > p->io_bytes) {
+      if (p->io) (void)hipFree(p->io);
+      p->io = nullptr;
+      p->io_bytes = 0;
+      if (hipMalloc(&p->io, need) != hipSuccess) return fail(CHS_ENOMEM, "staging allocation failed");
+      p->io_bytes = need;
+    }
+    double* base = static_cast<double*>(p->io);
+    if (hipMemcpyAsync(base, h, sizeof(double) * n, hipMemcpyHostToDevice, p->stream) != hipSuccess)
+      return fail(CHS_EHIP, "copy in");
+    din = base;
+    int slot = 1;
+    for (int q = 0; q < 6; ++q) dout[q] = outs[q] ? base + (size_t)n * slot++ : nullptr;
+  }
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  if (p->n <= 3 * kPropsLds)
+    hipLaunchKernelGGL(k_props_from_h<true>, dim3(grid), dim3(256), sizeof(double) * (size_t)p->n, p->stream, p->dev,
+                       p->n, din, n, dout[0], dout[1], dout[2], dout[3], dout[4], dout[5]);
+  else
+    hipLaunchKernelGGL(k_props_from_h<false>, dim3(grid), dim3(256), 0, p->stream, p->dev, p->n, din, n, dout[0],
+                       dout[1], dout[2], dout[3], dout[4], dout[5]);
+  if (hipGetLastError() != hipSuccess) return fail(CHS_EHIP, "property kernel launch");
+  if (mem == CHS_MEM_HOST)
+    for (int q = 0; q < 6; ++q)
+      if (outs[q] && hipMemcpyAsync(outs[q], dout[q], sizeof(double) * n, hipMemcpyDeviceToHost, p->stream) !=
+                         hipSuccess)
+        return fail(CHS_EHIP, "copy out");
+  if (hipStreamSynchronize(p->stream) != hipSuccess) return fail(CHS_EHIP, "synchronise");
   return CHS_OK;
 }
 

@@ -49,7 +49,7 @@ class Domain:
 
     def __init__(self, nc, lxx, lyb, lyt, lzz, **kw):
         self.lib = _lib.load()
-        self.desc = make_desc(nc, lxx, lyb, lyt, lzz, **kw)
+        self.desc = make_desc(nc, lxx, lyb, lyt, lzz, **{k: v for k, v in kw.items() if k != "_handle"})
         self.nc = tuple(int(n) for n in nc)
         py = self.nc[1] if kw.get("periodic_y", False) else self.nc[1] + 1
         self.np = (self.nc[0], py, self.nc[2])
@@ -59,13 +59,20 @@ class Domain:
             "pcc": (nx, ny, nz), "cpc": (nx, py, nz), "ccp": (nx, ny, nz), "ccc": (nx, ny, nz),
             "ppc": (nx, py, nz), "pcp": (nx, ny, nz), "cpp": (nx, py, nz),
         }
+        handle = kw.get("_handle")
+        if handle is not None:
+            self.h = C.c_void_p(handle)  # owned by a Batch
+            self.owned = False
+            return
         h = C.c_void_p()
         _lib.check(self.lib.chs_create(C.byref(self.desc), C.byref(h)))
         self.h = h
+        self.owned = True
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.chs_destroy(self.h)
+            if getattr(self, "owned", True):
+                self.lib.chs_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -113,3 +120,41 @@ class Domain:
         out = np.zeros(self.shapes["ccc"], order="F")
         _lib.check(self.lib.chs_divergence(self.h, out.ctypes.data, _lib.MEM_HOST))
         return out
+
+
+class Batch:
+    """Independent channels advanced together (chs_batch_*): each spec is a
+    dict of Domain arguments (nc, lxx, lyb, lyt, lzz and keywords).  Every
+    RK3 step of all instances runs as one captured graph when graph=True."""
+
+    def __init__(self, specs):
+        self.lib = _lib.load()
+        n = len(specs)
+        descs = (_lib.ChsDesc * n)()
+        for i, sp in enumerate(specs):
+            kw = {k: v for k, v in sp.items() if k not in ("nc", "lxx", "lyb", "lyt", "lzz")}
+            descs[i] = make_desc(sp["nc"], sp["lxx"], sp["lyb"], sp["lyt"], sp["lzz"], **kw)
+        h = C.c_void_p()
+        _lib.check(self.lib.chs_batch_create(descs, n, C.byref(h)))
+        self.h = h
+        self.domains = []
+        for i, sp in enumerate(specs):
+            kw = {k: v for k, v in sp.items() if k not in ("nc", "lxx", "lyb", "lyt", "lzz")}
+            hi = self.lib.chs_batch_instance(self.h, i)
+            self.domains.append(Domain(sp["nc"], sp["lxx"], sp["lyb"], sp["lyt"], sp["lzz"], _handle=hi, **kw))
+
+    def run(self, iter0, nsteps, graph=True):
+        _lib.check(self.lib.chs_batch_run(self.h, int(iter0), int(nsteps), int(bool(graph))))
+
+    def close(self):
+        if getattr(self, "h", None):
+            for d in self.domains:
+                d.close()
+            self.lib.chs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

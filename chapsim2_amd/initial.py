@@ -87,3 +87,25 @@ def tgv_fields(nc, lxx, lyb, lyt, lzz):
     X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
     pr = 1.0 / 16.0 * (np.cos(2.0 * X) + np.cos(2.0 * Y)) * (np.cos(2.0 * Z) + 2.0)
     return [np.asfortranarray(a) for a in (qx, qy, qz, pr)]
+
+
+def heated_channel_table(n=4096):
+    """Non-dimensional property table of the variable-property heated channel
+    case: n rows over h in [0, 2] with rho = 1/(1 + 0.8 tanh(5(h-1))),
+    cp = 1 + 7 exp(-((h-1)/0.05)^2) (a peak of height 8 at h = 1),
+    T = 1 + int_0^h dh'/cp (so that dh = cp dT), mu = exp(-0.6 (h-1)) and
+    kappa = 1 + 0.25 tanh(3(h-1)) (smooth monotone fits), beta = -rho'(h) cp / rho.
+    Rows are (P, H, T, D, M, K, CP, B), the column order of the reference's
+    property tables; use it with ref_t0 <= 0 (already non-dimensional)."""
+    h = np.linspace(0.0, 2.0, n)
+    fine = np.linspace(0.0, 2.0, 64 * (n - 1) + 1)
+    cpf = 1.0 + 7.0 * np.exp(-((fine - 1.0) / 0.05) ** 2)
+    integ = np.concatenate([[0.0], np.cumsum(0.5 * (1.0 / cpf[1:] + 1.0 / cpf[:-1]) * np.diff(fine))])
+    T = 1.0 + integ[::64]
+    rho = 1.0 / (1.0 + 0.8 * np.tanh(5.0 * (h - 1.0)))
+    cp = 1.0 + 7.0 * np.exp(-((h - 1.0) / 0.05) ** 2)
+    mu = np.exp(-0.6 * (h - 1.0))
+    kap = 1.0 + 0.25 * np.tanh(3.0 * (h - 1.0))
+    drho = -rho**2 * 0.8 * 5.0 / np.cosh(5.0 * (h - 1.0)) ** 2
+    beta = -drho * cp / rho
+    return np.stack([np.ones(n), h, T, rho, mu, kap, cp, beta], axis=1)

@@ -40,6 +40,16 @@ class PropertyTable:
                                                 *[outs[n].ctypes.data for n in NAMES], mode, _lib.MEM_HOST))
         return outs
 
+    def from_h(self, h):
+        """ftp_refresh_thermal_properties_from_H on every element of h;
+        returns a dict of d, m, k, t, b, cp arrays."""
+        x = np.ascontiguousarray(h, dtype=np.float64).ravel()
+        names = ("d", "m", "k", "t", "b", "cp")
+        outs = {n: np.zeros_like(x) for n in names}
+        _lib.check(self.lib.chs_props_from_h(self.h, x.ctypes.data, x.size, *[outs[n].ctypes.data for n in names],
+                                             _lib.MEM_HOST))
+        return outs
+
     def close(self):
         if self.h:
             self.lib.chs_props_destroy(self.h)

@@ -141,6 +141,22 @@ int chs_mass_check(chs_handle_t *h, double out[3]);
  * Get_divergence_flow (src/eq_continuity.f90:91-161). */
 int chs_divergence(chs_handle_t *h, double *div, int mem);
 
+/* ---- batched channels (no counterpart in the reference, which runs one
+ * channel per job) ---------------------------------------------------------
+ * n independent instances, each created from its own descriptor (uneven
+ * grids allowed).  chs_batch_run advances every instance nsteps RK3 steps
+ * (Update_Re + three sub-steps each); instances run on their own streams,
+ * and with use_graph the whole RK3 step of all instances is one captured
+ * graph (captured after one eager step, re-captured when 1/Re changes).
+ * chs_batch_instance gives the per-instance handle for chs_set_state,
+ * chs_get_state, chs_update_re, chs_substep and chs_mass_check; it stays
+ * owned by the batch. */
+typedef struct chs_batch chs_batch_t;
+int chs_batch_create(const chs_desc_t *descs, int n, chs_batch_t **out);
+int chs_batch_destroy(chs_batch_t *b);
+chs_handle_t *chs_batch_instance(chs_batch_t *b, int i);
+int chs_batch_run(chs_batch_t *b, int iter0, int nsteps, int use_graph);
+
 /* ---- thermal properties from a tabulated equation of state ------------- */
 
 typedef struct chs_props chs_props_t;
@@ -168,6 +184,37 @@ int chs_props_info(const chs_props_t *p, double *dhmin, double *dhmax, double re
  * the reference stops with an error. */
 int chs_props_from_rhoh(chs_props_t *p, const double *rhoh, long n, double *h, double *d, double *m, double *k,
                         double *t, double *b, double *cp, int mode, int mem);
+
+/* Per-cell properties from h: bisection on h and linear interpolation, with
+ * no clamping (the end segments extrapolate).  Replaces
+ * ftp_refresh_thermal_properties_from_H (src/input_thermo.f90:345-374).
+ * A table created with ref_t0 <= 0 is taken as already non-dimensional. */
+int chs_props_from_h(chs_props_t *p, const double *h, long n, double *d, double *m, double *k, double *t,
+                     double *b, double *cp, int mem);
+
+/* ---- MLP surrogate of the property table (no counterpart in the reference)
+ * h -> (T, rho, mu, kappa, cp); 1 -> 64 -> 64 -> 5, tanh; BF16 weights, FP32
+ * accumulation; trained on the GPU on nsamples samples of the table's own
+ * interpolation in h (uniform over the table's h range).  Parameters
+ * (4613, FP32 master copy): W1[64], b1[64], W2[64][64] (row = output unit),
+ * b2[64], W3[5][64], b3[5]; inputs and outputs are normalised with
+ * norm = (x centre, x scale, 5 output means, 5 output scales). */
+typedef struct chs_mlp chs_mlp_t;
+int chs_mlp_create(chs_props_t *table, int nsamples, unsigned seed, chs_mlp_t **out);
+int chs_mlp_destroy(chs_mlp_t *m);
+int chs_mlp_info(const chs_mlp_t *m, int *nparams, float norm[12]);
+int chs_mlp_get_params(chs_mlp_t *m, float *p32, unsigned short *p16);
+int chs_mlp_set_params(chs_mlp_t *m, const float *p32);
+/* Mean squared error over B given samples (normalised inputs x[B], targets
+ * t[B][5]) and its gradient with respect to the parameters, evaluated with
+ * the BF16 weights. */
+int chs_mlp_grad(chs_mlp_t *m, const float *x, const float *t, int B, float *grad, float *loss);
+/* steps Adam steps on random minibatches of the training set; loss of the
+ * last minibatch. */
+int chs_mlp_train(chs_mlp_t *m, int steps, int batch, float lr, float *loss);
+/* Surrogate evaluation for n enthalpies; any output may be NULL. */
+int chs_mlp_eval(chs_mlp_t *m, const double *h, long n, double *T, double *rho, double *mu, double *kappa,
+                 double *cp, int mem);
 
 #ifdef __cplusplus
 }
