@@ -34,6 +34,7 @@
 // forward sweep so the correction is applied in the same backward pass.
 #include "chs_lineop.h"
 
+#include <cmath>
 #include <vector>
 
 namespace chs {
@@ -98,6 +99,26 @@ std::string upload_plan(const LinePlan& p, DevPlan<T>* dp) {
         const int slot = p.rows[q].idx[t] - (s0 - 2);
         if (slot < 0 || slot >= W) { ok = false; break; }
         sw[q * W + slot] += p.rows[q].w[t];
+      }
+    }
+    // self-check of the slot mapping on a deterministic test line: the dense
+    // weights applied to the clamped window must give the recipe's value
+    if (ok) {
+      std::vector<double> v(p.n_in);
+      for (int k = 0; k < p.n_in; ++k) v[k] = 1.0 + 0.37 * k - 0.011 * k * (double)k;
+      for (int q = 0; q < 4 && ok; ++q) {
+        const int s0 = (q < 2) ? 0 : p.part_start[p.part_P - 1];
+        double direct = 0.0, dense = 0.0, mag = 0.0;
+        for (int t = 0; t < p.rows[q].nterm; ++t) {
+          direct += p.rows[q].w[t] * v[p.rows[q].idx[t]];
+          mag += std::fabs(p.rows[q].w[t] * v[p.rows[q].idx[t]]);
+        }
+        for (int t = 0; t < W; ++t) {
+          int k = s0 - 2 + t;
+          k = k < 0 ? 0 : (k > p.n_in - 1 ? p.n_in - 1 : k);
+          dense += sw[q * W + t] * v[k];
+        }
+        if (std::fabs(direct - dense) > 1e-12 * (mag + 1e-300)) ok = false;
       }
     }
     dp->win_ok = ok;

@@ -151,3 +151,26 @@ def test_ops_long_lines(po):
     finally:
         dm.close()
     assert nchecked > 40
+
+
+def test_ops_512_lines(po):
+    """Lines of 512 points (the operator benchmark's x and z extents) take
+    the single-thread-per-line kernel; parity against the oracle."""
+    from chapsim2_amd import operations as ops
+    from chapsim2_amd.domain import Domain
+
+    nc = (512, 24, 512)
+    dm = Domain(nc, LX, -1.0, 1.0, LZ, istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1)
+    odm = po.Dom(list(nc), [1, 0, 1], LX, -1.0, 1.0, LZ, 2, 0.075)
+    rng = np.random.default_rng(23)
+    try:
+        for d in (0, 2):
+            for op in (0, 4, 5):
+                ext_in, ext_out = list(nc), list(nc)
+                fi = np.asfortranarray(rng.standard_normal(ext_in))
+                ref = odm.op3d(d, op, 4, [1, 1], fi, ext_out)
+                out = np.zeros(ext_out, order="F")
+                ops.op3d(dm, d, op, fi, out, 4, [1, 1], None)
+                assert np.abs(out - ref).max() / np.abs(ref).max() < RTOL, (d, op)
+    finally:
+        dm.close()
