@@ -22,6 +22,8 @@ struct DevRow {
 };
 
 // Device copy of a LinePlan in precision T.
+constexpr int kCo = 8;  // entries per chunk coefficient row
+
 template <typename T>
 struct DevPlan {
   LinePlan host;
@@ -32,13 +34,11 @@ struct DevPlan {
   T* pw = nullptr;
   T* x1 = nullptr;
   T* metric = nullptr;
-  // partitioned solve (LinePlan::part_P chunks per line)
-  T* ld = nullptr;
-  T* lc = nullptr;
-  T* ar = nullptr;
-  T* sv = nullptr;
-  T* sw = nullptr;
+  // partitioned solve (LinePlan::part_P chunks per line): the 2P x 2P
+  // reduced-system inverse and the chunks' coefficient rows (kChunk rows of
+  // kCo entries per chunk)
   T* rinv = nullptr;
+  T* co = nullptr;
   // boundary rows 0, 1 (first chunk) and n-2, n-1 (last chunk) as dense
   // weights over the chunk's register window of kChunk + 4 inputs
   // (win_ok false when a recipe reaches outside that window)

@@ -47,8 +47,8 @@ namespace chs {
 
 template <typename T>
 DevPlan<T>::~DevPlan() {
-  for (void* p : {(void*)rows, (void*)a, (void*)c, (void*)d, (void*)pw, (void*)x1, (void*)metric, (void*)ld,
-                  (void*)lc, (void*)ar, (void*)sv, (void*)sw, (void*)rinv, (void*)swin})
+  for (void* p : {(void*)rows, (void*)a, (void*)c, (void*)d, (void*)pw, (void*)x1, (void*)metric, (void*)rinv,
+                  (void*)swin, (void*)co})
     if (p) (void)hipFree(p);
 }
 
@@ -59,6 +59,14 @@ static std::string up(const std::vector<double>& v, T** dst) {
   CHS_CHECK(hipMalloc(dst, sizeof(T) * t.size()));
   CHS_CHECK(hipMemcpy(*dst, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice));
   return "";
+}
+
+// Line index read for window position k, -2 <= k <= n + kChunk + 1:
+// wrapped on periodic lines (one period suffices, as periodic lines on the
+// partitioned kernels have n >= kChunk + 2), clamped to the line otherwise.
+__host__ __device__ __forceinline__ int window_index(int k, int n, bool wrap) {
+  if (wrap) return k < 0 ? k + n : (k >= n ? k - n : k);
+  return k < 0 ? 0 : (k > n - 1 ? n - 1 : k);
 }
 
 template <typename T>
@@ -82,22 +90,21 @@ std::string upload_plan(const LinePlan& p, DevPlan<T>* dp) {
   if (!(e = up(p.pw, &dp->pw)).empty()) return e;
   if (!(e = up(p.x1, &dp->x1)).empty()) return e;
   if (!(e = up(p.metric, &dp->metric)).empty()) return e;
-  if (!(e = up(p.ld, &dp->ld)).empty()) return e;
-  if (!(e = up(p.lc, &dp->lc)).empty()) return e;
-  if (!(e = up(p.a_raw, &dp->ar)).empty()) return e;
-  if (!(e = up(p.sv, &dp->sv)).empty()) return e;
-  if (!(e = up(p.sw, &dp->sw)).empty()) return e;
   if (!(e = up(p.rinv, &dp->rinv)).empty()) return e;
   // dense window weights of the four boundary rows
   {
     const int W = kChunk + 4;
+    const bool wrap = (p.ibc[0] == BC_PERIODIC);
     std::vector<double> sw(4 * W, 0.0);
-    bool ok = p.part_P >= 1 && (int)p.part_start.size() == p.part_P + 1;
+    bool ok = p.part_P >= 1 && (int)p.part_start.size() == p.part_P + 1 && (!wrap || p.n_in >= kChunk + 2);
     for (int q = 0; q < 4 && ok; ++q) {
       const int s0 = (q < 2) ? 0 : p.part_start[p.part_P - 1];
       for (int t = 0; t < p.rows[q].nterm; ++t) {
-        const int slot = p.rows[q].idx[t] - (s0 - 2);
-        if (slot < 0 || slot >= W) { ok = false; break; }
+        // the window slot holding line index idx (as the kernels index it)
+        int slot = -1;
+        for (int u = 0; u < W && slot < 0; ++u)
+          if (window_index(s0 - 2 + u, p.n_in, wrap) == p.rows[q].idx[t]) slot = u;
+        if (slot < 0) { ok = false; break; }
         sw[q * W + slot] += p.rows[q].w[t];
       }
     }
@@ -113,16 +120,41 @@ std::string upload_plan(const LinePlan& p, DevPlan<T>* dp) {
           direct += p.rows[q].w[t] * v[p.rows[q].idx[t]];
           mag += std::fabs(p.rows[q].w[t] * v[p.rows[q].idx[t]]);
         }
-        for (int t = 0; t < W; ++t) {
-          int k = s0 - 2 + t;
-          k = k < 0 ? 0 : (k > p.n_in - 1 ? p.n_in - 1 : k);
-          dense += sw[q * W + t] * v[k];
-        }
+        for (int t = 0; t < W; ++t) dense += sw[q * W + t] * v[window_index(s0 - 2 + t, p.n_in, wrap)];
         if (std::fabs(direct - dense) > 1e-12 * (mag + 1e-300)) ok = false;
       }
     }
     dp->win_ok = ok;
     if (ok && !(e = up(sw, &dp->swin)).empty()) return e;
+  }
+  // chunk coefficient rows: kChunk rows per chunk, kCo entries per row
+  //   0 ld (local pivot inverse), 1 a (sub-diagonal; 0 on the chunk's first
+  //   row), 2 lc (local super-diagonal; 0 on the chunk's last row), 3 and 4
+  //   the spike vectors, 5 the output metric (1 when absent); rows past the
+  //   chunk's length are identity rows
+  if (p.part_P >= 1 && (int)p.part_start.size() == p.part_P + 1 && p.part_P * kChunk >= p.n_out) {
+    std::vector<double> co((size_t)p.part_P * kChunk * kCo, 0.0);
+    const bool solve_data = !p.ld.empty();
+    for (int q = 0; q < p.part_P; ++q) {
+      const int s0 = p.part_start[q], len = p.part_start[q + 1] - s0;
+      if (len > kChunk || s0 != (q * p.n_out) / p.part_P) return "line plan: chunk layout differs from the kernels'";
+      for (int i = 0; i < kChunk; ++i) {
+        double* c = &co[((size_t)q * kChunk + i) * kCo];
+        const int r = s0 + i;
+        c[0] = 1.0;
+        c[5] = 1.0;
+        if (i >= len) continue;
+        if (solve_data) {
+          c[0] = p.ld[r];
+          c[1] = (i > 0) ? p.a_raw[r] : 0.0;
+          c[2] = (i < len - 1) ? p.lc[r] : 0.0;
+          c[3] = p.sv[r];
+          c[4] = p.sw[r];
+        }
+        if (!p.metric.empty()) c[5] = p.metric[r];
+      }
+    }
+    if (!(e = up(co, &dp->co)).empty()) return e;
   }
   return "";
 }
@@ -147,8 +179,10 @@ struct KArgs {
   T* scratch;
   // partitioned form
   int P, per, ls, nl;
-  const T* ld; const T* lc; const T* ar; const T* sv; const T* sw; const T* rinv;
-  const T* swin;  // 4 x (kChunk + 4) boundary-row window weights, or null
+  const T* rinv;
+  const T* swin;  // 4 x (kChunk + 4) boundary-row window weights
+  int wrap;       // periodic line: windows wrap around instead of clamping
+  const T* co;    // P x kChunk coefficient rows of kCo entries (see upload_plan)
 };
 
 // bulk offsets (p1,p2,p3,p4) per operator kind, relative to the output row
@@ -266,10 +300,10 @@ __global__ void __launch_bounds__(64) k_line(KArgs<T> P) {
 
 // Chunk rows [s0, s0+len) of one line: explicit RHS into y[] from the
 // window f[t] = input at line index s0-2+t (clamped to the line).  The four
-// boundary rows use their dense window weights; FA(k) (input at an arbitrary
-// line index) is only used when a boundary recipe reaches outside the window.
-template <typename T, int OPK, class FA>
-__device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int len, const T (&f)[kChunk + 4], FA&& Fa,
+// boundary rows use their dense window weights (plans whose boundary recipes
+// reach outside the window run on the one-thread-per-line kernel instead).
+template <typename T, int OPK>
+__device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int len, const T (&f)[kChunk + 4],
                                           const T (&fb4)[4], T (&y)[kChunk]) {
   using O = Offs<OPK>;
   constexpr int W = kChunk + 4;
@@ -280,13 +314,9 @@ __device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int len, con
   auto special = [&](int rq) -> T {
     const DevRow<T>& R = P.rows[rq];
     T v = T(0);
-    if (P.swin) {
-      const T* wv = P.swin + rq * W;
+    const T* wv = P.swin + rq * W;
 #pragma unroll
-      for (int t = 0; t < W; ++t) v = v + wv[t] * f[t];
-    } else {
-      for (int t = 0; t < R.nterm; ++t) v = v + R.w[t] * Fa(R.idx[t]);
-    }
+    for (int t = 0; t < W; ++t) v = v + wv[t] * f[t];
     if (P.fbc)
       for (int m = 0; m < 4; ++m) v = v + R.u[m] * fb4[m];
     return v * P.dd;
@@ -305,49 +335,82 @@ __device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int len, con
   }
 }
 
-// Window of one y/z line chunk from global memory: every load is issued
-// before any use (indices clamped into the line, so all addresses are
-// valid), then the input transform sA * (A + cB * B) [* Cm] is applied.
+// Element at byte offset `boff` from `base`: the y/z kernels address a line
+// as a wave-uniform base pointer plus a 32-bit per-lane byte offset.
 template <typename T>
-__device__ __forceinline__ void load_window(const KArgs<T>& P, long ib, long is, int s0, T (&f)[kChunk + 4]) {
-  constexpr int W = kChunk + 4;
-  const int nmax = P.n_in - 1;
-  int o[W];  // element offsets (arrays hold fewer than 2^31 elements, checked at launch)
-#pragma unroll
-  for (int t = 0; t < W; ++t) {
-    int k = s0 - 2 + t;
-    k = k < 0 ? 0 : (k > nmax ? nmax : k);
-    o[t] = (int)(ib + is * k);
-  }
-#pragma unroll
-  for (int t = 0; t < W; ++t) f[t] = P.A[o[t]];
+__device__ __forceinline__ T ld_at(const T* base, unsigned boff) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+template <typename T>
+__device__ __forceinline__ void st_at(T* base, unsigned boff, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+
+// Input transform sA * (A + cB * B) * Cm of `nv` elements: f holds A, g holds
+// the array loaded beside it (B if present, else Cm).  A third array (Cm when
+// B is also present) is fetched by `third`.
+template <typename T, int NV, class TH>
+__device__ __forceinline__ void transform_in(const KArgs<T>& P, T (&f)[NV], const T (&g)[NV], TH&& third) {
   if (P.B) {
-    T g[W];
 #pragma unroll
-    for (int t = 0; t < W; ++t) g[t] = P.B[o[t]];
-#pragma unroll
-    for (int t = 0; t < W; ++t) f[t] = f[t] + P.cB * g[t];
+    for (int t = 0; t < NV; ++t) f[t] = f[t] + P.cB * g[t];
   }
 #pragma unroll
-  for (int t = 0; t < W; ++t) f[t] = P.sA * f[t];
+  for (int t = 0; t < NV; ++t) f[t] = P.sA * f[t];
   if (P.Cm) {
-    T g[W];
+    if (!P.B) {
 #pragma unroll
-    for (int t = 0; t < W; ++t) g[t] = P.Cm[o[t]];
+      for (int t = 0; t < NV; ++t) f[t] = f[t] * g[t];
+    } else {
+      T h[NV];
 #pragma unroll
-    for (int t = 0; t < W; ++t) f[t] = f[t] * g[t];
+      for (int t = 0; t < NV; ++t) h[t] = third(t);
+#pragma unroll
+      for (int t = 0; t < NV; ++t) f[t] = f[t] * h[t];
+    }
   }
 }
 
-// Chunk-local Thomas sweeps; returns the chunk's first and last values.
+// Window of one y/z line chunk: rows s0-2 .. s0+kChunk+1 (wrapped on
+// periodic lines, clamped otherwise, so every address is valid).  The row offsets are wave-uniform; the
+// loads of A and of the second input array are all issued before any use.
 template <typename T>
-__device__ __forceinline__ void chunk_solve(const KArgs<T>& P, int s0, int len, T (&y)[kChunk], T& yf, T& yl) {
+__device__ __forceinline__ void load_window(const KArgs<T>& P, long base, unsigned ri, long is, int s0,
+                                            T (&f)[kChunk + 4]) {
+  constexpr int W = kChunk + 4;
+  long ro[W];
 #pragma unroll
-  for (int i = 0; i < kChunk; ++i)
-    if (i < len) y[i] = (i == 0) ? y[i] * P.ld[s0] : (y[i] - P.ar[s0 + i] * y[i - 1]) * P.ld[s0 + i];
+  for (int t = 0; t < W; ++t) ro[t] = base + is * window_index(s0 - 2 + t, P.n_in, P.wrap);
+  const T* S = P.B ? P.B : P.Cm;
+  T g[W];
+  if (S) {
 #pragma unroll
-  for (int i = kChunk - 2; i >= 0; --i)
-    if (i < len - 1) y[i] = y[i] - P.lc[s0 + i] * y[i + 1];
+    for (int t = 0; t < W; ++t) {
+      f[t] = ld_at(P.A + ro[t], ri);
+      g[t] = ld_at(S + ro[t], ri);
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < W; ++t) {
+      f[t] = ld_at(P.A + ro[t], ri);
+      g[t] = T(0);
+    }
+  }
+  transform_in<T, W>(P, f, g, [&](int t) { return ld_at(P.Cm + ro[t], ri); });
+}
+
+// Chunk-local Thomas sweeps over all kChunk rows of chunk q; returns the
+// chunk's first and last values.  The coefficient rows past the chunk's
+// length are identity rows (and the chunk's last row does not couple to the
+// next), so the sweeps need no row guards.
+template <typename T>
+__device__ __forceinline__ void chunk_solve(const KArgs<T>& P, int q, int len, T (&y)[kChunk], T& yf, T& yl) {
+  const T* c = P.co + (long)q * kChunk * kCo;
+  y[0] = y[0] * c[0];
+#pragma unroll
+  for (int i = 1; i < kChunk; ++i) y[i] = (y[i] - c[i * kCo + 1] * y[i - 1]) * c[i * kCo + 0];
+#pragma unroll
+  for (int i = kChunk - 2; i >= 0; --i) y[i] = y[i] - c[i * kCo + 2] * y[i + 1];
   yf = y[0];
   yl = y[0];
 #pragma unroll
@@ -358,7 +421,7 @@ __device__ __forceinline__ void chunk_solve(const KArgs<T>& P, int s0, int len, 
 // Reduced-system values coupling chunk q to its neighbours; ex(k) reads the
 // published first/last value k (0..2P-1) of the thread's line.
 template <typename T, class EX>
-__device__ __forceinline__ void chunk_couple(const KArgs<T>& P, int q, int s0, int len, EX&& ex, T (&y)[kChunk]) {
+__device__ __forceinline__ void chunk_couple(const KArgs<T>& P, int q, EX&& ex, T (&y)[kChunk]) {
   const int Pn = P.P, M = 2 * Pn;
   const int qm = (q + Pn - 1) % Pn, qp = (q + 1) % Pn;
   const T* rL = P.rinv + (long)(2 * qm + 1) * M;
@@ -371,65 +434,99 @@ __device__ __forceinline__ void chunk_couple(const KArgs<T>& P, int q, int s0, i
   }
   if (!P.per && q == 0) uL = T(0);
   if (!P.per && q == Pn - 1) uF = T(0);
+  const T* c = P.co + (long)q * kChunk * kCo;
 #pragma unroll
-  for (int i = 0; i < kChunk; ++i)
-    if (i < len) y[i] = y[i] - P.sv[s0 + i] * uL - P.sw[s0 + i] * uF;
+  for (int i = 0; i < kChunk; ++i) y[i] = y[i] - c[i * kCo + 3] * uL - c[i * kCo + 4] * uF;
 }
 
+// Row i of chunk q (line row r): metric scaling and Neumann overrides.
 template <typename T>
-__device__ __forceinline__ T chunk_post(const KArgs<T>& P, int r, T val, const T (&fb4)[4]) {
-  if (P.metric) val = val * P.metric[r];
+__device__ __forceinline__ T chunk_post(const KArgs<T>& P, int q, int i, int r, T val, const T (&fb4)[4]) {
+  val = val * P.co[((long)q * kChunk + i) * kCo + 5];
   if (r == 0 && P.neu_lo) val = fb4[0];
   if (r == P.n_out - 1 && P.neu_hi) val = fb4[1];
   return val;
 }
 
-template <typename T, int DIR, int OPK>
-__global__ void __launch_bounds__(64 * kMaxChunks) k_line_yz(KArgs<T> P) {
-  __shared__ T ex[2 * kMaxChunks * 64];
-  const int lane = threadIdx.x & 63;
-  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int Lr = blockIdx.x * 64 + lane;  // nlines < 2^31 (checked at launch)
-  const bool live = Lr < P.nlines;
-  const int L = live ? Lr : (int)P.nlines - 1;
-  long ib, is, ob, os, fb, fs;
+// First elements of y or z line L in the input, output and fbc arrays.
+template <int DIR, typename T>
+__device__ __forceinline__ void yz_offsets(const KArgs<T>& P, int L, long& ib, long& ob, long& fb) {
   if (DIR == 1) {
     const int i = L % P.e1, k = L / P.e1;
-    ib = i + (long)P.e1 * P.e2 * k; is = P.e1;
-    ob = i + (long)P.o1 * P.o2 * k; os = P.o1;
-    fb = i + (long)P.e1 * 4 * k; fs = P.e1;
+    ib = i + (long)P.e1 * P.e2 * k;
+    ob = i + (long)P.o1 * P.o2 * k;
+    fb = i + (long)P.e1 * 4 * k;
   } else {
-    const int i = L % P.e1, j = L / P.e1;
-    ib = i + (long)P.e1 * j; is = (long)P.e1 * P.e2;
-    ob = i + (long)P.o1 * j; os = (long)P.o1 * P.o2;
-    fb = i + (long)P.e1 * j; fs = (long)P.e1 * P.e2;
+    ib = L;  // x varies fastest and the (x, y) extents of input and output agree
+    ob = L;
+    fb = L;
   }
-  const int n_in = P.n_in, n_out = P.n_out, Pn = P.P;
-  const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
-  auto Fa = [&](int k) -> T {
-    const long o = ib + is * k;
-    T t = P.A[o];
-    if (P.B) t = t + P.cB * P.B[o];
-    t = P.sA * t;
-    if (P.Cm) t = t * P.Cm[o];
-    return t;
-  };
+}
+
+// Window of an interior chunk (rows s0-2 .. s0+kChunk+1 all inside the
+// line): consecutive rows from r0 = s0 - 2, no clamping.
+template <typename T>
+__device__ __forceinline__ void load_window_interior(const KArgs<T>& P, long r0, unsigned ri, long is,
+                                                     T (&f)[kChunk + 4]) {
+  constexpr int W = kChunk + 4;
+  const T* S = P.B ? P.B : P.Cm;
+  T g[W];
+  if (S) {
+#pragma unroll
+    for (int t = 0; t < W; ++t) {
+      f[t] = ld_at(P.A + (r0 + is * t), ri);
+      g[t] = ld_at(S + (r0 + is * t), ri);
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < W; ++t) {
+      f[t] = ld_at(P.A + (r0 + is * t), ri);
+      g[t] = T(0);
+    }
+  }
+  transform_in<T, W>(P, f, g, [&](int t) { return ld_at(P.Cm + (r0 + is * t), ri); });
+}
+
+// Explicit RHS of the kChunk rows of an interior chunk (bulk stencil only).
+template <typename T, int OPK>
+__device__ __forceinline__ void chunk_rhs_bulk(const KArgs<T>& P, const T (&f)[kChunk + 4], T (&y)[kChunk]) {
+  using O = Offs<OPK>;
+#pragma unroll
+  for (int i = 0; i < kChunk; ++i)
+    y[i] = (P.c1 * (f[i + 2 + O::p1] + P.sgn * f[i + 2 + O::p2]) +
+            P.c2 * (f[i + 2 + O::p3] + P.sgn * f[i + 2 + O::p4])) * P.dd;
+}
+
+// One chunk of a y/z workgroup.  EDGE: a first or last chunk (clamped or
+// wrapped window, boundary rows, fbc, Neumann rows); otherwise an interior
+// chunk, whose window lies inside the line and whose rows all use the bulk
+// stencil.
+template <bool EDGE, typename T, int DIR, int OPK>
+__device__ __forceinline__ void yz_chunk(const KArgs<T>& P, T* ex, int lane, int q, int s0, int len, bool live,
+                                         long ib0, long ob0, long fb, unsigned ri, unsigned ro, long is, long os) {
+  const int n_out = P.n_out;
   T f[kChunk + 4];
-  load_window<T>(P, ib, is, s0, f);
   T fb4[4] = {T(0), T(0), T(0), T(0)};
-  if (P.fbc && (q == 0 || q == Pn - 1))
-    for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[fb + fs * m];
   T y[kChunk];
-  chunk_rhs<T, OPK>(P, q, len, f, Fa, fb4, y);
+  if (EDGE) {
+    load_window<T>(P, ib0, ri, is, s0, f);
+    if (P.fbc && (q == 0 || q == P.P - 1))
+      for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[fb + is * m];
+    chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
+  } else {
+    load_window_interior<T>(P, ib0 + is * (s0 - 2), ri, is, f);
+    chunk_rhs_bulk<T, OPK>(P, f, y);
+  }
   if (P.solve) {
     T yf, yl;
-    chunk_solve<T>(P, s0, len, y, yf, yl);
+    chunk_solve<T>(P, q, len, y, yf, yl);
     ex[lane + 64 * (2 * q)] = yf;
     ex[lane + 64 * (2 * q + 1)] = yl;
     __syncthreads();
-    chunk_couple<T>(P, q, s0, len, [&](int k) { return ex[lane + 64 * k]; }, y);
+    chunk_couple<T>(P, q, [&](int k) { return ex[lane + 64 * k]; }, y);
   }
   if (!live) return;
+  T* const Ob = P.out + ob0;
   T prev[kChunk];
 #pragma unroll
   for (int i = 0; i < kChunk; ++i) prev[i] = T(0);
@@ -437,17 +534,43 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_yz(KArgs<T> P) {
 #pragma unroll
     for (int i = 0; i < kChunk; ++i) {
       const int r = (s0 + i < n_out) ? s0 + i : n_out - 1;
-      prev[i] = P.out[ob + os * r];
+      prev[i] = ld_at<T>(Ob + os * r, ro);
     }
   }
 #pragma unroll
   for (int i = 0; i < kChunk; ++i) {
     if (i < len) {
       const int r = s0 + i;
-      const T val = chunk_post<T>(P, r, y[i], fb4);
-      P.out[ob + os * r] = prev[i] + val * P.sOut;
+      const T val = EDGE ? chunk_post<T>(P, q, i, r, y[i], fb4) : y[i] * P.co[((long)q * kChunk + i) * kCo + 5];
+      st_at<T>(Ob + os * r, ro, prev[i] + val * P.sOut);
     }
   }
+}
+
+template <typename T, int DIR, int OPK>
+__global__ void __launch_bounds__(64 * kMaxChunks) k_line_yz(KArgs<T> P) {
+  __shared__ T ex[2 * kMaxChunks * 64];
+  const int lane = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int L0 = blockIdx.x * 64;  // nlines < 2^31 (checked at launch)
+  const int Lr = L0 + lane;
+  const bool live = Lr < P.nlines;
+  const int L = live ? Lr : (int)P.nlines - 1;
+  // block base (wave-uniform) + per-lane byte offset (< 2^32, checked at launch)
+  long ib, ob, fb, ib0, ob0, fb0;
+  yz_offsets<DIR>(P, L, ib, ob, fb);
+  yz_offsets<DIR>(P, L0, ib0, ob0, fb0);
+  const unsigned ri = (unsigned)((ib - ib0) * (long)sizeof(T));
+  const unsigned ro = (unsigned)((ob - ob0) * (long)sizeof(T));
+  const long is = (DIR == 1) ? (long)P.e1 : (long)P.e1 * P.e2;
+  const long os = (DIR == 1) ? (long)P.o1 : (long)P.o1 * P.o2;
+  const int n_in = P.n_in, n_out = P.n_out, Pn = P.P;
+  const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
+  const bool edge = (q == 0) || (q == Pn - 1) || (s0 < 2) || (s0 + kChunk + 1 > n_in - 1);
+  if (edge)
+    yz_chunk<true, T, DIR, OPK>(P, ex, lane, q, s0, len, live, ib0, ob0, fb, ri, ro, is, os);
+  else
+    yz_chunk<false, T, DIR, OPK>(P, ex, lane, q, s0, len, live, ib0, ob0, fb, ri, ro, is, os);
 }
 
 __device__ __forceinline__ int lds_pos(int k) { return k + (k >> 4); }
@@ -483,33 +606,28 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
   const bool live = L < P.nlines;
   const T* row = tile + (long)ll * LS;
   const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
-  auto Fa = [&](int k) -> T { return row[lds_pos(k)]; };
   T f[kChunk + 4];
 #pragma unroll
-  for (int t = 0; t < kChunk + 4; ++t) {
-    int k = s0 - 2 + t;
-    k = k < 0 ? 0 : (k >= n_in ? n_in - 1 : k);
-    f[t] = row[lds_pos(k)];
-  }
+  for (int t = 0; t < kChunk + 4; ++t) f[t] = row[lds_pos(window_index(s0 - 2 + t, n_in, P.wrap))];
   T fb4[4] = {T(0), T(0), T(0), T(0)};
   if (P.fbc && live && (q == 0 || q == Pn - 1))
     for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
   T y[kChunk];
-  chunk_rhs<T, OPK>(P, q, len, f, Fa, fb4, y);
+  chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
   if (P.solve) {
     T yf, yl;
-    chunk_solve<T>(P, s0, len, y, yf, yl);
+    chunk_solve<T>(P, q, len, y, yf, yl);
     T* ex = exb + (long)ll * 2 * Pn;
     ex[2 * q] = yf;
     ex[2 * q + 1] = yl;
     __syncthreads();
-    chunk_couple<T>(P, q, s0, len, [&](int k) { return ex[k]; }, y);
+    chunk_couple<T>(P, q, [&](int k) { return ex[k]; }, y);
   }
   __syncthreads();  // every thread has finished reading its input rows
   T* orow = tile + (long)ll * LS;
 #pragma unroll
   for (int i = 0; i < kChunk; ++i)
-    if (i < len) orow[lds_pos(s0 + i)] = chunk_post<T>(P, s0 + i, y[i], fb4);
+    if (i < len) orow[lds_pos(s0 + i)] = chunk_post<T>(P, q, i, s0 + i, y[i], fb4);
   __syncthreads();
   for (int e = tid; e < NL * n_out; e += nthr) {
     const int l = e / n_out, k = e - l * n_out;
@@ -585,31 +703,26 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
   const bool live = L < P.nlines;
   T* row = tile + lane * LS;
   const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
-  auto Fa = [&](int k) -> T { return row[k]; };
   T f[kChunk + 4];
 #pragma unroll
-  for (int t = 0; t < kChunk + 4; ++t) {
-    int k = s0 - 2 + t;
-    k = k < 0 ? 0 : (k >= n_in ? n_in - 1 : k);
-    f[t] = row[k];
-  }
+  for (int t = 0; t < kChunk + 4; ++t) f[t] = row[window_index(s0 - 2 + t, n_in, P.wrap)];
   T fb4[4] = {T(0), T(0), T(0), T(0)};
   if (P.fbc && live && (q == 0 || q == Pn - 1))
     for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
   T y[kChunk];
-  chunk_rhs<T, OPK>(P, q, len, f, Fa, fb4, y);
+  chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
   if (P.solve) {
     T yf, yl;
-    chunk_solve<T>(P, s0, len, y, yf, yl);
+    chunk_solve<T>(P, q, len, y, yf, yl);
     exb[lane + 64 * (2 * q)] = yf;
     exb[lane + 64 * (2 * q + 1)] = yl;
     __syncthreads();
-    chunk_couple<T>(P, q, s0, len, [&](int k) { return exb[lane + 64 * k]; }, y);
+    chunk_couple<T>(P, q, [&](int k) { return exb[lane + 64 * k]; }, y);
   }
   __syncthreads();  // every wave has finished reading its input rows
 #pragma unroll
   for (int i = 0; i < kChunk; ++i)
-    if (i < len) row[s0 + i] = chunk_post<T>(P, s0 + i, y[i], fb4);
+    if (i < len) row[s0 + i] = chunk_post<T>(P, q, i, s0 + i, y[i], fb4);
   __syncthreads();
   const int tout = 64 * n_out;
   for (int b0 = 0; b0 < tout; b0 += nthr * UB) {
@@ -725,11 +838,13 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
   if (a.nlines >= (1L << 31) - 64) return "line op: too many lines";
   if ((long)ein.n[0] * ein.n[1] * ein.n[2] >= (1L << 31) || (long)eout.n[0] * eout.n[1] * eout.n[2] >= (1L << 31))
     return "line op: arrays of 2^31 or more elements";
-  if (p.part_P <= kMaxChunks && p.n_out >= 4) {
+  if (p.part_P <= kMaxChunks && p.n_out >= 4 && D.win_ok && D.co) {
     a.P = p.part_P;
     a.per = (p.solve == 2);
-    a.ld = D.ld; a.lc = D.lc; a.ar = D.ar; a.sv = D.sv; a.sw = D.sw; a.rinv = D.rinv;
-    a.swin = D.win_ok ? D.swin : nullptr;
+    a.rinv = D.rinv;
+    a.swin = D.swin;
+    a.wrap = (p.ibc[0] == BC_PERIODIC);
+    a.co = D.co;
     const int nmax = p.n_in > p.n_out ? p.n_in : p.n_out;
     const int lsu = nmax | 1;  // odd row stride
     const size_t shm_u = sizeof(T) * ((size_t)64 * lsu + (size_t)64 * 2 * p.part_P);
@@ -751,6 +866,13 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
       dim3 g((unsigned)((a.nlines + a.nl - 1) / a.nl)), b((unsigned)(a.nl * p.part_P));
       dispatch_part<T, 0>(p.op, g, b, shm, s, a);
     } else {
+      // per-lane byte offsets from a workgroup's first line stay below 2^32
+      // (a y-line block spans at most 64/e1 + 2 planes of x-y)
+      if (dir == 1) {
+        const long e2m = ein.n[1] > eout.n[1] ? ein.n[1] : eout.n[1];
+        if ((long)ein.n[0] * e2m * (64 / ein.n[0] + 2) * (long)sizeof(T) >= (1L << 32))
+          return "line op: x-y plane too large for 32-bit line offsets";
+      }
       dim3 g((unsigned)((a.nlines + 63) / 64)), b((unsigned)(64 * p.part_P));
       if (dir == 1) dispatch_part<T, 1>(p.op, g, b, 0, s, a);
       else dispatch_part<T, 2>(p.op, g, b, 0, s, a);

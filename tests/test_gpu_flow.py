@@ -73,7 +73,14 @@ def test_exact_projection_steps_vs_oracle(po, nc, iacc):
     against the oracle's per-mode dense LU; after every step max|div u| is at
     round-off (the reference's single 2nd-order projection grows it instead).
     Even ny takes the mirror-symmetric even/odd path (ny = 144: two row
-    tiles per half), odd ny the general one."""
+    tiles per half), odd ny the general one.
+
+    Tolerances: velocity 1e-13 relative (observed 1e-15 .. 3e-15 over the
+    five steps); pressure 5e-11 (observed 2e-12 .. 1.0e-11, not growing):
+    the pressure increment solves the nearly singular composed operator,
+    whose smallest non-zero eigenvalues amplify round-off by their condition
+    number, and the two sides reach it by different algorithms (eigen-basis
+    products here, per-mode LU in the oracle)."""
     from chapsim2_amd.domain import PROJ_EXACT, Domain
     from chapsim2_amd.initial import channel_fields
 
@@ -92,7 +99,7 @@ def test_exact_projection_steps_vs_oracle(po, nc, iacc):
             of.step(it)
             dm.run(it - 1, 1)
             for name, g, r in zip(("qx", "qy", "qz", "pr"), dm.get_state(), of.get_state()):
-                assert rel(g, r) < 1e-11, (it, name, rel(g, r))
+                assert rel(g, r) < (5e-11 if name == "pr" else 1e-13), (it, name, rel(g, r))
             div = dm.mass_check()
             assert np.all(div < 1e-12), (it, div)
     finally:
