@@ -23,6 +23,8 @@ struct DevRow {
 
 // Device copy of a LinePlan in precision T.
 constexpr int kCo = 8;  // entries per chunk coefficient row
+// entries of a chunk coefficient row (pairs share a 16-byte slot)
+enum CoEntry : int { CO_LD = 0, CO_AR = 1, CO_SV = 2, CO_SW = 3, CO_LC = 4, CO_MET = 5 };
 
 template <typename T>
 struct DevPlan {

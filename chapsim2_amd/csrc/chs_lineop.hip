@@ -127,11 +127,11 @@ std::string upload_plan(const LinePlan& p, DevPlan<T>* dp) {
     dp->win_ok = ok;
     if (ok && !(e = up(sw, &dp->swin)).empty()) return e;
   }
-  // chunk coefficient rows: kChunk rows per chunk, kCo entries per row
-  //   0 ld (local pivot inverse), 1 a (sub-diagonal; 0 on the chunk's first
-  //   row), 2 lc (local super-diagonal; 0 on the chunk's last row), 3 and 4
-  //   the spike vectors, 5 the output metric (1 when absent); rows past the
-  //   chunk's length are identity rows
+  // chunk coefficient rows: kChunk rows per chunk, kCo entries per row,
+  // paired for 16-byte reads: (ld local pivot inverse, a sub-diagonal; a = 0
+  // on the chunk's first row), (the two spike vectors), (lc local
+  // super-diagonal, 0 on the chunk's last row; the output metric, 1 when
+  // absent); rows past the chunk's length are identity rows
   if (p.part_P >= 1 && (int)p.part_start.size() == p.part_P + 1 && p.part_P * kChunk >= p.n_out) {
     std::vector<double> co((size_t)p.part_P * kChunk * kCo, 0.0);
     const bool solve_data = !p.ld.empty();
@@ -141,17 +141,17 @@ std::string upload_plan(const LinePlan& p, DevPlan<T>* dp) {
       for (int i = 0; i < kChunk; ++i) {
         double* c = &co[((size_t)q * kChunk + i) * kCo];
         const int r = s0 + i;
-        c[0] = 1.0;
-        c[5] = 1.0;
+        c[CO_LD] = 1.0;
+        c[CO_MET] = 1.0;
         if (i >= len) continue;
         if (solve_data) {
-          c[0] = p.ld[r];
-          c[1] = (i > 0) ? p.a_raw[r] : 0.0;
-          c[2] = (i < len - 1) ? p.lc[r] : 0.0;
-          c[3] = p.sv[r];
-          c[4] = p.sw[r];
+          c[CO_LD] = p.ld[r];
+          c[CO_AR] = (i > 0) ? p.a_raw[r] : 0.0;
+          c[CO_LC] = (i < len - 1) ? p.lc[r] : 0.0;
+          c[CO_SV] = p.sv[r];
+          c[CO_SW] = p.sw[r];
         }
-        if (!p.metric.empty()) c[5] = p.metric[r];
+        if (!p.metric.empty()) c[CO_MET] = p.metric[r];
       }
     }
     if (!(e = up(co, &dp->co)).empty()) return e;
@@ -399,18 +399,18 @@ __device__ __forceinline__ void load_window(const KArgs<T>& P, long base, unsign
   transform_in<T, W>(P, f, g, [&](int t) { return ld_at(P.Cm + ro[t], ri); });
 }
 
-// Chunk-local Thomas sweeps over all kChunk rows of chunk q; returns the
-// chunk's first and last values.  The coefficient rows past the chunk's
-// length are identity rows (and the chunk's last row does not couple to the
-// next), so the sweeps need no row guards.
+// Chunk-local Thomas sweeps over all kChunk rows of a chunk whose
+// coefficient rows start at c; returns the chunk's first and last values.
+// The coefficient rows past the chunk's length are identity rows (and the
+// chunk's last row does not couple to the next), so the sweeps need no row
+// guards.
 template <typename T>
-__device__ __forceinline__ void chunk_solve(const KArgs<T>& P, int q, int len, T (&y)[kChunk], T& yf, T& yl) {
-  const T* c = P.co + (long)q * kChunk * kCo;
-  y[0] = y[0] * c[0];
+__device__ __forceinline__ void chunk_solve(const T* c, int len, T (&y)[kChunk], T& yf, T& yl) {
+  y[0] = y[0] * c[CO_LD];
 #pragma unroll
-  for (int i = 1; i < kChunk; ++i) y[i] = (y[i] - c[i * kCo + 1] * y[i - 1]) * c[i * kCo + 0];
+  for (int i = 1; i < kChunk; ++i) y[i] = (y[i] - c[i * kCo + CO_AR] * y[i - 1]) * c[i * kCo + CO_LD];
 #pragma unroll
-  for (int i = kChunk - 2; i >= 0; --i) y[i] = y[i] - c[i * kCo + 2] * y[i + 1];
+  for (int i = kChunk - 2; i >= 0; --i) y[i] = y[i] - c[i * kCo + CO_LC] * y[i + 1];
   yf = y[0];
   yl = y[0];
 #pragma unroll
@@ -418,14 +418,16 @@ __device__ __forceinline__ void chunk_solve(const KArgs<T>& P, int q, int len, T
     if (i == len - 1) yl = y[i];
 }
 
-// Reduced-system values coupling chunk q to its neighbours; ex(k) reads the
-// published first/last value k (0..2P-1) of the thread's line.
+// Reduced-system values coupling chunk q to its neighbours (rinv: the
+// 2P x 2P inverse; ex(k) reads the published first/last value k of the
+// thread's line), applied through the spike vectors of the chunk's rows c.
 template <typename T, class EX>
-__device__ __forceinline__ void chunk_couple(const KArgs<T>& P, int q, EX&& ex, T (&y)[kChunk]) {
+__device__ __forceinline__ void chunk_couple(const KArgs<T>& P, int q, const T* rinv, const T* c, EX&& ex,
+                                             T (&y)[kChunk]) {
   const int Pn = P.P, M = 2 * Pn;
   const int qm = (q + Pn - 1) % Pn, qp = (q + 1) % Pn;
-  const T* rL = P.rinv + (long)(2 * qm + 1) * M;
-  const T* rF = P.rinv + (long)(2 * qp) * M;
+  const T* rL = rinv + (2 * qm + 1) * M;
+  const T* rF = rinv + (2 * qp) * M;
   T uL = T(0), uF = T(0);
   for (int k = 0; k < M; ++k) {
     const T v = ex(k);
@@ -434,19 +436,26 @@ __device__ __forceinline__ void chunk_couple(const KArgs<T>& P, int q, EX&& ex, 
   }
   if (!P.per && q == 0) uL = T(0);
   if (!P.per && q == Pn - 1) uF = T(0);
-  const T* c = P.co + (long)q * kChunk * kCo;
 #pragma unroll
-  for (int i = 0; i < kChunk; ++i) y[i] = y[i] - c[i * kCo + 3] * uL - c[i * kCo + 4] * uF;
+  for (int i = 0; i < kChunk; ++i) y[i] = y[i] - c[i * kCo + CO_SV] * uL - c[i * kCo + CO_SW] * uF;
 }
 
-// Row i of chunk q (line row r): metric scaling and Neumann overrides.
+// Row i of a chunk (line row r): metric scaling and Neumann overrides.
 template <typename T>
-__device__ __forceinline__ T chunk_post(const KArgs<T>& P, int q, int i, int r, T val, const T (&fb4)[4]) {
-  val = val * P.co[((long)q * kChunk + i) * kCo + 5];
+__device__ __forceinline__ T chunk_post(const KArgs<T>& P, const T* c, int i, int r, T val, const T (&fb4)[4]) {
+  val = val * c[i * kCo + CO_MET];
   if (r == 0 && P.neu_lo) val = fb4[0];
   if (r == P.n_out - 1 && P.neu_hi) val = fb4[1];
   return val;
 }
+
+// Per-plan coefficient data a y/z workgroup copies into LDS: the chunk rows
+// (P x kChunk x kCo) and the reduced-system inverse (2P x 2P), at most
+// kStageMax values per thread (P <= kMaxChunks).
+constexpr int kStageMax = 3;
+static_assert(64 * kMaxChunks * kStageMax >= kMaxChunks * kChunk * kCo + 4 * kMaxChunks * kMaxChunks &&
+                  64 * kStageMax >= kChunk * kCo + 4,
+              "per-plan staging exceeds kStageMax values per thread");
 
 // First elements of y or z line L in the input, output and fbc arrays.
 template <int DIR, typename T>
@@ -502,8 +511,9 @@ __device__ __forceinline__ void chunk_rhs_bulk(const KArgs<T>& P, const T (&f)[k
 // chunk, whose window lies inside the line and whose rows all use the bulk
 // stencil.
 template <bool EDGE, typename T, int DIR, int OPK>
-__device__ __forceinline__ void yz_chunk(const KArgs<T>& P, T* ex, int lane, int q, int s0, int len, bool live,
-                                         long ib0, long ob0, long fb, unsigned ri, unsigned ro, long is, long os) {
+__device__ __forceinline__ void yz_chunk(const KArgs<T>& P, T* ex, T* sco, const T (&cv)[kStageMax], int lane, int q,
+                                         int s0, int len, bool live, long ib0, long ob0, long fb, unsigned ri,
+                                         unsigned ro, long is, long os) {
   const int n_out = P.n_out;
   T f[kChunk + 4];
   T fb4[4] = {T(0), T(0), T(0), T(0)};
@@ -512,18 +522,30 @@ __device__ __forceinline__ void yz_chunk(const KArgs<T>& P, T* ex, int lane, int
     load_window<T>(P, ib0, ri, is, s0, f);
     if (P.fbc && (q == 0 || q == P.P - 1))
       for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[fb + is * m];
-    chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
   } else {
     load_window_interior<T>(P, ib0 + is * (s0 - 2), ri, is, f);
-    chunk_rhs_bulk<T, OPK>(P, f, y);
   }
+  // the per-plan coefficients (loaded before the window) go to LDS
+  const int nst = P.P * kChunk * kCo + (P.rinv ? 4 * P.P * P.P : 0);
+#pragma unroll
+  for (int u = 0; u < kStageMax; ++u) {
+    const int idx = threadIdx.x + u * 64 * P.P;
+    if (idx < nst) sco[idx] = cv[u];
+  }
+  __syncthreads();
+  const T* c = sco + q * kChunk * kCo;
+  const T* rinv = sco + P.P * kChunk * kCo;
+  if (EDGE)
+    chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
+  else
+    chunk_rhs_bulk<T, OPK>(P, f, y);
   if (P.solve) {
     T yf, yl;
-    chunk_solve<T>(P, q, len, y, yf, yl);
+    chunk_solve<T>(c, len, y, yf, yl);
     ex[lane + 64 * (2 * q)] = yf;
     ex[lane + 64 * (2 * q + 1)] = yl;
     __syncthreads();
-    chunk_couple<T>(P, q, [&](int k) { return ex[lane + 64 * k]; }, y);
+    chunk_couple<T>(P, q, rinv, c, [&](int k) { return ex[lane + 64 * k]; }, y);
   }
   if (!live) return;
   T* const Ob = P.out + ob0;
@@ -541,7 +563,7 @@ __device__ __forceinline__ void yz_chunk(const KArgs<T>& P, T* ex, int lane, int
   for (int i = 0; i < kChunk; ++i) {
     if (i < len) {
       const int r = s0 + i;
-      const T val = EDGE ? chunk_post<T>(P, q, i, r, y[i], fb4) : y[i] * P.co[((long)q * kChunk + i) * kCo + 5];
+      const T val = EDGE ? chunk_post<T>(P, c, i, r, y[i], fb4) : y[i] * c[i * kCo + CO_MET];
       st_at<T>(Ob + os * r, ro, prev[i] + val * P.sOut);
     }
   }
@@ -550,6 +572,20 @@ __device__ __forceinline__ void yz_chunk(const KArgs<T>& P, T* ex, int lane, int
 template <typename T, int DIR, int OPK>
 __global__ void __launch_bounds__(64 * kMaxChunks) k_line_yz(KArgs<T> P) {
   __shared__ T ex[2 * kMaxChunks * 64];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sco = reinterpret_cast<T*>(smem_raw);  // P*kChunk*kCo chunk rows, then the 2P x 2P inverse
+  // issue the loads of the per-plan coefficients first (written to LDS once
+  // the chunk's window loads are in flight)
+  T cv[kStageMax];
+  {
+    // (plans without a line solve, the explicit schemes, have no inverse)
+    const int nco = P.P * kChunk * kCo, nst = nco + (P.rinv ? 4 * P.P * P.P : 0);
+#pragma unroll
+    for (int u = 0; u < kStageMax; ++u) {
+      const int idx = threadIdx.x + u * 64 * P.P;
+      cv[u] = (idx < nco) ? P.co[idx] : (idx < nst ? P.rinv[idx - nco] : T(0));
+    }
+  }
   const int lane = threadIdx.x & 63;
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int L0 = blockIdx.x * 64;  // nlines < 2^31 (checked at launch)
@@ -568,9 +604,9 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_yz(KArgs<T> P) {
   const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
   const bool edge = (q == 0) || (q == Pn - 1) || (s0 < 2) || (s0 + kChunk + 1 > n_in - 1);
   if (edge)
-    yz_chunk<true, T, DIR, OPK>(P, ex, lane, q, s0, len, live, ib0, ob0, fb, ri, ro, is, os);
+    yz_chunk<true, T, DIR, OPK>(P, ex, sco, cv, lane, q, s0, len, live, ib0, ob0, fb, ri, ro, is, os);
   else
-    yz_chunk<false, T, DIR, OPK>(P, ex, lane, q, s0, len, live, ib0, ob0, fb, ri, ro, is, os);
+    yz_chunk<false, T, DIR, OPK>(P, ex, sco, cv, lane, q, s0, len, live, ib0, ob0, fb, ri, ro, is, os);
 }
 
 __device__ __forceinline__ int lds_pos(int k) { return k + (k >> 4); }
@@ -616,18 +652,18 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
   chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
   if (P.solve) {
     T yf, yl;
-    chunk_solve<T>(P, q, len, y, yf, yl);
+    chunk_solve<T>(P.co + q * kChunk * kCo, len, y, yf, yl);
     T* ex = exb + (long)ll * 2 * Pn;
     ex[2 * q] = yf;
     ex[2 * q + 1] = yl;
     __syncthreads();
-    chunk_couple<T>(P, q, [&](int k) { return ex[k]; }, y);
+    chunk_couple<T>(P, q, P.rinv, P.co + q * kChunk * kCo, [&](int k) { return ex[k]; }, y);
   }
   __syncthreads();  // every thread has finished reading its input rows
   T* orow = tile + (long)ll * LS;
 #pragma unroll
   for (int i = 0; i < kChunk; ++i)
-    if (i < len) orow[lds_pos(s0 + i)] = chunk_post<T>(P, q, i, s0 + i, y[i], fb4);
+    if (i < len) orow[lds_pos(s0 + i)] = chunk_post<T>(P, P.co + q * kChunk * kCo, i, s0 + i, y[i], fb4);
   __syncthreads();
   for (int e = tid; e < NL * n_out; e += nthr) {
     const int l = e / n_out, k = e - l * n_out;
@@ -655,47 +691,51 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
   const int q = __builtin_amdgcn_readfirstlane(tid >> 6);
   const long L0 = (long)blockIdx.x * 64;
   const int n_in = P.n_in, n_out = P.n_out;
-  // staging: batches of UB elements per thread, all loads of a batch issued
-  // before any use (line indices clamped so every address is valid)
+  // staging: element e = l * n_in + k of the workgroup's 64 lines, thread
+  // tid taking e = tid, tid + nthr, ...; (l, k) advance by a fixed step, so
+  // no division per element.  Batches of UB elements, all loads of a batch
+  // issued before any use; the line is clamped so every address is valid.
   constexpr int UB = 8;
   const long Lmax = P.nlines - 1;
-  const int tin = 64 * n_in;
-  for (int b0 = 0; b0 < tin; b0 += nthr * UB) {
-    long o[UB];
-    T v[UB];
+  const int lmax = (int)((Lmax - L0) < 63 ? (Lmax - L0) : 63);  // last line of the workgroup
+  const T* const Ab = P.A + L0 * P.e1;
+  const T* const Bb = P.B ? P.B + L0 * P.e1 : nullptr;
+  const T* const Cb = P.Cm ? P.Cm + L0 * P.e1 : nullptr;
+  {
+    const int dl = nthr / n_in, dk = nthr - dl * n_in;
+    int l = tid / n_in, k = tid - l * n_in;
+    const int tin = 64 * n_in;
+    for (int b0 = 0; b0 < tin; b0 += nthr * UB) {
+      unsigned o[UB];
+      int ll[UB], kk[UB];
 #pragma unroll
-    for (int u = 0; u < UB; ++u) {
-      int e = b0 + tid + u * nthr;
-      e = e < tin ? e : tin - 1;
-      const int l = e / n_in, k = e - l * n_in;
-      const long L = (L0 + l) < Lmax ? (L0 + l) : Lmax;
-      o[u] = L * P.e1 + k;
-    }
-#pragma unroll
-    for (int u = 0; u < UB; ++u) v[u] = P.A[o[u]];
-    if (P.B) {
-      T g[UB];
-#pragma unroll
-      for (int u = 0; u < UB; ++u) g[u] = P.B[o[u]];
-#pragma unroll
-      for (int u = 0; u < UB; ++u) v[u] = v[u] + P.cB * g[u];
-    }
-#pragma unroll
-    for (int u = 0; u < UB; ++u) v[u] = P.sA * v[u];
-    if (P.Cm) {
-      T g[UB];
-#pragma unroll
-      for (int u = 0; u < UB; ++u) g[u] = P.Cm[o[u]];
-#pragma unroll
-      for (int u = 0; u < UB; ++u) v[u] = v[u] * g[u];
-    }
-#pragma unroll
-    for (int u = 0; u < UB; ++u) {
-      const int e = b0 + tid + u * nthr;
-      if (e < tin) {
-        const int l = e / n_in, k = e - l * n_in;
-        tile[l * LS + k] = v[u];
+      for (int u = 0; u < UB; ++u) {
+        ll[u] = l;
+        kk[u] = k;
+        o[u] = (unsigned)(((l < lmax ? l : lmax) * P.e1 + k) * (int)sizeof(T));
+        k += dk;
+        l += dl;
+        if (k >= n_in) { k -= n_in; ++l; }
       }
+      T v[UB], g[UB];
+      const T* S = Bb ? Bb : Cb;
+      if (S) {
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+          v[u] = ld_at(Ab, o[u]);
+          g[u] = ld_at(S, o[u]);
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+          v[u] = ld_at(Ab, o[u]);
+          g[u] = T(0);
+        }
+      }
+      transform_in<T, UB>(P, v, g, [&](int u) { return ld_at(Cb, o[u]); });
+#pragma unroll
+      for (int u = 0; u < UB; ++u)
+        if (ll[u] < 64) tile[ll[u] * LS + kk[u]] = v[u];
     }
   }
   __syncthreads();
@@ -713,39 +753,44 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
   chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
   if (P.solve) {
     T yf, yl;
-    chunk_solve<T>(P, q, len, y, yf, yl);
+    chunk_solve<T>(P.co + q * kChunk * kCo, len, y, yf, yl);
     exb[lane + 64 * (2 * q)] = yf;
     exb[lane + 64 * (2 * q + 1)] = yl;
     __syncthreads();
-    chunk_couple<T>(P, q, [&](int k) { return exb[lane + 64 * k]; }, y);
+    chunk_couple<T>(P, q, P.rinv, P.co + q * kChunk * kCo, [&](int k) { return exb[lane + 64 * k]; }, y);
   }
   __syncthreads();  // every wave has finished reading its input rows
 #pragma unroll
   for (int i = 0; i < kChunk; ++i)
-    if (i < len) row[s0 + i] = chunk_post<T>(P, q, i, s0 + i, y[i], fb4);
+    if (i < len) row[s0 + i] = chunk_post<T>(P, P.co + q * kChunk * kCo, i, s0 + i, y[i], fb4);
   __syncthreads();
-  const int tout = 64 * n_out;
-  for (int b0 = 0; b0 < tout; b0 += nthr * UB) {
-    long o[UB];
-    T val[UB], prev[UB];
+  {
+    const int dl = nthr / n_out, dk = nthr - dl * n_out;
+    int l = tid / n_out, k = tid - l * n_out;
+    const int tout = 64 * n_out;
+    T* const Ob = P.out + L0 * P.o1;
+    for (int b0 = 0; b0 < tout; b0 += nthr * UB) {
+      unsigned o[UB];
+      bool ok[UB];
+      T val[UB], prev[UB];
 #pragma unroll
-    for (int u = 0; u < UB; ++u) {
-      int e = b0 + tid + u * nthr;
-      e = e < tout ? e : tout - 1;
-      const int l = e / n_out, k = e - l * n_out;
-      const long Lw = (L0 + l) < Lmax ? (L0 + l) : Lmax;
-      o[u] = Lw * P.o1 + k;
-      val[u] = tile[l * LS + k];
-      prev[u] = T(0);
-    }
-    if (P.accumulate) {
+      for (int u = 0; u < UB; ++u) {
+        ok[u] = (l <= lmax);
+        const int lc = l < lmax ? l : lmax;
+        o[u] = (unsigned)((lc * P.o1 + k) * (int)sizeof(T));
+        val[u] = tile[lc * LS + k];
+        prev[u] = T(0);
+        k += dk;
+        l += dl;
+        if (k >= n_out) { k -= n_out; ++l; }
+      }
+      if (P.accumulate) {
 #pragma unroll
-      for (int u = 0; u < UB; ++u) prev[u] = P.out[o[u]];
-    }
+        for (int u = 0; u < UB; ++u) prev[u] = ld_at<T>(Ob, o[u]);
+      }
 #pragma unroll
-    for (int u = 0; u < UB; ++u) {
-      const int e = b0 + tid + u * nthr;
-      if (e < tout && L0 + e / n_out <= Lmax) P.out[o[u]] = prev[u] + val[u] * P.sOut;
+      for (int u = 0; u < UB; ++u)
+        if (ok[u]) st_at<T>(Ob, o[u], prev[u] + val[u] * P.sOut);
     }
   }
 }
@@ -788,12 +833,12 @@ static void dispatch_part(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, con
     }
   } else {
     switch (op) {
-      case MIDP_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_C2P>), g, b, 0, s, a); break;
-      case MIDP_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_P2C>), g, b, 0, s, a); break;
-      case D1_C2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2C>), g, b, 0, s, a); break;
-      case D1_P2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2P>), g, b, 0, s, a); break;
-      case D1_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2P>), g, b, 0, s, a); break;
-      case D1_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2C>), g, b, 0, s, a); break;
+      case MIDP_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_C2P>), g, b, shm, s, a); break;
+      case MIDP_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_P2C>), g, b, shm, s, a); break;
+      case D1_C2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2C>), g, b, shm, s, a); break;
+      case D1_P2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2P>), g, b, shm, s, a); break;
+      case D1_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2P>), g, b, shm, s, a); break;
+      case D1_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2C>), g, b, shm, s, a); break;
     }
   }
 }
@@ -842,6 +887,7 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
     a.P = p.part_P;
     a.per = (p.solve == 2);
     a.rinv = D.rinv;
+    if (p.solve && !D.rinv) return "line op: plan solves but has no reduced-system inverse";
     a.swin = D.swin;
     a.wrap = (p.ibc[0] == BC_PERIODIC);
     a.co = D.co;
@@ -874,8 +920,9 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
           return "line op: x-y plane too large for 32-bit line offsets";
       }
       dim3 g((unsigned)((a.nlines + 63) / 64)), b((unsigned)(64 * p.part_P));
-      if (dir == 1) dispatch_part<T, 1>(p.op, g, b, 0, s, a);
-      else dispatch_part<T, 2>(p.op, g, b, 0, s, a);
+      const size_t shm_yz = sizeof(T) * ((size_t)p.part_P * kChunk * kCo + 4 * (size_t)p.part_P * p.part_P);
+      if (dir == 1) dispatch_part<T, 1>(p.op, g, b, shm_yz, s, a);
+      else dispatch_part<T, 2>(p.op, g, b, shm_yz, s, a);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return std::string("line op launch: ") + hipGetErrorString(e);
