@@ -372,31 +372,29 @@ __device__ __forceinline__ void transform_in(const KArgs<T>& P, T (&f)[NV], cons
 }
 
 // Window of one y/z line chunk: rows s0-2 .. s0+kChunk+1 (wrapped on
-// periodic lines, clamped otherwise, so every address is valid).  The row offsets are wave-uniform; the
-// loads of A and of the second input array are all issued before any use.
-template <typename T>
+// periodic lines, clamped otherwise, so every address is valid).  The row
+// offsets are wave-uniform; the loads of A and (HAS_S) of the second input
+// array (B, or Cm when B is absent) are all issued before any use.
+template <typename T, bool HAS_S>
 __device__ __forceinline__ void load_window(const KArgs<T>& P, long base, unsigned ri, long is, int s0,
                                             T (&f)[kChunk + 4]) {
   constexpr int W = kChunk + 4;
   long ro[W];
 #pragma unroll
   for (int t = 0; t < W; ++t) ro[t] = base + is * window_index(s0 - 2 + t, P.n_in, P.wrap);
-  const T* S = P.B ? P.B : P.Cm;
-  T g[W];
-  if (S) {
+  if (HAS_S) {
+    const T* S = P.B ? P.B : P.Cm;
+    T g[W];
 #pragma unroll
     for (int t = 0; t < W; ++t) {
       f[t] = ld_at(P.A + ro[t], ri);
       g[t] = ld_at(S + ro[t], ri);
     }
+    transform_in<T, W>(P, f, g, [&](int t) { return ld_at(P.Cm + ro[t], ri); });
   } else {
 #pragma unroll
-    for (int t = 0; t < W; ++t) {
-      f[t] = ld_at(P.A + ro[t], ri);
-      g[t] = T(0);
-    }
+    for (int t = 0; t < W; ++t) f[t] = P.sA * ld_at(P.A + ro[t], ri);
   }
-  transform_in<T, W>(P, f, g, [&](int t) { return ld_at(P.Cm + ro[t], ri); });
 }
 
 // Chunk-local Thomas sweeps over all kChunk rows of a chunk whose
@@ -472,114 +470,23 @@ __device__ __forceinline__ void yz_offsets(const KArgs<T>& P, int L, long& ib, l
   }
 }
 
-// Window of an interior chunk (rows s0-2 .. s0+kChunk+1 all inside the
-// line): consecutive rows from r0 = s0 - 2, no clamping.
-template <typename T>
-__device__ __forceinline__ void load_window_interior(const KArgs<T>& P, long r0, unsigned ri, long is,
-                                                     T (&f)[kChunk + 4]) {
-  constexpr int W = kChunk + 4;
-  const T* S = P.B ? P.B : P.Cm;
-  T g[W];
-  if (S) {
-#pragma unroll
-    for (int t = 0; t < W; ++t) {
-      f[t] = ld_at(P.A + (r0 + is * t), ri);
-      g[t] = ld_at(S + (r0 + is * t), ri);
-    }
-  } else {
-#pragma unroll
-    for (int t = 0; t < W; ++t) {
-      f[t] = ld_at(P.A + (r0 + is * t), ri);
-      g[t] = T(0);
-    }
-  }
-  transform_in<T, W>(P, f, g, [&](int t) { return ld_at(P.Cm + (r0 + is * t), ri); });
-}
-
-// Explicit RHS of the kChunk rows of an interior chunk (bulk stencil only).
-template <typename T, int OPK>
-__device__ __forceinline__ void chunk_rhs_bulk(const KArgs<T>& P, const T (&f)[kChunk + 4], T (&y)[kChunk]) {
-  using O = Offs<OPK>;
-#pragma unroll
-  for (int i = 0; i < kChunk; ++i)
-    y[i] = (P.c1 * (f[i + 2 + O::p1] + P.sgn * f[i + 2 + O::p2]) +
-            P.c2 * (f[i + 2 + O::p3] + P.sgn * f[i + 2 + O::p4])) * P.dd;
-}
-
-// One chunk of a y/z workgroup.  EDGE: a first or last chunk (clamped or
-// wrapped window, boundary rows, fbc, Neumann rows); otherwise an interior
-// chunk, whose window lies inside the line and whose rows all use the bulk
-// stencil.
-template <bool EDGE, typename T, int DIR, int OPK>
-__device__ __forceinline__ void yz_chunk(const KArgs<T>& P, T* ex, T* sco, const T (&cv)[kStageMax], int lane, int q,
-                                         int s0, int len, bool live, long ib0, long ob0, long fb, unsigned ri,
-                                         unsigned ro, long is, long os) {
-  const int n_out = P.n_out;
-  T f[kChunk + 4];
-  T fb4[4] = {T(0), T(0), T(0), T(0)};
-  T y[kChunk];
-  if (EDGE) {
-    load_window<T>(P, ib0, ri, is, s0, f);
-    if (P.fbc && (q == 0 || q == P.P - 1))
-      for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[fb + is * m];
-  } else {
-    load_window_interior<T>(P, ib0 + is * (s0 - 2), ri, is, f);
-  }
-  // the per-plan coefficients (loaded before the window) go to LDS
-  const int nst = P.P * kChunk * kCo + (P.rinv ? 4 * P.P * P.P : 0);
-#pragma unroll
-  for (int u = 0; u < kStageMax; ++u) {
-    const int idx = threadIdx.x + u * 64 * P.P;
-    if (idx < nst) sco[idx] = cv[u];
-  }
-  __syncthreads();
-  const T* c = sco + q * kChunk * kCo;
-  const T* rinv = sco + P.P * kChunk * kCo;
-  if (EDGE)
-    chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
-  else
-    chunk_rhs_bulk<T, OPK>(P, f, y);
-  if (P.solve) {
-    T yf, yl;
-    chunk_solve<T>(c, len, y, yf, yl);
-    ex[lane + 64 * (2 * q)] = yf;
-    ex[lane + 64 * (2 * q + 1)] = yl;
-    __syncthreads();
-    chunk_couple<T>(P, q, rinv, c, [&](int k) { return ex[lane + 64 * k]; }, y);
-  }
-  if (!live) return;
-  T* const Ob = P.out + ob0;
-  T prev[kChunk];
-#pragma unroll
-  for (int i = 0; i < kChunk; ++i) prev[i] = T(0);
-  if (P.accumulate) {
-#pragma unroll
-    for (int i = 0; i < kChunk; ++i) {
-      const int r = (s0 + i < n_out) ? s0 + i : n_out - 1;
-      prev[i] = ld_at<T>(Ob + os * r, ro);
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < kChunk; ++i) {
-    if (i < len) {
-      const int r = s0 + i;
-      const T val = EDGE ? chunk_post<T>(P, c, i, r, y[i], fb4) : y[i] * c[i * kCo + CO_MET];
-      st_at<T>(Ob + os * r, ro, prev[i] + val * P.sOut);
-    }
-  }
-}
-
-template <typename T, int DIR, int OPK>
-__global__ void __launch_bounds__(64 * kMaxChunks) k_line_yz(KArgs<T> P) {
+// One y/z workgroup: 64 neighbouring lines x P chunks, wave q holding
+// chunk q of the 64 lines.  Single-input operators ask for 5 waves per SIMD
+// (at most 96 registers), so that two workgroups of 9 waves (lines of
+// 129 rows) share a CU.
+template <typename T, int DIR, int OPK, bool HAS_S>
+__global__ void __launch_bounds__(64 * kMaxChunks) __attribute__((amdgpu_waves_per_eu(HAS_S ? 4 : 5)))
+k_line_yz(KArgs<T> P) {
   __shared__ T ex[2 * kMaxChunks * 64];
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sco = reinterpret_cast<T*>(smem_raw);  // P*kChunk*kCo chunk rows, then the 2P x 2P inverse
-  // issue the loads of the per-plan coefficients first (written to LDS once
-  // the chunk's window loads are in flight)
+  // loads of the per-plan coefficients first (written to LDS once the
+  // chunk's window loads are in flight); plans without a line solve (the
+  // explicit schemes) have no inverse
+  const int nst = P.P * kChunk * kCo + (P.rinv ? 4 * P.P * P.P : 0);
   T cv[kStageMax];
   {
-    // (plans without a line solve, the explicit schemes, have no inverse)
-    const int nco = P.P * kChunk * kCo, nst = nco + (P.rinv ? 4 * P.P * P.P : 0);
+    const int nco = P.P * kChunk * kCo;
 #pragma unroll
     for (int u = 0; u < kStageMax; ++u) {
       const int idx = threadIdx.x + u * 64 * P.P;
@@ -600,13 +507,57 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_yz(KArgs<T> P) {
   const unsigned ro = (unsigned)((ob - ob0) * (long)sizeof(T));
   const long is = (DIR == 1) ? (long)P.e1 : (long)P.e1 * P.e2;
   const long os = (DIR == 1) ? (long)P.o1 : (long)P.o1 * P.o2;
-  const int n_in = P.n_in, n_out = P.n_out, Pn = P.P;
+  const int n_out = P.n_out, Pn = P.P;
   const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
-  const bool edge = (q == 0) || (q == Pn - 1) || (s0 < 2) || (s0 + kChunk + 1 > n_in - 1);
-  if (edge)
-    yz_chunk<true, T, DIR, OPK>(P, ex, sco, cv, lane, q, s0, len, live, ib0, ob0, fb, ri, ro, is, os);
-  else
-    yz_chunk<false, T, DIR, OPK>(P, ex, sco, cv, lane, q, s0, len, live, ib0, ob0, fb, ri, ro, is, os);
+  T f[kChunk + 4];
+  load_window<T, HAS_S>(P, ib0, ri, is, s0, f);
+  T fb4[4] = {T(0), T(0), T(0), T(0)};
+  if (P.fbc && (q == 0 || q == Pn - 1))
+    for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[fb + is * m];
+#pragma unroll
+  for (int u = 0; u < kStageMax; ++u) {
+    const int idx = threadIdx.x + u * 64 * Pn;
+    if (idx < nst) sco[idx] = cv[u];
+  }
+  __syncthreads();
+  const T* c = sco + q * kChunk * kCo;
+  const T* rinv = sco + Pn * kChunk * kCo;
+  T y[kChunk];
+  chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
+  if (P.solve) {
+    T yf, yl;
+    chunk_solve<T>(c, len, y, yf, yl);
+    ex[lane + 64 * (2 * q)] = yf;
+    ex[lane + 64 * (2 * q + 1)] = yl;
+    __syncthreads();
+    chunk_couple<T>(P, q, rinv, c, [&](int k) { return ex[lane + 64 * k]; }, y);
+  }
+  if (!live) return;
+  T* const Ob = P.out + ob0;
+  // accumulating outputs read the previous values in two batches of half a
+  // chunk (fewer registers held across the loads)
+  constexpr int H = kChunk / 2;
+#pragma unroll
+  for (int h = 0; h < kChunk; h += H) {
+    T prev[H];
+#pragma unroll
+    for (int i = 0; i < H; ++i) prev[i] = T(0);
+    if (P.accumulate) {
+#pragma unroll
+      for (int i = 0; i < H; ++i) {
+        const int r = (s0 + h + i < n_out) ? s0 + h + i : n_out - 1;
+        prev[i] = ld_at<T>(Ob + os * r, ro);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      if (h + i < len) {
+        const int r = s0 + h + i;
+        const T val = chunk_post<T>(P, c, h + i, r, y[h + i], fb4);
+        st_at<T>(Ob + os * r, ro, prev[i] + val * P.sOut);
+      }
+    }
+  }
 }
 
 __device__ __forceinline__ int lds_pos(int k) { return k + (k >> 4); }
@@ -680,12 +631,27 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
 // (row stride odd, so the 64 lanes reading one column of their own lines hit
 // distinct banks); wave q of the workgroup processes chunk q of all 64 lines,
 // which keeps the plan's per-row coefficients in scalar registers.
-template <typename T, int OPK>
+template <typename T, int OPK, bool HAS_S>
 __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* tile = reinterpret_cast<T*>(smem_raw);
   const int LS = P.ls, Pn = P.P;
-  T* exb = tile + (long)64 * LS;
+  // after the windows are read, the start of the tile holds the exchanged
+  // first/last chunk values (64 x 2P)
+  T* exb = tile;
+  // per-plan coefficients (chunk rows, then the reduced-system inverse) after
+  // the tile; their loads are issued first and written once staging is done
+  T* sco = tile + (long)64 * LS;
+  const int nst = Pn * kChunk * kCo + (P.rinv ? 4 * Pn * Pn : 0);
+  T cv[kStageMax];
+  {
+    const int nco = Pn * kChunk * kCo;
+#pragma unroll
+    for (int u = 0; u < kStageMax; ++u) {
+      const int idx = threadIdx.x + u * 64 * Pn;
+      cv[u] = (idx < nco) ? P.co[idx] : (idx < nst ? P.rinv[idx - nco] : T(0));
+    }
+  }
   const int tid = threadIdx.x, nthr = 64 * Pn;
   const int lane = tid & 63;
   const int q = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -717,28 +683,33 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
         l += dl;
         if (k >= n_in) { k -= n_in; ++l; }
       }
-      T v[UB], g[UB];
-      const T* S = Bb ? Bb : Cb;
-      if (S) {
+      T v[UB];
+      if (HAS_S) {
+        const T* S = Bb ? Bb : Cb;
+        T g[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
           v[u] = ld_at(Ab, o[u]);
           g[u] = ld_at(S, o[u]);
         }
+        transform_in<T, UB>(P, v, g, [&](int u) { return ld_at(Cb, o[u]); });
       } else {
 #pragma unroll
-        for (int u = 0; u < UB; ++u) {
-          v[u] = ld_at(Ab, o[u]);
-          g[u] = T(0);
-        }
+        for (int u = 0; u < UB; ++u) v[u] = P.sA * ld_at(Ab, o[u]);
       }
-      transform_in<T, UB>(P, v, g, [&](int u) { return ld_at(Cb, o[u]); });
 #pragma unroll
       for (int u = 0; u < UB; ++u)
         if (ll[u] < 64) tile[ll[u] * LS + kk[u]] = v[u];
     }
   }
+#pragma unroll
+  for (int u = 0; u < kStageMax; ++u) {
+    const int idx = threadIdx.x + u * 64 * Pn;
+    if (idx < nst) sco[idx] = cv[u];
+  }
   __syncthreads();
+  const T* c = sco + q * kChunk * kCo;
+  const T* rinv = sco + Pn * kChunk * kCo;
   const long L = L0 + lane;
   const bool live = L < P.nlines;
   T* row = tile + lane * LS;
@@ -753,16 +724,17 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
   chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
   if (P.solve) {
     T yf, yl;
-    chunk_solve<T>(P.co + q * kChunk * kCo, len, y, yf, yl);
+    chunk_solve<T>(c, len, y, yf, yl);
+    __syncthreads();  // every wave has read its window: the tile's start takes the exchange
     exb[lane + 64 * (2 * q)] = yf;
     exb[lane + 64 * (2 * q + 1)] = yl;
     __syncthreads();
-    chunk_couple<T>(P, q, P.rinv, P.co + q * kChunk * kCo, [&](int k) { return exb[lane + 64 * k]; }, y);
+    chunk_couple<T>(P, q, rinv, c, [&](int k) { return exb[lane + 64 * k]; }, y);
   }
-  __syncthreads();  // every wave has finished reading its input rows
+  __syncthreads();  // every wave has finished reading its window and the exchange
 #pragma unroll
   for (int i = 0; i < kChunk; ++i)
-    if (i < len) row[s0 + i] = chunk_post<T>(P, P.co + q * kChunk * kCo, i, s0 + i, y[i], fb4);
+    if (i < len) row[s0 + i] = chunk_post<T>(P, c, i, s0 + i, y[i], fb4);
   __syncthreads();
   {
     const int dl = nthr / n_out, dk = nthr - dl * n_out;
@@ -795,16 +767,22 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
   }
 }
 
+template <typename T, bool HAS_S>
+static void dispatch_xu_s(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
+  switch (op) {
+    case MIDP_C2P: hipLaunchKernelGGL((k_line_xu<T, MIDP_C2P, HAS_S>), g, b, shm, s, a); break;
+    case MIDP_P2C: hipLaunchKernelGGL((k_line_xu<T, MIDP_P2C, HAS_S>), g, b, shm, s, a); break;
+    case D1_C2C: hipLaunchKernelGGL((k_line_xu<T, D1_C2C, HAS_S>), g, b, shm, s, a); break;
+    case D1_P2P: hipLaunchKernelGGL((k_line_xu<T, D1_P2P, HAS_S>), g, b, shm, s, a); break;
+    case D1_C2P: hipLaunchKernelGGL((k_line_xu<T, D1_C2P, HAS_S>), g, b, shm, s, a); break;
+    case D1_P2C: hipLaunchKernelGGL((k_line_xu<T, D1_P2C, HAS_S>), g, b, shm, s, a); break;
+  }
+}
+
 template <typename T>
 static void dispatch_xu(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
-  switch (op) {
-    case MIDP_C2P: hipLaunchKernelGGL((k_line_xu<T, MIDP_C2P>), g, b, shm, s, a); break;
-    case MIDP_P2C: hipLaunchKernelGGL((k_line_xu<T, MIDP_P2C>), g, b, shm, s, a); break;
-    case D1_C2C: hipLaunchKernelGGL((k_line_xu<T, D1_C2C>), g, b, shm, s, a); break;
-    case D1_P2P: hipLaunchKernelGGL((k_line_xu<T, D1_P2P>), g, b, shm, s, a); break;
-    case D1_C2P: hipLaunchKernelGGL((k_line_xu<T, D1_C2P>), g, b, shm, s, a); break;
-    case D1_P2C: hipLaunchKernelGGL((k_line_xu<T, D1_P2C>), g, b, shm, s, a); break;
-  }
+  if (a.B || a.Cm) dispatch_xu_s<T, true>(op, g, b, shm, s, a);
+  else dispatch_xu_s<T, false>(op, g, b, shm, s, a);
 }
 
 // LDS bytes a workgroup may use, queried once per device.
@@ -820,6 +798,18 @@ static size_t max_lds_per_block() {
   return cached[dev];
 }
 
+template <typename T, int DIR, bool HAS_S>
+static void dispatch_yz(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
+  switch (op) {
+    case MIDP_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_C2P, HAS_S>), g, b, shm, s, a); break;
+    case MIDP_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_P2C, HAS_S>), g, b, shm, s, a); break;
+    case D1_C2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2C, HAS_S>), g, b, shm, s, a); break;
+    case D1_P2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2P, HAS_S>), g, b, shm, s, a); break;
+    case D1_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2P, HAS_S>), g, b, shm, s, a); break;
+    case D1_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2C, HAS_S>), g, b, shm, s, a); break;
+  }
+}
+
 template <typename T, int DIR>
 static void dispatch_part(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
   if (DIR == 0) {
@@ -831,15 +821,10 @@ static void dispatch_part(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, con
       case D1_C2P: hipLaunchKernelGGL((k_line_x<T, D1_C2P>), g, b, shm, s, a); break;
       case D1_P2C: hipLaunchKernelGGL((k_line_x<T, D1_P2C>), g, b, shm, s, a); break;
     }
+  } else if (a.B || a.Cm) {
+    dispatch_yz<T, DIR, true>(op, g, b, shm, s, a);
   } else {
-    switch (op) {
-      case MIDP_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_C2P>), g, b, shm, s, a); break;
-      case MIDP_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_P2C>), g, b, shm, s, a); break;
-      case D1_C2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2C>), g, b, shm, s, a); break;
-      case D1_P2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2P>), g, b, shm, s, a); break;
-      case D1_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2P>), g, b, shm, s, a); break;
-      case D1_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2C>), g, b, shm, s, a); break;
-    }
+    dispatch_yz<T, DIR, false>(op, g, b, shm, s, a);
   }
 }
 
@@ -893,7 +878,10 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
     a.co = D.co;
     const int nmax = p.n_in > p.n_out ? p.n_in : p.n_out;
     const int lsu = nmax | 1;  // odd row stride
-    const size_t shm_u = sizeof(T) * ((size_t)64 * lsu + (size_t)64 * 2 * p.part_P);
+    // tile of 64 lines (its start later holds the 64 x 2P exchange) and the
+    // per-plan coefficients
+    const size_t shm_u = sizeof(T) * ((size_t)64 * lsu + (size_t)p.part_P * kChunk * kCo +
+                                      4 * (size_t)p.part_P * p.part_P);
     if (dir == 0 && shm_u <= max_lds_per_block()) {
       a.ls = lsu;
       dim3 g((unsigned)((a.nlines + 63) / 64)), b((unsigned)(64 * p.part_P));
