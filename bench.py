@@ -67,6 +67,25 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n * 8, "copies": 5}))
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_channel_256x128x256.md")
+
+
+def measured_traffic(form, direction, nc):
+    """HBM bytes per launch of the line operators along `direction`, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
+    (copy-calibrated); None when the summary does not cover the run."""
+    if tuple(nc) != NC or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == form and cells[1] == direction:
+            try:
+                return float(cells[5]) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(nc, fields, cfg):
     """The oracle's restatement of the reference algorithm (FP64, single
     2nd-order projection per sub-step) on the host cores: one RK3 step."""
@@ -286,12 +305,13 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"compact line operators along {'xyz'[args.profile_class]} (k_line)",
+            "kernel": f"compact line operators along {'xyz'[args.profile_class]} "
+                      f"({'k_line_xu' if args.profile_class == 0 else 'k_line_yz'})",
             "achieved": achieved,
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": (achieved / PEAK_HBM_GBS) if achieved else None,
-            "traffic": None,
+            "traffic": measured_traffic(args.form, "xyz"[args.profile_class], nc),
             "launches": nl.value,
             "avg_launch_ms": ms.value / max(nl.value, 1),
         },
