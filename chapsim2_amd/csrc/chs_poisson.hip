@@ -252,6 +252,298 @@ __global__ void __launch_bounds__(kThreads) k_fft_s(cx<T>* __restrict__ W, int n
   }
 }
 
+// ---------------------------------------------------------------------------
+// Two-pass transforms for N = A * B in {64, 128, 256, 512}.  With n = a + A b
+// and k = k2 + B k1 (a, k1 < A; b, k2 < B):
+//   X[k2 + B k1] = sum_a W_A^(a k1) [ W_N^(a k2) sum_b x[a + A b] W_B^(b k2) ].
+// A threads per line: thread a loads x[a + A b] for all b and does the
+// B-point DFT in registers, applies W_N^(a k2) (table, index a k2 < N), and
+// writes to LDS (rows of A + 1 entries, so the A-point reads of pass 2 hit
+// distinct banks); after one barrier, thread a does the A-point DFTs for
+// k2 = a, a + A, ... in registers.  The register DFTs are radix-2 with
+// compile-time twiddles W_32^m = (-i)^(m/8) W_32^(m%8) from the first
+// octant below, exactly symmetric.
+__device__ constexpr double kOctC[8] = {1.0, 0.9807852804032304, 0.9238795325112867, 0.8314696123025452,
+                                        0.7071067811865476, 0.5555702330196022, 0.3826834323650898,
+                                        0.19509032201612825};
+__device__ constexpr double kOctS[8] = {0.0, 0.19509032201612825, 0.3826834323650898, 0.5555702330196022,
+                                        0.7071067811865476, 0.8314696123025452, 0.9238795325112867,
+                                        0.9807852804032304};
+
+// v * W_32^m, W = exp(-2 pi i / 32) (forward) or its conjugate (inverse)
+template <typename T, bool FWD>
+__device__ __forceinline__ cx<T> tw32(cx<T> v, int m) {
+  m &= 31;
+  const int q = m >> 3, r = m & 7;
+  cx<T> t = v;
+  if (r != 0) {
+    const T c = T(kOctC[r]), sn = FWD ? T(-kOctS[r]) : T(kOctS[r]);
+    t = {v.re * c - v.im * sn, v.re * sn + v.im * c};
+  }
+  // times (-i)^q (forward) or (+i)^q (inverse)
+  if (q == 1) t = FWD ? cx<T>{t.im, -t.re} : cx<T>{-t.im, t.re};
+  if (q == 2) t = {-t.re, -t.im};
+  if (q == 3) t = FWD ? cx<T>{-t.im, t.re} : cx<T>{t.im, -t.re};
+  return t;
+}
+
+__host__ __device__ constexpr int bitrev_c(int i, int R) {
+  int r = 0;
+  for (int b = 1; b < R; b <<= 1) {
+    r = (r << 1) | (i & 1);
+    i >>= 1;
+  }
+  return r;
+}
+
+// In-register DFT of R = 2^p <= 32 points (radix-2, decimation in time).
+template <typename T, int R, bool FWD>
+__device__ __forceinline__ void dft_reg(cx<T> (&v)[R]) {
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int j = bitrev_c(i, R);
+    if (j > i) {
+      const cx<T> t = v[i];
+      v[i] = v[j];
+      v[j] = t;
+    }
+  }
+#pragma unroll
+  for (int len = 2; len <= R; len <<= 1) {
+#pragma unroll
+    for (int i = 0; i < R; i += len) {
+#pragma unroll
+      for (int k = 0; k < len / 2; ++k) {
+        const cx<T> u = v[i + k];
+        const cx<T> t = tw32<T, FWD>(v[i + k + len / 2], k * (32 / len));
+        v[i + k] = cadd(u, t);
+        v[i + k + len / 2] = csub(u, t);
+      }
+    }
+  }
+}
+
+// Pass-1 twiddle v[k2] *= W_N^(a k2) (table of exp(-2 pi i q / N)).
+template <typename T, int B, bool FWD>
+__device__ __forceinline__ void twiddle_rows(cx<T> (&v)[B], int a, const cx<T>* __restrict__ twN) {
+#pragma unroll
+  for (int k2 = 1; k2 < B; ++k2) {
+    cx<T> w = twN[a * k2];
+    if (!FWD) w.im = -w.im;
+    v[k2] = cmul(v[k2], w);
+  }
+}
+
+template <int A, int B>
+struct Fft2 {
+  static constexpr int N = A * B;
+  static constexpr int LSL = B * (A + 1) + 1;  // LDS entries per line
+};
+
+// Strided complex transform (as k_fft_s) in two passes; LPW consecutive kx
+// lines per workgroup, thread = l + LPW * a.
+template <typename T, int A, int B, int LPW, bool FWD>
+__global__ void __launch_bounds__(LPW * A) k_fft2_s(cx<T>* __restrict__ W, int nxh, long ls, long os,
+                                                   const cx<T>* __restrict__ twN) {
+  using F = Fft2<A, B>;
+  extern __shared__ unsigned char smem[];
+  cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+  const int ntile = (nxh + LPW - 1) / LPW;
+  const int tile = blockIdx.x % ntile, o = blockIdx.x / ntile;
+  const int l = threadIdx.x % LPW, a = threadIdx.x / LPW;
+  const int kx = tile * LPW + l;
+  const bool ok = kx < nxh;
+  const long base = (long)(ok ? kx : nxh - 1) + os * o;
+  cx<T> v[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) v[b] = W[base + ls * (a + A * b)];
+  dft_reg<T, B, FWD>(v);
+  twiddle_rows<T, B, FWD>(v, a, twN);
+  cx<T>* lb = buf + l * F::LSL;
+#pragma unroll
+  for (int k2 = 0; k2 < B; ++k2) lb[k2 * (A + 1) + a] = v[k2];
+  __syncthreads();
+#pragma unroll
+  for (int g = 0; g < B / A; ++g) {
+    const int k2 = a + A * g;
+    cx<T> u[A];
+#pragma unroll
+    for (int j = 0; j < A; ++j) u[j] = lb[k2 * (A + 1) + j];
+    dft_reg<T, A, FWD>(u);
+    if (ok) {
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) W[base + ls * (k2 + B * k1)] = u[k1];
+    }
+  }
+}
+
+// x forward (real lines of 2N -> spectrum kx = 0..N) in two passes; LPW
+// lines per workgroup, thread = a + A * l (lanes along the line).
+template <typename T, int A, int B, int LPW>
+__global__ void __launch_bounds__(LPW * A) k_fft2_x_fwd(const T* __restrict__ in, cx<T>* __restrict__ W,
+                                                       long nlines, const cx<T>* __restrict__ twN,
+                                                       const cx<T>* __restrict__ rot, T in_scale) {
+  using F = Fft2<A, B>;
+  constexpr int N = F::N, nxh = N + 1;
+  extern __shared__ unsigned char smem[];
+  cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+  const int a = threadIdx.x % A, l = threadIdx.x / A;
+  const long L = (long)blockIdx.x * LPW + l;
+  const bool ok = L < nlines;
+  const T* src = in + (ok ? L : nlines - 1) * (2L * N);
+  cx<T> v[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int n = a + A * b;
+    v[b] = {src[2 * n] * in_scale, src[2 * n + 1] * in_scale};
+  }
+  dft_reg<T, B, true>(v);
+  twiddle_rows<T, B, true>(v, a, twN);
+  cx<T>* lb = buf + l * F::LSL;
+#pragma unroll
+  for (int k2 = 0; k2 < B; ++k2) lb[k2 * (A + 1) + a] = v[k2];
+  __syncthreads();
+  cx<T> z[B];  // z[g * A + k1] = Z[(a + A g) + B k1]
+#pragma unroll
+  for (int g = 0; g < B / A; ++g) {
+    const int k2 = a + A * g;
+    cx<T> u[A];
+#pragma unroll
+    for (int j = 0; j < A; ++j) u[j] = lb[k2 * (A + 1) + j];
+    dft_reg<T, A, true>(u);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) z[g * A + k1] = u[k1];
+  }
+  __syncthreads();  // pass-2 reads done: the line buffer takes Z in natural order
+#pragma unroll
+  for (int g = 0; g < B / A; ++g)
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) lb[(a + A * g) + B * k1] = z[g * A + k1];
+  __syncthreads();
+  if (!ok) return;
+  cx<T>* dst = W + L * nxh;
+  for (int k = a; k < nxh; k += A) {
+    const cx<T> zk = lb[k % N];
+    cx<T> zn = lb[(N - k) % N];
+    zn.im = -zn.im;  // conj(Z[N-k])
+    const cx<T> e = cscale(cadd(zk, zn), T(0.5));
+    const cx<T> d = csub(zk, zn);
+    const cx<T> od = {d.im * T(0.5), -d.re * T(0.5)};  // (Z - conj)/(2i)
+    dst[k] = cadd(e, cmul(rot[k], od));
+  }
+}
+
+// x inverse (spectrum kx = 0..N -> real lines of 2N, times `scale`) in two
+// passes; LPW lines per workgroup, thread = a + A * l.
+template <typename T, int A, int B, int LPW>
+__global__ void __launch_bounds__(LPW * A) k_fft2_x_inv(const cx<T>* __restrict__ W, T* __restrict__ out,
+                                                       long nlines, const cx<T>* __restrict__ twN,
+                                                       const cx<T>* __restrict__ rot, T scale) {
+  using F = Fft2<A, B>;
+  constexpr int N = F::N, nxh = N + 1;
+  extern __shared__ unsigned char smem[];
+  cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+  const int a = threadIdx.x % A, l = threadIdx.x / A;
+  const long L = (long)blockIdx.x * LPW + l;
+  const bool ok = L < nlines;
+  const cx<T>* src = W + (ok ? L : nlines - 1) * nxh;
+  cx<T>* lb = buf + l * F::LSL;
+  for (int k = a; k < N; k += A) {
+    const cx<T> xk = src[k];
+    cx<T> xn = src[N - k];
+    xn.im = -xn.im;  // conj(X[N-k])
+    const cx<T> e = cadd(xk, xn);
+    cx<T> r = rot[k];
+    r.im = -r.im;  // exp(+2 pi i k / nx)
+    const cx<T> od = cmul(csub(xk, xn), r);
+    lb[k] = {e.re - od.im, e.im + od.re};  // e + i*o
+  }
+  __syncthreads();
+  cx<T> v[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) v[b] = lb[a + A * b];
+  dft_reg<T, B, false>(v);
+  twiddle_rows<T, B, false>(v, a, twN);
+  __syncthreads();  // every thread has read its pass-1 input
+#pragma unroll
+  for (int k2 = 0; k2 < B; ++k2) lb[k2 * (A + 1) + a] = v[k2];
+  __syncthreads();
+  T* dst = out + L * (2L * N);
+#pragma unroll
+  for (int g = 0; g < B / A; ++g) {
+    const int k2 = a + A * g;
+    cx<T> u[A];
+#pragma unroll
+    for (int j = 0; j < A; ++j) u[j] = lb[k2 * (A + 1) + j];
+    dft_reg<T, A, false>(u);
+    if (ok) {
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) {
+        const int n = k2 + B * k1;
+        dst[2 * n] = u[k1].re * scale;
+        dst[2 * n + 1] = u[k1].im * scale;
+      }
+    }
+  }
+}
+
+// Launchers: the two-pass kernels for N in {64, 128, 256, 512}, the general
+// Stockham kernels otherwise.
+template <typename T, int A, int B>
+static size_t fft2_lds(int lpw) { return sizeof(cx<T>) * (size_t)lpw * Fft2<A, B>::LSL; }
+
+template <typename T>
+static void launch_fft_s(cx<T>* W, int nxh, int n, long ls, int no, long os, const Radices& rp, const cx<T>* tw,
+                         T sgn, hipStream_t s) {
+#define CHS_FFT2_S(A_, B_, LPW_)                                                                              \
+  {                                                                                                           \
+    dim3 g((unsigned)(((nxh + LPW_ - 1) / LPW_) * no));                                                       \
+    const size_t sh = fft2_lds<T, A_, B_>(LPW_);                                                              \
+    if (sgn < T(0))                                                                                           \
+      hipLaunchKernelGGL((k_fft2_s<T, A_, B_, LPW_, true>), g, dim3(LPW_ * A_), sh, s, W, nxh, ls, os, tw);  \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_fft2_s<T, A_, B_, LPW_, false>), g, dim3(LPW_ * A_), sh, s, W, nxh, ls, os, tw); \
+    return;                                                                                                   \
+  }
+  if (n == 64) CHS_FFT2_S(8, 8, 16)
+  if (n == 128) CHS_FFT2_S(8, 16, 16)
+  if (n == 256) CHS_FFT2_S(16, 16, 16)
+  if (n == 512) CHS_FFT2_S(16, 32, 8)
+#undef CHS_FFT2_S
+  const int lpb = kTileC / n;
+  dim3 g((unsigned)(((nxh + lpb - 1) / lpb) * no));
+  hipLaunchKernelGGL(k_fft_s<T>, g, dim3(kThreads), sizeof(cx<T>) * (size_t)lpb * n, s, W, nxh, n, ls, no, os, rp,
+                     tw, sgn);
+}
+
+template <typename T>
+static void launch_fft_x(bool fwd, const T* in, cx<T>* W, T* out, int nx, long nlines, const Radices& rp,
+                         const cx<T>* tw, const cx<T>* rot, T sc, hipStream_t s) {
+  const int N = nx / 2;
+#define CHS_FFT2_X(A_, B_, LPW_)                                                                              \
+  {                                                                                                           \
+    dim3 g((unsigned)((nlines + LPW_ - 1) / LPW_));                                                           \
+    const size_t sh = fft2_lds<T, A_, B_>(LPW_);                                                              \
+    if (fwd)                                                                                                  \
+      hipLaunchKernelGGL((k_fft2_x_fwd<T, A_, B_, LPW_>), g, dim3(LPW_ * A_), sh, s, in, W, nlines, tw, rot, sc); \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_fft2_x_inv<T, A_, B_, LPW_>), g, dim3(LPW_ * A_), sh, s, W, out, nlines, tw, rot, sc); \
+    return;                                                                                                   \
+  }
+  if (N == 64) CHS_FFT2_X(8, 8, 16)
+  if (N == 128) CHS_FFT2_X(8, 16, 16)
+  if (N == 256) CHS_FFT2_X(16, 16, 16)
+  if (N == 512) CHS_FFT2_X(16, 32, 8)
+#undef CHS_FFT2_X
+  const int lpb = kTileC / N;
+  dim3 g((unsigned)((nlines + lpb - 1) / lpb));
+  const size_t sh = sizeof(cx<T>) * (size_t)lpb * N;
+  if (fwd)
+    hipLaunchKernelGGL(k_fft_x_fwd<T>, g, dim3(kThreads), sh, s, in, W, nx, nlines, rp, tw, rot, sc);
+  else
+    hipLaunchKernelGGL(k_fft_x_inv<T>, g, dim3(kThreads), sh, s, W, out, nx, nlines, rp, tw, rot, sc);
+}
+
 // Triply periodic solve in spectral space: phi = -rhs / (kx'^2 + ky'^2 + kz'^2)
 // with the compact schemes' modified wavenumbers; the mean mode is set to 0.
 template <typename T>
@@ -725,26 +1017,15 @@ std::string Poisson::solve_3d_t(T* rhs, hipStream_t s, T in_scale) {
   const T* k2z = reinterpret_cast<const T*>(pk(lz64, lz32));
   cx<T>* W = reinterpret_cast<cx<T>*>(work);
   const long nlx = (long)ny * nz;
-  const int N = nx / 2;
-  const int lpbx = kTileC / N;
-  const size_t shx = sizeof(cx<T>) * (size_t)lpbx * N;
-  dim3 gx((unsigned)((nlx + lpbx - 1) / lpbx));
-  hipLaunchKernelGGL(k_fft_x_fwd<T>, gx, dim3(kThreads), shx, s, rhs, W, nx, nlx, rx, twx, rot, in_scale);
-  const int lpby = kTileC / ny, lpbz = kTileC / nz;
-  const size_t shy = sizeof(cx<T>) * (size_t)lpby * ny, shz = sizeof(cx<T>) * (size_t)lpbz * nz;
-  dim3 gy((unsigned)(((nxh + lpby - 1) / lpby) * nz)), gz((unsigned)(((nxh + lpbz - 1) / lpbz) * ny));
-  hipLaunchKernelGGL(k_fft_s<T>, gy, dim3(kThreads), shy, s, W, nxh, ny, (long)nxh, nz, (long)nxh * ny, ry, twy,
-                     T(-1));
-  hipLaunchKernelGGL(k_fft_s<T>, gz, dim3(kThreads), shz, s, W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz,
-                     T(-1));
+  launch_fft_x<T>(true, rhs, W, nullptr, nx, nlx, rx, twx, rot, in_scale, s);
+  launch_fft_s<T>(W, nxh, ny, (long)nxh, nz, (long)nxh * ny, ry, twy, T(-1), s);
+  launch_fft_s<T>(W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz, T(-1), s);
   const long nm = (long)nxh * ny * nz;
   hipLaunchKernelGGL(k_div_k2<T>, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s, W, nxh, ny, nz, k2x, k2y, k2z);
-  hipLaunchKernelGGL(k_fft_s<T>, gz, dim3(kThreads), shz, s, W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz,
-                     T(1));
-  hipLaunchKernelGGL(k_fft_s<T>, gy, dim3(kThreads), shy, s, W, nxh, ny, (long)nxh, nz, (long)nxh * ny, ry, twy,
-                     T(1));
+  launch_fft_s<T>(W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz, T(1), s);
+  launch_fft_s<T>(W, nxh, ny, (long)nxh, nz, (long)nxh * ny, ry, twy, T(1), s);
   const T scale = T(1.0 / (double(nx) * double(ny) * double(nz)));
-  hipLaunchKernelGGL(k_fft_x_inv<T>, gx, dim3(kThreads), shx, s, W, rhs, nx, nlx, rx, twx, rot, scale);
+  launch_fft_x<T>(false, nullptr, W, rhs, nx, nlx, rx, twx, rot, scale, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("poisson launch: ") + hipGetErrorString(e);
   return "";
@@ -766,22 +1047,13 @@ std::string Poisson::solve_t(T* rhs, hipStream_t s, T in_scale) {
   const T* fz = reinterpret_cast<const T*>(dbl ? (void*)fz64 : (void*)fz32);
   cx<T>* W = reinterpret_cast<cx<T>*>(work);
   const long nlx = (long)ny * nz;
-  const int N = nx / 2;
-  const int lpbx = kTileC / N;
-  const size_t shx = sizeof(cx<T>) * (size_t)lpbx * N;
-  dim3 gx((unsigned)((nlx + lpbx - 1) / lpbx));
-  hipLaunchKernelGGL(k_fft_x_fwd<T>, gx, dim3(kThreads), shx, s, rhs, W, nx, nlx, rx, twx, rot, in_scale);
-  const int lpbz = kTileC / nz;
-  const size_t shz = sizeof(cx<T>) * (size_t)lpbz * nz;
-  dim3 gz((unsigned)(((nxh + lpbz - 1) / lpbz) * ny));
-  hipLaunchKernelGGL(k_fft_s<T>, gz, dim3(kThreads), shz, s, W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz,
-                     T(-1));
+  launch_fft_x<T>(true, rhs, W, nullptr, nx, nlx, rx, twx, rot, in_scale, s);
+  launch_fft_s<T>(W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz, T(-1), s);
   const long nm = (long)nxh * nz;
   hipLaunchKernelGGL(k_ytdma<T>, dim3((unsigned)((nm + 63) / 64)), dim3(64), 0, s, W, nxh, ny, nz, ya64, fd, fz);
-  hipLaunchKernelGGL(k_fft_s<T>, gz, dim3(kThreads), shz, s, W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz,
-                     T(1));
+  launch_fft_s<T>(W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz, T(1), s);
   const T scale = T(1.0 / (double(nx) * double(nz)));
-  hipLaunchKernelGGL(k_fft_x_inv<T>, gx, dim3(kThreads), shx, s, W, rhs, nx, nlx, rx, twx, rot, scale);
+  launch_fft_x<T>(false, nullptr, W, rhs, nx, nlx, rx, twx, rot, scale, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("poisson launch: ") + hipGetErrorString(e);
   return "";
@@ -831,16 +1103,8 @@ std::string Poisson::solve_exact_t(T* rhs, hipStream_t s, T in_scale) {
   T* Wr = reinterpret_cast<T*>(work);
   T* W2r = reinterpret_cast<T*>(work2);
   const long nlx = (long)ny * nz;
-  const int N = nx / 2;
-  const int lpbx = kTileC / N;
-  const size_t shx = sizeof(cx<T>) * (size_t)lpbx * N;
-  dim3 gx((unsigned)((nlx + lpbx - 1) / lpbx));
-  hipLaunchKernelGGL(k_fft_x_fwd<T>, gx, dim3(kThreads), shx, s, rhs, W, nx, nlx, rx, twx, rot, in_scale);
-  const int lpbz = kTileC / nz;
-  const size_t shz = sizeof(cx<T>) * (size_t)lpbz * nz;
-  dim3 gz((unsigned)(((nxh + lpbz - 1) / lpbz) * ny));
-  hipLaunchKernelGGL(k_fft_s<T>, gz, dim3(kThreads), shz, s, W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz,
-                     T(-1));
+  launch_fft_x<T>(true, rhs, W, nullptr, nx, nlx, rx, twx, rot, in_scale, s);
+  launch_fft_s<T>(W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz, T(-1), s);
   const int ncr = 2 * nxh;
   const long ncol = (long)ncr * nz;
   if (Ve64) {
@@ -857,10 +1121,9 @@ std::string Poisson::solve_exact_t(T* rhs, hipStream_t s, T in_scale) {
     hipLaunchKernelGGL(k_ymix<T>, gm, dim3(256), 0, s, V, W2r, Wr, ny, ncr, ncol, (const T*)nullptr,
                        (const T*)nullptr, (const T*)nullptr, -1);
   }
-  hipLaunchKernelGGL(k_fft_s<T>, gz, dim3(kThreads), shz, s, W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz,
-                     T(1));
+  launch_fft_s<T>(W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz, T(1), s);
   const T scale = T(1.0 / (double(nx) * double(nz)));
-  hipLaunchKernelGGL(k_fft_x_inv<T>, gx, dim3(kThreads), shx, s, W, rhs, nx, nlx, rx, twx, rot, scale);
+  launch_fft_x<T>(false, nullptr, W, rhs, nx, nlx, rx, twx, rot, scale, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("poisson launch: ") + hipGetErrorString(e);
   return "";

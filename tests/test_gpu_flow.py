@@ -40,6 +40,27 @@ def test_poisson_vs_oracle(pair):
     assert rel(out, ref) < 1e-12
 
 
+@pytest.mark.parametrize("nc", [(128, 8, 64), (256, 8, 128), (512, 8, 256), (1024, 8, 512)])
+def test_poisson_transform_lengths(po, nc):
+    """FISHPACK-form Poisson solve where nx/2 and nz take each length of the
+    two-pass register FFT kernels (64, 128, 256, 512) and the x forward /
+    inverse real transforms; against the oracle's FFTPACK-form solve."""
+    from chapsim2_amd.domain import Domain
+    from chapsim2_amd.poisson import solve_fft_poisson
+
+    dm = Domain(nc, LX, -1.0, 1.0, LZ, **CFG)
+    of = po.Flow(list(nc), LX, -1.0, 1.0, LZ, wall_y=True, iacc=4, poisson=2, **CFG)
+    try:
+        rng = np.random.default_rng(7)
+        rhs = np.asfortranarray(rng.standard_normal(nc))
+        ref = of.poisson(rhs)
+        out = np.array(rhs, order="F")
+        solve_fft_poisson(out, dm)
+        assert rel(out, ref) < 1e-12, rel(out, ref)
+    finally:
+        dm.close()
+
+
 def test_momentum_rhs_vs_oracle(pair):
     dm, of = pair
     for isub in (0, 1):
@@ -67,7 +88,7 @@ def test_steps_vs_oracle(pair):
 
 
 @pytest.mark.parametrize("nc,iacc", [((32, 24, 16), 4), ((48, 40, 24), 2), ((40, 32, 30), 1),
-                                     ((32, 25, 16), 4), ((32, 144, 16), 4)])
+                                     ((32, 25, 16), 4), ((32, 144, 16), 4), ((128, 16, 64), 4)])
 def test_exact_projection_steps_vs_oracle(po, nc, iacc):
     """Exact projection (inverse of the composed D*G): GPU eigen-basis solve
     against the oracle's per-mode dense LU; after every step max|div u| is at

@@ -14,7 +14,7 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("nc,iacc", [((32, 32, 32), 4), ((48, 24, 40), 4), ((32, 32, 32), 2)])
+@pytest.mark.parametrize("nc,iacc", [((32, 32, 32), 4), ((48, 24, 40), 4), ((32, 32, 32), 2), ((64, 64, 64), 4)])
 def test_tgv_steps_vs_oracle(po, nc, iacc):
     from chapsim2_amd.domain import FFT_2DECOMP_3DFFT, Domain
     from chapsim2_amd.initial import tgv_fields
