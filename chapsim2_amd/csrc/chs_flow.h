@@ -83,7 +83,14 @@ class Flow {
   template <typename T>
   std::string op(RhsWork<T>& w, int dir, int opk, const int ibc2[2], const Field<T>& A, Field<T>& out,
                  const T* fbc, T sA = T(1), const Field<T>* B = nullptr, T cB = T(1), const Field<T>* Cm = nullptr,
-                 T sOut = T(1), int accumulate = 0);
+                 T sOut = T(1), int accumulate = 0, const Field<T>* D = nullptr, T cD = T(0),
+                 const Field<T>* E = nullptr, T cE = T(0));
+  // out = d(-ca * cb) + r d(ts * (ta + tc * tb)): a convective and a viscous
+  // operator of one momentum component along one direction
+  template <typename T>
+  std::string cov_tau(RhsWork<T>& w, int dir, int opk, const int bc_cov[2], const int bc_tau[2], Field<T>& out,
+                      const T* fbc_cov, const T* fbc_tau, const Field<T>& ca, const Field<T>& cb,
+                      const Field<T>& ta, T ts, const Field<T>* tb, T tc);
   template <typename T> std::string momentum_rhs_t(RhsWork<T>& w, int isub);
   template <typename T> std::string alloc_work(RhsWork<T>& w, bool alias_state);
   template <typename T> std::string alloc(Field<T>& f, int e1, int e2, int e3);

@@ -397,7 +397,8 @@ void Flow::update_re(int iter) {
 
 template <typename T>
 std::string Flow::op(RhsWork<T>& w, int dir, int opk, const int ibc2[2], const Field<T>& A, Field<T>& out,
-                     const T* fbc, T sA, const Field<T>* B, T cB, const Field<T>* Cm, T sOut, int accumulate) {
+                     const T* fbc, T sA, const Field<T>* B, T cB, const Field<T>* Cm, T sOut, int accumulate,
+                     const Field<T>* D, T cD, const Field<T>* E, T cE) {
   const DevPlan<T>* P = nullptr;
   SCHK(H->plan<T>(dir, opk, H->desc.iacc, ibc2, fbc != nullptr, A.e.n[dir], out.e.n[dir], &P));
   LineIn<T> in;
@@ -406,6 +407,10 @@ std::string Flow::op(RhsWork<T>& w, int dir, int opk, const int ibc2[2], const F
   in.Cm = Cm ? Cm->p : nullptr;
   in.sA = sA;
   in.cB = cB;
+  in.D = D ? D->p : nullptr;
+  in.cD = cD;
+  in.E = (D && E) ? E->p : nullptr;
+  in.cE = cE;
   LineOut<T> o;
   o.out = out.p;
   o.sOut = sOut;
@@ -415,7 +420,7 @@ std::string Flow::op(RhsWork<T>& w, int dir, int opk, const int ibc2[2], const F
   const bool timed = (H->prof_class == dir) && H->prof_pair(&e0, &e1);
   if (timed) {
     (void)hipEventRecord(e0, H->stream);
-    const int nin_arrays = 1 + (B ? 1 : 0) + (Cm ? 1 : 0);
+    const int nin_arrays = 1 + (B ? 1 : 0) + (Cm && Cm->p != A.p ? 1 : 0) + (D ? 1 : 0) + (D && E ? 1 : 0);
     H->prof_bytes += (double)sizeof(T) *
                      ((double)A.size() * nin_arrays + (double)out.size() * (1 + (accumulate ? 1 : 0)));
   }
@@ -462,6 +467,24 @@ std::string Flow::copy_rhs(double* mx, double* my, double* mz, int mem) {
     FCHK(hipStreamSynchronize(H->stream));
   }
   return "";
+}
+
+// The convective and viscous contributions of one momentum component along
+// one direction (src/eq_momentum2.f90:798-1507 applies them as two
+// operators).  When both use the same boundary kinds and neither has wall
+// values, they share one line plan and, the operator being linear, are
+// applied as one operator on the summed input
+//   -ca * cb + r ts ta + r ts tc tb.
+template <typename T>
+std::string Flow::cov_tau(RhsWork<T>& w, int dir, int opk, const int bc_cov[2], const int bc_tau[2], Field<T>& out,
+                          const T* fbc_cov, const T* fbc_tau, const Field<T>& ca, const Field<T>& cb,
+                          const Field<T>& ta, T ts, const Field<T>* tb, T tc) {
+  const T r = T(rre), one = T(1), m1 = T(-1);
+  if (!fbc_cov && !fbc_tau && bc_cov[0] == bc_tau[0] && bc_cov[1] == bc_tau[1])
+    return op<T>(w, dir, opk, bc_cov, ca, out, nullptr, m1, nullptr, one, &cb, one, 0, &ta, r * ts, tb,
+                 tb ? r * ts * tc : T(0));
+  SCHK(op<T>(w, dir, opk, bc_cov, ca, out, fbc_cov, m1, nullptr, one, &cb));
+  return op<T>(w, dir, opk, bc_tau, ta, out, fbc_tau, ts, tb, tc, nullptr, r, 1);
 }
 
 // Compute_momentum_rhs + Calculate_momentum_fractional_step + forcing.
@@ -515,35 +538,36 @@ std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub) {
     hipLaunchKernelGGL(k_wall_fbc<T>, grid1(nf2), dim3(256), 0, s, w.fby_div, w.fby_visc, w.plane_b, w.plane_c,
                        w.plane_d, nx, nz);
   }
-  const T r = T(rre), third = T(-ONE_THIRD);
+  const T third = T(-ONE_THIRD), two = T(2);
+  const T* fconv = wall_y ? w.fby_conv : nullptr;
+  const T* fvisc = wall_y ? w.fby_visc : nullptr;
   // ---- x-momentum, :798-948 ----
   Field<T>* X = w.acc[0];
-  SCHK(op<T>(w, 0, D1_C2P, mcov[0][0], w.qxix, X[0], nullptr, m1, nullptr, one, &w.qxix));
-  SCHK(op<T>(w, 1, D1_P2C, mcov[0][1], w.qyix, X[1], nullptr, m1, nullptr, one, &w.qxiy));
-  SCHK(op<T>(w, 2, D1_P2C, mcov[0][2], w.qzix, X[2], nullptr, m1, nullptr, one, &w.qxiz));
+  SCHK(cov_tau<T>(w, 0, D1_C2P, mcov[0][0], mtau[0][0], X[0], nullptr, nullptr, w.qxix, w.qxix, w.qxdx, two,
+                  &w.div, third));
+  SCHK(cov_tau<T>(w, 1, D1_P2C, mcov[0][1], mtau[0][1], X[1], nullptr, nullptr, w.qyix, w.qxiy, w.qxdy, one,
+                  &w.qydx, one));
+  SCHK(cov_tau<T>(w, 2, D1_P2C, mcov[0][2], mtau[0][2], X[2], nullptr, nullptr, w.qzix, w.qxiz, w.qxdz, one,
+                  &w.qzdx, one));
   SCHK(op<T>(w, 0, D1_C2P, B[0][VP], w.pr, X[3], nullptr, m1));
-  SCHK(op<T>(w, 0, D1_C2P, mtau[0][0], w.qxdx, X[0], nullptr, T(2), &w.div, third, nullptr, r, 1));
-  SCHK(op<T>(w, 1, D1_P2C, mtau[0][1], w.qxdy, X[1], nullptr, one, &w.qydx, one, nullptr, r, 1));
-  SCHK(op<T>(w, 2, D1_P2C, mtau[0][2], w.qxdz, X[2], nullptr, one, &w.qzdx, one, nullptr, r, 1));
   // ---- y-momentum, :960-1235 ----
   Field<T>* Y = w.acc[1];
-  SCHK(op<T>(w, 0, D1_P2C, mcov[1][0], w.qxiy, Y[0], nullptr, m1, nullptr, one, &w.qyix));
-  SCHK(op<T>(w, 1, D1_C2P, mcov[1][1], w.qyiy, Y[1], wall_y ? w.fby_conv : nullptr, m1, nullptr, one, &w.qyiy));
-  SCHK(op<T>(w, 2, D1_P2C, mcov[1][2], w.qyiz, Y[2], nullptr, m1, nullptr, one, &w.qziy));
+  SCHK(cov_tau<T>(w, 0, D1_P2C, mcov[1][0], mtau[1][0], Y[0], nullptr, nullptr, w.qxiy, w.qyix, w.qxdy, one,
+                  &w.qydx, one));
+  SCHK(cov_tau<T>(w, 1, D1_C2P, mcov[1][1], mtau[1][1], Y[1], fconv, fvisc, w.qyiy, w.qyiy, w.qydy, two, &w.div,
+                  third));
+  SCHK(cov_tau<T>(w, 2, D1_P2C, mcov[1][2], mtau[1][2], Y[2], nullptr, nullptr, w.qyiz, w.qziy, w.qzdy, one,
+                  &w.qydz, one));
   SCHK(op<T>(w, 1, D1_C2P, B[1][VP], w.pr, Y[3], wall_y ? w.fby_pn : nullptr, m1));
-  SCHK(op<T>(w, 0, D1_P2C, mtau[1][0], w.qxdy, Y[0], nullptr, one, &w.qydx, one, nullptr, r, 1));
-  SCHK(op<T>(w, 1, D1_C2P, mtau[1][1], w.qydy, Y[1], wall_y ? w.fby_visc : nullptr, T(2), &w.div, third, nullptr, r,
-             1));
-  SCHK(op<T>(w, 2, D1_P2C, mtau[1][2], w.qzdy, Y[2], nullptr, one, &w.qydz, one, nullptr, r, 1));
   // ---- z-momentum, :1248-1480 ----
   Field<T>* Z = w.acc[2];
-  SCHK(op<T>(w, 0, D1_P2C, mcov[2][0], w.qxiz, Z[0], nullptr, m1, nullptr, one, &w.qzix));
-  SCHK(op<T>(w, 1, D1_P2C, mcov[2][1], w.qyiz, Z[1], nullptr, m1, nullptr, one, &w.qziy));
-  SCHK(op<T>(w, 2, D1_C2P, mcov[2][2], w.qziz, Z[2], nullptr, m1, nullptr, one, &w.qziz));
+  SCHK(cov_tau<T>(w, 0, D1_P2C, mcov[2][0], mtau[2][0], Z[0], nullptr, nullptr, w.qxiz, w.qzix, w.qxdz, one,
+                  &w.qzdx, one));
+  SCHK(cov_tau<T>(w, 1, D1_P2C, mcov[2][1], mtau[2][1], Z[1], nullptr, nullptr, w.qyiz, w.qziy, w.qzdy, one,
+                  &w.qydz, one));
+  SCHK(cov_tau<T>(w, 2, D1_C2P, mcov[2][2], mtau[2][2], Z[2], nullptr, nullptr, w.qziz, w.qziz, w.qzdz, two,
+                  &w.div, third));
   SCHK(op<T>(w, 2, D1_C2P, B[2][VP], w.pr, Z[3], nullptr, m1));
-  SCHK(op<T>(w, 0, D1_P2C, mtau[2][0], w.qxdz, Z[0], nullptr, one, &w.qzdx, one, nullptr, r, 1));
-  SCHK(op<T>(w, 1, D1_P2C, mtau[2][1], w.qzdy, Z[1], nullptr, one, &w.qydz, one, nullptr, r, 1));
-  SCHK(op<T>(w, 2, D1_C2P, mtau[2][2], w.qzdz, Z[2], nullptr, T(2), &w.div, third, nullptr, r, 1));
   // ---- fractional step and forcing ----
   const Mesh& m = H->mesh;
   const bool forced = (H->desc.idriven == 1);

@@ -52,7 +52,7 @@ struct DevPlan {
 template <typename T>
 std::string upload_plan(const LinePlan& p, DevPlan<T>* dp);
 
-// Input transform f = sA * (A + cB * B) [* Cm]; B and Cm optional.
+// Input transform f = sA * (A + cB * B) [* Cm] [+ cD * D [+ cE * E]]; B, Cm, D, E optional.
 template <typename T>
 struct LineIn {
   const T* A = nullptr;
@@ -60,6 +60,13 @@ struct LineIn {
   const T* Cm = nullptr;
   T sA = T(1);
   T cB = T(1);
+  // additive terms: f += cD * D + cE * E (E only together with D); a sum of
+  // operators that share one plan is applied as one operator on the summed
+  // input
+  const T* D = nullptr;
+  const T* E = nullptr;
+  T cD = T(0);
+  T cE = T(0);
 };
 
 // Output out = (accumulate ? out : 0) + value * sOut

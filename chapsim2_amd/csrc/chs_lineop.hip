@@ -168,6 +168,7 @@ struct KArgs {
   int o1, o2;      // output extents 1,2
   long nlines;
   const T* A; const T* B; const T* Cm; T sA, cB;
+  const T* D; const T* E; T cD, cE;
   const T* fbc;
   T c1, c2, dd, sgn;
   const DevRow<T>* rows;
@@ -222,6 +223,8 @@ __global__ void __launch_bounds__(64) k_line(KArgs<T> P) {
     if (P.B) t = t + P.cB * P.B[o];
     t = P.sA * t;
     if (P.Cm) t = t * P.Cm[o];
+    if (P.D) t = t + P.cD * P.D[o];
+    if (P.E) t = t + P.cE * P.E[o];
     return t;
   };
   T fb4[4] = {T(0), T(0), T(0), T(0)};
@@ -373,17 +376,19 @@ __device__ __forceinline__ void transform_in(const KArgs<T>& P, T (&f)[NV], cons
 
 // Window of one y/z line chunk: rows s0-2 .. s0+kChunk+1 (wrapped on
 // periodic lines, clamped otherwise, so every address is valid).  The row
-// offsets are wave-uniform; the loads of A and (HAS_S) of the second input
-// array (B, or Cm when B is absent) are all issued before any use.
-template <typename T, bool HAS_S>
+// offsets are wave-uniform.  IN = 0: A only; 1: A and the second input array
+// (B, or Cm when B is absent), all loads issued before any use; 2: as 1 (the
+// second array optional), then the additive arrays D (and E) as a second
+// batch.
+template <typename T, int IN>
 __device__ __forceinline__ void load_window(const KArgs<T>& P, long base, unsigned ri, long is, int s0,
                                             T (&f)[kChunk + 4]) {
   constexpr int W = kChunk + 4;
   long ro[W];
 #pragma unroll
   for (int t = 0; t < W; ++t) ro[t] = base + is * window_index(s0 - 2 + t, P.n_in, P.wrap);
-  if (HAS_S) {
-    const T* S = P.B ? P.B : P.Cm;
+  const T* S = P.B ? P.B : P.Cm;
+  if (IN == 1 || (IN == 2 && S)) {
     T g[W];
 #pragma unroll
     for (int t = 0; t < W; ++t) {
@@ -394,6 +399,34 @@ __device__ __forceinline__ void load_window(const KArgs<T>& P, long base, unsign
   } else {
 #pragma unroll
     for (int t = 0; t < W; ++t) f[t] = P.sA * ld_at(P.A + ro[t], ri);
+  }
+  if (IN == 2) {
+    // The additive arrays are loaded after the first batch has been
+    // combined, half a window of one array at a time: their lane offset
+    // passes through an empty asm that reads the window's last value, so the
+    // compiler cannot hoist these loads next to the first batch (registers
+    // for the window and both arrays together spill).
+    unsigned rj = ri;
+    asm volatile("" : "+v"(rj) : "v"(f[W - 1]));
+    constexpr int H = W / 2;
+#pragma unroll
+    for (int h = 0; h < W; h += H) {
+      T d[H];
+#pragma unroll
+      for (int t = 0; t < H; ++t) d[t] = ld_at(P.D + ro[h + t], rj);
+#pragma unroll
+      for (int t = 0; t < H; ++t) f[h + t] = f[h + t] + P.cD * d[t];
+    }
+    if (P.E) {
+#pragma unroll
+      for (int h = 0; h < W; h += H) {
+        T e[H];
+#pragma unroll
+        for (int t = 0; t < H; ++t) e[t] = ld_at(P.E + ro[h + t], rj);
+#pragma unroll
+        for (int t = 0; t < H; ++t) f[h + t] = f[h + t] + P.cE * e[t];
+      }
+    }
   }
 }
 
@@ -471,11 +504,11 @@ __device__ __forceinline__ void yz_offsets(const KArgs<T>& P, int L, long& ib, l
 }
 
 // One y/z workgroup: 64 neighbouring lines x P chunks, wave q holding
-// chunk q of the 64 lines.  Single-input operators ask for 5 waves per SIMD
-// (at most 96 registers), so that two workgroups of 9 waves (lines of
-// 129 rows) share a CU.
-template <typename T, int DIR, int OPK, bool HAS_S>
-__global__ void __launch_bounds__(64 * kMaxChunks) __attribute__((amdgpu_waves_per_eu(HAS_S ? 4 : 5)))
+// chunk q of the 64 lines; IN as in load_window.  Single-input operators ask
+// for 5 waves per SIMD (at most 96 registers), so that two workgroups of
+// 9 waves (lines of 129 rows) share a CU.
+template <typename T, int DIR, int OPK, int IN>
+__global__ void __launch_bounds__(64 * kMaxChunks) __attribute__((amdgpu_waves_per_eu(IN == 0 ? 5 : 4)))
 k_line_yz(KArgs<T> P) {
   __shared__ T ex[2 * kMaxChunks * 64];
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -510,7 +543,7 @@ k_line_yz(KArgs<T> P) {
   const int n_out = P.n_out, Pn = P.P;
   const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
   T f[kChunk + 4];
-  load_window<T, HAS_S>(P, ib0, ri, is, s0, f);
+  load_window<T, IN>(P, ib0, ri, is, s0, f);
   T fb4[4] = {T(0), T(0), T(0), T(0)};
   if (P.fbc && (q == 0 || q == Pn - 1))
     for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[fb + is * m];
@@ -585,6 +618,8 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
       if (P.B) v = v + P.cB * P.B[o];
       v = P.sA * v;
       if (P.Cm) v = v * P.Cm[o];
+      if (P.D) v = v + P.cD * P.D[o];
+      if (P.E) v = v + P.cE * P.E[o];
     }
     tile[(long)l * LS + lds_pos(k)] = v;
   }
@@ -631,7 +666,7 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
 // (row stride odd, so the 64 lanes reading one column of their own lines hit
 // distinct banks); wave q of the workgroup processes chunk q of all 64 lines,
 // which keeps the plan's per-row coefficients in scalar registers.
-template <typename T, int OPK, bool HAS_S>
+template <typename T, int OPK, int IN>
 __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* tile = reinterpret_cast<T*>(smem_raw);
@@ -667,6 +702,8 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
   const T* const Ab = P.A + L0 * P.e1;
   const T* const Bb = P.B ? P.B + L0 * P.e1 : nullptr;
   const T* const Cb = P.Cm ? P.Cm + L0 * P.e1 : nullptr;
+  const T* const Db = P.D ? P.D + L0 * P.e1 : nullptr;
+  const T* const Eb = P.E ? P.E + L0 * P.e1 : nullptr;
   {
     const int dl = nthr / n_in, dk = nthr - dl * n_in;
     int l = tid / n_in, k = tid - l * n_in;
@@ -684,8 +721,8 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
         if (k >= n_in) { k -= n_in; ++l; }
       }
       T v[UB];
-      if (HAS_S) {
-        const T* S = Bb ? Bb : Cb;
+      const T* S = Bb ? Bb : Cb;
+      if (IN == 1 || (IN == 2 && S)) {
         T g[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
@@ -696,6 +733,16 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
       } else {
 #pragma unroll
         for (int u = 0; u < UB; ++u) v[u] = P.sA * ld_at(Ab, o[u]);
+      }
+      if (IN == 2) {
+        T d[UB], e[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+          d[u] = ld_at(Db, o[u]);
+          e[u] = Eb ? ld_at(Eb, o[u]) : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < UB; ++u) v[u] = v[u] + P.cD * d[u] + P.cE * e[u];
       }
 #pragma unroll
       for (int u = 0; u < UB; ++u)
@@ -767,22 +814,29 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
   }
 }
 
-template <typename T, bool HAS_S>
+template <typename T, int IN>
 static void dispatch_xu_s(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
   switch (op) {
-    case MIDP_C2P: hipLaunchKernelGGL((k_line_xu<T, MIDP_C2P, HAS_S>), g, b, shm, s, a); break;
-    case MIDP_P2C: hipLaunchKernelGGL((k_line_xu<T, MIDP_P2C, HAS_S>), g, b, shm, s, a); break;
-    case D1_C2C: hipLaunchKernelGGL((k_line_xu<T, D1_C2C, HAS_S>), g, b, shm, s, a); break;
-    case D1_P2P: hipLaunchKernelGGL((k_line_xu<T, D1_P2P, HAS_S>), g, b, shm, s, a); break;
-    case D1_C2P: hipLaunchKernelGGL((k_line_xu<T, D1_C2P, HAS_S>), g, b, shm, s, a); break;
-    case D1_P2C: hipLaunchKernelGGL((k_line_xu<T, D1_P2C, HAS_S>), g, b, shm, s, a); break;
+    case MIDP_C2P: hipLaunchKernelGGL((k_line_xu<T, MIDP_C2P, IN>), g, b, shm, s, a); break;
+    case MIDP_P2C: hipLaunchKernelGGL((k_line_xu<T, MIDP_P2C, IN>), g, b, shm, s, a); break;
+    case D1_C2C: hipLaunchKernelGGL((k_line_xu<T, D1_C2C, IN>), g, b, shm, s, a); break;
+    case D1_P2P: hipLaunchKernelGGL((k_line_xu<T, D1_P2P, IN>), g, b, shm, s, a); break;
+    case D1_C2P: hipLaunchKernelGGL((k_line_xu<T, D1_C2P, IN>), g, b, shm, s, a); break;
+    case D1_P2C: hipLaunchKernelGGL((k_line_xu<T, D1_P2C, IN>), g, b, shm, s, a); break;
   }
 }
 
+// number of input arrays beyond A, as the kernels' IN parameter
+template <typename T>
+static int input_kind(const KArgs<T>& a) { return a.D ? 2 : ((a.B || a.Cm) ? 1 : 0); }
+
 template <typename T>
 static void dispatch_xu(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
-  if (a.B || a.Cm) dispatch_xu_s<T, true>(op, g, b, shm, s, a);
-  else dispatch_xu_s<T, false>(op, g, b, shm, s, a);
+  switch (input_kind(a)) {
+    case 0: dispatch_xu_s<T, 0>(op, g, b, shm, s, a); break;
+    case 1: dispatch_xu_s<T, 1>(op, g, b, shm, s, a); break;
+    default: dispatch_xu_s<T, 2>(op, g, b, shm, s, a); break;
+  }
 }
 
 // LDS bytes a workgroup may use, queried once per device.
@@ -798,21 +852,21 @@ static size_t max_lds_per_block() {
   return cached[dev];
 }
 
-template <typename T, int DIR, bool HAS_S>
+template <typename T, int DIR, int IN>
 static void dispatch_yz(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
   switch (op) {
-    case MIDP_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_C2P, HAS_S>), g, b, shm, s, a); break;
-    case MIDP_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_P2C, HAS_S>), g, b, shm, s, a); break;
-    case D1_C2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2C, HAS_S>), g, b, shm, s, a); break;
-    case D1_P2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2P, HAS_S>), g, b, shm, s, a); break;
-    case D1_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2P, HAS_S>), g, b, shm, s, a); break;
-    case D1_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2C, HAS_S>), g, b, shm, s, a); break;
+    case MIDP_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_C2P, IN>), g, b, shm, s, a); break;
+    case MIDP_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_P2C, IN>), g, b, shm, s, a); break;
+    case D1_C2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2C, IN>), g, b, shm, s, a); break;
+    case D1_P2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2P, IN>), g, b, shm, s, a); break;
+    case D1_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2P, IN>), g, b, shm, s, a); break;
+    case D1_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2C, IN>), g, b, shm, s, a); break;
   }
 }
 
 template <typename T, int DIR>
 static void dispatch_part(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
-  if (DIR == 0) {
+  if constexpr (DIR == 0) {
     switch (op) {
       case MIDP_C2P: hipLaunchKernelGGL((k_line_x<T, MIDP_C2P>), g, b, shm, s, a); break;
       case MIDP_P2C: hipLaunchKernelGGL((k_line_x<T, MIDP_P2C>), g, b, shm, s, a); break;
@@ -821,10 +875,12 @@ static void dispatch_part(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, con
       case D1_C2P: hipLaunchKernelGGL((k_line_x<T, D1_C2P>), g, b, shm, s, a); break;
       case D1_P2C: hipLaunchKernelGGL((k_line_x<T, D1_P2C>), g, b, shm, s, a); break;
     }
-  } else if (a.B || a.Cm) {
-    dispatch_yz<T, DIR, true>(op, g, b, shm, s, a);
   } else {
-    dispatch_yz<T, DIR, false>(op, g, b, shm, s, a);
+    switch (input_kind(a)) {
+      case 0: dispatch_yz<T, DIR, 0>(op, g, b, shm, s, a); break;
+      case 1: dispatch_yz<T, DIR, 1>(op, g, b, shm, s, a); break;
+      default: dispatch_yz<T, DIR, 2>(op, g, b, shm, s, a); break;
+    }
   }
 }
 
@@ -855,6 +911,7 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
   a.o1 = eout.n[0]; a.o2 = eout.n[1];
   a.nlines = (dir == 0) ? (long)ein.n[1] * ein.n[2] : (dir == 1) ? (long)ein.n[0] * ein.n[2] : (long)ein.n[0] * ein.n[1];
   a.A = in.A; a.B = in.B; a.Cm = in.Cm; a.sA = in.sA; a.cB = in.cB;
+  a.D = in.D; a.E = in.D ? in.E : nullptr; a.cD = in.cD; a.cE = in.cE;
   a.fbc = fbc;
   a.c1 = T(p.c1); a.c2 = T(p.c2); a.dd = T(p.dd); a.sgn = T(p.sgn);
   a.rows = D.rows;
