@@ -799,6 +799,303 @@ __global__ void __launch_bounds__(256) k_ymix_sym(const T* __restrict__ Me, cons
   }
 }
 
+// Both stages of the mirror-symmetric mixing in one pass (h = ny/2 <= 64):
+// a workgroup holds the folded even/odd halves of 64 columns in LDS, applies
+// Vie / Vio, the scaling 1/(lam + lx + lz) (null mode set to 0), then Ve / Vo,
+// and unfolds into the output, so the spectrum is read and written once.
+// The 64 x 64 blocks stream through LDS in K-tiles of 16 (the next tile
+// prefetched into registers); each thread accumulates 8 rows x 4 columns of
+// one half.
+constexpr int kFuC = 64, kFuK = 16, kFuH = 64;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_ymix_fused(const T* __restrict__ Vie, const T* __restrict__ Vio,
+                                                   const T* __restrict__ Ve, const T* __restrict__ Vo,
+                                                   const T* in, T* out, int ny, int ncr,
+                                                   long ncol, const T* __restrict__ lame, const T* __restrict__ lamo,
+                                                   const T* __restrict__ lx, const T* __restrict__ lz, int null_even) {
+  // in and out may be the same array: a workgroup reads all of its columns
+  // before it writes any of them
+  constexpr int BW = kFuC + 1;        // row stride of the column block (odd: transposed writes spread)
+  constexpr int AW = 2 * kFuH + 4;    // row stride of the matrix tile
+  extern __shared__ unsigned char smem_raw[];
+  T* Bs = reinterpret_cast<T*>(smem_raw);  // [half][row][BW]
+  T* As = Bs + 2 * kFuH * BW;               // [k][AW]: rows 0..63 even block, 64..127 odd block
+  const int h = ny / 2;
+  const int tid = threadIdx.x;
+  const int hf = tid >> 7, rg = (tid >> 4) & 7, cg = tid & 15;
+  const int r0 = rg * 8, c0 = cg * 4;
+  const long g0 = (long)blockIdx.x * kFuC;
+  // ---- fold the 64 columns: e(j) = x(j) + x(ny-1-j), o(j) = x(j) - x(ny-1-j)
+  {
+    const int c = tid & 63;
+    const long g = g0 + c;
+    const bool cok = g < ncol;
+    const long gc = cok ? g : ncol - 1;
+    const long kz = gc / ncr, cc = gc - kz * ncr;
+    const long base = cc + (long)ncr * (long)ny * kz;
+    constexpr int NJ = kFuH / 4;
+    T a[NJ], b[NJ];
+#pragma unroll
+    for (int u = 0; u < NJ; ++u) {
+      const int j = (tid >> 6) + 4 * u;
+      const int jc = j < h ? j : h - 1;
+      a[u] = in[base + (long)ncr * jc];
+      b[u] = in[base + (long)ncr * (ny - 1 - jc)];
+    }
+#pragma unroll
+    for (int u = 0; u < NJ; ++u) {
+      const int j = (tid >> 6) + 4 * u;
+      const bool ok = cok && j < h;
+      Bs[(0 * kFuH + j) * BW + c] = ok ? a[u] + b[u] : T(0);
+      Bs[(1 * kFuH + j) * BW + c] = ok ? a[u] - b[u] : T(0);
+    }
+  }
+  // one K-tile of the two blocks: As[k][row] = M_half[row][k0 + k]
+  auto load_tile = [&](const T* Me, const T* Mo, int k0, T (&v)[8]) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + 256 * u;  // 0 .. 2047 = 128 rows x 16 k
+      const int row = e >> 4, k = e & 15;
+      const int half = row >> 6, i = row & 63, gk = k0 + k;
+      const T* M = half ? Mo : Me;
+      v[u] = (i < h && gk < h) ? M[(long)i * h + gk] : T(0);
+    }
+  };
+  auto store_tile = [&](const T (&v)[8]) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + 256 * u;
+      const int row = e >> 4, k = e & 15;
+      As[k * AW + row] = v[u];
+    }
+  };
+  auto matmul = [&](const T* Me, const T* Mo, T (&acc)[8][4]) {
+#pragma unroll
+    for (int p = 0; p < 8; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[p][q] = T(0);
+    T v[8];
+    load_tile(Me, Mo, 0, v);
+    for (int k0 = 0; k0 < h; k0 += kFuK) {
+      __syncthreads();  // previous tile consumed (and, first time, the column block written)
+      store_tile(v);
+      __syncthreads();
+      if (k0 + kFuK < h) load_tile(Me, Mo, k0 + kFuK, v);
+#pragma unroll
+      for (int k = 0; k < kFuK; ++k) {
+        T a[8], b[4];
+#pragma unroll
+        for (int p = 0; p < 8; ++p) a[p] = As[k * AW + hf * kFuH + r0 + p];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) b[q] = Bs[(hf * kFuH + k0 + k) * BW + c0 + q];
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[p][q] = acc[p][q] + a[p] * b[q];
+      }
+    }
+    __syncthreads();  // every thread has read the column block
+  };
+  T acc[8][4];
+  // ---- stage 1: Vi, then the eigenvalue scaling
+  matmul(Vie, Vio, acc);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const long g = g0 + c0 + q;
+    const long gc = g < ncol ? g : ncol - 1;
+    const long kz = gc / ncr, kx = (gc - kz * ncr) >> 1;
+    const T shift = lx[kx] + lz[kz];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const int i = r0 + p;
+      T y = T(0);
+      if (i < h) {
+        const bool null_mode = (hf == 0 && i == null_even && kx == 0 && kz == 0);
+        y = null_mode ? T(0) : acc[p][q] / ((hf ? lamo[i] : lame[i]) + shift);
+      }
+      Bs[(hf * kFuH + i) * BW + c0 + q] = y;
+    }
+  }
+  // ---- stage 2: V (matmul's first barrier orders these writes before use)
+  matmul(Ve, Vo, acc);
+#pragma unroll
+  for (int p = 0; p < 8; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Bs[(hf * kFuH + r0 + p) * BW + c0 + q] = acc[p][q];
+  __syncthreads();
+  // ---- unfold: phi(j) = (e(j) + o(j))/2, phi(ny-1-j) = (e(j) - o(j))/2
+  {
+    const int c = tid & 63;
+    const long g = g0 + c;
+    if (g < ncol) {
+      const long kz = g / ncr, cc = g - kz * ncr;
+      const long base = cc + (long)ncr * (long)ny * kz;
+#pragma unroll
+      for (int u = 0; u < kFuH / 4; ++u) {
+        const int j = (tid >> 6) + 4 * u;
+        if (j >= h) break;
+        const T ev = Bs[j * BW + c], od = Bs[(kFuH + j) * BW + c];
+        out[base + (long)ncr * j] = T(0.5) * (ev + od);
+        out[base + (long)ncr * (ny - 1 - j)] = T(0.5) * (ev - od);
+      }
+    }
+  }
+}
+
+// FP64 mirror-symmetric mixing on the matrix cores (h = ny/2 <= 64), both
+// stages in one pass.  v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16]
+// and B[l/16][l%16], and register v of lane l holds D[4v + l/16][l%16].
+// Workgroup = 8 waves; wave w owns rows 16*(w%4) .. +16 of half w/4 (even,
+// odd) and reads that row block of Vi (then of V) into registers from
+// L2 before each phase.  It loops over tiles of 64 columns: fold into LDS, D = Vi B (16
+// K-steps x 4 column blocks), scale by 1/(lam + lx + lz), back to LDS, D = V Y,
+// back to LDS, unfold into the output (in place: a tile is read before it is
+// written, and no other workgroup touches it).
+typedef double chs_d4 __attribute__((ext_vector_type(4)));
+constexpr int kMmC = 64;            // columns per tile
+constexpr int kMmW = kMmC + 1;      // LDS row stride
+constexpr int kMmWaves = 8;
+
+__global__ void __launch_bounds__(64 * kMmWaves) k_ymix_mfma(const double* __restrict__ Vie,
+                                                             const double* __restrict__ Vio,
+                                                             const double* __restrict__ Ve,
+                                                             const double* __restrict__ Vo, double* W, int ny,
+                                                             int ncr, long ncol, const double* __restrict__ lame,
+                                                             const double* __restrict__ lamo,
+                                                             const double* __restrict__ lx,
+                                                             const double* __restrict__ lz, int null_even) {
+  __shared__ double Bs[2 * 64 * kMmW];  // [half][row][column]
+  const int h = ny / 2;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int half = w >> 2, rb = w & 3;
+  const int li = lane & 15, lk = lane >> 4;
+  // row block of a matrix for this wave: a[ks] = M[16 rb + li][4 ks + lk]
+  // (read from L2 before each phase)
+  auto load_block = [&](const double* M, double (&a)[16]) {
+    const int i = 16 * rb + li;
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+      const int k = 4 * ks + lk;
+      a[ks] = (i < h && k < h) ? M[(long)i * h + k] : 0.0;
+    }
+  };
+  const double* Mi = half ? Vio : Vie;
+  const double* Mv = half ? Vo : Ve;
+  const double* lam = half ? lamo : lame;
+  double* Bh = Bs + half * 64 * kMmW;
+  const long ntile = (ncol + kMmC - 1) / kMmC;
+  for (long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    const long g0 = tile * kMmC;
+    // ---- fold: 64 columns x 64 row pairs, 8 per thread
+    {
+      const int c = tid & 63;
+      const long g = g0 + c;
+      const bool cok = g < ncol;
+      const long gc = cok ? g : ncol - 1;
+      const long kz = gc / ncr, cc = gc - kz * ncr;
+      const long base = cc + (long)ncr * (long)ny * kz;
+#pragma unroll
+      for (int u0 = 0; u0 < 8; u0 += 4) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = (tid >> 6) + 8 * (u0 + u);
+          const int jc = j < h ? j : h - 1;
+          a[u] = W[base + (long)ncr * jc];
+          b[u] = W[base + (long)ncr * (ny - 1 - jc)];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = (tid >> 6) + 8 * (u0 + u);
+          const bool ok = cok && j < h;
+          Bs[j * kMmW + c] = ok ? a[u] + b[u] : 0.0;
+          Bs[(64 + j) * kMmW + c] = ok ? a[u] - b[u] : 0.0;
+        }
+      }
+    }
+    __syncthreads();
+    // ---- D = Vi B over this wave's 16 rows and the 64 columns
+    double a1[16];
+    load_block(Mi, a1);
+    chs_d4 acc[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[cb] = chs_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int ks = 0; ks < 16; ++ks) {
+      const double* brow = Bh + (4 * ks + lk) * kMmW + li;
+      double bv[4];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) bv[cb] = brow[16 * cb];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb)
+        acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[ks], bv[cb], acc[cb], 0, 0, 0);
+    }
+    __syncthreads();  // every wave has read the folded block
+    // ---- scale by 1/(lam + lx + lz) (null mode to 0) and store Y
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+      const int c = 16 * cb + li;
+      const long g = g0 + c;
+      const long gc = g < ncol ? g : ncol - 1;
+      const long kz = gc / ncr, kx = (gc - kz * ncr) >> 1;
+      const double shift = lx[kx] + lz[kz];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * rb + 4 * v + lk;
+        double y = 0.0;
+        if (i < h) {
+          const bool null_mode = (half == 0 && i == null_even && kx == 0 && kz == 0);
+          y = null_mode ? 0.0 : acc[cb][v] / (lam[i] + shift);
+        }
+        Bh[i * kMmW + c] = y;
+      }
+    }
+    __syncthreads();
+    // ---- D = V Y
+    double a2[16];
+    load_block(Mv, a2);
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[cb] = chs_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int ks = 0; ks < 16; ++ks) {
+      const double* brow = Bh + (4 * ks + lk) * kMmW + li;
+      double bv[4];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) bv[cb] = brow[16 * cb];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb)
+        acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[ks], bv[cb], acc[cb], 0, 0, 0);
+    }
+    __syncthreads();  // every wave has read Y
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Bh[(16 * rb + 4 * v + lk) * kMmW + 16 * cb + li] = acc[cb][v];
+    __syncthreads();
+    // ---- unfold: phi(j) = (e(j) + o(j))/2, phi(ny-1-j) = (e(j) - o(j))/2
+    {
+      const int c = tid & 63;
+      const long g = g0 + c;
+      if (g < ncol) {
+        const long kz = g / ncr, cc = g - kz * ncr;
+        const long base = cc + (long)ncr * (long)ny * kz;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int j = (tid >> 6) + 8 * u;
+          if (j < h) {
+            const double ev = Bs[j * kMmW + c], od = Bs[(64 + j) * kMmW + c];
+            W[base + (long)ncr * j] = 0.5 * (ev + od);
+            W[base + (long)ncr * (ny - 1 - j)] = 0.5 * (ev - od);
+          }
+        }
+      }
+    }
+    __syncthreads();  // the next tile's fold overwrites the block
+  }
+}
+
 Poisson::~Poisson() {
   for (void* p : {(void*)tw_x64, (void*)tw_x32, (void*)tw_z64, (void*)tw_z32, (void*)rot64, (void*)rot32,
                   (void*)ya64, (void*)yc64, (void*)ya32, (void*)yc32, (void*)fd64, (void*)fz64, (void*)fd32,
@@ -1107,7 +1404,19 @@ std::string Poisson::solve_exact_t(T* rhs, hipStream_t s, T in_scale) {
   launch_fft_s<T>(W, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz, T(-1), s);
   const int ncr = 2 * nxh;
   const long ncol = (long)ncr * nz;
-  if (Ve64) {
+  if (Ve64 && dbl && ny / 2 <= 64) {
+    const long ntile = (ncol + kMmC - 1) / kMmC;
+    const unsigned nwg = (unsigned)(ntile < 2048 ? ntile : 2048);
+    hipLaunchKernelGGL(k_ymix_mfma, dim3(nwg), dim3(64 * kMmWaves), 0, s, Vie64, Vio64, Ve64, Vo64,
+                       reinterpret_cast<double*>(Wr), ny, ncr, ncol, lame64, lamo64,
+                       reinterpret_cast<const double*>(lxv), reinterpret_cast<const double*>(lzv), null_even);
+  } else if (Ve64 && ny / 2 <= kFuH) {
+    auto pick = [&](double* d, float* f) { return reinterpret_cast<const T*>(dbl ? (void*)d : (void*)f); };
+    const size_t sh = sizeof(T) * ((size_t)2 * kFuH * (kFuC + 1) + (size_t)kFuK * (2 * kFuH + 4));
+    hipLaunchKernelGGL(k_ymix_fused<T>, dim3((unsigned)((ncol + kFuC - 1) / kFuC)), dim3(256), sh, s,
+                       pick(Vie64, Vie32), pick(Vio64, Vio32), pick(Ve64, Ve32), pick(Vo64, Vo32), Wr, Wr, ny, ncr,
+                       ncol, pick(lame64, lame32), pick(lamo64, lamo32), lxv, lzv, null_even);
+  } else if (Ve64) {
     auto pick = [&](double* d, float* f) { return reinterpret_cast<const T*>(dbl ? (void*)d : (void*)f); };
     const int h = ny / 2;
     dim3 gs((unsigned)((ncol + kMixC - 1) / kMixC), (unsigned)((h + kSymR - 1) / kSymR));
