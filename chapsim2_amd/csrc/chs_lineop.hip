@@ -402,29 +402,39 @@ __device__ __forceinline__ void load_window(const KArgs<T>& P, long base, unsign
   }
   if (IN == 2) {
     // The additive arrays are loaded after the first batch has been
-    // combined, half a window of one array at a time: their lane offset
-    // passes through an empty asm that reads the window's last value, so the
-    // compiler cannot hoist these loads next to the first batch (registers
-    // for the window and both arrays together spill).
+    // combined (FP32: together; FP64: one after the other): their lane
+    // offset passes through an empty asm that reads the window's last value,
+    // so the compiler cannot hoist these loads next to the earlier batch
+    // (registers for the window and all arrays together spill).
     unsigned rj = ri;
     asm volatile("" : "+v"(rj) : "v"(f[W - 1]));
-    constexpr int H = W / 2;
+    if (sizeof(T) == 4) {
+      // FP32: both arrays in one batch
+      T d[W], e[W];
 #pragma unroll
-    for (int h = 0; h < W; h += H) {
-      T d[H];
+      for (int t = 0; t < W; ++t) {
+        d[t] = ld_at(P.D + ro[t], rj);
+        e[t] = P.E ? ld_at(P.E + ro[t], rj) : T(0);
+      }
 #pragma unroll
-      for (int t = 0; t < H; ++t) d[t] = ld_at(P.D + ro[h + t], rj);
+      for (int t = 0; t < W; ++t) f[t] = f[t] + P.cD * d[t] + P.cE * e[t];
+    } else {
+      // FP64: one array at a time
+      {
+        T d[W];
 #pragma unroll
-      for (int t = 0; t < H; ++t) f[h + t] = f[h + t] + P.cD * d[t];
-    }
-    if (P.E) {
+        for (int t = 0; t < W; ++t) d[t] = ld_at(P.D + ro[t], rj);
 #pragma unroll
-      for (int h = 0; h < W; h += H) {
-        T e[H];
+        for (int t = 0; t < W; ++t) f[t] = f[t] + P.cD * d[t];
+      }
+      if (P.E) {
+        unsigned rk = rj;
+        asm volatile("" : "+v"(rk) : "v"(f[W - 1]));
+        T e[W];
 #pragma unroll
-        for (int t = 0; t < H; ++t) e[t] = ld_at(P.E + ro[h + t], rj);
+        for (int t = 0; t < W; ++t) e[t] = ld_at(P.E + ro[t], rk);
 #pragma unroll
-        for (int t = 0; t < H; ++t) f[h + t] = f[h + t] + P.cE * e[t];
+        for (int t = 0; t < W; ++t) f[t] = f[t] + P.cE * e[t];
       }
     }
   }
