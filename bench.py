@@ -190,6 +190,7 @@ def batch_bench(args):
     import numpy as np
     import torch
 
+    from chapsim2_amd import _lib
     from chapsim2_amd.domain import Batch
     from chapsim2_amd.initial import channel_fields
 
@@ -207,6 +208,7 @@ def batch_bench(args):
     sec = time.perf_counter() - t0
     pts = sum(s["nc"][0] * s["nc"][1] * s["nc"][2] for s in specs)
     div = max(float(d.mass_check().max()) for d in b.domains)
+    tuned = int(_lib.load().chs_tune_report(None, 0))
     out = {
         "metric": "Mpts·RK3-steps/s, batched channels",
         "value": pts / 1e6 * args.steps / sec, "unit": "Mpts·RK3-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -216,7 +218,7 @@ def batch_bench(args):
                                ", ".join("x".join(str(v) for v in s["nc"]) for s in specs) +
                                "; Re_b 2800/6900, tanh beta in [1,2], CP6, mixed precision, exact projection, "
                                "one captured graph per RK3 step",
-                   "total_points": pts, "max_div_u": div},
+                   "total_points": pts, "max_div_u": div, "autotuned_shapes": tuned},
     }
     b.close()
     print(json.dumps(out))

@@ -21,7 +21,7 @@ EXPORTS = [
     "chs_props_create", "chs_props_destroy", "chs_props_info", "chs_props_from_rhoh", "chs_props_from_h",
     "chs_mlp_create", "chs_mlp_destroy", "chs_mlp_info", "chs_mlp_get_params", "chs_mlp_set_params",
     "chs_mlp_grad", "chs_mlp_train", "chs_mlp_eval",
-    "chs_inspect_coeff", "chs_inspect_op1d", "chs_inspect_projection", "chs_profile_begin", "chs_profile_end",
+    "chs_inspect_coeff", "chs_inspect_op1d", "chs_inspect_projection", "chs_profile_begin", "chs_profile_end", "chs_tune_report",
 ]
 
 
@@ -108,6 +108,7 @@ def load():
     L.chs_inspect_projection.argtypes = [C.POINTER(ChsDesc), dp, dp, dp, dp, dp]
     L.chs_profile_begin.argtypes = [vp, C.c_int]
     L.chs_profile_end.argtypes = [vp, dp, C.POINTER(C.c_longlong), dp]
+    L.chs_tune_report.argtypes = [C.c_char_p, C.c_int]
     for name in EXPORTS:
         if name != "chs_batch_instance":
             getattr(L, name).restype = getattr(L, name).restype or C.c_int

@@ -21,6 +21,7 @@ SOURCES = [
     "chs_plan.cpp",
     "chs_geom.cpp",
     "chs_exact.cpp",
+    "chs_tune.hip",
     "chs_lineop.hip",
     "chs_poisson.hip",
     "chs_flow.hip",

@@ -7,6 +7,7 @@
 #include "chs_handle.h"
 #include "chs_exact.h"
 #include "chs_poisson.h"
+#include "chs_tune.h"
 
 namespace chs {
 
@@ -507,6 +508,18 @@ int chs_profile_begin(chs_handle_t* H, int kernel_class) {
   h.prof_used = 0;
   h.prof_bytes = 0.0;
   return CHS_OK;
+}
+
+int chs_tune_report(char* buf, int len) {
+  const std::string r = chs::tune_report();
+  if (buf && len > 0) {
+    const size_t n = r.size() < (size_t)(len - 1) ? r.size() : (size_t)(len - 1);
+    std::memcpy(buf, r.data(), n);
+    buf[n] = 0;
+  }
+  int lines = 0;
+  for (char c : r) lines += (c == '\n');
+  return lines;
 }
 
 int chs_profile_end(chs_handle_t* H, double* ms_total, long long* launches, double* alg_bytes) {

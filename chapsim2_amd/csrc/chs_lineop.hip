@@ -33,6 +33,7 @@
 // precomputed, and the basic solution's first entry accumulated during the
 // forward sweep so the correction is applied in the same backward pass.
 #include "chs_lineop.h"
+#include "chs_tune.h"
 
 #include <cmath>
 #include <vector>
@@ -949,23 +950,46 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
     // per-plan coefficients
     const size_t shm_u = sizeof(T) * ((size_t)64 * lsu + (size_t)p.part_P * kChunk * kCo +
                                       4 * (size_t)p.part_P * p.part_P);
-    if (dir == 0 && shm_u <= max_lds_per_block()) {
-      a.ls = lsu;
-      dim3 g((unsigned)((a.nlines + 63) / 64)), b((unsigned)(64 * p.part_P));
-      dispatch_xu<T>(p.op, g, b, shm_u, s, a);
-    } else if (dir == 0) {
+    if (dir == 0) {
+      // two tile shapes for x lines: 64 whole lines in LDS (k_line_xu) or
+      // 256/P lines with skewed rows (k_line_x); timed once per shape when a
+      // scratch output is at hand (chs_tune.h), else the 64-line tile
+      const bool xu_fits = shm_u <= max_lds_per_block();
       int ls = lds_pos_host(nmax) + 1;
       ls += ((8 - ls) % 32 + 32) % 32;  // row stride = 8 (mod 32) doubles: neighbouring lines in other banks
-      a.ls = ls;
       // lines per workgroup: at most 256 threads and 48 KiB of LDS
       int nl = 256 / p.part_P;
       const int cap = (int)((48 * 1024) / (sizeof(T) * (size_t)(ls + 2 * p.part_P)));
       if (nl > cap) nl = cap;
-      if (nl < 1) return "line op: line does not fit in LDS";
-      a.nl = nl;
-      const size_t shm = sizeof(T) * ((size_t)a.nl * ls + (size_t)a.nl * 2 * p.part_P);
-      dim3 g((unsigned)((a.nlines + a.nl - 1) / a.nl)), b((unsigned)(a.nl * p.part_P));
-      dispatch_part<T, 0>(p.op, g, b, shm, s, a);
+      auto run = [&](int v, KArgs<T> ka) -> bool {
+        if (v == 0) {
+          if (!xu_fits) return false;
+          ka.ls = lsu;
+          dim3 g((unsigned)((ka.nlines + 63) / 64)), b((unsigned)(64 * p.part_P));
+          dispatch_xu<T>(p.op, g, b, shm_u, s, ka);
+        } else {
+          if (nl < 1) return false;
+          ka.ls = ls;
+          ka.nl = nl;
+          const size_t shm = sizeof(T) * ((size_t)nl * ls + (size_t)nl * 2 * p.part_P);
+          dim3 g((unsigned)((ka.nlines + nl - 1) / nl)), b((unsigned)(nl * p.part_P));
+          dispatch_part<T, 0>(p.op, g, b, shm, s, ka);
+        }
+        return hipGetLastError() == hipSuccess;
+      };
+      int v = -1;
+      if (scratch && xu_fits && nl >= 1) {
+        KArgs<T> ts = a;
+        ts.out = scratch;  // the candidates write the scratch array, never the output
+        ts.accumulate = 0;
+        const std::string key = std::string("line_x/") + (sizeof(T) == 8 ? "f64/" : "f32/") + std::to_string(p.op) +
+                                "/" + std::to_string(p.n_in) + "/" + std::to_string(p.n_out) + "/" +
+                                std::to_string(a.nlines) + "/" + std::to_string(input_kind(a));
+        v = tune(key, 2, s, [&](int vv) { return run(vv, ts); });
+      }
+      if (v < 0) v = xu_fits ? 0 : 1;
+      if (v == 1 && nl < 1) return "line op: line does not fit in LDS";
+      if (!run(v, a)) return std::string("line op launch: ") + hipGetErrorString(hipGetLastError());
     } else {
       // per-lane byte offsets from a workgroup's first line stay below 2^32
       // (a y-line block spans at most 64/e1 + 2 planes of x-y)

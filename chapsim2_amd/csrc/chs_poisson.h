@@ -38,6 +38,9 @@ class Poisson {
 
  private:
   static std::string create_3d(const Mesh& m, int iacc, std::unique_ptr<Poisson>* out);
+  // times the launch-shape variants of this solver's FP64 transforms once
+  // per shape (the choices are cached for the process)
+  std::string tune(bool ydir);
   template <typename T> std::string solve_t(T* rhs, hipStream_t s, T in_scale);
   template <typename T> std::string solve_3d_t(T* rhs, hipStream_t s, T in_scale);
   std::vector<int> rad_x, rad_y, rad_z;  // radix plans of nx/2, ny and nz

@@ -19,6 +19,7 @@
 #include <cstring>
 
 #include "chs_plan.h"
+#include "chs_tune.h"
 
 namespace chs {
 
@@ -488,28 +489,95 @@ __global__ void __launch_bounds__(LPW * A) k_fft2_x_inv(const cx<T>* __restrict_
 }
 
 // Launchers: the two-pass kernels for N in {64, 128, 256, 512}, the general
-// Stockham kernels otherwise.
+// Stockham kernels otherwise.  The lines per workgroup of a two-pass kernel
+// (variant v: 4, 8 or 16) are chosen by timing the variants once per shape
+// (chs_tune.h); untuned shapes use 16 (8 for N = 512).
 template <typename T, int A, int B>
 static size_t fft2_lds(int lpw) { return sizeof(cx<T>) * (size_t)lpw * Fft2<A, B>::LSL; }
+
+static int fft2_default_variant(int n) { return n >= 512 ? 1 : 2; }
+
+static std::string fft_key(const char* what, size_t tsize, int n, long nl, long no) {
+  return std::string(what) + (tsize == 8 ? "/f64/" : "/f32/") + std::to_string(n) + "/" + std::to_string(nl) + "x" +
+         std::to_string(no);
+}
+
+template <typename T, int A, int B, int LPW>
+static bool fft2_s_launch(cx<T>* W, int nxh, long ls, int no, long os, const cx<T>* tw, T sgn, hipStream_t s) {
+  dim3 g((unsigned)(((nxh + LPW - 1) / LPW) * no));
+  const size_t sh = fft2_lds<T, A, B>(LPW);
+  if (sgn < T(0))
+    hipLaunchKernelGGL((k_fft2_s<T, A, B, LPW, true>), g, dim3(LPW * A), sh, s, W, nxh, ls, os, tw);
+  else
+    hipLaunchKernelGGL((k_fft2_s<T, A, B, LPW, false>), g, dim3(LPW * A), sh, s, W, nxh, ls, os, tw);
+  return hipGetLastError() == hipSuccess;
+}
+
+template <typename T, int A, int B>
+static bool fft2_s_lpw(int v, cx<T>* W, int nxh, long ls, int no, long os, const cx<T>* tw, T sgn, hipStream_t s) {
+  switch (v) {
+    case 0: return fft2_s_launch<T, A, B, 4>(W, nxh, ls, no, os, tw, sgn, s);
+    case 1: return fft2_s_launch<T, A, B, 8>(W, nxh, ls, no, os, tw, sgn, s);
+    case 2: return fft2_s_launch<T, A, B, 16>(W, nxh, ls, no, os, tw, sgn, s);
+  }
+  return false;
+}
+
+// Two-pass strided transform of length n with variant v; false when n is not
+// a two-pass length.
+template <typename T>
+static bool fft2_s_variant(int v, cx<T>* W, int nxh, int n, long ls, int no, long os, const cx<T>* tw, T sgn,
+                           hipStream_t s) {
+  switch (n) {
+    case 64: return fft2_s_lpw<T, 8, 8>(v, W, nxh, ls, no, os, tw, sgn, s);
+    case 128: return fft2_s_lpw<T, 8, 16>(v, W, nxh, ls, no, os, tw, sgn, s);
+    case 256: return fft2_s_lpw<T, 16, 16>(v, W, nxh, ls, no, os, tw, sgn, s);
+    case 512: return fft2_s_lpw<T, 16, 32>(v, W, nxh, ls, no, os, tw, sgn, s);
+  }
+  return false;
+}
+
+template <typename T, int A, int B, int LPW>
+static bool fft2_x_launch(bool fwd, const T* in, cx<T>* W, T* out, long nlines, const cx<T>* tw, const cx<T>* rot,
+                          T sc, hipStream_t s) {
+  dim3 g((unsigned)((nlines + LPW - 1) / LPW));
+  const size_t sh = fft2_lds<T, A, B>(LPW);
+  if (fwd)
+    hipLaunchKernelGGL((k_fft2_x_fwd<T, A, B, LPW>), g, dim3(LPW * A), sh, s, in, W, nlines, tw, rot, sc);
+  else
+    hipLaunchKernelGGL((k_fft2_x_inv<T, A, B, LPW>), g, dim3(LPW * A), sh, s, W, out, nlines, tw, rot, sc);
+  return hipGetLastError() == hipSuccess;
+}
+
+template <typename T, int A, int B>
+static bool fft2_x_lpw(int v, bool fwd, const T* in, cx<T>* W, T* out, long nlines, const cx<T>* tw,
+                       const cx<T>* rot, T sc, hipStream_t s) {
+  switch (v) {
+    case 0: return fft2_x_launch<T, A, B, 4>(fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 1: return fft2_x_launch<T, A, B, 8>(fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 2: return fft2_x_launch<T, A, B, 16>(fwd, in, W, out, nlines, tw, rot, sc, s);
+  }
+  return false;
+}
+
+template <typename T>
+static bool fft2_x_variant(int v, bool fwd, const T* in, cx<T>* W, T* out, int nx, long nlines, const cx<T>* tw,
+                           const cx<T>* rot, T sc, hipStream_t s) {
+  switch (nx / 2) {
+    case 64: return fft2_x_lpw<T, 8, 8>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 128: return fft2_x_lpw<T, 8, 16>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 256: return fft2_x_lpw<T, 16, 16>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 512: return fft2_x_lpw<T, 16, 32>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+  }
+  return false;
+}
 
 template <typename T>
 static void launch_fft_s(cx<T>* W, int nxh, int n, long ls, int no, long os, const Radices& rp, const cx<T>* tw,
                          T sgn, hipStream_t s) {
-#define CHS_FFT2_S(A_, B_, LPW_)                                                                              \
-  {                                                                                                           \
-    dim3 g((unsigned)(((nxh + LPW_ - 1) / LPW_) * no));                                                       \
-    const size_t sh = fft2_lds<T, A_, B_>(LPW_);                                                              \
-    if (sgn < T(0))                                                                                           \
-      hipLaunchKernelGGL((k_fft2_s<T, A_, B_, LPW_, true>), g, dim3(LPW_ * A_), sh, s, W, nxh, ls, os, tw);  \
-    else                                                                                                      \
-      hipLaunchKernelGGL((k_fft2_s<T, A_, B_, LPW_, false>), g, dim3(LPW_ * A_), sh, s, W, nxh, ls, os, tw); \
-    return;                                                                                                   \
-  }
-  if (n == 64) CHS_FFT2_S(8, 8, 16)
-  if (n == 128) CHS_FFT2_S(8, 16, 16)
-  if (n == 256) CHS_FFT2_S(16, 16, 16)
-  if (n == 512) CHS_FFT2_S(16, 32, 8)
-#undef CHS_FFT2_S
+  int v = tuned_choice(fft_key(sgn < T(0) ? "fft_s+" : "fft_s-", sizeof(T), n, nxh, no));
+  if (v < 0) v = fft2_default_variant(n);
+  if (fft2_s_variant<T>(v, W, nxh, n, ls, no, os, tw, sgn, s)) return;
   const int lpb = kTileC / n;
   dim3 g((unsigned)(((nxh + lpb - 1) / lpb) * no));
   hipLaunchKernelGGL(k_fft_s<T>, g, dim3(kThreads), sizeof(cx<T>) * (size_t)lpb * n, s, W, nxh, n, ls, no, os, rp,
@@ -520,21 +588,9 @@ template <typename T>
 static void launch_fft_x(bool fwd, const T* in, cx<T>* W, T* out, int nx, long nlines, const Radices& rp,
                          const cx<T>* tw, const cx<T>* rot, T sc, hipStream_t s) {
   const int N = nx / 2;
-#define CHS_FFT2_X(A_, B_, LPW_)                                                                              \
-  {                                                                                                           \
-    dim3 g((unsigned)((nlines + LPW_ - 1) / LPW_));                                                           \
-    const size_t sh = fft2_lds<T, A_, B_>(LPW_);                                                              \
-    if (fwd)                                                                                                  \
-      hipLaunchKernelGGL((k_fft2_x_fwd<T, A_, B_, LPW_>), g, dim3(LPW_ * A_), sh, s, in, W, nlines, tw, rot, sc); \
-    else                                                                                                      \
-      hipLaunchKernelGGL((k_fft2_x_inv<T, A_, B_, LPW_>), g, dim3(LPW_ * A_), sh, s, W, out, nlines, tw, rot, sc); \
-    return;                                                                                                   \
-  }
-  if (N == 64) CHS_FFT2_X(8, 8, 16)
-  if (N == 128) CHS_FFT2_X(8, 16, 16)
-  if (N == 256) CHS_FFT2_X(16, 16, 16)
-  if (N == 512) CHS_FFT2_X(16, 32, 8)
-#undef CHS_FFT2_X
+  int v = tuned_choice(fft_key(fwd ? "fft_x+" : "fft_x-", sizeof(T), N, nlines, 1));
+  if (v < 0) v = fft2_default_variant(N);
+  if (fft2_x_variant<T>(v, fwd, in, W, out, nx, nlines, tw, rot, sc, s)) return;
   const int lpb = kTileC / N;
   dim3 g((unsigned)((nlines + lpb - 1) / lpb));
   const size_t sh = sizeof(cx<T>) * (size_t)lpb * N;
@@ -542,6 +598,30 @@ static void launch_fft_x(bool fwd, const T* in, cx<T>* W, T* out, int nx, long n
     hipLaunchKernelGGL(k_fft_x_fwd<T>, g, dim3(kThreads), sh, s, in, W, nx, nlines, rp, tw, rot, sc);
   else
     hipLaunchKernelGGL(k_fft_x_inv<T>, g, dim3(kThreads), sh, s, W, out, nx, nlines, rp, tw, rot, sc);
+}
+
+// Times the two-pass variants of every transform this solver launches (on
+// `tmp`, an array as large as the work array, and the work array itself;
+// their contents are overwritten by every solve).
+template <typename T>
+static void tune_transforms(int nx, int ny, int nz, int nxh, bool ydir, cx<T>* work, cx<T>* tmp, const cx<T>* twx,
+                            const cx<T>* twy, const cx<T>* twz, const cx<T>* rot) {
+  hipStream_t s = nullptr;
+  const long nlx = (long)ny * nz;
+  T* real = reinterpret_cast<T*>(tmp);
+  for (int fwd = 1; fwd >= 0; --fwd) {
+    tune(fft_key(fwd ? "fft_x+" : "fft_x-", sizeof(T), nx / 2, nlx, 1), 3, s, [&](int v) {
+      return fft2_x_variant<T>(v, fwd != 0, real, work, real, nx, nlx, twx, rot, T(1), s);
+    });
+    const T sgn = fwd ? T(-1) : T(1);
+    tune(fft_key(fwd ? "fft_s+" : "fft_s-", sizeof(T), nz, nxh, ny), 3, s, [&](int v) {
+      return fft2_s_variant<T>(v, work, nxh, nz, (long)nxh * ny, ny, (long)nxh, twz, sgn, s);
+    });
+    if (ydir)
+      tune(fft_key(fwd ? "fft_s+" : "fft_s-", sizeof(T), ny, nxh, nz), 3, s, [&](int v) {
+        return fft2_s_variant<T>(v, work, nxh, ny, (long)nxh, nz, (long)nxh * ny, twy, sgn, s);
+      });
+  }
 }
 
 // Triply periodic solve in spectral space: phi = -rhs / (kx'^2 + ky'^2 + kz'^2)
@@ -1216,6 +1296,7 @@ std::string Poisson::create(const Mesh& m, int kind, int iacc, std::unique_ptr<P
   if (!(e = upv(fd, &p->fd32)).empty() || !(e = upv(fz, &p->fz32)).empty()) return e;
   p->work_bytes = sizeof(double) * 2 * (size_t)p->nxh * ny * p->nz;
   PCHK(hipMalloc(&p->work, p->work_bytes));
+  if (!(e = p->tune(false)).empty()) return e;
   *out = std::move(p);
   return "";
 }
@@ -1290,7 +1371,20 @@ std::string Poisson::create_3d(const Mesh& m, int iacc, std::unique_ptr<Poisson>
   if (!(e = upv(k2[2], &p->lz64)).empty() || !(e = upv(k2[2], &p->lz32)).empty()) return e;
   p->work_bytes = sizeof(double) * 2 * (size_t)p->nxh * p->ny * p->nz;
   PCHK(hipMalloc(&p->work, p->work_bytes));
+  if (!(e = p->tune(true)).empty()) return e;
   *out = std::move(p);
+  return "";
+}
+
+std::string Poisson::tune(bool ydir) {
+  void* tmp = nullptr;
+  PCHK(hipMalloc(&tmp, work_bytes));
+  tune_transforms<double>(nx, ny, nz, nxh, ydir, reinterpret_cast<cx<double>*>(work), reinterpret_cast<cx<double>*>(tmp),
+                          reinterpret_cast<const cx<double>*>(tw_x64), reinterpret_cast<const cx<double>*>(tw_y64),
+                          reinterpret_cast<const cx<double>*>(tw_z64), reinterpret_cast<const cx<double>*>(rot64));
+  const hipError_t sync = hipDeviceSynchronize();
+  (void)hipFree(tmp);
+  if (sync != hipSuccess) return std::string("poisson tuning: ") + hipGetErrorString(sync);
   return "";
 }
 

@@ -43,6 +43,13 @@ int chs_inspect_projection(const chs_desc_t *desc, double *ly, double *lam, doub
 int chs_profile_begin(chs_handle_t *h, int kernel_class);
 int chs_profile_end(chs_handle_t *h, double *ms_total, long long *launches, double *alg_bytes);
 
+/* Launch shapes chosen so far by the autotuner (FFT lines per workgroup,
+ * x-line tile), one "family/shape=choice (us per variant)" line each, for
+ * every handle of the process.  Writes at most len-1 characters and a
+ * terminating 0; returns the number of tuned shapes.  No counterpart in the
+ * reference, which has no tunable launch shapes. */
+int chs_tune_report(char *buf, int len);
+
 #ifdef __cplusplus
 }
 #endif

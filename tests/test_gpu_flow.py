@@ -57,6 +57,14 @@ def test_poisson_transform_lengths(po, nc):
         out = np.array(rhs, order="F")
         solve_fft_poisson(out, dm)
         assert rel(out, ref) < 1e-12, rel(out, ref)
+        # the launch shapes of these transforms were chosen by timing them
+        import ctypes as C
+
+        buf = C.create_string_buffer(1 << 16)
+        assert dm.lib.chs_tune_report(buf, len(buf)) > 0
+        report = buf.value.decode()
+        for key in (f"fft_x+/f64/{nc[0] // 2}/", f"fft_x-/f64/{nc[0] // 2}/", f"fft_s+/f64/{nc[2]}/"):
+            assert key in report, (key, report)
     finally:
         dm.close()
 
