@@ -1,26 +1,29 @@
 // Compact line operators on gfx950.
 //
-// Partitioned kernels (k_line_yz, k_line_x; lines of up to kChunk*kMaxChunks
-// rows).  A line of n rows is cut into P = ceil(n/kChunk) chunks and one
-// thread owns one chunk, so a launch carries P times more threads than lines
-// (a 256x128x256 grid has only 32768-65536 lines per direction, which is too
-// few for 256 CUs).  Each thread evaluates the explicit RHS of its rows from a
-// register window, runs the chunk-local Thomas sweeps with host-factored
-// coefficients, publishes its first and last values through LDS, and corrects
-// its rows with the spike vectors and two rows of the inverse of the 2P x 2P
-// reduced system (the same algebra for wall-bounded and periodic lines; a
-// periodic line's corner entries close the reduced system cyclically).
+// Partitioned kernels (k_line_yz, k_line_xu, k_line_x; lines of up to
+// kChunkLong*kMaxChunks rows).  A line of n rows is cut into P chunks of at
+// most CH rows (CH = kChunk up to kChunk*kMaxChunks rows, kChunkLong beyond)
+// and one thread owns one chunk, so a launch carries P times more threads
+// than lines (a 256x128x256 grid has only 32768-65536 lines per direction,
+// which is too few for 256 CUs).  Each thread evaluates the explicit RHS of
+// its rows from a register window, runs the chunk-local Thomas sweeps with
+// host-factored coefficients, publishes its first and last values through
+// LDS, and corrects its rows with the spike vectors and two rows of the
+// inverse of the 2P x 2P reduced system (the same algebra for wall-bounded
+// and periodic lines; a periodic line's corner entries close the reduced
+// system cyclically).
 //  - y and z lines: a workgroup is 64 neighbouring lines x P chunks; wave q of
 //    the workgroup holds chunk q of 64 lines, so every load and store of a
 //    wavefront is a contiguous 64-element segment along x and the plan's
 //    per-row coefficients are wave-uniform.
-//  - x lines (contiguous in memory): a workgroup stages 256/P whole lines in
-//    LDS with coalesced loads (applying the input transform), threads work on
-//    their chunks from LDS, and the results go back through LDS to coalesced
-//    stores.  Row k of a line sits at LDS offset k + k/16 so that the 16 chunks
-//    of a line fall into distinct banks.
+//  - x lines (contiguous in memory) of up to kChunk*kMaxChunks rows: a
+//    workgroup stages 64 whole lines in LDS with coalesced loads (applying
+//    the input transform), wave q works on chunk q of all 64 lines from LDS,
+//    and the results go back through LDS to coalesced stores (k_line_xu; or
+//    256/P lines with skewed rows, k_line_x).  Longer x lines take the y/z
+//    kernel with x addressing (lanes one line apart, rows contiguous).
 //
-// Single-thread-per-line kernel (k_line; longer lines).  For y- and z-lines consecutive threads own
+// Single-thread-per-line kernel (k_line; lines beyond kChunkLong*kMaxChunks rows).  For y- and z-lines consecutive threads own
 // consecutive x positions, so every load and store of a wavefront is a
 // contiguous 64-element segment.  The explicit RHS (src/basics_operations2.f90
 // :1980-2719) is evaluated on the fly with a 5-point register window; the four
@@ -62,9 +65,9 @@ static std::string up(const std::vector<double>& v, T** dst) {
   return "";
 }
 
-// Line index read for window position k, -2 <= k <= n + kChunk + 1:
+// Line index read for window position k, -2 <= k <= n + CH + 1 (CH the chunk rows):
 // wrapped on periodic lines (one period suffices, as periodic lines on the
-// partitioned kernels have n >= kChunk + 2), clamped to the line otherwise.
+// partitioned kernels have n >= CH + 2), clamped to the line otherwise.
 __host__ __device__ __forceinline__ int window_index(int k, int n, bool wrap) {
   if (wrap) return k < 0 ? k + n : (k >= n ? k - n : k);
   return k < 0 ? 0 : (k > n - 1 ? n - 1 : k);
@@ -94,10 +97,10 @@ std::string upload_plan(const LinePlan& p, DevPlan<T>* dp) {
   if (!(e = up(p.rinv, &dp->rinv)).empty()) return e;
   // dense window weights of the four boundary rows
   {
-    const int W = kChunk + 4;
+    const int W = p.part_ch + 4;
     const bool wrap = (p.ibc[0] == BC_PERIODIC);
     std::vector<double> sw(4 * W, 0.0);
-    bool ok = p.part_P >= 1 && (int)p.part_start.size() == p.part_P + 1 && (!wrap || p.n_in >= kChunk + 2);
+    bool ok = p.part_P >= 1 && (int)p.part_start.size() == p.part_P + 1 && (!wrap || p.n_in >= p.part_ch + 2);
     for (int q = 0; q < 4 && ok; ++q) {
       const int s0 = (q < 2) ? 0 : p.part_start[p.part_P - 1];
       for (int t = 0; t < p.rows[q].nterm; ++t) {
@@ -128,19 +131,20 @@ std::string upload_plan(const LinePlan& p, DevPlan<T>* dp) {
     dp->win_ok = ok;
     if (ok && !(e = up(sw, &dp->swin)).empty()) return e;
   }
-  // chunk coefficient rows: kChunk rows per chunk, kCo entries per row,
+  // chunk coefficient rows: CH = part_ch rows per chunk, kCo entries per row,
   // paired for 16-byte reads: (ld local pivot inverse, a sub-diagonal; a = 0
   // on the chunk's first row), (the two spike vectors), (lc local
   // super-diagonal, 0 on the chunk's last row; the output metric, 1 when
   // absent); rows past the chunk's length are identity rows
-  if (p.part_P >= 1 && (int)p.part_start.size() == p.part_P + 1 && p.part_P * kChunk >= p.n_out) {
-    std::vector<double> co((size_t)p.part_P * kChunk * kCo, 0.0);
+  const int CH = p.part_ch;
+  if (p.part_P >= 1 && (int)p.part_start.size() == p.part_P + 1 && p.part_P * CH >= p.n_out) {
+    std::vector<double> co((size_t)p.part_P * CH * kCo, 0.0);
     const bool solve_data = !p.ld.empty();
     for (int q = 0; q < p.part_P; ++q) {
       const int s0 = p.part_start[q], len = p.part_start[q + 1] - s0;
-      if (len > kChunk || s0 != (q * p.n_out) / p.part_P) return "line plan: chunk layout differs from the kernels'";
-      for (int i = 0; i < kChunk; ++i) {
-        double* c = &co[((size_t)q * kChunk + i) * kCo];
+      if (len > CH || s0 != (q * p.n_out) / p.part_P) return "line plan: chunk layout differs from the kernels'";
+      for (int i = 0; i < CH; ++i) {
+        double* c = &co[((size_t)q * CH + i) * kCo];
         const int r = s0 + i;
         c[CO_LD] = 1.0;
         c[CO_MET] = 1.0;
@@ -182,9 +186,9 @@ struct KArgs {
   // partitioned form
   int P, per, ls, nl;
   const T* rinv;
-  const T* swin;  // 4 x (kChunk + 4) boundary-row window weights
+  const T* swin;  // 4 x (CH + 4) boundary-row window weights (CH = the plan's chunk rows)
   int wrap;       // periodic line: windows wrap around instead of clamping
-  const T* co;    // P x kChunk coefficient rows of kCo entries (see upload_plan)
+  const T* co;    // P x CH coefficient rows of kCo entries (see upload_plan)
 };
 
 // bulk offsets (p1,p2,p3,p4) per operator kind, relative to the output row
@@ -305,16 +309,15 @@ __global__ void __launch_bounds__(64) k_line(KArgs<T> P) {
 // Chunk rows [s0, s0+len) of one line: explicit RHS into y[] from the
 // window f[t] = input at line index s0-2+t (clamped to the line).  The four
 // boundary rows use their dense window weights (plans whose boundary recipes
-// reach outside the window run on the one-thread-per-line kernel instead).
-template <typename T, int OPK>
-__device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int len, const T (&f)[kChunk + 4],
-                                          const T (&fb4)[4], T (&y)[kChunk]) {
+// reach outside the window run on the one-thread-per-line kernel instead);
+// they are formed first, so that the interior rows can then consume the
+// window in order (a window value is dead once the row two below it is
+// done, and y[] takes its registers).
+template <typename T, int OPK, int CH>
+__device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int len, const T (&f)[CH + 4],
+                                          const T (&fb4)[4], T (&y)[CH]) {
   using O = Offs<OPK>;
-  constexpr int W = kChunk + 4;
-#pragma unroll
-  for (int i = 0; i < kChunk; ++i)
-    y[i] = (P.c1 * (f[i + 2 + O::p1] + P.sgn * f[i + 2 + O::p2]) +
-            P.c2 * (f[i + 2 + O::p3] + P.sgn * f[i + 2 + O::p4])) * P.dd;
+  constexpr int W = CH + 4;
   auto special = [&](int rq) -> T {
     const DevRow<T>& R = P.rows[rq];
     T v = T(0);
@@ -325,14 +328,27 @@ __device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int len, con
       for (int m = 0; m < 4; ++m) v = v + R.u[m] * fb4[m];
     return v * P.dd;
   };
-  if (q == 0) {
-    y[0] = special(0);
-    y[1] = special(1);
+  const bool first = q == 0, last = q == P.P - 1;
+  T v0 = T(0), v1 = T(0), v2 = T(0), v3 = T(0);
+  if (first) {
+    v0 = special(0);
+    v1 = special(1);
   }
-  if (q == P.P - 1) {
-    const T v2 = special(2), v3 = special(3);
+  if (last) {
+    v2 = special(2);
+    v3 = special(3);
+  }
 #pragma unroll
-    for (int i = 0; i < kChunk; ++i) {
+  for (int i = 0; i < CH; ++i)
+    y[i] = (P.c1 * (f[i + 2 + O::p1] + P.sgn * f[i + 2 + O::p2]) +
+            P.c2 * (f[i + 2 + O::p3] + P.sgn * f[i + 2 + O::p4])) * P.dd;
+  if (first) {
+    y[0] = v0;
+    y[1] = v1;
+  }
+  if (last) {
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
       if (i == len - 2) y[i] = v2;
       if (i == len - 1) y[i] = v3;
     }
@@ -375,31 +391,33 @@ __device__ __forceinline__ void transform_in(const KArgs<T>& P, T (&f)[NV], cons
   }
 }
 
-// Window of one y/z line chunk: rows s0-2 .. s0+kChunk+1 (wrapped on
-// periodic lines, clamped otherwise, so every address is valid).  The row
+// Window rows [T0, T1) of one line chunk: line indices s0-2+t (wrapped on
+// periodic lines, clamped otherwise, so every address is valid); the row
 // offsets are wave-uniform.  IN = 0: A only; 1: A and the second input array
 // (B, or Cm when B is absent), all loads issued before any use; 2: as 1 (the
 // second array optional), then the additive arrays D (and E) as a second
 // batch.
-template <typename T, int IN>
-__device__ __forceinline__ void load_window(const KArgs<T>& P, long base, unsigned ri, long is, int s0,
-                                            T (&f)[kChunk + 4]) {
-  constexpr int W = kChunk + 4;
-  long ro[W];
+template <typename T, int IN, int CH, int T0, int T1>
+__device__ __forceinline__ void load_rows(const KArgs<T>& P, long base, unsigned ri, long is, int s0,
+                                          T (&f)[CH + 4]) {
+  constexpr int NB = T1 - T0;
+  long ro[NB];
 #pragma unroll
-  for (int t = 0; t < W; ++t) ro[t] = base + is * window_index(s0 - 2 + t, P.n_in, P.wrap);
+  for (int t = 0; t < NB; ++t) ro[t] = base + is * window_index(s0 - 2 + T0 + t, P.n_in, P.wrap);
+  T* fw = f + T0;
+  T(&fv)[NB] = *reinterpret_cast<T(*)[NB]>(fw);
   const T* S = P.B ? P.B : P.Cm;
   if (IN == 1 || (IN == 2 && S)) {
-    T g[W];
+    T g[NB];
 #pragma unroll
-    for (int t = 0; t < W; ++t) {
-      f[t] = ld_at(P.A + ro[t], ri);
+    for (int t = 0; t < NB; ++t) {
+      fv[t] = ld_at(P.A + ro[t], ri);
       g[t] = ld_at(S + ro[t], ri);
     }
-    transform_in<T, W>(P, f, g, [&](int t) { return ld_at(P.Cm + ro[t], ri); });
+    transform_in<T, NB>(P, fv, g, [&](int t) { return ld_at(P.Cm + ro[t], ri); });
   } else {
 #pragma unroll
-    for (int t = 0; t < W; ++t) f[t] = P.sA * ld_at(P.A + ro[t], ri);
+    for (int t = 0; t < NB; ++t) fv[t] = P.sA * ld_at(P.A + ro[t], ri);
   }
   if (IN == 2) {
     // The additive arrays are loaded after the first batch has been
@@ -408,64 +426,84 @@ __device__ __forceinline__ void load_window(const KArgs<T>& P, long base, unsign
     // so the compiler cannot hoist these loads next to the earlier batch
     // (registers for the window and all arrays together spill).
     unsigned rj = ri;
-    asm volatile("" : "+v"(rj) : "v"(f[W - 1]));
+    asm volatile("" : "+v"(rj) : "v"(fv[NB - 1]));
     if (sizeof(T) == 4) {
       // FP32: both arrays in one batch
-      T d[W], e[W];
+      T d[NB], e[NB];
 #pragma unroll
-      for (int t = 0; t < W; ++t) {
+      for (int t = 0; t < NB; ++t) {
         d[t] = ld_at(P.D + ro[t], rj);
         e[t] = P.E ? ld_at(P.E + ro[t], rj) : T(0);
       }
 #pragma unroll
-      for (int t = 0; t < W; ++t) f[t] = f[t] + P.cD * d[t] + P.cE * e[t];
+      for (int t = 0; t < NB; ++t) fv[t] = fv[t] + P.cD * d[t] + P.cE * e[t];
     } else {
       // FP64: one array at a time
       {
-        T d[W];
+        T d[NB];
 #pragma unroll
-        for (int t = 0; t < W; ++t) d[t] = ld_at(P.D + ro[t], rj);
+        for (int t = 0; t < NB; ++t) d[t] = ld_at(P.D + ro[t], rj);
 #pragma unroll
-        for (int t = 0; t < W; ++t) f[t] = f[t] + P.cD * d[t];
+        for (int t = 0; t < NB; ++t) fv[t] = fv[t] + P.cD * d[t];
       }
       if (P.E) {
         unsigned rk = rj;
-        asm volatile("" : "+v"(rk) : "v"(f[W - 1]));
-        T e[W];
+        asm volatile("" : "+v"(rk) : "v"(fv[NB - 1]));
+        T e[NB];
 #pragma unroll
-        for (int t = 0; t < W; ++t) e[t] = ld_at(P.E + ro[t], rk);
+        for (int t = 0; t < NB; ++t) e[t] = ld_at(P.E + ro[t], rk);
 #pragma unroll
-        for (int t = 0; t < W; ++t) f[t] = f[t] + P.cE * e[t];
+        for (int t = 0; t < NB; ++t) fv[t] = fv[t] + P.cE * e[t];
       }
     }
   }
 }
 
-// Chunk-local Thomas sweeps over all kChunk rows of a chunk whose
+// Window of one line chunk of CH rows: rows s0-2 .. s0+CH+1.  FP64 windows
+// of long chunks (CH > kChunk) are loaded in two halves, the second half's
+// loads waiting (through an empty asm on its lane offset) until the first
+// half is combined, so the raw input arrays of the whole window never hold
+// registers at the same time.
+template <typename T, int IN, int CH>
+__device__ __forceinline__ void load_window(const KArgs<T>& P, long base, unsigned ri, long is, int s0,
+                                            T (&f)[CH + 4]) {
+  constexpr int W = CH + 4;
+  if constexpr (sizeof(T) == 8 && CH > kChunk) {
+    constexpr int H = W / 2;
+    load_rows<T, IN, CH, 0, H>(P, base, ri, is, s0, f);
+    unsigned rh = ri;
+    asm volatile("" : "+v"(rh) : "v"(f[H - 1]));
+    load_rows<T, IN, CH, H, W>(P, base, rh, is, s0, f);
+  } else {
+    load_rows<T, IN, CH, 0, W>(P, base, ri, is, s0, f);
+  }
+}
+
+// Chunk-local Thomas sweeps over all CH rows of a chunk whose
 // coefficient rows start at c; returns the chunk's first and last values.
 // The coefficient rows past the chunk's length are identity rows (and the
 // chunk's last row does not couple to the next), so the sweeps need no row
 // guards.
-template <typename T>
-__device__ __forceinline__ void chunk_solve(const T* c, int len, T (&y)[kChunk], T& yf, T& yl) {
+template <typename T, int CH>
+__device__ __forceinline__ void chunk_solve(const T* c, int len, T (&y)[CH], T& yf, T& yl) {
   y[0] = y[0] * c[CO_LD];
 #pragma unroll
-  for (int i = 1; i < kChunk; ++i) y[i] = (y[i] - c[i * kCo + CO_AR] * y[i - 1]) * c[i * kCo + CO_LD];
+  for (int i = 1; i < CH; ++i) y[i] = (y[i] - c[i * kCo + CO_AR] * y[i - 1]) * c[i * kCo + CO_LD];
 #pragma unroll
-  for (int i = kChunk - 2; i >= 0; --i) y[i] = y[i] - c[i * kCo + CO_LC] * y[i + 1];
+  for (int i = CH - 2; i >= 0; --i) y[i] = y[i] - c[i * kCo + CO_LC] * y[i + 1];
   yf = y[0];
   yl = y[0];
 #pragma unroll
-  for (int i = 1; i < kChunk; ++i)
+  for (int i = 1; i < CH; ++i)
     if (i == len - 1) yl = y[i];
 }
 
 // Reduced-system values coupling chunk q to its neighbours (rinv: the
 // 2P x 2P inverse; ex(k) reads the published first/last value k of the
 // thread's line), applied through the spike vectors of the chunk's rows c.
-template <typename T, class EX>
+template <typename T, int CH, class EX>
 __device__ __forceinline__ void chunk_couple(const KArgs<T>& P, int q, const T* rinv, const T* c, EX&& ex,
-                                             T (&y)[kChunk]) {
+                                             T (&y)[CH]) {
   const int Pn = P.P, M = 2 * Pn;
   const int qm = (q + Pn - 1) % Pn, qp = (q + 1) % Pn;
   const T* rL = rinv + (2 * qm + 1) * M;
@@ -479,7 +517,7 @@ __device__ __forceinline__ void chunk_couple(const KArgs<T>& P, int q, const T* 
   if (!P.per && q == 0) uL = T(0);
   if (!P.per && q == Pn - 1) uF = T(0);
 #pragma unroll
-  for (int i = 0; i < kChunk; ++i) y[i] = y[i] - c[i * kCo + CO_SV] * uL - c[i * kCo + CO_SW] * uF;
+  for (int i = 0; i < CH; ++i) y[i] = y[i] - c[i * kCo + CO_SV] * uL - c[i * kCo + CO_SW] * uF;
 }
 
 // Row i of a chunk (line row r): metric scaling and Neumann overrides.
@@ -491,18 +529,24 @@ __device__ __forceinline__ T chunk_post(const KArgs<T>& P, const T* c, int i, in
   return val;
 }
 
-// Per-plan coefficient data a y/z workgroup copies into LDS: the chunk rows
-// (P x kChunk x kCo) and the reduced-system inverse (2P x 2P), at most
-// kStageMax values per thread (P <= kMaxChunks).
-constexpr int kStageMax = 3;
-static_assert(64 * kMaxChunks * kStageMax >= kMaxChunks * kChunk * kCo + 4 * kMaxChunks * kMaxChunks &&
+// Per-plan coefficient data a workgroup of P waves copies into LDS: the chunk
+// rows (P x CH x kCo) and the reduced-system inverse (2P x 2P), at most
+// stage_max(CH) = CH kCo / 64 + 1 values per thread (P <= kMaxChunks).
+__host__ __device__ constexpr int stage_max(int ch) { return ch * kCo / 64 + 1; }
+constexpr int kStageMax = stage_max(kChunk);
+static_assert(64 * kMaxChunks * stage_max(kChunkLong) >= kMaxChunks * kChunkLong * kCo + 4 * kMaxChunks * kMaxChunks &&
+                  64 * kMaxChunks * kStageMax >= kMaxChunks * kChunk * kCo + 4 * kMaxChunks * kMaxChunks &&
                   64 * kStageMax >= kChunk * kCo + 4,
-              "per-plan staging exceeds kStageMax values per thread");
+              "per-plan staging exceeds stage_max values per thread");
 
-// First elements of y or z line L in the input, output and fbc arrays.
+// First elements of line L in the input, output and fbc arrays.
 template <int DIR, typename T>
 __device__ __forceinline__ void yz_offsets(const KArgs<T>& P, int L, long& ib, long& ob, long& fb) {
-  if (DIR == 1) {
+  if (DIR == 0) {  // x lines (long lines only): contiguous rows, lanes one line apart
+    ib = (long)L * P.e1;
+    ob = (long)L * P.o1;
+    fb = 4L * L;
+  } else if (DIR == 1) {
     const int i = L % P.e1, k = L / P.e1;
     ib = i + (long)P.e1 * P.e2 * k;
     ob = i + (long)P.o1 * P.o2 * k;
@@ -514,25 +558,28 @@ __device__ __forceinline__ void yz_offsets(const KArgs<T>& P, int L, long& ib, l
   }
 }
 
-// One y/z workgroup: 64 neighbouring lines x P chunks, wave q holding
-// chunk q of the 64 lines; IN as in load_window.  Single-input operators ask
-// for 5 waves per SIMD (at most 96 registers), so that two workgroups of
-// 9 waves (lines of 129 rows) share a CU.
-template <typename T, int DIR, int OPK, int IN>
-__global__ void __launch_bounds__(64 * kMaxChunks) __attribute__((amdgpu_waves_per_eu(IN == 0 ? 5 : 4)))
-k_line_yz(KArgs<T> P) {
+// One y/z workgroup: 64 neighbouring lines x P chunks of CH rows, wave q
+// holding chunk q of the 64 lines; IN as in load_window.  CH = kChunkLong
+// also serves x lines longer than a 64-line LDS tile (DIR = 0: the lanes
+// are 64 consecutive lines, each reading its own contiguous rows).
+// Single-input operators ask for 5 waves per SIMD (at most 96 registers), so
+// that two workgroups of 9 waves (lines of 129 rows) share a CU.
+template <typename T, int DIR, int OPK, int IN, int CH>
+__global__ void __launch_bounds__(64 * kMaxChunks)
+    __attribute__((amdgpu_waves_per_eu(IN == 0 && CH == kChunk ? 5 : 4))) k_line_yz(KArgs<T> P) {
   __shared__ T ex[2 * kMaxChunks * 64];
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sco = reinterpret_cast<T*>(smem_raw);  // P*kChunk*kCo chunk rows, then the 2P x 2P inverse
+  T* sco = reinterpret_cast<T*>(smem_raw);  // P*CH*kCo chunk rows, then the 2P x 2P inverse
   // loads of the per-plan coefficients first (written to LDS once the
   // chunk's window loads are in flight); plans without a line solve (the
   // explicit schemes) have no inverse
-  const int nst = P.P * kChunk * kCo + (P.rinv ? 4 * P.P * P.P : 0);
-  T cv[kStageMax];
+  constexpr int NS = stage_max(CH);
+  const int nst = P.P * CH * kCo + (P.rinv ? 4 * P.P * P.P : 0);
+  T cv[NS];
   {
-    const int nco = P.P * kChunk * kCo;
+    const int nco = P.P * CH * kCo;
 #pragma unroll
-    for (int u = 0; u < kStageMax; ++u) {
+    for (int u = 0; u < NS; ++u) {
       const int idx = threadIdx.x + u * 64 * P.P;
       cv[u] = (idx < nco) ? P.co[idx] : (idx < nst ? P.rinv[idx - nco] : T(0));
     }
@@ -549,28 +596,34 @@ k_line_yz(KArgs<T> P) {
   yz_offsets<DIR>(P, L0, ib0, ob0, fb0);
   const unsigned ri = (unsigned)((ib - ib0) * (long)sizeof(T));
   const unsigned ro = (unsigned)((ob - ob0) * (long)sizeof(T));
-  const long is = (DIR == 1) ? (long)P.e1 : (long)P.e1 * P.e2;
-  const long os = (DIR == 1) ? (long)P.o1 : (long)P.o1 * P.o2;
+  const long is = (DIR == 0) ? 1L : (DIR == 1) ? (long)P.e1 : (long)P.e1 * P.e2;
+  const long os = (DIR == 0) ? 1L : (DIR == 1) ? (long)P.o1 : (long)P.o1 * P.o2;
   const int n_out = P.n_out, Pn = P.P;
   const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
-  T f[kChunk + 4];
-  load_window<T, IN>(P, ib0, ri, is, s0, f);
+  auto stage = [&]() {
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      const int idx = threadIdx.x + u * 64 * Pn;
+      if (idx < nst) sco[idx] = cv[u];
+    }
+  };
+  // long chunks (FP64: the window and y[] fill the registers) store the
+  // staged coefficients before their window loads
+  if constexpr (CH > kChunk) stage();
+  T f[CH + 4];
+  load_window<T, IN, CH>(P, ib0, ri, is, s0, f);
   T fb4[4] = {T(0), T(0), T(0), T(0)};
   if (P.fbc && (q == 0 || q == Pn - 1))
     for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[fb + is * m];
-#pragma unroll
-  for (int u = 0; u < kStageMax; ++u) {
-    const int idx = threadIdx.x + u * 64 * Pn;
-    if (idx < nst) sco[idx] = cv[u];
-  }
+  if constexpr (CH <= kChunk) stage();
   __syncthreads();
-  const T* c = sco + q * kChunk * kCo;
-  const T* rinv = sco + Pn * kChunk * kCo;
-  T y[kChunk];
+  const T* c = sco + q * CH * kCo;
+  const T* rinv = sco + Pn * CH * kCo;
+  T y[CH];
   chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
   if (P.solve) {
     T yf, yl;
-    chunk_solve<T>(c, len, y, yf, yl);
+    chunk_solve(c, len, y, yf, yl);
     ex[lane + 64 * (2 * q)] = yf;
     ex[lane + 64 * (2 * q + 1)] = yl;
     __syncthreads();
@@ -580,9 +633,9 @@ k_line_yz(KArgs<T> P) {
   T* const Ob = P.out + ob0;
   // accumulating outputs read the previous values in two batches of half a
   // chunk (fewer registers held across the loads)
-  constexpr int H = kChunk / 2;
+  constexpr int H = CH / 2;
 #pragma unroll
-  for (int h = 0; h < kChunk; h += H) {
+  for (int h = 0; h < CH; h += H) {
     T prev[H];
 #pragma unroll
     for (int i = 0; i < H; ++i) prev[i] = T(0);
@@ -649,7 +702,7 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
   chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
   if (P.solve) {
     T yf, yl;
-    chunk_solve<T>(P.co + q * kChunk * kCo, len, y, yf, yl);
+    chunk_solve(P.co + q * kChunk * kCo, len, y, yf, yl);
     T* ex = exb + (long)ll * 2 * Pn;
     ex[2 * q] = yf;
     ex[2 * q + 1] = yl;
@@ -782,7 +835,7 @@ __global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
   chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
   if (P.solve) {
     T yf, yl;
-    chunk_solve<T>(c, len, y, yf, yl);
+    chunk_solve(c, len, y, yf, yl);
     __syncthreads();  // every wave has read its window: the tile's start takes the exchange
     exb[lane + 64 * (2 * q)] = yf;
     exb[lane + 64 * (2 * q + 1)] = yl;
@@ -863,15 +916,26 @@ static size_t max_lds_per_block() {
   return cached[dev];
 }
 
-template <typename T, int DIR, int IN>
+template <typename T, int DIR, int IN, int CH = kChunk>
 static void dispatch_yz(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
   switch (op) {
-    case MIDP_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_C2P, IN>), g, b, shm, s, a); break;
-    case MIDP_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_P2C, IN>), g, b, shm, s, a); break;
-    case D1_C2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2C, IN>), g, b, shm, s, a); break;
-    case D1_P2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2P, IN>), g, b, shm, s, a); break;
-    case D1_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2P, IN>), g, b, shm, s, a); break;
-    case D1_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2C, IN>), g, b, shm, s, a); break;
+    case MIDP_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_C2P, IN, CH>), g, b, shm, s, a); break;
+    case MIDP_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_P2C, IN, CH>), g, b, shm, s, a); break;
+    case D1_C2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2C, IN, CH>), g, b, shm, s, a); break;
+    case D1_P2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2P, IN, CH>), g, b, shm, s, a); break;
+    case D1_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2P, IN, CH>), g, b, shm, s, a); break;
+    case D1_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2C, IN, CH>), g, b, shm, s, a); break;
+  }
+}
+
+// lines of kChunkLong-row chunks (more than kChunk * kMaxChunks rows), any
+// direction
+template <typename T, int DIR>
+static void dispatch_long(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
+  switch (input_kind(a)) {
+    case 0: dispatch_yz<T, DIR, 0, kChunkLong>(op, g, b, shm, s, a); break;
+    case 1: dispatch_yz<T, DIR, 1, kChunkLong>(op, g, b, shm, s, a); break;
+    default: dispatch_yz<T, DIR, 2, kChunkLong>(op, g, b, shm, s, a); break;
   }
 }
 
@@ -950,7 +1014,20 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
     // per-plan coefficients
     const size_t shm_u = sizeof(T) * ((size_t)64 * lsu + (size_t)p.part_P * kChunk * kCo +
                                       4 * (size_t)p.part_P * p.part_P);
-    if (dir == 0) {
+    if (p.part_ch == kChunkLong) {
+      // long lines: the lane-per-line kernel with chunks of kChunkLong rows;
+      // per-lane byte offsets from a workgroup's first line stay below 2^32
+      const long span = (dir == 0) ? 64L * (ein.n[0] > eout.n[0] ? ein.n[0] : eout.n[0])
+                         : (dir == 1) ? (long)ein.n[0] * (ein.n[1] > eout.n[1] ? ein.n[1] : eout.n[1]) *
+                                            (64 / ein.n[0] + 2)
+                                      : 64L;
+      if (span * (long)sizeof(T) >= (1L << 32)) return "line op: lines too far apart for 32-bit offsets";
+      dim3 g((unsigned)((a.nlines + 63) / 64)), b((unsigned)(64 * p.part_P));
+      const size_t shm = sizeof(T) * ((size_t)p.part_P * kChunkLong * kCo + 4 * (size_t)p.part_P * p.part_P);
+      if (dir == 0) dispatch_long<T, 0>(p.op, g, b, shm, s, a);
+      else if (dir == 1) dispatch_long<T, 1>(p.op, g, b, shm, s, a);
+      else dispatch_long<T, 2>(p.op, g, b, shm, s, a);
+    } else if (dir == 0) {
       // two tile shapes for x lines: 64 whole lines in LDS (k_line_xu) or
       // 256/P lines with skewed rows (k_line_x); timed once per shape when a
       // scratch output is at hand (chs_tune.h), else the 64-line tile

@@ -628,7 +628,8 @@ std::string make_line_plan(const LineGeom& g, Op op, int acc, const int ibc0[2],
   // chunk boundaries of the partitioned evaluation (all plans)
   {
     const int n = n_out;
-    int P = (n + kChunk - 1) / kChunk;
+    p.part_ch = (n <= kChunk * kMaxChunks) ? kChunk : kChunkLong;
+    int P = (n + p.part_ch - 1) / p.part_ch;
     if (P < 1) P = 1;
     p.part_P = P;
     p.part_start.resize(P + 1);

@@ -39,6 +39,7 @@ struct LineGeom {
 
 constexpr int kMaxRowTerms = 12;
 constexpr int kChunk = 16;      // rows per thread in the partitioned solve
+constexpr int kChunkLong = 32;  // rows per thread for lines longer than kChunk * kMaxChunks
 constexpr int kMaxChunks = 16;  // chunks (waves) per line
 
 // One boundary row: rhs = (sum_t w[t] * f[idx[t]] + sum_m u[m] * fbc[m]) * dd
@@ -73,9 +74,11 @@ struct LinePlan {
   // factors) and corrected with precomputed spike vectors sv (coupling to the
   // previous chunk's last unknown) and sw (to the next chunk's first unknown),
   // whose values follow from the 2P x 2P reduced system (inverse rinv).
-  // Chunk q holds rows [q*n_out/P, (q+1)*n_out/P) with P = ceil(n_out/kChunk);
+  // Chunk q holds rows [q*n_out/P, (q+1)*n_out/P) with P = ceil(n_out/part_ch),
+  // part_ch = kChunk up to kChunk * kMaxChunks rows and kChunkLong beyond;
   // the GPU kernels use this form when P <= kMaxChunks.
   int part_P = 1;
+  int part_ch = kChunk;
   std::vector<int> part_start;     // P+1 chunk boundaries
   std::vector<double> a_raw, b_raw, c_raw;
   std::vector<double> ld, lc, sv, sw, rinv;

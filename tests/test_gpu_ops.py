@@ -153,24 +153,47 @@ def test_ops_long_lines(po):
     assert nchecked > 40
 
 
-def test_ops_512_lines(po):
-    """Lines of 512 points (the operator benchmark's x and z extents) take
-    the single-thread-per-line kernel; parity against the oracle."""
+@pytest.mark.parametrize("nc,d", [((512, 8, 24), 0), ((300, 8, 12), 0), ((16, 400, 8), 1), ((16, 257, 8), 1),
+                                  ((16, 511, 8), 1), ((8, 12, 512), 2), ((16, 8, 450), 2)])
+def test_ops_long_chunk_lines(po, nc, d):
+    """Lines of 257-512 rows take the partitioned kernels with chunks of up
+    to 32 rows (x lines with the lanes one line apart, y and z lines with
+    the lanes along x; y lines of 512 p-located rows exceed 16 chunks and
+    take the single-thread-per-line kernel); every operator and accuracy,
+    periodic x and z, wall-bounded y with Dirichlet, Neumann and interior
+    boundary kinds, against the oracle."""
     from chapsim2_amd import operations as ops
     from chapsim2_amd.domain import Domain
 
-    nc = (512, 24, 512)
     dm = Domain(nc, LX, -1.0, 1.0, LZ, istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1)
     odm = po.Dom(list(nc), [1, 0, 1], LX, -1.0, 1.0, LZ, 2, 0.075)
-    rng = np.random.default_rng(23)
+    rng = np.random.default_rng(23 + d)
+    nchecked = 0
     try:
-        for d in (0, 2):
-            for op in (0, 4, 5):
-                ext_in, ext_out = list(nc), list(nc)
-                fi = np.asfortranarray(rng.standard_normal(ext_in))
-                ref = odm.op3d(d, op, 4, [1, 1], fi, ext_out)
-                out = np.zeros(ext_out, order="F")
-                ops.op3d(dm, d, op, fi, out, 4, [1, 1], None)
-                assert np.abs(out - ref).max() / np.abs(ref).max() < RTOL, (d, op)
+        for op in range(6):
+            for acc in (1, 2, 3, 4):
+                bcs = [((1, 1), False)] if d != 1 else [((4, 4), True), ((5, 5), True), ((2, 3), False),
+                                                         ((0, 6), True)]
+                for bc, fb in bcs:
+                    per = bc[0] == 1
+                    ext_in, ext_out = list(nc), list(nc)
+                    ext_in[d] = _n(nc[d], per, INP[op])
+                    ext_out[d] = _n(nc[d], per, OUT[op])
+                    fi = np.asfortranarray(rng.standard_normal(ext_in))
+                    fshape = list(ext_in)
+                    fshape[d] = 4
+                    fbc = np.asfortranarray(rng.standard_normal(fshape)) if fb else None
+                    try:
+                        ref = odm.op3d(d, op, acc, list(bc), fi, ext_out, fbc)
+                    except ValueError:
+                        continue
+                    if not np.all(np.isfinite(ref)) or np.abs(ref).max() > 1e15:
+                        continue
+                    out = np.zeros(ext_out, order="F")
+                    ops.op3d(dm, d, op, fi, out, acc, list(bc), fbc)
+                    err = np.abs(out - ref).max() / max(1.0, np.abs(ref).max())
+                    assert err < RTOL, (nc, d, op, acc, bc, fb, err)
+                    nchecked += 1
     finally:
         dm.close()
+    assert nchecked >= 24
