@@ -47,7 +47,8 @@ def parse():
                     help="run N independent uneven channels (64^3 .. 192x128x192, mixed precision) together "
                          "as one captured graph per RK3 step instead")
     ap.add_argument("--ops", action="store_true",
-                    help="time each of the 18 compact operators (CP6, channel boundary kinds) instead")
+                    help="isolated operator benchmark instead: the 18 compact operators, d2 along each axis and "
+                         "the Poisson solve, FP64 and FP32, on band-limited inputs (use --nc 512 256 512)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -114,10 +115,45 @@ def cpu_baseline(nc, fields, cfg):
                       f"reference algorithm (FP64, single 2nd-order projection), {dt:.1f} s"}
 
 
+def band_limited(shape, seed, kfrac=1.0 / 3.0):
+    """Seeded band-limited random field of `shape` (x, y, z), returned in
+    C order (z, y, x): Fourier modes with |k_i| <= N_i * kfrac in every
+    direction, random phases, amplitude |k|^(-11/6) (a |k|^(-5/3) shell
+    spectrum), zero mean, scaled to max |f| = 1."""
+    import numpy as np
+    import scipy.fft as sf
+
+    n1, n2, n3 = shape
+    rng = np.random.default_rng(seed)
+    kx = np.arange(n1 // 2 + 1, dtype=np.float64)
+    ky = np.fft.fftfreq(n2, 1.0 / n2)
+    kz = np.fft.fftfreq(n3, 1.0 / n3)
+    k2 = kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2
+    keep = ((np.abs(kz)[:, None, None] <= n3 * kfrac) & (np.abs(ky)[None, :, None] <= n2 * kfrac) &
+            (kx[None, None, :] <= n1 * kfrac) & (k2 > 0))
+    amp = np.where(keep, np.power(np.maximum(k2, 1.0), -11.0 / 12.0), 0.0)
+    del k2, keep
+    spec = amp * np.exp(2j * np.pi * rng.random(amp.shape))
+    del amp
+    try:
+        workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        workers = 1
+    f = sf.irfftn(spec, s=(n3, n2, n1), workers=workers)
+    del spec
+    f /= np.abs(f).max()
+    return f
+
+
 def ops_bench(args):
-    """Each compact operator on device-resident arrays of the channel grid:
-    average kernel time from HIP events and the algorithmic bandwidth
-    (one array read, one written)."""
+    """Isolated operator benchmark: every compact operator (CP6, channel
+    boundary kinds), the second derivative d2 = 1der_P2C after 1der_C2P
+    along each axis, and the FFT(x,z) + TDMA(y) Poisson solve of a zero-mean
+    right-hand side, each in FP64 and FP32 on device-resident seeded
+    band-limited inputs.  Per kernel: average time from HIP events on the
+    solver's stream, algorithmic bandwidth (arrays read + written; five
+    read+write passes for the Poisson solve) and the FP32 result's error
+    relative to the FP64 one (max |f32 - f64| / max |f64|)."""
     import ctypes as C
 
     import numpy as np
@@ -132,37 +168,133 @@ def ops_bench(args):
     names = ["midp_C2P", "midp_P2C", "1der_C2C", "1der_P2P", "1der_C2P", "1der_P2C"]
     inp = {0: "c", 1: "p", 2: "c", 3: "p", 4: "c", 5: "p"}
     out = {0: "p", 1: "c", 2: "c", 3: "p", 4: "p", 5: "c"}
+    t0 = time.perf_counter()
+    # one field on the y-node extent; cell-centred inputs take its first ny planes
+    base = torch.from_numpy(band_limited((nc[0], nc[1] + 1, nc[2]), seed=20261202)).to("cuda")
+    gen_s = time.perf_counter() - t0
+    vp = C.c_void_p
+
+    def ext(d, loc):
+        e = list(nc)
+        if d == 1 and loc == "p":
+            e[1] = nc[1] + 1
+        return e
+
+    def field(e):
+        return base[:, : e[1], :].contiguous()
+
+    def bc(d, op):
+        return ([5, 5] if inp[op] == "c" else [4, 4]) if d == 1 else [1, 1]
+
+    def op_call(d, op, fi, fo, ei, eo, fbc):
+        if fi.dtype == torch.float64:
+            return L.chs_op3d(dm.h, d, op, 4, _lib.ivec(bc(d, op)), vp(fi.data_ptr()), _lib.ivec(ei),
+                              vp(fo.data_ptr()), _lib.ivec(eo), vp(fbc.data_ptr()) if fbc is not None else None,
+                              _lib.MEM_DEVICE)
+        return L.chs_op3d_f32(dm.h, d, op, 4, _lib.ivec(bc(d, op)), vp(fi.data_ptr()), _lib.ivec(ei),
+                              vp(fo.data_ptr()), _lib.ivec(eo), vp(fbc.data_ptr()) if fbc is not None else None)
+
+    def timed(cls, fn):
+        for _ in range(3):
+            fn()
+        _lib.check(L.chs_profile_begin(dm.h, cls))
+        for _ in range(args.steps):
+            fn()
+        ms, nl, byt = C.c_double(), C.c_longlong(), C.c_double()
+        _lib.check(L.chs_profile_end(dm.h, C.byref(ms), C.byref(nl), C.byref(byt)))
+        t = ms.value / args.steps  # per application (d2: both launches)
+        return round(t * 1e3, 2), round(byt.value / args.steps / (t * 1e-3) / 1e9, 1)
+
+    def rel(a, b):
+        return float((a.double() - b).abs().max() / b.abs().max())
+
+    def fbc_for(d, ei, dtype):
+        if d != 1:
+            return None
+        fshape = list(ei)
+        fshape[1] = 4
+        return torch.zeros(fshape[2], fshape[1], fshape[0], dtype=dtype, device="cuda")
+
     res = []
     for d in range(3):
         for op in range(6):
-            ei, eo = list(nc), list(nc)
-            if d == 1:
-                ei[1] = nc[1] + (inp[op] == "p")
-                eo[1] = nc[1] + (out[op] == "p")
-                ibc = [5, 5] if inp[op] == "c" else [4, 4]
-            else:
-                ibc = [1, 1]
-            fi = torch.rand(ei[2], ei[1], ei[0], dtype=torch.float64, device="cuda")
-            fo = torch.zeros(eo[2], eo[1], eo[0], dtype=torch.float64, device="cuda")
-            fshape = list(ei)
-            fshape[d] = 4
-            fbc = torch.zeros(fshape[2], fshape[1], fshape[0], dtype=torch.float64, device="cuda") if d == 1 else None
+            ei, eo = ext(d, inp[op]), ext(d, out[op])
+            row = {"op": f"Get_{'xyz'[d]}_{names[op]}_3D"}
+            outs = {}
+            for dt, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+                fi = field(ei).to(dt)
+                fo = torch.zeros(eo[2], eo[1], eo[0], dtype=dt, device="cuda")
+                fbc = fbc_for(d, ei, dt)
+                torch.cuda.synchronize()
+                us, gbs = timed(d, lambda: _lib.check(op_call(d, op, fi, fo, ei, eo, fbc)))
+                row[f"{tag}_us"], row[f"{tag}_GBps"] = us, gbs
+                outs[tag] = fo
+            row["f32_err_vs_f64"] = rel(outs["f32"], outs["f64"])
+            res.append(row)
+            del outs
+    # d2 = 1der_P2C(1der_C2P(f)) along each axis (cell-centred in and out)
+    d2 = []
+    for d in range(3):
+        ec, ep = ext(d, "c"), ext(d, "p")
+        row = {"op": f"d2_{'xyz'[d]}"}
+        outs = {}
+        for dt, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+            fi = field(ec).to(dt)
+            mid = torch.zeros(ep[2], ep[1], ep[0], dtype=dt, device="cuda")
+            fo = torch.zeros(ec[2], ec[1], ec[0], dtype=dt, device="cuda")
+            fb1, fb2 = fbc_for(d, ec, dt), fbc_for(d, ep, dt)
             torch.cuda.synchronize()
-            args_ = (dm.h, d, op, 4, _lib.ivec(ibc), C.c_void_p(fi.data_ptr()), _lib.ivec(ei),
-                     C.c_void_p(fo.data_ptr()), _lib.ivec(eo), C.c_void_p(fbc.data_ptr()) if fbc is not None else None,
-                     _lib.MEM_DEVICE)
-            for _ in range(3):
-                _lib.check(L.chs_op3d(*args_))
-            _lib.check(L.chs_profile_begin(dm.h, d))
-            for _ in range(args.steps):
-                _lib.check(L.chs_op3d(*args_))
+
+            def both():
+                _lib.check(op_call(d, 4, fi, mid, ec, ep, fb1))
+                _lib.check(op_call(d, 5, mid, fo, ep, ec, fb2))
+
+            us, gbs = timed(d, both)
+            row[f"{tag}_us"], row[f"{tag}_GBps"] = us, gbs
+            outs[tag] = fo
+        row["f32_err_vs_f64"] = rel(outs["f32"], outs["f64"])
+        d2.append(row)
+        del outs
+    # Poisson solve of a zero-mean cell-centred right-hand side
+    rhs = field(ext(0, "c"))
+    rhs -= rhs.mean()
+    pois = {"op": "solve_fft_poisson"}
+    outs = {}
+    for dt, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+        src = rhs.to(dt)
+        work = src.clone()
+        fn = (lambda: _lib.check(L.chs_poisson(dm.h, vp(work.data_ptr()), _lib.MEM_DEVICE))) if tag == "f64" \
+            else (lambda: _lib.check(L.chs_poisson_f32(dm.h, vp(work.data_ptr()))))
+        for _ in range(2):
+            work.copy_(src)
+            torch.cuda.synchronize()
+            fn()
+        tot_ms, tot_b = 0.0, 0.0
+        for _ in range(args.steps):
+            work.copy_(src)
+            torch.cuda.synchronize()
+            _lib.check(L.chs_profile_begin(dm.h, 3))
+            fn()
             ms, nl, byt = C.c_double(), C.c_longlong(), C.c_double()
             _lib.check(L.chs_profile_end(dm.h, C.byref(ms), C.byref(nl), C.byref(byt)))
-            t = ms.value / nl.value
-            res.append({"op": f"Get_{'xyz'[d]}_{names[op]}_3D", "us": round(t * 1e3, 2),
-                        "GBps": round(byt.value / nl.value / (t * 1e-3) / 1e9, 1)})
+            tot_ms += ms.value
+            tot_b += byt.value
+        t = tot_ms / args.steps
+        pois[f"{tag}_us"] = round(t * 1e3, 2)
+        pois[f"{tag}_GBps"] = round(tot_b / args.steps / (t * 1e-3) / 1e9, 1)
+        outs[tag] = work
+    pois["f32_err_vs_f64"] = rel(outs["f32"], outs["f64"])
+    del outs
     dm.close()
-    print(json.dumps({"ops_bench": res, "grid": list(nc), "launches_each": args.steps}))
+    # launch-shape choices the autotuner made for these kernels (key=choice
+    # and the time of each candidate)
+    buf = C.create_string_buffer(1 << 16)
+    L.chs_tune_report(buf, len(buf))
+    tuned = buf.value.decode().splitlines()
+    print(json.dumps({"ops_bench": res, "d2": d2, "poisson": pois, "grid": list(nc),
+                      "inputs": "seeded band-limited random fields (modes |k_i| <= N_i/3, |k|^-5/3 spectrum, "
+                                f"seed 20261202, generated in {gen_s:.1f} s); zero-mean Poisson right-hand side",
+                      "applications_each": args.steps, "autotuned": tuned}))
 
 
 def batch_specs(n, seed=20261202):

@@ -22,6 +22,7 @@ EXPORTS = [
     "chs_mlp_create", "chs_mlp_destroy", "chs_mlp_info", "chs_mlp_get_params", "chs_mlp_set_params",
     "chs_mlp_grad", "chs_mlp_train", "chs_mlp_eval",
     "chs_inspect_coeff", "chs_inspect_op1d", "chs_inspect_projection", "chs_profile_begin", "chs_profile_end", "chs_tune_report",
+    "chs_op3d_f32", "chs_poisson_f32", "chs_tune_force",
 ]
 
 
@@ -75,6 +76,8 @@ def load():
     L.chs_sync.argtypes = [vp]
     L.chs_op3d.argtypes = [vp, C.c_int, C.c_int, C.c_int, ip, vp, ip, vp, ip, vp, C.c_int]
     L.chs_poisson.argtypes = [vp, vp, C.c_int]
+    L.chs_op3d_f32.argtypes = [vp, C.c_int, C.c_int, C.c_int, ip, vp, ip, vp, ip, vp]
+    L.chs_poisson_f32.argtypes = [vp, vp]
     L.chs_set_state.argtypes = [vp, vp, vp, vp, vp, C.c_int]
     L.chs_get_state.argtypes = [vp, vp, vp, vp, vp, C.c_int]
     L.chs_update_re.argtypes = [vp, C.c_int]
@@ -109,6 +112,7 @@ def load():
     L.chs_profile_begin.argtypes = [vp, C.c_int]
     L.chs_profile_end.argtypes = [vp, dp, C.POINTER(C.c_longlong), dp]
     L.chs_tune_report.argtypes = [C.c_char_p, C.c_int]
+    L.chs_tune_force.argtypes = [C.c_char_p, C.c_int]
     for name in EXPORTS:
         if name != "chs_batch_instance":
             getattr(L, name).restype = getattr(L, name).restype or C.c_int

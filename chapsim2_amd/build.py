@@ -23,6 +23,7 @@ SOURCES = [
     "chs_exact.cpp",
     "chs_tune.hip",
     "chs_lineop.hip",
+    "chs_lineop_x.hip",
     "chs_poisson.hip",
     "chs_flow.hip",
     "chs_api.hip",

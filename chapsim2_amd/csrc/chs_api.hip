@@ -91,6 +91,23 @@ static int fail(int code, const std::string& msg) {
 #define API_CHECK_HANDLE(h) \
   if (!(h)) return fail(CHS_EINVAL, "null handle")
 
+// One Poisson solve on the handle's stream; under launch profiling class 3
+// it is bracketed by events and counted as five field passes (forward x and
+// z transforms, the y solves, inverse z and x), each reading and writing the
+// n cells once.
+template <typename T>
+static std::string poisson_timed(chs::Handle& h, T* rhs, size_t n) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const bool timed = (h.prof_class == 3) && h.prof_pair(&e0, &e1);
+  if (timed) {
+    (void)hipEventRecord(e0, h.stream);
+    h.prof_bytes += 10.0 * sizeof(T) * (double)n;
+  }
+  std::string e = h.poisson->solve(rhs, h.stream);
+  if (timed) (void)hipEventRecord(e1, h.stream);
+  return e;
+}
+
 extern "C" {
 
 int chs_abi_version(void) { return CHS_ABI_VERSION; }
@@ -240,11 +257,57 @@ int chs_poisson(chs_handle_t* H, double* rhs, int mem) {
       return fail(CHS_EHIP, "copy in");
     d = h.io[0].as<double>();
   }
-  e = h.poisson->solve(d, h.stream);
+  e = poisson_timed(h, d, n);
   if (!e.empty()) return fail(CHS_EHIP, e);
   if (mem == CHS_MEM_HOST &&
       hipMemcpyAsync(rhs, d, sizeof(double) * n, hipMemcpyDeviceToHost, h.stream) != hipSuccess)
     return fail(CHS_EHIP, "copy out");
+  hipError_t he = hipStreamSynchronize(h.stream);
+  if (he != hipSuccess) return fail(CHS_EHIP, hipGetErrorString(he));
+  return CHS_OK;
+}
+
+int chs_op3d_f32(chs_handle_t* H, int dir, int op, int iacc, const int ibc[2], const float* fi, const int shp_in[3],
+                 float* fo, const int shp_out[3], const float* fbc) {
+  API_CHECK_HANDLE(H);
+  chs::Handle& h = H->h;
+  if (dir < 0 || dir > 2 || op < 0 || op > 5 || !ibc || !fi || !fo || !shp_in || !shp_out)
+    return fail(CHS_EINVAL, "invalid argument");
+  for (int d = 0; d < 3; ++d)
+    if (shp_in[d] <= 0 || shp_out[d] <= 0) return fail(CHS_EINVAL, "non-positive extent");
+  const chs::DevPlan<float>* P = nullptr;
+  std::string e = h.plan<float>(dir, op, iacc, ibc, fbc != nullptr, shp_in[dir], shp_out[dir], &P);
+  if (!e.empty()) return fail(CHS_EINVAL, e);
+  chs::Ext3 ein{{shp_in[0], shp_in[1], shp_in[2]}}, eout{{shp_out[0], shp_out[1], shp_out[2]}};
+  const size_t nin = (size_t)shp_in[0] * shp_in[1] * shp_in[2];
+  const size_t nout = (size_t)shp_out[0] * shp_out[1] * shp_out[2];
+  if ((e = h.io[2].ensure(sizeof(float) * nout)).size()) return fail(CHS_ENOMEM, e);
+  chs::LineIn<float> in;
+  in.A = fi;
+  chs::LineOut<float> o;
+  o.out = fo;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const bool timed = (h.prof_class == dir) && h.prof_pair(&e0, &e1);
+  if (timed) {
+    (void)hipEventRecord(e0, h.stream);
+    h.prof_bytes += 4.0 * ((double)nin + (double)nout);
+  }
+  e = chs::launch_line_op<float>(*P, ein, eout, in, fbc, o, h.io[2].as<float>(), h.stream);
+  if (timed) (void)hipEventRecord(e1, h.stream);
+  if (!e.empty()) return fail(CHS_EINVAL, e);
+  hipError_t he = hipStreamSynchronize(h.stream);
+  if (he != hipSuccess) return fail(CHS_EHIP, hipGetErrorString(he));
+  return CHS_OK;
+}
+
+int chs_poisson_f32(chs_handle_t* H, float* rhs) {
+  API_CHECK_HANDLE(H);
+  chs::Handle& h = H->h;
+  if (!h.poisson) return fail(CHS_EUNSUP, "no Poisson solver for this configuration");
+  if (!rhs) return fail(CHS_EINVAL, "invalid argument");
+  const size_t n = (size_t)h.mesh.nc[0] * h.mesh.nc[1] * h.mesh.nc[2];
+  std::string e = poisson_timed(h, rhs, n);
+  if (!e.empty()) return fail(CHS_EHIP, e);
   hipError_t he = hipStreamSynchronize(h.stream);
   if (he != hipSuccess) return fail(CHS_EHIP, hipGetErrorString(he));
   return CHS_OK;
@@ -501,7 +564,7 @@ extern "C" {
 
 int chs_profile_begin(chs_handle_t* H, int kernel_class) {
   API_CHECK_HANDLE(H);
-  if (kernel_class < 0 || kernel_class > 2) return fail(CHS_EINVAL, "kernel class must be 0..2");
+  if (kernel_class < 0 || kernel_class > 3) return fail(CHS_EINVAL, "kernel class must be 0..3");
   chs::Handle& h = H->h;
   (void)hipStreamSynchronize(h.stream);
   h.prof_class = kernel_class;
@@ -520,6 +583,11 @@ int chs_tune_report(char* buf, int len) {
   int lines = 0;
   for (char c : r) lines += (c == '\n');
   return lines;
+}
+
+int chs_tune_force(const char* prefix, int variant) {
+  chs::tune_force(prefix ? std::string(prefix) : std::string(), variant);
+  return CHS_OK;
 }
 
 int chs_profile_end(chs_handle_t* H, double* ms_total, long long* launches, double* alg_bytes) {

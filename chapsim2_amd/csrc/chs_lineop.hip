@@ -35,7 +35,7 @@
 // Solve_TDMA_cyclic (:68-90) with the RHS-independent correction vector
 // precomputed, and the basic solution's first entry accumulated during the
 // forward sweep so the correction is applied in the same backward pass.
-#include "chs_lineop.h"
+#include "chs_lineop_dev.h"
 #include "chs_tune.h"
 
 #include <cmath>
@@ -63,14 +63,6 @@ static std::string up(const std::vector<double>& v, T** dst) {
   CHS_CHECK(hipMalloc(dst, sizeof(T) * t.size()));
   CHS_CHECK(hipMemcpy(*dst, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice));
   return "";
-}
-
-// Line index read for window position k, -2 <= k <= n + CH + 1 (CH the chunk rows):
-// wrapped on periodic lines (one period suffices, as periodic lines on the
-// partitioned kernels have n >= CH + 2), clamped to the line otherwise.
-__host__ __device__ __forceinline__ int window_index(int k, int n, bool wrap) {
-  if (wrap) return k < 0 ? k + n : (k >= n ? k - n : k);
-  return k < 0 ? 0 : (k > n - 1 ? n - 1 : k);
 }
 
 template <typename T>
@@ -164,41 +156,6 @@ std::string upload_plan(const LinePlan& p, DevPlan<T>* dp) {
   return "";
 }
 
-static inline int lds_pos_host(int k) { return k + (k >> 4); }
-
-template <typename T>
-struct KArgs {
-  int n_in, n_out;
-  int e1, e2;      // input extents 1,2
-  int o1, o2;      // output extents 1,2
-  long nlines;
-  const T* A; const T* B; const T* Cm; T sA, cB;
-  const T* D; const T* E; T cD, cE;
-  const T* fbc;
-  T c1, c2, dd, sgn;
-  const DevRow<T>* rows;
-  int solve; T b0;
-  const T* a; const T* c; const T* d; const T* pw; const T* x1;
-  T an, cn, den;
-  const T* metric; int neu_lo, neu_hi;
-  T* out; T sOut; int accumulate;
-  T* scratch;
-  // partitioned form
-  int P, per, ls, nl;
-  const T* rinv;
-  const T* swin;  // 4 x (CH + 4) boundary-row window weights (CH = the plan's chunk rows)
-  int wrap;       // periodic line: windows wrap around instead of clamping
-  const T* co;    // P x CH coefficient rows of kCo entries (see upload_plan)
-};
-
-// bulk offsets (p1,p2,p3,p4) per operator kind, relative to the output row
-template <int OPK> struct Offs;
-template <> struct Offs<MIDP_C2P> { static constexpr int p1 = 0, p2 = -1, p3 = 1, p4 = -2; };
-template <> struct Offs<MIDP_P2C> { static constexpr int p1 = 0, p2 = 1, p3 = -1, p4 = 2; };
-template <> struct Offs<D1_C2C> { static constexpr int p1 = 1, p2 = -1, p3 = 2, p4 = -2; };
-template <> struct Offs<D1_P2P> { static constexpr int p1 = 1, p2 = -1, p3 = 2, p4 = -2; };
-template <> struct Offs<D1_C2P> { static constexpr int p1 = 0, p2 = -1, p3 = 1, p4 = -2; };
-template <> struct Offs<D1_P2C> { static constexpr int p1 = 1, p2 = 0, p3 = 2, p4 = -1; };
 
 template <typename T, int DIR, int OPK>
 __global__ void __launch_bounds__(64) k_line(KArgs<T> P) {
@@ -302,242 +259,6 @@ __global__ void __launch_bounds__(64) k_line(KArgs<T> P) {
     emit(r, (P.solve == 2) ? xb + P.x1[r] * xn : xb);
   }
 }
-
-// ---------------------------------------------------------------------------
-// partitioned kernels
-
-// Chunk rows [s0, s0+len) of one line: explicit RHS into y[] from the
-// window f[t] = input at line index s0-2+t (clamped to the line).  The four
-// boundary rows use their dense window weights (plans whose boundary recipes
-// reach outside the window run on the one-thread-per-line kernel instead);
-// they are formed first, so that the interior rows can then consume the
-// window in order (a window value is dead once the row two below it is
-// done, and y[] takes its registers).
-template <typename T, int OPK, int CH>
-__device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int len, const T (&f)[CH + 4],
-                                          const T (&fb4)[4], T (&y)[CH]) {
-  using O = Offs<OPK>;
-  constexpr int W = CH + 4;
-  auto special = [&](int rq) -> T {
-    const DevRow<T>& R = P.rows[rq];
-    T v = T(0);
-    const T* wv = P.swin + rq * W;
-#pragma unroll
-    for (int t = 0; t < W; ++t) v = v + wv[t] * f[t];
-    if (P.fbc)
-      for (int m = 0; m < 4; ++m) v = v + R.u[m] * fb4[m];
-    return v * P.dd;
-  };
-  const bool first = q == 0, last = q == P.P - 1;
-  T v0 = T(0), v1 = T(0), v2 = T(0), v3 = T(0);
-  if (first) {
-    v0 = special(0);
-    v1 = special(1);
-  }
-  if (last) {
-    v2 = special(2);
-    v3 = special(3);
-  }
-#pragma unroll
-  for (int i = 0; i < CH; ++i)
-    y[i] = (P.c1 * (f[i + 2 + O::p1] + P.sgn * f[i + 2 + O::p2]) +
-            P.c2 * (f[i + 2 + O::p3] + P.sgn * f[i + 2 + O::p4])) * P.dd;
-  if (first) {
-    y[0] = v0;
-    y[1] = v1;
-  }
-  if (last) {
-#pragma unroll
-    for (int i = 0; i < CH; ++i) {
-      if (i == len - 2) y[i] = v2;
-      if (i == len - 1) y[i] = v3;
-    }
-  }
-}
-
-// Element at byte offset `boff` from `base`: the y/z kernels address a line
-// as a wave-uniform base pointer plus a 32-bit per-lane byte offset.
-template <typename T>
-__device__ __forceinline__ T ld_at(const T* base, unsigned boff) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
-}
-template <typename T>
-__device__ __forceinline__ void st_at(T* base, unsigned boff, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
-}
-
-// Input transform sA * (A + cB * B) * Cm of `nv` elements: f holds A, g holds
-// the array loaded beside it (B if present, else Cm).  A third array (Cm when
-// B is also present) is fetched by `third`.
-template <typename T, int NV, class TH>
-__device__ __forceinline__ void transform_in(const KArgs<T>& P, T (&f)[NV], const T (&g)[NV], TH&& third) {
-  if (P.B) {
-#pragma unroll
-    for (int t = 0; t < NV; ++t) f[t] = f[t] + P.cB * g[t];
-  }
-#pragma unroll
-  for (int t = 0; t < NV; ++t) f[t] = P.sA * f[t];
-  if (P.Cm) {
-    if (!P.B) {
-#pragma unroll
-      for (int t = 0; t < NV; ++t) f[t] = f[t] * g[t];
-    } else {
-      T h[NV];
-#pragma unroll
-      for (int t = 0; t < NV; ++t) h[t] = third(t);
-#pragma unroll
-      for (int t = 0; t < NV; ++t) f[t] = f[t] * h[t];
-    }
-  }
-}
-
-// Window rows [T0, T1) of one line chunk: line indices s0-2+t (wrapped on
-// periodic lines, clamped otherwise, so every address is valid); the row
-// offsets are wave-uniform.  IN = 0: A only; 1: A and the second input array
-// (B, or Cm when B is absent), all loads issued before any use; 2: as 1 (the
-// second array optional), then the additive arrays D (and E) as a second
-// batch.
-template <typename T, int IN, int CH, int T0, int T1>
-__device__ __forceinline__ void load_rows(const KArgs<T>& P, long base, unsigned ri, long is, int s0,
-                                          T (&f)[CH + 4]) {
-  constexpr int NB = T1 - T0;
-  long ro[NB];
-#pragma unroll
-  for (int t = 0; t < NB; ++t) ro[t] = base + is * window_index(s0 - 2 + T0 + t, P.n_in, P.wrap);
-  T* fw = f + T0;
-  T(&fv)[NB] = *reinterpret_cast<T(*)[NB]>(fw);
-  const T* S = P.B ? P.B : P.Cm;
-  if (IN == 1 || (IN == 2 && S)) {
-    T g[NB];
-#pragma unroll
-    for (int t = 0; t < NB; ++t) {
-      fv[t] = ld_at(P.A + ro[t], ri);
-      g[t] = ld_at(S + ro[t], ri);
-    }
-    transform_in<T, NB>(P, fv, g, [&](int t) { return ld_at(P.Cm + ro[t], ri); });
-  } else {
-#pragma unroll
-    for (int t = 0; t < NB; ++t) fv[t] = P.sA * ld_at(P.A + ro[t], ri);
-  }
-  if (IN == 2) {
-    // The additive arrays are loaded after the first batch has been
-    // combined (FP32: together; FP64: one after the other): their lane
-    // offset passes through an empty asm that reads the window's last value,
-    // so the compiler cannot hoist these loads next to the earlier batch
-    // (registers for the window and all arrays together spill).
-    unsigned rj = ri;
-    asm volatile("" : "+v"(rj) : "v"(fv[NB - 1]));
-    if (sizeof(T) == 4) {
-      // FP32: both arrays in one batch
-      T d[NB], e[NB];
-#pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        d[t] = ld_at(P.D + ro[t], rj);
-        e[t] = P.E ? ld_at(P.E + ro[t], rj) : T(0);
-      }
-#pragma unroll
-      for (int t = 0; t < NB; ++t) fv[t] = fv[t] + P.cD * d[t] + P.cE * e[t];
-    } else {
-      // FP64: one array at a time
-      {
-        T d[NB];
-#pragma unroll
-        for (int t = 0; t < NB; ++t) d[t] = ld_at(P.D + ro[t], rj);
-#pragma unroll
-        for (int t = 0; t < NB; ++t) fv[t] = fv[t] + P.cD * d[t];
-      }
-      if (P.E) {
-        unsigned rk = rj;
-        asm volatile("" : "+v"(rk) : "v"(fv[NB - 1]));
-        T e[NB];
-#pragma unroll
-        for (int t = 0; t < NB; ++t) e[t] = ld_at(P.E + ro[t], rk);
-#pragma unroll
-        for (int t = 0; t < NB; ++t) fv[t] = fv[t] + P.cE * e[t];
-      }
-    }
-  }
-}
-
-// Window of one line chunk of CH rows: rows s0-2 .. s0+CH+1.  FP64 windows
-// of long chunks (CH > kChunk) are loaded in two halves, the second half's
-// loads waiting (through an empty asm on its lane offset) until the first
-// half is combined, so the raw input arrays of the whole window never hold
-// registers at the same time.
-template <typename T, int IN, int CH>
-__device__ __forceinline__ void load_window(const KArgs<T>& P, long base, unsigned ri, long is, int s0,
-                                            T (&f)[CH + 4]) {
-  constexpr int W = CH + 4;
-  if constexpr (sizeof(T) == 8 && CH > kChunk) {
-    constexpr int H = W / 2;
-    load_rows<T, IN, CH, 0, H>(P, base, ri, is, s0, f);
-    unsigned rh = ri;
-    asm volatile("" : "+v"(rh) : "v"(f[H - 1]));
-    load_rows<T, IN, CH, H, W>(P, base, rh, is, s0, f);
-  } else {
-    load_rows<T, IN, CH, 0, W>(P, base, ri, is, s0, f);
-  }
-}
-
-// Chunk-local Thomas sweeps over all CH rows of a chunk whose
-// coefficient rows start at c; returns the chunk's first and last values.
-// The coefficient rows past the chunk's length are identity rows (and the
-// chunk's last row does not couple to the next), so the sweeps need no row
-// guards.
-template <typename T, int CH>
-__device__ __forceinline__ void chunk_solve(const T* c, int len, T (&y)[CH], T& yf, T& yl) {
-  y[0] = y[0] * c[CO_LD];
-#pragma unroll
-  for (int i = 1; i < CH; ++i) y[i] = (y[i] - c[i * kCo + CO_AR] * y[i - 1]) * c[i * kCo + CO_LD];
-#pragma unroll
-  for (int i = CH - 2; i >= 0; --i) y[i] = y[i] - c[i * kCo + CO_LC] * y[i + 1];
-  yf = y[0];
-  yl = y[0];
-#pragma unroll
-  for (int i = 1; i < CH; ++i)
-    if (i == len - 1) yl = y[i];
-}
-
-// Reduced-system values coupling chunk q to its neighbours (rinv: the
-// 2P x 2P inverse; ex(k) reads the published first/last value k of the
-// thread's line), applied through the spike vectors of the chunk's rows c.
-template <typename T, int CH, class EX>
-__device__ __forceinline__ void chunk_couple(const KArgs<T>& P, int q, const T* rinv, const T* c, EX&& ex,
-                                             T (&y)[CH]) {
-  const int Pn = P.P, M = 2 * Pn;
-  const int qm = (q + Pn - 1) % Pn, qp = (q + 1) % Pn;
-  const T* rL = rinv + (2 * qm + 1) * M;
-  const T* rF = rinv + (2 * qp) * M;
-  T uL = T(0), uF = T(0);
-  for (int k = 0; k < M; ++k) {
-    const T v = ex(k);
-    uL = uL + rL[k] * v;
-    uF = uF + rF[k] * v;
-  }
-  if (!P.per && q == 0) uL = T(0);
-  if (!P.per && q == Pn - 1) uF = T(0);
-#pragma unroll
-  for (int i = 0; i < CH; ++i) y[i] = y[i] - c[i * kCo + CO_SV] * uL - c[i * kCo + CO_SW] * uF;
-}
-
-// Row i of a chunk (line row r): metric scaling and Neumann overrides.
-template <typename T>
-__device__ __forceinline__ T chunk_post(const KArgs<T>& P, const T* c, int i, int r, T val, const T (&fb4)[4]) {
-  val = val * c[i * kCo + CO_MET];
-  if (r == 0 && P.neu_lo) val = fb4[0];
-  if (r == P.n_out - 1 && P.neu_hi) val = fb4[1];
-  return val;
-}
-
-// Per-plan coefficient data a workgroup of P waves copies into LDS: the chunk
-// rows (P x CH x kCo) and the reduced-system inverse (2P x 2P), at most
-// stage_max(CH) = CH kCo / 64 + 1 values per thread (P <= kMaxChunks).
-__host__ __device__ constexpr int stage_max(int ch) { return ch * kCo / 64 + 1; }
-constexpr int kStageMax = stage_max(kChunk);
-static_assert(64 * kMaxChunks * stage_max(kChunkLong) >= kMaxChunks * kChunkLong * kCo + 4 * kMaxChunks * kMaxChunks &&
-                  64 * kMaxChunks * kStageMax >= kMaxChunks * kChunk * kCo + 4 * kMaxChunks * kMaxChunks &&
-                  64 * kStageMax >= kChunk * kCo + 4,
-              "per-plan staging exceeds stage_max values per thread");
 
 // First elements of line L in the input, output and fbc arrays.
 template <int DIR, typename T>
@@ -657,251 +378,13 @@ __global__ void __launch_bounds__(64 * kMaxChunks)
   }
 }
 
-__device__ __forceinline__ int lds_pos(int k) { return k + (k >> 4); }
-
-template <typename T, int OPK>
-__global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* tile = reinterpret_cast<T*>(smem_raw);
-  const int NL = P.nl, LS = P.ls, Pn = P.P;
-  T* exb = tile + (long)NL * LS;
-  const int tid = threadIdx.x;
-  const int nthr = NL * Pn;
-  const int ll = tid / Pn, q = tid - ll * Pn;
-  const long L0 = (long)blockIdx.x * NL;
-  const int n_in = P.n_in, n_out = P.n_out;
-  // stage NL lines (input transform applied)
-  // (an x line L starts at L * e1 in the input and at L * o1 in the output)
-  for (int e = tid; e < NL * n_in; e += nthr) {
-    const int l = e / n_in, k = e - l * n_in;
-    const long L = L0 + l;
-    T v = T(0);
-    if (L < P.nlines) {
-      const long o = L * P.e1 + k;
-      v = P.A[o];
-      if (P.B) v = v + P.cB * P.B[o];
-      v = P.sA * v;
-      if (P.Cm) v = v * P.Cm[o];
-      if (P.D) v = v + P.cD * P.D[o];
-      if (P.E) v = v + P.cE * P.E[o];
-    }
-    tile[(long)l * LS + lds_pos(k)] = v;
-  }
-  __syncthreads();
-  const long L = L0 + ll;
-  const bool live = L < P.nlines;
-  const T* row = tile + (long)ll * LS;
-  const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
-  T f[kChunk + 4];
-#pragma unroll
-  for (int t = 0; t < kChunk + 4; ++t) f[t] = row[lds_pos(window_index(s0 - 2 + t, n_in, P.wrap))];
-  T fb4[4] = {T(0), T(0), T(0), T(0)};
-  if (P.fbc && live && (q == 0 || q == Pn - 1))
-    for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
-  T y[kChunk];
-  chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
-  if (P.solve) {
-    T yf, yl;
-    chunk_solve(P.co + q * kChunk * kCo, len, y, yf, yl);
-    T* ex = exb + (long)ll * 2 * Pn;
-    ex[2 * q] = yf;
-    ex[2 * q + 1] = yl;
-    __syncthreads();
-    chunk_couple<T>(P, q, P.rinv, P.co + q * kChunk * kCo, [&](int k) { return ex[k]; }, y);
-  }
-  __syncthreads();  // every thread has finished reading its input rows
-  T* orow = tile + (long)ll * LS;
-#pragma unroll
-  for (int i = 0; i < kChunk; ++i)
-    if (i < len) orow[lds_pos(s0 + i)] = chunk_post<T>(P, P.co + q * kChunk * kCo, i, s0 + i, y[i], fb4);
-  __syncthreads();
-  for (int e = tid; e < NL * n_out; e += nthr) {
-    const int l = e / n_out, k = e - l * n_out;
-    const long Lw = L0 + l;
-    if (Lw >= P.nlines) continue;
-    const long o = Lw * P.o1 + k;
-    const T val = tile[(long)l * LS + lds_pos(k)];
-    const T prev = P.accumulate ? P.out[o] : T(0);
-    P.out[o] = prev + val * P.sOut;
-  }
-}
 
 // x lines with a wave-uniform chunk: a workgroup stages 64 whole lines in LDS
 // (row stride odd, so the 64 lanes reading one column of their own lines hit
 // distinct banks); wave q of the workgroup processes chunk q of all 64 lines,
 // which keeps the plan's per-row coefficients in scalar registers.
-template <typename T, int OPK, int IN>
-__global__ void __launch_bounds__(64 * kMaxChunks) k_line_xu(KArgs<T> P) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* tile = reinterpret_cast<T*>(smem_raw);
-  const int LS = P.ls, Pn = P.P;
-  // after the windows are read, the start of the tile holds the exchanged
-  // first/last chunk values (64 x 2P)
-  T* exb = tile;
-  // per-plan coefficients (chunk rows, then the reduced-system inverse) after
-  // the tile; their loads are issued first and written once staging is done
-  T* sco = tile + (long)64 * LS;
-  const int nst = Pn * kChunk * kCo + (P.rinv ? 4 * Pn * Pn : 0);
-  T cv[kStageMax];
-  {
-    const int nco = Pn * kChunk * kCo;
-#pragma unroll
-    for (int u = 0; u < kStageMax; ++u) {
-      const int idx = threadIdx.x + u * 64 * Pn;
-      cv[u] = (idx < nco) ? P.co[idx] : (idx < nst ? P.rinv[idx - nco] : T(0));
-    }
-  }
-  const int tid = threadIdx.x, nthr = 64 * Pn;
-  const int lane = tid & 63;
-  const int q = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const long L0 = (long)blockIdx.x * 64;
-  const int n_in = P.n_in, n_out = P.n_out;
-  // staging: element e = l * n_in + k of the workgroup's 64 lines, thread
-  // tid taking e = tid, tid + nthr, ...; (l, k) advance by a fixed step, so
-  // no division per element.  Batches of UB elements, all loads of a batch
-  // issued before any use; the line is clamped so every address is valid.
-  constexpr int UB = 8;
-  const long Lmax = P.nlines - 1;
-  const int lmax = (int)((Lmax - L0) < 63 ? (Lmax - L0) : 63);  // last line of the workgroup
-  const T* const Ab = P.A + L0 * P.e1;
-  const T* const Bb = P.B ? P.B + L0 * P.e1 : nullptr;
-  const T* const Cb = P.Cm ? P.Cm + L0 * P.e1 : nullptr;
-  const T* const Db = P.D ? P.D + L0 * P.e1 : nullptr;
-  const T* const Eb = P.E ? P.E + L0 * P.e1 : nullptr;
-  {
-    const int dl = nthr / n_in, dk = nthr - dl * n_in;
-    int l = tid / n_in, k = tid - l * n_in;
-    const int tin = 64 * n_in;
-    for (int b0 = 0; b0 < tin; b0 += nthr * UB) {
-      unsigned o[UB];
-      int ll[UB], kk[UB];
-#pragma unroll
-      for (int u = 0; u < UB; ++u) {
-        ll[u] = l;
-        kk[u] = k;
-        o[u] = (unsigned)(((l < lmax ? l : lmax) * P.e1 + k) * (int)sizeof(T));
-        k += dk;
-        l += dl;
-        if (k >= n_in) { k -= n_in; ++l; }
-      }
-      T v[UB];
-      const T* S = Bb ? Bb : Cb;
-      if (IN == 1 || (IN == 2 && S)) {
-        T g[UB];
-#pragma unroll
-        for (int u = 0; u < UB; ++u) {
-          v[u] = ld_at(Ab, o[u]);
-          g[u] = ld_at(S, o[u]);
-        }
-        transform_in<T, UB>(P, v, g, [&](int u) { return ld_at(Cb, o[u]); });
-      } else {
-#pragma unroll
-        for (int u = 0; u < UB; ++u) v[u] = P.sA * ld_at(Ab, o[u]);
-      }
-      if (IN == 2) {
-        T d[UB], e[UB];
-#pragma unroll
-        for (int u = 0; u < UB; ++u) {
-          d[u] = ld_at(Db, o[u]);
-          e[u] = Eb ? ld_at(Eb, o[u]) : T(0);
-        }
-#pragma unroll
-        for (int u = 0; u < UB; ++u) v[u] = v[u] + P.cD * d[u] + P.cE * e[u];
-      }
-#pragma unroll
-      for (int u = 0; u < UB; ++u)
-        if (ll[u] < 64) tile[ll[u] * LS + kk[u]] = v[u];
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < kStageMax; ++u) {
-    const int idx = threadIdx.x + u * 64 * Pn;
-    if (idx < nst) sco[idx] = cv[u];
-  }
-  __syncthreads();
-  const T* c = sco + q * kChunk * kCo;
-  const T* rinv = sco + Pn * kChunk * kCo;
-  const long L = L0 + lane;
-  const bool live = L < P.nlines;
-  T* row = tile + lane * LS;
-  const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
-  T f[kChunk + 4];
-#pragma unroll
-  for (int t = 0; t < kChunk + 4; ++t) f[t] = row[window_index(s0 - 2 + t, n_in, P.wrap)];
-  T fb4[4] = {T(0), T(0), T(0), T(0)};
-  if (P.fbc && live && (q == 0 || q == Pn - 1))
-    for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
-  T y[kChunk];
-  chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
-  if (P.solve) {
-    T yf, yl;
-    chunk_solve(c, len, y, yf, yl);
-    __syncthreads();  // every wave has read its window: the tile's start takes the exchange
-    exb[lane + 64 * (2 * q)] = yf;
-    exb[lane + 64 * (2 * q + 1)] = yl;
-    __syncthreads();
-    chunk_couple<T>(P, q, rinv, c, [&](int k) { return exb[lane + 64 * k]; }, y);
-  }
-  __syncthreads();  // every wave has finished reading its window and the exchange
-#pragma unroll
-  for (int i = 0; i < kChunk; ++i)
-    if (i < len) row[s0 + i] = chunk_post<T>(P, c, i, s0 + i, y[i], fb4);
-  __syncthreads();
-  {
-    const int dl = nthr / n_out, dk = nthr - dl * n_out;
-    int l = tid / n_out, k = tid - l * n_out;
-    const int tout = 64 * n_out;
-    T* const Ob = P.out + L0 * P.o1;
-    for (int b0 = 0; b0 < tout; b0 += nthr * UB) {
-      unsigned o[UB];
-      bool ok[UB];
-      T val[UB], prev[UB];
-#pragma unroll
-      for (int u = 0; u < UB; ++u) {
-        ok[u] = (l <= lmax);
-        const int lc = l < lmax ? l : lmax;
-        o[u] = (unsigned)((lc * P.o1 + k) * (int)sizeof(T));
-        val[u] = tile[lc * LS + k];
-        prev[u] = T(0);
-        k += dk;
-        l += dl;
-        if (k >= n_out) { k -= n_out; ++l; }
-      }
-      if (P.accumulate) {
-#pragma unroll
-        for (int u = 0; u < UB; ++u) prev[u] = ld_at<T>(Ob, o[u]);
-      }
-#pragma unroll
-      for (int u = 0; u < UB; ++u)
-        if (ok[u]) st_at<T>(Ob, o[u], prev[u] + val[u] * P.sOut);
-    }
-  }
-}
 
-template <typename T, int IN>
-static void dispatch_xu_s(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
-  switch (op) {
-    case MIDP_C2P: hipLaunchKernelGGL((k_line_xu<T, MIDP_C2P, IN>), g, b, shm, s, a); break;
-    case MIDP_P2C: hipLaunchKernelGGL((k_line_xu<T, MIDP_P2C, IN>), g, b, shm, s, a); break;
-    case D1_C2C: hipLaunchKernelGGL((k_line_xu<T, D1_C2C, IN>), g, b, shm, s, a); break;
-    case D1_P2P: hipLaunchKernelGGL((k_line_xu<T, D1_P2P, IN>), g, b, shm, s, a); break;
-    case D1_C2P: hipLaunchKernelGGL((k_line_xu<T, D1_C2P, IN>), g, b, shm, s, a); break;
-    case D1_P2C: hipLaunchKernelGGL((k_line_xu<T, D1_P2C, IN>), g, b, shm, s, a); break;
-  }
-}
 
-// number of input arrays beyond A, as the kernels' IN parameter
-template <typename T>
-static int input_kind(const KArgs<T>& a) { return a.D ? 2 : ((a.B || a.Cm) ? 1 : 0); }
-
-template <typename T>
-static void dispatch_xu(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
-  switch (input_kind(a)) {
-    case 0: dispatch_xu_s<T, 0>(op, g, b, shm, s, a); break;
-    case 1: dispatch_xu_s<T, 1>(op, g, b, shm, s, a); break;
-    default: dispatch_xu_s<T, 2>(op, g, b, shm, s, a); break;
-  }
-}
 
 // LDS bytes a workgroup may use, queried once per device.
 static size_t max_lds_per_block() {
@@ -939,23 +422,13 @@ static void dispatch_long(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, con
   }
 }
 
+// y and z lines of kChunk-row chunks
 template <typename T, int DIR>
 static void dispatch_part(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
-  if constexpr (DIR == 0) {
-    switch (op) {
-      case MIDP_C2P: hipLaunchKernelGGL((k_line_x<T, MIDP_C2P>), g, b, shm, s, a); break;
-      case MIDP_P2C: hipLaunchKernelGGL((k_line_x<T, MIDP_P2C>), g, b, shm, s, a); break;
-      case D1_C2C: hipLaunchKernelGGL((k_line_x<T, D1_C2C>), g, b, shm, s, a); break;
-      case D1_P2P: hipLaunchKernelGGL((k_line_x<T, D1_P2P>), g, b, shm, s, a); break;
-      case D1_C2P: hipLaunchKernelGGL((k_line_x<T, D1_C2P>), g, b, shm, s, a); break;
-      case D1_P2C: hipLaunchKernelGGL((k_line_x<T, D1_P2C>), g, b, shm, s, a); break;
-    }
-  } else {
-    switch (input_kind(a)) {
-      case 0: dispatch_yz<T, DIR, 0>(op, g, b, shm, s, a); break;
-      case 1: dispatch_yz<T, DIR, 1>(op, g, b, shm, s, a); break;
-      default: dispatch_yz<T, DIR, 2>(op, g, b, shm, s, a); break;
-    }
+  switch (input_kind(a)) {
+    case 0: dispatch_yz<T, DIR, 0>(op, g, b, shm, s, a); break;
+    case 1: dispatch_yz<T, DIR, 1>(op, g, b, shm, s, a); break;
+    default: dispatch_yz<T, DIR, 2>(op, g, b, shm, s, a); break;
   }
 }
 
@@ -1009,76 +482,82 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
     a.wrap = (p.ibc[0] == BC_PERIODIC);
     a.co = D.co;
     const int nmax = p.n_in > p.n_out ? p.n_in : p.n_out;
-    const int lsu = nmax | 1;  // odd row stride
-    // tile of 64 lines (its start later holds the 64 x 2P exchange) and the
-    // per-plan coefficients
-    const size_t shm_u = sizeof(T) * ((size_t)64 * lsu + (size_t)p.part_P * kChunk * kCo +
-                                      4 * (size_t)p.part_P * p.part_P);
-    if (p.part_ch == kChunkLong) {
-      // long lines: the lane-per-line kernel with chunks of kChunkLong rows;
-      // per-lane byte offsets from a workgroup's first line stay below 2^32
-      const long span = (dir == 0) ? 64L * (ein.n[0] > eout.n[0] ? ein.n[0] : eout.n[0])
-                         : (dir == 1) ? (long)ein.n[0] * (ein.n[1] > eout.n[1] ? ein.n[1] : eout.n[1]) *
-                                            (64 / ein.n[0] + 2)
-                                      : 64L;
-      if (span * (long)sizeof(T) >= (1L << 32)) return "line op: lines too far apart for 32-bit offsets";
-      dim3 g((unsigned)((a.nlines + 63) / 64)), b((unsigned)(64 * p.part_P));
-      const size_t shm = sizeof(T) * ((size_t)p.part_P * kChunkLong * kCo + 4 * (size_t)p.part_P * p.part_P);
-      if (dir == 0) dispatch_long<T, 0>(p.op, g, b, shm, s, a);
-      else if (dir == 1) dispatch_long<T, 1>(p.op, g, b, shm, s, a);
-      else dispatch_long<T, 2>(p.op, g, b, shm, s, a);
-    } else if (dir == 0) {
-      // two tile shapes for x lines: 64 whole lines in LDS (k_line_xu) or
-      // 256/P lines with skewed rows (k_line_x); timed once per shape when a
-      // scratch output is at hand (chs_tune.h), else the 64-line tile
-      const bool xu_fits = shm_u <= max_lds_per_block();
+    const int CHp = p.part_ch;
+    const size_t P2 = 4 * (size_t)p.part_P * p.part_P;  // reduced-system inverse
+    // per-lane byte offsets of the lane-per-line kernel (y/z lines, long x
+    // lines) from a workgroup's first line stay below 2^32 (a y-line block
+    // spans at most 64/e1 + 2 planes of x-y)
+    const long span = (dir == 0) ? 64L * nmax
+                       : (dir == 1) ? (long)ein.n[0] * (ein.n[1] > eout.n[1] ? ein.n[1] : eout.n[1]) *
+                                          (64 / ein.n[0] + 2)
+                                    : 64L;
+    const bool lanes_ok = span * (long)sizeof(T) < (1L << 32);
+    const dim3 gl((unsigned)((a.nlines + 63) / 64)), bl((unsigned)(64 * p.part_P));
+    const size_t shm_l = sizeof(T) * ((size_t)p.part_P * CHp * kCo + P2);
+    if (dir == 0) {
+      // x lines, four launch shapes (timed once per shape when a scratch
+      // output is at hand, chs_tune.h; else the first that fits):
+      //  0: 64 whole lines in LDS, one chunk per wave (coefficient rows in LDS)
+      //  1: 32 whole lines in LDS, two chunks per wave (coefficient rows
+      //     through the vector cache): half the LDS, two workgroups per CU
+      //  2: 256/P lines with skewed rows
+      //  3: lines of kChunkLong-row chunks only: the lane-per-line kernel
+      const size_t lds = max_lds_per_block();
+      const int lsu = nmax | 1;  // odd row stride
+      const size_t shm64 = sizeof(T) * ((size_t)64 * lsu + (size_t)p.part_P * CHp * kCo + P2);
+      const size_t shm32 = sizeof(T) * ((size_t)32 * lsu + P2);
       int ls = lds_pos_host(nmax) + 1;
       ls += ((8 - ls) % 32 + 32) % 32;  // row stride = 8 (mod 32) doubles: neighbouring lines in other banks
-      // lines per workgroup: at most 256 threads and 48 KiB of LDS
+      // lines per workgroup: at most 256 threads and 48 KiB of LDS (long
+      // lines: half the LDS of a CU)
       int nl = 256 / p.part_P;
-      const int cap = (int)((48 * 1024) / (sizeof(T) * (size_t)(ls + 2 * p.part_P)));
+      const size_t skew_lds = CHp == kChunk ? (size_t)48 * 1024 : lds / 2;
+      const int cap = (int)(skew_lds / (sizeof(T) * (size_t)(ls + 2 * p.part_P)));
       if (nl > cap) nl = cap;
       auto run = [&](int v, KArgs<T> ka) -> bool {
-        if (v == 0) {
-          if (!xu_fits) return false;
+        if (v == 0 || v == 1) {
+          const int nlw = v == 0 ? 64 : 32;
+          const size_t shm = v == 0 ? shm64 : shm32;
+          if (shm > lds) return false;
           ka.ls = lsu;
-          dim3 g((unsigned)((ka.nlines + 63) / 64)), b((unsigned)(64 * p.part_P));
-          dispatch_xu<T>(p.op, g, b, shm_u, s, ka);
-        } else {
+          dim3 g((unsigned)((ka.nlines + nlw - 1) / nlw)), b((unsigned)(nlw * p.part_P));
+          if (!launch_x_tile<T>(p.op, CHp, nlw, g, b, shm, s, ka)) return false;
+        } else if (v == 2) {
           if (nl < 1) return false;
           ka.ls = ls;
           ka.nl = nl;
           const size_t shm = sizeof(T) * ((size_t)nl * ls + (size_t)nl * 2 * p.part_P);
           dim3 g((unsigned)((ka.nlines + nl - 1) / nl)), b((unsigned)(nl * p.part_P));
-          dispatch_part<T, 0>(p.op, g, b, shm, s, ka);
+          launch_x_skew<T>(p.op, CHp, g, b, shm, s, ka);
+        } else {
+          if (CHp != kChunkLong || !lanes_ok) return false;
+          dispatch_long<T, 0>(p.op, gl, bl, shm_l, s, ka);
         }
         return hipGetLastError() == hipSuccess;
       };
+      const int nvar = CHp == kChunkLong ? 4 : 3;
       int v = -1;
-      if (scratch && xu_fits && nl >= 1) {
+      if (scratch) {
         KArgs<T> ts = a;
         ts.out = scratch;  // the candidates write the scratch array, never the output
         ts.accumulate = 0;
         const std::string key = std::string("line_x/") + (sizeof(T) == 8 ? "f64/" : "f32/") + std::to_string(p.op) +
                                 "/" + std::to_string(p.n_in) + "/" + std::to_string(p.n_out) + "/" +
                                 std::to_string(a.nlines) + "/" + std::to_string(input_kind(a));
-        v = tune(key, 2, s, [&](int vv) { return run(vv, ts); });
+        v = tune(key, nvar, s, [&](int vv) { return run(vv, ts); });
       }
-      if (v < 0) v = xu_fits ? 0 : 1;
-      if (v == 1 && nl < 1) return "line op: line does not fit in LDS";
-      if (!run(v, a)) return std::string("line op launch: ") + hipGetErrorString(hipGetLastError());
+      bool done = false;
+      if (v >= 0) done = run(v, a);
+      for (int w = 0; w < nvar && !done; ++w) done = run(w, a);
+      if (!done) return std::string("line op: no x-line launch shape fits: ") + hipGetErrorString(hipGetLastError());
+    } else if (CHp == kChunkLong) {
+      if (!lanes_ok) return "line op: lines too far apart for 32-bit offsets";
+      if (dir == 1) dispatch_long<T, 1>(p.op, gl, bl, shm_l, s, a);
+      else dispatch_long<T, 2>(p.op, gl, bl, shm_l, s, a);
     } else {
-      // per-lane byte offsets from a workgroup's first line stay below 2^32
-      // (a y-line block spans at most 64/e1 + 2 planes of x-y)
-      if (dir == 1) {
-        const long e2m = ein.n[1] > eout.n[1] ? ein.n[1] : eout.n[1];
-        if ((long)ein.n[0] * e2m * (64 / ein.n[0] + 2) * (long)sizeof(T) >= (1L << 32))
-          return "line op: x-y plane too large for 32-bit line offsets";
-      }
-      dim3 g((unsigned)((a.nlines + 63) / 64)), b((unsigned)(64 * p.part_P));
-      const size_t shm_yz = sizeof(T) * ((size_t)p.part_P * kChunk * kCo + 4 * (size_t)p.part_P * p.part_P);
-      if (dir == 1) dispatch_part<T, 1>(p.op, g, b, shm_yz, s, a);
-      else dispatch_part<T, 2>(p.op, g, b, shm_yz, s, a);
+      if (!lanes_ok) return "line op: x-y plane too large for 32-bit line offsets";
+      if (dir == 1) dispatch_part<T, 1>(p.op, gl, bl, shm_l, s, a);
+      else dispatch_part<T, 2>(p.op, gl, bl, shm_l, s, a);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return std::string("line op launch: ") + hipGetErrorString(e);

@@ -1,0 +1,302 @@
+// x-line kernels of the partitioned line operators: whole x lines (contiguous
+// in memory) are staged in LDS with coalesced loads, every thread of a
+// workgroup then evaluates one chunk of one line from LDS (explicit RHS,
+// chunk-local Thomas sweeps, reduced-system coupling; the algebra of
+// chs_lineop_dev.h), and the results return through LDS to coalesced stores.
+#include "chs_lineop_dev.h"
+
+namespace chs {
+
+// 256/P lines per workgroup with skewed LDS rows (lds_pos), thread
+// (line tid / P, chunk tid % P).
+template <typename T, int OPK, int CH>
+__global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* tile = reinterpret_cast<T*>(smem_raw);
+  const int NL = P.nl, LS = P.ls, Pn = P.P;
+  T* exb = tile + (long)NL * LS;
+  const int tid = threadIdx.x;
+  const int nthr = NL * Pn;
+  const int ll = tid / Pn, q = tid - ll * Pn;
+  const long L0 = (long)blockIdx.x * NL;
+  const int n_in = P.n_in, n_out = P.n_out;
+  // stage NL lines (input transform applied)
+  // (an x line L starts at L * e1 in the input and at L * o1 in the output)
+  for (int e = tid; e < NL * n_in; e += nthr) {
+    const int l = e / n_in, k = e - l * n_in;
+    const long L = L0 + l;
+    T v = T(0);
+    if (L < P.nlines) {
+      const long o = L * P.e1 + k;
+      v = P.A[o];
+      if (P.B) v = v + P.cB * P.B[o];
+      v = P.sA * v;
+      if (P.Cm) v = v * P.Cm[o];
+      if (P.D) v = v + P.cD * P.D[o];
+      if (P.E) v = v + P.cE * P.E[o];
+    }
+    tile[(long)l * LS + lds_pos(k)] = v;
+  }
+  __syncthreads();
+  const long L = L0 + ll;
+  const bool live = L < P.nlines;
+  const T* row = tile + (long)ll * LS;
+  const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
+  T f[CH + 4];
+#pragma unroll
+  for (int t = 0; t < CH + 4; ++t) f[t] = row[lds_pos(window_index(s0 - 2 + t, n_in, P.wrap))];
+  T fb4[4] = {T(0), T(0), T(0), T(0)};
+  if (P.fbc && live && (q == 0 || q == Pn - 1))
+    for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
+  T y[CH];
+  chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
+  if (P.solve) {
+    T yf, yl;
+    chunk_solve(P.co + q * CH * kCo, len, y, yf, yl);
+    T* ex = exb + (long)ll * 2 * Pn;
+    ex[2 * q] = yf;
+    ex[2 * q + 1] = yl;
+    __syncthreads();
+    chunk_couple<T>(P, q, P.rinv, P.co + q * CH * kCo, [&](int k) { return ex[k]; }, y);
+  }
+  __syncthreads();  // every thread has finished reading its input rows
+  T* orow = tile + (long)ll * LS;
+#pragma unroll
+  for (int i = 0; i < CH; ++i)
+    if (i < len) orow[lds_pos(s0 + i)] = chunk_post<T>(P, P.co + q * CH * kCo, i, s0 + i, y[i], fb4);
+  __syncthreads();
+  for (int e = tid; e < NL * n_out; e += nthr) {
+    const int l = e / n_out, k = e - l * n_out;
+    const long Lw = L0 + l;
+    if (Lw >= P.nlines) continue;
+    const long o = Lw * P.o1 + k;
+    const T val = tile[(long)l * LS + lds_pos(k)];
+    const T prev = P.accumulate ? P.out[o] : T(0);
+    P.out[o] = prev + val * P.sOut;
+  }
+}
+
+// NLW whole lines per workgroup of NLW x P threads in LDS (row stride
+// ls = n | 1), thread tid on line tid % NLW and chunk tid / NLW.  NLW = 64:
+// one chunk per wave, the chunk coefficient rows staged in LDS after the
+// tile (wave-uniform reads).  NLW = 32: two chunks per wave and only the
+// reduced-system inverse in LDS, the coefficient rows read through the vector
+// cache; the tile takes half the LDS of the 64-line form, so lines twice as
+// long fit, or two workgroups share a CU.  IN as in load_window.
+template <typename T, int OPK, int IN, int CH, int NLW>
+__global__ void __launch_bounds__(NLW * kMaxChunks) k_line_xu(KArgs<T> P) {
+  constexpr bool kCoLds = (NLW == 64);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* tile = reinterpret_cast<T*>(smem_raw);
+  const int LS = P.ls, Pn = P.P;
+  // after the windows are read, the start of the tile holds the exchanged
+  // first/last chunk values (NLW x 2P)
+  T* exb = tile;
+  // per-plan values after the tile (chunk rows if kCoLds, then the
+  // reduced-system inverse); their loads are issued first and written once
+  // staging is done
+  T* sco = tile + (long)NLW * LS;
+  const int nco = kCoLds ? Pn * CH * kCo : 0;
+  const int nst = nco + (P.rinv ? 4 * Pn * Pn : 0);
+  constexpr int NS = kCoLds ? stage_max(CH) : (4 * kMaxChunks + NLW - 1) / NLW;
+  static_assert(NLW * kMaxChunks * NS >= (kCoLds ? kMaxChunks * CH * kCo : 0) + 4 * kMaxChunks * kMaxChunks,
+                "per-plan staging exceeds NS values per thread");
+  const int tid = threadIdx.x, nthr = NLW * Pn;
+  T cv[NS];
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const int idx = tid + u * nthr;
+    cv[u] = (idx < nco) ? P.co[idx] : (idx < nst ? P.rinv[idx - nco] : T(0));
+  }
+  const int lane = tid & (NLW - 1);
+  const int q = kCoLds ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid / NLW;
+  const long L0 = (long)blockIdx.x * NLW;
+  const int n_in = P.n_in, n_out = P.n_out;
+  // staging: element e = l * n_in + k of the workgroup's NLW lines, thread
+  // tid taking e = tid, tid + nthr, ...; (l, k) advance by a fixed step, so
+  // no division per element.  Batches of UB elements, all loads of a batch
+  // issued before any use; the line is clamped so every address is valid.
+  constexpr int UB = 8;
+  const long Lmax = P.nlines - 1;
+  const int lmax = (int)((Lmax - L0) < NLW - 1 ? (Lmax - L0) : NLW - 1);  // last line of the workgroup
+  const T* const Ab = P.A + L0 * P.e1;
+  const T* const Bb = P.B ? P.B + L0 * P.e1 : nullptr;
+  const T* const Cb = P.Cm ? P.Cm + L0 * P.e1 : nullptr;
+  const T* const Db = P.D ? P.D + L0 * P.e1 : nullptr;
+  const T* const Eb = P.E ? P.E + L0 * P.e1 : nullptr;
+  {
+    const int dl = nthr / n_in, dk = nthr - dl * n_in;
+    int l = tid / n_in, k = tid - l * n_in;
+    const int tin = NLW * n_in;
+    for (int b0 = 0; b0 < tin; b0 += nthr * UB) {
+      unsigned o[UB];
+      int ll[UB], kk[UB];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) {
+        ll[u] = l;
+        kk[u] = k;
+        o[u] = (unsigned)(((l < lmax ? l : lmax) * P.e1 + k) * (int)sizeof(T));
+        k += dk;
+        l += dl;
+        if (k >= n_in) { k -= n_in; ++l; }
+      }
+      T v[UB];
+      const T* S = Bb ? Bb : Cb;
+      if (IN == 1 || (IN == 2 && S)) {
+        T g[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+          v[u] = ld_at(Ab, o[u]);
+          g[u] = ld_at(S, o[u]);
+        }
+        transform_in<T, UB>(P, v, g, [&](int u) { return ld_at(Cb, o[u]); });
+      } else {
+#pragma unroll
+        for (int u = 0; u < UB; ++u) v[u] = P.sA * ld_at(Ab, o[u]);
+      }
+      if (IN == 2) {
+        T d[UB], e[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+          d[u] = ld_at(Db, o[u]);
+          e[u] = Eb ? ld_at(Eb, o[u]) : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < UB; ++u) v[u] = v[u] + P.cD * d[u] + P.cE * e[u];
+      }
+#pragma unroll
+      for (int u = 0; u < UB; ++u)
+        if (ll[u] < NLW) tile[ll[u] * LS + kk[u]] = v[u];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const int idx = tid + u * nthr;
+    if (idx < nst) sco[idx] = cv[u];
+  }
+  __syncthreads();
+  const T* c = (kCoLds ? sco : P.co) + q * CH * kCo;
+  const T* rinv = sco + nco;
+  const long L = L0 + lane;
+  const bool live = L < P.nlines;
+  T* row = tile + lane * LS;
+  const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
+  T f[CH + 4];
+#pragma unroll
+  for (int t = 0; t < CH + 4; ++t) f[t] = row[window_index(s0 - 2 + t, n_in, P.wrap)];
+  T fb4[4] = {T(0), T(0), T(0), T(0)};
+  if (P.fbc && live && (q == 0 || q == Pn - 1))
+    for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
+  T y[CH];
+  chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
+  if (P.solve) {
+    T yf, yl;
+    chunk_solve(c, len, y, yf, yl);
+    __syncthreads();  // every wave has read its window: the tile's start takes the exchange
+    exb[lane + NLW * (2 * q)] = yf;
+    exb[lane + NLW * (2 * q + 1)] = yl;
+    __syncthreads();
+    chunk_couple<T>(P, q, rinv, c, [&](int k) { return exb[lane + NLW * k]; }, y);
+  }
+  __syncthreads();  // every wave has finished reading its window and the exchange
+#pragma unroll
+  for (int i = 0; i < CH; ++i)
+    if (i < len) row[s0 + i] = chunk_post<T>(P, c, i, s0 + i, y[i], fb4);
+  __syncthreads();
+  {
+    const int dl = nthr / n_out, dk = nthr - dl * n_out;
+    int l = tid / n_out, k = tid - l * n_out;
+    const int tout = NLW * n_out;
+    T* const Ob = P.out + L0 * P.o1;
+    for (int b0 = 0; b0 < tout; b0 += nthr * UB) {
+      unsigned o[UB];
+      bool ok[UB];
+      T val[UB], prev[UB];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) {
+        ok[u] = (l <= lmax);
+        const int lc = l < lmax ? l : lmax;
+        o[u] = (unsigned)((lc * P.o1 + k) * (int)sizeof(T));
+        val[u] = tile[lc * LS + k];
+        prev[u] = T(0);
+        k += dk;
+        l += dl;
+        if (k >= n_out) { k -= n_out; ++l; }
+      }
+      if (P.accumulate) {
+#pragma unroll
+        for (int u = 0; u < UB; ++u) prev[u] = ld_at<T>(Ob, o[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < UB; ++u)
+        if (ok[u]) st_at<T>(Ob, o[u], prev[u] + val[u] * P.sOut);
+    }
+  }
+}
+
+template <typename T, int IN, int CH, int NLW>
+static void tile_ops(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
+  switch (op) {
+    case MIDP_C2P: hipLaunchKernelGGL((k_line_xu<T, MIDP_C2P, IN, CH, NLW>), g, b, shm, s, a); break;
+    case MIDP_P2C: hipLaunchKernelGGL((k_line_xu<T, MIDP_P2C, IN, CH, NLW>), g, b, shm, s, a); break;
+    case D1_C2C: hipLaunchKernelGGL((k_line_xu<T, D1_C2C, IN, CH, NLW>), g, b, shm, s, a); break;
+    case D1_P2P: hipLaunchKernelGGL((k_line_xu<T, D1_P2P, IN, CH, NLW>), g, b, shm, s, a); break;
+    case D1_C2P: hipLaunchKernelGGL((k_line_xu<T, D1_C2P, IN, CH, NLW>), g, b, shm, s, a); break;
+    case D1_P2C: hipLaunchKernelGGL((k_line_xu<T, D1_P2C, IN, CH, NLW>), g, b, shm, s, a); break;
+  }
+}
+
+template <typename T, int CH, int NLW>
+static void tile_in(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
+  switch (input_kind(a)) {
+    case 0: tile_ops<T, 0, CH, NLW>(op, g, b, shm, s, a); break;
+    case 1: tile_ops<T, 1, CH, NLW>(op, g, b, shm, s, a); break;
+    default: tile_ops<T, 2, CH, NLW>(op, g, b, shm, s, a); break;
+  }
+}
+
+template <typename T>
+bool launch_x_tile(int op, int ch, int nlw, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
+  if (nlw != 64 && nlw != 32) return false;
+  if (ch == kChunk) {
+    if (nlw == 64) tile_in<T, kChunk, 64>(op, g, b, shm, s, a);
+    else tile_in<T, kChunk, 32>(op, g, b, shm, s, a);
+    return true;
+  }
+  if (ch != kChunkLong) return false;
+  if (nlw == 32) {
+    tile_in<T, kChunkLong, 32>(op, g, b, shm, s, a);
+    return true;
+  }
+  // 64 lines of long chunks: FP32 only (an FP64 tile exceeds the LDS)
+  if constexpr (sizeof(T) == 4) {
+    tile_in<T, kChunkLong, 64>(op, g, b, shm, s, a);
+    return true;
+  }
+  return false;
+}
+
+template <typename T, int CH>
+static void skew_ops(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
+  switch (op) {
+    case MIDP_C2P: hipLaunchKernelGGL((k_line_x<T, MIDP_C2P, CH>), g, b, shm, s, a); break;
+    case MIDP_P2C: hipLaunchKernelGGL((k_line_x<T, MIDP_P2C, CH>), g, b, shm, s, a); break;
+    case D1_C2C: hipLaunchKernelGGL((k_line_x<T, D1_C2C, CH>), g, b, shm, s, a); break;
+    case D1_P2P: hipLaunchKernelGGL((k_line_x<T, D1_P2P, CH>), g, b, shm, s, a); break;
+    case D1_C2P: hipLaunchKernelGGL((k_line_x<T, D1_C2P, CH>), g, b, shm, s, a); break;
+    case D1_P2C: hipLaunchKernelGGL((k_line_x<T, D1_P2C, CH>), g, b, shm, s, a); break;
+  }
+}
+
+template <typename T>
+void launch_x_skew(int op, int ch, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
+  if (ch == kChunk) skew_ops<T, kChunk>(op, g, b, shm, s, a);
+  else skew_ops<T, kChunkLong>(op, g, b, shm, s, a);
+}
+
+template bool launch_x_tile<double>(int, int, int, dim3, dim3, size_t, hipStream_t, const KArgs<double>&);
+template bool launch_x_tile<float>(int, int, int, dim3, dim3, size_t, hipStream_t, const KArgs<float>&);
+template void launch_x_skew<double>(int, int, dim3, dim3, size_t, hipStream_t, const KArgs<double>&);
+template void launch_x_skew<float>(int, int, dim3, dim3, size_t, hipStream_t, const KArgs<float>&);
+
+}  // namespace chs

@@ -22,4 +22,9 @@ int tune(const std::string& key, int nvar, hipStream_t s, const std::function<bo
 // "key=choice (t0/t1/.. us)" lines of every tuned shape, for reports.
 std::string tune_report();
 
+// While variant >= 0, tune() returns `variant` (or -1 if it is >= nvar)
+// without timing or caching for every key that starts with `prefix`, so
+// tests can run each launch shape; variant < 0 restores tuning.
+void tune_force(const std::string& prefix, int variant);
+
 }  // namespace chs

@@ -14,7 +14,15 @@ struct Entry {
 };
 std::mutex g_mu;
 std::map<std::string, Entry> g_cache;
+std::string g_force_prefix;
+int g_force_variant = -1;
 }  // namespace
+
+void tune_force(const std::string& prefix, int variant) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_force_prefix = prefix;
+  g_force_variant = variant;
+}
 
 int tuned_choice(const std::string& key) {
   std::lock_guard<std::mutex> lk(g_mu);
@@ -23,6 +31,11 @@ int tuned_choice(const std::string& key) {
 }
 
 int tune(const std::string& key, int nvar, hipStream_t s, const std::function<bool(int)>& launch) {
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_force_variant >= 0 && key.compare(0, g_force_prefix.size(), g_force_prefix) == 0)
+      return g_force_variant < nvar ? g_force_variant : -1;
+  }
   const int have = tuned_choice(key);
   if (have >= 0) return have;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;

@@ -37,11 +37,22 @@ int chs_inspect_projection(const chs_desc_t *desc, double *ly, double *lam, doub
 
 /* Launch timing of one kernel class during subsequent solver and chs_op3d
  * calls, with HIP events on the handle's stream: class 0/1/2 = x/y/z line
- * operators.  chs_profile_end returns the summed
+ * operators, 3 = whole chs_poisson / chs_poisson_f32 solves (counted as five
+ * passes that each read and write the field).  chs_profile_end returns the summed
  * event time, the number of timed launches and their algorithmic bytes
- * (arrays read plus arrays written, 8 bytes per element). */
+ * (arrays read plus arrays written, 8 bytes per FP64 and 4 per FP32
+ * element). */
 int chs_profile_begin(chs_handle_t *h, int kernel_class);
 int chs_profile_end(chs_handle_t *h, double *ms_total, long long *launches, double *alg_bytes);
+
+/* FP32 forms of chs_op3d and chs_poisson, on device memory only: the same
+ * operator plans and kernels in single precision, as the mixed-precision
+ * solver runs them.  For the operator benchmark and precision tests; no
+ * counterpart in the reference, which is FP64 throughout.  fbc may be NULL;
+ * rhs is solved in place (cell-centred, nx*ny*nz). */
+int chs_op3d_f32(chs_handle_t *h, int dir, int op, int iacc, const int ibc[2], const float *fi,
+                 const int shp_in[3], float *fo, const int shp_out[3], const float *fbc);
+int chs_poisson_f32(chs_handle_t *h, float *rhs);
 
 /* Launch shapes chosen so far by the autotuner (FFT lines per workgroup,
  * x-line tile), one "family/shape=choice (us per variant)" line each, for
@@ -49,6 +60,13 @@ int chs_profile_end(chs_handle_t *h, double *ms_total, long long *launches, doub
  * terminating 0; returns the number of tuned shapes.  No counterpart in the
  * reference, which has no tunable launch shapes. */
 int chs_tune_report(char *buf, int len);
+
+/* While variant >= 0, every autotuned launch whose family key starts with
+ * prefix (e.g. "line_x/") takes launch shape `variant` instead of the timed
+ * choice (the next shape that applies when it does not); variant < 0 restores
+ * tuning.  Process-wide.  For tests that check every launch shape against
+ * the oracle; no counterpart in the reference. */
+int chs_tune_force(const char *prefix, int variant);
 
 #ifdef __cplusplus
 }

@@ -197,3 +197,119 @@ def test_ops_long_chunk_lines(po, nc, d):
     finally:
         dm.close()
     assert nchecked >= 24
+
+
+def test_ops_fp32_vs_oracle(chan, po):
+    """FP32 operators (chs_op3d_f32, the kernels of the mixed-precision
+    solver) against the FP64 oracle: every operator along every direction
+    with the channel boundary kinds, error relative to the largest output
+    within 2e-6 (FP32 round-off, 6e-8, amplified by the compact operators'
+    stencil sums and line solves)."""
+    import ctypes as C
+
+    import torch
+
+    from chapsim2_amd import _lib
+
+    odm = po.Dom(list(NC), [1, 0, 1], LX, -1.0, 1.0, LZ, 2, 0.075)
+    rng = np.random.default_rng(29)
+    worst = 0.0
+    for d in (0, 1, 2):
+        for op in range(6):
+            bc = [1, 1] if d != 1 else ([5, 5] if INP[op] == "c" else [4, 4])
+            ext_in, ext_out = list(NC), list(NC)
+            ext_in[d] = _n(NC[d], d != 1, INP[op])
+            ext_out[d] = _n(NC[d], d != 1, OUT[op])
+            fi = np.asfortranarray(rng.standard_normal(ext_in))
+            fbc = None
+            if d == 1:
+                fshape = list(ext_in)
+                fshape[1] = 4
+                fbc = np.asfortranarray(rng.standard_normal(fshape))
+            ref = odm.op3d(d, op, 4, bc, fi, ext_out, fbc)
+            ti = torch.from_numpy(np.ascontiguousarray(fi.T)).float().cuda()
+            to = torch.zeros(ext_out[::-1], dtype=torch.float32, device="cuda")
+            tb = torch.from_numpy(np.ascontiguousarray(fbc.T)).float().cuda() if fbc is not None else None
+            _lib.check(chan.lib.chs_op3d_f32(chan.h, d, op, 4, _lib.ivec(bc), C.c_void_p(ti.data_ptr()),
+                                             _lib.ivec(ext_in), C.c_void_p(to.data_ptr()), _lib.ivec(ext_out),
+                                             C.c_void_p(tb.data_ptr()) if tb is not None else None))
+            got = to.double().cpu().numpy().T
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            print(f"f32 op dir {d} op {op}: {err:.2e}")
+            assert err < 2e-6, (d, op, err)
+            worst = max(worst, err)
+    assert worst > 1e-9  # the arithmetic really was FP32
+
+
+def test_poisson_fp32_vs_oracle(chan, po):
+    """FP32 Poisson solve (chs_poisson_f32) against the oracle's FP64
+    FFTPACK-form solve of the same zero-mean right-hand side."""
+    import ctypes as C
+
+    import torch
+
+    from chapsim2_amd import _lib
+
+    of = po.Flow(list(NC), LX, -1.0, 1.0, LZ, wall_y=True, iacc=4, poisson=2, istret=2, rstret=0.075, dt=5e-4,
+                 ren=2800.0, idriven=1)
+    rng = np.random.default_rng(31)
+    rhs = rng.standard_normal(NC)
+    rhs -= rhs.mean()
+    rhs = np.asfortranarray(rhs)
+    ref = of.poisson(rhs)
+    t = torch.from_numpy(np.ascontiguousarray(rhs.T)).float().cuda()
+    _lib.check(chan.lib.chs_poisson_f32(chan.h, C.c_void_p(t.data_ptr())))
+    got = t.double().cpu().numpy().T
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"f32 poisson: {err:.2e}")
+    assert 1e-9 < err < 1e-5, err
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_x_launch_shapes_vs_oracle(po, variant):
+    """Every x-line launch shape, forced through chs_tune_force (0: 64 whole
+    lines in LDS, 1: 32 whole lines in LDS with two chunks per wave, 2: skewed
+    rows, 3: lane-per-line, long lines only; a shape that does not apply
+    falls back to the next that does): all operators on lines of 250 rows
+    (16-row chunks), 300 and 512 rows (32-row chunks), CP6 and the explicit
+    CD4, then the momentum right-hand side (merged operators reading two to
+    four arrays along x), against the oracle."""
+    from chapsim2_amd import _lib
+    from chapsim2_amd import operations as ops
+    from chapsim2_amd.domain import Domain
+
+    lib = _lib.load()
+    _lib.check(lib.chs_tune_force(b"line_x/", variant))
+    try:
+        for nc in ((250, 8, 12), (300, 8, 12), (512, 8, 8)):
+            dm = Domain(nc, LX, -1.0, 1.0, LZ, istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1)
+            odm = po.Dom(list(nc), [1, 0, 1], LX, -1.0, 1.0, LZ, 2, 0.075)
+            rng = np.random.default_rng(41 + variant)
+            try:
+                for op in range(6):
+                    for acc in (2, 4):
+                        fi = np.asfortranarray(rng.standard_normal(nc))
+                        ref = odm.op3d(0, op, acc, [1, 1], fi, list(nc), None)
+                        out = np.zeros(nc, order="F")
+                        ops.op3d(dm, 0, op, fi, out, acc, [1, 1], None)
+                        err = np.abs(out - ref).max() / np.abs(ref).max()
+                        assert err < RTOL, (variant, nc, op, acc, err)
+            finally:
+                dm.close()
+        from chapsim2_amd.initial import channel_fields
+
+        cfg = dict(istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1)
+        dm = Domain(NC, LX, -1.0, 1.0, LZ, **cfg)
+        of = po.Flow(list(NC), LX, -1.0, 1.0, LZ, wall_y=True, iacc=4, poisson=2, **cfg)
+        try:
+            fields = channel_fields(NC, LX, -1.0, 1.0, LZ, 2, 0.075, noise=0.1, seed=777)
+            dm.set_state(*fields)
+            of.set_state(*fields)
+            for isub in (0, 1):
+                for g, r, name in zip(dm.momentum_rhs(isub), of.momentum_rhs(isub), ("mx", "my", "mz")):
+                    err = np.abs(g - r).max() / np.abs(r).max()
+                    assert err < RTOL, (variant, isub, name, err)
+        finally:
+            dm.close()
+    finally:
+        _lib.check(lib.chs_tune_force(b"line_x/", -1))
