@@ -42,6 +42,7 @@ struct RhsWork {
   T *fby_conv = nullptr, *fby_visc = nullptr, *fby_div = nullptr;
   T *plane_a = nullptr, *plane_b = nullptr, *plane_c = nullptr, *plane_d = nullptr;
   T* scratch = nullptr;
+  T* scratch_side[2] = {nullptr, nullptr};  // the same for operators on the side streams
   long scratch_n = 0;
   bool ready = false;
 };
@@ -79,18 +80,36 @@ class Flow {
   long npartial = 0;  // entries of partial (at least kMaxPartials)
   double* dyw_dev = nullptr;
 
+  // Independent operators of a sub-step run on three streams (lane 0 = the
+  // handle's stream, lanes 1 and 2 = side streams), forked from and joined
+  // back into the handle's stream with events, so that their launches
+  // overlap and fill each other's partial waves.  Off until the first
+  // sub-step has planned and tuned every operator one at a time, while the
+  // handle's stream is being captured into a graph (the batched form's
+  // captured steps already run their instances on separate streams), and
+  // while launch profiling is active (each launch is then timed alone).
+  bool concurrent = false;
+
  private:
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+  bool lanes_on = false;  // concurrent, and the handle's stream is not capturing
+  void set_lanes();
+  hipStream_t lane_stream(int lane) const { return (lane == 0 || !lanes_on) ? H->stream : side[lane - 1]; }
+  std::string fork();  // the side streams wait for the work issued so far on the handle's stream
+  std::string join();  // the handle's stream waits for the work issued so far on the side streams
+
   template <typename T>
   std::string op(RhsWork<T>& w, int dir, int opk, const int ibc2[2], const Field<T>& A, Field<T>& out,
                  const T* fbc, T sA = T(1), const Field<T>* B = nullptr, T cB = T(1), const Field<T>* Cm = nullptr,
                  T sOut = T(1), int accumulate = 0, const Field<T>* D = nullptr, T cD = T(0),
-                 const Field<T>* E = nullptr, T cE = T(0));
+                 const Field<T>* E = nullptr, T cE = T(0), int lane = 0);
   // out = d(-ca * cb) + r d(ts * (ta + tc * tb)): a convective and a viscous
   // operator of one momentum component along one direction
   template <typename T>
   std::string cov_tau(RhsWork<T>& w, int dir, int opk, const int bc_cov[2], const int bc_tau[2], Field<T>& out,
                       const T* fbc_cov, const T* fbc_tau, const Field<T>& ca, const Field<T>& cb,
-                      const Field<T>& ta, T ts, const Field<T>* tb, T tc);
+                      const Field<T>& ta, T ts, const Field<T>* tb, T tc, int lane = 0);
   template <typename T> std::string momentum_rhs_t(RhsWork<T>& w, int isub);
   template <typename T> std::string alloc_work(RhsWork<T>& w, bool alias_state);
   template <typename T> std::string alloc(Field<T>& f, int e1, int e2, int e3);

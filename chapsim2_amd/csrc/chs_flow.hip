@@ -256,6 +256,12 @@ static void tau_bc(const int ftp[2], const int q[2], int o[2]) {
 
 // ---------------------------------------------------------------------------
 Flow::~Flow() {
+  for (hipStream_t st : side)
+    if (st) (void)hipStreamSynchronize(st);
+  for (hipEvent_t e : {ev_fork, ev_join[0], ev_join[1]})
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t st : side)
+    if (st) (void)hipStreamDestroy(st);
   for (void* p : owned) (void)hipFree(p);
 }
 
@@ -297,8 +303,10 @@ std::string Flow::alloc_work(RhsWork<T>& w, bool alias_state) {
   long big = 0;
   for (auto* fl : {&w.qx, &w.qy, &w.qz, &w.qxdy, &w.qydz, &w.qxiz}) big = std::max(big, fl->size());
   w.scratch_n = big;
-  FCHK(hipMalloc(&w.scratch, sizeof(T) * (size_t)big));
-  owned.push_back(w.scratch);
+  for (T** p : {&w.scratch, &w.scratch_side[0], &w.scratch_side[1]}) {
+    FCHK(hipMalloc(p, sizeof(T) * (size_t)big));
+    owned.push_back(*p);
+  }
   // wall values (nx,4,nz) for u, v, w, p; constant walls as initialise_fbc_flow_given
   const chs_desc_t& d = H->desc;
   const long nf = (long)nx_ * 4 * nz_;
@@ -385,6 +393,10 @@ std::string Flow::create(Handle* H, std::unique_ptr<Flow>* out) {
     FCHK(hipMemcpy(f.dyw_dev, f.dyw.data(), sizeof(double) * ny, hipMemcpyHostToDevice));
     f.owned.push_back(f.dyw_dev);
   }
+  // side streams (default priority) and the events that fork and join them
+  for (hipStream_t* st : {&f.side[0], &f.side[1]}) FCHK(hipStreamCreate(st));
+  for (hipEvent_t* ev : {&f.ev_fork, &f.ev_join[0], &f.ev_join[1]})
+    FCHK(hipEventCreateWithFlags(ev, hipEventDisableTiming));
   FCHK(hipStreamSynchronize(H->stream));
   *out = std::move(F);
   return "";
@@ -398,7 +410,7 @@ void Flow::update_re(int iter) {
 template <typename T>
 std::string Flow::op(RhsWork<T>& w, int dir, int opk, const int ibc2[2], const Field<T>& A, Field<T>& out,
                      const T* fbc, T sA, const Field<T>* B, T cB, const Field<T>* Cm, T sOut, int accumulate,
-                     const Field<T>* D, T cD, const Field<T>* E, T cE) {
+                     const Field<T>* D, T cD, const Field<T>* E, T cE, int lane) {
   const DevPlan<T>* P = nullptr;
   SCHK(H->plan<T>(dir, opk, H->desc.iacc, ibc2, fbc != nullptr, A.e.n[dir], out.e.n[dir], &P));
   LineIn<T> in;
@@ -416,17 +428,43 @@ std::string Flow::op(RhsWork<T>& w, int dir, int opk, const int ibc2[2], const F
   o.sOut = sOut;
   o.accumulate = accumulate;
   if (out.size() > w.scratch_n) return "flow: scratch too small";
+  const hipStream_t s = lane_stream(lane);
+  T* const scr = (lane == 0 || !lanes_on) ? w.scratch : w.scratch_side[lane - 1];
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const bool timed = (H->prof_class == dir) && H->prof_pair(&e0, &e1);
   if (timed) {
-    (void)hipEventRecord(e0, H->stream);
+    (void)hipEventRecord(e0, s);
     const int nin_arrays = 1 + (B ? 1 : 0) + (Cm && Cm->p != A.p ? 1 : 0) + (D ? 1 : 0) + (D && E ? 1 : 0);
     H->prof_bytes += (double)sizeof(T) *
                      ((double)A.size() * nin_arrays + (double)out.size() * (1 + (accumulate ? 1 : 0)));
   }
-  std::string r = launch_line_op<T>(*P, A.e, out.e, in, fbc, o, w.scratch, H->stream);
-  if (timed) (void)hipEventRecord(e1, H->stream);
+  std::string r = launch_line_op<T>(*P, A.e, out.e, in, fbc, o, scr, s);
+  if (timed) (void)hipEventRecord(e1, s);
   return r;
+}
+
+void Flow::set_lanes() {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  // (launch profiling times each kernel on its own, so it keeps one stream)
+  lanes_on = concurrent && side[0] && H->prof_class < 0 && hipStreamIsCapturing(H->stream, &cs) == hipSuccess &&
+             cs == hipStreamCaptureStatusNone;
+}
+
+std::string Flow::fork() {
+  if (!lanes_on) return "";
+  if (hipEventRecord(ev_fork, H->stream) != hipSuccess) return "flow: fork record";
+  for (hipStream_t st : side)
+    if (hipStreamWaitEvent(st, ev_fork, 0) != hipSuccess) return "flow: fork wait";
+  return "";
+}
+
+std::string Flow::join() {
+  if (!lanes_on) return "";
+  for (int i = 0; i < 2; ++i) {
+    if (hipEventRecord(ev_join[i], side[i]) != hipSuccess) return "flow: join record";
+    if (hipStreamWaitEvent(H->stream, ev_join[i], 0) != hipSuccess) return "flow: join wait";
+  }
+  return "";
 }
 
 std::string Flow::set_state(const double* hx, const double* hy, const double* hz, const double* hp, int mem) {
@@ -478,13 +516,14 @@ std::string Flow::copy_rhs(double* mx, double* my, double* mz, int mem) {
 template <typename T>
 std::string Flow::cov_tau(RhsWork<T>& w, int dir, int opk, const int bc_cov[2], const int bc_tau[2], Field<T>& out,
                           const T* fbc_cov, const T* fbc_tau, const Field<T>& ca, const Field<T>& cb,
-                          const Field<T>& ta, T ts, const Field<T>* tb, T tc) {
+                          const Field<T>& ta, T ts, const Field<T>* tb, T tc, int lane) {
   const T r = T(rre), one = T(1), m1 = T(-1);
   if (!fbc_cov && !fbc_tau && bc_cov[0] == bc_tau[0] && bc_cov[1] == bc_tau[1])
     return op<T>(w, dir, opk, bc_cov, ca, out, nullptr, m1, nullptr, one, &cb, one, 0, &ta, r * ts, tb,
-                 tb ? r * ts * tc : T(0));
-  SCHK(op<T>(w, dir, opk, bc_cov, ca, out, fbc_cov, m1, nullptr, one, &cb));
-  return op<T>(w, dir, opk, bc_tau, ta, out, fbc_tau, ts, tb, tc, nullptr, r, 1);
+                 tb ? r * ts * tc : T(0), lane);
+  SCHK(op<T>(w, dir, opk, bc_cov, ca, out, fbc_cov, m1, nullptr, one, &cb, one, 0, nullptr, T(0), nullptr, T(0),
+             lane));
+  return op<T>(w, dir, opk, bc_tau, ta, out, fbc_tau, ts, tb, tc, nullptr, r, 1, nullptr, T(0), nullptr, T(0), lane);
 }
 
 // Compute_momentum_rhs + Calculate_momentum_fractional_step + forcing.
@@ -497,25 +536,34 @@ std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub) {
   const T* fv = wall_y ? w.fby_v : nullptr;
   const T* fw = wall_y ? w.fby_w : nullptr;
   const T one = T(1), m1 = T(-1);
-  // ---- first-level operators, src/eq_momentum2.f90:347-506 ----
-  SCHK(op<T>(w, 0, MIDP_P2C, B[0][VU], w.qx, w.qxix, nullptr));
-  SCHK(op<T>(w, 0, D1_P2C, B[0][VU], w.qx, w.qxdx, nullptr));
-  SCHK(op<T>(w, 1, D1_C2P, B[1][VU], w.qx, w.qxdy, fu));
-  SCHK(op<T>(w, 1, MIDP_C2P, B[1][VU], w.qx, w.qxiy, fu));
-  SCHK(op<T>(w, 2, MIDP_C2P, B[2][VU], w.qx, w.qxiz, nullptr));
-  SCHK(op<T>(w, 2, D1_C2P, B[2][VU], w.qx, w.qxdz, nullptr));
-  SCHK(op<T>(w, 0, D1_C2P, B[0][VV], w.qy, w.qydx, nullptr));
-  SCHK(op<T>(w, 0, MIDP_C2P, B[0][VV], w.qy, w.qyix, nullptr));
-  SCHK(op<T>(w, 1, D1_P2C, B[1][VV], w.qy, w.qydy, nullptr));
-  SCHK(op<T>(w, 1, MIDP_P2C, B[1][VV], w.qy, w.qyiy, fv));
-  SCHK(op<T>(w, 2, D1_C2P, B[2][VV], w.qy, w.qydz, nullptr));
-  SCHK(op<T>(w, 2, MIDP_C2P, B[2][VV], w.qy, w.qyiz, nullptr));
-  SCHK(op<T>(w, 0, D1_C2P, B[0][VW], w.qz, w.qzdx, nullptr));
-  SCHK(op<T>(w, 0, MIDP_C2P, B[0][VW], w.qz, w.qzix, nullptr));
-  SCHK(op<T>(w, 1, MIDP_C2P, B[1][VW], w.qz, w.qziy, fw));
-  SCHK(op<T>(w, 1, D1_C2P, B[1][VW], w.qz, w.qzdy, fw));
-  SCHK(op<T>(w, 2, MIDP_P2C, B[2][VW], w.qz, w.qziz, nullptr));
-  SCHK(op<T>(w, 2, D1_P2C, B[2][VW], w.qz, w.qzdz, nullptr));
+  // one operator of the plain form on a lane
+  auto op1 = [&](int lane, int dir, int opk, const int* bc, const Field<T>& A, Field<T>& out, const T* fbc,
+                 T sA = T(1)) {
+    return op<T>(w, dir, opk, bc, A, out, fbc, sA, nullptr, T(1), nullptr, T(1), 0, nullptr, T(0), nullptr, T(0),
+                 lane);
+  };
+  // ---- first-level operators, src/eq_momentum2.f90:347-506 (one lane per
+  // velocity component) ----
+  SCHK(fork());
+  SCHK(op1(0, 0, MIDP_P2C, B[0][VU], w.qx, w.qxix, nullptr));
+  SCHK(op1(0, 0, D1_P2C, B[0][VU], w.qx, w.qxdx, nullptr));
+  SCHK(op1(0, 1, D1_C2P, B[1][VU], w.qx, w.qxdy, fu));
+  SCHK(op1(0, 1, MIDP_C2P, B[1][VU], w.qx, w.qxiy, fu));
+  SCHK(op1(0, 2, MIDP_C2P, B[2][VU], w.qx, w.qxiz, nullptr));
+  SCHK(op1(0, 2, D1_C2P, B[2][VU], w.qx, w.qxdz, nullptr));
+  SCHK(op1(1, 0, D1_C2P, B[0][VV], w.qy, w.qydx, nullptr));
+  SCHK(op1(1, 0, MIDP_C2P, B[0][VV], w.qy, w.qyix, nullptr));
+  SCHK(op1(1, 1, D1_P2C, B[1][VV], w.qy, w.qydy, nullptr));
+  SCHK(op1(1, 1, MIDP_P2C, B[1][VV], w.qy, w.qyiy, fv));
+  SCHK(op1(1, 2, D1_C2P, B[2][VV], w.qy, w.qydz, nullptr));
+  SCHK(op1(1, 2, MIDP_C2P, B[2][VV], w.qy, w.qyiz, nullptr));
+  SCHK(op1(2, 0, D1_C2P, B[0][VW], w.qz, w.qzdx, nullptr));
+  SCHK(op1(2, 0, MIDP_C2P, B[0][VW], w.qz, w.qzix, nullptr));
+  SCHK(op1(2, 1, MIDP_C2P, B[1][VW], w.qz, w.qziy, fw));
+  SCHK(op1(2, 1, D1_C2P, B[1][VW], w.qz, w.qzdy, fw));
+  SCHK(op1(2, 2, MIDP_P2C, B[2][VW], w.qz, w.qziz, nullptr));
+  SCHK(op1(2, 2, D1_P2C, B[2][VW], w.qz, w.qzdz, nullptr));
+  SCHK(join());
   hipLaunchKernelGGL(k_div3<T>, grid1(w.div.size()), dim3(256), 0, s, w.div.p, w.qxdx.p, w.qydy.p, w.qzdz.p,
                      w.div.size());
   // ---- y-wall values of the divergence and of the y-normal stress ----
@@ -541,44 +589,48 @@ std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub) {
   const T third = T(-ONE_THIRD), two = T(2);
   const T* fconv = wall_y ? w.fby_conv : nullptr;
   const T* fvisc = wall_y ? w.fby_visc : nullptr;
-  // ---- x-momentum, :798-948 ----
+  // ---- second-level operators, one lane per momentum component; each
+  // lane ends with its fractional-step combine ----
+  SCHK(fork());
+  // x-momentum, :798-948
   Field<T>* X = w.acc[0];
   SCHK(cov_tau<T>(w, 0, D1_C2P, mcov[0][0], mtau[0][0], X[0], nullptr, nullptr, w.qxix, w.qxix, w.qxdx, two,
-                  &w.div, third));
+                  &w.div, third, 0));
   SCHK(cov_tau<T>(w, 1, D1_P2C, mcov[0][1], mtau[0][1], X[1], nullptr, nullptr, w.qyix, w.qxiy, w.qxdy, one,
-                  &w.qydx, one));
+                  &w.qydx, one, 0));
   SCHK(cov_tau<T>(w, 2, D1_P2C, mcov[0][2], mtau[0][2], X[2], nullptr, nullptr, w.qzix, w.qxiz, w.qxdz, one,
-                  &w.qzdx, one));
-  SCHK(op<T>(w, 0, D1_C2P, B[0][VP], w.pr, X[3], nullptr, m1));
-  // ---- y-momentum, :960-1235 ----
+                  &w.qzdx, one, 0));
+  SCHK(op1(0, 0, D1_C2P, B[0][VP], w.pr, X[3], nullptr, m1));
+  // y-momentum, :960-1235
   Field<T>* Y = w.acc[1];
   SCHK(cov_tau<T>(w, 0, D1_P2C, mcov[1][0], mtau[1][0], Y[0], nullptr, nullptr, w.qxiy, w.qyix, w.qxdy, one,
-                  &w.qydx, one));
+                  &w.qydx, one, 1));
   SCHK(cov_tau<T>(w, 1, D1_C2P, mcov[1][1], mtau[1][1], Y[1], fconv, fvisc, w.qyiy, w.qyiy, w.qydy, two, &w.div,
-                  third));
+                  third, 1));
   SCHK(cov_tau<T>(w, 2, D1_P2C, mcov[1][2], mtau[1][2], Y[2], nullptr, nullptr, w.qyiz, w.qziy, w.qzdy, one,
-                  &w.qydz, one));
-  SCHK(op<T>(w, 1, D1_C2P, B[1][VP], w.pr, Y[3], wall_y ? w.fby_pn : nullptr, m1));
-  // ---- z-momentum, :1248-1480 ----
+                  &w.qydz, one, 1));
+  SCHK(op1(1, 1, D1_C2P, B[1][VP], w.pr, Y[3], wall_y ? w.fby_pn : nullptr, m1));
+  // z-momentum, :1248-1480
   Field<T>* Z = w.acc[2];
   SCHK(cov_tau<T>(w, 0, D1_P2C, mcov[2][0], mtau[2][0], Z[0], nullptr, nullptr, w.qxiz, w.qzix, w.qxdz, one,
-                  &w.qzdx, one));
+                  &w.qzdx, one, 2));
   SCHK(cov_tau<T>(w, 1, D1_P2C, mcov[2][1], mtau[2][1], Z[1], nullptr, nullptr, w.qyiz, w.qziy, w.qzdy, one,
-                  &w.qydz, one));
+                  &w.qydz, one, 2));
   SCHK(cov_tau<T>(w, 2, D1_C2P, mcov[2][2], mtau[2][2], Z[2], nullptr, nullptr, w.qziz, w.qziz, w.qzdz, two,
-                  &w.div, third));
-  SCHK(op<T>(w, 2, D1_C2P, B[2][VP], w.pr, Z[3], nullptr, m1));
-  // ---- fractional step and forcing ----
+                  &w.div, third, 2));
+  SCHK(op1(2, 2, D1_C2P, B[2][VP], w.pr, Z[3], nullptr, m1));
+  // fractional step (and the forcing's block partials of component x)
   const Mesh& m = H->mesh;
   const bool forced = (H->desc.idriven == 1);
   Field<T>* rhs0[3] = {&w.mx0, &w.my0, &w.mz0};
   for (int c = 0; c < 3; ++c) {
     long n = w.acc[c][0].size();
     double* part = (c == 0 && forced) ? partial : nullptr;
-    hipLaunchKernelGGL(k_combine<T>, grid1(n), dim3(256), 0, s, w.acc[c][0].p, w.acc[c][1].p, w.acc[c][2].p,
-                       w.acc[c][3].p, rhs0[c]->p, n, T(tg[isub]), T(tz[isub]), T(ta[isub]), T(dt),
+    hipLaunchKernelGGL(k_combine<T>, grid1(n), dim3(256), 0, lane_stream(c), w.acc[c][0].p, w.acc[c][1].p,
+                       w.acc[c][2].p, w.acc[c][3].p, rhs0[c]->p, n, T(tg[isub]), T(tz[isub]), T(ta[isub]), T(dt),
                        w.acc[c][0].e.n[0], w.acc[c][0].e.n[1], dyw_dev, m.h[0], m.h[2], part);
   }
+  SCHK(join());
   if (forced) {
     int nb = (int)((w.acc[0][0].size() + 255) / 256);
     hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, s, partial, nb, vol, bulk);
@@ -589,6 +641,7 @@ std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub) {
 }
 
 std::string Flow::momentum_rhs(int isub) {
+  set_lanes();
   if (!mixed) return momentum_rhs_t<double>(w64, isub);
   // rounded copies of the state for the FP32 right-hand side
   hipStream_t s = H->stream;
@@ -601,15 +654,15 @@ std::string Flow::momentum_rhs(int isub) {
 
 template <typename T>
 std::string Flow::update_state(RhsWork<T>& w) {
-  hipStream_t s = H->stream;
   const bool forced = (H->desc.idriven == 1);
-  hipLaunchKernelGGL(k_update<T>, grid1(qx.size()), dim3(256), 0, s, qx.p, w.acc[0][0].p, forced ? bulk : nullptr,
-                     qx.size(), px, ny, (const double*)nullptr);
-  hipLaunchKernelGGL(k_update<T>, grid1(qy.size()), dim3(256), 0, s, qy.p, w.acc[1][0].p, (const double*)nullptr,
-                     qy.size(), nx, py, wall_y ? w64.fby_v : nullptr);
-  hipLaunchKernelGGL(k_update<T>, grid1(qz.size()), dim3(256), 0, s, qz.p, w.acc[2][0].p, (const double*)nullptr,
-                     qz.size(), nx, ny, (const double*)nullptr);
-  return "";
+  SCHK(fork());
+  hipLaunchKernelGGL(k_update<T>, grid1(qx.size()), dim3(256), 0, lane_stream(0), qx.p, w.acc[0][0].p,
+                     forced ? bulk : nullptr, qx.size(), px, ny, (const double*)nullptr);
+  hipLaunchKernelGGL(k_update<T>, grid1(qy.size()), dim3(256), 0, lane_stream(1), qy.p, w.acc[1][0].p,
+                     (const double*)nullptr, qy.size(), nx, py, wall_y ? w64.fby_v : nullptr);
+  hipLaunchKernelGGL(k_update<T>, grid1(qz.size()), dim3(256), 0, lane_stream(2), qz.p, w.acc[2][0].p,
+                     (const double*)nullptr, qz.size(), nx, ny, (const double*)nullptr);
+  return join();
 }
 
 std::string Flow::divergence(double* d) {
@@ -631,16 +684,24 @@ std::string Flow::substep(int isub) {
   if (H->desc.projection == CHS_PROJ_EXACT) SCHK(H->poisson->solve_exact(pcor.p, s, coeff));
   else SCHK(H->poisson->solve(pcor.p, s, coeff));
   hipLaunchKernelGGL(k_add, grid1(pr.size()), dim3(256), 0, s, pr.p, pcor.p, pr.size());
-  // Correct_massflux
+  // Correct_massflux, one lane per velocity component
   const double sc = dt * ta[isub] * sigma2p;
-  SCHK(op<double>(w64, 0, D1_C2P, ibc[0][VP], pcor, qx, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1));
+  SCHK(fork());
+  SCHK(op<double>(w64, 0, D1_C2P, ibc[0][VP], pcor, qx, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1, nullptr, 0.0,
+                  nullptr, 0.0, 0));
   SCHK(op<double>(w64, 1, D1_C2P, ibc[1][VP], pcor, qy, wall_y ? w64.fby_p : nullptr, 1.0, nullptr, 1.0, nullptr,
-                  -sc, 1));
-  SCHK(op<double>(w64, 2, D1_C2P, ibc[2][VP], pcor, qz, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1));
+                  -sc, 1, nullptr, 0.0, nullptr, 0.0, 1));
+  SCHK(op<double>(w64, 2, D1_C2P, ibc[2][VP], pcor, qz, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1, nullptr, 0.0,
+                  nullptr, 0.0, 2));
   if (wall_y)
-    hipLaunchKernelGGL(k_enforce_walls, grid1((long)nx * nz), dim3(256), 0, s, qy.p, nx, py, nz, w64.fby_v);
+    hipLaunchKernelGGL(k_enforce_walls, grid1((long)nx * nz), dim3(256), 0, lane_stream(1), qy.p, nx, py, nz,
+                       w64.fby_v);
+  SCHK(join());
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("substep: ") + hipGetErrorString(e);
+  // every operator of a sub-step is planned and tuned now: later sub-steps
+  // may overlap independent operators
+  concurrent = (side[0] != nullptr);
   return "";
 }
 

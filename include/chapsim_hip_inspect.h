@@ -38,7 +38,10 @@ int chs_inspect_projection(const chs_desc_t *desc, double *ly, double *lam, doub
 /* Launch timing of one kernel class during subsequent solver and chs_op3d
  * calls, with HIP events on the handle's stream: class 0/1/2 = x/y/z line
  * operators, 3 = whole chs_poisson / chs_poisson_f32 solves (counted as five
- * passes that each read and write the field).  chs_profile_end returns the summed
+ * passes that each read and write the field).  While a class is profiled,
+ * sub-steps issue every kernel on the handle's stream, one at a time (they
+ * otherwise overlap independent operators on side streams), so each timed
+ * launch has the GPU to itself.  chs_profile_end returns the summed
  * event time, the number of timed launches and their algorithmic bytes
  * (arrays read plus arrays written, 8 bytes per FP64 and 4 per FP32
  * element). */
