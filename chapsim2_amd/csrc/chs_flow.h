@@ -104,6 +104,12 @@ class Flow {
                  const T* fbc, T sA = T(1), const Field<T>* B = nullptr, T cB = T(1), const Field<T>* Cm = nullptr,
                  T sOut = T(1), int accumulate = 0, const Field<T>* D = nullptr, T cD = T(0),
                  const Field<T>* E = nullptr, T cE = T(0), int lane = 0);
+  // the interpolation and the first derivative of A between the same
+  // locations (plans opa with bca / fbca and opb with bcb / fbcb) in one
+  // pass where the line kernels allow it, else one after the other
+  template <typename T>
+  std::string op2(RhsWork<T>& w, int lane, int dir, const Field<T>& A, int opa, const int bca[2], Field<T>& outa,
+                  const T* fbca, int opb, const int bcb[2], Field<T>& outb, const T* fbcb);
   // out = d(-ca * cb) + r d(ts * (ta + tc * tb)): a convective and a viscous
   // operator of one momentum component along one direction
   template <typename T>

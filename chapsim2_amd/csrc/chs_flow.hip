@@ -450,6 +450,40 @@ void Flow::set_lanes() {
              cs == hipStreamCaptureStatusNone;
 }
 
+template <typename T>
+std::string Flow::op2(RhsWork<T>& w, int lane, int dir, const Field<T>& A, int opa, const int bca[2], Field<T>& outa,
+                      const T* fbca, int opb, const int bcb[2], Field<T>& outb, const T* fbcb) {
+  const DevPlan<T>* PA = nullptr;
+  const DevPlan<T>* PB = nullptr;
+  SCHK(H->plan<T>(dir, opa, H->desc.iacc, bca, fbca != nullptr, A.e.n[dir], outa.e.n[dir], &PA));
+  SCHK(H->plan<T>(dir, opb, H->desc.iacc, bcb, fbcb != nullptr, A.e.n[dir], outb.e.n[dir], &PB));
+  bool same = true;
+  for (int k = 0; k < 3; ++k) same = same && outa.e.n[k] == outb.e.n[k];
+  const hipStream_t s = lane_stream(lane);
+  LineOut<T> oa, ob;
+  oa.out = outa.p;
+  ob.out = outb.p;
+  bool ok = false;
+  if (same) SCHK(launch_line_op2<T>(*PA, *PB, A.e, outa.e, A.p, fbca, fbcb, oa, ob, s, &ok, true));
+  if (!ok) {
+    auto one = [&](int opk, const int* bc, Field<T>& out, const T* fbc) {
+      return op<T>(w, dir, opk, bc, A, out, fbc, T(1), nullptr, T(1), nullptr, T(1), 0, nullptr, T(0), nullptr,
+                   T(0), lane);
+    };
+    SCHK(one(opa, bca, outa, fbca));
+    return one(opb, bcb, outb, fbcb);
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const bool timed = (H->prof_class == dir) && H->prof_pair(&e0, &e1);
+  if (timed) {
+    (void)hipEventRecord(e0, s);
+    H->prof_bytes += (double)sizeof(T) * ((double)A.size() + (double)outa.size() + (double)outb.size());
+  }
+  std::string r = launch_line_op2<T>(*PA, *PB, A.e, outa.e, A.p, fbca, fbcb, oa, ob, s, &ok);
+  if (timed) (void)hipEventRecord(e1, s);
+  return r;
+}
+
 std::string Flow::fork() {
   if (!lanes_on) return "";
   if (hipEventRecord(ev_fork, H->stream) != hipSuccess) return "flow: fork record";
@@ -543,26 +577,18 @@ std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub) {
                  lane);
   };
   // ---- first-level operators, src/eq_momentum2.f90:347-506 (one lane per
-  // velocity component) ----
+  // velocity component; the interpolation and the derivative of a component
+  // along one direction share one pass over it) ----
   SCHK(fork());
-  SCHK(op1(0, 0, MIDP_P2C, B[0][VU], w.qx, w.qxix, nullptr));
-  SCHK(op1(0, 0, D1_P2C, B[0][VU], w.qx, w.qxdx, nullptr));
-  SCHK(op1(0, 1, D1_C2P, B[1][VU], w.qx, w.qxdy, fu));
-  SCHK(op1(0, 1, MIDP_C2P, B[1][VU], w.qx, w.qxiy, fu));
-  SCHK(op1(0, 2, MIDP_C2P, B[2][VU], w.qx, w.qxiz, nullptr));
-  SCHK(op1(0, 2, D1_C2P, B[2][VU], w.qx, w.qxdz, nullptr));
-  SCHK(op1(1, 0, D1_C2P, B[0][VV], w.qy, w.qydx, nullptr));
-  SCHK(op1(1, 0, MIDP_C2P, B[0][VV], w.qy, w.qyix, nullptr));
-  SCHK(op1(1, 1, D1_P2C, B[1][VV], w.qy, w.qydy, nullptr));
-  SCHK(op1(1, 1, MIDP_P2C, B[1][VV], w.qy, w.qyiy, fv));
-  SCHK(op1(1, 2, D1_C2P, B[2][VV], w.qy, w.qydz, nullptr));
-  SCHK(op1(1, 2, MIDP_C2P, B[2][VV], w.qy, w.qyiz, nullptr));
-  SCHK(op1(2, 0, D1_C2P, B[0][VW], w.qz, w.qzdx, nullptr));
-  SCHK(op1(2, 0, MIDP_C2P, B[0][VW], w.qz, w.qzix, nullptr));
-  SCHK(op1(2, 1, MIDP_C2P, B[1][VW], w.qz, w.qziy, fw));
-  SCHK(op1(2, 1, D1_C2P, B[1][VW], w.qz, w.qzdy, fw));
-  SCHK(op1(2, 2, MIDP_P2C, B[2][VW], w.qz, w.qziz, nullptr));
-  SCHK(op1(2, 2, D1_P2C, B[2][VW], w.qz, w.qzdz, nullptr));
+  SCHK(op2<T>(w, 0, 0, w.qx, MIDP_P2C, B[0][VU], w.qxix, nullptr, D1_P2C, B[0][VU], w.qxdx, nullptr));
+  SCHK(op2<T>(w, 0, 1, w.qx, D1_C2P, B[1][VU], w.qxdy, fu, MIDP_C2P, B[1][VU], w.qxiy, fu));
+  SCHK(op2<T>(w, 0, 2, w.qx, MIDP_C2P, B[2][VU], w.qxiz, nullptr, D1_C2P, B[2][VU], w.qxdz, nullptr));
+  SCHK(op2<T>(w, 1, 0, w.qy, D1_C2P, B[0][VV], w.qydx, nullptr, MIDP_C2P, B[0][VV], w.qyix, nullptr));
+  SCHK(op2<T>(w, 1, 1, w.qy, D1_P2C, B[1][VV], w.qydy, nullptr, MIDP_P2C, B[1][VV], w.qyiy, fv));
+  SCHK(op2<T>(w, 1, 2, w.qy, D1_C2P, B[2][VV], w.qydz, nullptr, MIDP_C2P, B[2][VV], w.qyiz, nullptr));
+  SCHK(op2<T>(w, 2, 0, w.qz, D1_C2P, B[0][VW], w.qzdx, nullptr, MIDP_C2P, B[0][VW], w.qzix, nullptr));
+  SCHK(op2<T>(w, 2, 1, w.qz, MIDP_C2P, B[1][VW], w.qziy, fw, D1_C2P, B[1][VW], w.qzdy, fw));
+  SCHK(op2<T>(w, 2, 2, w.qz, MIDP_P2C, B[2][VW], w.qziz, nullptr, D1_P2C, B[2][VW], w.qzdz, nullptr));
   SCHK(join());
   hipLaunchKernelGGL(k_div3<T>, grid1(w.div.size()), dim3(256), 0, s, w.div.p, w.qxdx.p, w.qydy.p, w.qzdz.p,
                      w.div.size());

@@ -86,6 +86,18 @@ template <typename T>
 std::string launch_line_op(const DevPlan<T>& P, const Ext3& ein, const Ext3& eout, const LineIn<T>& in,
                            const T* fbc, const LineOut<T>& out, T* scratch, hipStream_t s);
 
+// Two operators of one line geometry on the same input in one pass: the
+// interpolation and the first derivative between the same locations
+// (MIDP_P2C with D1_P2C, or MIDP_C2P with D1_C2P; any order), whose plans
+// share the extents, chunk partition and boundary wrap, each with its own
+// wall values and output.  *done is false, with no error and nothing
+// launched, when the pair cannot share a pass; the caller then applies the
+// operators one by one.  check_only: only decide *done, launch nothing.
+template <typename T>
+std::string launch_line_op2(const DevPlan<T>& A, const DevPlan<T>& B, const Ext3& ein, const Ext3& eout,
+                            const T* in, const T* fbcA, const T* fbcB, const LineOut<T>& outA,
+                            const LineOut<T>& outB, hipStream_t s, bool* done, bool check_only = false);
+
 // Evaluate only the first and last output rows of a y-direction plan on every
 // (x, z) line, writing planes of extents (e1, 2, e3).  Valid when the plan's
 // end rows are decoupled from the line (explicit rows, or LHS rows (0,1,0)),

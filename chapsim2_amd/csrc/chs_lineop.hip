@@ -279,6 +279,50 @@ __device__ __forceinline__ void yz_offsets(const KArgs<T>& P, int L, long& ib, l
   }
 }
 
+// The rest of a y/z chunk after its RHS rows y[]: chunk-local solve, the
+// exchange of first/last values through ex (64 lanes x 2P) and the spike
+// correction (plans with a line solve), then metric scaling, Neumann rows and
+// the (accumulating) store of the chunk's rows at Ob + os * row + ro.
+template <typename T, int CH>
+__device__ __forceinline__ void yz_finish(const KArgs<T>& P, const T* c, const T* rinv, T* ex, int lane, int q,
+                                          int s0, int len, bool live, T* Ob, unsigned ro, long os, T (&y)[CH],
+                                          const T (&fb4)[4]) {
+  if (P.solve) {
+    T yf, yl;
+    chunk_solve(c, len, y, yf, yl);
+    ex[lane + 64 * (2 * q)] = yf;
+    ex[lane + 64 * (2 * q + 1)] = yl;
+    __syncthreads();
+    chunk_couple<T>(P, q, rinv, c, [&](int k) { return ex[lane + 64 * k]; }, y);
+  }
+  if (!live) return;
+  const int n_out = P.n_out;
+  // accumulating outputs read the previous values in two batches of half a
+  // chunk (fewer registers held across the loads)
+  constexpr int H = CH / 2;
+#pragma unroll
+  for (int h = 0; h < CH; h += H) {
+    T prev[H];
+#pragma unroll
+    for (int i = 0; i < H; ++i) prev[i] = T(0);
+    if (P.accumulate) {
+#pragma unroll
+      for (int i = 0; i < H; ++i) {
+        const int r = (s0 + h + i < n_out) ? s0 + h + i : n_out - 1;
+        prev[i] = ld_at<T>(Ob + os * r, ro);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      if (h + i < len) {
+        const int r = s0 + h + i;
+        const T val = chunk_post<T>(P, c, h + i, r, y[h + i], fb4);
+        st_at<T>(Ob + os * r, ro, prev[i] + val * P.sOut);
+      }
+    }
+  }
+}
+
 // One y/z workgroup: 64 neighbouring lines x P chunks of CH rows, wave q
 // holding chunk q of the 64 lines; IN as in load_window.  CH = kChunkLong
 // also serves x lines longer than a 64-line LDS tile (DIR = 0: the lanes
@@ -342,40 +386,78 @@ __global__ void __launch_bounds__(64 * kMaxChunks)
   const T* rinv = sco + Pn * CH * kCo;
   T y[CH];
   chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
-  if (P.solve) {
-    T yf, yl;
-    chunk_solve(c, len, y, yf, yl);
-    ex[lane + 64 * (2 * q)] = yf;
-    ex[lane + 64 * (2 * q + 1)] = yl;
-    __syncthreads();
-    chunk_couple<T>(P, q, rinv, c, [&](int k) { return ex[lane + 64 * k]; }, y);
+  yz_finish<T, CH>(P, c, rinv, ex, lane, q, s0, len, Lr < P.nlines, P.out + ob0, ro, os, y, fb4);
+}
+
+// Two operators of one line plan geometry (the interpolation and the first
+// derivative between the same locations: PAIR 0 = MIDP_P2C + D1_P2C, 1 =
+// MIDP_C2P + D1_C2P) applied to one input in one pass: the window is read
+// once and kept in registers, each operator then runs the RHS, solve and
+// store of k_line_yz with its own plan (P, then Q) and output.  The two
+// plans share the extents, the chunk partition and the boundary wrap.
+template <typename T, int DIR, int PAIR, int CH>
+__global__ void __launch_bounds__(64 * kMaxChunks) __attribute__((amdgpu_waves_per_eu(4)))
+    k_line_yz2(KArgs<T> P, KArgs<T> Q) {
+  constexpr int OPA = PAIR == 0 ? MIDP_P2C : MIDP_C2P;
+  constexpr int OPB = PAIR == 0 ? D1_P2C : D1_C2P;
+  __shared__ T ex[2 * kMaxChunks * 64];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  // P's chunk rows and inverse, then Q's
+  T* scoP = reinterpret_cast<T*>(smem_raw);
+  const int Pn = P.P;
+  const int nco = Pn * CH * kCo;
+  const int nstP = nco + (P.rinv ? 4 * Pn * Pn : 0);
+  const int nstQ = nco + (Q.rinv ? 4 * Pn * Pn : 0);
+  T* scoQ = scoP + nstP;
+  constexpr int NS = stage_max(CH);
+  T cvP[NS], cvQ[NS];
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const int idx = threadIdx.x + u * 64 * Pn;
+    cvP[u] = (idx < nco) ? P.co[idx] : (idx < nstP ? P.rinv[idx - nco] : T(0));
+    cvQ[u] = (idx < nco) ? Q.co[idx] : (idx < nstQ ? Q.rinv[idx - nco] : T(0));
   }
-  if (!live) return;
-  T* const Ob = P.out + ob0;
-  // accumulating outputs read the previous values in two batches of half a
-  // chunk (fewer registers held across the loads)
-  constexpr int H = CH / 2;
+  const int lane = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int L0 = blockIdx.x * 64;
+  const int Lr = L0 + lane;
+  const bool live = Lr < P.nlines;
+  const int L = live ? Lr : (int)P.nlines - 1;
+  long ib, ob, fb, ib0, ob0, fb0;
+  yz_offsets<DIR>(P, L, ib, ob, fb);
+  yz_offsets<DIR>(P, L0, ib0, ob0, fb0);
+  const unsigned ri = (unsigned)((ib - ib0) * (long)sizeof(T));
+  const unsigned ro = (unsigned)((ob - ob0) * (long)sizeof(T));
+  const long is = (DIR == 1) ? (long)P.e1 : (long)P.e1 * P.e2;
+  const long os = (DIR == 1) ? (long)P.o1 : (long)P.o1 * P.o2;
+  const int n_out = P.n_out;
+  const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
+  auto stage = [&]() {
 #pragma unroll
-  for (int h = 0; h < CH; h += H) {
-    T prev[H];
-#pragma unroll
-    for (int i = 0; i < H; ++i) prev[i] = T(0);
-    if (P.accumulate) {
-#pragma unroll
-      for (int i = 0; i < H; ++i) {
-        const int r = (s0 + h + i < n_out) ? s0 + h + i : n_out - 1;
-        prev[i] = ld_at<T>(Ob + os * r, ro);
-      }
+    for (int u = 0; u < NS; ++u) {
+      const int idx = threadIdx.x + u * 64 * Pn;
+      if (idx < nstP) scoP[idx] = cvP[u];
+      if (idx < nstQ) scoQ[idx] = cvQ[u];
     }
-#pragma unroll
-    for (int i = 0; i < H; ++i) {
-      if (h + i < len) {
-        const int r = s0 + h + i;
-        const T val = chunk_post<T>(P, c, h + i, r, y[h + i], fb4);
-        st_at<T>(Ob + os * r, ro, prev[i] + val * P.sOut);
-      }
-    }
+  };
+  if constexpr (CH > kChunk) stage();
+  T f[CH + 4];
+  load_window<T, 0, CH>(P, ib0, ri, is, s0, f);
+  T fbP[4] = {T(0), T(0), T(0), T(0)}, fbQ[4] = {T(0), T(0), T(0), T(0)};
+  if (q == 0 || q == Pn - 1) {
+    if (P.fbc)
+      for (int m = 0; m < 4; ++m) fbP[m] = P.fbc[fb + is * m];
+    if (Q.fbc)
+      for (int m = 0; m < 4; ++m) fbQ[m] = Q.fbc[fb + is * m];
   }
+  if constexpr (CH <= kChunk) stage();
+  __syncthreads();
+  T y[CH];
+  chunk_rhs<T, OPA>(P, q, len, f, fbP, y);
+  yz_finish<T, CH>(P, scoP + q * CH * kCo, scoP + nco, ex, lane, q, s0, len, live, P.out + ob0, ro, os, y, fbP);
+  __syncthreads();  // every wave has read the first operator's exchange
+  chunk_rhs<T, OPB>(Q, q, len, f, fbQ, y);
+  yz_finish<T, CH>(Q, scoQ + q * CH * kCo, scoQ + nco, ex, lane, q, s0, len, live, Q.out + ob0, ro, os, y, fbQ);
 }
 
 
@@ -444,16 +526,19 @@ static void dispatch_op(int op, dim3 g, dim3 b, hipStream_t s, const KArgs<T>& a
   }
 }
 
+// Kernel arguments of plan D on arrays of extents ein -> eout (common part
+// of every line kernel; the partitioned fields are set by the caller).
 template <typename T>
-std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eout, const LineIn<T>& in, const T* fbc,
-                           const LineOut<T>& out, T* scratch, hipStream_t s) {
+static std::string line_args(const DevPlan<T>& D, const Ext3& ein, const Ext3& eout, const LineIn<T>& in,
+                             const T* fbc, const LineOut<T>& out, T* scratch, KArgs<T>* ap) {
   const LinePlan& p = D.host;
   const int dir = p.dir;
   if (ein.n[dir] != p.n_in || eout.n[dir] != p.n_out) return "line op: extent along the line differs from plan";
   for (int k = 0; k < 3; ++k)
     if (k != dir && ein.n[k] != eout.n[k]) return "line op: cross extents of input and output differ";
   if (p.has_fbc != (fbc != nullptr)) return "line op: fbc presence differs from plan";
-  KArgs<T> a{};
+  KArgs<T>& a = *ap;
+  a = KArgs<T>{};
   a.n_in = p.n_in; a.n_out = p.n_out;
   a.e1 = ein.n[0]; a.e2 = ein.n[1];
   a.o1 = eout.n[0]; a.o2 = eout.n[1];
@@ -469,18 +554,45 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
   a.metric = D.metric; a.neu_lo = p.neu_lo; a.neu_hi = p.neu_hi;
   a.out = out.out; a.sOut = out.sOut; a.accumulate = out.accumulate;
   a.scratch = scratch;
-  if (a.nlines <= 0) return "";
   if (a.nlines >= (1L << 31) - 64) return "line op: too many lines";
   if ((long)ein.n[0] * ein.n[1] * ein.n[2] >= (1L << 31) || (long)eout.n[0] * eout.n[1] * eout.n[2] >= (1L << 31))
     return "line op: arrays of 2^31 or more elements";
-  if (p.part_P <= kMaxChunks && p.n_out >= 4 && D.win_ok && D.co) {
-    a.P = p.part_P;
-    a.per = (p.solve == 2);
-    a.rinv = D.rinv;
+  return "";
+}
+
+// The plan runs on the partitioned kernels.
+template <typename T>
+static bool partitioned(const DevPlan<T>& D) {
+  const LinePlan& p = D.host;
+  return p.part_P <= kMaxChunks && p.n_out >= 4 && D.win_ok && D.co;
+}
+
+// Partitioned-kernel fields of the arguments of plan D.
+template <typename T>
+static void part_args(const DevPlan<T>& D, KArgs<T>* a) {
+  const LinePlan& p = D.host;
+  a->P = p.part_P;
+  a->per = (p.solve == 2);
+  a->rinv = D.rinv;
+  a->swin = D.swin;
+  a->wrap = (p.ibc[0] == BC_PERIODIC);
+  a->co = D.co;
+}
+
+template <typename T>
+std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eout, const LineIn<T>& in, const T* fbc,
+                           const LineOut<T>& out, T* scratch, hipStream_t s) {
+  const LinePlan& p = D.host;
+  const int dir = p.dir;
+  KArgs<T> a;
+  {
+    const std::string e = line_args(D, ein, eout, in, fbc, out, scratch, &a);
+    if (!e.empty()) return e;
+  }
+  if (a.nlines <= 0) return "";
+  if (partitioned(D)) {
+    part_args(D, &a);
     if (p.solve && !D.rinv) return "line op: plan solves but has no reduced-system inverse";
-    a.swin = D.swin;
-    a.wrap = (p.ibc[0] == BC_PERIODIC);
-    a.co = D.co;
     const int nmax = p.n_in > p.n_out ? p.n_in : p.n_out;
     const int CHp = p.part_ch;
     const size_t P2 = 4 * (size_t)p.part_P * p.part_P;  // reduced-system inverse
@@ -573,6 +685,93 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
   return "";
 }
 
+// paired y/z kernels: chunks of kChunk rows, and of kChunkLong rows in FP32
+// (an FP64 pair of long chunks does not fit the registers; launch_line_op2
+// applies such pairs one by one)
+template <typename T, int DIR>
+static void dispatch_yz2(int pair, int ch, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a,
+                         const KArgs<T>& a2) {
+  if (ch == kChunk) {
+    if (pair == 0) hipLaunchKernelGGL((k_line_yz2<T, DIR, 0, kChunk>), g, b, shm, s, a, a2);
+    else hipLaunchKernelGGL((k_line_yz2<T, DIR, 1, kChunk>), g, b, shm, s, a, a2);
+  } else if constexpr (sizeof(T) == 4) {
+    if (pair == 0) hipLaunchKernelGGL((k_line_yz2<T, DIR, 0, kChunkLong>), g, b, shm, s, a, a2);
+    else hipLaunchKernelGGL((k_line_yz2<T, DIR, 1, kChunkLong>), g, b, shm, s, a, a2);
+  }
+}
+
+template <typename T>
+std::string launch_line_op2(const DevPlan<T>& DA, const DevPlan<T>& DB, const Ext3& ein, const Ext3& eout,
+                            const T* in, const T* fbcA, const T* fbcB, const LineOut<T>& outA,
+                            const LineOut<T>& outB, hipStream_t s, bool* done, bool check_only) {
+  *done = false;
+  // the interpolation goes first
+  const bool swap = (DA.host.op == D1_P2C || DA.host.op == D1_C2P);
+  const DevPlan<T>& M = swap ? DB : DA;
+  const DevPlan<T>& G = swap ? DA : DB;
+  const T* fM = swap ? fbcB : fbcA;
+  const T* fG = swap ? fbcA : fbcB;
+  const LineOut<T>& oM = swap ? outB : outA;
+  const LineOut<T>& oG = swap ? outA : outB;
+  const LinePlan &pm = M.host, &pg = G.host;
+  const int pair = (pm.op == MIDP_P2C && pg.op == D1_P2C) ? 0 : (pm.op == MIDP_C2P && pg.op == D1_C2P) ? 1 : -1;
+  if (pair < 0 || pm.dir != pg.dir || pm.n_in != pg.n_in || pm.n_out != pg.n_out) return "";
+  if (!partitioned(M) || !partitioned(G) || pm.part_P != pg.part_P || pm.part_ch != pg.part_ch) return "";
+  if ((pm.ibc[0] == BC_PERIODIC) != (pg.ibc[0] == BC_PERIODIC)) return "";
+  if ((pm.solve && !M.rinv) || (pg.solve && !G.rinv)) return "";
+  LineIn<T> li;
+  li.A = in;
+  KArgs<T> a, b;
+  std::string e = line_args(M, ein, eout, li, fM, oM, (T*)nullptr, &a);
+  if (e.empty()) e = line_args(G, ein, eout, li, fG, oG, (T*)nullptr, &b);
+  if (!e.empty()) return e;
+  if (a.nlines <= 0) {
+    *done = true;
+    return "";
+  }
+  if (sizeof(T) == 8 && pm.part_ch != kChunk) return "";
+  if (pm.dir == 0 && (pm.part_ch != kChunk ||
+                      sizeof(T) * ((size_t)64 * ((pm.n_in > pm.n_out ? pm.n_in : pm.n_out) | 1) +
+                                   2 * ((size_t)pm.part_P * pm.part_ch * kCo + 4 * (size_t)pm.part_P * pm.part_P)) >
+                          max_lds_per_block()))
+    return "";
+  if (pm.dir == 1 && (long)ein.n[0] * (ein.n[1] > eout.n[1] ? ein.n[1] : eout.n[1]) * (64 / ein.n[0] + 2) *
+                             (long)sizeof(T) >= (1L << 32))
+    return "";
+  if (check_only) {
+    *done = true;
+    return "";
+  }
+  part_args(M, &a);
+  part_args(G, &b);
+  const int dir = pm.dir, CHp = pm.part_ch, Pn = pm.part_P;
+  const int nmax = pm.n_in > pm.n_out ? pm.n_in : pm.n_out;
+  const size_t P2 = 4 * (size_t)Pn * Pn, cset = (size_t)Pn * CHp * kCo;
+  if (dir == 0) {
+    // 64 whole lines in LDS with both plans' coefficients, or not at all
+    // (the two operators then run one by one, each on its tuned shape)
+    const int lsu = nmax | 1;
+    const size_t shm = sizeof(T) * ((size_t)64 * lsu + 2 * (cset + P2));
+    if (CHp != kChunk || shm > max_lds_per_block()) return "";
+    a.ls = b.ls = lsu;
+    dim3 g((unsigned)((a.nlines + 63) / 64)), bb((unsigned)(64 * Pn));
+    if (!launch_x_tile2<T>(pair, CHp, 64, g, bb, shm, s, a, b)) return "";
+  } else {
+    const long span = (dir == 1) ? (long)ein.n[0] * (ein.n[1] > eout.n[1] ? ein.n[1] : eout.n[1]) *
+                                       (64 / ein.n[0] + 2)
+                                 : 64L;
+    if (span * (long)sizeof(T) >= (1L << 32)) return "";
+    dim3 g((unsigned)((a.nlines + 63) / 64)), bb((unsigned)(64 * Pn));
+    const size_t shm = sizeof(T) * 2 * (cset + P2);
+    if (dir == 1) dispatch_yz2<T, 1>(pair, CHp, g, bb, shm, s, a, b);
+    else dispatch_yz2<T, 2>(pair, CHp, g, bb, shm, s, a, b);
+  }
+  hipError_t he = hipGetLastError();
+  if (he != hipSuccess) return std::string("line op launch: ") + hipGetErrorString(he);
+  *done = true;
+  return "";
+}
+
 // First and last output rows of a y-direction plan on every (x, z) line.
 template <typename T>
 __global__ void __launch_bounds__(64) k_wall_rows(KArgs<T> P, int e3, T* __restrict__ planes) {
@@ -637,6 +836,12 @@ template struct DevPlan<double>;
 template struct DevPlan<float>;
 template std::string upload_plan<double>(const LinePlan&, DevPlan<double>*);
 template std::string upload_plan<float>(const LinePlan&, DevPlan<float>*);
+template std::string launch_line_op2<double>(const DevPlan<double>&, const DevPlan<double>&, const Ext3&,
+                                             const Ext3&, const double*, const double*, const double*,
+                                             const LineOut<double>&, const LineOut<double>&, hipStream_t, bool*, bool);
+template std::string launch_line_op2<float>(const DevPlan<float>&, const DevPlan<float>&, const Ext3&, const Ext3&,
+                                            const float*, const float*, const float*, const LineOut<float>&,
+                                            const LineOut<float>&, hipStream_t, bool*, bool);
 template std::string launch_line_op<double>(const DevPlan<double>&, const Ext3&, const Ext3&, const LineIn<double>&,
                                             const double*, const LineOut<double>&, double*, hipStream_t);
 template std::string launch_wall_rows<double>(const DevPlan<double>&, const Ext3&, const double*, const double*,

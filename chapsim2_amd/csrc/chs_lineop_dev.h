@@ -300,6 +300,12 @@ inline int input_kind(const KArgs<T>& a) { return a.D ? 2 : ((a.B || a.Cm) ? 1 :
 // there is no such kernel: 64 FP64 lines of long chunks):
 template <typename T>
 bool launch_x_tile(int op, int ch, int nlw, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a);
+// Two operators of one plan geometry on one input (k_line_xu2; pair 0 =
+// MIDP_P2C + D1_P2C, 1 = MIDP_C2P + D1_C2P, plans a and a2), whole lines in
+// LDS as above (nlw 64 or 32; chunks of kChunk rows only):
+template <typename T>
+bool launch_x_tile2(int pair, int ch, int nlw, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a,
+                    const KArgs<T>& a2);
 // 256/P lines per workgroup with skewed LDS rows, chunks of CH rows:
 template <typename T>
 void launch_x_skew(int op, int ch, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a);

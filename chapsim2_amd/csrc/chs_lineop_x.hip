@@ -76,6 +76,62 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
   }
 }
 
+// The rest of a staged x chunk after its RHS rows y[]: chunk-local solve,
+// exchange of first/last values through the start of the tile (NLW x 2P,
+// once every window has been read), spike correction, then the chunk's rows
+// back into the tile and the NLW lines out with coalesced (accumulating)
+// stores.
+template <typename T, int CH, int NLW>
+__device__ __forceinline__ void xu_finish(const KArgs<T>& P, const T* c, const T* rinv, T* tile, int lane, int q,
+                                          int s0, int len, long L0, int lmax, T (&y)[CH], const T (&fb4)[4]) {
+  constexpr int UB = 8;
+  const int LS = P.ls, Pn = P.P, n_out = P.n_out;
+  const int tid = threadIdx.x, nthr = NLW * Pn;
+  T* exb = tile;
+  T* row = tile + lane * LS;
+  if (P.solve) {
+    T yf, yl;
+    chunk_solve(c, len, y, yf, yl);
+    __syncthreads();  // every wave has read its window: the tile's start takes the exchange
+    exb[lane + NLW * (2 * q)] = yf;
+    exb[lane + NLW * (2 * q + 1)] = yl;
+    __syncthreads();
+    chunk_couple<T>(P, q, rinv, c, [&](int k) { return exb[lane + NLW * k]; }, y);
+  }
+  __syncthreads();  // every wave has finished reading its window and the exchange
+#pragma unroll
+  for (int i = 0; i < CH; ++i)
+    if (i < len) row[s0 + i] = chunk_post<T>(P, c, i, s0 + i, y[i], fb4);
+  __syncthreads();
+  const int dl = nthr / n_out, dk = nthr - dl * n_out;
+  int l = tid / n_out, k = tid - l * n_out;
+  const int tout = NLW * n_out;
+  T* const Ob = P.out + L0 * P.o1;
+  for (int b0 = 0; b0 < tout; b0 += nthr * UB) {
+    unsigned o[UB];
+    bool ok[UB];
+    T val[UB], prev[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      ok[u] = (l <= lmax);
+      const int lc = l < lmax ? l : lmax;
+      o[u] = (unsigned)((lc * P.o1 + k) * (int)sizeof(T));
+      val[u] = tile[lc * LS + k];
+      prev[u] = T(0);
+      k += dk;
+      l += dl;
+      if (k >= n_out) { k -= n_out; ++l; }
+    }
+    if (P.accumulate) {
+#pragma unroll
+      for (int u = 0; u < UB; ++u) prev[u] = ld_at<T>(Ob, o[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u)
+      if (ok[u]) st_at<T>(Ob, o[u], prev[u] + val[u] * P.sOut);
+  }
+}
+
 // NLW whole lines per workgroup of NLW x P threads in LDS (row stride
 // ls = n | 1), thread tid on line tid % NLW and chunk tid / NLW.  NLW = 64:
 // one chunk per wave, the chunk coefficient rows staged in LDS after the
@@ -189,49 +245,102 @@ __global__ void __launch_bounds__(NLW * kMaxChunks) k_line_xu(KArgs<T> P) {
     for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
   T y[CH];
   chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
-  if (P.solve) {
-    T yf, yl;
-    chunk_solve(c, len, y, yf, yl);
-    __syncthreads();  // every wave has read its window: the tile's start takes the exchange
-    exb[lane + NLW * (2 * q)] = yf;
-    exb[lane + NLW * (2 * q + 1)] = yl;
-    __syncthreads();
-    chunk_couple<T>(P, q, rinv, c, [&](int k) { return exb[lane + NLW * k]; }, y);
-  }
-  __syncthreads();  // every wave has finished reading its window and the exchange
+  xu_finish<T, CH, NLW>(P, c, rinv, tile, lane, q, s0, len, L0, lmax, y, fb4);
+}
+
+// Two operators of one line plan geometry on one input (PAIR 0 = MIDP_P2C +
+// D1_P2C, 1 = MIDP_C2P + D1_C2P; plans P and Q with the same extents, chunk
+// partition and boundary wrap) in one pass over x lines staged as in
+// k_line_xu: the tile is loaded once, every thread keeps its window in
+// registers, and each operator then runs the RHS, solve and store with its
+// own plan and output (the tile is reused for the first operator's results
+// once every window is read).  Per-plan values after the tile: NLW = 64 both
+// plans' chunk rows and inverses, NLW = 32 only the inverses.
+template <typename T, int PAIR, int CH, int NLW>
+__global__ void __launch_bounds__(NLW * kMaxChunks) k_line_xu2(KArgs<T> P, KArgs<T> Q) {
+  constexpr int OPA = PAIR == 0 ? MIDP_P2C : MIDP_C2P;
+  constexpr int OPB = PAIR == 0 ? D1_P2C : D1_C2P;
+  constexpr bool kCoLds = (NLW == 64);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* tile = reinterpret_cast<T*>(smem_raw);
+  const int LS = P.ls, Pn = P.P;
+  T* scoP = tile + (long)NLW * LS;
+  const int nco = kCoLds ? Pn * CH * kCo : 0;
+  const int nstP = nco + (P.rinv ? 4 * Pn * Pn : 0);
+  const int nstQ = nco + (Q.rinv ? 4 * Pn * Pn : 0);
+  T* scoQ = scoP + nstP;
+  constexpr int NS = kCoLds ? stage_max(CH) : (4 * kMaxChunks + NLW - 1) / NLW;
+  static_assert(NLW * kMaxChunks * NS >= (kCoLds ? kMaxChunks * CH * kCo : 0) + 4 * kMaxChunks * kMaxChunks,
+                "per-plan staging exceeds NS values per thread");
+  const int tid = threadIdx.x, nthr = NLW * Pn;
+  T cvP[NS], cvQ[NS];
 #pragma unroll
-  for (int i = 0; i < CH; ++i)
-    if (i < len) row[s0 + i] = chunk_post<T>(P, c, i, s0 + i, y[i], fb4);
-  __syncthreads();
+  for (int u = 0; u < NS; ++u) {
+    const int idx = tid + u * nthr;
+    cvP[u] = (idx < nco) ? P.co[idx] : (idx < nstP ? P.rinv[idx - nco] : T(0));
+    cvQ[u] = (idx < nco) ? Q.co[idx] : (idx < nstQ ? Q.rinv[idx - nco] : T(0));
+  }
+  const int lane = tid & (NLW - 1);
+  const int q = kCoLds ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid / NLW;
+  const long L0 = (long)blockIdx.x * NLW;
+  const int n_in = P.n_in, n_out = P.n_out;
+  constexpr int UB = 8;
+  const long Lmax = P.nlines - 1;
+  const int lmax = (int)((Lmax - L0) < NLW - 1 ? (Lmax - L0) : NLW - 1);
+  const T* const Ab = P.A + L0 * P.e1;
   {
-    const int dl = nthr / n_out, dk = nthr - dl * n_out;
-    int l = tid / n_out, k = tid - l * n_out;
-    const int tout = NLW * n_out;
-    T* const Ob = P.out + L0 * P.o1;
-    for (int b0 = 0; b0 < tout; b0 += nthr * UB) {
+    const int dl = nthr / n_in, dk = nthr - dl * n_in;
+    int l = tid / n_in, k = tid - l * n_in;
+    const int tin = NLW * n_in;
+    for (int b0 = 0; b0 < tin; b0 += nthr * UB) {
       unsigned o[UB];
-      bool ok[UB];
-      T val[UB], prev[UB];
+      int ll[UB], kk[UB];
 #pragma unroll
       for (int u = 0; u < UB; ++u) {
-        ok[u] = (l <= lmax);
-        const int lc = l < lmax ? l : lmax;
-        o[u] = (unsigned)((lc * P.o1 + k) * (int)sizeof(T));
-        val[u] = tile[lc * LS + k];
-        prev[u] = T(0);
+        ll[u] = l;
+        kk[u] = k;
+        o[u] = (unsigned)(((l < lmax ? l : lmax) * P.e1 + k) * (int)sizeof(T));
         k += dk;
         l += dl;
-        if (k >= n_out) { k -= n_out; ++l; }
+        if (k >= n_in) { k -= n_in; ++l; }
       }
-      if (P.accumulate) {
+      T v[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) prev[u] = ld_at<T>(Ob, o[u]);
-      }
+      for (int u = 0; u < UB; ++u) v[u] = ld_at(Ab, o[u]);
 #pragma unroll
       for (int u = 0; u < UB; ++u)
-        if (ok[u]) st_at<T>(Ob, o[u], prev[u] + val[u] * P.sOut);
+        if (ll[u] < NLW) tile[ll[u] * LS + kk[u]] = v[u];
     }
   }
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const int idx = tid + u * nthr;
+    if (idx < nstP) scoP[idx] = cvP[u];
+    if (idx < nstQ) scoQ[idx] = cvQ[u];
+  }
+  __syncthreads();
+  const T* cP = (kCoLds ? scoP : P.co) + q * CH * kCo;
+  const T* cQ = (kCoLds ? scoQ : Q.co) + q * CH * kCo;
+  const long L = L0 + lane;
+  const bool live = L < P.nlines;
+  const T* row = tile + lane * LS;
+  const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
+  T f[CH + 4];
+#pragma unroll
+  for (int t = 0; t < CH + 4; ++t) f[t] = row[window_index(s0 - 2 + t, n_in, P.wrap)];
+  T fbP[4] = {T(0), T(0), T(0), T(0)}, fbQ[4] = {T(0), T(0), T(0), T(0)};
+  if (live && (q == 0 || q == Pn - 1)) {
+    if (P.fbc)
+      for (int m = 0; m < 4; ++m) fbP[m] = P.fbc[4 * L + m];
+    if (Q.fbc)
+      for (int m = 0; m < 4; ++m) fbQ[m] = Q.fbc[4 * L + m];
+  }
+  T y[CH];
+  chunk_rhs<T, OPA>(P, q, len, f, fbP, y);
+  xu_finish<T, CH, NLW>(P, cP, scoP + nco, tile, lane, q, s0, len, L0, lmax, y, fbP);
+  __syncthreads();  // the first operator's stores have read the tile
+  chunk_rhs<T, OPB>(Q, q, len, f, fbQ, y);
+  xu_finish<T, CH, NLW>(Q, cQ, scoQ + nco, tile, lane, q, s0, len, L0, lmax, y, fbQ);
 }
 
 template <typename T, int IN, int CH, int NLW>
@@ -294,8 +403,29 @@ void launch_x_skew(int op, int ch, dim3 g, dim3 b, size_t shm, hipStream_t s, co
   else skew_ops<T, kChunkLong>(op, g, b, shm, s, a);
 }
 
+template <typename T, int CH, int NLW>
+static void tile2_pair(int pair, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a, const KArgs<T>& a2) {
+  if (pair == 0) hipLaunchKernelGGL((k_line_xu2<T, 0, CH, NLW>), g, b, shm, s, a, a2);
+  else hipLaunchKernelGGL((k_line_xu2<T, 1, CH, NLW>), g, b, shm, s, a, a2);
+}
+
+template <typename T>
+bool launch_x_tile2(int pair, int ch, int nlw, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a,
+                    const KArgs<T>& a2) {
+  if (pair != 0 && pair != 1) return false;
+  if (ch != kChunk) return false;
+  if (nlw == 64) tile2_pair<T, kChunk, 64>(pair, g, b, shm, s, a, a2);
+  else if (nlw == 32) tile2_pair<T, kChunk, 32>(pair, g, b, shm, s, a, a2);
+  else return false;
+  return true;
+}
+
 template bool launch_x_tile<double>(int, int, int, dim3, dim3, size_t, hipStream_t, const KArgs<double>&);
 template bool launch_x_tile<float>(int, int, int, dim3, dim3, size_t, hipStream_t, const KArgs<float>&);
+template bool launch_x_tile2<double>(int, int, int, dim3, dim3, size_t, hipStream_t, const KArgs<double>&,
+                                     const KArgs<double>&);
+template bool launch_x_tile2<float>(int, int, int, dim3, dim3, size_t, hipStream_t, const KArgs<float>&,
+                                    const KArgs<float>&);
 template void launch_x_skew<double>(int, int, dim3, dim3, size_t, hipStream_t, const KArgs<double>&);
 template void launch_x_skew<float>(int, int, dim3, dim3, size_t, hipStream_t, const KArgs<float>&);
 
