@@ -132,6 +132,11 @@ __device__ __forceinline__ void xu_finish(const KArgs<T>& P, const T* c, const T
   }
 }
 
+// Waves per SIMD the tile kernels are compiled for: the 32-line tile of
+// 16-row chunks (at most 8 waves) keeps to 128 registers so that two of its
+// workgroups share a CU (one loads while the other computes).
+__host__ __device__ constexpr int xu_waves(int ch, int nlw) { return (nlw == 32 && ch == kChunk) ? 4 : 1; }
+
 // NLW whole lines per workgroup of NLW x P threads in LDS (row stride
 // ls = n | 1), thread tid on line tid % NLW and chunk tid / NLW.  NLW = 64:
 // one chunk per wave, the chunk coefficient rows staged in LDS after the
@@ -140,7 +145,8 @@ __device__ __forceinline__ void xu_finish(const KArgs<T>& P, const T* c, const T
 // cache; the tile takes half the LDS of the 64-line form, so lines twice as
 // long fit, or two workgroups share a CU.  IN as in load_window.
 template <typename T, int OPK, int IN, int CH, int NLW>
-__global__ void __launch_bounds__(NLW * kMaxChunks) k_line_xu(KArgs<T> P) {
+__global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_per_eu(xu_waves(CH, NLW))))
+    k_line_xu(KArgs<T> P) {
   constexpr bool kCoLds = (NLW == 64);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* tile = reinterpret_cast<T*>(smem_raw);
@@ -257,7 +263,8 @@ __global__ void __launch_bounds__(NLW * kMaxChunks) k_line_xu(KArgs<T> P) {
 // once every window is read).  Per-plan values after the tile: NLW = 64 both
 // plans' chunk rows and inverses, NLW = 32 only the inverses.
 template <typename T, int PAIR, int CH, int NLW>
-__global__ void __launch_bounds__(NLW * kMaxChunks) k_line_xu2(KArgs<T> P, KArgs<T> Q) {
+__global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_per_eu(xu_waves(CH, NLW))))
+    k_line_xu2(KArgs<T> P, KArgs<T> Q) {
   constexpr int OPA = PAIR == 0 ? MIDP_P2C : MIDP_C2P;
   constexpr int OPB = PAIR == 0 ? D1_P2C : D1_C2P;
   constexpr bool kCoLds = (NLW == 64);
