@@ -280,20 +280,20 @@ __device__ __forceinline__ void yz_offsets(const KArgs<T>& P, int L, long& ib, l
 }
 
 // The rest of a y/z chunk after its RHS rows y[]: chunk-local solve, the
-// exchange of first/last values through ex (64 lanes x 2P) and the spike
+// exchange of first/last values through ex (NLW lanes x 2P) and the spike
 // correction (plans with a line solve), then metric scaling, Neumann rows and
 // the (accumulating) store of the chunk's rows at Ob + os * row + ro.
-template <typename T, int CH>
+template <typename T, int CH, int NLW = 64>
 __device__ __forceinline__ void yz_finish(const KArgs<T>& P, const T* c, const T* rinv, T* ex, int lane, int q,
                                           int s0, int len, bool live, T* Ob, unsigned ro, long os, T (&y)[CH],
                                           const T (&fb4)[4]) {
   if (P.solve) {
     T yf, yl;
     chunk_solve(c, len, y, yf, yl);
-    ex[lane + 64 * (2 * q)] = yf;
-    ex[lane + 64 * (2 * q + 1)] = yl;
+    ex[lane + NLW * (2 * q)] = yf;
+    ex[lane + NLW * (2 * q + 1)] = yl;
     __syncthreads();
-    chunk_couple<T>(P, q, rinv, c, [&](int k) { return ex[lane + 64 * k]; }, y);
+    chunk_couple<T>(P, q, rinv, c, [&](int k) { return ex[lane + NLW * k]; }, y);
   }
   if (!live) return;
   const int n_out = P.n_out;
@@ -323,35 +323,47 @@ __device__ __forceinline__ void yz_finish(const KArgs<T>& P, const T* c, const T
   }
 }
 
-// One y/z workgroup: 64 neighbouring lines x P chunks of CH rows, wave q
-// holding chunk q of the 64 lines; IN as in load_window.  CH = kChunkLong
-// also serves x lines longer than a 64-line LDS tile (DIR = 0: the lanes
-// are 64 consecutive lines, each reading its own contiguous rows).
-// Single-input operators ask for 5 waves per SIMD (at most 96 registers), so
-// that two workgroups of 9 waves (lines of 129 rows) share a CU.
-template <typename T, int DIR, int OPK, int IN, int CH>
-__global__ void __launch_bounds__(64 * kMaxChunks)
+// Coefficient values each thread of an NLW x P workgroup stages into LDS
+// (P <= kMaxChunks chunks of CH rows of kCo entries, and the 2P x 2P inverse).
+__host__ __device__ constexpr int stage_n(int ch, int nlw) {
+  return ch * kCo / nlw + (4 * kMaxChunks + nlw - 1) / nlw;
+}
+
+// One y/z workgroup: NLW neighbouring lines x P chunks of CH rows.  NLW = 64:
+// wave q holds chunk q of the 64 lines; NLW = 32: lanes 0-31 and 32-63 of a
+// wave hold two consecutive chunks of 32 lines (half the workgroup, so two
+// of them share a CU where one 64-line workgroup would fill it); IN as in
+// load_window.  CH = kChunkLong also serves x lines longer than a 64-line LDS
+// tile (DIR = 0, NLW = 64: the lanes are 64 consecutive lines, each reading
+// its own contiguous rows).  Single-input operators ask for 5 waves per SIMD
+// (at most 96 registers), so that two workgroups of 9 waves (lines of 129
+// rows) share a CU.
+template <typename T, int DIR, int OPK, int IN, int CH, int NLW = 64>
+__global__ void __launch_bounds__(NLW * kMaxChunks)
     __attribute__((amdgpu_waves_per_eu(IN == 0 && CH == kChunk ? 5 : 4))) k_line_yz(KArgs<T> P) {
-  __shared__ T ex[2 * kMaxChunks * 64];
+  __shared__ T ex[2 * kMaxChunks * NLW];
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sco = reinterpret_cast<T*>(smem_raw);  // P*CH*kCo chunk rows, then the 2P x 2P inverse
   // loads of the per-plan coefficients first (written to LDS once the
   // chunk's window loads are in flight); plans without a line solve (the
   // explicit schemes) have no inverse
-  constexpr int NS = stage_max(CH);
+  constexpr int NS = stage_n(CH, NLW);
+  static_assert(NLW * kMaxChunks * NS >= kMaxChunks * CH * kCo + 4 * kMaxChunks * kMaxChunks,
+                "per-plan staging exceeds NS values per thread");
+  const int nthr = NLW * P.P;
   const int nst = P.P * CH * kCo + (P.rinv ? 4 * P.P * P.P : 0);
   T cv[NS];
   {
     const int nco = P.P * CH * kCo;
 #pragma unroll
     for (int u = 0; u < NS; ++u) {
-      const int idx = threadIdx.x + u * 64 * P.P;
+      const int idx = threadIdx.x + u * nthr;
       cv[u] = (idx < nco) ? P.co[idx] : (idx < nst ? P.rinv[idx - nco] : T(0));
     }
   }
-  const int lane = threadIdx.x & 63;
-  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int L0 = blockIdx.x * 64;  // nlines < 2^31 (checked at launch)
+  const int lane = threadIdx.x & (NLW - 1);
+  const int q = NLW == 64 ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : (int)threadIdx.x / NLW;
+  const int L0 = blockIdx.x * NLW;  // nlines < 2^31 (checked at launch)
   const int Lr = L0 + lane;
   const bool live = Lr < P.nlines;
   const int L = live ? Lr : (int)P.nlines - 1;
@@ -368,7 +380,7 @@ __global__ void __launch_bounds__(64 * kMaxChunks)
   auto stage = [&]() {
 #pragma unroll
     for (int u = 0; u < NS; ++u) {
-      const int idx = threadIdx.x + u * 64 * Pn;
+      const int idx = threadIdx.x + u * nthr;
       if (idx < nst) sco[idx] = cv[u];
     }
   };
@@ -386,7 +398,7 @@ __global__ void __launch_bounds__(64 * kMaxChunks)
   const T* rinv = sco + Pn * CH * kCo;
   T y[CH];
   chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
-  yz_finish<T, CH>(P, c, rinv, ex, lane, q, s0, len, Lr < P.nlines, P.out + ob0, ro, os, y, fb4);
+  yz_finish<T, CH, NLW>(P, c, rinv, ex, lane, q, s0, len, live, P.out + ob0, ro, os, y, fb4);
 }
 
 // Two operators of one line plan geometry (the interpolation and the first
@@ -481,15 +493,15 @@ static size_t max_lds_per_block() {
   return cached[dev];
 }
 
-template <typename T, int DIR, int IN, int CH = kChunk>
+template <typename T, int DIR, int IN, int CH = kChunk, int NLW = 64>
 static void dispatch_yz(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
   switch (op) {
-    case MIDP_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_C2P, IN, CH>), g, b, shm, s, a); break;
-    case MIDP_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_P2C, IN, CH>), g, b, shm, s, a); break;
-    case D1_C2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2C, IN, CH>), g, b, shm, s, a); break;
-    case D1_P2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2P, IN, CH>), g, b, shm, s, a); break;
-    case D1_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2P, IN, CH>), g, b, shm, s, a); break;
-    case D1_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2C, IN, CH>), g, b, shm, s, a); break;
+    case MIDP_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_C2P, IN, CH, NLW>), g, b, shm, s, a); break;
+    case MIDP_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, MIDP_P2C, IN, CH, NLW>), g, b, shm, s, a); break;
+    case D1_C2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2C, IN, CH, NLW>), g, b, shm, s, a); break;
+    case D1_P2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2P, IN, CH, NLW>), g, b, shm, s, a); break;
+    case D1_C2P: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_C2P, IN, CH, NLW>), g, b, shm, s, a); break;
+    case D1_P2C: hipLaunchKernelGGL((k_line_yz<T, DIR, D1_P2C, IN, CH, NLW>), g, b, shm, s, a); break;
   }
 }
 
@@ -504,13 +516,20 @@ static void dispatch_long(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, con
   }
 }
 
-// y and z lines of kChunk-row chunks
-template <typename T, int DIR>
+// y and z lines of kChunk-row chunks, NLW lines per workgroup (32-line
+// workgroups for single-input operators only: with two chunks per wave the
+// row offsets are per lane, and those of several input arrays do not fit
+// the registers)
+template <typename T, int DIR, int NLW>
 static void dispatch_part(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
-  switch (input_kind(a)) {
-    case 0: dispatch_yz<T, DIR, 0>(op, g, b, shm, s, a); break;
-    case 1: dispatch_yz<T, DIR, 1>(op, g, b, shm, s, a); break;
-    default: dispatch_yz<T, DIR, 2>(op, g, b, shm, s, a); break;
+  if constexpr (NLW == 32) {
+    if (input_kind(a) == 0) dispatch_yz<T, DIR, 0, kChunk, 32>(op, g, b, shm, s, a);
+  } else {
+    switch (input_kind(a)) {
+      case 0: dispatch_yz<T, DIR, 0, kChunk, NLW>(op, g, b, shm, s, a); break;
+      case 1: dispatch_yz<T, DIR, 1, kChunk, NLW>(op, g, b, shm, s, a); break;
+      default: dispatch_yz<T, DIR, 2, kChunk, NLW>(op, g, b, shm, s, a); break;
+    }
   }
 }
 
@@ -667,9 +686,35 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
       if (dir == 1) dispatch_long<T, 1>(p.op, gl, bl, shm_l, s, a);
       else dispatch_long<T, 2>(p.op, gl, bl, shm_l, s, a);
     } else {
+      // y and z lines of 16-row chunks: 64 or (single-input operators) 32
+      // lines per workgroup, timed once per shape when a scratch output is
+      // at hand (else 64)
       if (!lanes_ok) return "line op: x-y plane too large for 32-bit line offsets";
-      if (dir == 1) dispatch_part<T, 1>(p.op, gl, bl, shm_l, s, a);
-      else dispatch_part<T, 2>(p.op, gl, bl, shm_l, s, a);
+      auto run = [&](int v, const KArgs<T>& ka) -> bool {
+        if (v == 1 && input_kind(ka) != 0) return false;
+        const int nlw = v == 0 ? 64 : 32;
+        dim3 g((unsigned)((ka.nlines + nlw - 1) / nlw)), b((unsigned)(nlw * p.part_P));
+        if (v == 0) {
+          if (dir == 1) dispatch_part<T, 1, 64>(p.op, g, b, shm_l, s, ka);
+          else dispatch_part<T, 2, 64>(p.op, g, b, shm_l, s, ka);
+        } else {
+          if (dir == 1) dispatch_part<T, 1, 32>(p.op, g, b, shm_l, s, ka);
+          else dispatch_part<T, 2, 32>(p.op, g, b, shm_l, s, ka);
+        }
+        return hipGetLastError() == hipSuccess;
+      };
+      int v = -1;
+      if (scratch) {
+        KArgs<T> ts = a;
+        ts.out = scratch;  // the candidates write the scratch array, never the output
+        ts.accumulate = 0;
+        const std::string key = std::string("line_yz/") + (sizeof(T) == 8 ? "f64/" : "f32/") +
+                                std::to_string(dir) + "/" + std::to_string(p.op) + "/" + std::to_string(p.n_in) +
+                                "/" + std::to_string(p.n_out) + "/" + std::to_string(a.nlines) + "/" +
+                                std::to_string(input_kind(a));
+        v = tune(key, 2, s, [&](int vv) { return run(vv, ts); });
+      }
+      if (!run(v < 0 ? 0 : v, a)) return std::string("line op launch: ") + hipGetErrorString(hipGetLastError());
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return std::string("line op launch: ") + hipGetErrorString(e);

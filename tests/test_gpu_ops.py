@@ -313,3 +313,52 @@ def test_x_launch_shapes_vs_oracle(po, variant):
             dm.close()
     finally:
         _lib.check(lib.chs_tune_force(b"line_x/", -1))
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_yz_launch_shapes_vs_oracle(po, variant):
+    """Both y/z launch shapes, forced through chs_tune_force (0: 64 lines per
+    workgroup, one chunk per wave; 1: 32 lines per workgroup, two chunks per
+    wave, single-input operators), on y lines of 24/25 and 129/130 rows and z
+    lines of 16 and 256 rows, every operator with CP6 and the explicit CD4,
+    wall-bounded y with Dirichlet, Neumann and interpolation boundary kinds,
+    against the oracle."""
+    from chapsim2_amd import _lib
+    from chapsim2_amd import operations as ops
+    from chapsim2_amd.domain import Domain
+
+    lib = _lib.load()
+    _lib.check(lib.chs_tune_force(b"line_yz/", variant))
+    try:
+        for nc in ((40, 24, 16), (24, 129, 256)):
+            dm = Domain(nc, LX, -1.0, 1.0, LZ, istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1)
+            odm = po.Dom(list(nc), [1, 0, 1], LX, -1.0, 1.0, LZ, 2, 0.075)
+            rng = np.random.default_rng(61 + variant)
+            try:
+                for d in (1, 2):
+                    for op in range(6):
+                        for acc in (2, 4):
+                            bcs = [((1, 1), False)] if d != 1 else [((4, 4), True), ((5, 5), True), ((2, 3), False)]
+                            for bc, fb in bcs:
+                                per = bc[0] == 1
+                                ext_in, ext_out = list(nc), list(nc)
+                                ext_in[d] = _n(nc[d], per, INP[op])
+                                ext_out[d] = _n(nc[d], per, OUT[op])
+                                fi = np.asfortranarray(rng.standard_normal(ext_in))
+                                fshape = list(ext_in)
+                                fshape[d] = 4
+                                fbc = np.asfortranarray(rng.standard_normal(fshape)) if fb else None
+                                try:
+                                    ref = odm.op3d(d, op, acc, list(bc), fi, ext_out, fbc)
+                                except ValueError:
+                                    continue
+                                if not np.all(np.isfinite(ref)) or np.abs(ref).max() > 1e15:
+                                    continue
+                                out = np.zeros(ext_out, order="F")
+                                ops.op3d(dm, d, op, fi, out, acc, list(bc), fbc)
+                                err = np.abs(out - ref).max() / max(1.0, np.abs(ref).max())
+                                assert err < RTOL, (variant, nc, d, op, acc, bc, fb, err)
+            finally:
+                dm.close()
+    finally:
+        _lib.check(lib.chs_tune_force(b"line_yz/", -1))
