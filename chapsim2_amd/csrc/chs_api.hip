@@ -499,6 +499,13 @@ int chs_momentum_rhs(chs_handle_t* H, int isub, double* mx, double* my, double* 
   return CHS_OK;
 }
 
+int chs_cfl(chs_handle_t* H, double out[2]) {
+  FLOW_OR_FAIL(H);
+  if (!out) return fail(CHS_EINVAL, "invalid argument");
+  RET(F.cfl(out));
+  return CHS_OK;
+}
+
 int chs_mass_check(chs_handle_t* H, double out[3]) {
   FLOW_OR_FAIL(H);
   RET(F.mass_check(out));

@@ -59,6 +59,7 @@ class Flow {
   std::string momentum_rhs(int isub);  // leaves dt-scaled rhs in acc_x of each component
   std::string divergence(double* div_dev);
   std::string mass_check(double out[3]);
+  std::string cfl(double out[2]);  // convective CFL and diffusion numbers
   std::string copy_rhs(double* mx, double* my, double* mz, int mem);
 
   Handle* H = nullptr;
@@ -79,6 +80,7 @@ class Flow {
   double *partial = nullptr, *bulk = nullptr;
   long npartial = 0;  // entries of partial (at least kMaxPartials)
   double* dyw_dev = nullptr;
+  double* ymc_dev = nullptr;  // ds/dy at cell centres (stretched y only; made on first use)
 
   // Independent operators of a sub-step run on three streams (lane 0 = the
   // handle's stream, lanes 1 and 2 = side streams), forked from and joined

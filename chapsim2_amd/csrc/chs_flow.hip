@@ -217,6 +217,31 @@ __global__ void k_maxabs_xwin(const double* __restrict__ d, int n1, long n, int 
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// Block maxima of |u h1r_x dt + v h1r_y dt ymc(j) + w h1r_z dt| over cell
+// centres (x fastest), the sum formed in the order of Check_cfl_convection
+// (src/tools_solver.f90:301-344); ymc may be null (uniform y).
+__global__ void k_cfl_max(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
+                          int n1, int n2, long n, double hx, double hy, double hz, double dt,
+                          const double* __restrict__ ymc, double* __restrict__ partial) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int j = (int)((i / n1) % n2);
+    const double cx = u[i] * hx * dt;
+    double cy = v[i] * hy * dt;
+    if (ymc) cy = cy * ymc[j];
+    const double c = (cx + cy) + w[i] * hz * dt;
+    m = nanmax(m, fabs(c));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = nanmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 static inline dim3 grid1(long n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
 
 // ---------------------------------------------------------------------------
@@ -728,6 +753,39 @@ std::string Flow::substep(int isub) {
   // every operator of a sub-step is planned and tuned now: later sub-steps
   // may overlap independent operators
   concurrent = (side[0] != nullptr);
+  return "";
+}
+
+std::string Flow::cfl(double out[2]) {
+  const Mesh& m = H->mesh;
+  const bool stretched = m.stretched;
+  if (stretched && !ymc_dev) {
+    FCHK(hipMalloc(&ymc_dev, sizeof(double) * (size_t)ny));
+    owned.push_back(ymc_dev);
+    FCHK(hipMemcpy(ymc_dev, m.ymc.data(), sizeof(double) * (size_t)ny, hipMemcpyHostToDevice));
+  }
+  // cell-centred velocities: the P2C interpolations of the first-level
+  // operators (their work arrays are free between sub-steps)
+  SCHK(op<double>(w64, 0, MIDP_P2C, ibc[0][VU], qx, w64.qxix, nullptr));
+  SCHK(op<double>(w64, 1, MIDP_P2C, ibc[1][VV], qy, w64.qyiy, wall_y ? w64.fby_v : nullptr));
+  SCHK(op<double>(w64, 2, MIDP_P2C, ibc[2][VW], qz, w64.qziz, nullptr));
+  const int nb = kMaxPartials;
+  hipLaunchKernelGGL(k_cfl_max, dim3(nb), dim3(256), 0, H->stream, w64.qxix.p, w64.qyiy.p, w64.qziz.p, nx, ny,
+                     w64.qxix.size(), m.h1r[0], m.h1r[1], m.h1r[2], dt, stretched ? ymc_dev : nullptr, partial);
+  std::vector<double> h(nb);
+  FCHK(hipMemcpyAsync(h.data(), partial, sizeof(double) * nb, hipMemcpyDeviceToHost, H->stream));
+  FCHK(hipStreamSynchronize(H->stream));
+  double mx = 0.0;
+  for (double v : h) mx = (v > mx || v != v) ? v : mx;
+  out[0] = mx;
+  // diffusion number, Cartesian constant properties (:220-259)
+  double d = 0.0;
+  for (int j = 0; j < ny; ++j) {
+    const double dyi = 1.0 / (m.yp[j + 1] - m.yp[j]);
+    const double r = m.h2r[0] + m.h2r[2] + dyi * dyi;
+    if (r > d) d = r;
+  }
+  out[1] = d * 2.0 * dt * rre;
   return "";
 }
 

@@ -116,6 +116,14 @@ class Domain:
         _lib.check(self.lib.chs_mass_check(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def cfl(self):
+        """(convective CFL number, diffusion number) of the current state,
+        Check_cfl_convection / Check_cfl_diffusion
+        (src/tools_solver.f90:220-375)."""
+        out = np.zeros(2)
+        _lib.check(self.lib.chs_cfl(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
     def divergence(self):
         out = np.zeros(self.shapes["ccc"], order="F")
         _lib.check(self.lib.chs_divergence(self.h, out.ctypes.data, _lib.MEM_HOST))

@@ -137,6 +137,13 @@ int chs_momentum_rhs(chs_handle_t *h, int isub, double *mx, double *my, double *
  * Check_element_mass_conservation (src/eq_continuity.f90:337-397). */
 int chs_mass_check(chs_handle_t *h, double out[3]);
 
+/* out[0]: convective CFL number max|u dt/dx + v dt/dy + w dt/dz| of the
+ * current velocity (P2C interpolations to cell centres, y spacing through the
+ * stretching map); out[1]: diffusion number.  Replaces Check_cfl_convection
+ * and Check_cfl_diffusion (src/tools_solver.f90:220-375; Cartesian, constant
+ * properties), which print these values; the caller prints or checks them. */
+int chs_cfl(chs_handle_t *h, double out[2]);
+
 /* Divergence of the current velocity, (nc1,nc2,nc3).  Replaces
  * Get_divergence_flow (src/eq_continuity.f90:91-161). */
 int chs_divergence(chs_handle_t *h, double *div, int mem);

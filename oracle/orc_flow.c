@@ -649,6 +649,46 @@ void orc_mass_check(orc_flow *f, double out[3]) {
   free(div);
 }
 
+/* Check_cfl_convection and Check_cfl_diffusion, src/tools_solver.f90:220-375
+ * (Cartesian, constant properties).  out[0] = max |u h1r(1) dt + v h1r(2)
+ * yMappingcc(j) dt + w h1r(3) dt| over cells, with u, v, w the P2C
+ * interpolations of qx, qy, qz (accumulated in the reference's order, x
+ * then y then z); out[1] = max_j (h2r(1) + h2r(3) + 1 / (yp(j+1) - yp(j))^2)
+ * 2 dt rre. */
+void orc_cfl(orc_flow *f, double out[2]) {
+  int nx = f->nx, ny = f->ny, nz = f->nz, px = f->px, py = f->py, pz = f->pz;
+  long Nccc = (long)nx * ny * nz;
+  int (*B)[5][2] = f->ibc;
+  const orc_dom *dm = &f->dm;
+  const double dt = f->dt;
+  double *u = arr(Nccc), *v = arr(Nccc), *w = arr(Nccc);
+  op(f, 0, ORC_MIDP_P2C, B[0][VU], f->qx, px, ny, nz, u, nx, ny, nz, f->fbx[VU]);
+  op(f, 1, ORC_MIDP_P2C, B[1][VV], f->qy, nx, py, nz, v, nx, ny, nz, f->fby[VV]);
+  op(f, 2, ORC_MIDP_P2C, B[2][VW], f->qz, nx, ny, pz, w, nx, ny, nz, f->fbz[VW]);
+  double m = 0.0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        long t = i + (long)nx * (j + (long)ny * k);
+        double cx = u[t] * dm->h1r[0] * dt;
+        double cy = v[t] * dm->h1r[1] * dt;
+        if (dm->stretching) cy = cy * dm->ymc[j];
+        double c = (cx + cy) + w[t] * dm->h1r[2] * dt;
+        if (fabs(c) > m) m = fabs(c);
+      }
+  out[0] = m;
+  double d = 0.0;
+  for (int j = 0; j < ny; ++j) {
+    double dyi = 1.0 / (dm->yp[j + 1] - dm->yp[j]);
+    double r = dm->h2r[0] + dm->h2r[2] + dyi * dyi;
+    if (r > d) d = r;
+  }
+  out[1] = d * 2.0 * dt * f->rre;
+  free(u);
+  free(v);
+  free(w);
+}
+
 /* ---------------------------------------------------------------------- */
 /* Poisson: FISHPACK path, src/poisson_fishpack.f90:354-490                */
 /* ---------------------------------------------------------------------- */

@@ -112,7 +112,7 @@ static void set_intrpl_row(int f, int r, int n, double al, const double w[6],
 }
 
 static void build_tables(void) {
-  double al[NACC + 1], a[NACC + 1], b[NACC + 1], c[NACC + 1];
+  double al[NACC + 1], a[NACC + 1], b[NACC + 1];
   for (int f = 0; f < 6; ++f)
     for (int r = 0; r < NL; ++r)
       for (int bc = 0; bc < NBC; ++bc)

@@ -95,6 +95,7 @@ def lib():
         L.orc_project.argtypes = [C.c_void_p, C.c_int]
         L.orc_divergence.argtypes = [C.c_void_p, dp]
         L.orc_mass_check.argtypes = [C.c_void_p, dp]
+        L.orc_cfl.argtypes = [C.c_void_p, dp]
         L.orc_momentum_rhs.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
         L.orc_poisson_solve.argtypes = [C.c_void_p, dp]
         L.orc_vol_average_x.restype = C.c_double
@@ -299,6 +300,12 @@ class Flow:
     def mass_check(self):
         out = np.zeros(3)
         lib().orc_mass_check(self.h, _p(out))
+        return out
+
+    def cfl(self):
+        """(convective CFL, diffusion number), src/tools_solver.f90:220-375."""
+        out = np.zeros(2)
+        lib().orc_cfl(self.h, _p(out))
         return out
 
     def poisson(self, rhs):

@@ -223,3 +223,23 @@ def test_restart_resume_matches_continuous_run(po, tmp_path):
     finally:
         a.close()
         b.close()
+
+
+def test_cfl_vs_oracle(pair):
+    """chs_cfl (convective CFL from the P2C-interpolated velocity, diffusion
+    number) against the oracle's Check_cfl_convection / Check_cfl_diffusion
+    (src/tools_solver.f90:220-375) on the noisy stretched channel, before and
+    after two RK3 steps."""
+    dm, of = pair
+    for step in range(2):
+        if step:
+            of.update_re(0)
+            dm.update_re(0)
+            of.substep(0)
+            dm.substep(0)
+            of.step(1)
+            dm.run(0, 1)
+        g, r = dm.cfl(), of.cfl()
+        assert abs(g[0] - r[0]) <= 1e-13 * r[0], (step, g, r)
+        assert abs(g[1] - r[1]) <= 1e-14 * r[1], (step, g, r)
+        assert r[0] > 0

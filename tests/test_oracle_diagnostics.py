@@ -1,0 +1,49 @@
+"""The oracle's CFL diagnostics (Check_cfl_convection / Check_cfl_diffusion,
+src/tools_solver.f90:220-375) against closed forms."""
+import numpy as np
+
+LX, LZ = 4 * np.pi, 4 * np.pi / 3
+
+
+def test_cfl_of_uniform_flow(po):
+    """A uniform (U, 0, W) flow is interpolated exactly, so the convective
+    CFL number is |U dt/dx + W dt/dz| (the x and z contributions add with
+    their signs); on a uniform y grid the diffusion number is
+    (1/dx^2 + 1/dz^2 + 1/dy^2) 2 dt / Re."""
+    nc = (16, 12, 8)
+    dt, re = 2e-3, 400.0
+    f = po.Flow(list(nc), LX, -1.0, 1.0, LZ, istret=0, wall_y=False, iacc=4, poisson=2, dt=dt, ren=re)
+    for U, W in ((1.5, 0.25), (-0.75, 2.0)):
+        qx = np.full(f.shapes["qx"], U, order="F")
+        qy = np.zeros(f.shapes["qy"], order="F")
+        qz = np.full(f.shapes["qz"], W, order="F")
+        pr = np.zeros(f.shapes["pr"], order="F")
+        f.set_state(qx, qy, qz, pr)
+        f.update_re(1)
+        conv, diff = f.cfl()
+        dx, dy, dz = LX / nc[0], 2.0 / nc[1], LZ / nc[2]
+        assert abs(conv - abs(U * dt / dx + W * dt / dz)) < 1e-13
+        assert abs(diff - (1 / dx**2 + 1 / dz**2 + 1 / dy**2) * 2 * dt / re) < 1e-13 * diff
+
+
+def test_cfl_of_wall_normal_flow_on_stretched_grid(po):
+    """A uniform wall-normal velocity V on the stretched channel grid: the
+    cell-centred value is V away from the walls, and the CFL number is the
+    largest V dt / dy_c, dy_c = h2 / yMappingcc (yMappingcc = ds/dy at the
+    centres), attained at the finest cells next to the walls."""
+    nc = (16, 24, 8)
+    dt, re = 5e-4, 2800.0
+    f = po.Flow(list(nc), LX, -1.0, 1.0, LZ, istret=2, rstret=0.075, wall_y=True, iacc=4, poisson=2, dt=dt,
+                ren=re, uwall=((0.0, 0.7, 0.0), (0.0, 0.7, 0.0)))
+    V = 0.7
+    qx = np.zeros(f.shapes["qx"], order="F")
+    qy = np.full(f.shapes["qy"], V, order="F")
+    qz = np.zeros(f.shapes["qz"], order="F")
+    f.set_state(qx, qy, qz, np.zeros(f.shapes["pr"], order="F"))
+    f.update_re(1)
+    conv, diff = f.cfl()
+    h2 = 1.0 / nc[1]
+    ymc = f.dom.ymc
+    assert abs(conv - V * dt * ymc.max() / h2) < 1e-12 * conv
+    yp = f.dom.yp
+    assert abs(diff - ((nc[0] / LX) ** 2 + (nc[2] / LZ) ** 2 + (1 / np.diff(yp).min()) ** 2) * 2 * dt / re) < 1e-12 * diff
