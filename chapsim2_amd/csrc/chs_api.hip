@@ -506,6 +506,13 @@ int chs_cfl(chs_handle_t* H, double out[2]) {
   return CHS_OK;
 }
 
+int chs_bulk(chs_handle_t* H, double out[2]) {
+  FLOW_OR_FAIL(H);
+  if (!out) return fail(CHS_EINVAL, "invalid argument");
+  RET(F.bulk_monitor(out));
+  return CHS_OK;
+}
+
 int chs_mass_check(chs_handle_t* H, double out[3]) {
   FLOW_OR_FAIL(H);
   RET(F.mass_check(out));

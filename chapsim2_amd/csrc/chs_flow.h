@@ -59,7 +59,9 @@ class Flow {
   std::string momentum_rhs(int isub);  // leaves dt-scaled rhs in acc_x of each component
   std::string divergence(double* div_dev);
   std::string mass_check(double out[3]);
-  std::string cfl(double out[2]);  // convective CFL and diffusion numbers
+  std::string cfl(double out[2]);   // convective CFL and diffusion numbers
+  std::string bulk_monitor(double out[2]);  // bulk mean kinetic energy and bulk qx
+  std::string cell_velocity();      // P2C interpolations of qx, qy, qz into w64.qxix, qyiy, qziz
   std::string copy_rhs(double* mx, double* my, double* mz, int mem);
 
   Handle* H = nullptr;

@@ -242,6 +242,28 @@ __global__ void k_cfl_max(const double* __restrict__ u, const double* __restrict
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// Block sums of f(i) dyw(j) over an x-pencil array (x fastest, j = row
+// along y), f = a (b, c null) or 1/2 (a^2 + b^2 + c^2): the volume integrals
+// of Get_volumetric_average_3d_for_var_xcx (src/tools_general.f90:1202-1268)
+// before the dx dz factor.
+__global__ void k_vol_sum(const double* __restrict__ a, const double* __restrict__ b, const double* __restrict__ c,
+                          int n1, int n2, long n, const double* __restrict__ dyw, double* __restrict__ partial) {
+  __shared__ double red[256];
+  double sum = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int j = (int)((i / n1) % n2);
+    const double f = b ? 0.5 * (a[i] * a[i] + b[i] * b[i] + c[i] * c[i]) : a[i];
+    sum += f * dyw[j];
+  }
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 static inline dim3 grid1(long n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
 
 // ---------------------------------------------------------------------------
@@ -756,6 +778,36 @@ std::string Flow::substep(int isub) {
   return "";
 }
 
+std::string Flow::cell_velocity() {
+  // the P2C interpolations of the first-level operators (their work arrays
+  // are free between sub-steps)
+  SCHK(op<double>(w64, 0, MIDP_P2C, ibc[0][VU], qx, w64.qxix, nullptr));
+  SCHK(op<double>(w64, 1, MIDP_P2C, ibc[1][VV], qy, w64.qyiy, wall_y ? w64.fby_v : nullptr));
+  return op<double>(w64, 2, MIDP_P2C, ibc[2][VW], qz, w64.qziz, nullptr);
+}
+
+std::string Flow::bulk_monitor(double out[2]) {
+  SCHK(cell_velocity());
+  const int nb = kMaxPartials;
+  std::vector<double> h(nb);
+  const double* src[2] = {w64.qxix.p, qx.p};
+  const int n1[2] = {nx, px};
+  for (int q = 0; q < 2; ++q) {
+    const long n = (long)n1[q] * ny * nz;
+    hipLaunchKernelGGL(k_vol_sum, dim3(nb), dim3(256), 0, H->stream, src[q], q == 0 ? w64.qyiy.p : nullptr,
+                       q == 0 ? w64.qziz.p : nullptr, n1[q], ny, n, dyw_dev, partial);
+    FCHK(hipMemcpyAsync(h.data(), partial, sizeof(double) * nb, hipMemcpyDeviceToHost, H->stream));
+    FCHK(hipStreamSynchronize(H->stream));
+    double sum = 0.0;
+    for (double v : h) sum += v;
+    // volume of the array's cells: n1 nz dx dz sum_j dy_j
+    double sy = 0.0;
+    for (int j = 0; j < ny; ++j) sy += dyw[j];
+    out[q] = sum / ((double)n1[q] * nz * sy);
+  }
+  return "";
+}
+
 std::string Flow::cfl(double out[2]) {
   const Mesh& m = H->mesh;
   const bool stretched = m.stretched;
@@ -764,11 +816,7 @@ std::string Flow::cfl(double out[2]) {
     owned.push_back(ymc_dev);
     FCHK(hipMemcpy(ymc_dev, m.ymc.data(), sizeof(double) * (size_t)ny, hipMemcpyHostToDevice));
   }
-  // cell-centred velocities: the P2C interpolations of the first-level
-  // operators (their work arrays are free between sub-steps)
-  SCHK(op<double>(w64, 0, MIDP_P2C, ibc[0][VU], qx, w64.qxix, nullptr));
-  SCHK(op<double>(w64, 1, MIDP_P2C, ibc[1][VV], qy, w64.qyiy, wall_y ? w64.fby_v : nullptr));
-  SCHK(op<double>(w64, 2, MIDP_P2C, ibc[2][VW], qz, w64.qziz, nullptr));
+  SCHK(cell_velocity());
   const int nb = kMaxPartials;
   hipLaunchKernelGGL(k_cfl_max, dim3(nb), dim3(256), 0, H->stream, w64.qxix.p, w64.qyiy.p, w64.qziz.p, nx, ny,
                      w64.qxix.size(), m.h1r[0], m.h1r[1], m.h1r[2], dt, stretched ? ymc_dev : nullptr, partial);

@@ -124,6 +124,13 @@ class Domain:
         _lib.check(self.lib.chs_cfl(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def bulk(self):
+        """(bulk mean kinetic energy, bulk qx) of the current state, the
+        values of write_monitor_bulk (src/io_monitor.f90:207-229)."""
+        out = np.zeros(2)
+        _lib.check(self.lib.chs_bulk(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
     def divergence(self):
         out = np.zeros(self.shapes["ccc"], order="F")
         _lib.check(self.lib.chs_divergence(self.h, out.ctypes.data, _lib.MEM_HOST))

@@ -39,7 +39,7 @@ module chapsim_hip
   end type chs_desc_t
 
   public :: chs_abi_version, chs_create, chs_destroy, chs_sync, chs_op3d, chs_poisson
-  public :: chs_set_state, chs_get_state, chs_update_re, chs_substep, chs_run, chs_mass_check, chs_cfl
+  public :: chs_set_state, chs_get_state, chs_update_re, chs_substep, chs_run, chs_mass_check, chs_cfl, chs_bulk
   public :: chs_divergence, chs_inspect_coeff, chs_inspect_op1d, chs_error_message, chs_check
   public :: Get_x_midp_C2P_3D, Get_x_midp_P2C_3D, Get_x_1der_C2C_3D, Get_x_1der_P2P_3D, &
             Get_x_1der_C2P_3D, Get_x_1der_P2C_3D, Get_y_midp_C2P_3D, Get_y_midp_P2C_3D, &
@@ -117,6 +117,12 @@ module chapsim_hip
     end function
     ! Check_cfl_convection / Check_cfl_diffusion (src/tools_solver.f90:220-375)
     integer(c_int) function chs_cfl(h, out) bind(C, name="chs_cfl")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out(2)
+    end function
+    ! write_monitor_bulk values (src/io_monitor.f90:207-229)
+    integer(c_int) function chs_bulk(h, out) bind(C, name="chs_bulk")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(out) :: out(2)

@@ -144,6 +144,13 @@ int chs_mass_check(chs_handle_t *h, double out[3]);
  * properties), which print these values; the caller prints or checks them. */
 int chs_cfl(chs_handle_t *h, double out[2]);
 
+/* out[0]: volumetric average of 1/2 (u^2 + v^2 + w^2) (velocity interpolated
+ * to the cell centres), out[1]: volumetric average of qx.  The bulk values
+ * the reference's monitor writes (write_monitor_bulk,
+ * src/io_monitor.f90:207-229, with Get_volumetric_average_3d_for_var_xcx,
+ * src/tools_general.f90:1202-1268). */
+int chs_bulk(chs_handle_t *h, double out[2]);
+
 /* Divergence of the current velocity, (nc1,nc2,nc3).  Replaces
  * Get_divergence_flow (src/eq_continuity.f90:91-161). */
 int chs_divergence(chs_handle_t *h, double *div, int mem);

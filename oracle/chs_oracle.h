@@ -110,6 +110,7 @@ void orc_flow_step(orc_flow *f, int iter);
 void orc_divergence(orc_flow *f, double *div);
 void orc_mass_check(orc_flow *f, double out[3]);
 void orc_cfl(orc_flow *f, double out[2]);
+void orc_bulk(orc_flow *f, double out[2]);
 void orc_momentum_rhs(orc_flow *f, int isub, double *mx, double *my,
                       double *mz);
 void orc_poisson_solve(orc_flow *f, double *rhs);

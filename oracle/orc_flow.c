@@ -689,6 +689,26 @@ void orc_cfl(orc_flow *f, double out[2]) {
   free(w);
 }
 
+/* Bulk monitor values of write_monitor_bulk (src/io_monitor.f90:207-229):
+ * out[0] = volumetric average of 1/2 (u^2 + v^2 + w^2) with u, v, w the P2C
+ * interpolations of qx, qy, qz to the cell centres, out[1] = volumetric
+ * average of qx (Get_volumetric_average_3d_for_var_xcx). */
+void orc_bulk(orc_flow *f, double out[2]) {
+  int nx = f->nx, ny = f->ny, nz = f->nz, px = f->px, py = f->py, pz = f->pz;
+  long Nccc = (long)nx * ny * nz;
+  int (*B)[5][2] = f->ibc;
+  double *u = arr(Nccc), *v = arr(Nccc), *w = arr(Nccc);
+  op(f, 0, ORC_MIDP_P2C, B[0][VU], f->qx, px, ny, nz, u, nx, ny, nz, f->fbx[VU]);
+  op(f, 1, ORC_MIDP_P2C, B[1][VV], f->qy, nx, py, nz, v, nx, ny, nz, f->fby[VV]);
+  op(f, 2, ORC_MIDP_P2C, B[2][VW], f->qz, nx, ny, pz, w, nx, ny, nz, f->fbz[VW]);
+  for (long t = 0; t < Nccc; ++t) u[t] = 0.5 * (u[t] * u[t] + v[t] * v[t] + w[t] * w[t]);
+  out[0] = orc_vol_average_x(f, u, nx);
+  out[1] = orc_vol_average_x(f, f->qx, px);
+  free(u);
+  free(v);
+  free(w);
+}
+
 /* ---------------------------------------------------------------------- */
 /* Poisson: FISHPACK path, src/poisson_fishpack.f90:354-490                */
 /* ---------------------------------------------------------------------- */

@@ -96,6 +96,7 @@ def lib():
         L.orc_divergence.argtypes = [C.c_void_p, dp]
         L.orc_mass_check.argtypes = [C.c_void_p, dp]
         L.orc_cfl.argtypes = [C.c_void_p, dp]
+        L.orc_bulk.argtypes = [C.c_void_p, dp]
         L.orc_momentum_rhs.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
         L.orc_poisson_solve.argtypes = [C.c_void_p, dp]
         L.orc_vol_average_x.restype = C.c_double
@@ -306,6 +307,12 @@ class Flow:
         """(convective CFL, diffusion number), src/tools_solver.f90:220-375."""
         out = np.zeros(2)
         lib().orc_cfl(self.h, _p(out))
+        return out
+
+    def bulk(self):
+        """(bulk mean kinetic energy, bulk qx), src/io_monitor.f90:207-229."""
+        out = np.zeros(2)
+        lib().orc_bulk(self.h, _p(out))
         return out
 
     def poisson(self, rhs):

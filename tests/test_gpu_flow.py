@@ -243,3 +243,13 @@ def test_cfl_vs_oracle(pair):
         assert abs(g[0] - r[0]) <= 1e-13 * r[0], (step, g, r)
         assert abs(g[1] - r[1]) <= 1e-14 * r[1], (step, g, r)
         assert r[0] > 0
+
+
+def test_bulk_monitor_vs_oracle(pair):
+    """chs_bulk (bulk mean kinetic energy of the cell-centred velocity and
+    bulk qx, the values of write_monitor_bulk) against the oracle on the
+    noisy stretched channel."""
+    dm, of = pair
+    g, r = dm.bulk(), of.bulk()
+    assert np.all(np.abs(g - r) <= 1e-13 * np.abs(r)), (g, r)
+    assert abs(r[1] - 1.0) < 0.1  # the initial field carries a unit bulk velocity

@@ -47,3 +47,19 @@ def test_cfl_of_wall_normal_flow_on_stretched_grid(po):
     assert abs(conv - V * dt * ymc.max() / h2) < 1e-12 * conv
     yp = f.dom.yp
     assert abs(diff - ((nc[0] / LX) ** 2 + (nc[2] / LZ) ** 2 + (1 / np.diff(yp).min()) ** 2) * 2 * dt / re) < 1e-12 * diff
+
+
+def test_bulk_monitor_of_uniform_flow(po):
+    """For a uniform (U, V, W) flow (wall values V) the interpolated
+    velocity is exact, so the bulk mean kinetic energy is (U^2 + V^2 + W^2)/2
+    and the bulk qx is U, on the stretched channel grid (the volume weights
+    h2 / yMappingcc sum to the channel height)."""
+    nc = (16, 24, 8)
+    U, V, W = 1.25, -0.5, 0.75
+    f = po.Flow(list(nc), LX, -1.0, 1.0, LZ, istret=2, rstret=0.075, wall_y=True, iacc=4, poisson=2, dt=5e-4,
+                ren=2800.0, uwall=((0.0, V, 0.0), (0.0, V, 0.0)))
+    f.set_state(np.full(f.shapes["qx"], U, order="F"), np.full(f.shapes["qy"], V, order="F"),
+                np.full(f.shapes["qz"], W, order="F"), np.zeros(f.shapes["pr"], order="F"))
+    mke, qx = f.bulk()
+    assert abs(mke - 0.5 * (U * U + V * V + W * W)) < 1e-13
+    assert abs(qx - U) < 1e-13
