@@ -90,13 +90,16 @@ std::string launch_line_op(const DevPlan<T>& P, const Ext3& ein, const Ext3& eou
 // interpolation and the first derivative between the same locations
 // (MIDP_P2C with D1_P2C, or MIDP_C2P with D1_C2P; any order), whose plans
 // share the extents, chunk partition and boundary wrap, each with its own
-// wall values and output.  *done is false, with no error and nothing
-// launched, when the pair cannot share a pass; the caller then applies the
-// operators one by one.  check_only: only decide *done, launch nothing.
+// wall values and output.  *passes: 1 when both ran in one pass, 2 when they
+// ran one by one (x lines where that was timed faster; scratch takes the
+// timing candidates' output), 0 with no error and nothing launched when the
+// pair cannot share a pass (the caller then applies the operators itself).
+// check_only: only decide *passes (timing the x shapes once if needed).
 template <typename T>
 std::string launch_line_op2(const DevPlan<T>& A, const DevPlan<T>& B, const Ext3& ein, const Ext3& eout,
                             const T* in, const T* fbcA, const T* fbcB, const LineOut<T>& outA,
-                            const LineOut<T>& outB, hipStream_t s, bool* done, bool check_only = false);
+                            const LineOut<T>& outB, T* scratch, hipStream_t s, int* passes,
+                            bool check_only = false);
 
 // Evaluate only the first and last output rows of a y-direction plan on every
 // (x, z) line, writing planes of extents (e1, 2, e3).  Valid when the plan's

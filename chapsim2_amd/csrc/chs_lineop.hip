@@ -748,8 +748,8 @@ static void dispatch_yz2(int pair, int ch, dim3 g, dim3 b, size_t shm, hipStream
 template <typename T>
 std::string launch_line_op2(const DevPlan<T>& DA, const DevPlan<T>& DB, const Ext3& ein, const Ext3& eout,
                             const T* in, const T* fbcA, const T* fbcB, const LineOut<T>& outA,
-                            const LineOut<T>& outB, hipStream_t s, bool* done, bool check_only) {
-  *done = false;
+                            const LineOut<T>& outB, T* scratch, hipStream_t s, int* passes, bool check_only) {
+  *passes = 0;
   // the interpolation goes first
   const bool swap = (DA.host.op == D1_P2C || DA.host.op == D1_C2P);
   const DevPlan<T>& M = swap ? DB : DA;
@@ -771,49 +771,85 @@ std::string launch_line_op2(const DevPlan<T>& DA, const DevPlan<T>& DB, const Ex
   if (e.empty()) e = line_args(G, ein, eout, li, fG, oG, (T*)nullptr, &b);
   if (!e.empty()) return e;
   if (a.nlines <= 0) {
-    *done = true;
+    *passes = 1;
     return "";
   }
-  if (sizeof(T) == 8 && pm.part_ch != kChunk) return "";
-  if (pm.dir == 0 && (pm.part_ch != kChunk ||
-                      sizeof(T) * ((size_t)64 * ((pm.n_in > pm.n_out ? pm.n_in : pm.n_out) | 1) +
-                                   2 * ((size_t)pm.part_P * pm.part_ch * kCo + 4 * (size_t)pm.part_P * pm.part_P)) >
-                          max_lds_per_block()))
-    return "";
-  if (pm.dir == 1 && (long)ein.n[0] * (ein.n[1] > eout.n[1] ? ein.n[1] : eout.n[1]) * (64 / ein.n[0] + 2) *
-                             (long)sizeof(T) >= (1L << 32))
-    return "";
-  if (check_only) {
-    *done = true;
-    return "";
-  }
-  part_args(M, &a);
-  part_args(G, &b);
   const int dir = pm.dir, CHp = pm.part_ch, Pn = pm.part_P;
   const int nmax = pm.n_in > pm.n_out ? pm.n_in : pm.n_out;
   const size_t P2 = 4 * (size_t)Pn * Pn, cset = (size_t)Pn * CHp * kCo;
+  const size_t lds = max_lds_per_block();
+  const int lsu = nmax | 1;
+  // x lines: 64 whole lines with both plans' coefficients in LDS, else 32
+  // lines with the coefficients read through the vector cache
+  const size_t shm64 = sizeof(T) * ((size_t)64 * lsu + 2 * (cset + P2));
+  const size_t shm32 = sizeof(T) * (size_t)32 * lsu;
+  const int nlw = shm64 <= lds ? 64 : (shm32 <= lds ? 32 : 0);
+  if (sizeof(T) == 8 && CHp != kChunk) return "";
+  if (dir == 0 && (CHp != kChunk || nlw == 0)) return "";
+  if (dir == 1 && (long)ein.n[0] * (ein.n[1] > eout.n[1] ? ein.n[1] : eout.n[1]) * (64 / ein.n[0] + 2) *
+                          (long)sizeof(T) >= (1L << 32))
+    return "";
+  part_args(M, &a);
+  part_args(G, &b);
   if (dir == 0) {
-    // 64 whole lines in LDS with both plans' coefficients, or not at all
-    // (the two operators then run one by one, each on its tuned shape)
-    const int lsu = nmax | 1;
-    const size_t shm = sizeof(T) * ((size_t)64 * lsu + 2 * (cset + P2));
-    if (CHp != kChunk || shm > max_lds_per_block()) return "";
+    // one pass over the tile, or (when timed faster, once per shape) the two
+    // operators one by one on their own tuned shapes
     a.ls = b.ls = lsu;
-    dim3 g((unsigned)((a.nlines + 63) / 64)), bb((unsigned)(64 * Pn));
-    if (!launch_x_tile2<T>(pair, CHp, 64, g, bb, shm, s, a, b)) return "";
-  } else {
-    const long span = (dir == 1) ? (long)ein.n[0] * (ein.n[1] > eout.n[1] ? ein.n[1] : eout.n[1]) *
-                                       (64 / ein.n[0] + 2)
-                                 : 64L;
-    if (span * (long)sizeof(T) >= (1L << 32)) return "";
-    dim3 g((unsigned)((a.nlines + 63) / 64)), bb((unsigned)(64 * Pn));
-    const size_t shm = sizeof(T) * 2 * (cset + P2);
-    if (dir == 1) dispatch_yz2<T, 1>(pair, CHp, g, bb, shm, s, a, b);
-    else dispatch_yz2<T, 2>(pair, CHp, g, bb, shm, s, a, b);
+    auto run = [&](int v, KArgs<T> ka, KArgs<T> kb, T* oa, T* ob, T* scr) -> bool {
+      if (v == 0) {
+        dim3 g((unsigned)((ka.nlines + nlw - 1) / nlw)), bb((unsigned)(nlw * Pn));
+        if (!launch_x_tile2<T>(pair, CHp, nlw, g, bb, nlw == 64 ? shm64 : shm32, s, ka, kb)) return false;
+        return hipGetLastError() == hipSuccess;
+      }
+      LineOut<T> la = oM, lb = oG;
+      la.out = oa;
+      lb.out = ob;
+      la.accumulate = lb.accumulate = ka.accumulate;
+      return launch_line_op<T>(M, ein, eout, li, fM, la, scr, s).empty() &&
+             launch_line_op<T>(G, ein, eout, li, fG, lb, scr, s).empty();
+    };
+    int v = -1;
+    if (scratch) {
+      KArgs<T> ta = a, tb = b;
+      ta.out = tb.out = scratch;  // the candidates write the scratch array, never the outputs
+      ta.accumulate = tb.accumulate = 0;
+      const std::string key = std::string("line_x2/") + (sizeof(T) == 8 ? "f64/" : "f32/") + std::to_string(pair) +
+                              "/" + std::to_string(pm.n_in) + "/" + std::to_string(pm.n_out) + "/" +
+                              std::to_string(a.nlines);
+      if (check_only) {
+        v = tuned_choice(key);
+        if (v < 0) {
+          // not timed yet: time it now (the candidates only write scratch)
+          v = tune(key, 2, s, [&](int vv) { return run(vv, ta, tb, scratch, scratch, scratch); });
+        }
+      } else {
+        v = tuned_choice(key);
+      }
+    }
+    if (v < 0) v = 0;
+    if (check_only) {
+      *passes = v == 0 ? 1 : 2;
+      return "";
+    }
+    if (!run(v, a, b, oM.out, oG.out, scratch)) return std::string("line op launch: ") + hipGetErrorString(hipGetLastError());
+    *passes = v == 0 ? 1 : 2;
+    return "";
   }
+  if (check_only) {
+    *passes = 1;
+    return "";
+  }
+  const long span = (dir == 1) ? (long)ein.n[0] * (ein.n[1] > eout.n[1] ? ein.n[1] : eout.n[1]) *
+                                     (64 / ein.n[0] + 2)
+                               : 64L;
+  if (span * (long)sizeof(T) >= (1L << 32)) return "";
+  dim3 g((unsigned)((a.nlines + 63) / 64)), bb((unsigned)(64 * Pn));
+  const size_t shm = sizeof(T) * 2 * (cset + P2);
+  if (dir == 1) dispatch_yz2<T, 1>(pair, CHp, g, bb, shm, s, a, b);
+  else dispatch_yz2<T, 2>(pair, CHp, g, bb, shm, s, a, b);
   hipError_t he = hipGetLastError();
   if (he != hipSuccess) return std::string("line op launch: ") + hipGetErrorString(he);
-  *done = true;
+  *passes = 1;
   return "";
 }
 
@@ -883,10 +919,11 @@ template std::string upload_plan<double>(const LinePlan&, DevPlan<double>*);
 template std::string upload_plan<float>(const LinePlan&, DevPlan<float>*);
 template std::string launch_line_op2<double>(const DevPlan<double>&, const DevPlan<double>&, const Ext3&,
                                              const Ext3&, const double*, const double*, const double*,
-                                             const LineOut<double>&, const LineOut<double>&, hipStream_t, bool*, bool);
+                                             const LineOut<double>&, const LineOut<double>&, double*, hipStream_t,
+                                             int*, bool);
 template std::string launch_line_op2<float>(const DevPlan<float>&, const DevPlan<float>&, const Ext3&, const Ext3&,
                                             const float*, const float*, const float*, const LineOut<float>&,
-                                            const LineOut<float>&, hipStream_t, bool*, bool);
+                                            const LineOut<float>&, float*, hipStream_t, int*, bool);
 template std::string launch_line_op<double>(const DevPlan<double>&, const Ext3&, const Ext3&, const LineIn<double>&,
                                             const double*, const LineOut<double>&, double*, hipStream_t);
 template std::string launch_wall_rows<double>(const DevPlan<double>&, const Ext3&, const double*, const double*,

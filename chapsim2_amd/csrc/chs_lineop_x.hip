@@ -260,8 +260,9 @@ __global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_
 // k_line_xu: the tile is loaded once, every thread keeps its window in
 // registers, and each operator then runs the RHS, solve and store with its
 // own plan and output (the tile is reused for the first operator's results
-// once every window is read).  Per-plan values after the tile: NLW = 64 both
-// plans' chunk rows and inverses, NLW = 32 only the inverses.
+// once every window is read).  Per-plan values: NLW = 64 both plans' chunk
+// rows and inverses in LDS after the tile; NLW = 32 none (read through the
+// vector cache), so that two workgroups of FP64 lines of 256 rows share a CU.
 template <typename T, int PAIR, int CH, int NLW>
 __global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_per_eu(xu_waves(CH, NLW))))
     k_line_xu2(KArgs<T> P, KArgs<T> Q) {
@@ -273,11 +274,11 @@ __global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_
   const int LS = P.ls, Pn = P.P;
   T* scoP = tile + (long)NLW * LS;
   const int nco = kCoLds ? Pn * CH * kCo : 0;
-  const int nstP = nco + (P.rinv ? 4 * Pn * Pn : 0);
-  const int nstQ = nco + (Q.rinv ? 4 * Pn * Pn : 0);
+  const int nstP = kCoLds ? nco + (P.rinv ? 4 * Pn * Pn : 0) : 0;
+  const int nstQ = kCoLds ? nco + (Q.rinv ? 4 * Pn * Pn : 0) : 0;
   T* scoQ = scoP + nstP;
-  constexpr int NS = kCoLds ? stage_max(CH) : (4 * kMaxChunks + NLW - 1) / NLW;
-  static_assert(NLW * kMaxChunks * NS >= (kCoLds ? kMaxChunks * CH * kCo : 0) + 4 * kMaxChunks * kMaxChunks,
+  constexpr int NS = kCoLds ? stage_max(CH) : 1;
+  static_assert(!kCoLds || NLW * kMaxChunks * NS >= kMaxChunks * CH * kCo + 4 * kMaxChunks * kMaxChunks,
                 "per-plan staging exceeds NS values per thread");
   const int tid = threadIdx.x, nthr = NLW * Pn;
   T cvP[NS], cvQ[NS];
@@ -328,6 +329,8 @@ __global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_
   __syncthreads();
   const T* cP = (kCoLds ? scoP : P.co) + q * CH * kCo;
   const T* cQ = (kCoLds ? scoQ : Q.co) + q * CH * kCo;
+  const T* rP = kCoLds ? scoP + nco : P.rinv;
+  const T* rQ = kCoLds ? scoQ + nco : Q.rinv;
   const long L = L0 + lane;
   const bool live = L < P.nlines;
   const T* row = tile + lane * LS;
@@ -344,10 +347,10 @@ __global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_
   }
   T y[CH];
   chunk_rhs<T, OPA>(P, q, len, f, fbP, y);
-  xu_finish<T, CH, NLW>(P, cP, scoP + nco, tile, lane, q, s0, len, L0, lmax, y, fbP);
+  xu_finish<T, CH, NLW>(P, cP, rP, tile, lane, q, s0, len, L0, lmax, y, fbP);
   __syncthreads();  // the first operator's stores have read the tile
   chunk_rhs<T, OPB>(Q, q, len, f, fbQ, y);
-  xu_finish<T, CH, NLW>(Q, cQ, scoQ + nco, tile, lane, q, s0, len, L0, lmax, y, fbQ);
+  xu_finish<T, CH, NLW>(Q, cQ, rQ, tile, lane, q, s0, len, L0, lmax, y, fbQ);
 }
 
 template <typename T, int IN, int CH, int NLW>
