@@ -1,31 +1,37 @@
-// Compact line operators on gfx950.
+// Compact line operators on gfx950 (host side, y/z kernels and the
+// single-thread form; the x-line tile kernels are in chs_lineop_x.hip).
 //
-// Partitioned kernels (k_line_yz, k_line_xu, k_line_x; lines of up to
-// kChunkLong*kMaxChunks rows).  A line of n rows is cut into P chunks of at
-// most CH rows (CH = kChunk up to kChunk*kMaxChunks rows, kChunkLong beyond)
-// and one thread owns one chunk, so a launch carries P times more threads
-// than lines (a 256x128x256 grid has only 32768-65536 lines per direction,
-// which is too few for 256 CUs).  Each thread evaluates the explicit RHS of
-// its rows from a register window, runs the chunk-local Thomas sweeps with
-// host-factored coefficients, publishes its first and last values through
-// LDS, and corrects its rows with the spike vectors and two rows of the
-// inverse of the 2P x 2P reduced system (the same algebra for wall-bounded
-// and periodic lines; a periodic line's corner entries close the reduced
-// system cyclically).
-//  - y and z lines: a workgroup is 64 neighbouring lines x P chunks; wave q of
-//    the workgroup holds chunk q of 64 lines, so every load and store of a
-//    wavefront is a contiguous 64-element segment along x and the plan's
-//    per-row coefficients are wave-uniform.
-//  - x lines (contiguous in memory) of up to kChunk*kMaxChunks rows: a
-//    workgroup stages 64 whole lines in LDS with coalesced loads (applying
-//    the input transform), wave q works on chunk q of all 64 lines from LDS,
-//    and the results go back through LDS to coalesced stores (k_line_xu; or
-//    256/P lines with skewed rows, k_line_x).  Longer x lines take the y/z
-//    kernel with x addressing (lanes one line apart, rows contiguous).
+// Partitioned kernels (k_line_yz, k_line_yz2, k_line_xu, k_line_xu2,
+// k_line_x; lines of up to kChunkLong*kMaxChunks rows).  A line of n rows is
+// cut into P chunks of at most CH rows (CH = kChunk up to kChunk*kMaxChunks
+// rows, kChunkLong beyond) and one thread owns one chunk, so a launch carries
+// P times more threads than lines (a 256x128x256 grid has only 32768-65536
+// lines per direction, which is too few for 256 CUs).  Each thread evaluates
+// the explicit RHS of its rows from a register window, runs the chunk-local
+// Thomas sweeps with host-factored coefficients, publishes its first and
+// last values through LDS, and corrects its rows with the spike vectors and
+// two rows of the inverse of the 2P x 2P reduced system (the same algebra
+// for wall-bounded and periodic lines; a periodic line's corner entries
+// close the reduced system cyclically).
+//  - y and z lines: a workgroup is 64 (or, single-input operators, 32)
+//    neighbouring lines x P chunks; with 64 lines wave q holds chunk q, so
+//    every load and store of a wavefront is a contiguous 64-element segment
+//    along x and the plan's per-row coefficients are wave-uniform.
+//  - x lines (contiguous in memory) of up to kChunkLong*kMaxChunks rows: a
+//    workgroup stages 64 or 32 whole lines in LDS with coalesced loads
+//    (applying the input transform), works on the chunks from LDS, and
+//    returns the results through LDS to coalesced stores (k_line_xu; or
+//    256/P lines with skewed rows, k_line_x); long x lines may also take the
+//    y/z kernel with x addressing (lanes one line apart, rows contiguous).
+//  - k_line_yz2 / k_line_xu2 apply an interpolation and a first derivative
+//    between the same locations to one input in one pass.
+// The launch shape of each x, y/z and paired-x operator shape is timed once
+// per process (chs_tune.h).
 //
-// Single-thread-per-line kernel (k_line; lines beyond kChunkLong*kMaxChunks rows).  For y- and z-lines consecutive threads own
-// consecutive x positions, so every load and store of a wavefront is a
-// contiguous 64-element segment.  The explicit RHS (src/basics_operations2.f90
+// Single-thread-per-line kernel (k_line; lines beyond kChunkLong*kMaxChunks
+// rows).  For y- and z-lines consecutive threads own consecutive x
+// positions, so every load and store of a wavefront is a contiguous
+// 64-element segment.  The explicit RHS (src/basics_operations2.f90
 // :1980-2719) is evaluated on the fly with a 5-point register window; the four
 // boundary rows use the affine recipes built on the host.  The forward Thomas
 // sweep (src/basics_algorithms.f90:36-66) writes its intermediate values to a
