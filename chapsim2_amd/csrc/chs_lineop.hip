@@ -45,6 +45,7 @@
 #include "chs_tune.h"
 
 #include <cmath>
+#include <cstdint>
 #include <vector>
 
 namespace chs {
@@ -585,6 +586,18 @@ static std::string line_args(const DevPlan<T>& D, const Ext3& ein, const Ext3& e
   return "";
 }
 
+// x tile kernels may move 16 bytes per thread access: every line length a
+// multiple of 16 bytes and every array 16-byte aligned.
+template <typename T>
+static int x_vec_ok(const KArgs<T>& a) {
+  constexpr int VW = 16 / sizeof(T);
+  auto al = [](const void* q) { return q == nullptr || ((uintptr_t)q % 16) == 0; };
+  return (a.n_in % VW == 0 && a.n_out % VW == 0 && a.e1 == a.n_in && a.o1 == a.n_out && al(a.A) && al(a.B) &&
+          al(a.Cm) && al(a.D) && al(a.E) && al(a.out))
+             ? 1
+             : 0;
+}
+
 // The plan runs on the partitioned kernels.
 template <typename T>
 static bool partitioned(const DevPlan<T>& D) {
@@ -652,6 +665,7 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
       const int cap = (int)(skew_lds / (sizeof(T) * (size_t)(ls + 2 * p.part_P)));
       if (nl > cap) nl = cap;
       auto run = [&](int v, KArgs<T> ka) -> bool {
+        ka.vec = x_vec_ok(ka);
         if (v == 0 || v == 1) {
           const int nlw = v == 0 ? 64 : 32;
           const size_t shm = v == 0 ? shm64 : shm32;
@@ -802,6 +816,7 @@ std::string launch_line_op2(const DevPlan<T>& DA, const DevPlan<T>& DB, const Ex
     // operators one by one on their own tuned shapes
     a.ls = b.ls = lsu;
     auto run = [&](int v, KArgs<T> ka, KArgs<T> kb, T* oa, T* ob, T* scr) -> bool {
+      ka.vec = kb.vec = x_vec_ok(ka) && x_vec_ok(kb);
       if (v == 0) {
         dim3 g((unsigned)((ka.nlines + nlw - 1) / nlw)), bb((unsigned)(nlw * Pn));
         if (!launch_x_tile2<T>(pair, CHp, nlw, g, bb, nlw == 64 ? shm64 : shm32, s, ka, kb)) return false;

@@ -37,6 +37,7 @@ struct KArgs {
   T* scratch;
   // partitioned form
   int P, per, ls, nl;
+  int vec;        // x tile kernels: lines read and written 16 bytes per thread access
   const T* rinv;
   const T* swin;  // 4 x (CH + 4) boundary-row window weights (CH = the plan's chunk rows)
   int wrap;       // periodic line: windows wrap around instead of clamping

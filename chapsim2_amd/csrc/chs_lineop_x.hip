@@ -76,6 +76,135 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
   }
 }
 
+// G consecutive values of one x line (G = 16 bytes / sizeof(T) where the
+// lines' lengths and the arrays allow it, else 1).
+template <typename T, int G>
+struct alignas(sizeof(T) * G) Grp {
+  T v[G];
+};
+template <typename T, int G>
+__device__ __forceinline__ Grp<T, G> ld_grp(const T* base, unsigned boff) {
+  return *reinterpret_cast<const Grp<T, G>*>(reinterpret_cast<const char*>(base) + boff);
+}
+template <typename T, int G>
+__device__ __forceinline__ void st_grp(T* base, unsigned boff, const Grp<T, G>& g) {
+  *reinterpret_cast<Grp<T, G>*>(reinterpret_cast<char*>(base) + boff) = g;
+}
+
+// Stage NLW whole x lines (input transform applied) into tile rows of
+// stride LS: group e = l * (n_in / G) + kg of the workgroup's lines, thread
+// tid taking e = tid, tid + nthr, ...; (l, kg) advance by a fixed step, so no
+// division per group.  Batches of UB groups, all loads of a batch issued
+// before any use; the line is clamped so every address is valid.
+template <typename T, int IN, int NLW, int G>
+__device__ __forceinline__ void stage_tile(const KArgs<T>& P, T* tile, long L0, int lmax) {
+  constexpr int UB = G == 1 ? 8 : 4;
+  const int LS = P.ls, n_in = P.n_in, ng = n_in / G;
+  const int tid = threadIdx.x, nthr = NLW * P.P;
+  const T* const Ab = P.A + L0 * P.e1;
+  const T* const Bb = P.B ? P.B + L0 * P.e1 : nullptr;
+  const T* const Cb = P.Cm ? P.Cm + L0 * P.e1 : nullptr;
+  const T* const Db = P.D ? P.D + L0 * P.e1 : nullptr;
+  const T* const Eb = P.E ? P.E + L0 * P.e1 : nullptr;
+  const int dl = nthr / ng, dk = nthr - dl * ng;
+  int l = tid / ng, k = tid - l * ng;
+  const int tin = NLW * ng;
+  for (int b0 = 0; b0 < tin; b0 += nthr * UB) {
+    unsigned o[UB];
+    int ll[UB], kk[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      ll[u] = l;
+      kk[u] = k * G;
+      o[u] = (unsigned)(((l < lmax ? l : lmax) * P.e1 + k * G) * (int)sizeof(T));
+      k += dk;
+      l += dl;
+      if (k >= ng) { k -= ng; ++l; }
+    }
+    T v[UB * G];
+    const T* S = Bb ? Bb : Cb;
+    auto put = [&](T (&dst)[UB * G], const T* base) {
+      Grp<T, G> g[UB];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) g[u] = ld_grp<T, G>(base, o[u]);
+#pragma unroll
+      for (int u = 0; u < UB; ++u)
+#pragma unroll
+        for (int c = 0; c < G; ++c) dst[u * G + c] = g[u].v[c];
+    };
+    if (IN == 1 || (IN == 2 && S)) {
+      T g[UB * G];
+      put(v, Ab);
+      put(g, S);
+      transform_in<T, UB * G>(P, v, g, [&](int t) { return ld_at(Cb, o[t / G] + (unsigned)(sizeof(T) * (t % G))); });
+    } else {
+      put(v, Ab);
+#pragma unroll
+      for (int t = 0; t < UB * G; ++t) v[t] = P.sA * v[t];
+    }
+    if (IN == 2) {
+      T d[UB * G], e[UB * G];
+      put(d, Db);
+      if (Eb) put(e, Eb);
+      else
+#pragma unroll
+        for (int t = 0; t < UB * G; ++t) e[t] = T(0);
+#pragma unroll
+      for (int t = 0; t < UB * G; ++t) v[t] = v[t] + P.cD * d[t] + P.cE * e[t];
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u)
+      if (ll[u] < NLW)
+#pragma unroll
+        for (int c = 0; c < G; ++c) tile[ll[u] * LS + kk[u] + c] = v[u * G + c];
+  }
+}
+
+// The NLW lines of the tile (rows of stride LS) out to P.out with coalesced
+// (accumulating) stores of G values per thread, as stage_tile reads them.
+template <typename T, int NLW, int G>
+__device__ __forceinline__ void store_tile(const KArgs<T>& P, const T* tile, long L0, int lmax) {
+  constexpr int UB = G == 1 ? 8 : 4;
+  const int LS = P.ls, n_out = P.n_out, ng = n_out / G;
+  const int tid = threadIdx.x, nthr = NLW * P.P;
+  const int dl = nthr / ng, dk = nthr - dl * ng;
+  int l = tid / ng, k = tid - l * ng;
+  const int tout = NLW * ng;
+  T* const Ob = P.out + L0 * P.o1;
+  for (int b0 = 0; b0 < tout; b0 += nthr * UB) {
+    unsigned o[UB];
+    bool ok[UB];
+    Grp<T, G> val[UB], prev[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      ok[u] = (l <= lmax);
+      const int lc = l < lmax ? l : lmax;
+      o[u] = (unsigned)((lc * P.o1 + k * G) * (int)sizeof(T));
+#pragma unroll
+      for (int c = 0; c < G; ++c) {
+        val[u].v[c] = tile[lc * LS + k * G + c];
+        prev[u].v[c] = T(0);
+      }
+      k += dk;
+      l += dl;
+      if (k >= ng) { k -= ng; ++l; }
+    }
+    if (P.accumulate) {
+#pragma unroll
+      for (int u = 0; u < UB; ++u) prev[u] = ld_grp<T, G>(Ob, o[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      if (ok[u]) {
+        Grp<T, G> r;
+#pragma unroll
+        for (int c = 0; c < G; ++c) r.v[c] = prev[u].v[c] + val[u].v[c] * P.sOut;
+        st_grp<T, G>(Ob, o[u], r);
+      }
+    }
+  }
+}
+
 // The rest of a staged x chunk after its RHS rows y[]: chunk-local solve,
 // exchange of first/last values through the start of the tile (NLW x 2P,
 // once every window has been read), spike correction, then the chunk's rows
@@ -84,9 +213,7 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
 template <typename T, int CH, int NLW>
 __device__ __forceinline__ void xu_finish(const KArgs<T>& P, const T* c, const T* rinv, T* tile, int lane, int q,
                                           int s0, int len, long L0, int lmax, T (&y)[CH], const T (&fb4)[4]) {
-  constexpr int UB = 8;
-  const int LS = P.ls, Pn = P.P, n_out = P.n_out;
-  const int tid = threadIdx.x, nthr = NLW * Pn;
+  const int LS = P.ls;
   T* exb = tile;
   T* row = tile + lane * LS;
   if (P.solve) {
@@ -103,33 +230,8 @@ __device__ __forceinline__ void xu_finish(const KArgs<T>& P, const T* c, const T
   for (int i = 0; i < CH; ++i)
     if (i < len) row[s0 + i] = chunk_post<T>(P, c, i, s0 + i, y[i], fb4);
   __syncthreads();
-  const int dl = nthr / n_out, dk = nthr - dl * n_out;
-  int l = tid / n_out, k = tid - l * n_out;
-  const int tout = NLW * n_out;
-  T* const Ob = P.out + L0 * P.o1;
-  for (int b0 = 0; b0 < tout; b0 += nthr * UB) {
-    unsigned o[UB];
-    bool ok[UB];
-    T val[UB], prev[UB];
-#pragma unroll
-    for (int u = 0; u < UB; ++u) {
-      ok[u] = (l <= lmax);
-      const int lc = l < lmax ? l : lmax;
-      o[u] = (unsigned)((lc * P.o1 + k) * (int)sizeof(T));
-      val[u] = tile[lc * LS + k];
-      prev[u] = T(0);
-      k += dk;
-      l += dl;
-      if (k >= n_out) { k -= n_out; ++l; }
-    }
-    if (P.accumulate) {
-#pragma unroll
-      for (int u = 0; u < UB; ++u) prev[u] = ld_at<T>(Ob, o[u]);
-    }
-#pragma unroll
-    for (int u = 0; u < UB; ++u)
-      if (ok[u]) st_at<T>(Ob, o[u], prev[u] + val[u] * P.sOut);
-  }
+  if (P.vec) store_tile<T, NLW, 16 / sizeof(T)>(P, tile, L0, lmax);
+  else store_tile<T, NLW, 1>(P, tile, L0, lmax);
 }
 
 // Waves per SIMD the tile kernels are compiled for: the 32-line tile of
@@ -174,63 +276,10 @@ __global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_
   const int q = kCoLds ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid / NLW;
   const long L0 = (long)blockIdx.x * NLW;
   const int n_in = P.n_in, n_out = P.n_out;
-  // staging: element e = l * n_in + k of the workgroup's NLW lines, thread
-  // tid taking e = tid, tid + nthr, ...; (l, k) advance by a fixed step, so
-  // no division per element.  Batches of UB elements, all loads of a batch
-  // issued before any use; the line is clamped so every address is valid.
-  constexpr int UB = 8;
   const long Lmax = P.nlines - 1;
   const int lmax = (int)((Lmax - L0) < NLW - 1 ? (Lmax - L0) : NLW - 1);  // last line of the workgroup
-  const T* const Ab = P.A + L0 * P.e1;
-  const T* const Bb = P.B ? P.B + L0 * P.e1 : nullptr;
-  const T* const Cb = P.Cm ? P.Cm + L0 * P.e1 : nullptr;
-  const T* const Db = P.D ? P.D + L0 * P.e1 : nullptr;
-  const T* const Eb = P.E ? P.E + L0 * P.e1 : nullptr;
-  {
-    const int dl = nthr / n_in, dk = nthr - dl * n_in;
-    int l = tid / n_in, k = tid - l * n_in;
-    const int tin = NLW * n_in;
-    for (int b0 = 0; b0 < tin; b0 += nthr * UB) {
-      unsigned o[UB];
-      int ll[UB], kk[UB];
-#pragma unroll
-      for (int u = 0; u < UB; ++u) {
-        ll[u] = l;
-        kk[u] = k;
-        o[u] = (unsigned)(((l < lmax ? l : lmax) * P.e1 + k) * (int)sizeof(T));
-        k += dk;
-        l += dl;
-        if (k >= n_in) { k -= n_in; ++l; }
-      }
-      T v[UB];
-      const T* S = Bb ? Bb : Cb;
-      if (IN == 1 || (IN == 2 && S)) {
-        T g[UB];
-#pragma unroll
-        for (int u = 0; u < UB; ++u) {
-          v[u] = ld_at(Ab, o[u]);
-          g[u] = ld_at(S, o[u]);
-        }
-        transform_in<T, UB>(P, v, g, [&](int u) { return ld_at(Cb, o[u]); });
-      } else {
-#pragma unroll
-        for (int u = 0; u < UB; ++u) v[u] = P.sA * ld_at(Ab, o[u]);
-      }
-      if (IN == 2) {
-        T d[UB], e[UB];
-#pragma unroll
-        for (int u = 0; u < UB; ++u) {
-          d[u] = ld_at(Db, o[u]);
-          e[u] = Eb ? ld_at(Eb, o[u]) : T(0);
-        }
-#pragma unroll
-        for (int u = 0; u < UB; ++u) v[u] = v[u] + P.cD * d[u] + P.cE * e[u];
-      }
-#pragma unroll
-      for (int u = 0; u < UB; ++u)
-        if (ll[u] < NLW) tile[ll[u] * LS + kk[u]] = v[u];
-    }
-  }
+  if (P.vec) stage_tile<T, IN, NLW, 16 / sizeof(T)>(P, tile, L0, lmax);
+  else stage_tile<T, IN, NLW, 1>(P, tile, L0, lmax);
 #pragma unroll
   for (int u = 0; u < NS; ++u) {
     const int idx = tid + u * nthr;
@@ -292,34 +341,10 @@ __global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_
   const int q = kCoLds ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid / NLW;
   const long L0 = (long)blockIdx.x * NLW;
   const int n_in = P.n_in, n_out = P.n_out;
-  constexpr int UB = 8;
   const long Lmax = P.nlines - 1;
   const int lmax = (int)((Lmax - L0) < NLW - 1 ? (Lmax - L0) : NLW - 1);
-  const T* const Ab = P.A + L0 * P.e1;
-  {
-    const int dl = nthr / n_in, dk = nthr - dl * n_in;
-    int l = tid / n_in, k = tid - l * n_in;
-    const int tin = NLW * n_in;
-    for (int b0 = 0; b0 < tin; b0 += nthr * UB) {
-      unsigned o[UB];
-      int ll[UB], kk[UB];
-#pragma unroll
-      for (int u = 0; u < UB; ++u) {
-        ll[u] = l;
-        kk[u] = k;
-        o[u] = (unsigned)(((l < lmax ? l : lmax) * P.e1 + k) * (int)sizeof(T));
-        k += dk;
-        l += dl;
-        if (k >= n_in) { k -= n_in; ++l; }
-      }
-      T v[UB];
-#pragma unroll
-      for (int u = 0; u < UB; ++u) v[u] = ld_at(Ab, o[u]);
-#pragma unroll
-      for (int u = 0; u < UB; ++u)
-        if (ll[u] < NLW) tile[ll[u] * LS + kk[u]] = v[u];
-    }
-  }
+  if (P.vec) stage_tile<T, 0, NLW, 16 / sizeof(T)>(P, tile, L0, lmax);
+  else stage_tile<T, 0, NLW, 1>(P, tile, L0, lmax);
 #pragma unroll
   for (int u = 0; u < NS; ++u) {
     const int idx = tid + u * nthr;
