@@ -50,9 +50,36 @@ def parse():
     ap.add_argument("--ops", action="store_true",
                     help="isolated operator benchmark instead: the 18 compact operators, d2 along each axis and "
                          "the Poisson solve, FP64 and FP32, on band-limited inputs (use --nc 512 256 512)")
+    ap.add_argument("--tune-to", default=None, metavar="FILE",
+                    help="write the launch shapes the autotuner chose (chs_tune_report) to FILE at the end")
+    ap.add_argument("--tune-from", default=None, metavar="FILE",
+                    help="take the launch shapes listed in FILE (written by --tune-to) instead of timing them, "
+                         "so a run under a profiler uses the shapes of an unprofiled run")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates HBM byte counters)")
     return ap.parse_args()
+
+
+def tune_from(path):
+    """Preset the autotuner with the "key=choice (...)" lines of a report."""
+    from chapsim2_amd import _lib
+
+    L = _lib.load()
+    for line in open(path):
+        key, sep, rest = line.strip().partition("=")
+        if sep and rest:
+            _lib.check(L.chs_tune_set(key.encode(), int(rest.split()[0])))
+
+
+def tune_to(path):
+    import ctypes as C
+
+    from chapsim2_amd import _lib
+
+    buf = C.create_string_buffer(1 << 20)
+    _lib.load().chs_tune_report(buf, len(buf))
+    with open(path, "w") as fh:
+        fh.write(buf.value.decode())
 
 
 def copy_probe(mib):
@@ -366,6 +393,8 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    if args.tune_from:
+        tune_from(args.tune_from)
     if args.ops:
         ops_bench(args)
         return
@@ -462,6 +491,8 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(nc, fields, cfg)
     dm.close()
+    if args.tune_to:
+        tune_to(args.tune_to)
     print(json.dumps(out))
 
 

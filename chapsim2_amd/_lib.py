@@ -22,7 +22,7 @@ EXPORTS = [
     "chs_mlp_create", "chs_mlp_destroy", "chs_mlp_info", "chs_mlp_get_params", "chs_mlp_set_params",
     "chs_mlp_grad", "chs_mlp_train", "chs_mlp_eval",
     "chs_inspect_coeff", "chs_inspect_op1d", "chs_inspect_projection", "chs_profile_begin", "chs_profile_end", "chs_tune_report",
-    "chs_op3d_f32", "chs_poisson_f32", "chs_tune_force",
+    "chs_op3d_f32", "chs_poisson_f32", "chs_tune_force", "chs_tune_set",
 ]
 
 
@@ -115,6 +115,7 @@ def load():
     L.chs_profile_end.argtypes = [vp, dp, C.POINTER(C.c_longlong), dp]
     L.chs_tune_report.argtypes = [C.c_char_p, C.c_int]
     L.chs_tune_force.argtypes = [C.c_char_p, C.c_int]
+    L.chs_tune_set.argtypes = [C.c_char_p, C.c_int]
     for name in EXPORTS:
         if name != "chs_batch_instance":
             getattr(L, name).restype = getattr(L, name).restype or C.c_int

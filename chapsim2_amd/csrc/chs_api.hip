@@ -599,6 +599,12 @@ int chs_tune_report(char* buf, int len) {
   return lines;
 }
 
+int chs_tune_set(const char* key, int choice) {
+  if (!key || choice < 0) return fail(CHS_EINVAL, "invalid argument");
+  chs::tune_set(std::string(key), choice);
+  return CHS_OK;
+}
+
 int chs_tune_force(const char* prefix, int variant) {
   chs::tune_force(prefix ? std::string(prefix) : std::string(), variant);
   return CHS_OK;

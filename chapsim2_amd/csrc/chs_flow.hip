@@ -513,7 +513,9 @@ std::string Flow::op2(RhsWork<T>& w, int lane, int dir, const Field<T>& A, int o
   T* const scr = (lane == 0 || !lanes_on) ? w.scratch : w.scratch_side[lane - 1];
   int passes = 0;
   if (same) SCHK(launch_line_op2<T>(*PA, *PB, A.e, outa.e, A.p, fbca, fbcb, oa, ob, scr, s, &passes, true));
-  if (passes == 0) {
+  // not one pass: the two operators one by one (each launch timed on its own
+  // when profiling)
+  if (passes != 1) {
     auto one = [&](int opk, const int* bc, Field<T>& out, const T* fbc) {
       return op<T>(w, dir, opk, bc, A, out, fbc, T(1), nullptr, T(1), nullptr, T(1), 0, nullptr, T(0), nullptr,
                    T(0), lane);
@@ -525,7 +527,7 @@ std::string Flow::op2(RhsWork<T>& w, int lane, int dir, const Field<T>& A, int o
   const bool timed = (H->prof_class == dir) && H->prof_pair(&e0, &e1);
   if (timed) {
     (void)hipEventRecord(e0, s);
-    H->prof_bytes += (double)sizeof(T) * ((double)A.size() * passes + (double)outa.size() + (double)outb.size());
+    H->prof_bytes += (double)sizeof(T) * ((double)A.size() + (double)outa.size() + (double)outb.size());
   }
   std::string r = launch_line_op2<T>(*PA, *PB, A.e, outa.e, A.p, fbca, fbcb, oa, ob, scr, s, &passes);
   if (timed) (void)hipEventRecord(e1, s);

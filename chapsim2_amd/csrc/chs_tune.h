@@ -27,4 +27,8 @@ std::string tune_report();
 // tests can run each launch shape; variant < 0 restores tuning.
 void tune_force(const std::string& prefix, int variant);
 
+// Presets the choice for `key` as if it had been timed (e.g. from an earlier
+// run's report), so a run under a profiler takes the same launch shapes.
+void tune_set(const std::string& key, int choice);
+
 }  // namespace chs

@@ -24,6 +24,13 @@ void tune_force(const std::string& prefix, int variant) {
   g_force_variant = variant;
 }
 
+void tune_set(const std::string& key, int choice) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Entry e;
+  e.choice = choice;
+  g_cache[key] = e;
+}
+
 int tuned_choice(const std::string& key) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_cache.find(key);

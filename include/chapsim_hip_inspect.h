@@ -71,6 +71,11 @@ int chs_tune_report(char *buf, int len);
  * the oracle; no counterpart in the reference. */
 int chs_tune_force(const char *prefix, int variant);
 
+/* Presets the launch shape of one tuned key (a "key" of chs_tune_report) as
+ * if it had been timed, so that a run under a profiler takes the shapes an
+ * earlier run chose.  No counterpart in the reference. */
+int chs_tune_set(const char *key, int choice);
+
 #ifdef __cplusplus
 }
 #endif
