@@ -1,6 +1,7 @@
 // extern "C" entry points of libchapsim_hip (declared in include/chapsim_hip.h).
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <string>
 
 #include "chs_flow.h"
@@ -371,6 +372,9 @@ int chs_run(chs_handle_t* H, int iter0, int nsteps) {
 // batched channels: independent instances advanced together, each on its own
 // stream, optionally as one captured graph per RK3 step (fork from a main
 // stream to every instance stream, join back)
+// host threads issuing the instances of an eager batch step
+constexpr int kBatchThreads = 8;
+
 struct chs_batch {
   std::vector<chs_handle_t*> inst;
   hipStream_t main = nullptr;
@@ -470,11 +474,29 @@ int chs_batch_run(chs_batch_t* b, int iter0, int nsteps, int use_graph) {
       F.update_re(it);
       if (b->exec && F.rre != b->rre_captured[i]) recapture = true;
     }
-    if (!use_graph || !b->warmed) {
-      // eager step: every instance on its own stream (plans are created here)
+    if (!b->warmed) {
+      // first step: every instance on its own stream, one after the other
+      // (plans are created and launch shapes timed here)
       for (chs_handle_t* h : b->inst)
         for (int isub = 1; isub <= 3; ++isub) RET(h->h.flow->substep(isub));
       b->warmed = true;
+      continue;
+    }
+    if (!use_graph) {
+      // eager step: the instances' launches issued from several host threads
+      // (each instance from one thread, on its own streams)
+      const int n = (int)b->inst.size();
+      const int nt = n < kBatchThreads ? n : kBatchThreads;
+      std::vector<std::string> errs(nt);
+      std::vector<std::thread> th;
+      for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t]() {
+          for (int i = t; i < n && errs[t].empty(); i += nt)
+            for (int isub = 1; isub <= 3 && errs[t].empty(); ++isub) errs[t] = b->inst[i]->h.flow->substep(isub);
+        });
+      for (std::thread& x : th) x.join();
+      for (const std::string& e : errs)
+        if (!e.empty()) return fail(CHS_EHIP, e);
       continue;
     }
     if (recapture) {
