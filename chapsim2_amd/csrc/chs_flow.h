@@ -56,7 +56,10 @@ class Flow {
   std::string get_state(double* qx, double* qy, double* qz, double* pr, int mem);
   void update_re(int iter);
   std::string substep(int isub);
-  std::string momentum_rhs(int isub);  // leaves dt-scaled rhs in acc_x of each component
+  // leaves the dt-scaled rhs in acc_x of each component; with fuse it
+  // also applies the state update (the stored rhs is then only valid for x
+  // of a forced flow)
+  std::string momentum_rhs(int isub, bool fuse = false);
   std::string divergence(double* div_dev);
   std::string mass_check(double out[3]);
   std::string cfl(double out[2]);   // convective CFL and diffusion numbers
@@ -120,10 +123,9 @@ class Flow {
   std::string cov_tau(RhsWork<T>& w, int dir, int opk, const int bc_cov[2], const int bc_tau[2], Field<T>& out,
                       const T* fbc_cov, const T* fbc_tau, const Field<T>& ca, const Field<T>& cb,
                       const Field<T>& ta, T ts, const Field<T>* tb, T tc, int lane = 0);
-  template <typename T> std::string momentum_rhs_t(RhsWork<T>& w, int isub);
+  template <typename T> std::string momentum_rhs_t(RhsWork<T>& w, int isub, bool fuse);
   template <typename T> std::string alloc_work(RhsWork<T>& w, bool alias_state);
   template <typename T> std::string alloc(Field<T>& f, int e1, int e2, int e3);
-  template <typename T> std::string update_state(RhsWork<T>& w);
   std::vector<void*> owned;
 };
 

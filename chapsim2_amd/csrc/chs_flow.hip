@@ -56,11 +56,14 @@ __global__ void k_div3(T* __restrict__ d, const T* __restrict__ a, const T* __re
 
 // rhs = dt * (tg*((ax + ay) + az) + tz*rhs0 + ta*pfc); rhs0 = (ax + ay) + az.
 // Optionally accumulates the x-pencil volume integral of rhs per block (FP64).
+// With q given, the state update of k_update is applied in place of storing
+// rhs: q += rhs, then the wall planes j = 0 and n2 - 1 are set from fbw
+// (extents n1 x n2 x n / (n1 n2), as q's).
 template <typename T>
 __global__ void k_combine(T* __restrict__ ax, const T* __restrict__ ay, const T* __restrict__ az,
                           const T* __restrict__ pfc, T* __restrict__ r0, long n, T tg, T tz, T ta, T dt, int n1,
                           int n2, const double* __restrict__ dyw, double dxdz_x, double dxdz_z,
-                          double* __restrict__ partial) {
+                          double* __restrict__ partial, double* __restrict__ q, const double* __restrict__ fbw) {
   __shared__ double red[256];
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double w = 0.0;
@@ -72,7 +75,18 @@ __global__ void k_combine(T* __restrict__ ax, const T* __restrict__ ay, const T*
     r0[i] = cur;
     tot = tot + ta * pfc[i];
     T rv = dt * tot;
-    ax[i] = rv;
+    if (q) {
+      double v = q[i] + (double)rv;
+      if (fbw) {
+        long i1 = i % n1, rest = i / n1;
+        long j = rest % n2, k = rest / n2;
+        if (j == 0) v = fbw[i1 + (long)n1 * (0 + 4 * k)];
+        else if (j == n2 - 1) v = fbw[i1 + (long)n1 * (1 + 4 * k)];
+      }
+      q[i] = v;
+    } else {
+      ax[i] = rv;
+    }
     if (partial) {
       int j = (int)((i / n1) % n2);
       w = (double)rv * dyw[j] * dxdz_x * dxdz_z;
@@ -613,7 +627,7 @@ std::string Flow::cov_tau(RhsWork<T>& w, int dir, int opk, const int bc_cov[2], 
 // Compute_momentum_rhs + Calculate_momentum_fractional_step + forcing.
 // Result: dt-scaled rhs in w.acc[c][0]; bulk (if forced) in this->bulk.
 template <typename T>
-std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub) {
+std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub, bool fuse) {
   hipStream_t s = H->stream;
   const int(*B)[5][2] = ibc;
   const T* fu = wall_y ? w.fby_u : nullptr;
@@ -698,47 +712,43 @@ std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub) {
   // fractional step (and the forcing's block partials of component x)
   const Mesh& m = H->mesh;
   const bool forced = (H->desc.idriven == 1);
+  // with fuse, components whose update needs no bulk correction are
+  // updated here (x is, when the flow is forced: the correction needs the
+  // whole volume's sum first)
   Field<T>* rhs0[3] = {&w.mx0, &w.my0, &w.mz0};
+  double* qs[3] = {qx.p, qy.p, qz.p};
   for (int c = 0; c < 3; ++c) {
     long n = w.acc[c][0].size();
     double* part = (c == 0 && forced) ? partial : nullptr;
+    double* qf = (fuse && !(c == 0 && forced)) ? qs[c] : nullptr;
+    const double* fbw = (qf && c == 1 && wall_y) ? w64.fby_v : nullptr;
     hipLaunchKernelGGL(k_combine<T>, grid1(n), dim3(256), 0, lane_stream(c), w.acc[c][0].p, w.acc[c][1].p,
                        w.acc[c][2].p, w.acc[c][3].p, rhs0[c]->p, n, T(tg[isub]), T(tz[isub]), T(ta[isub]), T(dt),
-                       w.acc[c][0].e.n[0], w.acc[c][0].e.n[1], dyw_dev, m.h[0], m.h[2], part);
+                       w.acc[c][0].e.n[0], w.acc[c][0].e.n[1], dyw_dev, m.h[0], m.h[2], part, qf, fbw);
   }
-  SCHK(join());
   if (forced) {
     int nb = (int)((w.acc[0][0].size() + 255) / 256);
-    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, s, partial, nb, vol, bulk);
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, lane_stream(0), partial, nb, vol, bulk);
+    if (fuse)
+      hipLaunchKernelGGL(k_update<T>, grid1(qx.size()), dim3(256), 0, lane_stream(0), qx.p, w.acc[0][0].p,
+                         (const double*)bulk, qx.size(), px, ny, (const double*)nullptr);
   }
+  SCHK(join());
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("momentum_rhs: ") + hipGetErrorString(e);
   return "";
 }
 
-std::string Flow::momentum_rhs(int isub) {
+std::string Flow::momentum_rhs(int isub, bool fuse) {
   set_lanes();
-  if (!mixed) return momentum_rhs_t<double>(w64, isub);
+  if (!mixed) return momentum_rhs_t<double>(w64, isub, fuse);
   // rounded copies of the state for the FP32 right-hand side
   hipStream_t s = H->stream;
   hipLaunchKernelGGL((k_cvt<float, double>), grid1(qx.size()), dim3(256), 0, s, w32.qx.p, qx.p, qx.size());
   hipLaunchKernelGGL((k_cvt<float, double>), grid1(qy.size()), dim3(256), 0, s, w32.qy.p, qy.p, qy.size());
   hipLaunchKernelGGL((k_cvt<float, double>), grid1(qz.size()), dim3(256), 0, s, w32.qz.p, qz.p, qz.size());
   hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr.size()), dim3(256), 0, s, w32.pr.p, pr.p, pr.size());
-  return momentum_rhs_t<float>(w32, isub);
-}
-
-template <typename T>
-std::string Flow::update_state(RhsWork<T>& w) {
-  const bool forced = (H->desc.idriven == 1);
-  SCHK(fork());
-  hipLaunchKernelGGL(k_update<T>, grid1(qx.size()), dim3(256), 0, lane_stream(0), qx.p, w.acc[0][0].p,
-                     forced ? bulk : nullptr, qx.size(), px, ny, (const double*)nullptr);
-  hipLaunchKernelGGL(k_update<T>, grid1(qy.size()), dim3(256), 0, lane_stream(1), qy.p, w.acc[1][0].p,
-                     (const double*)nullptr, qy.size(), nx, py, wall_y ? w64.fby_v : nullptr);
-  hipLaunchKernelGGL(k_update<T>, grid1(qz.size()), dim3(256), 0, lane_stream(2), qz.p, w.acc[2][0].p,
-                     (const double*)nullptr, qz.size(), nx, ny, (const double*)nullptr);
-  return join();
+  return momentum_rhs_t<float>(w32, isub, fuse);
 }
 
 std::string Flow::divergence(double* d) {
@@ -751,9 +761,7 @@ std::string Flow::divergence(double* d) {
 
 std::string Flow::substep(int isub) {
   hipStream_t s = H->stream;
-  SCHK(momentum_rhs(isub));
-  if (mixed) SCHK(update_state<float>(w32));
-  else SCHK(update_state<double>(w64));
+  SCHK(momentum_rhs(isub, true));
   // Poisson RHS: div(q) / (alpha * sigma2 * dt), then the solve
   SCHK(divergence(pcor.p));
   const double coeff = 1.0 / (ta[isub] * sigma2p * dt);
