@@ -652,10 +652,13 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
       //     through the vector cache): half the LDS, two workgroups per CU
       //  2: 256/P lines with skewed rows
       //  3: lines of kChunkLong-row chunks only: the lane-per-line kernel
+      //  4: lines of kChunk-row chunks only: 16 whole lines in LDS, four
+      //     chunks per wave (nothing else in LDS), four workgroups per CU
       const size_t lds = max_lds_per_block();
       const int lsu = nmax | 1;  // odd row stride
       const size_t shm64 = sizeof(T) * ((size_t)64 * lsu + (size_t)p.part_P * CHp * kCo + P2);
       const size_t shm32 = sizeof(T) * ((size_t)32 * lsu + P2);
+      const size_t shm16 = sizeof(T) * (size_t)16 * lsu;
       int ls = lds_pos_host(nmax) + 1;
       ls += ((8 - ls) % 32 + 32) % 32;  // row stride = 8 (mod 32) doubles: neighbouring lines in other banks
       // lines per workgroup: at most 256 threads and 48 KiB of LDS (long
@@ -666,9 +669,10 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
       if (nl > cap) nl = cap;
       auto run = [&](int v, KArgs<T> ka) -> bool {
         ka.vec = x_vec_ok(ka);
-        if (v == 0 || v == 1) {
-          const int nlw = v == 0 ? 64 : 32;
-          const size_t shm = v == 0 ? shm64 : shm32;
+        if (v == 0 || v == 1 || v == 4) {
+          if (v == 4 && CHp != kChunk) return false;
+          const int nlw = v == 0 ? 64 : (v == 1 ? 32 : 16);
+          const size_t shm = v == 0 ? shm64 : (v == 1 ? shm32 : shm16);
           if (shm > lds) return false;
           ka.ls = lsu;
           dim3 g((unsigned)((ka.nlines + nlw - 1) / nlw)), b((unsigned)(nlw * p.part_P));
@@ -686,7 +690,7 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
         }
         return hipGetLastError() == hipSuccess;
       };
-      const int nvar = CHp == kChunkLong ? 4 : 3;
+      const int nvar = 5;
       int v = -1;
       if (scratch) {
         KArgs<T> ts = a;

@@ -237,7 +237,7 @@ __device__ __forceinline__ void xu_finish(const KArgs<T>& P, const T* c, const T
 // Waves per SIMD the tile kernels are compiled for: the 32-line tile of
 // 16-row chunks (at most 8 waves) keeps to 128 registers so that two of its
 // workgroups share a CU (one loads while the other computes).
-__host__ __device__ constexpr int xu_waves(int ch, int nlw) { return (nlw == 32 && ch == kChunk) ? 4 : 1; }
+__host__ __device__ constexpr int xu_waves(int ch, int nlw) { return (nlw <= 32 && ch == kChunk) ? 4 : 1; }
 
 // NLW whole lines per workgroup of NLW x P threads in LDS (row stride
 // ls = n | 1), thread tid on line tid % NLW and chunk tid / NLW.  NLW = 64:
@@ -245,11 +245,14 @@ __host__ __device__ constexpr int xu_waves(int ch, int nlw) { return (nlw == 32 
 // tile (wave-uniform reads).  NLW = 32: two chunks per wave and only the
 // reduced-system inverse in LDS, the coefficient rows read through the vector
 // cache; the tile takes half the LDS of the 64-line form, so lines twice as
-// long fit, or two workgroups share a CU.  IN as in load_window.
+// long fit, or two workgroups share a CU.  NLW = 16: four chunks per wave,
+// nothing but the tile in LDS (the inverse too is read through the vector
+// cache), four workgroups per CU.  IN as in load_window.
 template <typename T, int OPK, int IN, int CH, int NLW>
 __global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_per_eu(xu_waves(CH, NLW))))
     k_line_xu(KArgs<T> P) {
   constexpr bool kCoLds = (NLW == 64);
+  constexpr bool kRinvLds = (NLW >= 32);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* tile = reinterpret_cast<T*>(smem_raw);
   const int LS = P.ls, Pn = P.P;
@@ -261,9 +264,10 @@ __global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_
   // staging is done
   T* sco = tile + (long)NLW * LS;
   const int nco = kCoLds ? Pn * CH * kCo : 0;
-  const int nst = nco + (P.rinv ? 4 * Pn * Pn : 0);
-  constexpr int NS = kCoLds ? stage_max(CH) : (4 * kMaxChunks + NLW - 1) / NLW;
-  static_assert(NLW * kMaxChunks * NS >= (kCoLds ? kMaxChunks * CH * kCo : 0) + 4 * kMaxChunks * kMaxChunks,
+  const int nst = nco + ((kRinvLds && P.rinv) ? 4 * Pn * Pn : 0);
+  constexpr int NS = kCoLds ? stage_max(CH) : (kRinvLds ? (4 * kMaxChunks + NLW - 1) / NLW : 1);
+  static_assert(!kRinvLds ||
+                    NLW * kMaxChunks * NS >= (kCoLds ? kMaxChunks * CH * kCo : 0) + 4 * kMaxChunks * kMaxChunks,
                 "per-plan staging exceeds NS values per thread");
   const int tid = threadIdx.x, nthr = NLW * Pn;
   T cv[NS];
@@ -287,7 +291,7 @@ __global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_
   }
   __syncthreads();
   const T* c = (kCoLds ? sco : P.co) + q * CH * kCo;
-  const T* rinv = sco + nco;
+  const T* rinv = kRinvLds ? sco + nco : P.rinv;
   const long L = L0 + lane;
   const bool live = L < P.nlines;
   T* row = tile + lane * LS;
@@ -401,13 +405,14 @@ static void tile_in(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KAr
 
 template <typename T>
 bool launch_x_tile(int op, int ch, int nlw, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
-  if (nlw != 64 && nlw != 32) return false;
+  if (nlw != 64 && nlw != 32 && nlw != 16) return false;
   if (ch == kChunk) {
     if (nlw == 64) tile_in<T, kChunk, 64>(op, g, b, shm, s, a);
-    else tile_in<T, kChunk, 32>(op, g, b, shm, s, a);
+    else if (nlw == 32) tile_in<T, kChunk, 32>(op, g, b, shm, s, a);
+    else tile_in<T, kChunk, 16>(op, g, b, shm, s, a);
     return true;
   }
-  if (ch != kChunkLong) return false;
+  if (ch != kChunkLong || nlw == 16) return false;
   if (nlw == 32) {
     tile_in<T, kChunkLong, 32>(op, g, b, shm, s, a);
     return true;

@@ -265,11 +265,12 @@ def test_poisson_fp32_vs_oracle(chan, po):
     assert 1e-9 < err < 1e-5, err
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_x_launch_shapes_vs_oracle(po, variant):
     """Every x-line launch shape, forced through chs_tune_force (0: 64 whole
     lines in LDS, 1: 32 whole lines in LDS with two chunks per wave, 2: skewed
-    rows, 3: lane-per-line, long lines only; a shape that does not apply
+    rows, 3: lane-per-line, long lines only, 4: 16 whole lines in LDS with
+    four chunks per wave, 16-row chunks only; a shape that does not apply
     falls back to the next that does): all operators on lines of 250 rows
     (16-row chunks), 300 and 512 rows (32-row chunks), CP6 and the explicit
     CD4, then the momentum right-hand side (merged operators reading two to
