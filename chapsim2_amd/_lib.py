@@ -16,7 +16,7 @@ MEM_HOST, MEM_DEVICE = 0, 1
 EXPORTS = [
     "chs_abi_version", "chs_last_error", "chs_create", "chs_destroy", "chs_sync", "chs_op3d",
     "chs_poisson", "chs_set_state", "chs_get_state", "chs_update_re", "chs_substep", "chs_run",
-    "chs_momentum_rhs", "chs_mass_check", "chs_cfl", "chs_bulk", "chs_divergence",
+    "chs_momentum_rhs", "chs_mass_check", "chs_cfl", "chs_bulk", "chs_velocity_range", "chs_divergence",
     "chs_batch_create", "chs_batch_destroy", "chs_batch_instance", "chs_batch_run",
     "chs_props_create", "chs_props_destroy", "chs_props_info", "chs_props_from_rhoh", "chs_props_from_h",
     "chs_mlp_create", "chs_mlp_destroy", "chs_mlp_info", "chs_mlp_get_params", "chs_mlp_set_params",
@@ -87,6 +87,7 @@ def load():
     L.chs_mass_check.argtypes = [vp, dp]
     L.chs_cfl.argtypes = [vp, dp]
     L.chs_bulk.argtypes = [vp, dp]
+    L.chs_velocity_range.argtypes = [vp, dp]
     L.chs_divergence.argtypes = [vp, vp, C.c_int]
     L.chs_inspect_coeff.restype = C.c_double
     L.chs_inspect_coeff.argtypes = [C.c_int] * 6

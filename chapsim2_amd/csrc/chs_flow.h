@@ -64,6 +64,9 @@ class Flow {
   std::string mass_check(double out[3]);
   std::string cfl(double out[2]);   // convective CFL and diffusion numbers
   std::string bulk_monitor(double out[2]);  // bulk mean kinetic energy and bulk qx
+  // max and min of qx, qy, qz (out[2c], out[2c+1]) and the count of
+  // non-finite values among them
+  std::string velocity_range(double out[6], long* nonfinite);
   std::string cell_velocity();      // P2C interpolations of qx, qy, qz into w64.qxix, qyiy, qziz
   std::string copy_rhs(double* mx, double* my, double* mz, int mem);
 

@@ -256,6 +256,39 @@ __global__ void k_cfl_max(const double* __restrict__ u, const double* __restrict
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// Block maxima and minima of one field and its count of non-finite values
+// (partial[b], partial[nb + b], partial[2 nb + b] for block b of nb).  The
+// comparisons skip NaN and start from -1e20 / 1e20, as the loops of
+// Find_max_min_3d (src/tools_general.f90:1071-1113) do; max and min are
+// exact, so the block order does not matter.
+__global__ void k_range(const double* __restrict__ a, long n, double* __restrict__ partial) {
+  __shared__ double smax[256], smin[256], scnt[256];
+  double mx = -1.0e20, mn = 1.0e20, cnt = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double v = a[i];
+    if (v > mx) mx = v;
+    if (v < mn) mn = v;
+    if (!isfinite(v)) cnt += 1.0;
+  }
+  smax[threadIdx.x] = mx;
+  smin[threadIdx.x] = mn;
+  scnt[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      if (smax[threadIdx.x + s] > smax[threadIdx.x]) smax[threadIdx.x] = smax[threadIdx.x + s];
+      if (smin[threadIdx.x + s] < smin[threadIdx.x]) smin[threadIdx.x] = smin[threadIdx.x + s];
+      scnt[threadIdx.x] = scnt[threadIdx.x] + scnt[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = smax[0];
+    partial[gridDim.x + blockIdx.x] = smin[0];
+    partial[2 * gridDim.x + blockIdx.x] = scnt[0];
+  }
+}
+
 // Block sums of f(i) dyw(j) over an x-pencil array (x fastest, j = row
 // along y), f = a (b, c null) or 1/2 (a^2 + b^2 + c^2): the volume integrals
 // of Get_volumetric_average_3d_for_var_xcx (src/tools_general.f90:1202-1268)
@@ -444,7 +477,7 @@ std::string Flow::create(Handle* H, std::unique_ptr<Flow>* out) {
   SCHK(f.alloc_work(f.w64, true));
   if (f.mixed) SCHK(f.alloc_work(f.w32, false));
   {
-    long nblk = std::max<long>((f.qx.size() + 255) / 256, kMaxPartials);
+    long nblk = std::max<long>((f.qx.size() + 255) / 256, 3 * kMaxPartials);
     f.npartial = nblk;
     FCHK(hipMalloc(&f.partial, sizeof(double) * (size_t)(nblk + 8)));
     f.owned.push_back(f.partial);
@@ -816,6 +849,28 @@ std::string Flow::bulk_monitor(double out[2]) {
     for (int j = 0; j < ny; ++j) sy += dyw[j];
     out[q] = sum / ((double)n1[q] * nz * sy);
   }
+  return "";
+}
+
+std::string Flow::velocity_range(double out[6], long* nonfinite) {
+  const int nb = kMaxPartials;
+  std::vector<double> h(3 * (size_t)nb);
+  const Field<double>* q[3] = {&qx, &qy, &qz};
+  long bad = 0;
+  for (int c = 0; c < 3; ++c) {
+    hipLaunchKernelGGL(k_range, dim3(nb), dim3(256), 0, H->stream, q[c]->p, q[c]->size(), partial);
+    FCHK(hipMemcpyAsync(h.data(), partial, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, H->stream));
+    FCHK(hipStreamSynchronize(H->stream));
+    double mx = -1.0e20, mn = 1.0e20;
+    for (int b = 0; b < nb; ++b) {
+      if (h[b] > mx) mx = h[b];
+      if (h[nb + b] < mn) mn = h[nb + b];
+      bad += (long)h[2 * nb + b];
+    }
+    out[2 * c] = mx;
+    out[2 * c + 1] = mn;
+  }
+  *nonfinite = bad;
   return "";
 }
 

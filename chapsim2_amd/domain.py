@@ -131,6 +131,19 @@ class Domain:
         _lib.check(self.lib.chs_bulk(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def velocity_range(self, check=True):
+        """[max qx, min qx, max qy, min qy, max qz, min qz] of the current
+        state, the per-step check of Find_max_min_3d
+        (src/tools_general.f90:1071-1113).  The library reports NaN or
+        infinite velocity values as CHS_ENAN: with ``check`` that raises
+        ChsError; without it the return code comes back as the second item."""
+        out = np.zeros(6)
+        rc = self.lib.chs_velocity_range(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        if check:
+            _lib.check(rc)
+            return out
+        return out, rc
+
     def divergence(self):
         out = np.zeros(self.shapes["ccc"], order="F")
         _lib.check(self.lib.chs_divergence(self.h, out.ctypes.data, _lib.MEM_HOST))

@@ -40,6 +40,7 @@ module chapsim_hip
 
   public :: chs_abi_version, chs_create, chs_destroy, chs_sync, chs_op3d, chs_poisson
   public :: chs_set_state, chs_get_state, chs_update_re, chs_substep, chs_run, chs_mass_check, chs_cfl, chs_bulk
+  public :: chs_velocity_range
   public :: chs_divergence, chs_inspect_coeff, chs_inspect_op1d, chs_error_message, chs_check
   public :: Get_x_midp_C2P_3D, Get_x_midp_P2C_3D, Get_x_1der_C2C_3D, Get_x_1der_P2P_3D, &
             Get_x_1der_C2P_3D, Get_x_1der_P2C_3D, Get_y_midp_C2P_3D, Get_y_midp_P2C_3D, &
@@ -126,6 +127,12 @@ module chapsim_hip
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(out) :: out(2)
+    end function
+    ! Find_max_min_3d of qx, qy, qz (src/tools_general.f90:1071-1113); CHS_ENAN on non-finite values
+    integer(c_int) function chs_velocity_range(h, out) bind(C, name="chs_velocity_range")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out(6)
     end function
     integer(c_int) function chs_divergence(h, div, mem) bind(C, name="chs_divergence")
       import :: c_int, c_ptr, c_double

@@ -151,6 +151,14 @@ int chs_cfl(chs_handle_t *h, double out[2]);
  * src/tools_general.f90:1202-1268). */
 int chs_bulk(chs_handle_t *h, double out[2]);
 
+/* out = {max qx, min qx, max qy, min qy, max qz, min qz} of the current
+ * state: the per-step check of Find_max_min_3d (src/tools_general.f90:
+ * 1071-1113, called at src/chapsim.f90:329-331), whose comparisons skip NaN
+ * and start from -1e20 / 1e20.  Returns CHS_ENAN (with out filled) when any
+ * velocity value is NaN or infinite, so a caller can stop a run that has
+ * blown up. */
+int chs_velocity_range(chs_handle_t *h, double out[6]);
+
 /* Divergence of the current velocity, (nc1,nc2,nc3).  Replaces
  * Get_divergence_flow (src/eq_continuity.f90:91-161). */
 int chs_divergence(chs_handle_t *h, double *div, int mem);

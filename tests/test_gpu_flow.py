@@ -245,6 +245,38 @@ def test_cfl_vs_oracle(pair):
         assert r[0] > 0
 
 
+def test_velocity_range_and_nonfinite_detection(pair):
+    """chs_velocity_range (the per-step Find_max_min_3d of qx, qy, qz) is
+    exact: the max and min numpy takes of the state, before and after an RK3
+    step.  A NaN or an infinity planted in the state makes it return
+    CHS_ENAN with the range of the rest (NaN skipped, an infinity taken as
+    the extreme it is), and the Python wrapper raise."""
+    from chapsim2_amd import _lib
+
+    dm, _ = pair
+
+    def expect(qx, qy, qz):
+        return [np.nanmax(qx), np.nanmin(qx), np.nanmax(qy), np.nanmin(qy), np.nanmax(qz), np.nanmin(qz)]
+
+    qx, qy, qz, pr = dm.get_state()
+    assert np.array_equal(dm.velocity_range(), expect(qx, qy, qz))
+    dm.run(1, 1)
+    s = dm.get_state()
+    assert np.array_equal(dm.velocity_range(), expect(*s[:3]))
+    qy2 = np.array(qy, order="F")
+    qy2[3, 5, 7] = np.nan
+    qz2 = np.array(qz, order="F")
+    qz2[1, 2, 3] = -np.inf
+    dm.set_state(qx, qy2, qz2, pr)
+    out, rc = dm.velocity_range(check=False)
+    assert rc == _lib.CHS_ENAN
+    assert np.array_equal(out, expect(qx, qy2, qz2)) and out[5] == -np.inf
+    with pytest.raises(_lib.ChsError):
+        dm.velocity_range()
+    dm.set_state(qx, qy, qz, pr)
+    assert np.array_equal(dm.velocity_range(), expect(qx, qy, qz))
+
+
 def test_bulk_monitor_vs_oracle(pair):
     """chs_bulk (bulk mean kinetic energy of the cell-centred velocity and
     bulk qx, the values of write_monitor_bulk) against the oracle on the
