@@ -403,6 +403,9 @@ static std::string batch_capture(chs_batch* b) {
   for (size_t i = 0; i < b->inst.size() && err.empty(); ++i) {
     chs::Handle& h = b->inst[i]->h;
     if (hipStreamWaitEvent(h.stream, b->fork, 0) != hipSuccess) { err = "fork wait"; break; }
+    // a replay may follow a set_state: the captured step converts the state
+    // it starts from
+    h.flow->q32_fresh = false;
     for (int isub = 1; isub <= 3 && err.empty(); ++isub) err = h.flow->substep(isub);
     if (err.empty() && hipEventRecord(b->join[i], h.stream) != hipSuccess) err = "join record";
     if (err.empty() && hipStreamWaitEvent(b->main, b->join[i], 0) != hipSuccess) err = "join wait";

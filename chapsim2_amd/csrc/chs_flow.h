@@ -74,6 +74,11 @@ class Flow {
   int nx, ny, nz, px, py, pz;
   bool wall_y = true;
   bool mixed = false;   // momentum right-hand side in FP32 (CHS_PREC_MIXED)
+  // mixed form: the FP32 copies of qx, qy, qz, pr (w32) equal the rounded
+  // state.  A sub-step leaves them so (its corrections and pressure update
+  // store both); set_state clears it, and so does the start of a graph
+  // capture, so that a captured step converts the state it is replayed on.
+  bool q32_fresh = false;
   double rre = 1.0, dt = 0.0, sigma2p = 1.0;
   double tg[4], tz[4], ta[4];
   int ibc[3][5][2];     // [dir][u,v,w,p,T][side]
@@ -113,7 +118,7 @@ class Flow {
   std::string op(RhsWork<T>& w, int dir, int opk, const int ibc2[2], const Field<T>& A, Field<T>& out,
                  const T* fbc, T sA = T(1), const Field<T>* B = nullptr, T cB = T(1), const Field<T>* Cm = nullptr,
                  T sOut = T(1), int accumulate = 0, const Field<T>* D = nullptr, T cD = T(0),
-                 const Field<T>* E = nullptr, T cE = T(0), int lane = 0);
+                 const Field<T>* E = nullptr, T cE = T(0), int lane = 0, float* out32 = nullptr);
   // the interpolation and the first derivative of A between the same
   // locations (plans opa with bca / fbca and opb with bcb / fbcb) in one
   // pass where the line kernels allow it, else one after the other

@@ -158,17 +158,29 @@ __global__ void k_cvt(TO* __restrict__ o, const TI* __restrict__ a, long n) {
   if (i < n) o[i] = (TO)a[i];
 }
 
-__global__ void k_add(double* __restrict__ a, const double* __restrict__ b, long n) {
+// a += b (and, with a32, its FP32 rounding into a32)
+__global__ void k_add(double* __restrict__ a, const double* __restrict__ b, long n, float* __restrict__ a32) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = a[i] + b[i];
+  if (i < n) {
+    const double v = a[i] + b[i];
+    a[i] = v;
+    if (a32) a32[i] = (float)v;
+  }
 }
 
-__global__ void k_enforce_walls(double* __restrict__ q, int n1, int n2, int n3, const double* __restrict__ fbw) {
+__global__ void k_enforce_walls(double* __restrict__ q, int n1, int n2, int n3, const double* __restrict__ fbw,
+                                float* __restrict__ q32) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)n1 * n3) return;
   long i = t % n1, k = t / n1;
-  q[i + (long)n1 * (0 + (long)n2 * k)] = fbw[i + (long)n1 * (0 + 4 * k)];
-  q[i + (long)n1 * ((n2 - 1) + (long)n2 * k)] = fbw[i + (long)n1 * (1 + 4 * k)];
+  const long o0 = i + (long)n1 * (0 + (long)n2 * k), o1 = i + (long)n1 * ((n2 - 1) + (long)n2 * k);
+  const double v0 = fbw[i + (long)n1 * (0 + 4 * k)], v1 = fbw[i + (long)n1 * (1 + 4 * k)];
+  q[o0] = v0;
+  q[o1] = v1;
+  if (q32) {
+    q32[o0] = (float)v0;
+    q32[o1] = (float)v1;
+  }
 }
 
 // planes (n1,2,n3) <- a(:,0,:), a(:,n2-1,:)
@@ -504,7 +516,7 @@ void Flow::update_re(int iter) {
 template <typename T>
 std::string Flow::op(RhsWork<T>& w, int dir, int opk, const int ibc2[2], const Field<T>& A, Field<T>& out,
                      const T* fbc, T sA, const Field<T>* B, T cB, const Field<T>* Cm, T sOut, int accumulate,
-                     const Field<T>* D, T cD, const Field<T>* E, T cE, int lane) {
+                     const Field<T>* D, T cD, const Field<T>* E, T cE, int lane, float* out32) {
   const DevPlan<T>* P = nullptr;
   SCHK(H->plan<T>(dir, opk, H->desc.iacc, ibc2, fbc != nullptr, A.e.n[dir], out.e.n[dir], &P));
   LineIn<T> in;
@@ -521,6 +533,7 @@ std::string Flow::op(RhsWork<T>& w, int dir, int opk, const int ibc2[2], const F
   o.out = out.p;
   o.sOut = sOut;
   o.accumulate = accumulate;
+  o.out32 = out32;
   if (out.size() > w.scratch_n) return "flow: scratch too small";
   const hipStream_t s = lane_stream(lane);
   T* const scr = (lane == 0 || !lanes_on) ? w.scratch : w.scratch_side[lane - 1];
@@ -530,7 +543,8 @@ std::string Flow::op(RhsWork<T>& w, int dir, int opk, const int ibc2[2], const F
     (void)hipEventRecord(e0, s);
     const int nin_arrays = 1 + (B ? 1 : 0) + (Cm && Cm->p != A.p ? 1 : 0) + (D ? 1 : 0) + (D && E ? 1 : 0);
     H->prof_bytes += (double)sizeof(T) *
-                     ((double)A.size() * nin_arrays + (double)out.size() * (1 + (accumulate ? 1 : 0)));
+                     ((double)A.size() * nin_arrays + (double)out.size() * (1 + (accumulate ? 1 : 0))) +
+                 (out32 ? 4.0 * (double)out.size() : 0.0);
   }
   std::string r = launch_line_op<T>(*P, A.e, out.e, in, fbc, o, scr, s);
   if (timed) (void)hipEventRecord(e1, s);
@@ -599,6 +613,7 @@ std::string Flow::join() {
 }
 
 std::string Flow::set_state(const double* hx, const double* hy, const double* hz, const double* hp, int mem) {
+  q32_fresh = false;
   hipMemcpyKind k = (mem == CHS_MEM_HOST) ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
   FCHK(hipMemcpyAsync(qx.p, hx, sizeof(double) * qx.size(), k, H->stream));
   FCHK(hipMemcpyAsync(qy.p, hy, sizeof(double) * qy.size(), k, H->stream));
@@ -775,13 +790,19 @@ std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub, bool fuse) {
 std::string Flow::momentum_rhs(int isub, bool fuse) {
   set_lanes();
   if (!mixed) return momentum_rhs_t<double>(w64, isub, fuse);
-  // rounded copies of the state for the FP32 right-hand side
-  hipStream_t s = H->stream;
-  hipLaunchKernelGGL((k_cvt<float, double>), grid1(qx.size()), dim3(256), 0, s, w32.qx.p, qx.p, qx.size());
-  hipLaunchKernelGGL((k_cvt<float, double>), grid1(qy.size()), dim3(256), 0, s, w32.qy.p, qy.p, qy.size());
-  hipLaunchKernelGGL((k_cvt<float, double>), grid1(qz.size()), dim3(256), 0, s, w32.qz.p, qz.p, qz.size());
-  hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr.size()), dim3(256), 0, s, w32.pr.p, pr.p, pr.size());
-  return momentum_rhs_t<float>(w32, isub, fuse);
+  // rounded copies of the state for the FP32 right-hand side, unless the
+  // last sub-step left them
+  if (!q32_fresh) {
+    hipStream_t s = H->stream;
+    hipLaunchKernelGGL((k_cvt<float, double>), grid1(qx.size()), dim3(256), 0, s, w32.qx.p, qx.p, qx.size());
+    hipLaunchKernelGGL((k_cvt<float, double>), grid1(qy.size()), dim3(256), 0, s, w32.qy.p, qy.p, qy.size());
+    hipLaunchKernelGGL((k_cvt<float, double>), grid1(qz.size()), dim3(256), 0, s, w32.qz.p, qz.p, qz.size());
+    hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr.size()), dim3(256), 0, s, w32.pr.p, pr.p, pr.size());
+    q32_fresh = true;
+  }
+  const std::string r = momentum_rhs_t<float>(w32, isub, fuse);
+  if (fuse) q32_fresh = false;  // the state was updated
+  return r;
 }
 
 std::string Flow::divergence(double* d) {
@@ -800,22 +821,25 @@ std::string Flow::substep(int isub) {
   const double coeff = 1.0 / (ta[isub] * sigma2p * dt);
   if (H->desc.projection == CHS_PROJ_EXACT) SCHK(H->poisson->solve_exact(pcor.p, s, coeff));
   else SCHK(H->poisson->solve(pcor.p, s, coeff));
-  hipLaunchKernelGGL(k_add, grid1(pr.size()), dim3(256), 0, s, pr.p, pcor.p, pr.size());
+  // the mixed form's FP32 copies of the new pressure and velocity are
+  // stored by the kernels that make them
+  hipLaunchKernelGGL(k_add, grid1(pr.size()), dim3(256), 0, s, pr.p, pcor.p, pr.size(), mixed ? w32.pr.p : nullptr);
   // Correct_massflux, one lane per velocity component
   const double sc = dt * ta[isub] * sigma2p;
   SCHK(fork());
   SCHK(op<double>(w64, 0, D1_C2P, ibc[0][VP], pcor, qx, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1, nullptr, 0.0,
-                  nullptr, 0.0, 0));
+                  nullptr, 0.0, 0, mixed ? w32.qx.p : nullptr));
   SCHK(op<double>(w64, 1, D1_C2P, ibc[1][VP], pcor, qy, wall_y ? w64.fby_p : nullptr, 1.0, nullptr, 1.0, nullptr,
-                  -sc, 1, nullptr, 0.0, nullptr, 0.0, 1));
+                  -sc, 1, nullptr, 0.0, nullptr, 0.0, 1, mixed ? w32.qy.p : nullptr));
   SCHK(op<double>(w64, 2, D1_C2P, ibc[2][VP], pcor, qz, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1, nullptr, 0.0,
-                  nullptr, 0.0, 2));
+                  nullptr, 0.0, 2, mixed ? w32.qz.p : nullptr));
   if (wall_y)
     hipLaunchKernelGGL(k_enforce_walls, grid1((long)nx * nz), dim3(256), 0, lane_stream(1), qy.p, nx, py, nz,
-                       w64.fby_v);
+                       w64.fby_v, mixed ? w32.qy.p : nullptr);
   SCHK(join());
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("substep: ") + hipGetErrorString(e);
+  q32_fresh = mixed;
   // every operator of a sub-step is planned and tuned now: later sub-steps
   // may overlap independent operators
   concurrent = (side[0] != nullptr);

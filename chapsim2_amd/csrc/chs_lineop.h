@@ -75,6 +75,7 @@ struct LineOut {
   T* out = nullptr;
   T sOut = T(1);
   int accumulate = 0;
+  float* out32 = nullptr;  // also store each result rounded to FP32 here (single operators)
 };
 
 // Extents of the input / output arrays (Fortran order, x fastest).

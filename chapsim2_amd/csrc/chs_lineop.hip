@@ -213,7 +213,9 @@ __global__ void __launch_bounds__(64) k_line(KArgs<T> P) {
     if (r == n_out - 1 && P.neu_hi) val = fb4[1];
     const long o = ob + os * r;
     T prev = P.accumulate ? P.out[o] : T(0);
-    P.out[o] = prev + val * P.sOut;
+    const T v = prev + val * P.sOut;
+    P.out[o] = v;
+    if (P.out32) P.out32[o] = (float)v;
   };
   using O = Offs<OPK>;
   const int m = (P.solve == 2) ? n_out - 1 : n_out;
@@ -289,8 +291,9 @@ __device__ __forceinline__ void yz_offsets(const KArgs<T>& P, int L, long& ib, l
 // The rest of a y/z chunk after its RHS rows y[]: chunk-local solve, the
 // exchange of first/last values through ex (NLW lanes x 2P) and the spike
 // correction (plans with a line solve), then metric scaling, Neumann rows and
-// the (accumulating) store of the chunk's rows at Ob + os * row + ro.
-template <typename T, int CH, int NLW = 64>
+// the (accumulating) store of the chunk's rows at Ob + os * row + ro (and,
+// F32 and P.out32 set, their FP32 rounding at the same place of P.out32).
+template <typename T, int CH, int NLW = 64, bool F32 = false>
 __device__ __forceinline__ void yz_finish(const KArgs<T>& P, const T* c, const T* rinv, T* ex, int lane, int q,
                                           int s0, int len, bool live, T* Ob, unsigned ro, long os, T (&y)[CH],
                                           const T (&fb4)[4]) {
@@ -304,6 +307,9 @@ __device__ __forceinline__ void yz_finish(const KArgs<T>& P, const T* c, const T
   }
   if (!live) return;
   const int n_out = P.n_out;
+  // the FP32 copy: same element offsets, byte offsets scaled to 4-byte values
+  float* const Ob32 = (F32 && P.out32) ? P.out32 + (Ob - P.out) : nullptr;
+  const unsigned ro32 = ro / (unsigned)(sizeof(T) / sizeof(float));
   // accumulating outputs read the previous values in two batches of half a
   // chunk (fewer registers held across the loads)
   constexpr int H = CH / 2;
@@ -324,7 +330,9 @@ __device__ __forceinline__ void yz_finish(const KArgs<T>& P, const T* c, const T
       if (h + i < len) {
         const int r = s0 + h + i;
         const T val = chunk_post<T>(P, c, h + i, r, y[h + i], fb4);
-        st_at<T>(Ob + os * r, ro, prev[i] + val * P.sOut);
+        const T v = prev[i] + val * P.sOut;
+        st_at<T>(Ob + os * r, ro, v);
+        if (F32 && Ob32) st_at<float>(Ob32 + os * r, ro32, (float)v);
       }
     }
   }
@@ -405,7 +413,7 @@ __global__ void __launch_bounds__(NLW * kMaxChunks)
   const T* rinv = sco + Pn * CH * kCo;
   T y[CH];
   chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
-  yz_finish<T, CH, NLW>(P, c, rinv, ex, lane, q, s0, len, live, P.out + ob0, ro, os, y, fb4);
+  yz_finish<T, CH, NLW, IN == 0 && sizeof(T) == 8>(P, c, rinv, ex, lane, q, s0, len, live, P.out + ob0, ro, os, y, fb4);
 }
 
 // Two operators of one line plan geometry (the interpolation and the first
@@ -579,6 +587,7 @@ static std::string line_args(const DevPlan<T>& D, const Ext3& ein, const Ext3& e
   a.an = T(p.cyc_an); a.cn = T(p.cyc_cn); a.den = T(p.cyc_den);
   a.metric = D.metric; a.neu_lo = p.neu_lo; a.neu_hi = p.neu_hi;
   a.out = out.out; a.sOut = out.sOut; a.accumulate = out.accumulate;
+  a.out32 = out.out32;
   a.scratch = scratch;
   if (a.nlines >= (1L << 31) - 64) return "line op: too many lines";
   if ((long)ein.n[0] * ein.n[1] * ein.n[2] >= (1L << 31) || (long)eout.n[0] * eout.n[1] * eout.n[2] >= (1L << 31))
@@ -628,6 +637,8 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
     if (!e.empty()) return e;
   }
   if (a.nlines <= 0) return "";
+  if (a.out32 && (sizeof(T) != 8 || input_kind(a) != 0))
+    return "line op: FP32 copies of the output are for one-input FP64 operators";
   if (partitioned(D)) {
     part_args(D, &a);
     if (p.solve && !D.rinv) return "line op: plan solves but has no reduced-system inverse";
@@ -696,6 +707,7 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
         KArgs<T> ts = a;
         ts.out = scratch;  // the candidates write the scratch array, never the output
         ts.accumulate = 0;
+        ts.out32 = nullptr;
         const std::string key = std::string("line_x/") + (sizeof(T) == 8 ? "f64/" : "f32/") + std::to_string(p.op) +
                                 "/" + std::to_string(p.n_in) + "/" + std::to_string(p.n_out) + "/" +
                                 std::to_string(a.nlines) + "/" + std::to_string(input_kind(a));
@@ -732,6 +744,7 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
         KArgs<T> ts = a;
         ts.out = scratch;  // the candidates write the scratch array, never the output
         ts.accumulate = 0;
+        ts.out32 = nullptr;
         const std::string key = std::string("line_yz/") + (sizeof(T) == 8 ? "f64/" : "f32/") +
                                 std::to_string(dir) + "/" + std::to_string(p.op) + "/" + std::to_string(p.n_in) +
                                 "/" + std::to_string(p.n_out) + "/" + std::to_string(a.nlines) + "/" +
@@ -788,6 +801,7 @@ std::string launch_line_op2(const DevPlan<T>& DA, const DevPlan<T>& DB, const Ex
   if (!partitioned(M) || !partitioned(G) || pm.part_P != pg.part_P || pm.part_ch != pg.part_ch) return "";
   if ((pm.ibc[0] == BC_PERIODIC) != (pg.ibc[0] == BC_PERIODIC)) return "";
   if ((pm.solve && !M.rinv) || (pg.solve && !G.rinv)) return "";
+  if (outA.out32 || outB.out32) return "";  // FP32 copies: single operators only
   LineIn<T> li;
   li.A = in;
   KArgs<T> a, b;

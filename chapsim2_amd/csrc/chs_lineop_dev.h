@@ -34,6 +34,7 @@ struct KArgs {
   T an, cn, den;
   const T* metric; int neu_lo, neu_hi;
   T* out; T sOut; int accumulate;
+  float* out32;   // optional FP32 copy of every value stored to out (same layout)
   T* scratch;
   // partitioned form
   int P, per, ls, nl;

@@ -72,7 +72,9 @@ __global__ void __launch_bounds__(256) k_line_x(KArgs<T> P) {
     const long o = Lw * P.o1 + k;
     const T val = tile[(long)l * LS + lds_pos(k)];
     const T prev = P.accumulate ? P.out[o] : T(0);
-    P.out[o] = prev + val * P.sOut;
+    const T v = prev + val * P.sOut;
+    P.out[o] = v;
+    if (P.out32) P.out32[o] = (float)v;
   }
 }
 
@@ -162,7 +164,7 @@ __device__ __forceinline__ void stage_tile(const KArgs<T>& P, T* tile, long L0, 
 
 // The NLW lines of the tile (rows of stride LS) out to P.out with coalesced
 // (accumulating) stores of G values per thread, as stage_tile reads them.
-template <typename T, int NLW, int G>
+template <typename T, int NLW, int G, bool F32 = false>
 __device__ __forceinline__ void store_tile(const KArgs<T>& P, const T* tile, long L0, int lmax) {
   constexpr int UB = G == 1 ? 8 : 4;
   const int LS = P.ls, n_out = P.n_out, ng = n_out / G;
@@ -171,6 +173,7 @@ __device__ __forceinline__ void store_tile(const KArgs<T>& P, const T* tile, lon
   int l = tid / ng, k = tid - l * ng;
   const int tout = NLW * ng;
   T* const Ob = P.out + L0 * P.o1;
+  float* const Ob32 = (F32 && P.out32) ? P.out32 + L0 * P.o1 : nullptr;
   for (int b0 = 0; b0 < tout; b0 += nthr * UB) {
     unsigned o[UB];
     bool ok[UB];
@@ -200,6 +203,12 @@ __device__ __forceinline__ void store_tile(const KArgs<T>& P, const T* tile, lon
 #pragma unroll
         for (int c = 0; c < G; ++c) r.v[c] = prev[u].v[c] + val[u].v[c] * P.sOut;
         st_grp<T, G>(Ob, o[u], r);
+        if (F32 && Ob32) {
+          Grp<float, G> r32;
+#pragma unroll
+          for (int c = 0; c < G; ++c) r32.v[c] = (float)r.v[c];
+          st_grp<float, G>(Ob32, o[u] / (unsigned)(sizeof(T) / sizeof(float)), r32);
+        }
       }
     }
   }
@@ -210,7 +219,7 @@ __device__ __forceinline__ void store_tile(const KArgs<T>& P, const T* tile, lon
 // once every window has been read), spike correction, then the chunk's rows
 // back into the tile and the NLW lines out with coalesced (accumulating)
 // stores.
-template <typename T, int CH, int NLW>
+template <typename T, int CH, int NLW, bool F32 = false>
 __device__ __forceinline__ void xu_finish(const KArgs<T>& P, const T* c, const T* rinv, T* tile, int lane, int q,
                                           int s0, int len, long L0, int lmax, T (&y)[CH], const T (&fb4)[4]) {
   const int LS = P.ls;
@@ -230,8 +239,8 @@ __device__ __forceinline__ void xu_finish(const KArgs<T>& P, const T* c, const T
   for (int i = 0; i < CH; ++i)
     if (i < len) row[s0 + i] = chunk_post<T>(P, c, i, s0 + i, y[i], fb4);
   __syncthreads();
-  if (P.vec) store_tile<T, NLW, 16 / sizeof(T)>(P, tile, L0, lmax);
-  else store_tile<T, NLW, 1>(P, tile, L0, lmax);
+  if (P.vec) store_tile<T, NLW, 16 / sizeof(T), F32>(P, tile, L0, lmax);
+  else store_tile<T, NLW, 1, F32>(P, tile, L0, lmax);
 }
 
 // Waves per SIMD the tile kernels are compiled for: the 32-line tile of
@@ -304,7 +313,7 @@ __global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_
     for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
   T y[CH];
   chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
-  xu_finish<T, CH, NLW>(P, c, rinv, tile, lane, q, s0, len, L0, lmax, y, fb4);
+  xu_finish<T, CH, NLW, IN == 0 && sizeof(T) == 8>(P, c, rinv, tile, lane, q, s0, len, L0, lmax, y, fb4);
 }
 
 // Two operators of one line plan geometry on one input (PAIR 0 = MIDP_P2C +

@@ -245,6 +245,43 @@ def test_cfl_vs_oracle(pair):
         assert r[0] > 0
 
 
+def test_mixed_state_copies_follow_set_state():
+    """Mixed form: a sub-step leaves the FP32 copies of the new velocity and
+    pressure (stored by the corrections and the pressure update) for the
+    next right-hand side, and set_state makes them stale.  A solver stepped
+    twice, then given other fields and stepped again, must land bit for bit
+    where a fresh solver given those fields and stepped once lands; and the
+    momentum right-hand side from the copies a step left must equal the one
+    from copies converted afresh."""
+    from chapsim2_amd.domain import PREC_MIXED, PROJ_EXACT, Domain
+    from chapsim2_amd.initial import channel_fields
+
+    nc = (32, 24, 16)
+    kw = dict(projection=PROJ_EXACT, precision=PREC_MIXED, **CFG)
+    a = Domain(nc, LX, -1.0, 1.0, LZ, **kw)
+    b = Domain(nc, LX, -1.0, 1.0, LZ, **kw)
+    try:
+        f1 = channel_fields(nc, LX, -1.0, 1.0, LZ, 2, 0.075, noise=0.1, seed=71)
+        f2 = channel_fields(nc, LX, -1.0, 1.0, LZ, 2, 0.075, noise=0.1, seed=72)
+        a.set_state(*f1)
+        a.substep(0)
+        a.run(0, 2)
+        a.set_state(*f2)
+        b.set_state(*f2)
+        a.run(2, 1)
+        b.run(2, 1)
+        for x, y in zip(a.get_state(), b.get_state()):
+            assert np.array_equal(x, y)
+        b.run(3, 1)
+        a.run(3, 1)
+        b.set_state(*b.get_state())  # same state, copies converted afresh
+        for x, y in zip(a.momentum_rhs(1), b.momentum_rhs(1)):
+            assert np.array_equal(x, y)
+    finally:
+        a.close()
+        b.close()
+
+
 def test_velocity_range_and_nonfinite_detection(pair):
     """chs_velocity_range (the per-step Find_max_min_3d of qx, qy, qz) is
     exact: the max and min numpy takes of the state, before and after an RK3
