@@ -28,6 +28,8 @@
  *               of the wall-bounded CD2 discretisation.  The per-term
  *               ordering inside Compute_momentum_rhs beyond those checks is
  *               "parity unpinned" (no running reference exists here).
+ *   mass check - qx = cos(2 pi x / Lx) alone: every window's max |div| is
+ *               the closed form -T(kh)/h sin(k x_c) over its own x range.
  */
 #include "chs_oracle.h"
 

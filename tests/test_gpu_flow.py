@@ -282,6 +282,25 @@ def test_mixed_state_copies_follow_set_state():
         b.close()
 
 
+def test_mass_check_windows_closed_form_on_gpu():
+    """chs_mass_check on qx = cos(2 pi x / LX) alone against the closed-form
+    maxima of the three x windows (the construction that pins the oracle's
+    windows in test_oracle_diagnostics)."""
+    from chapsim2_amd.domain import Domain
+    from test_oracle_diagnostics import LX as LXD, LZ as LZD, _window_reference
+
+    nc = (48, 8, 8)
+    dm = Domain(nc, LXD, -1.0, 1.0, LZD, istret=0, rstret=0.0, dt=1e-3, ren=100.0, idriven=0)
+    try:
+        qx, ref, _ = _window_reference(nc)
+        dm.set_state(qx, np.zeros(dm.shapes["qy"], order="F"), np.zeros(dm.shapes["qz"], order="F"),
+                     np.zeros(dm.shapes["pr"], order="F"))
+        out = dm.mass_check()
+        assert np.allclose(out, ref, rtol=1e-12, atol=0), (out, ref)
+    finally:
+        dm.close()
+
+
 def test_velocity_range_and_nonfinite_detection(pair):
     """chs_velocity_range (the per-step Find_max_min_3d of qx, qy, qz) is
     exact: the max and min numpy takes of the state, before and after an RK3

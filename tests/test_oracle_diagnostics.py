@@ -63,3 +63,40 @@ def test_bulk_monitor_of_uniform_flow(po):
     mke, qx = f.bulk()
     assert abs(mke - 0.5 * (U * U + V * V + W * W)) < 1e-13
     assert abs(qx - U) < 1e-13
+
+
+def _window_reference(nc):
+    """max |div| of qx = cos(2 pi x / LX) (qy = qz = 0) over the three x
+    windows of Check_element_mass_conservation (1:4, 5:n-4, n-5:n,
+    src/eq_continuity.f90:387-389): the CP6 staggered first derivative of a
+    Fourier mode is -T(kh)/h sin(k x_c) with the scheme's transfer function
+    T(kh) = (2a sin(kh/2) + 2b sin(3kh/2)) / (1 + 2 alpha cos kh), alpha =
+    9/62, a = 63/62, b = 17/186 (the textbook staggered sixth-order scheme)."""
+    n = nc[0]
+    h = LX / n
+    k = 2 * np.pi / LX
+    xf = np.arange(n) * h
+    al, a, b = 9 / 62, 63 / 62, 17 / 186
+    kh = k * h
+    T = (2 * a * np.sin(kh / 2) + 2 * b * np.sin(1.5 * kh)) / (1 + 2 * al * np.cos(kh))
+    d = np.abs(T / h * np.sin(k * (xf + h / 2)))
+    qx = np.asfortranarray(np.broadcast_to(np.cos(k * xf)[:, None, None], (n, nc[1], nc[2])))
+    return qx, [d[0:4].max(), d[4:n - 4].max(), d[n - 6:n].max()], d
+
+
+def test_mass_check_windows_closed_form(po):
+    """The oracle's Check_element_mass_conservation on qx = cos(2 pi x / LX)
+    alone: each window's max |div| is the closed form over exactly its own x
+    range.  On 48 cells the windows' maxima sit on their inner ends, so a
+    window one cell longer or shorter changes the result (checked on the
+    closed form itself)."""
+    nc = (48, 8, 8)
+    f = po.Flow(list(nc), LX, -1.0, 1.0, LZ, istret=0, wall_y=True, iacc=4, poisson=2, dt=1e-3, ren=100.0)
+    qx, ref, d = _window_reference(nc)
+    n = nc[0]
+    assert d[0:5].max() > ref[0] * 1.1 and d[0:3].max() < ref[0] / 1.1
+    assert d[n - 7:n].max() > ref[2] * 1.05 and d[n - 5:n].max() < ref[2] / 1.05
+    f.set_state(qx, np.zeros(f.shapes["qy"], order="F"), np.zeros(f.shapes["qz"], order="F"),
+                np.zeros(f.shapes["pr"], order="F"))
+    out = f.mass_check()
+    assert np.allclose(out, ref, rtol=1e-12, atol=0), (out, ref)
