@@ -47,6 +47,20 @@ def test_batch_matches_single_instances(graph):
             for name, x, y in zip(("qx", "qy", "qz", "pr"), db.get_state(), ds.get_state()):
                 assert np.array_equal(x, y), (i, name, np.abs(x - y).max())
             assert np.all(db.mass_check() < 1e-12)
+        # a new state for the mixed instance between runs: the next step (a
+        # replay of the captured graph with graph=True) must round the new
+        # state, not reuse the FP32 copies the last step left
+        s1 = specs[1]
+        f2 = channel_fields(s1["nc"], LX, -1.0, 1.0, LZ, 2, s1["rstret"], noise=0.1, seed=99)
+        b.domains[1].set_state(*f2)
+        b.run(4, 1, graph=graph)
+        kw = {k: v for k, v in s1.items() if k not in ("nc", "lxx", "lyb", "lyt", "lzz")}
+        d = Domain(s1["nc"], LX, -1.0, 1.0, LZ, **kw)
+        singles.append(d)
+        d.set_state(*f2)
+        d.run(4, 1)
+        for name, x, y in zip(("qx", "qy", "qz", "pr"), b.domains[1].get_state(), d.get_state()):
+            assert np.array_equal(x, y), ("after set_state", name, np.abs(x - y).max())
     finally:
         for d in singles:
             d.close()
