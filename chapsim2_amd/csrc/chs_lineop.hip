@@ -538,7 +538,12 @@ static void dispatch_long(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, con
 template <typename T, int DIR, int NLW>
 static void dispatch_part(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
   if constexpr (NLW == 32) {
-    if (input_kind(a) == 0) dispatch_yz<T, DIR, 0, kChunk, 32>(op, g, b, shm, s, a);
+    // merged operators only in FP32 (the FP64 forms spill at 32 lines)
+    switch (input_kind(a)) {
+      case 0: dispatch_yz<T, DIR, 0, kChunk, 32>(op, g, b, shm, s, a); break;
+      case 1: if constexpr (sizeof(T) == 4) dispatch_yz<T, DIR, 1, kChunk, 32>(op, g, b, shm, s, a); break;
+      default: if constexpr (sizeof(T) == 4) dispatch_yz<T, DIR, 2, kChunk, 32>(op, g, b, shm, s, a); break;
+    }
   } else {
     switch (input_kind(a)) {
       case 0: dispatch_yz<T, DIR, 0, kChunk, NLW>(op, g, b, shm, s, a); break;
@@ -722,12 +727,12 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
       if (dir == 1) dispatch_long<T, 1>(p.op, gl, bl, shm_l, s, a);
       else dispatch_long<T, 2>(p.op, gl, bl, shm_l, s, a);
     } else {
-      // y and z lines of 16-row chunks: 64 or (single-input operators) 32
-      // lines per workgroup, timed once per shape when a scratch output is
-      // at hand (else 64)
+      // y and z lines of 16-row chunks: 64 or (single-input operators, and
+      // every FP32 operator) 32 lines per workgroup, timed once per shape
+      // when a scratch output is at hand (else 64)
       if (!lanes_ok) return "line op: x-y plane too large for 32-bit line offsets";
       auto run = [&](int v, const KArgs<T>& ka) -> bool {
-        if (v == 1 && input_kind(ka) != 0) return false;
+        if (v == 1 && input_kind(ka) != 0 && sizeof(T) != 4) return false;
         const int nlw = v == 0 ? 64 : 32;
         dim3 g((unsigned)((ka.nlines + nlw - 1) / nlw)), b((unsigned)(nlw * p.part_P));
         if (v == 0) {
@@ -751,7 +756,9 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
                                 std::to_string(input_kind(a));
         v = tune(key, 2, s, [&](int vv) { return run(vv, ts); });
       }
-      if (!run(v < 0 ? 0 : v, a)) return std::string("line op launch: ") + hipGetErrorString(hipGetLastError());
+      // (a forced shape that does not apply falls back to 64 lines)
+      if (!run(v < 0 ? 0 : v, a) && (v <= 0 || !run(0, a)))
+        return std::string("line op launch: ") + hipGetErrorString(hipGetLastError());
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return std::string("line op launch: ") + hipGetErrorString(e);

@@ -320,10 +320,12 @@ def test_x_launch_shapes_vs_oracle(po, variant):
 def test_yz_launch_shapes_vs_oracle(po, variant):
     """Both y/z launch shapes, forced through chs_tune_force (0: 64 lines per
     workgroup, one chunk per wave; 1: 32 lines per workgroup, two chunks per
-    wave, single-input operators), on y lines of 24/25 and 129/130 rows and z
-    lines of 16 and 256 rows, every operator with CP6 and the explicit CD4,
-    wall-bounded y with Dirichlet, Neumann and interpolation boundary kinds,
-    against the oracle."""
+    wave, single-input operators and every FP32 operator), on y lines of
+    24/25 and 129/130 rows and z lines of 16 and 256 rows, every operator with
+    CP6 and the explicit CD4, wall-bounded y with Dirichlet, Neumann and
+    interpolation boundary kinds, against the oracle; then the FP32 merged
+    operators of the mixed momentum right-hand side against the 64-line
+    shape."""
     from chapsim2_amd import _lib
     from chapsim2_amd import operations as ops
     from chapsim2_amd.domain import Domain
@@ -361,5 +363,25 @@ def test_yz_launch_shapes_vs_oracle(po, variant):
                                 assert err < RTOL, (variant, nc, d, op, acc, bc, fb, err)
             finally:
                 dm.close()
+        # FP32 merged operators (two to four input arrays: the mixed form's
+        # momentum right-hand side) in this shape against the 64-line shape
+        from chapsim2_amd.domain import PREC_MIXED, PROJ_EXACT
+        from chapsim2_amd.initial import channel_fields
+
+        nc = (32, 40, 48)
+        fields = channel_fields(nc, LX, -1.0, 1.0, LZ, 2, 0.075, noise=0.1, seed=5)
+        rhs = []
+        for v in (0, variant):
+            _lib.check(lib.chs_tune_force(b"line_yz/", v))
+            dm = Domain(nc, LX, -1.0, 1.0, LZ, istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1,
+                        projection=PROJ_EXACT, precision=PREC_MIXED)
+            try:
+                dm.set_state(*fields)
+                rhs.append(dm.momentum_rhs(1))
+            finally:
+                dm.close()
+        for name, a, b in zip(("mx", "my", "mz"), *rhs):
+            err = np.abs(a - b).max() / np.abs(b).max()
+            assert err < 1e-6, (variant, name, err)
     finally:
         _lib.check(lib.chs_tune_force(b"line_yz/", -1))
