@@ -1066,33 +1066,36 @@ __global__ void __launch_bounds__(64 * kMmWaves) k_ymix_mfma(const double* __res
   const double* lam = half ? lamo : lame;
   double* Bh = Bs + half * 64 * kMmW;
   const long ntile = (ncol + kMmC - 1) / kMmC;
+  // the fold's inputs (rows j and ny-1-j of column g0 + tid % 64, j = tid / 64
+  // + 8u) of a workgroup's next tile are loaded while it finishes the current
+  // one (clamped to valid rows and columns, so every address is in range)
+  double pa[8], pb[8];
+  auto fold_load = [&](long g0n) {
+    const long g = g0n + (tid & 63);
+    const long gc = g < ncol ? g : ncol - 1;
+    const long kz = gc / ncr, cc = gc - kz * ncr;
+    const long base = cc + (long)ncr * (long)ny * kz;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = (tid >> 6) + 8 * u;
+      const int jc = j < h ? j : h - 1;
+      pa[u] = W[base + (long)ncr * jc];
+      pb[u] = W[base + (long)ncr * (ny - 1 - jc)];
+    }
+  };
+  if ((long)blockIdx.x < ntile) fold_load((long)blockIdx.x * kMmC);
   for (long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
     const long g0 = tile * kMmC;
     // ---- fold: 64 columns x 64 row pairs, 8 per thread
     {
       const int c = tid & 63;
-      const long g = g0 + c;
-      const bool cok = g < ncol;
-      const long gc = cok ? g : ncol - 1;
-      const long kz = gc / ncr, cc = gc - kz * ncr;
-      const long base = cc + (long)ncr * (long)ny * kz;
+      const bool cok = g0 + c < ncol;
 #pragma unroll
-      for (int u0 = 0; u0 < 8; u0 += 4) {
-        double a[4], b[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int j = (tid >> 6) + 8 * (u0 + u);
-          const int jc = j < h ? j : h - 1;
-          a[u] = W[base + (long)ncr * jc];
-          b[u] = W[base + (long)ncr * (ny - 1 - jc)];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int j = (tid >> 6) + 8 * (u0 + u);
-          const bool ok = cok && j < h;
-          Bs[j * kMmW + c] = ok ? a[u] + b[u] : 0.0;
-          Bs[(64 + j) * kMmW + c] = ok ? a[u] - b[u] : 0.0;
-        }
+      for (int u = 0; u < 8; ++u) {
+        const int j = (tid >> 6) + 8 * u;
+        const bool ok = cok && j < h;
+        Bs[j * kMmW + c] = ok ? pa[u] + pb[u] : 0.0;
+        Bs[(64 + j) * kMmW + c] = ok ? pa[u] - pb[u] : 0.0;
       }
     }
     __syncthreads();
@@ -1136,6 +1139,9 @@ __global__ void __launch_bounds__(64 * kMmWaves) k_ymix_mfma(const double* __res
     // ---- D = V Y
     double a2[16];
     load_block(Mv, a2);
+    // (issued after every other load of this tile, so waiting for those does
+    // not wait for these)
+    if (tile + gridDim.x < ntile) fold_load((tile + gridDim.x) * kMmC);
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) acc[cb] = chs_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 2
@@ -1499,8 +1505,15 @@ std::string Poisson::solve_exact_t(T* rhs, hipStream_t s, T in_scale) {
   const int ncr = 2 * nxh;
   const long ncol = (long)ncr * nz;
   if (Ve64 && dbl && ny / 2 <= 64) {
+    // one workgroup per CU (the kernel's registers allow one per CU), each
+    // looping over its tiles with the next tile's loads overlapping the
+    // current one's products
     const long ntile = (ncol + kMmC - 1) / kMmC;
-    const unsigned nwg = (unsigned)(ntile < 2048 ? ntile : 2048);
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    const unsigned nwg = (unsigned)(ntile < cus ? ntile : cus);
     hipLaunchKernelGGL(k_ymix_mfma, dim3(nwg), dim3(64 * kMmWaves), 0, s, Vie64, Vio64, Ve64, Vo64,
                        reinterpret_cast<double*>(Wr), ny, ncr, ncol, lame64, lamo64,
                        reinterpret_cast<const double*>(lxv), reinterpret_cast<const double*>(lzv), null_even);
