@@ -6,7 +6,7 @@
 // data are samples of the table interpolation itself
 // (ftp_refresh_thermal_properties_from_H, src/input_thermo.f90:345-374, run on
 // the GPU over the table handle), so the table stays both the fallback and
-// the judge of the surrogate.
+// the yardstick of the surrogate.
 //
 // Kernels: a sampling kernel (uniform h, table interpolation, normalised
 // targets); an inference kernel (one thread per sample, weights staged in LDS

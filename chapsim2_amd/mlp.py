@@ -1,6 +1,6 @@
 """MLP surrogate of a property table (libchapsim_hip chs_mlp_*):
 h -> (T, rho, mu, kappa, cp), trained on the GPU on samples of the table's
-own interpolation; the table stays the fallback and the judge."""
+own interpolation; the table stays the fallback and the yardstick."""
 import ctypes as C
 
 import numpy as np

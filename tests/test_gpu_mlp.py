@@ -62,7 +62,7 @@ def test_training_on_table_samples(table):
         # training gains more than an order of magnitude on every output
         assert np.all(e1 < 0.1 * e0)
         assert np.all(e1 < 0.05)
-        # the table interpolation (the judge) is reproduced to a few % of each
+        # the table interpolation (the yardstick) is reproduced to a few % of each
         # output's spread everywhere except inside the narrow cp peak
         out = net(h)
         away = np.abs(h - 1.0) > 0.15
