@@ -16,20 +16,33 @@
  *   Re          src/tools_solver.f90:30-47
  *   RK3         src/input_general.f90:842-888
  *
- * Pinning (tests/test_oracle_*.py):
- *   mesh      - closed form of the tanh map and its derivative.
+ * Pinning (tests/test_oracle_pins.py, test_oracle_fft_poisson.py,
+ * test_oracle_diagnostics.py):
+ *   mesh      - closed form of the tanh map; yMapping against the cosh form
+ *               of dy/deta and a finite difference of the nodes.
  *   RFFTF/B   - numpy.fft on random data (the FFTPACK definitions).
  *   Poisson   - the assembled 2nd-order operator applied to the solution
  *               reproduces the RHS (dense check on small grids) and the
- *               singular-pivot gauge sets the last (0,0) unknown to zero.
- *   sub-step  - the CD2 configuration projects to machine-zero divergence
- *               (discretely consistent pair); Taylor-Green decay against
- *               the analytic exp(-2 nu t) law; Poiseuille is a steady state
- *               of the wall-bounded CD2 discretisation.  The per-term
- *               ordering inside Compute_momentum_rhs beyond those checks is
- *               "parity unpinned" (no running reference exists here).
- *   mass check - qx = cos(2 pi x / Lx) alone: every window's max |div| is
- *               the closed form -T(kh)/h sin(k x_c) over its own x range.
+ *               singular-pivot gauge; the 3-D path divides a Fourier mode by
+ *               the published CP6 modified wavenumbers.
+ *   exact projection - zero divergence after one projection, idempotence,
+ *               equality with FISHPACK for CD2.
+ *   momentum RHS - manufactured fields on the stretched channel: every
+ *               component converges to the continuous stress-form right-hand
+ *               side (first order at the wall rows, the x/z design order in
+ *               the core), per field and all together; y-mirror symmetry;
+ *               x/z translation invariance; plane Couette flow with a moving
+ *               wall is an exact steady state.
+ *   RK3, initial step - the 2-D Taylor-Green mode (in each coordinate
+ *               plane) is multiplied by the third-order stability polynomial
+ *               1 + z + z^2/2 + z^3/6 per step, z = -2 nu (T/h)^2 dt, to
+ *               round-off; the initial step by 1 + z; the accumulated
+ *               pressure balances the convective term.
+ *   forcing   - constant-flux forcing keeps the volume-weighted bulk
+ *               velocity to round-off.
+ *   walls     - the wall-normal velocity on the wall nodes equals the wall
+ *               values after a sub-step.
+ *   diagnostics - CFL, bulk values and mass-check windows by closed forms.
  */
 #include "chs_oracle.h"
 

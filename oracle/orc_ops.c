@@ -11,19 +11,21 @@
  *   1-D drivers src/basics_operations2.f90:2739-3933
  *   3-D drivers src/basics_operations2.f90:3952-4790
  *
- * Pinning (tests/test_oracle_*.py):
+ * Pinning (tests/test_oracle_ops.py, test_oracle_tdma.py, test_oracle_pins.py):
  *   TDMA      - the reference's two known-answer vectors
  *               (src/basics_algorithms.f90:122-127, 168-181) and dense solves.
- *   tables    - closed-form order conditions: every interior/boundary row is
- *               exact on polynomials up to its design order; periodic rows
- *               reproduce the Fourier symbols of the schemes.
- *   operators - exactness on polynomials (wall lines) and trigonometric
- *               modes (periodic lines) and the dense B^-1 A brute force on
- *               short lines; 3-D drivers against the 1-D driver line by line.
- *   The exact choice of boundary branch per BC kind (e.g. which rows are
- *   one-sided) is pinned by the polynomial-exactness order each branch
- *   achieves; a branch whose order does not single it out (the Neumann
- *   ghost rows with a non-zero gradient) is "parity unpinned".
+ *   tables    - periodic rows reproduce the published transfer functions of
+ *               all 24 (operator, accuracy) pairs.
+ *   closures  - every operator is exact on linear data at every row of a
+ *               uniform wall-bounded line (Dirichlet, Neumann, interpolation
+ *               closures), and converges on a smooth non-polynomial profile
+ *               at every row, on uniform and tanh-stretched lines, at first
+ *               order or better (derivatives) and second order or better
+ *               (interpolations) for all three closure kinds.  The Neumann
+ *               rows of the compact P2P / C2P derivatives on stretched lines
+ *               are pinned for the zero wall gradient only (the reference
+ *               enters the physical gradient unscaled, consistent only then).
+ *   3-D       - the 3-D drivers equal the 1-D driver applied line by line.
  */
 #include "chs_oracle.h"
 
