@@ -9,13 +9,19 @@ fractional step, Poisson solve, projection).
 
     python bench.py [--steps K] [--warmup W] [--form fp64c|mixed|fp64r]
 
-Prints one JSON line.  `value` is Mpts*RK3-steps/s with all fields resident
-in HBM; `roofline` reports the achieved HBM bandwidth of the dominant kernel
-class (line operators along one direction) from HIP events recorded on the
-solver's stream over one further RK3 step in which every kernel runs alone
-(the timed steps overlap independent operators on three streams);
-`cpu_baseline` times the CPU oracle (a plain-C OpenMP restatement of the
-reference solver) on one RK3 step of the same workload on the host cores.
+Prints one JSON line.  By default both forms of the step run from the same
+seeded field: FP64 (`fp64c`) and mixed precision (`mixed`, the headline:
+`value` is its Mpts*RK3-steps/s with all fields resident in HBM), followed
+by 16 uneven batched channels (`forms.batch`).  `mixed_vs_fp64` holds the
+throughput ratio and the mixed form's per-field error against the FP64 form
+after the same steps; each form reports max|div u| and its step time with
+the reference loop's per-step diagnostics.  `roofline` gives the achieved
+HBM bandwidth of the dominant kernel class of the headline form (line
+operators along x) from HIP events recorded on the solver's stream over one
+further RK3 step in which every kernel runs alone (the timed steps overlap
+independent operators on three streams).  `cpu_baseline` times the CPU
+oracle (a plain-C OpenMP restatement of the reference solver) on the same
+workload on the host cores.  With --form only that form runs.
 """
 import argparse
 import json
@@ -34,12 +40,16 @@ PEAK_HBM_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--form", default="fp64c", choices=["fp64c", "mixed", "fp64r"],
-                    help="fp64c: FP64 with the exact projection (max|div u| at round-off); "
+    ap.add_argument("--form", default=None, choices=["fp64c", "mixed", "fp64r"],
+                    help="run one form only: fp64c: FP64 with the exact projection (max|div u| at round-off); "
                          "mixed: FP32 momentum right-hand side, FP64 state, divergence and exact projection; "
                          "fp64r: FP64 with the reference's single 2nd-order projection")
+    ap.add_argument("--diag-steps", type=int, default=5,
+                    help="steps timed with the per-step diagnostics (both-form run)")
+    ap.add_argument("--batch-default", type=int, default=16, metavar="N",
+                    help="uneven batched channels timed after the two forms (0: none)")
     ap.add_argument("--profile-class", type=int, default=0,
                     help="line-operator direction whose launches are timed (0 x, 1 y, 2 z)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -117,7 +127,10 @@ def measured_traffic(form, direction, nc):
 
 def cpu_baseline(nc, fields, cfg):
     """The oracle's restatement of the reference algorithm (FP64, single
-    2nd-order projection per sub-step) on the host cores: one RK3 step."""
+    2nd-order projection per sub-step) on the host cores, on bounded samples
+    of the same workload: with every core of this process's share (at most
+    16), (T(1 + 2 steps) - T(1 step)) / 2, which leaves out the first step;
+    on one core, one sub-step (a third of an RK3 step)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
@@ -127,20 +140,36 @@ def cpu_baseline(nc, fields, cfg):
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
-    pyoracle.set_threads(cores)
     lx, lz = 4 * 3.141592653589793, 4 * 3.141592653589793 / 3
-    of = pyoracle.Flow(list(nc), lx, -1.0, 1.0, lz, wall_y=True, iacc=4, poisson=2, **cfg)
-    of.set_state(*fields)
+    mpts = nc[0] * nc[1] * nc[2] / 1e6
+
+    def flow():
+        of = pyoracle.Flow(list(nc), lx, -1.0, 1.0, lz, wall_y=True, iacc=4, poisson=2, **cfg)
+        of.set_state(*fields)
+        return of
+
+    pyoracle.set_threads(cores)
+    of = flow()
+    t = [time.perf_counter()]
+    for it in (1, 2, 3):
+        of.step(it)
+        t.append(time.perf_counter())
+    per_step = (t[3] - t[1]) / 2
+    del of
+    pyoracle.set_threads(1)
+    of = flow()
     of.update_re(1)
     t0 = time.perf_counter()
-    for isub in (1, 2, 3):
-        of.substep(isub)
-    dt = time.perf_counter() - t0
-    mpts = nc[0] * nc[1] * nc[2] / 1e6
-    return {"value": mpts / dt, "unit": "Mpts·RK3-steps/s", "cores": pyoracle.get_threads(),
-            "kind": "port",
-            "sample": f"one RK3 step (3 sub-steps) of the same {nc[0]}x{nc[1]}x{nc[2]} channel with the "
-                      f"reference algorithm (FP64, single 2nd-order projection), {dt:.1f} s"}
+    of.substep(1)
+    one = (time.perf_counter() - t0) * 3
+    del of
+    pyoracle.set_threads(cores)
+    return {"value": mpts / per_step, "unit": "Mpts·RK3-steps/s", "cores": cores, "kind": "port",
+            "sample": f"the same {nc[0]}x{nc[1]}x{nc[2]} channel with the reference algorithm (FP64, single "
+                      f"2nd-order projection): 3 RK3 steps on {cores} threads, timed as (T(3) - T(1)) / 2 = "
+                      f"{per_step:.2f} s per step",
+            "one_core": {"value": mpts / one, "cores": 1,
+                         "sample": f"one sub-step on 1 thread, {one / 3:.1f} s (x3 per RK3 step)"}}
 
 
 def band_limited(shape, seed, kfrac=1.0 / 3.0):
@@ -346,15 +375,17 @@ def batch_specs(n, seed=20261202):
     return specs
 
 
-def batch_bench(args):
-    import numpy as np
+def batch_run(n, args):
+    """n uneven channels (batch_specs) advanced together: one eager RK3 step
+    (plans and launch shapes), W warm-up and K timed graph-captured steps;
+    every instance's max|div u| afterwards."""
     import torch
 
     from chapsim2_amd import _lib
     from chapsim2_amd.domain import Batch
     from chapsim2_amd.initial import channel_fields
 
-    specs = batch_specs(args.batch)
+    specs = batch_specs(n)
     b = Batch(specs)
     for i, (d, s) in enumerate(zip(b.domains, specs)):
         d.set_state(*channel_fields(s["nc"], s["lxx"], -1.0, 1.0, s["lzz"], 2, s["rstret"], noise=0.1, seed=100 + i))
@@ -369,19 +400,122 @@ def batch_bench(args):
     pts = sum(s["nc"][0] * s["nc"][1] * s["nc"][2] for s in specs)
     div = max(float(d.mass_check().max()) for d in b.domains)
     tuned = int(_lib.load().chs_tune_report(None, 0))
+    b.close()
+    return {"value": pts / 1e6 * args.steps / sec, "ms_per_step": sec * 1e3 / args.steps, "max_div_u": div,
+            "what": f"{len(specs)} periodic channels, grids " +
+                    ", ".join("x".join(str(v) for v in s["nc"]) for s in specs) +
+                    "; Re_b 2800/6900, tanh beta in [1,2], CP6, mixed precision, exact projection, "
+                    "one captured graph per RK3 step",
+            "total_points": pts, "autotuned_shapes": tuned}
+
+
+def batch_bench(args):
+    r = batch_run(args.batch, args)
     out = {
         "metric": "Mpts·RK3-steps/s, batched channels",
-        "value": pts / 1e6 * args.steps / sec, "unit": "Mpts·RK3-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": max(args.warmup, 2), "ms_per_step": sec * 1e3 / args.steps, "higher_is_better": True,
+        "value": r["value"], "unit": "Mpts·RK3-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": max(args.warmup, 2), "ms_per_step": r["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": f"{len(specs)} periodic channels, grids " +
-                               ", ".join("x".join(str(v) for v in s["nc"]) for s in specs) +
-                               "; Re_b 2800/6900, tanh beta in [1,2], CP6, mixed precision, exact projection, "
-                               "one captured graph per RK3 step",
-                   "total_points": pts, "max_div_u": div, "autotuned_shapes": tuned},
+        "config": {"workload": r["what"], "total_points": r["total_points"], "max_div_u": r["max_div_u"],
+                   "autotuned_shapes": r["autotuned_shapes"]},
     }
-    b.close()
     print(json.dumps(out))
+
+
+FORMS = {
+    "fp64c": "FP64 throughout, exact projection (inverse of the compact D*G per sub-step)",
+    "mixed": "FP32 momentum right-hand side (44 line operators, products, fractional step); FP64 state, "
+             "bulk-flux sums, divergence, exact projection and correction",
+    "fp64r": "FP64 with the reference's single FFT(x,z)+TDMA(y) 2nd-order projection per sub-step",
+}
+
+
+def run_form(form, nc, fields, cfg, args, diag_steps):
+    """One form of the channel step: the initial Euler step, W warm-up steps,
+    K timed RK3 steps bracketed by device synchronisations, one further step
+    with every kernel issued alone and the launches of one line-operator
+    class timed with HIP events, then diag_steps steps each followed by the
+    reference loop's per-step diagnostics (mass-conservation windows, velocity
+    range, CFL, bulk monitor; src/chapsim.f90:319-335).  Returns the result
+    record and the final state on the host."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from chapsim2_amd import _lib
+    from chapsim2_amd.domain import PREC_FP64, PREC_MIXED, PROJ_EXACT, PROJ_REFERENCE, Domain
+
+    lx, lz = 4 * np.pi, 4 * np.pi / 3
+    proj = PROJ_REFERENCE if form == "fp64r" else PROJ_EXACT
+    prec = PREC_MIXED if form == "mixed" else PREC_FP64
+    dm = Domain(nc, lx, -1.0, 1.0, lz, projection=proj, precision=prec, **cfg)
+    dm.set_state(*fields)
+    dm.update_re(0)
+    dm.substep(0)  # initial Euler step + projection (src/chapsim.f90:135)
+    it = 0
+    if args.warmup > 0:
+        dm.run(it, args.warmup)
+        it += args.warmup
+    torch.cuda.synchronize()
+    L = dm.lib
+    t0 = time.perf_counter()
+    dm.run(it, args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    it += args.steps
+    # launch timing of one operator class over one further RK3 step: while
+    # profiled, the solver issues its kernels one at a time (in the timed
+    # steps independent operators overlap on three streams, which stretches
+    # each launch's own duration)
+    ms, nl, byt = C.c_double(), C.c_longlong(), C.c_double()
+    _lib.check(L.chs_profile_begin(dm.h, args.profile_class))
+    dm.run(it, 1)
+    _lib.check(L.chs_profile_end(dm.h, C.byref(ms), C.byref(nl), C.byref(byt)))
+    it += 1
+    # steps with the per-step diagnostics of the reference loop
+    diag_ms = None
+    if diag_steps > 0:
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(diag_steps):
+            dm.run(it, 1)
+            it += 1
+            dm.cfl()
+            dm.mass_check()
+            dm.velocity_range()
+            dm.bulk()
+        torch.cuda.synchronize()
+        diag_ms = (time.perf_counter() - t2) * 1e3 / diag_steps
+    sec = t1 - t0
+    mpts = nc[0] * nc[1] * nc[2] / 1e6
+    div = float(dm.mass_check().max())
+    achieved = (byt.value / max(nl.value, 1)) / (ms.value / max(nl.value, 1) * 1e-3) / 1e9 if nl.value else None
+    res = {
+        "value": mpts * args.steps / sec,
+        "ms_per_step": sec * 1e3 / args.steps,
+        "ms_per_step_with_diagnostics": diag_ms,
+        "max_div_u": div,
+        "steps_total": it,
+        "what": FORMS[form],
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"compact line operators along {'xyz'[args.profile_class]} "
+                      f"({'k_line_xu' if args.profile_class == 0 else 'k_line_yz'})",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / PEAK_HBM_GBS) if achieved else None,
+            "traffic": measured_traffic(form, "xyz"[args.profile_class], nc),
+            "launches": nl.value,
+            "avg_launch_ms": ms.value / max(nl.value, 1),
+            "measured_on": "one RK3 step after the timed region with every kernel issued alone "
+                           "(HIP events on the solver's stream)",
+        },
+    }
+    state = dm.get_state()
+    dm.close()
+    return res, state
 
 
 def main():
@@ -402,95 +536,60 @@ def main():
         batch_bench(args)
         return
     import numpy as np
-    import torch
 
-    from chapsim2_amd import _lib
-    from chapsim2_amd.domain import Domain
     from chapsim2_amd.initial import channel_fields
 
     nc = tuple(args.nc)
     lx, lz = 4 * np.pi, 4 * np.pi / 3
     cfg = dict(istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1)
     fields = channel_fields(nc, lx, -1.0, 1.0, lz, 2, 0.075, noise=0.1, seed=20261202)
-    from chapsim2_amd.domain import PROJ_EXACT, PROJ_REFERENCE
-
-    from chapsim2_amd.domain import PREC_FP64, PREC_MIXED
-
-    proj = PROJ_REFERENCE if args.form == "fp64r" else PROJ_EXACT
-    prec = PREC_MIXED if args.form == "mixed" else PREC_FP64
-    dm = Domain(nc, lx, -1.0, 1.0, lz, projection=proj, precision=prec, **cfg)
-    dm.set_state(*fields)
-    dm.update_re(0)
-    dm.substep(0)  # initial Euler step + projection (src/chapsim.f90:135)
-    it = 0
-    if args.warmup > 0:
-        dm.run(it, args.warmup)
-        it += args.warmup
-    torch.cuda.synchronize()
-    L = dm.lib
-    t0 = time.perf_counter()
-    dm.run(it, args.steps)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    it += args.steps
-    import ctypes as C
-
-    # launch timing of one operator class over one further RK3 step: while
-    # profiled, the solver issues its kernels one at a time (in the timed
-    # steps independent operators overlap on three streams, which stretches
-    # each launch's own duration)
-    ms = C.c_double()
-    nl = C.c_longlong()
-    byt = C.c_double()
-    _lib.check(L.chs_profile_begin(dm.h, args.profile_class))
-    dm.run(it, 1)
-    _lib.check(L.chs_profile_end(dm.h, C.byref(ms), C.byref(nl), C.byref(byt)))
-    it += 1
-    sec = t1 - t0
-    mpts = nc[0] * nc[1] * nc[2] / 1e6
-    value = mpts * args.steps / sec
-    div = dm.mass_check()
-    achieved = (byt.value / max(nl.value, 1)) / (ms.value / max(nl.value, 1) * 1e-3) / 1e9 if nl.value else None
+    workload = (f"periodic channel {nc[0]}x{nc[1]}x{nc[2]}, 4pi x 2 x 4pi/3, tanh beta 1.5, CP6, "
+                f"Re_b 2800 constant flux, dt 5e-4, laminar + 10% noise (seed 20261202)")
+    forms = [args.form] if args.form else ["fp64c", "mixed"]
+    diag = 0 if args.form else args.diag_steps
+    results, states = {}, {}
+    for form in forms:
+        results[form], states[form] = run_form(form, nc, fields, cfg, args, diag)
+    head = forms[-1]  # the headline form: mixed when both run
+    r = results[head]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": r["value"],
         "unit": "Mpts·RK3-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": sec * 1e3 / args.steps,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if args.form == "mixed" else "f64",
+        "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic",
-        "config": {
-            "workload": f"periodic channel {nc[0]}x{nc[1]}x{nc[2]}, 4pi x 2 x 4pi/3, tanh beta 1.5, CP6, "
-                        f"Re_b 2800 constant flux, dt 5e-4, laminar + 10% noise (seed 20261202)",
-            "form": args.form,
-            "projection": ("exact inverse of the compact D*G per sub-step (FFT x,z + eigen-basis y)"
-                           if proj == PROJ_EXACT else
-                           "single FFT(x,z)+TDMA(y) 2nd-order projection per sub-step (reference-faithful)"),
-            "max_div_u": float(div.max()),
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": f"compact line operators along {'xyz'[args.profile_class]} "
-                      f"({'k_line_xu' if args.profile_class == 0 else 'k_line_yz'})",
-            "achieved": achieved,
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": (achieved / PEAK_HBM_GBS) if achieved else None,
-            "traffic": measured_traffic(args.form, "xyz"[args.profile_class], nc),
-            "launches": nl.value,
-            "avg_launch_ms": ms.value / max(nl.value, 1),
-            "measured_on": "one RK3 step after the timed region with every kernel issued alone "
-                           "(HIP events on the solver's stream)",
-        },
+        "config": {"workload": workload, "form": head, "max_div_u": r["max_div_u"],
+                   "div_bar_1e-10_met": bool(r["max_div_u"] <= 1e-10)},
+        "roofline": r["roofline"],
     }
+    if len(forms) > 1:
+        a, b = states["mixed"], states["fp64c"]
+        errs = {}
+        for name, x, y in zip(("qx", "qy", "qz", "pr"), a, b):
+            if name == "pr":  # the pressure is defined up to a constant
+                x, y = x - x.mean(), y - y.mean()
+            errs[name] = float(np.abs(x - y).max() / np.abs(y).max())
+        out["mixed_vs_fp64"] = {
+            "ratio": results["mixed"]["value"] / results["fp64c"]["value"],
+            "rel_err_per_field": errs,
+            "after_steps": results["mixed"]["steps_total"],
+            "err_definition": "max |q_mixed - q_fp64| / max |q_fp64| per field after the same steps from the "
+                              "same initial field (pressure mean removed)",
+        }
+        if args.batch_default > 0:
+            results["batch"] = batch_run(args.batch_default, args)
+            out["mixed_vs_fp64"]["batch_ratio"] = results["batch"]["value"] / results["fp64c"]["value"]
+    out["forms"] = {k: {kk: vv for kk, vv in v.items() if kk != "roofline" or k != head} for k, v in
+                    results.items()}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(nc, fields, cfg)
-    dm.close()
     if args.tune_to:
         tune_to(args.tune_to)
     print(json.dumps(out))

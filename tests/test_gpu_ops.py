@@ -89,10 +89,9 @@ def test_named_operators_match_generic(chan, po):
     assert np.abs(a - ref).max() / np.abs(ref).max() < RTOL
 
 
-def test_ops_polynomial_exact_on_gpu(chan):
-    """A CP6 P2C derivative of a quadratic in x is exact in the periodic
-    interior only for trigonometric modes; here check linearity instead:
-    op(a f + b g) == a op(f) + b op(g) at full size."""
+def test_ops_linearity_on_gpu(chan):
+    """The y-line C2P derivative with Dirichlet walls is linear in its input
+    at full size: op(a f + b g) == a op(f) + b op(g) (wall values zero)."""
     from chapsim2_amd import operations as ops
 
     rng = np.random.default_rng(11)
