@@ -12,7 +12,10 @@ fractional step, Poisson solve, projection).
 Prints one JSON line.  By default both forms of the step run from the same
 seeded field: FP64 (`fp64c`) and mixed precision (`mixed`, the headline:
 `value` is its Mpts*RK3-steps/s with all fields resident in HBM), followed
-by 16 uneven batched channels (`forms.batch`).  `mixed_vs_fp64` holds the
+by 16 uneven batched channels (`forms.batch`) and the heated channel with
+variable properties (`forms.heated`: property refresh from the table and
+from the MLP surrogate, each timed, their fields compared after 200 RK3
+steps).  `mixed_vs_fp64` holds the
 throughput ratio and the mixed form's per-field error against the FP64 form
 after the same steps; each form reports max|div u| and its step time with
 the reference loop's per-step diagnostics.  `roofline` gives the achieved
@@ -53,6 +56,8 @@ def parse():
     ap.add_argument("--profile-class", type=int, default=0,
                     help="line-operator direction whose launches are timed (0 x, 1 y, 2 z)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-heated", action="store_true",
+                    help="skip the heated variable-property channel (table and MLP property refresh)")
     ap.add_argument("--nc", type=int, nargs=3, default=list(NC))
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="run N independent uneven channels (64^3 .. 192x128x192, mixed precision) together "
@@ -518,6 +523,103 @@ def run_form(form, nc, fields, cfg, args, diag_steps):
     return res, state
 
 
+HEATED_STEPS = 200
+
+
+def heated_run(nc, fields, cfg, args):
+    """The heated channel with variable properties (mixed precision, exact
+    projection): isothermal walls at h = 1.2, bulk enthalpy 0.8, the
+    4096-row table of chapsim2_amd.initial.heated_channel_table on its
+    monotone rho*h window, reference Prandtl number 1.  The surrogate
+    (1 -> 64 -> 64 -> 5, BF16 weights) is trained on the GPU on 10^6 samples of
+    the table's interpolation.  Two solvers start from the same state, one
+    refreshing the properties from the table and one from the surrogate;
+    each times W warm-up and K steps, then both run on to 200 RK3 steps and
+    their fields are compared.  Also times one property refresh of every
+    cell in both modes."""
+    import numpy as np
+    import torch
+
+    from chapsim2_amd.domain import PREC_MIXED, PROJ_EXACT, Domain
+    from chapsim2_amd.initial import HEATED_H_RANGE, heated_channel_table
+    from chapsim2_amd.mlp import Surrogate
+    from chapsim2_amd.props import PROPS_MLP, PROPS_TABLE, PropertyTable
+
+    lx, lz = 4 * np.pi, 4 * np.pi / 3
+    tab = heated_channel_table(4096, *HEATED_H_RANGE)
+    pt = PropertyTable(tab, -1.0)
+    t0 = time.perf_counter()
+    sur = Surrogate(pt, nsamples=1_000_000, seed=7)
+    loss = sur.train(6000, batch=4096, lr=2e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    pt.set_surrogate(sur)
+    hgrid = np.linspace(0.8, 1.2, 4001)
+    ref = pt.from_h(hgrid)
+    fit = sur(hgrid)
+    fit_err = {k: float(np.abs(fit[k] - ref[k]).max() / np.abs(ref[k]).max()) for k in ("t", "d", "m", "k")}
+    tw = float(np.interp(1.2, tab[:, 1], tab[:, 2]))
+    ti = float(np.interp(0.8, tab[:, 1], tab[:, 2]))
+    mpts = nc[0] * nc[1] * nc[2] / 1e6
+    res, doms = {}, {}
+    for name, mode in (("table", PROPS_TABLE), ("mlp", PROPS_MLP)):
+        dm = Domain(nc, lx, -1.0, 1.0, lz, projection=PROJ_EXACT, precision=PREC_MIXED, **cfg)
+        dm.set_state(*fields)
+        dm.thermo_enable(pt, (tw, tw), 1.0, ti, mode)
+        dm.update_re(0)
+        dm.substep(0)
+        dm.run(0, args.warmup)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        dm.run(args.warmup, args.steps)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t1
+        done = args.warmup + args.steps
+        if done < HEATED_STEPS:
+            dm.run(done, HEATED_STEPS - done)
+        res[name] = {"value": mpts * args.steps / sec, "ms_per_step": sec * 1e3 / args.steps,
+                     "max_div_g": float(dm.mass_check().max()), "steps_total": max(done, HEATED_STEPS)}
+        doms[name] = dm
+    a, b = doms["mlp"], doms["table"]
+    ta, tb = a.thermo_get(), b.thermo_get()
+    qa, qb = a.get_state(), b.get_state()
+    diff = {k: float(np.abs(ta[k] - tb[k]).max() / np.abs(tb[k]).max()) for k in ("t", "d", "m", "k", "h")}
+    diff.update({k: float(np.abs(x - y).max() / np.abs(y).max()) for k, x, y in zip(("qx", "qy", "qz"), qa, qb)})
+    hrange = (float(tb["h"].min()), float(tb["h"].max()))
+    for dm in doms.values():
+        dm.close()
+    # one property refresh of every cell, device arrays
+    n = nc[0] * nc[1] * nc[2]
+    rh = torch.from_numpy(np.interp(np.random.default_rng(1).uniform(0.8, 1.2, n), tab[:, 1],
+                                    tab[:, 1] * tab[:, 3])).cuda()
+    outs = [torch.empty_like(rh) for _ in range(5)]
+    refresh_ms = {}
+    for name, mode in (("table", PROPS_TABLE), ("mlp", PROPS_MLP)):
+        import ctypes as C
+
+        def call():
+            return pt.lib.chs_props_from_rhoh(pt.h, C.c_void_p(rh.data_ptr()), n,
+                                              *[C.c_void_p(o.data_ptr()) for o in outs], None, None, mode, 1)
+        call()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(5):
+            call()
+        torch.cuda.synchronize()
+        refresh_ms[name] = (time.perf_counter() - t1) * 1e3 / 5
+    return {
+        "table": res["table"], "mlp": res["mlp"],
+        "mlp_vs_table_after_steps": HEATED_STEPS, "mlp_vs_table_rel_diff": diff,
+        "h_range_after": hrange,
+        "surrogate": {"train_s": train_s, "adam_steps": 6000, "batch": 4096, "samples": 1_000_000,
+                      "last_loss": loss, "fit_rel_err_h_0.8_1.2": fit_err},
+        "refresh_ms_all_cells": refresh_ms,
+        "what": f"heated channel {nc[0]}x{nc[1]}x{nc[2]}, isothermal walls h = 1.2, bulk h = 0.8, "
+                "4096-row table over the monotone rho*h window h in [0.65, 1.225], Pr 1, mixed precision, "
+                "exact projection; table vs MLP property refresh",
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -586,6 +688,8 @@ def main():
         if args.batch_default > 0:
             results["batch"] = batch_run(args.batch_default, args)
             out["mixed_vs_fp64"]["batch_ratio"] = results["batch"]["value"] / results["fp64c"]["value"]
+        if not args.no_heated:
+            results["heated"] = heated_run(nc, fields, cfg, args)
     out["forms"] = {k: {kk: vv for kk, vv in v.items() if kk != "roofline" or k != head} for k, v in
                     results.items()}
     if not args.no_cpu_baseline:

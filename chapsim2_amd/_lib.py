@@ -16,14 +16,22 @@ MEM_HOST, MEM_DEVICE = 0, 1
 EXPORTS = [
     "chs_abi_version", "chs_last_error", "chs_create", "chs_destroy", "chs_sync", "chs_op3d",
     "chs_poisson", "chs_set_state", "chs_get_state", "chs_update_re", "chs_substep", "chs_run",
-    "chs_momentum_rhs", "chs_mass_check", "chs_cfl", "chs_bulk", "chs_velocity_range", "chs_divergence",
+    "chs_momentum_rhs", "chs_mass_check", "chs_cfl", "chs_bulk", "chs_velocity_range",
+    "chs_divergence",
     "chs_batch_create", "chs_batch_destroy", "chs_batch_instance", "chs_batch_run",
     "chs_props_create", "chs_props_destroy", "chs_props_info", "chs_props_from_rhoh", "chs_props_from_h",
     "chs_mlp_create", "chs_mlp_destroy", "chs_mlp_info", "chs_mlp_get_params", "chs_mlp_set_params",
     "chs_mlp_grad", "chs_mlp_train", "chs_mlp_eval",
     "chs_inspect_coeff", "chs_inspect_op1d", "chs_inspect_projection", "chs_profile_begin", "chs_profile_end", "chs_tune_report",
     "chs_op3d_f32", "chs_poisson_f32", "chs_tune_force", "chs_tune_set",
+    "chs_props_set_surrogate", "chs_thermo_enable", "chs_thermo_set_rhoh", "chs_thermo_set_mode", "chs_thermo_get",
+    "chs_energy_rhs",
 ]
+
+
+class ChsThermoDesc(C.Structure):
+    _fields_ = [("props", C.c_void_p), ("mode", C.c_int), ("t_wall", C.c_double * 2), ("prandtl", C.c_double),
+                ("t_init", C.c_double)]
 
 
 class ChsDesc(C.Structure):
@@ -88,6 +96,12 @@ def load():
     L.chs_cfl.argtypes = [vp, dp]
     L.chs_bulk.argtypes = [vp, dp]
     L.chs_velocity_range.argtypes = [vp, dp]
+    L.chs_props_set_surrogate.argtypes = [vp, vp]
+    L.chs_thermo_enable.argtypes = [vp, C.POINTER(ChsThermoDesc)]
+    L.chs_thermo_set_rhoh.argtypes = [vp, vp, C.c_int]
+    L.chs_thermo_set_mode.argtypes = [vp, C.c_int]
+    L.chs_thermo_get.argtypes = [vp] + [vp] * 9 + [C.c_int]
+    L.chs_energy_rhs.argtypes = [vp, C.c_int, vp, C.c_int]
     L.chs_divergence.argtypes = [vp, vp, C.c_int]
     L.chs_inspect_coeff.restype = C.c_double
     L.chs_inspect_coeff.argtypes = [C.c_int] * 6

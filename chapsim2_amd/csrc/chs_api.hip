@@ -357,10 +357,68 @@ int chs_substep(chs_handle_t* H, int isub) {
 
 int chs_run(chs_handle_t* H, int iter0, int nsteps) {
   FLOW_OR_FAIL(H);
+  if (F.thermo && hipMemsetAsync(F.props_flag, 0, sizeof(int), H->h.stream) != hipSuccess)
+    return fail(CHS_EHIP, "flag reset");
   for (int it = iter0 + 1; it <= iter0 + nsteps; ++it) {
     F.update_re(it);
     for (int isub = 1; isub <= 3; ++isub) RET(F.substep(isub));
   }
+  int nbad = 0;
+  if (F.thermo &&
+      hipMemcpyAsync(&nbad, F.props_flag, sizeof(int), hipMemcpyDeviceToHost, H->h.stream) != hipSuccess)
+    return fail(CHS_EHIP, "flag read");
+  hipError_t he = hipStreamSynchronize(H->h.stream);
+  if (he != hipSuccess) return fail(CHS_EHIP, hipGetErrorString(he));
+  // the reference stops when rho*h leaves the table (src/input_thermo.f90:402-413)
+  if (nbad) return fail(CHS_ERANGE, "rho*h left the property table (" + std::to_string(nbad) + " cell refreshes clamped)");
+  return CHS_OK;
+}
+
+int chs_thermo_enable(chs_handle_t* H, const chs_thermo_desc_t* d) {
+  FLOW_OR_FAIL(H);
+  if (!d) return fail(CHS_EINVAL, "invalid argument");
+  const std::string e = F.thermo_enable(*d);
+  if (!e.empty()) return fail(e.find("not supported") != std::string::npos || e.find("need y walls") != std::string::npos
+                                  ? CHS_EUNSUP : CHS_EINVAL, e);
+  return CHS_OK;
+}
+
+int chs_thermo_set_rhoh(chs_handle_t* H, const double* rhoh, int mem) {
+  FLOW_OR_FAIL(H);
+  if (!rhoh || (mem != CHS_MEM_HOST && mem != CHS_MEM_DEVICE)) return fail(CHS_EINVAL, "invalid argument");
+  long nb = 0;
+  RET(F.thermo_set_rhoh(rhoh, mem, &nb));
+  if (nb) return fail(CHS_ERANGE, "rho*h is out of the table's range (" + std::to_string(nb) + " cells clamped)");
+  return CHS_OK;
+}
+
+int chs_thermo_set_mode(chs_handle_t* H, int mode) {
+  FLOW_OR_FAIL(H);
+  if (!F.thermo) return fail(CHS_EINVAL, "variable properties not enabled");
+  if (mode != CHS_PROPS_TABLE && mode != CHS_PROPS_MLP) return fail(CHS_EINVAL, "unknown property mode");
+  if (mode == CHS_PROPS_MLP && !F.props->surrogate) return fail(CHS_EINVAL, "no surrogate attached to the table");
+  F.props_mode = mode;
+  return CHS_OK;
+}
+
+int chs_thermo_get(chs_handle_t* H, double* rhoh, double* hh, double* t, double* k, double* d, double* m, double* gx,
+                   double* gy, double* gz, int mem) {
+  FLOW_OR_FAIL(H);
+  if (mem != CHS_MEM_HOST && mem != CHS_MEM_DEVICE) return fail(CHS_EINVAL, "invalid mem kind");
+  double* out[9] = {rhoh, hh, t, k, d, m, gx, gy, gz};
+  RET(F.thermo_get(out, mem));
+  return CHS_OK;
+}
+
+int chs_energy_rhs(chs_handle_t* H, int isub, double* rhs, int mem) {
+  FLOW_OR_FAIL(H);
+  if (!F.thermo) return fail(CHS_EINVAL, "variable properties not enabled");
+  if (!rhs || isub < 1 || isub > 3 || (mem != CHS_MEM_HOST && mem != CHS_MEM_DEVICE))
+    return fail(CHS_EINVAL, "invalid argument");
+  RET(F.energy_rhs(isub));
+  hipMemcpyKind k = (mem == CHS_MEM_HOST) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  if (hipMemcpyAsync(rhs, F.eres.p, sizeof(double) * F.eres.size(), k, H->h.stream) != hipSuccess)
+    return fail(CHS_EHIP, "copy out");
   hipError_t he = hipStreamSynchronize(H->h.stream);
   if (he != hipSuccess) return fail(CHS_EHIP, hipGetErrorString(he));
   return CHS_OK;

@@ -8,6 +8,7 @@
 
 #include "chs_handle.h"
 #include "chs_poisson.h"
+#include "chs_props.h"
 
 namespace chs {
 
@@ -37,6 +38,14 @@ struct RhsWork {
   Field<T> qzdx, qzix, qziy, qzdy, qziz, qzdz, div;
   // accumulators per momentum component: [comp][x-part, y-part, z-part, pfc]
   Field<T> acc[3][4];
+  // variable properties: g = rho u and mu (FP64: the state itself; FP32:
+  // rounded copies), the interpolated g of each convective product, the
+  // interpolated viscosities and the wall viscosity (nx,4,nz)
+  Field<T> gx, gy, gz, visc;
+  Field<T> gxix, gxiy, gxiz, gyix, gyiy, gyiz, gzix, gziy, gziz;
+  Field<T> muiy, mupcc, muixy, muixz, muiyz;
+  T* fby_mu = nullptr;
+  T* fby_mwall = nullptr;  // wall viscosity of the y interpolation (rows 3,4 = rows 1,2)
   // wall data (nx,4,nz) and wall-plane scratch
   T *fby_u = nullptr, *fby_v = nullptr, *fby_w = nullptr, *fby_p = nullptr, *fby_pn = nullptr;
   T *fby_conv = nullptr, *fby_visc = nullptr, *fby_div = nullptr;
@@ -68,6 +77,29 @@ class Flow {
   // non-finite values among them
   std::string velocity_range(double out[6], long* nonfinite);
   std::string cell_velocity();      // P2C interpolations of qx, qy, qz into w64.qxix, qyiy, qziz
+
+  // ---- variable properties (heated channel, isothermal y walls) ----
+  // set-up: property table (and mode TABLE / MLP), wall temperatures,
+  // reference Prandtl number, uniform initial state from t_init; g from q
+  std::string thermo_enable(const chs_thermo_desc_t& d);
+  // rho*h per cell (FP64, ccc): properties refreshed, g = q rho
+  std::string thermo_set_rhoh(const double* rhoh, int mem, long* clamped);
+  std::string thermo_get(double* out[9], int mem);
+  // energy sub-step (Solve_energy_eq): rhs, rho*h += rhs, property refresh
+  std::string energy_substep(int isub);
+  // energy right-hand side only, dt-scaled (into eres)
+  std::string energy_rhs(int isub);
+  // q <-> g with the density interpolated to each velocity location
+  std::string convert_qg(bool to_g);
+  bool thermo = false;
+  int props_mode = CHS_PROPS_TABLE;
+  const chs_props* props = nullptr;
+  double wall_prop[2][7] = {};  // h, d, m, k, t, b, cp of each y wall
+  double prandtl = 1.0, rprren = 0.0;
+  Field<double> rhoh, hent, temp, kcond, dens, eres, erhs0, ex, ey, ez;
+  Field<double> hpcc, hcpc, hccp, kpcc, kcpc, kccp, tpcc, tcpc, tccp, dface[3];
+  double *fby_h = nullptr, *fby_k = nullptr, *fby_d = nullptr, *fby_t = nullptr, *fby_q = nullptr;
+  int* props_flag = nullptr;
   std::string copy_rhs(double* mx, double* my, double* mz, int mem);
 
   Handle* H = nullptr;
@@ -132,6 +164,10 @@ class Flow {
                       const T* fbc_cov, const T* fbc_tau, const Field<T>& ca, const Field<T>& cb,
                       const Field<T>& ta, T ts, const Field<T>* tb, T tc, int lane = 0);
   template <typename T> std::string momentum_rhs_t(RhsWork<T>& w, int isub, bool fuse);
+  template <typename T> std::string thermo_first_level(RhsWork<T>& w);
+  template <typename T> std::string alloc_thermo_work(RhsWork<T>& w);
+  std::string refresh_props();
+  std::string energy_terms(int isub);  // ex, ey, ez: the x, y, z parts of the energy right-hand side
   template <typename T> std::string alloc_work(RhsWork<T>& w, bool alias_state);
   template <typename T> std::string alloc(Field<T>& f, int e1, int e2, int e3);
   std::vector<void*> owned;

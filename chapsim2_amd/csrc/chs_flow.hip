@@ -24,6 +24,7 @@
 
 namespace chs {
 
+
 #define FCHK(x)                                                             \
   do {                                                                      \
     hipError_t e_ = (x);                                                    \
@@ -194,21 +195,75 @@ __global__ void k_extract_walls(T* __restrict__ pl, const T* __restrict__ a, int
 }
 
 // div and visc wall values (nx,4,nz) from the three wall-plane derivatives
+// (times the wall viscosity mu (nx,4,nz) with variable properties, else 1)
 template <typename T>
 __global__ void k_wall_fbc(T* __restrict__ fdiv, T* __restrict__ fvisc, const T* __restrict__ dxw,
-                           const T* __restrict__ dyw, const T* __restrict__ dzw, int n1, int n3) {
+                           const T* __restrict__ dyw, const T* __restrict__ dzw, int n1, int n3,
+                           const T* __restrict__ mu) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)n1 * n3) return;
   long i = t % n1, k = t / n1;
   for (int s = 0; s < 2; ++s) {
     long p = i + (long)n1 * (s + 2 * k);
     T dv = dxw[p] + dyw[p] + dzw[p];
-    T vs = (dyw[p] - T(ONE_THIRD) * dv) * T(2) * T(1);
+    T vs = (dyw[p] - T(ONE_THIRD) * dv) * T(2) * (mu ? mu[i + (long)n1 * (s + 4 * k)] : T(1));
     fdiv[i + (long)n1 * (s + 4 * k)] = dv;
     fdiv[i + (long)n1 * (s + 2 + 4 * k)] = dv;
     fvisc[i + (long)n1 * (s + 4 * k)] = vs;
     fvisc[i + (long)n1 * (s + 2 + 4 * k)] = vs;
   }
+}
+
+// ---- variable-property kernels ----
+template <typename T>
+__global__ void k_fill(T* __restrict__ o, T v, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = v;
+}
+// wall rows j = 0 and n2 - 1 of a (times b when given) into (n1,4,n3):
+// rows 1,2 and again rows 3,4 (extract_dirichlet_fbcy, src/bc_dirichlet.f90:42-56)
+template <typename T>
+__global__ void k_wall_rows4(T* __restrict__ o, const T* __restrict__ a, const T* __restrict__ b, int n1, int n2,
+                             int n3) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)n1 * n3) return;
+  long i = t % n1, k = t / n1;
+  for (int s = 0; s < 2; ++s) {
+    const long src = i + (long)n1 * ((s ? n2 - 1 : 0) + (long)n2 * k);
+    const T v = b ? a[src] * b[src] : a[src];
+    o[i + (long)n1 * (s + 4 * k)] = v;
+    o[i + (long)n1 * (s + 2 + 4 * k)] = v;
+  }
+}
+// g = q d (IQ2G) or q = g / d (IG2Q), src/para_conversion.f90:41-85
+__global__ void k_q2g(double* __restrict__ g, const double* __restrict__ q, const double* __restrict__ d, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) g[i] = q[i] * d[i];
+}
+__global__ void k_g2q(double* __restrict__ q, const double* __restrict__ g, const double* __restrict__ d, long n,
+                      float* __restrict__ q32) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double v = g[i] / d[i];
+    q[i] = v;
+    if (q32) q32[i] = (float)v;
+  }
+}
+// energy: rhs = ex + (ey + ez) (the pencil order of src/eq_energy.f90:386-391),
+// Calculate_energy_fractional_step (:148-176): res = dt (tg rhs + tz rhs0),
+// rhs0 = rhs; with update, rho*h += res (Solve_energy_eq, :440)
+__global__ void k_energy_combine(const double* __restrict__ ex, const double* __restrict__ ey,
+                                 const double* __restrict__ ez, double* __restrict__ r0, double* __restrict__ res,
+                                 double* __restrict__ rhoh, long n, double tg, double tz, double dt) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double yz = ey[i] + ez[i];
+  const double cur = ex[i] + yz;
+  double tot = tg * cur + tz * r0[i];
+  r0[i] = cur;
+  const double rv = dt * tot;
+  if (res) res[i] = rv;
+  if (rhoh) rhoh[i] = rhoh[i] + rv;
 }
 
 template <typename T>
@@ -511,6 +566,8 @@ std::string Flow::create(Handle* H, std::unique_ptr<Flow>* out) {
 void Flow::update_re(int iter) {
   const chs_desc_t& d = H->desc;
   rre = (iter < d.init_re_to) ? 1.0 / d.reninit : 1.0 / d.ren;
+  // Update_PrGr, src/tools_solver.f90:63: rPrRen = rre k0 / (m0 cp0)
+  rprren = rre / prandtl;
 }
 
 template <typename T>
@@ -620,6 +677,7 @@ std::string Flow::set_state(const double* hx, const double* hy, const double* hz
   FCHK(hipMemcpyAsync(qz.p, hz, sizeof(double) * qz.size(), k, H->stream));
   FCHK(hipMemcpyAsync(pr.p, hp, sizeof(double) * pr.size(), k, H->stream));
   FCHK(hipMemsetAsync(pcor.p, 0, sizeof(double) * pcor.size(), H->stream));
+  if (thermo) SCHK(convert_qg(true));
   FCHK(hipStreamSynchronize(H->stream));
   return "";
 }
@@ -701,6 +759,7 @@ std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub, bool fuse) {
   SCHK(op2<T>(w, 2, 0, w.qz, D1_C2P, B[0][VW], w.qzdx, nullptr, MIDP_C2P, B[0][VW], w.qzix, nullptr));
   SCHK(op2<T>(w, 2, 1, w.qz, MIDP_C2P, B[1][VW], w.qziy, fw, D1_C2P, B[1][VW], w.qzdy, fw));
   SCHK(op2<T>(w, 2, 2, w.qz, MIDP_P2C, B[2][VW], w.qziz, nullptr, D1_P2C, B[2][VW], w.qzdz, nullptr));
+  if (thermo) SCHK(thermo_first_level<T>(w));
   SCHK(join());
   hipLaunchKernelGGL(k_div3<T>, grid1(w.div.size()), dim3(256), 0, s, w.div.p, w.qxdx.p, w.qydy.p, w.qzdz.p,
                      w.div.size());
@@ -722,7 +781,7 @@ std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub, bool fuse) {
     Field<T> pa2{w.plane_a, Ext3{{nx, 2, pz}}}, pd{w.plane_d, Ext3{{nx, 2, nz}}};
     SCHK(op<T>(w, 2, D1_P2C, B[2][VW], pa2, pd, nullptr));
     hipLaunchKernelGGL(k_wall_fbc<T>, grid1(nf2), dim3(256), 0, s, w.fby_div, w.fby_visc, w.plane_b, w.plane_c,
-                       w.plane_d, nx, nz);
+                       w.plane_d, nx, nz, (const T*)(thermo ? w.fby_mu : nullptr));
   }
   const T third = T(-ONE_THIRD), two = T(2);
   const T* fconv = wall_y ? w.fby_conv : nullptr;
@@ -730,32 +789,64 @@ std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub, bool fuse) {
   // ---- second-level operators, one lane per momentum component; each
   // lane ends with its fractional-step combine ----
   SCHK(fork());
-  // x-momentum, :798-948
   Field<T>* X = w.acc[0];
+  Field<T>* Y = w.acc[1];
+  Field<T>* Z = w.acc[2];
+  if (!thermo) {
+  // x-momentum, :798-948
   SCHK(cov_tau<T>(w, 0, D1_C2P, mcov[0][0], mtau[0][0], X[0], nullptr, nullptr, w.qxix, w.qxix, w.qxdx, two,
                   &w.div, third, 0));
   SCHK(cov_tau<T>(w, 1, D1_P2C, mcov[0][1], mtau[0][1], X[1], nullptr, nullptr, w.qyix, w.qxiy, w.qxdy, one,
                   &w.qydx, one, 0));
   SCHK(cov_tau<T>(w, 2, D1_P2C, mcov[0][2], mtau[0][2], X[2], nullptr, nullptr, w.qzix, w.qxiz, w.qxdz, one,
                   &w.qzdx, one, 0));
-  SCHK(op1(0, 0, D1_C2P, B[0][VP], w.pr, X[3], nullptr, m1));
   // y-momentum, :960-1235
-  Field<T>* Y = w.acc[1];
   SCHK(cov_tau<T>(w, 0, D1_P2C, mcov[1][0], mtau[1][0], Y[0], nullptr, nullptr, w.qxiy, w.qyix, w.qxdy, one,
                   &w.qydx, one, 1));
   SCHK(cov_tau<T>(w, 1, D1_C2P, mcov[1][1], mtau[1][1], Y[1], fconv, fvisc, w.qyiy, w.qyiy, w.qydy, two, &w.div,
                   third, 1));
   SCHK(cov_tau<T>(w, 2, D1_P2C, mcov[1][2], mtau[1][2], Y[2], nullptr, nullptr, w.qyiz, w.qziy, w.qzdy, one,
                   &w.qydz, one, 1));
-  SCHK(op1(1, 1, D1_C2P, B[1][VP], w.pr, Y[3], wall_y ? w.fby_pn : nullptr, m1));
   // z-momentum, :1248-1480
-  Field<T>* Z = w.acc[2];
   SCHK(cov_tau<T>(w, 0, D1_P2C, mcov[2][0], mtau[2][0], Z[0], nullptr, nullptr, w.qxiz, w.qzix, w.qxdz, one,
                   &w.qzdx, one, 2));
   SCHK(cov_tau<T>(w, 1, D1_P2C, mcov[2][1], mtau[2][1], Z[1], nullptr, nullptr, w.qyiz, w.qziy, w.qzdy, one,
                   &w.qydz, one, 2));
   SCHK(cov_tau<T>(w, 2, D1_C2P, mcov[2][2], mtau[2][2], Z[2], nullptr, nullptr, w.qziz, w.qziz, w.qzdz, two,
                   &w.div, third, 2));
+  } else {
+    // variable properties: the convective product takes interpolated g as
+    // its first factor and every viscous operand is multiplied by the
+    // interpolated viscosity (the is_thermo branches of :805-1480), so the
+    // convective and viscous operators stay two launches each
+    const T r = T(rre);
+    auto ct = [&](int lane, int dir, int opk, const int* bcc, const int* bct, Field<T>& out, const T* fbcc,
+                  const T* fbct, const Field<T>& ca, const Field<T>& cb, const Field<T>& ta, T ts, const Field<T>* tb,
+                  T tc, const Field<T>& mu) -> std::string {
+      SCHK(op<T>(w, dir, opk, bcc, ca, out, fbcc, m1, nullptr, one, &cb, one, 0, nullptr, T(0), nullptr, T(0), lane));
+      return op<T>(w, dir, opk, bct, ta, out, fbct, ts, tb, tc, &mu, r, 1, nullptr, T(0), nullptr, T(0), lane);
+    };
+    SCHK(ct(0, 0, D1_C2P, mcov[0][0], mtau[0][0], X[0], nullptr, nullptr, w.gxix, w.qxix, w.qxdx, two, &w.div, third,
+            w.visc));
+    SCHK(ct(0, 1, D1_P2C, mcov[0][1], mtau[0][1], X[1], nullptr, nullptr, w.gyix, w.qxiy, w.qxdy, one, &w.qydx, one,
+            w.muixy));
+    SCHK(ct(0, 2, D1_P2C, mcov[0][2], mtau[0][2], X[2], nullptr, nullptr, w.gzix, w.qxiz, w.qxdz, one, &w.qzdx, one,
+            w.muixz));
+    SCHK(ct(1, 0, D1_P2C, mcov[1][0], mtau[1][0], Y[0], nullptr, nullptr, w.gxiy, w.qyix, w.qxdy, one, &w.qydx, one,
+            w.muixy));
+    SCHK(ct(1, 1, D1_C2P, mcov[1][1], mtau[1][1], Y[1], fconv, fvisc, w.qyiy, w.gyiy, w.qydy, two, &w.div, third,
+            w.visc));
+    SCHK(ct(1, 2, D1_P2C, mcov[1][2], mtau[1][2], Y[2], nullptr, nullptr, w.qyiz, w.gziy, w.qzdy, one, &w.qydz, one,
+            w.muiyz));
+    SCHK(ct(2, 0, D1_P2C, mcov[2][0], mtau[2][0], Z[0], nullptr, nullptr, w.gxiz, w.qzix, w.qxdz, one, &w.qzdx, one,
+            w.muixz));
+    SCHK(ct(2, 1, D1_P2C, mcov[2][1], mtau[2][1], Z[1], nullptr, nullptr, w.gyiz, w.qziy, w.qzdy, one, &w.qydz, one,
+            w.muiyz));
+    SCHK(ct(2, 2, D1_C2P, mcov[2][2], mtau[2][2], Z[2], nullptr, nullptr, w.gziz, w.qziz, w.qzdz, two, &w.div, third,
+            w.visc));
+  }
+  SCHK(op1(0, 0, D1_C2P, B[0][VP], w.pr, X[3], nullptr, m1));
+  SCHK(op1(1, 1, D1_C2P, B[1][VP], w.pr, Y[3], wall_y ? w.fby_pn : nullptr, m1));
   SCHK(op1(2, 2, D1_C2P, B[2][VP], w.pr, Z[3], nullptr, m1));
   // fractional step (and the forcing's block partials of component x)
   const Mesh& m = H->mesh;
@@ -764,7 +855,8 @@ std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub, bool fuse) {
   // updated here (x is, when the flow is forced: the correction needs the
   // whole volume's sum first)
   Field<T>* rhs0[3] = {&w.mx0, &w.my0, &w.mz0};
-  double* qs[3] = {qx.p, qy.p, qz.p};
+  // the updated state: q, or g = rho u with variable properties
+  double* qs[3] = {thermo ? w64.gx.p : qx.p, thermo ? w64.gy.p : qy.p, thermo ? w64.gz.p : qz.p};
   for (int c = 0; c < 3; ++c) {
     long n = w.acc[c][0].size();
     double* part = (c == 0 && forced) ? partial : nullptr;
@@ -778,7 +870,7 @@ std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub, bool fuse) {
     int nb = (int)((w.acc[0][0].size() + 255) / 256);
     hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, lane_stream(0), partial, nb, vol, bulk);
     if (fuse)
-      hipLaunchKernelGGL(k_update<T>, grid1(qx.size()), dim3(256), 0, lane_stream(0), qx.p, w.acc[0][0].p,
+      hipLaunchKernelGGL(k_update<T>, grid1(qx.size()), dim3(256), 0, lane_stream(0), qs[0], w.acc[0][0].p,
                          (const double*)bulk, qx.size(), px, ny, (const double*)nullptr);
   }
   SCHK(join());
@@ -800,21 +892,36 @@ std::string Flow::momentum_rhs(int isub, bool fuse) {
     hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr.size()), dim3(256), 0, s, w32.pr.p, pr.p, pr.size());
     q32_fresh = true;
   }
+  if (thermo) {
+    // g and mu change every sub-step (projection, property refresh)
+    hipStream_t s = H->stream;
+    for (auto pr2 : {std::make_pair(&w32.gx, &w64.gx), std::make_pair(&w32.gy, &w64.gy),
+                     std::make_pair(&w32.gz, &w64.gz), std::make_pair(&w32.visc, &w64.visc)})
+      hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr2.second->size()), dim3(256), 0, s, pr2.first->p,
+                         pr2.second->p, pr2.second->size());
+  }
   const std::string r = momentum_rhs_t<float>(w32, isub, fuse);
   if (fuse) q32_fresh = false;  // the state was updated
   return r;
 }
 
 std::string Flow::divergence(double* d) {
+  // of q, or of g = rho u with variable properties (src/eq_continuity.f90:114-117)
   Field<double> D{d, Ext3{{nx, ny, nz}}};
-  SCHK(op<double>(w64, 0, D1_P2C, ibc[0][VU], qx, D, nullptr));
-  SCHK(op<double>(w64, 1, D1_P2C, ibc[1][VV], qy, D, nullptr, 1.0, nullptr, 1.0, nullptr, 1.0, 1));
-  SCHK(op<double>(w64, 2, D1_P2C, ibc[2][VW], qz, D, nullptr, 1.0, nullptr, 1.0, nullptr, 1.0, 1));
+  const Field<double>& ux = thermo ? w64.gx : qx;
+  const Field<double>& uy = thermo ? w64.gy : qy;
+  const Field<double>& uz = thermo ? w64.gz : qz;
+  SCHK(op<double>(w64, 0, D1_P2C, ibc[0][VU], ux, D, nullptr));
+  SCHK(op<double>(w64, 1, D1_P2C, ibc[1][VV], uy, D, nullptr, 1.0, nullptr, 1.0, nullptr, 1.0, 1));
+  SCHK(op<double>(w64, 2, D1_P2C, ibc[2][VW], uz, D, nullptr, 1.0, nullptr, 1.0, nullptr, 1.0, 1));
   return "";
 }
 
 std::string Flow::substep(int isub) {
   hipStream_t s = H->stream;
+  // variable properties: the energy equation first (src/chapsim.f90:300-304;
+  // the initial sub-step is momentum only, :135)
+  if (thermo && isub > 0) SCHK(energy_substep(isub));
   SCHK(momentum_rhs(isub, true));
   // Poisson RHS: div(q) / (alpha * sigma2 * dt), then the solve
   SCHK(divergence(pcor.p));
@@ -824,19 +931,25 @@ std::string Flow::substep(int isub) {
   // the mixed form's FP32 copies of the new pressure and velocity are
   // stored by the kernels that make them
   hipLaunchKernelGGL(k_add, grid1(pr.size()), dim3(256), 0, s, pr.p, pcor.p, pr.size(), mixed ? w32.pr.p : nullptr);
-  // Correct_massflux, one lane per velocity component
+  // Correct_massflux, one lane per velocity component (on g with variable
+  // properties, :1609-1613, then q = g / rho, :1915)
   const double sc = dt * ta[isub] * sigma2p;
+  Field<double>& ux = thermo ? w64.gx : qx;
+  Field<double>& uy = thermo ? w64.gy : qy;
+  Field<double>& uz = thermo ? w64.gz : qz;
+  const bool st32 = mixed && !thermo;
   SCHK(fork());
-  SCHK(op<double>(w64, 0, D1_C2P, ibc[0][VP], pcor, qx, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1, nullptr, 0.0,
-                  nullptr, 0.0, 0, mixed ? w32.qx.p : nullptr));
-  SCHK(op<double>(w64, 1, D1_C2P, ibc[1][VP], pcor, qy, wall_y ? w64.fby_p : nullptr, 1.0, nullptr, 1.0, nullptr,
-                  -sc, 1, nullptr, 0.0, nullptr, 0.0, 1, mixed ? w32.qy.p : nullptr));
-  SCHK(op<double>(w64, 2, D1_C2P, ibc[2][VP], pcor, qz, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1, nullptr, 0.0,
-                  nullptr, 0.0, 2, mixed ? w32.qz.p : nullptr));
+  SCHK(op<double>(w64, 0, D1_C2P, ibc[0][VP], pcor, ux, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1, nullptr, 0.0,
+                  nullptr, 0.0, 0, st32 ? w32.qx.p : nullptr));
+  SCHK(op<double>(w64, 1, D1_C2P, ibc[1][VP], pcor, uy, wall_y ? w64.fby_p : nullptr, 1.0, nullptr, 1.0, nullptr,
+                  -sc, 1, nullptr, 0.0, nullptr, 0.0, 1, st32 ? w32.qy.p : nullptr));
+  SCHK(op<double>(w64, 2, D1_C2P, ibc[2][VP], pcor, uz, nullptr, 1.0, nullptr, 1.0, nullptr, -sc, 1, nullptr, 0.0,
+                  nullptr, 0.0, 2, st32 ? w32.qz.p : nullptr));
   if (wall_y)
-    hipLaunchKernelGGL(k_enforce_walls, grid1((long)nx * nz), dim3(256), 0, lane_stream(1), qy.p, nx, py, nz,
-                       w64.fby_v, mixed ? w32.qy.p : nullptr);
+    hipLaunchKernelGGL(k_enforce_walls, grid1((long)nx * nz), dim3(256), 0, lane_stream(1), uy.p, nx, py, nz,
+                       w64.fby_v, st32 ? w32.qy.p : nullptr);
   SCHK(join());
+  if (thermo) SCHK(convert_qg(false));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("substep: ") + hipGetErrorString(e);
   q32_fresh = mixed;
@@ -941,6 +1054,288 @@ std::string Flow::mass_check(double out[3]) {
     for (double v : h) mx = (v > mx || v != v) ? v : mx;
     out[w] = mx;
   }
+  return "";
+}
+
+// ---------------------------------------------------------------------------
+// Variable properties (heated channel with isothermal, stationary y walls)
+// ---------------------------------------------------------------------------
+namespace {
+// table branch of ftp_refresh_thermal_properties_from_T_undim
+// (src/input_thermo.f90:213-241): out = h, d, m, k, t, b, cp
+void props_from_t_host(const chs_props* p, double t, double out[7]) {
+  const std::vector<double>& c = p->col[CHS_PC_T];
+  int i1 = 0, i2 = p->n - 1;
+  while (i2 - i1 > 1) {
+    const int im = i1 + (i2 - i1) / 2;
+    const double d1 = c[i1] - t, dm = c[im] - t;
+    if (d1 * dm > 1.0e-20) i1 = im;
+    else i2 = im;
+  }
+  const double w1 = (c[i2] - t) / (c[i2] - c[i1]), w2 = 1.0 - w1;
+  auto in = [&](int col) {
+    const double a = w1 * p->col[col][i1];
+    const double b = w2 * p->col[col][i2];
+    return a + b;
+  };
+  out[1] = in(CHS_PC_D);
+  out[2] = in(CHS_PC_M);
+  out[3] = in(CHS_PC_K);
+  out[0] = in(CHS_PC_H);
+  out[5] = in(CHS_PC_B);
+  out[6] = in(CHS_PC_CP);
+  out[4] = t;
+}
+}  // namespace
+
+template <typename T>
+std::string Flow::alloc_thermo_work(RhsWork<T>& w) {
+  SCHK(alloc(w.gx, px, ny, nz)); SCHK(alloc(w.gy, nx, py, nz)); SCHK(alloc(w.gz, nx, ny, pz));
+  SCHK(alloc(w.visc, nx, ny, nz));
+  SCHK(alloc(w.gxix, nx, ny, nz)); SCHK(alloc(w.gxiy, px, py, nz)); SCHK(alloc(w.gxiz, px, ny, pz));
+  SCHK(alloc(w.gyix, px, py, nz)); SCHK(alloc(w.gyiy, nx, ny, nz)); SCHK(alloc(w.gyiz, nx, py, pz));
+  SCHK(alloc(w.gzix, px, ny, pz)); SCHK(alloc(w.gziy, nx, py, pz)); SCHK(alloc(w.gziz, nx, ny, nz));
+  SCHK(alloc(w.muiy, nx, py, nz)); SCHK(alloc(w.mupcc, px, ny, nz)); SCHK(alloc(w.muixy, px, py, nz));
+  SCHK(alloc(w.muixz, px, ny, pz)); SCHK(alloc(w.muiyz, nx, py, pz));
+  const long nf = (long)nx * 4 * nz;
+  std::vector<T> hm(nf);
+  for (long t = 0; t < nf; ++t) hm[t] = T(wall_prop[(t / nx) % 2][2]);
+  for (T** q : {&w.fby_mu, &w.fby_mwall}) {
+    FCHK(hipMalloc(q, sizeof(T) * nf));
+    owned.push_back(*q);
+  }
+  FCHK(hipMemcpy(w.fby_mwall, hm.data(), sizeof(T) * nf, hipMemcpyHostToDevice));
+  return "";
+}
+
+// interpolated g of the convective products and the viscosity interpolations
+// (src/eq_momentum2.f90:576-703), issued on the three lanes of the
+// first-level operators
+template <typename T>
+std::string Flow::thermo_first_level(RhsWork<T>& w) {
+  const int(*B)[5][2] = ibc;
+  int ftp[3][2];
+  for (int d = 0; d < 3; ++d)
+    for (int s2 = 0; s2 < 2; ++s2) ftp[d][s2] = (ibc[d][VT][s2] == BC_NEUMANN) ? BC_DIRICHLET : ibc[d][VT][s2];
+  const T* fu = wall_y ? w.fby_u : nullptr;
+  const T* fv = wall_y ? w.fby_v : nullptr;
+  const T* fw = wall_y ? w.fby_w : nullptr;
+  auto ip = [&](int lane, int dir, int opk, const int* bc, const Field<T>& A, Field<T>& out, const T* fbc) {
+    return op<T>(w, dir, opk, bc, A, out, fbc, T(1), nullptr, T(1), nullptr, T(1), 0, nullptr, T(0), nullptr, T(0),
+                 lane);
+  };
+  SCHK(ip(0, 0, MIDP_P2C, B[0][VU], w.gx, w.gxix, nullptr));
+  SCHK(ip(0, 1, MIDP_C2P, B[1][VU], w.gx, w.gxiy, fu));
+  SCHK(ip(0, 2, MIDP_C2P, B[2][VU], w.gx, w.gxiz, nullptr));
+  SCHK(ip(1, 0, MIDP_C2P, B[0][VV], w.gy, w.gyix, nullptr));
+  SCHK(ip(1, 1, MIDP_P2C, B[1][VV], w.gy, w.gyiy, fv));
+  SCHK(ip(1, 2, MIDP_C2P, B[2][VV], w.gy, w.gyiz, nullptr));
+  SCHK(ip(2, 0, MIDP_C2P, B[0][VW], w.gz, w.gzix, nullptr));
+  SCHK(ip(2, 1, MIDP_C2P, B[1][VW], w.gz, w.gziy, fw));
+  SCHK(ip(2, 2, MIDP_P2C, B[2][VW], w.gz, w.gziz, nullptr));
+  // muiy_cpc (y, wall viscosity), muixy_ppc = x of it, muiyz_cpp = z of it,
+  // its wall rows as fbcy_mu_c4c; mu_pcc and muixz_pcp = z of mu_pcc
+  SCHK(ip(1, 1, MIDP_C2P, ftp[1], w.visc, w.muiy, wall_y ? w.fby_mwall : nullptr));
+  SCHK(ip(1, 0, MIDP_C2P, ftp[0], w.muiy, w.muixy, nullptr));
+  SCHK(ip(1, 2, MIDP_C2P, ftp[2], w.muiy, w.muiyz, nullptr));
+  if (wall_y)
+    hipLaunchKernelGGL(k_wall_rows4<T>, grid1((long)nx * nz), dim3(256), 0, lane_stream(1), w.fby_mu, w.muiy.p,
+                       (const T*)nullptr, nx, py, nz);
+  SCHK(ip(2, 0, MIDP_C2P, ftp[0], w.visc, w.mupcc, nullptr));
+  SCHK(ip(2, 2, MIDP_C2P, ftp[2], w.mupcc, w.muixz, nullptr));
+  return "";
+}
+
+std::string Flow::thermo_enable(const chs_thermo_desc_t& d) {
+  if (thermo) return "variable properties: already enabled";
+  if (!wall_y) return "variable properties: need y walls";
+  const chs_desc_t& hd = H->desc;
+  for (int s2 = 0; s2 < 2; ++s2)
+    for (int v = 0; v < 3; ++v)
+      if (hd.uwall[s2][v] != 0.0) return "variable properties: moving walls are not supported";
+  if (!d.props) return "variable properties: no property table";
+  if (d.mode != CHS_PROPS_TABLE && d.mode != CHS_PROPS_MLP) return "variable properties: unknown mode";
+  if (d.mode == CHS_PROPS_MLP && !d.props->surrogate) return "variable properties: no surrogate attached";
+  if (!(d.prandtl > 0.0)) return "variable properties: the Prandtl number must be positive";
+  props = d.props;
+  props_mode = d.mode;
+  prandtl = d.prandtl;
+  for (int s2 = 0; s2 < 2; ++s2) props_from_t_host(props, d.t_wall[s2], wall_prop[s2]);
+  double p0[7];
+  props_from_t_host(props, d.t_init, p0);
+  // FP64 fields
+  for (Field<double>* f : {&rhoh, &hent, &temp, &kcond, &dens, &eres, &erhs0, &ex, &ey, &ez})
+    SCHK(alloc(*f, nx, ny, nz));
+  SCHK(alloc(hpcc, px, ny, nz)); SCHK(alloc(hcpc, nx, py, nz)); SCHK(alloc(hccp, nx, ny, pz));
+  SCHK(alloc(kpcc, px, ny, nz)); SCHK(alloc(kcpc, nx, py, nz)); SCHK(alloc(kccp, nx, ny, pz));
+  SCHK(alloc(tpcc, px, ny, nz)); SCHK(alloc(tcpc, nx, py, nz)); SCHK(alloc(tccp, nx, ny, pz));
+  SCHK(alloc(dface[0], px, ny, nz)); SCHK(alloc(dface[1], nx, py, nz)); SCHK(alloc(dface[2], nx, ny, pz));
+  SCHK(alloc_thermo_work<double>(w64));
+  if (mixed) SCHK(alloc_thermo_work<float>(w32));
+  // wall arrays (nx,4,nz): h, k, d of the walls (rows 3,4 repeat rows 1,2);
+  // T of the walls in rows 1,2 and zero in rows 3,4 (get_fbcy_iTh,
+  // src/bc_general.f90:784-805); the wall flux scratch
+  const long nf = (long)nx * 4 * nz;
+  std::vector<double> hb(nf);
+  auto wall = [&](double** q, int col, bool rows34) -> std::string {
+    for (long t = 0; t < nf; ++t) {
+      const long r = (t / nx) % 4;
+      hb[t] = (!rows34 && r >= 2) ? 0.0 : wall_prop[r % 2][col];
+    }
+    FCHK(hipMalloc(q, sizeof(double) * nf));
+    owned.push_back(*q);
+    FCHK(hipMemcpy(*q, hb.data(), sizeof(double) * nf, hipMemcpyHostToDevice));
+    return "";
+  };
+  SCHK(wall(&fby_h, 0, true));
+  SCHK(wall(&fby_k, 3, true));
+  SCHK(wall(&fby_d, 1, true));
+  SCHK(wall(&fby_t, 4, false));
+  FCHK(hipMalloc(&fby_q, sizeof(double) * nf));
+  owned.push_back(fby_q);
+  FCHK(hipMalloc(&props_flag, sizeof(int)));
+  owned.push_back(props_flag);
+  FCHK(hipMemset(props_flag, 0, sizeof(int)));
+  // initialise_thermal_properties: uniform state from t_init
+  hipStream_t s = H->stream;
+  const long n = rhoh.size();
+  hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(256), 0, s, dens.p, p0[1], n);
+  hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(256), 0, s, w64.visc.p, p0[2], n);
+  hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(256), 0, s, rhoh.p, p0[1] * p0[0], n);
+  hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(256), 0, s, hent.p, p0[0], n);
+  hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(256), 0, s, kcond.p, p0[3], n);
+  hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(256), 0, s, temp.p, p0[4], n);
+  thermo = true;
+  update_re(0);
+  SCHK(convert_qg(true));
+  FCHK(hipStreamSynchronize(s));
+  return "";
+}
+
+std::string Flow::refresh_props() {
+  // Update_thermal_properties, src/eq_energy.f90:14-47 (dDensm1 = dDens
+  // after the refresh, so Calculate_drhodt is zero and is not formed)
+  return props_refresh_dh(props, props_mode, rhoh.p, rhoh.size(), hent.p, dens.p, w64.visc.p, kcond.p, temp.p,
+                          props_flag, H->stream);
+}
+
+std::string Flow::thermo_set_rhoh(const double* r, int mem, long* clamped) {
+  if (!thermo) return "variable properties: not enabled";
+  hipMemcpyKind k = (mem == CHS_MEM_HOST) ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  FCHK(hipMemcpyAsync(rhoh.p, r, sizeof(double) * rhoh.size(), k, H->stream));
+  FCHK(hipMemsetAsync(props_flag, 0, sizeof(int), H->stream));
+  SCHK(refresh_props());
+  SCHK(convert_qg(true));
+  int nb = 0;
+  FCHK(hipMemcpyAsync(&nb, props_flag, sizeof(int), hipMemcpyDeviceToHost, H->stream));
+  FCHK(hipStreamSynchronize(H->stream));
+  *clamped = nb;
+  return "";
+}
+
+std::string Flow::thermo_get(double* out[9], int mem) {
+  if (!thermo) return "variable properties: not enabled";
+  const Field<double>* src[9] = {&rhoh, &hent, &temp, &kcond, &dens, &w64.visc, &w64.gx, &w64.gy, &w64.gz};
+  hipMemcpyKind k = (mem == CHS_MEM_HOST) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  for (int q = 0; q < 9; ++q)
+    if (out[q]) FCHK(hipMemcpyAsync(out[q], src[q]->p, sizeof(double) * src[q]->size(), k, H->stream));
+  FCHK(hipStreamSynchronize(H->stream));
+  return "";
+}
+
+std::string Flow::convert_qg(bool to_g) {
+  // convert_primary_conservative (src/para_conversion.f90:5-227): density at
+  // each velocity location by midp C2P with the property BCs and the wall
+  // density; stationary walls keep zero wall values of q and g
+  int ftp[3][2];
+  for (int d = 0; d < 3; ++d)
+    for (int s2 = 0; s2 < 2; ++s2) ftp[d][s2] = (ibc[d][VT][s2] == BC_NEUMANN) ? BC_DIRICHLET : ibc[d][VT][s2];
+  SCHK(op<double>(w64, 0, MIDP_C2P, ftp[0], dens, dface[0], nullptr));
+  SCHK(op<double>(w64, 1, MIDP_C2P, ftp[1], dens, dface[1], wall_y ? fby_d : nullptr));
+  SCHK(op<double>(w64, 2, MIDP_C2P, ftp[2], dens, dface[2], nullptr));
+  Field<double>* q[3] = {&qx, &qy, &qz};
+  Field<double>* g[3] = {&w64.gx, &w64.gy, &w64.gz};
+  Field<float>* q32[3] = {&w32.qx, &w32.qy, &w32.qz};
+  for (int c = 0; c < 3; ++c) {
+    const long n = q[c]->size();
+    if (to_g)
+      hipLaunchKernelGGL(k_q2g, grid1(n), dim3(256), 0, H->stream, g[c]->p, q[c]->p, dface[c].p, n);
+    else
+      hipLaunchKernelGGL(k_g2q, grid1(n), dim3(256), 0, H->stream, q[c]->p, g[c]->p, dface[c].p, n,
+                         mixed ? q32[c]->p : nullptr);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("convert_qg: ") + hipGetErrorString(e);
+  return "";
+}
+
+std::string Flow::energy_rhs(int isub) {
+  SCHK(energy_terms(isub));
+  const long n = rhoh.size();
+  hipLaunchKernelGGL(k_energy_combine, grid1(n), dim3(256), 0, H->stream, ex.p, ey.p, ez.p, erhs0.p, eres.p,
+                     (double*)nullptr, n, tg[isub], tz[isub], dt);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("energy_rhs: ") + hipGetErrorString(e);
+  return "";
+}
+
+std::string Flow::energy_terms(int isub) {
+  (void)isub;
+  // Compute_energy_rhs (src/eq_energy.f90:178-396): convection -d(g_i h_i),
+  // diffusion rPrRen d(k_i dT/dx_i); x on lane 0, y on lane 1, z on lane 2.
+  // The y flux takes its own wall rows as wall values (extract_dirichlet_
+  // fbcy) before the outer derivative: the reference guards that with
+  // is_fbcx_velo_required (:356), which is false on this channel and would
+  // pass MAXP wall values (wall rows ~1e19); the y-wall test is followed.
+  set_lanes();
+  int ftp[3][2], econv[3][2], edifu[3][2];
+  for (int d = 0; d < 3; ++d)
+    for (int s2 = 0; s2 < 2; ++s2) ftp[d][s2] = (ibc[d][VT][s2] == BC_NEUMANN) ? BC_DIRICHLET : ibc[d][VT][s2];
+  for (int d = 0; d < 3; ++d) {
+    prod_bc(ibc[d][d], ftp[d], econv[d]);
+    tau_bc(ftp[d], ibc[d][VT], edifu[d]);
+  }
+  const double m1 = -1.0, r = rprren;
+  auto o = [&](int lane, int dir, int opk, const int* bc, const Field<double>& A, Field<double>& out,
+               const double* fbc, double sA, const Field<double>* Cm, double sOut, int acc) {
+    return op<double>(w64, dir, opk, bc, A, out, fbc, sA, nullptr, 1.0, Cm, sOut, acc, nullptr, 0.0, nullptr, 0.0,
+                      lane);
+  };
+  SCHK(fork());
+  // x
+  SCHK(o(0, 0, MIDP_C2P, ftp[0], hent, hpcc, nullptr, 1.0, nullptr, 1.0, 0));
+  SCHK(o(0, 0, MIDP_C2P, ftp[0], kcond, kpcc, nullptr, 1.0, nullptr, 1.0, 0));
+  SCHK(o(0, 0, D1_P2C, econv[0], w64.gx, ex, nullptr, m1, &hpcc, 1.0, 0));
+  SCHK(o(0, 0, D1_C2P, ibc[0][VT], temp, tpcc, nullptr, 1.0, nullptr, 1.0, 0));
+  SCHK(o(0, 0, D1_P2C, edifu[0], tpcc, ex, nullptr, 1.0, &kpcc, r, 1));
+  // y
+  SCHK(o(1, 1, MIDP_C2P, ftp[1], hent, hcpc, fby_h, 1.0, nullptr, 1.0, 0));
+  SCHK(o(1, 1, MIDP_C2P, ftp[1], kcond, kcpc, fby_k, 1.0, nullptr, 1.0, 0));
+  SCHK(o(1, 1, D1_P2C, econv[1], w64.gy, ey, nullptr, m1, &hcpc, 1.0, 0));
+  SCHK(o(1, 1, D1_C2P, ibc[1][VT], temp, tcpc, fby_t, 1.0, nullptr, 1.0, 0));
+  hipLaunchKernelGGL(k_wall_rows4<double>, grid1((long)nx * nz), dim3(256), 0, lane_stream(1), fby_q, tcpc.p,
+                     (const double*)kcpc.p, nx, py, nz);
+  SCHK(o(1, 1, D1_P2C, edifu[1], tcpc, ey, fby_q, 1.0, &kcpc, r, 1));
+  // z
+  SCHK(o(2, 2, MIDP_C2P, ftp[2], hent, hccp, nullptr, 1.0, nullptr, 1.0, 0));
+  SCHK(o(2, 2, MIDP_C2P, ftp[2], kcond, kccp, nullptr, 1.0, nullptr, 1.0, 0));
+  SCHK(o(2, 2, D1_P2C, econv[2], w64.gz, ez, nullptr, m1, &hccp, 1.0, 0));
+  SCHK(o(2, 2, D1_C2P, ibc[2][VT], temp, tccp, nullptr, 1.0, nullptr, 1.0, 0));
+  SCHK(o(2, 2, D1_P2C, edifu[2], tccp, ez, nullptr, 1.0, &kccp, r, 1));
+  SCHK(join());
+  return "";
+}
+
+std::string Flow::energy_substep(int isub) {
+  // Solve_energy_eq, src/eq_energy.f90:398-458 (the isub 3 density history
+  // shift only feeds Calculate_drhodt, which is zero)
+  SCHK(energy_terms(isub));
+  const long n = rhoh.size();
+  hipLaunchKernelGGL(k_energy_combine, grid1(n), dim3(256), 0, H->stream, ex.p, ey.p, ez.p, erhs0.p,
+                     (double*)nullptr, rhoh.p, n, tg[isub], tz[isub], dt);
+  SCHK(refresh_props());
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("energy_substep: ") + hipGetErrorString(e);
   return "";
 }
 

@@ -308,6 +308,77 @@ int reduce_grads(chs_mlp* m, int B, int nblk, float* loss) {
 
 }  // namespace
 
+namespace {
+
+// Properties from rho*h with the surrogate (flow solver's refresh, mode
+// CHS_PROPS_MLP): clamp to [dhmin, dhmax] (counted), bisection on the
+// table's rho*h column and linear interpolation for the first h (as
+// ftp_refresh_thermal_properties_from_DH, src/input_thermo.f90:393-446),
+// then the network for the properties from h, h = rho*h / rho, and the
+// network again.  One thread per cell; parameters in LDS as in k_eval.
+__global__ void __launch_bounds__(256) k_mlp_from_dh(const uint16_t* __restrict__ p16, Norm nm,
+                                                     const double* __restrict__ tab, int nt, double dhmin,
+                                                     double dhmax, const double* __restrict__ rhoh, long N,
+                                                     double* __restrict__ oh, double* __restrict__ od,
+                                                     double* __restrict__ om, double* __restrict__ ok,
+                                                     double* __restrict__ ot, int* __restrict__ flag) {
+  __shared__ float sp[NP];
+  stage_params(p16, sp);
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double v = rhoh[c];
+  if (v < dhmin) {
+    v = dhmin;
+    atomicAdd(flag, 1);
+  } else if (v > dhmax) {
+    v = dhmax;
+    atomicAdd(flag, 1);
+  }
+  const double* crh = tab + (long)CHS_PC_RH * nt;
+  const double* ch = tab + (long)CHS_PC_H * nt;
+  int i1, i2;
+  bisect(crh, nt, v, &i1, &i2);
+  const double w1 = (crh[i2] - v) / (crh[i2] - crh[i1]), w2 = 1.0 - w1;
+  double h = w1 * ch[i1] + w2 * ch[i2];
+  float out[NO];
+  for (int pass = 0; pass < 2; ++pass) {
+    const float xv = (float)((h - nm.xc) / nm.xs);
+    float a1[H1];
+#pragma unroll
+    for (int i = 0; i < H1; ++i) a1[i] = tanhf(sp[OW1 + i] * xv + sp[OB1 + i]);
+#pragma unroll
+    for (int o = 0; o < NO; ++o) out[o] = sp[OB3 + o];
+    for (int j = 0; j < H2; ++j) {
+      float z = sp[OB2 + j];
+#pragma unroll
+      for (int i = 0; i < H1; ++i) z += sp[OW2 + j * H1 + i] * a1[i];
+      const float a2 = tanhf(z);
+#pragma unroll
+      for (int o = 0; o < NO; ++o) out[o] += sp[OW3 + o * H2 + j] * a2;
+    }
+    if (pass == 0) h = v / ((double)out[1] * nm.sd[1] + nm.mu[1]);
+  }
+  if (oh) oh[c] = h;
+  if (ot) ot[c] = (double)out[0] * nm.sd[0] + nm.mu[0];
+  if (od) od[c] = (double)out[1] * nm.sd[1] + nm.mu[1];
+  if (om) om[c] = (double)out[2] * nm.sd[2] + nm.mu[2];
+  if (ok) ok[c] = (double)out[3] * nm.sd[3] + nm.mu[3];
+}
+
+}  // namespace
+
+namespace chs {
+std::string mlp_refresh_dh(chs_mlp* m, const chs_props* p, const double* rhoh, long n, double* h, double* d,
+                           double* mu, double* k, double* t, int* flag, hipStream_t s) {
+  if (n <= 0) return "";
+  hipLaunchKernelGGL(k_mlp_from_dh, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->p16, m->nm, p->dev, p->n,
+                     p->dhmin, p->dhmax, rhoh, n, h, d, mu, k, t, flag);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("surrogate kernel: ") + hipGetErrorString(e);
+  return "";
+}
+}  // namespace chs
+
 extern "C" {
 
 int chs_mlp_create(chs_props_t* table, int nsamples, unsigned seed, chs_mlp_t** out) {

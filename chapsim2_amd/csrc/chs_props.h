@@ -3,12 +3,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <vector>
 
 #include "../../include/chapsim_hip.h"
 
 // column order of chs_props::col and of the device copy
 enum { CHS_PC_T = 0, CHS_PC_D, CHS_PC_M, CHS_PC_K, CHS_PC_B, CHS_PC_CP, CHS_PC_H, CHS_PC_RH };
+
+struct chs_mlp;
 
 struct chs_props {
   int n = 0;
@@ -20,6 +23,7 @@ struct chs_props {
   void* io = nullptr;          // host-memory staging
   size_t io_bytes = 0;
   hipStream_t stream = nullptr;
+  chs_mlp* surrogate = nullptr;  // used by CHS_PROPS_MLP (not owned)
   ~chs_props() {
     if (dev) (void)hipFree(dev);
     if (flag) (void)hipFree(flag);
@@ -27,3 +31,15 @@ struct chs_props {
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
+
+namespace chs {
+// Per-cell properties from rho*h for device arrays on stream s (no host
+// synchronisation): the table path of chs_props_from_rhoh, or with the
+// surrogate the same clamp and bisection on rho*h for the first h and the
+// MLP for the properties from h (twice, as the table path).  Outputs may be
+// null; the out-of-range count is added to *flag (device int).
+std::string props_refresh_dh(const chs_props* p, int mode, const double* rhoh, long n, double* h, double* d,
+                             double* m, double* k, double* t, int* flag, hipStream_t s);
+std::string mlp_refresh_dh(chs_mlp* m, const chs_props* p, const double* rhoh, long n, double* h, double* d,
+                           double* mu, double* k, double* t, int* flag, hipStream_t s);
+}  // namespace chs

@@ -150,7 +150,35 @@ __global__ void __launch_bounds__(256) k_props_from_h(const double* __restrict__
 
 }  // namespace
 
+namespace chs {
+std::string props_refresh_dh(const chs_props* p, int mode, const double* rhoh, long n, double* h, double* d,
+                             double* m, double* k, double* t, int* flag, hipStream_t s) {
+  if (n <= 0) return "";
+  if (mode == CHS_PROPS_MLP) {
+    if (!p->surrogate) return "properties: no surrogate attached to the table";
+    return mlp_refresh_dh(p->surrogate, p, rhoh, n, h, d, m, k, t, flag, s);
+  }
+  if (mode != CHS_PROPS_TABLE) return "properties: unknown mode";
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  if (p->n <= kPropsLds)
+    hipLaunchKernelGGL(k_props_from_dh<true>, dim3(grid), dim3(256), sizeof(double) * 3 * (size_t)p->n, s, p->dev,
+                       p->n, p->dhmin, p->dhmax, rhoh, n, h, d, m, k, t, (double*)nullptr, (double*)nullptr, flag);
+  else
+    hipLaunchKernelGGL(k_props_from_dh<false>, dim3(grid), dim3(256), 0, s, p->dev, p->n, p->dhmin, p->dhmax, rhoh,
+                       n, h, d, m, k, t, (double*)nullptr, (double*)nullptr, flag);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("property kernel: ") + hipGetErrorString(e);
+  return "";
+}
+}  // namespace chs
+
 extern "C" {
+
+int chs_props_set_surrogate(chs_props_t* p, chs_mlp_t* m) {
+  if (!p) return fail(CHS_EINVAL, "null handle");
+  p->surrogate = m;
+  return CHS_OK;
+}
 
 int chs_props_create(const double* table, int nrows, double ref_t0, chs_props_t** out) {
   if (!table || !out || nrows < 2) return fail(CHS_EINVAL, "invalid argument");
@@ -248,7 +276,9 @@ int chs_props_info(const chs_props_t* p, double* dhmin, double* dhmax, double re
 int chs_props_from_rhoh(chs_props_t* p, const double* rhoh, long n, double* h, double* d, double* m, double* k,
                         double* t, double* b, double* cp, int mode, int mem) {
   if (!p || !rhoh || n < 0) return fail(CHS_EINVAL, "invalid argument");
-  if (mode != CHS_PROPS_TABLE) return fail(CHS_EUNSUP, "unknown property mode");
+  if (mode != CHS_PROPS_TABLE && mode != CHS_PROPS_MLP) return fail(CHS_EUNSUP, "unknown property mode");
+  if (mode == CHS_PROPS_MLP && !p->surrogate) return fail(CHS_EINVAL, "no surrogate attached to the table");
+  if (mode == CHS_PROPS_MLP && (b || cp)) return fail(CHS_EUNSUP, "the surrogate gives h, d, m, k and t only");
   if (mem != CHS_MEM_HOST && mem != CHS_MEM_DEVICE) return fail(CHS_EINVAL, "invalid mem kind");
   if (n == 0) return CHS_OK;
   double* outs[7] = {h, d, m, k, t, b, cp};
@@ -274,7 +304,11 @@ int chs_props_from_rhoh(chs_props_t* p, const double* rhoh, long n, double* h, d
   }
   if (hipMemsetAsync(p->flag, 0, sizeof(int), p->stream) != hipSuccess) return fail(CHS_EHIP, "flag reset");
   const unsigned grid = (unsigned)((n + 255) / 256);
-  if (p->n <= kPropsLds) {
+  if (mode == CHS_PROPS_MLP) {
+    const std::string e = chs::mlp_refresh_dh(p->surrogate, p, din, n, dout[0], dout[1], dout[2], dout[3], dout[4],
+                                              p->flag, p->stream);
+    if (!e.empty()) return fail(CHS_EHIP, e);
+  } else if (p->n <= kPropsLds) {
     const size_t shm = sizeof(double) * 3 * (size_t)p->n;
     hipLaunchKernelGGL(k_props_from_dh<true>, dim3(grid), dim3(256), shm, p->stream, p->dev, p->n, p->dhmin,
                        p->dhmax, din, n, dout[0], dout[1], dout[2], dout[3], dout[4], dout[5], dout[6], p->flag);

@@ -144,6 +144,43 @@ class Domain:
             return out
         return out, rc
 
+    # ---- variable properties (heated channel) ------------------------------
+    def thermo_enable(self, table, t_wall, prandtl, t_init, mode=0):
+        """Isothermal y walls at t_wall (bottom, top; non-dimensional), the
+        reference Prandtl number, a uniform initial state at t_init, the
+        property table (chapsim2_amd.props.PropertyTable, kept alive here)
+        and its mode (0 table, 1 surrogate attached to the table)."""
+        d = _lib.ChsThermoDesc()
+        d.props = table.h.value if hasattr(table.h, "value") else table.h
+        d.mode = int(mode)
+        d.t_wall[0], d.t_wall[1] = float(t_wall[0]), float(t_wall[1])
+        d.prandtl, d.t_init = float(prandtl), float(t_init)
+        self._table = table
+        _lib.check(self.lib.chs_thermo_enable(self.h, C.byref(d)))
+
+    def thermo_set_mode(self, mode):
+        _lib.check(self.lib.chs_thermo_set_mode(self.h, int(mode)))
+
+    def thermo_set_rhoh(self, rhoh):
+        r = np.asfortranarray(rhoh, dtype=np.float64)
+        assert r.shape == self.shapes["ccc"], r.shape
+        _lib.check(self.lib.chs_thermo_set_rhoh(self.h, r.ctypes.data, _lib.MEM_HOST))
+
+    def thermo_get(self):
+        """dict of rhoh, h, t, k, d, m (cell centres) and gx, gy, gz"""
+        names = ("rhoh", "h", "t", "k", "d", "m")
+        out = {k: np.zeros(self.shapes["ccc"], order="F") for k in names}
+        for k, sh in (("gx", "qx"), ("gy", "qy"), ("gz", "qz")):
+            out[k] = np.zeros(self.shapes[sh], order="F")
+        _lib.check(self.lib.chs_thermo_get(self.h, *[out[k].ctypes.data for k in names + ("gx", "gy", "gz")],
+                                           _lib.MEM_HOST))
+        return out
+
+    def energy_rhs(self, isub):
+        out = np.zeros(self.shapes["ccc"], order="F")
+        _lib.check(self.lib.chs_energy_rhs(self.h, int(isub), out.ctypes.data, _lib.MEM_HOST))
+        return out
+
     def divergence(self):
         out = np.zeros(self.shapes["ccc"], order="F")
         _lib.check(self.lib.chs_divergence(self.h, out.ctypes.data, _lib.MEM_HOST))

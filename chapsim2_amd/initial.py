@@ -89,16 +89,29 @@ def tgv_fields(nc, lxx, lyb, lyt, lzz):
     return [np.asfortranarray(a) for a in (qx, qy, qz, pr)]
 
 
-def heated_channel_table(n=4096):
+# enthalpy range of the heated channel's property table: rho*h is monotone
+# (decreasing) on it and it holds the bulk (0.8) and wall (1.2) enthalpies
+HEATED_H_RANGE = (0.65, 1.225)
+
+
+def heated_channel_table(n=4096, h0=0.0, h1=2.0):
     """Non-dimensional property table of the variable-property heated channel
-    case: n rows over h in [0, 2] with rho = 1/(1 + 0.8 tanh(5(h-1))),
+    case: n rows over h in [h0, h1] with rho = 1/(1 + 0.8 tanh(5(h-1))),
     cp = 1 + 7 exp(-((h-1)/0.05)^2) (a peak of height 8 at h = 1),
     T = 1 + int_0^h dh'/cp (so that dh = cp dT), mu = exp(-0.6 (h-1)) and
     kappa = 1 + 0.25 tanh(3(h-1)) (smooth monotone fits), beta = -rho'(h) cp / rho.
     Rows are (P, H, T, D, M, K, CP, B), the column order of the reference's
-    property tables; use it with ref_t0 <= 0 (already non-dimensional)."""
-    h = np.linspace(0.0, 2.0, n)
-    fine = np.linspace(0.0, 2.0, 64 * (n - 1) + 1)
+    property tables; use it with ref_t0 <= 0 (already non-dimensional).
+
+    rho*h = h rho(h) is not monotonic over [0, 2] (a maximum of 2.640 at
+    h = 0.6306, a minimum of 0.7436 at h = 1.2282), so the bisection on rho*h
+    of ftp_refresh_thermal_properties_from_DH (src/input_thermo.f90:393-446)
+    can land on the wrong branch there (rho*h = 1 gives h = 0.2 instead of
+    1.0).  A solver run uses HEATED_H_RANGE, the monotone part that holds
+    the heated channel's enthalpies (bulk 0.8, walls 1.2); T is integrated
+    from h0."""
+    h = np.linspace(h0, h1, n)
+    fine = np.linspace(h0, h1, 64 * (n - 1) + 1)
     cpf = 1.0 + 7.0 * np.exp(-((fine - 1.0) / 0.05) ** 2)
     integ = np.concatenate([[0.0], np.cumsum(0.5 * (1.0 / cpf[1:] + 1.0 / cpf[:-1]) * np.diff(fine))])
     T = 1.0 + integ[::64]

@@ -182,7 +182,10 @@ int chs_batch_run(chs_batch_t *b, int iter0, int nsteps, int use_graph);
 /* ---- thermal properties from a tabulated equation of state ------------- */
 
 typedef struct chs_props chs_props_t;
-enum { CHS_PROPS_TABLE = 0 };
+typedef struct chs_mlp chs_mlp_t;
+/* property evaluation: the table's interpolation, or the MLP surrogate
+ * attached to the table (chs_props_set_surrogate) for the properties from h */
+enum { CHS_PROPS_TABLE = 0, CHS_PROPS_MLP = 1 };
 
 /* Build the non-dimensional property table from nrows dimensional rows of
  * (P, H, T, D, M, K, CP, B) (the columns of the reference's NIST tables):
@@ -206,6 +209,13 @@ int chs_props_info(const chs_props_t *p, double *dhmin, double *dhmax, double re
  * the reference stops with an error. */
 int chs_props_from_rhoh(chs_props_t *p, const double *rhoh, long n, double *h, double *d, double *m, double *k,
                         double *t, double *b, double *cp, int mode, int mem);
+/* Attach (or with NULL detach) the MLP surrogate that mode CHS_PROPS_MLP of
+ * chs_props_from_rhoh and of the flow solver's property refresh use: the
+ * first h still comes from the table's rho*h column, the properties from h
+ * (both evaluations of ftp_refresh_thermal_properties_from_H) from the
+ * network; b and cp are not given in that mode.  No counterpart in the
+ * reference. */
+int chs_props_set_surrogate(chs_props_t *p, chs_mlp_t *m);
 
 /* Per-cell properties from h: bisection on h and linear interpolation, with
  * no clamping (the end segments extrapolate).  Replaces
@@ -221,7 +231,6 @@ int chs_props_from_h(chs_props_t *p, const double *h, long n, double *d, double 
  * (4613, FP32 master copy): W1[64], b1[64], W2[64][64] (row = output unit),
  * b2[64], W3[5][64], b3[5]; inputs and outputs are normalised with
  * norm = (x centre, x scale, 5 output means, 5 output scales). */
-typedef struct chs_mlp chs_mlp_t;
 int chs_mlp_create(chs_props_t *table, int nsamples, unsigned seed, chs_mlp_t **out);
 int chs_mlp_destroy(chs_mlp_t *m);
 int chs_mlp_info(const chs_mlp_t *m, int *nparams, float norm[12]);
@@ -237,6 +246,44 @@ int chs_mlp_train(chs_mlp_t *m, int steps, int batch, float lr, float *loss);
 /* Surrogate evaluation for n enthalpies; any output may be NULL. */
 int chs_mlp_eval(chs_mlp_t *m, const double *h, long n, double *T, double *rho, double *mu, double *kappa,
                  double *cp, int mem);
+
+/* ---- variable-property flow (heated channel) --------------------------
+ * Isothermal y walls (Dirichlet T, stationary), x and z periodic: the
+ * energy equation and the variable-density / variable-viscosity momentum
+ * terms of the reference's thermo path.  Each sub-step of chs_substep /
+ * chs_run then follows src/chapsim.f90:300-304: Solve_energy_eq
+ * (src/eq_energy.f90:398-458: Compute_energy_rhs :178-396, rho*h += rhs,
+ * Update_thermal_properties :14-47) and Solve_momentum_eq with g = rho u
+ * (src/eq_momentum2.f90:576-703 and the thermo branches of :798-1507,
+ * 1799-1932; convert_primary_conservative, src/para_conversion.f90:5-227).
+ * The initial sub-step (isub 0) is momentum only, as at src/chapsim.f90:135.
+ * FP64 and mixed precision (CHS_PREC_MIXED: the momentum right-hand side in
+ * FP32; energy, properties and projection FP64). */
+typedef struct chs_thermo_desc {
+  chs_props_t *props;  /* property table (kept by the caller while the handle lives) */
+  int mode;            /* CHS_PROPS_TABLE or CHS_PROPS_MLP */
+  double t_wall[2];    /* non-dimensional wall temperatures, y bottom / top */
+  double prandtl;      /* reference Prandtl number mu0 cp0 / k0: 1/(Re Pr) = rPrRen (src/tools_solver.f90:63) */
+  double t_init;       /* uniform initial temperature (initialise_thermal_properties,
+                          src/input_thermo.f90:1070-1103) */
+} chs_thermo_desc_t;
+/* Enable the path on a flow handle (walls and wall properties from T by the
+ * table, src/bc_dirichlet.f90:433-437; uniform initial state; g from the
+ * current velocity).  CHS_EUNSUP for moving walls or no y walls. */
+int chs_thermo_enable(chs_handle_t *h, const chs_thermo_desc_t *d);
+/* rho*h per cell (nc1,nc2,nc3): properties refreshed from it, g = rho u of
+ * the current velocity.  CHS_ERANGE when cells were clamped. */
+int chs_thermo_set_rhoh(chs_handle_t *h, const double *rhoh, int mem);
+/* switch the property refresh between table and surrogate */
+int chs_thermo_set_mode(chs_handle_t *h, int mode);
+/* rho*h, h, T, k, rho, mu (cell centres) and gx, gy, gz (the velocity
+ * locations); any pointer may be NULL */
+int chs_thermo_get(chs_handle_t *h, double *rhoh, double *hh, double *t, double *k, double *d, double *m,
+                   double *gx, double *gy, double *gz, int mem);
+/* dt-scaled energy right-hand side of sub-step isub (Compute_energy_rhs with
+ * Calculate_energy_fractional_step, which also stores it as the next
+ * sub-step's rhs0), cell centres */
+int chs_energy_rhs(chs_handle_t *h, int isub, double *rhs, int mem);
 
 #ifdef __cplusplus
 }

@@ -126,11 +126,22 @@ typedef struct orc_props {
   double b[ORC_PROPS_MAX], cp[ORC_PROPS_MAX], h[ORC_PROPS_MAX], rhoh[ORC_PROPS_MAX];
   double ref[7]; /* T0, d, m, k, h, b, cp of the reference state (dimensional) */
   double dhmin, dhmax;
+  int dimensional; /* 0: rows were given non-dimensional (ref_t0 <= 0) */
 } orc_props;
 /* raw: nlist rows of (P, H, T, D, M, K, CP, B), dimensional */
 int orc_props_build(const double *raw, int nlist, double ref_t0, orc_props *p);
 void orc_props_from_h(const orc_props *p, double h, double out[6]);
 int orc_props_from_dh(const orc_props *p, double rhoh, double out[7]);
+void orc_props_from_t(const orc_props *p, double t, double out[7]);
+
+/* variable-property path (heated channel with isothermal y walls) */
+int orc_thermo_enable(orc_flow *f, const orc_props *p, const double t_wall[2], double prandtl,
+                      double t_init);
+long orc_thermo_set_rhoh(orc_flow *f, const double *rhoh);
+void orc_thermo_get(const orc_flow *f, double *rhoh, double *h, double *t, double *k, double *d, double *m,
+                    double *gx, double *gy, double *gz);
+void orc_energy_rhs(orc_flow *f, int isub, double *rhs);
+long orc_solve_energy_eq(orc_flow *f, int isub);
 
 #ifdef __cplusplus
 }

@@ -252,6 +252,14 @@ struct orc_flow {
   /* exact projection: wall-normal block D_y G_y (row-major), eigenvalues of
    * the periodic blocks per FFTPACK coefficient, left null vector of ly */
   double *ly, *lxe, *lze, *lnull;
+  /* variable-property (thermo) state: src/eq_energy.f90, src/para_conversion.f90 */
+  int thermo;
+  const orc_props *props;
+  double wall[2][7];   /* h, d, m, k, t, b, cp of the isothermal y walls */
+  double prandtl, rprren;
+  double *gx, *gy, *gz;                       /* conservative variables rho u */
+  double *dens, *visc, *rhoh, *hent, *temp, *kcond, *erhs0, *densm1, *densm2;
+  double *fby_ftp[4];  /* (nx,4,nz) wall arrays of h, k, m, d (rows 3,4 = rows 1,2) */
 };
 
 /* build_bc_symm_operation, src/bc_general.f90:455-510; returns JBC_PROD */
@@ -552,6 +560,9 @@ void orc_flow_destroy(orc_flow *f) {
   free(f->xrt); free(f->zrt); free(f->pa); free(f->pb); free(f->pc);
   free(f->kx2); free(f->ky2); free(f->kz2);
   free(f->ly); free(f->lxe); free(f->lze); free(f->lnull);
+  free(f->gx); free(f->gy); free(f->gz); free(f->dens); free(f->visc); free(f->rhoh);
+  free(f->hent); free(f->temp); free(f->kcond); free(f->erhs0); free(f->densm1); free(f->densm2);
+  for (int v = 0; v < 4; ++v) free(f->fby_ftp[v]);
   orc_dom_free(&f->dm);
   free(f);
 }
@@ -587,6 +598,8 @@ void orc_flow_update_re(orc_flow *f, int iter) {
     f->rre = 1.0 / f->cfg.reninit;
   else
     f->rre = 1.0 / f->cfg.ren;
+  /* Update_PrGr, src/tools_solver.f90:63: rPrRen = rre k0 / (m0 cp0) */
+  if (f->thermo) f->rprren = f->rre / f->prandtl;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -617,30 +630,37 @@ double orc_vol_average_x(orc_flow *f, const double *var, int n1) {
   return fo / vol;
 }
 
-/* enforce_velo_from_fbc, src/bc_dirichlet.f90:461-515 (y walls) */
+/* enforce_velo_from_fbc, src/bc_dirichlet.f90:461-515 (y walls), on qy or,
+ * with variable properties, on gy (the wall values of g are the wall
+ * velocities times the wall density: zero on the stationary walls the
+ * thermo path supports) */
 static void enforce_velo(orc_flow *f) {
+  double *vy = f->thermo ? f->gy : f->qy;
   if (f->ibc[1][VV][0] == BDIR || f->ibc[1][VV][1] == BDIR) {
     for (int k = 0; k < f->nz; ++k)
       for (int i = 0; i < f->nx; ++i) {
         if (f->ibc[1][VV][0] == BDIR)
-          f->qy[i + (long)f->nx * (0 + (long)f->py * k)] = f->fby[VV][i + (long)f->nx * (0 + 4L * k)];
+          vy[i + (long)f->nx * (0 + (long)f->py * k)] = f->fby[VV][i + (long)f->nx * (0 + 4L * k)];
         if (f->ibc[1][VV][1] == BDIR)
-          f->qy[i + (long)f->nx * ((f->py - 1) + (long)f->py * k)] = f->fby[VV][i + (long)f->nx * (1 + 4L * k)];
+          vy[i + (long)f->nx * ((f->py - 1) + (long)f->py * k)] = f->fby[VV][i + (long)f->nx * (1 + 4L * k)];
       }
   }
 }
 
-/* Get_divergence_flow, src/eq_continuity.f90:91-161 */
+/* Get_divergence_flow, src/eq_continuity.f90:91-161: of q, or of g = rho u
+ * with variable properties (:114-117) */
 void orc_divergence(orc_flow *f, double *div) {
   int nx = f->nx, ny = f->ny, nz = f->nz;
   long n = (long)nx * ny * nz;
+  const double *ux = f->thermo ? f->gx : f->qx, *uy = f->thermo ? f->gy : f->qy;
+  const double *uz = f->thermo ? f->gz : f->qz;
   double *d0 = arr(n);
   for (long t = 0; t < n; ++t) div[t] = 0.0;
-  op(f, 0, ORC_1DER_P2C, f->ibc[0][VU], f->qx, f->px, ny, nz, d0, nx, ny, nz, f->fbx[VU]);
+  op(f, 0, ORC_1DER_P2C, f->ibc[0][VU], ux, f->px, ny, nz, d0, nx, ny, nz, f->fbx[VU]);
   for (long t = 0; t < n; ++t) div[t] = div[t] + d0[t];
-  op(f, 1, ORC_1DER_P2C, f->ibc[1][VV], f->qy, nx, f->py, nz, d0, nx, ny, nz, NULL);
+  op(f, 1, ORC_1DER_P2C, f->ibc[1][VV], uy, nx, f->py, nz, d0, nx, ny, nz, NULL);
   for (long t = 0; t < n; ++t) div[t] = div[t] + d0[t];
-  op(f, 2, ORC_1DER_P2C, f->ibc[2][VW], f->qz, nx, ny, f->pz, d0, nx, ny, nz, NULL);
+  op(f, 2, ORC_1DER_P2C, f->ibc[2][VW], uz, nx, ny, f->pz, d0, nx, ny, nz, NULL);
   for (long t = 0; t < n; ++t) div[t] = div[t] + d0[t];
   free(d0);
 }
@@ -994,6 +1014,48 @@ void orc_momentum_rhs(orc_flow *f, int isub, double *mx, double *my,
           fbdiv_y[i + (long)nx * (s + 2 + 4L * k)] = v;
         }
   }
+  /* variable properties (:576-703): interpolations of g = rho u, the
+   * advecting factor of each convective product, and of the viscosity;
+   * with constant properties g is q and mu is 1 */
+  const int th = f->thermo;
+  double *gxix = qxix, *gxiy = qxiy, *gxiz = qxiz, *gyix = qyix, *gyiy = qyiy, *gyiz = qyiz;
+  double *gzix = qzix, *gziy = qziy, *gziz = qziz;
+  double *mu_c = NULL, *muixy = NULL, *muixz = NULL, *muiyz = NULL, *fbmu_y = NULL;
+  if (th) {
+    gxix = arr(Nccc); gxiy = arr(Nppc); gxiz = arr(Npcp);
+    gyix = arr(Nppc); gyiy = arr(Nccc); gyiz = arr(Ncpp);
+    gzix = arr(Npcp); gziy = arr(Ncpp); gziz = arr(Nccc);
+    op(f, 0, ORC_MIDP_P2C, B[0][VU], f->gx, px, ny, nz, gxix, nx, ny, nz, f->fbx[VU]);
+    op(f, 1, ORC_MIDP_C2P, B[1][VU], f->gx, px, ny, nz, gxiy, px, py, nz, f->fby[VU]);
+    op(f, 2, ORC_MIDP_C2P, B[2][VU], f->gx, px, ny, nz, gxiz, px, ny, pz, f->fbz[VU]);
+    op(f, 0, ORC_MIDP_C2P, B[0][VV], f->gy, nx, py, nz, gyix, px, py, nz, f->fbx[VV]);
+    op(f, 1, ORC_MIDP_P2C, B[1][VV], f->gy, nx, py, nz, gyiy, nx, ny, nz, f->fby[VV]);
+    op(f, 2, ORC_MIDP_C2P, B[2][VV], f->gy, nx, py, nz, gyiz, nx, py, pz, f->fbz[VV]);
+    op(f, 0, ORC_MIDP_C2P, B[0][VW], f->gz, nx, ny, pz, gzix, px, ny, pz, f->fbx[VW]);
+    op(f, 1, ORC_MIDP_C2P, B[1][VW], f->gz, nx, ny, pz, gziy, nx, py, pz, f->fby[VW]);
+    op(f, 2, ORC_MIDP_P2C, B[2][VW], f->gz, nx, ny, pz, gziz, nx, ny, nz, f->fbz[VW]);
+    /* mu_ccc; muiy_cpc (y, wall viscosity); muixy_ppc = x of muiy_cpc;
+     * muixz_pcp = z of x; muiyz_cpp = z of muiy_cpc; wall rows of muiy_cpc
+     * as fbcy_mu_c4c (:648-698) */
+    mu_c = f->visc;
+    double *muiy = arr(Ncpc), *mupcc = arr(Npcc);
+    muixy = arr(Nppc); muixz = arr(Npcp); muiyz = arr(Ncpp);
+    op(f, 1, ORC_MIDP_C2P, f->ftp[1], f->visc, nx, ny, nz, muiy, nx, py, nz, f->fby_ftp[2]);
+    op(f, 0, ORC_MIDP_C2P, f->ftp[0], muiy, nx, py, nz, muixy, px, py, nz, NULL);
+    op(f, 0, ORC_MIDP_C2P, f->ftp[0], f->visc, nx, ny, nz, mupcc, px, ny, nz, NULL);
+    op(f, 2, ORC_MIDP_C2P, f->ftp[2], mupcc, px, ny, nz, muixz, px, ny, pz, NULL);
+    op(f, 2, ORC_MIDP_C2P, f->ftp[2], muiy, nx, py, nz, muiyz, nx, py, pz, NULL);
+    fbmu_y = arr((long)nx * 4 * nz);
+    for (int k = 0; k < nz; ++k)
+      for (int i = 0; i < nx; ++i)
+        for (int s = 0; s < 2; ++s) {
+          double v = muiy[i + (long)nx * ((s ? py - 1 : 0) + (long)py * k)];
+          fbmu_y[i + (long)nx * (s + 4L * k)] = v;
+          fbmu_y[i + (long)nx * (s + 2 + 4L * k)] = v;
+        }
+    free(muiy); free(mupcc);
+  }
+#define MU(a, t) ((a) ? (a)[t] : 1.0)
   double *maxp_x = fill_maxp(4L * (ny > py ? ny : py) * (nz > pz ? nz : pz));
   double *maxp_y = fill_maxp((long)(nx > px ? nx : px) * 4 * (nz > pz ? nz : pz));
   double *maxp_z = fill_maxp((long)(nx > px ? nx : px) * (ny > py ? ny : py) * 4);
@@ -1010,23 +1072,23 @@ void orc_momentum_rhs(orc_flow *f, int isub, double *mx, double *my,
   for (long t = 0; t < (long)nx * ny * 4; ++t) nfbz[t] = -f->fbz[VP][t];
 
   /* ---------------- x-momentum ---------------- */
-  for (long t = 0; t < Nccc; ++t) tc[t] = -qxix[t] * qxix[t];
+  for (long t = 0; t < Nccc; ++t) tc[t] = -gxix[t] * qxix[t];
   op(f, 0, ORC_1DER_C2P, f->mcov[0][0], tc, nx, ny, nz, r1, px, ny, nz, maxp_x);
   for (long t = 0; t < Npcc; ++t) mx[t] = 0.0 + r1[t];
-  for (long t = 0; t < Nppc; ++t) tppc[t] = -qyix[t] * qxiy[t];
+  for (long t = 0; t < Nppc; ++t) tppc[t] = -gyix[t] * qxiy[t];
   op(f, 1, ORC_1DER_P2C, f->mcov[0][1], tppc, px, py, nz, r1, px, ny, nz, NULL);
   for (long t = 0; t < Npcc; ++t) acc_y[t] = 0.0 + r1[t];
-  for (long t = 0; t < Npcp; ++t) tpcp[t] = -qzix[t] * qxiz[t];
+  for (long t = 0; t < Npcp; ++t) tpcp[t] = -gzix[t] * qxiz[t];
   op(f, 2, ORC_1DER_P2C, f->mcov[0][2], tpcp, px, ny, pz, r1, px, ny, nz, NULL);
   for (long t = 0; t < Npcc; ++t) acc_z[t] = 0.0 + r1[t];
   op(f, 0, ORC_1DER_C2P, B[0][VP], mpres, nx, ny, nz, pfc, px, ny, nz, nfbx);
-  for (long t = 0; t < Nccc; ++t) tc[t] = (2.0 * 1.0) * (qxdx[t] - ONE_THIRD * div[t]);
+  for (long t = 0; t < Nccc; ++t) tc[t] = (2.0 * MU(mu_c, t)) * (qxdx[t] - ONE_THIRD * div[t]);
   op(f, 0, ORC_1DER_C2P, f->mtau[0][0], tc, nx, ny, nz, r1, px, ny, nz, maxp_x);
   for (long t = 0; t < Npcc; ++t) mx[t] = mx[t] + r1[t] * rre;
-  for (long t = 0; t < Nppc; ++t) tppc[t] = (qxdy[t] + qydx[t]) * 1.0;
+  for (long t = 0; t < Nppc; ++t) tppc[t] = (qxdy[t] + qydx[t]) * MU(muixy, t);
   op(f, 1, ORC_1DER_P2C, f->mtau[0][1], tppc, px, py, nz, r1, px, ny, nz, NULL);
   for (long t = 0; t < Npcc; ++t) acc_y[t] = acc_y[t] + r1[t] * rre;
-  for (long t = 0; t < Npcp; ++t) tpcp[t] = (qxdz[t] + qzdx[t]) * 1.0;
+  for (long t = 0; t < Npcp; ++t) tpcp[t] = (qxdz[t] + qzdx[t]) * MU(muixz, t);
   op(f, 2, ORC_1DER_P2C, f->mtau[0][2], tpcp, px, ny, pz, r1, px, ny, nz, NULL);
   for (long t = 0; t < Npcc; ++t) acc_z[t] = acc_z[t] + r1[t] * rre;
   for (long t = 0; t < Npcc; ++t) mx[t] = mx[t] + acc_y[t];
@@ -1041,7 +1103,7 @@ void orc_momentum_rhs(orc_flow *f, int isub, double *mx, double *my,
   }
 
   /* ---------------- y-momentum ---------------- */
-  for (long t = 0; t < Nppc; ++t) tppc[t] = -qxiy[t] * qyix[t];
+  for (long t = 0; t < Nppc; ++t) tppc[t] = -gxiy[t] * qyix[t];
   op(f, 0, ORC_1DER_P2C, f->mcov[1][0], tppc, px, py, nz, r1, nx, py, nz, NULL);
   for (long t = 0; t < Ncpc; ++t) my[t] = 0.0 + r1[t];
   double *fbc_y = maxp_y, *fbc_own = NULL;
@@ -1050,10 +1112,10 @@ void orc_momentum_rhs(orc_flow *f, int isub, double *mx, double *my,
     for (long t = 0; t < nfy; ++t) fbc_own[t] = -f->fby[VV][t] * f->fby[VV][t];
     fbc_y = fbc_own;
   }
-  for (long t = 0; t < Nccc; ++t) tc[t] = -qyiy[t] * qyiy[t];
+  for (long t = 0; t < Nccc; ++t) tc[t] = -qyiy[t] * gyiy[t];
   op(f, 1, ORC_1DER_C2P, f->mcov[1][1], tc, nx, ny, nz, r1, nx, py, nz, fbc_y);
   for (long t = 0; t < Ncpc; ++t) acc_y[t] = 0.0 + r1[t];
-  for (long t = 0; t < Ncpp; ++t) tcpp[t] = -qyiz[t] * qziy[t];
+  for (long t = 0; t < Ncpp; ++t) tcpp[t] = -qyiz[t] * gziy[t];
   op(f, 2, ORC_1DER_P2C, f->mcov[1][2], tcpp, nx, py, pz, r1, nx, py, nz, NULL);
   for (long t = 0; t < Ncpc; ++t) acc_z[t] = 0.0 + r1[t];
   {
@@ -1062,7 +1124,7 @@ void orc_momentum_rhs(orc_flow *f, int isub, double *mx, double *my,
     for (long t = 0; t < Ncpc; ++t) pfc[t] = 0.0 + pf[t];
     free(pf);
   }
-  for (long t = 0; t < Nppc; ++t) tppc[t] = (qxdy[t] + qydx[t]) * 1.0;
+  for (long t = 0; t < Nppc; ++t) tppc[t] = (qxdy[t] + qydx[t]) * MU(muixy, t);
   op(f, 0, ORC_1DER_P2C, f->mtau[1][0], tppc, px, py, nz, r1, nx, py, nz, NULL);
   for (long t = 0; t < Ncpc; ++t) my[t] = my[t] + r1[t] * rre;
   fbc_y = maxp_y;
@@ -1076,13 +1138,13 @@ void orc_momentum_rhs(orc_flow *f, int isub, double *mx, double *my,
           fbc_own[i + (long)nx * (s + 2 + 4L * k)] = v;
         }
     for (long t = 0; t < nfy; ++t)
-      fbc_own[t] = (fbc_own[t] - ONE_THIRD * fbdiv_y[t]) * 2.0 * 1.0;
+      fbc_own[t] = (fbc_own[t] - ONE_THIRD * fbdiv_y[t]) * 2.0 * MU(fbmu_y, t);
     fbc_y = fbc_own;
   }
-  for (long t = 0; t < Nccc; ++t) tc[t] = (qydy[t] - ONE_THIRD * div[t]) * 2.0 * 1.0;
+  for (long t = 0; t < Nccc; ++t) tc[t] = (qydy[t] - ONE_THIRD * div[t]) * 2.0 * MU(mu_c, t);
   op(f, 1, ORC_1DER_C2P, f->mtau[1][1], tc, nx, ny, nz, r1, nx, py, nz, fbc_y);
   for (long t = 0; t < Ncpc; ++t) acc_y[t] = acc_y[t] + r1[t] * rre;
-  for (long t = 0; t < Ncpp; ++t) tcpp[t] = (qzdy[t] + qydz[t] - 0.0) * 1.0;
+  for (long t = 0; t < Ncpp; ++t) tcpp[t] = (qzdy[t] + qydz[t] - 0.0) * MU(muiyz, t);
   op(f, 2, ORC_1DER_P2C, f->mtau[1][2], tcpp, nx, py, pz, r1, nx, py, nz, NULL);
   for (long t = 0; t < Ncpc; ++t) acc_z[t] = acc_z[t] + r1[t] * rre;
   for (long t = 0; t < Ncpc; ++t) my[t] = my[t] + acc_y[t];
@@ -1096,13 +1158,13 @@ void orc_momentum_rhs(orc_flow *f, int isub, double *mx, double *my,
   }
 
   /* ---------------- z-momentum ---------------- */
-  for (long t = 0; t < Npcp; ++t) tpcp[t] = -qxiz[t] * qzix[t];
+  for (long t = 0; t < Npcp; ++t) tpcp[t] = -gxiz[t] * qzix[t];
   op(f, 0, ORC_1DER_P2C, f->mcov[2][0], tpcp, px, ny, pz, r1, nx, ny, pz, NULL);
   for (long t = 0; t < Nccp; ++t) mz[t] = 0.0 + r1[t];
-  for (long t = 0; t < Ncpp; ++t) tcpp[t] = -qyiz[t] * qziy[t];
+  for (long t = 0; t < Ncpp; ++t) tcpp[t] = -gyiz[t] * qziy[t];
   op(f, 1, ORC_1DER_P2C, f->mcov[2][1], tcpp, nx, py, pz, r1, nx, ny, pz, NULL);
   for (long t = 0; t < Nccp; ++t) acc_y[t] = 0.0 + r1[t];
-  for (long t = 0; t < Nccc; ++t) tc[t] = -qziz[t] * qziz[t];
+  for (long t = 0; t < Nccc; ++t) tc[t] = -gziz[t] * qziz[t];
   op(f, 2, ORC_1DER_C2P, f->mcov[2][2], tc, nx, ny, nz, r1, nx, ny, pz, maxp_z);
   for (long t = 0; t < Nccp; ++t) acc_z[t] = 0.0 + r1[t];
   {
@@ -1111,13 +1173,14 @@ void orc_momentum_rhs(orc_flow *f, int isub, double *mx, double *my,
     for (long t = 0; t < Nccp; ++t) pfc[t] = 0.0 + pf[t];
     free(pf);
   }
-  for (long t = 0; t < Npcp; ++t) tpcp[t] = (qxdz[t] + qzdx[t]) * 1.0;
+  for (long t = 0; t < Npcp; ++t) tpcp[t] = (qxdz[t] + qzdx[t]) * MU(muixz, t);
   op(f, 0, ORC_1DER_P2C, f->mtau[2][0], tpcp, px, ny, pz, r1, nx, ny, pz, NULL);
   for (long t = 0; t < Nccp; ++t) mz[t] = mz[t] + r1[t] * rre;
-  for (long t = 0; t < Ncpp; ++t) tcpp[t] = (qzdy[t] + qydz[t]) * 1.0;
+  for (long t = 0; t < Ncpp; ++t) tcpp[t] = (qzdy[t] + qydz[t]) * MU(muiyz, t);
   op(f, 1, ORC_1DER_P2C, f->mtau[2][1], tcpp, nx, py, pz, r1, nx, ny, pz, NULL);
   for (long t = 0; t < Nccp; ++t) acc_y[t] = acc_y[t] + r1[t] * rre;
-  for (long t = 0; t < Nccc; ++t) tc[t] = ((qzdz[t] - ONE_THIRD * div[t]) + 0.0) * 2.0 * 1.0;
+  for (long t = 0; t < Nccc; ++t) tc[t] = ((qzdz[t] - ONE_THIRD * div[t]) + 0.0) * 2.0 * MU(mu_c, t);
+#undef MU
   op(f, 2, ORC_1DER_C2P, f->mtau[2][2], tc, nx, ny, nz, r1, nx, ny, pz, maxp_z);
   for (long t = 0; t < Nccp; ++t) acc_z[t] = acc_z[t] + r1[t] * rre;
   for (long t = 0; t < Nccp; ++t) mz[t] = mz[t] + acc_y[t];
@@ -1141,6 +1204,10 @@ void orc_momentum_rhs(orc_flow *f, int isub, double *mx, double *my,
   free(maxp_x); free(maxp_y); free(maxp_z); free(tc); free(tppc); free(tpcp);
   free(tcpp); free(r1); free(acc_y); free(acc_z); free(pfc); free(mpres);
   free(nfbx); free(nfby); free(nfbz); free(fbc_own);
+  if (th) {
+    free(gxix); free(gxiy); free(gxiz); free(gyix); free(gyiy); free(gyiz);
+    free(gzix); free(gziy); free(gziz); free(muixy); free(muixz); free(muiyz); free(fbmu_y);
+  }
 }
 
 /* the projection half of Solve_momentum_eq (:1866-1921): Poisson RHS,
@@ -1169,20 +1236,56 @@ void orc_project(orc_flow *f, int isub) {
   free(mx); free(my); free(mz); free(div);
 }
 
-/* Solve_momentum_eq, src/eq_momentum2.f90:1799-1932 */
+/* convert_primary_conservative, src/para_conversion.f90:5-227: density
+ * interpolated to each velocity location (midp C2P with the property BCs and
+ * the wall density), g = q d (IQ2G) or q = g / d (IG2Q).  The walls are
+ * stationary, so the wall values of g and q are zero and stay so. */
+static void convert_qg(orc_flow *f, int to_g) {
+  int nx = f->nx, ny = f->ny, nz = f->nz, px = f->px, py = f->py, pz = f->pz;
+  long Npcc = (long)px * ny * nz, Ncpc = (long)nx * py * nz, Nccp = (long)nx * ny * pz;
+  double *dx = arr(Npcc), *dy = arr(Ncpc), *dz = arr(Nccp);
+  op(f, 0, ORC_MIDP_C2P, f->ftp[0], f->dens, nx, ny, nz, dx, px, ny, nz, NULL);
+  op(f, 1, ORC_MIDP_C2P, f->ftp[1], f->dens, nx, ny, nz, dy, nx, py, nz, f->fby_ftp[3]);
+  op(f, 2, ORC_MIDP_C2P, f->ftp[2], f->dens, nx, ny, nz, dz, nx, ny, pz, NULL);
+  if (to_g) {
+    for (long t = 0; t < Npcc; ++t) f->gx[t] = f->qx[t] * dx[t];
+    for (long t = 0; t < Ncpc; ++t) f->gy[t] = f->qy[t] * dy[t];
+    for (long t = 0; t < Nccp; ++t) f->gz[t] = f->qz[t] * dz[t];
+  } else {
+    for (long t = 0; t < Npcc; ++t) f->qx[t] = f->gx[t] / dx[t];
+    for (long t = 0; t < Ncpc; ++t) f->qy[t] = f->gy[t] / dy[t];
+    for (long t = 0; t < Nccp; ++t) f->qz[t] = f->gz[t] / dz[t];
+  }
+  free(dx); free(dy); free(dz);
+}
+
+/* Solve_momentum_eq, src/eq_momentum2.f90:1799-1932; with variable
+ * properties the right-hand side advances g, the divergence and the
+ * correction act on g, the Poisson right-hand side carries drhodt
+ * (Calculate_drhodt, src/eq_continuity.f90:26-78) and q = g / d closes the
+ * sub-step (:1915) */
 void orc_solve_momentum_eq(orc_flow *f, int isub) {
   int nx = f->nx, ny = f->ny, nz = f->nz, px = f->px, py = f->py, pz = f->pz;
   long Npcc = (long)px * ny * nz, Ncpc = (long)nx * py * nz, Nccp = (long)nx * ny * pz;
   long Nccc = (long)nx * ny * nz;
+  double *ux = f->thermo ? f->gx : f->qx, *uy = f->thermo ? f->gy : f->qy;
+  double *uz = f->thermo ? f->gz : f->qz;
   double *mx = arr(Npcc), *my = arr(Ncpc), *mz = arr(Nccp);
   orc_momentum_rhs(f, isub, mx, my, mz);
-  for (long t = 0; t < Npcc; ++t) f->qx[t] = f->qx[t] + mx[t];
-  for (long t = 0; t < Ncpc; ++t) f->qy[t] = f->qy[t] + my[t];
-  for (long t = 0; t < Nccp; ++t) f->qz[t] = f->qz[t] + mz[t];
+  for (long t = 0; t < Npcc; ++t) ux[t] = ux[t] + mx[t];
+  for (long t = 0; t < Ncpc; ++t) uy[t] = uy[t] + my[t];
+  for (long t = 0; t < Nccp; ++t) uz[t] = uz[t] + mz[t];
   enforce_velo(f);
   /* solve_poisson, :1663-1718 */
   double *div = arr(Nccc);
   for (long t = 0; t < Nccc; ++t) f->pcor[t] = 0.0;
+  if (f->thermo) {
+    /* Calculate_drhodt, default (Euler) branch, :70-72.  Update_thermal_
+     * properties sets dDensm1 = dDens right after refreshing the density
+     * (src/eq_energy.f90:44-45), so this difference is exactly zero */
+    for (long t = 0; t < Nccc; ++t) f->pcor[t] = f->dens[t] - f->densm1[t];
+    for (long t = 0; t < Nccc; ++t) f->pcor[t] = f->pcor[t] / f->dt;
+  }
   orc_divergence(f, div);
   for (long t = 0; t < Nccc; ++t) f->pcor[t] = f->pcor[t] + div[t];
   double coeff = 1.0 / (f->ta[isub] * f->sigma2p * f->dt);
@@ -1192,17 +1295,216 @@ void orc_solve_momentum_eq(orc_flow *f, int isub) {
   /* Correct_massflux, :1577-1660 */
   double s = f->dt * f->ta[isub] * f->sigma2p;
   op(f, 0, ORC_1DER_C2P, f->ibc[0][VP], f->pcor, nx, ny, nz, mx, px, ny, nz, f->fbx[VP]);
-  for (long t = 0; t < Npcc; ++t) f->qx[t] = f->qx[t] - s * mx[t];
+  for (long t = 0; t < Npcc; ++t) ux[t] = ux[t] - s * mx[t];
   op(f, 1, ORC_1DER_C2P, f->ibc[1][VP], f->pcor, nx, ny, nz, my, nx, py, nz, f->fby[VP]);
-  for (long t = 0; t < Ncpc; ++t) f->qy[t] = f->qy[t] - s * my[t];
+  for (long t = 0; t < Ncpc; ++t) uy[t] = uy[t] - s * my[t];
   op(f, 2, ORC_1DER_C2P, f->ibc[2][VP], f->pcor, nx, ny, nz, mz, nx, ny, pz, f->fbz[VP]);
-  for (long t = 0; t < Nccp; ++t) f->qz[t] = f->qz[t] - s * mz[t];
+  for (long t = 0; t < Nccp; ++t) uz[t] = uz[t] - s * mz[t];
   enforce_velo(f);
+  if (f->thermo) convert_qg(f, 0);
   free(mx); free(my); free(mz); free(div);
 }
 
-/* one time step of Solve_eqs_iteration, src/chapsim.f90:243-303 */
+/* ---------------------------------------------------------------------- */
+/* Variable properties (config: heated channel, isothermal walls)          */
+/* ---------------------------------------------------------------------- */
+/* Update_thermal_properties, src/eq_energy.f90:14-47 (isothermal walls: the
+ * Neumann wall branches :51-143 do not apply): every cell from rho*h by
+ * ftp_refresh_thermal_properties_from_DH, then dDensm2 = dDensm1,
+ * dDensm1 = dDens.  Returns the number of clamped cells. */
+static long refresh_props(orc_flow *f) {
+  long n = (long)f->nx * f->ny * f->nz, bad = 0;
+  for (long t = 0; t < n; ++t) {
+    double o[7];
+    bad += orc_props_from_dh(f->props, f->rhoh[t], o);
+    f->hent[t] = o[0];
+    f->temp[t] = o[4];
+    f->kcond[t] = o[3];
+    f->dens[t] = o[1];
+    f->visc[t] = o[2];
+  }
+  for (long t = 0; t < n; ++t) f->densm2[t] = f->densm1[t];
+  for (long t = 0; t < n; ++t) f->densm1[t] = f->dens[t];
+  return bad;
+}
+
+/* Set up the variable-property path: property table, wall temperatures of
+ * the isothermal y walls (non-dimensional; wall properties from T by the
+ * table, bc_dirichlet.f90:433-437 with input_thermo.f90:213-241), the
+ * reference Prandtl number m0 cp0 / k0 (1/(Re Pr) = rPrRen,
+ * tools_solver.f90:63) and a uniform initial state from T (initialise_
+ * thermal_properties, input_thermo.f90:1070-1103).  Stationary y walls and
+ * x, z periodic only.  Returns 0, or -1 when the configuration is outside
+ * that. */
+int orc_thermo_enable(orc_flow *f, const orc_props *p, const double t_wall[2], double prandtl, double t_init) {
+  if (!f->cfg.wall_y) return -1;
+  for (int s2 = 0; s2 < 2; ++s2)
+    for (int v = 0; v < 3; ++v)
+      if (f->cfg.uwall[s2][v] != 0.0) return -1;
+  int nx = f->nx, ny = f->ny, nz = f->nz, px = f->px, py = f->py, pz = f->pz;
+  long Nccc = (long)nx * ny * nz;
+  f->thermo = 1;
+  f->props = p;
+  f->prandtl = prandtl;
+  for (int s2 = 0; s2 < 2; ++s2) orc_props_from_t(p, t_wall[s2], f->wall[s2]);
+  f->gx = arr((long)px * ny * nz); f->gy = arr((long)nx * py * nz); f->gz = arr((long)nx * ny * pz);
+  f->dens = arr(Nccc); f->visc = arr(Nccc); f->rhoh = arr(Nccc); f->hent = arr(Nccc);
+  f->temp = arr(Nccc); f->kcond = arr(Nccc); f->erhs0 = arr(Nccc); f->densm1 = arr(Nccc);
+  f->densm2 = arr(Nccc);
+  /* wall arrays (nx,4,nz) of h, k, m, d (rows 3,4 repeat rows 1,2) */
+  const int col[4] = {0, 3, 2, 1};
+  for (int v = 0; v < 4; ++v) {
+    f->fby_ftp[v] = arr((long)nx * 4 * nz);
+    for (int k = 0; k < nz; ++k)
+      for (int r = 0; r < 4; ++r)
+        for (int i = 0; i < nx; ++i) f->fby_ftp[v][i + (long)nx * (r + 4L * k)] = f->wall[r % 2][col[v]];
+  }
+  double o[7];
+  orc_props_from_t(p, t_init, o);
+  for (long t = 0; t < Nccc; ++t) {
+    f->dens[t] = o[1];
+    f->visc[t] = o[2];
+    f->rhoh[t] = o[1] * o[0];
+    f->hent[t] = o[0];
+    f->kcond[t] = o[3];
+    f->temp[t] = o[4];
+    f->densm1[t] = o[1];
+    f->densm2[t] = o[1];
+  }
+  orc_flow_update_re(f, 0);
+  convert_qg(f, 1);
+  return 0;
+}
+
+/* rho*h given per cell: properties refreshed from it and g = q d of the
+ * current velocity.  Returns the number of clamped cells. */
+long orc_thermo_set_rhoh(orc_flow *f, const double *rhoh) {
+  long n = (long)f->nx * f->ny * f->nz;
+  memcpy(f->rhoh, rhoh, sizeof(double) * (size_t)n);
+  long bad = refresh_props(f);
+  convert_qg(f, 1);
+  return bad;
+}
+
+/* state of the variable-property path: rho*h, h, T, k, d, mu (ccc each) and
+ * g (pcc, cpc, ccp); any pointer may be NULL */
+void orc_thermo_get(const orc_flow *f, double *rhoh, double *h, double *t, double *k, double *d, double *m,
+                    double *gx, double *gy, double *gz) {
+  size_t n = (size_t)f->nx * f->ny * f->nz;
+  if (rhoh) memcpy(rhoh, f->rhoh, sizeof(double) * n);
+  if (h) memcpy(h, f->hent, sizeof(double) * n);
+  if (t) memcpy(t, f->temp, sizeof(double) * n);
+  if (k) memcpy(k, f->kcond, sizeof(double) * n);
+  if (d) memcpy(d, f->dens, sizeof(double) * n);
+  if (m) memcpy(m, f->visc, sizeof(double) * n);
+  if (gx) memcpy(gx, f->gx, sizeof(double) * (size_t)f->px * f->ny * f->nz);
+  if (gy) memcpy(gy, f->gy, sizeof(double) * (size_t)f->nx * f->py * f->nz);
+  if (gz) memcpy(gz, f->gz, sizeof(double) * (size_t)f->nx * f->ny * f->pz);
+}
+
+/* Compute_energy_rhs, src/eq_energy.f90:178-396, with Calculate_energy_
+ * fractional_step (:148-176): out = dt (gamma rhs + zeta rhs0), rhs0 = rhs.
+ * rhs = -d(g_i h_i)/dx_i + rPrRen d(k_i dT/dx_i)/dx_i with h and k
+ * interpolated to the faces (midp C2P, property BCs, wall h / k), dT/dx_i
+ * by 1der C2P with the T BCs (wall T, get_fbcy_iTh: rows 1,2 only), the
+ * flux's wall values taken from its own wall rows (extract_dirichlet_fbcy)
+ * before the outer P2C derivative.  The reference guards that last step
+ * with is_fbcx_velo_required (:356); on a channel with periodic x that is
+ * false and the wall values passed would be MAXP (1e20), turning the wall
+ * rows of the y diffusion into ~1e19: the y-wall test
+ * (is_fbcy_velo_required) is followed here.  Sums: the x part, then
+ * (y part + z part) as in the pencil order of :386-391. */
+void orc_energy_rhs(orc_flow *f, int isub, double *rhs) {
+  int nx = f->nx, ny = f->ny, nz = f->nz, px = f->px, py = f->py, pz = f->pz;
+  long Npcc = (long)px * ny * nz, Ncpc = (long)nx * py * nz, Nccp = (long)nx * ny * pz;
+  long Nccc = (long)nx * ny * nz;
+  int (*B)[5][2] = f->ibc;
+  int econv[3][2], edifu[3][2];
+  for (int d = 0; d < 3; ++d) {
+    bc_prod(B[d][d], f->ftp[d], econv[d]);
+    bc_tau(f->ftp[d], B[d][VT], edifu[d]);
+  }
+  double *hpcc = arr(Npcc), *hcpc = arr(Ncpc), *hccp = arr(Nccp);
+  double *kpcc = arr(Npcc), *kcpc = arr(Ncpc), *kccp = arr(Nccp);
+  op(f, 0, ORC_MIDP_C2P, f->ftp[0], f->hent, nx, ny, nz, hpcc, px, ny, nz, NULL);
+  op(f, 1, ORC_MIDP_C2P, f->ftp[1], f->hent, nx, ny, nz, hcpc, nx, py, nz, f->fby_ftp[0]);
+  op(f, 2, ORC_MIDP_C2P, f->ftp[2], f->hent, nx, ny, nz, hccp, nx, ny, pz, NULL);
+  op(f, 0, ORC_MIDP_C2P, f->ftp[0], f->kcond, nx, ny, nz, kpcc, px, ny, nz, NULL);
+  op(f, 1, ORC_MIDP_C2P, f->ftp[1], f->kcond, nx, ny, nz, kcpc, nx, py, nz, f->fby_ftp[1]);
+  op(f, 2, ORC_MIDP_C2P, f->ftp[2], f->kcond, nx, ny, nz, kccp, nx, ny, pz, NULL);
+  double *ex = arr(Nccc), *ey = arr(Nccc), *ez = arr(Nccc), *r = arr(Nccc);
+  double *apcc = arr(Npcc), *acpc = arr(Ncpc), *accp = arr(Nccp);
+  /* convection */
+  for (long t = 0; t < Npcc; ++t) apcc[t] = -f->gx[t] * hpcc[t];
+  op(f, 0, ORC_1DER_P2C, econv[0], apcc, px, ny, nz, r, nx, ny, nz, NULL);
+  for (long t = 0; t < Nccc; ++t) ex[t] = 0.0 + r[t];
+  for (long t = 0; t < Ncpc; ++t) acpc[t] = -f->gy[t] * hcpc[t];
+  op(f, 1, ORC_1DER_P2C, econv[1], acpc, nx, py, nz, r, nx, ny, nz, NULL);
+  for (long t = 0; t < Nccc; ++t) ey[t] = 0.0 + r[t];
+  for (long t = 0; t < Nccp; ++t) accp[t] = -f->gz[t] * hccp[t];
+  op(f, 2, ORC_1DER_P2C, econv[2], accp, nx, ny, pz, r, nx, ny, nz, NULL);
+  for (long t = 0; t < Nccc; ++t) ez[t] = 0.0 + r[t];
+  /* diffusion */
+  long nfy = (long)nx * 4 * nz;
+  double *fbt = arr(nfy), *fbq = arr(nfy);
+  for (int k = 0; k < nz; ++k)
+    for (int i = 0; i < nx; ++i)
+      for (int s2 = 0; s2 < 2; ++s2) fbt[i + (long)nx * (s2 + 4L * k)] = (B[1][VT][s2] == BDIR) ? f->wall[s2][4] : 0.0;
+  op(f, 0, ORC_1DER_C2P, B[0][VT], f->temp, nx, ny, nz, apcc, px, ny, nz, NULL);
+  for (long t = 0; t < Npcc; ++t) apcc[t] = apcc[t] * kpcc[t];
+  op(f, 0, ORC_1DER_P2C, edifu[0], apcc, px, ny, nz, r, nx, ny, nz, NULL);
+  for (long t = 0; t < Nccc; ++t) ex[t] = ex[t] + r[t] * f->rprren;
+  op(f, 1, ORC_1DER_C2P, B[1][VT], f->temp, nx, ny, nz, acpc, nx, py, nz, fbt);
+  for (long t = 0; t < Ncpc; ++t) acpc[t] = acpc[t] * kcpc[t];
+  for (int k = 0; k < nz; ++k)
+    for (int i = 0; i < nx; ++i)
+      for (int s2 = 0; s2 < 2; ++s2) {
+        double v = acpc[i + (long)nx * ((s2 ? py - 1 : 0) + (long)py * k)];
+        fbq[i + (long)nx * (s2 + 4L * k)] = v;
+        fbq[i + (long)nx * (s2 + 2 + 4L * k)] = v;
+      }
+  op(f, 1, ORC_1DER_P2C, edifu[1], acpc, nx, py, nz, r, nx, ny, nz, fbq);
+  for (long t = 0; t < Nccc; ++t) ey[t] = ey[t] + r[t] * f->rprren;
+  op(f, 2, ORC_1DER_C2P, B[2][VT], f->temp, nx, ny, nz, accp, nx, ny, pz, NULL);
+  for (long t = 0; t < Nccp; ++t) accp[t] = accp[t] * kccp[t];
+  op(f, 2, ORC_1DER_P2C, edifu[2], accp, nx, ny, pz, r, nx, ny, nz, NULL);
+  for (long t = 0; t < Nccc; ++t) ez[t] = ez[t] + r[t] * f->rprren;
+  for (long t = 0; t < Nccc; ++t) ey[t] = ey[t] + ez[t];
+  for (long t = 0; t < Nccc; ++t) rhs[t] = ex[t] + ey[t];
+  /* fractional step */
+  for (long t = 0; t < Nccc; ++t) {
+    double cur = rhs[t], last = f->erhs0[t];
+    double tot = f->tg[isub] * cur + f->tz[isub] * last;
+    f->erhs0[t] = cur;
+    rhs[t] = f->dt * tot;
+  }
+  free(hpcc); free(hcpc); free(hccp); free(kpcc); free(kcpc); free(kccp);
+  free(ex); free(ey); free(ez); free(r); free(apcc); free(acpc); free(accp); free(fbt); free(fbq);
+}
+
+/* Solve_energy_eq, src/eq_energy.f90:398-458: (isub 3: density history
+ * shifted), energy right-hand side, rho*h += rhs, properties refreshed.
+ * Returns the number of clamped cells of the refresh. */
+long orc_solve_energy_eq(orc_flow *f, int isub) {
+  long n = (long)f->nx * f->ny * f->nz;
+  if (isub == 3) {
+    for (long t = 0; t < n; ++t) f->densm2[t] = f->densm1[t];
+    for (long t = 0; t < n; ++t) f->densm1[t] = f->dens[t];
+  }
+  double *r = arr(n);
+  orc_energy_rhs(f, isub, r);
+  for (long t = 0; t < n; ++t) f->rhoh[t] = f->rhoh[t] + r[t];
+  free(r);
+  return refresh_props(f);
+}
+
+/* one time step of Solve_eqs_iteration, src/chapsim.f90:243-303: per
+ * sub-step the energy equation (variable properties) then the momentum
+ * equation (:300-304) */
 void orc_flow_step(orc_flow *f, int iter) {
   orc_flow_update_re(f, iter);
-  for (int isub = 1; isub <= 3; ++isub) orc_solve_momentum_eq(f, isub);
+  for (int isub = 1; isub <= 3; ++isub) {
+    if (f->thermo) orc_solve_energy_eq(f, isub);
+    orc_solve_momentum_eq(f, isub);
+  }
 }

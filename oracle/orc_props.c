@@ -5,7 +5,10 @@
  *   table build   src/input_thermo.f90:617-693 (read, sort by T, reference
  *                 state from T, non-dimensionalisation, rho*h, dhmin/dhmax)
  *   sort          src/input_thermo.f90:512-556 (selection by minimum T)
- *   from T        src/input_thermo.f90:121-154 (bisection on T, linear weights)
+ *   from T        src/input_thermo.f90:121-154 (bisection on T, linear weights;
+ *                 reference state) and :213-241 (the same on the
+ *                 non-dimensional table: ftp_refresh_thermal_properties_
+ *                 from_T_undim, wall properties of isothermal walls)
  *   from H        src/input_thermo.f90:345-374 (bisection on h)
  *   from rho*h    src/input_thermo.f90:393-446 (clamp to [dhmin, dhmax],
  *                 bisection on rho*h, h interpolated, properties from h,
@@ -63,6 +66,18 @@ int orc_props_build(const double *raw, int nlist, double ref_t0, orc_props *p) {
       cols[q][i] = cols[q][kmin];
       cols[q][kmin] = buf;
     }
+  }
+  p->dimensional = ref_t0 > 0.0;
+  if (!p->dimensional) {
+    /* rows already non-dimensional: no reference state, no scaling */
+    int imn = 0, imx = 0;
+    for (int i = 1; i < nlist; ++i) {
+      if (p->rhoh[i] < p->rhoh[imn]) imn = i;
+      if (p->rhoh[i] > p->rhoh[imx]) imx = i;
+    }
+    p->dhmin = p->rhoh[imn];
+    p->dhmax = p->rhoh[imx];
+    return 0;
   }
   /* reference state from T0 (dimensional table) */
   int i1, i2;
@@ -126,4 +141,19 @@ int orc_props_from_dh(const orc_props *p, double rhoh, double out[7]) {
   out[0] = h;
   for (int q = 0; q < 6; ++q) out[1 + q] = pr[q];
   return clamped;
+}
+
+/* ftp_refresh_thermal_properties_from_T_undim, table branch
+ * (src/input_thermo.f90:213-241): out = h, d, m, k, t, b, cp */
+void orc_props_from_t(const orc_props *p, double t, double out[7]) {
+  int i1, i2;
+  bisect(p->t, p->n, t, &i1, &i2);
+  double w1 = (p->t[i2] - t) / (p->t[i2] - p->t[i1]), w2 = 1.0 - w1;
+  out[1] = w1 * p->d[i1] + w2 * p->d[i2];
+  out[2] = w1 * p->m[i1] + w2 * p->m[i2];
+  out[3] = w1 * p->k[i1] + w2 * p->k[i2];
+  out[0] = w1 * p->h[i1] + w2 * p->h[i2];
+  out[5] = w1 * p->b[i1] + w2 * p->b[i2];
+  out[6] = w1 * p->cp[i1] + w2 * p->cp[i2];
+  out[4] = t;
 }

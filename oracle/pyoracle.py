@@ -101,6 +101,15 @@ def lib():
         L.orc_poisson_solve.argtypes = [C.c_void_p, dp]
         L.orc_vol_average_x.restype = C.c_double
         L.orc_vol_average_x.argtypes = [C.c_void_p, dp, C.c_int]
+        L.orc_props_from_t.argtypes = [C.c_void_p, C.c_double, dp]
+        L.orc_thermo_enable.restype = C.c_int
+        L.orc_thermo_enable.argtypes = [C.c_void_p, C.c_void_p, dp, C.c_double, C.c_double]
+        L.orc_thermo_set_rhoh.restype = C.c_long
+        L.orc_thermo_set_rhoh.argtypes = [C.c_void_p, dp]
+        L.orc_thermo_get.argtypes = [C.c_void_p] + [dp] * 9
+        L.orc_energy_rhs.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_solve_energy_eq.restype = C.c_long
+        L.orc_solve_energy_eq.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
         _lib = L
@@ -320,6 +329,41 @@ class Flow:
         lib().orc_poisson_solve(self.h, r.ctypes.data_as(C.POINTER(C.c_double)))
         return r
 
+    # ---- variable-property path ----
+    def thermo_enable(self, props, t_wall, prandtl, t_init):
+        """Isothermal y walls at t_wall (non-dimensional), reference Prandtl
+        number, uniform initial state at t_init; g from the current q."""
+        self._props = props  # the table must outlive the solver's use of it
+        tw = np.ascontiguousarray(t_wall, dtype=np.float64)
+        rc = lib().orc_thermo_enable(self.h, C.addressof(props.s), _p(tw), float(prandtl), float(t_init))
+        if rc != 0:
+            raise ValueError("variable properties need stationary y walls")
+        nx, ny, nz = self.dom.nc
+        self.shapes["ccc"] = (nx, ny, nz)
+
+    def thermo_set_rhoh(self, rhoh):
+        r = np.asfortranarray(rhoh, dtype=np.float64)
+        assert r.shape == self.shapes["pr"]
+        return int(lib().orc_thermo_set_rhoh(self.h, r.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def thermo_get(self):
+        """dict of rhoh, h, t, k, d, m (cell centres) and gx, gy, gz"""
+        names = ("rhoh", "h", "t", "k", "d", "m")
+        out = {k: np.zeros(self.shapes["pr"], order="F") for k in names}
+        for k, sh in (("gx", "qx"), ("gy", "qy"), ("gz", "qz")):
+            out[k] = np.zeros(self.shapes[sh], order="F")
+        lib().orc_thermo_get(self.h, *[out[k].ctypes.data_as(C.POINTER(C.c_double))
+                                       for k in names + ("gx", "gy", "gz")])
+        return out
+
+    def energy_rhs(self, isub):
+        out = np.zeros(self.shapes["pr"], order="F")
+        lib().orc_energy_rhs(self.h, isub, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def energy_substep(self, isub):
+        return int(lib().orc_solve_energy_eq(self.h, isub))
+
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
@@ -332,7 +376,7 @@ def get_threads():
 class OrcProps(C.Structure):
     _MAX = 16384
     _fields_ = [("n", C.c_int)] + [(nm, C.c_double * 16384) for nm in ("t", "d", "m", "k", "b", "cp", "h", "rhoh")] + [
-        ("ref", C.c_double * 7), ("dhmin", C.c_double), ("dhmax", C.c_double)]
+        ("ref", C.c_double * 7), ("dhmin", C.c_double), ("dhmax", C.c_double), ("dimensional", C.c_int)]
 
 
 class Props:
@@ -359,6 +403,13 @@ class Props:
     def from_h(self, h):
         out = (C.c_double * 6)()
         lib().orc_props_from_h(C.byref(self.s), C.c_double(h), out)
+        return np.array(out[:])
+
+    def from_t(self, t):
+        """(h, d, m, k, t, b, cp) from the temperature (table branch of
+        ftp_refresh_thermal_properties_from_T_undim)"""
+        out = (C.c_double * 7)()
+        lib().orc_props_from_t(C.byref(self.s), C.c_double(t), out)
         return np.array(out[:])
 
     def from_dh(self, rhoh):

@@ -61,6 +61,41 @@ def channel_rhs(u, v, w, p, nu, at_x, at_y, at_z):
     return out
 
 
+def channel_rhs_var(u, v, w, p, nu, rho, mu, at_x, at_y, at_z):
+    """Continuous momentum right-hand sides with variable density and
+    viscosity, the form of the reference's thermo path:
+
+      rhs_i = - d_j (rho u_j u_i) - d_i p + nu d_j [mu (d_j u_i + d_i u_j - 2/3 delta_ij div u)]
+
+    rho and mu are functions of y alone, given as (f, f') pairs of callables."""
+    q = (u, v, w)
+    out = []
+    for i, pts in enumerate((at_x, at_y, at_z)):
+        def d(f, *ax):
+            a = [0, 0, 0]
+            for s in ax:
+                a[s] += 1
+            return f.ev(a[0], a[1], a[2], *pts)
+
+        y = np.asarray(pts[1], float)[None, :, None]
+        r, r1 = rho[0](y), rho[1](y)
+        m, m1 = mu[0](y), mu[1](y)
+        qi = d(q[i])
+        conv = np.zeros_like(qi)
+        for j in range(3):
+            conv += r * (d(q[j], j) * qi + d(q[j]) * d(q[i], j))
+        conv += r1 * d(q[1]) * qi  # (d_y rho) u_y u_i
+        div = sum(d(q[j], j) for j in range(3))
+        div_i = sum(d(q[j], j, i) for j in range(3))
+        visc = np.zeros_like(qi)
+        for j in range(3):
+            visc += d(q[i], j, j) + d(q[j], i, j)
+        visc -= 2.0 / 3.0 * div_i
+        s_iy = d(q[i], 1) + d(q[1], i) - (2.0 / 3.0 * div if i == 1 else 0.0)
+        out.append(-conv - d(p, i) + nu * (m * visc + m1 * s_iy))
+    return out
+
+
 class EtaProfile:
     """y profile g(eta(y)) on the two-sided tanh mesh (src/geometry.f90:
     151-269, istret = 2, y in [-1, 1]): eta = 1/2 + atanh(m y) / beta with
