@@ -50,8 +50,6 @@ class ChsDesc(C.Structure):
         ("uwall", (C.c_double * 3) * 2),
         ("precision", C.c_int),
         ("projection", C.c_int),
-        ("div_tol", C.c_double),
-        ("max_refine", C.c_int),
     ]
 
 

@@ -654,6 +654,12 @@ std::string Flow::op2(RhsWork<T>& w, int lane, int dir, const Field<T>& A, int o
 
 std::string Flow::fork() {
   if (!lanes_on) return "";
+  // the side streams are never forked inside a graph capture (set_lanes
+  // turns the lanes off while the handle's stream captures); should that
+  // change, fail here instead of building the nested capture
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(H->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return "flow: side-stream lanes cannot be used while the stream is captured";
   if (hipEventRecord(ev_fork, H->stream) != hipSuccess) return "flow: fork record";
   for (hipStream_t st : side)
     if (hipStreamWaitEvent(st, ev_fork, 0) != hipSuccess) return "flow: fork wait";

@@ -19,8 +19,7 @@ PROJ_REFERENCE, PROJ_EXACT = 0, 1
 
 def make_desc(nc, lxx, lyb, lyt, lzz, periodic_y=False, istret=0, rstret=0.0, iacc=IACCU_CP6,
               dt=1e-3, ren=1.0, reninit=None, init_re_to=0, idriven=0, poisson=FFT_FISHPACK_2DFFT,
-              uwall=((0.0, 0.0, 0.0), (0.0, 0.0, 0.0)), precision=PREC_FP64, projection=0,
-              div_tol=1e-10, max_refine=50):
+              uwall=((0.0, 0.0, 0.0), (0.0, 0.0, 0.0)), precision=PREC_FP64, projection=0):
     d = _lib.ChsDesc()
     for k in range(3):
         d.nc[k] = int(nc[k])
@@ -39,8 +38,6 @@ def make_desc(nc, lxx, lyb, lyt, lzz, periodic_y=False, istret=0, rstret=0.0, ia
             d.uwall[s][v] = float(uwall[s][v])
     d.precision = int(precision)
     d.projection = int(projection)
-    d.div_tol = float(div_tol)
-    d.max_refine = int(max_refine)
     return d
 
 

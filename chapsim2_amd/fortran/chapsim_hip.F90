@@ -14,6 +14,7 @@ module chapsim_hip
   integer(c_int), parameter, public :: CHS_PROJ_REFERENCE = 0, CHS_PROJ_EXACT = 1
   integer(c_int), parameter, public :: CHS_PREC_FP64 = 0, CHS_PREC_MIXED = 1
   integer(c_int), parameter, public :: CHS_POISSON_FISHPACK = 2
+  integer(c_int), parameter, public :: CHS_PROPS_TABLE = 0, CHS_PROPS_MLP = 1
   ! operator kinds of chs_op3d
   integer(c_int), parameter, public :: CHS_MIDP_C2P = 0, CHS_MIDP_P2C = 1, CHS_1DER_C2C = 2, &
                                        CHS_1DER_P2P = 3, CHS_1DER_C2P = 4, CHS_1DER_P2C = 5
@@ -34,14 +35,23 @@ module chapsim_hip
     real(c_double) :: uwall(3, 2)
     integer(c_int) :: precision
     integer(c_int) :: projection
-    real(c_double) :: div_tol
-    integer(c_int) :: max_refine
   end type chs_desc_t
+
+  !> chs_thermo_desc_t: the heated channel's property table and walls
+  type, bind(C), public :: chs_thermo_desc_t
+    type(c_ptr) :: props
+    integer(c_int) :: mode
+    real(c_double) :: t_wall(2)
+    real(c_double) :: prandtl
+    real(c_double) :: t_init
+  end type chs_thermo_desc_t
 
   public :: chs_abi_version, chs_create, chs_destroy, chs_sync, chs_op3d, chs_poisson
   public :: chs_set_state, chs_get_state, chs_update_re, chs_substep, chs_run, chs_mass_check, chs_cfl, chs_bulk
   public :: chs_velocity_range
   public :: chs_divergence, chs_inspect_coeff, chs_inspect_op1d, chs_error_message, chs_check
+  public :: chs_props_create, chs_props_destroy, chs_props_from_rhoh, chs_props_set_surrogate
+  public :: chs_thermo_enable, chs_thermo_set_rhoh, chs_thermo_set_mode, chs_thermo_get, chs_energy_rhs
   public :: Get_x_midp_C2P_3D, Get_x_midp_P2C_3D, Get_x_1der_C2C_3D, Get_x_1der_P2P_3D, &
             Get_x_1der_C2P_3D, Get_x_1der_P2C_3D, Get_y_midp_C2P_3D, Get_y_midp_P2C_3D, &
             Get_y_1der_C2C_3D, Get_y_1der_P2P_3D, Get_y_1der_C2P_3D, Get_y_1der_P2C_3D, &
@@ -139,6 +149,61 @@ module chapsim_hip
       type(c_ptr), value :: h
       real(c_double), intent(inout) :: div(*)
       integer(c_int), value :: mem
+    end function
+    ! property table (buildup_property_relations_from_table, src/input_thermo.f90:617-693)
+    integer(c_int) function chs_props_create(table, nrows, ref_t0, p) bind(C, name="chs_props_create")
+      import :: c_int, c_ptr, c_double
+      real(c_double), intent(in) :: table(*)
+      integer(c_int), value :: nrows
+      real(c_double), value :: ref_t0
+      type(c_ptr), intent(out) :: p
+    end function
+    integer(c_int) function chs_props_destroy(p) bind(C, name="chs_props_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p
+    end function
+    ! ftp_refresh_thermal_properties_from_DH per element (src/input_thermo.f90:393-446);
+    ! outputs are c_loc of arrays or c_null_ptr
+    integer(c_int) function chs_props_from_rhoh(p, rhoh, n, h, d, m, k, t, b, cp, mode, mem) &
+        bind(C, name="chs_props_from_rhoh")
+      import :: c_int, c_long, c_ptr, c_double
+      type(c_ptr), value :: p
+      real(c_double), intent(in) :: rhoh(*)
+      integer(c_long), value :: n
+      type(c_ptr), value :: h, d, m, k, t, b, cp
+      integer(c_int), value :: mode, mem
+    end function
+    integer(c_int) function chs_props_set_surrogate(p, mlp) bind(C, name="chs_props_set_surrogate")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p, mlp
+    end function
+    ! variable-property heated channel (Solve_energy_eq + the thermo branches of Solve_momentum_eq)
+    integer(c_int) function chs_thermo_enable(h, desc) bind(C, name="chs_thermo_enable")
+      import :: c_int, c_ptr, chs_thermo_desc_t
+      type(c_ptr), value :: h
+      type(chs_thermo_desc_t), intent(in) :: desc
+    end function
+    integer(c_int) function chs_thermo_set_rhoh(h, rhoh, mem) bind(C, name="chs_thermo_set_rhoh")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: rhoh(*)
+      integer(c_int), value :: mem
+    end function
+    integer(c_int) function chs_thermo_set_mode(h, mode) bind(C, name="chs_thermo_set_mode")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: mode
+    end function
+    integer(c_int) function chs_thermo_get(h, rhoh, hh, t, k, d, m, gx, gy, gz, mem) bind(C, name="chs_thermo_get")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, rhoh, hh, t, k, d, m, gx, gy, gz
+      integer(c_int), value :: mem
+    end function
+    integer(c_int) function chs_energy_rhs(h, isub, rhs, mem) bind(C, name="chs_energy_rhs")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: isub, mem
+      real(c_double), intent(inout) :: rhs(*)
     end function
     real(c_double) function chs_inspect_coeff(fam, is_rhs, row, k, bc, acc) bind(C, name="chs_inspect_coeff")
       import :: c_int, c_double

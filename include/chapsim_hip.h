@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define CHS_ABI_VERSION 1
+#define CHS_ABI_VERSION 2
 
 enum {
   CHS_OK = 0,
@@ -82,8 +82,6 @@ typedef struct chs_desc {
   double uwall[2][3];   /* y-wall velocities (bottom, top) x (u, v, w) */
   int precision;        /* CHS_PREC_* */
   int projection;       /* CHS_PROJ_* */
-  double div_tol;       /* target max|div u| for projection = 1 */
-  int max_refine;       /* cap on refinement sweeps for projection = 1 */
 } chs_desc_t;
 
 typedef struct chs_handle chs_handle_t;

@@ -14,7 +14,6 @@ program binding_check
   d%iacc = 4; d%dt = 1.0d-3; d%ren = 1.0d0; d%reninit = 1.0d0
   d%init_re_to = 0; d%idriven = 0; d%poisson = CHS_POISSON_FISHPACK
   d%uwall = 0.0d0; d%precision = CHS_PREC_FP64; d%projection = CHS_PROJ_REFERENCE
-  d%div_tol = 1.0d-10; d%max_refine = 0
   write (*, '(A,I0)') 'abi ', chs_abi_version()
   write (*, '(A,ES25.17)') 'coeff ', chs_inspect_coeff(2_c_int, 1_c_int, 3_c_int, 1_c_int, 0_c_int, 4_c_int)
   do j = 1, 24

@@ -34,7 +34,7 @@ LX, LZ = 4 * np.pi, 4 * np.pi / 3
 CFG = dict(istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1)
 
 
-from tolerances import EPS32, EPS64, ly_kappa, pressure_tol  # noqa: F401
+from tolerances import EPS32, EPS64, ly_kappa, pressure_tol, rel_mean_removed, ulp_perturbed  # noqa: F401
 
 
 def rel(a, b):
@@ -55,6 +55,18 @@ def _grad_rel(of, pg, pr):
         gb = dom.op3d(d, 4, 4, bc, pr, shp, fbc)
         worst = max(worst, rel(ga, gb))
     return worst
+
+
+def _floor_tols(of, n_proj, ref, pert):
+    """pressure and pressure-gradient tolerances: the kappa bound, or ten
+    times the oracle's own response to a one-ulp perturbation of the initial
+    velocity (pert: its final state), whichever is larger"""
+    kb = pressure_tol(of.dom, n_proj)
+    sp = rel_mean_removed(pert[3], ref[3])
+    sg = _grad_rel(of, pert[3], ref[3])
+    print(f"pressure tolerance: kappa bound {kb:.1e}, oracle one-ulp response pr {sp:.1e} grad_p {sg:.1e}",
+          flush=True)
+    return max(kb, 10.0 * sp), max(kb, 10.0 * sg)
 
 
 def _compare(tag, got, ref, of, vtol, ptol, gtol):
@@ -89,15 +101,18 @@ def test_fp64_exact_full_size_vs_oracle(po):
         div = dm.mass_check()
     finally:
         dm.close()
-    of = po.Flow(list(nc), LX, -1.0, 1.0, LZ, wall_y=True, iacc=4, poisson=2, projection=1, **CFG)
-    of.set_state(*fields)
-    of.update_re(0)
-    of.substep(0)
-    for it in (1, 2):
-        of.step(it)
-    pt = pressure_tol(of.dom, 7)
+    finals = []
+    for st in (fields, ulp_perturbed(fields)):
+        of = po.Flow(list(nc), LX, -1.0, 1.0, LZ, wall_y=True, iacc=4, poisson=2, projection=1, **CFG)
+        of.set_state(*st)
+        of.update_re(0)
+        of.substep(0)
+        for it in (1, 2):
+            of.step(it)
+        finals.append(of.get_state())
+    pt, gt = _floor_tols(of, 7, finals[0], finals[1])
     print(f"kappa(L_y) {ly_kappa(of.dom):.3e}", flush=True)
-    _, bad = _compare("256x128x256 fp64c 2 steps:", got, of.get_state(), of, 1e-12, pt, pt)
+    _, bad = _compare("256x128x256 fp64c 2 steps:", got, finals[0], of, 1e-12, pt, gt)
     print("max|div u|", div, flush=True)
     assert not bad, bad
     assert np.all(div < 1e-12)
@@ -117,23 +132,26 @@ def test_channel_200_steps_vs_oracle(po, mixed):
     dm = Domain(nc, LX, -1.0, 1.0, LZ, projection=PROJ_EXACT, precision=PREC_MIXED if mixed else PREC_FP64,
                 **CFG)
     of = po.Flow(list(nc), LX, -1.0, 1.0, LZ, wall_y=True, iacc=4, poisson=2, projection=1, **CFG)
+    op_ = po.Flow(list(nc), LX, -1.0, 1.0, LZ, wall_y=True, iacc=4, poisson=2, projection=1, **CFG)
     try:
         dm.set_state(*fields)
         of.set_state(*fields)
-        dm.update_re(0)
-        of.update_re(0)
-        dm.substep(0)
-        of.substep(0)
+        op_.set_state(*ulp_perturbed(fields))
+        for x in (dm, of, op_):
+            x.update_re(0)
+            x.substep(0)
         for k in range(4):
             dm.run(50 * k, 50)
             for it in range(50 * k + 1, 50 * k + 51):
                 of.step(it)
+                op_.step(it)
             tag = f"{'mixed' if mixed else 'fp64c'} step {50 * (k + 1)}:"
             if mixed:
                 _, bad = _compare(tag, dm.get_state(), of.get_state(), of, 1e-6, 1e-3, 1e-3)
             else:
-                pt = pressure_tol(of.dom, 1 + 3 * 50 * (k + 1))
-                _, bad = _compare(tag, dm.get_state(), of.get_state(), of, 1e-12, pt, pt)
+                ref = of.get_state()
+                pt, gt = _floor_tols(of, 1 + 3 * 50 * (k + 1), ref, op_.get_state())
+                _, bad = _compare(tag, dm.get_state(), ref, of, 1e-12, pt, gt)
             assert not bad, (tag, bad)
             assert np.all(dm.mass_check() < 1e-12)
     finally:
@@ -221,17 +239,20 @@ def test_batched_channels_vs_oracle(po, i):
     specs, fields, s64, d64 = _batch_states(PREC_FP64)
     _, _, s32, d32 = _batch_states(PREC_MIXED)
     s = specs[i]
-    of = po.Flow(list(s["nc"]), s["lxx"], -1.0, 1.0, s["lzz"], wall_y=True, iacc=4, poisson=2, projection=1,
-                 istret=2, rstret=s["rstret"], dt=s["dt"], ren=s["ren"], idriven=1)
-    of.set_state(*fields[i])
-    of.update_re(0)
-    of.substep(0)
-    for it in (1, 2, 3):
-        of.step(it)
-    ref = of.get_state()
-    pt = pressure_tol(of.dom, 10)
+    finals = []
+    for st in (fields[i], ulp_perturbed(fields[i])):
+        of = po.Flow(list(s["nc"]), s["lxx"], -1.0, 1.0, s["lzz"], wall_y=True, iacc=4, poisson=2, projection=1,
+                     istret=2, rstret=s["rstret"], dt=s["dt"], ren=s["ren"], idriven=1)
+        of.set_state(*st)
+        of.update_re(0)
+        of.substep(0)
+        for it in (1, 2, 3):
+            of.step(it)
+        finals.append(of.get_state())
+    ref = finals[0]
+    pt, gt = _floor_tols(of, 10, ref, finals[1])
     print(f"kappa(L_y) {ly_kappa(of.dom):.3e}", flush=True)
-    _, bad64 = _compare(f"batch[{i}] {s['nc']} fp64:", s64[i], ref, of, 1e-12, pt, pt)
+    _, bad64 = _compare(f"batch[{i}] {s['nc']} fp64:", s64[i], ref, of, 1e-12, pt, gt)
     _, bad32 = _compare(f"batch[{i}] {s['nc']} mixed:", s32[i], ref, of, 1e-6, 1e-3, 1e-3)
     assert not bad64 and not bad32, (bad64, bad32)
     assert d64[i] < 1e-12 and d32[i] < 1e-12
@@ -249,15 +270,18 @@ def test_exact_projection_multi_tile_ymix(po):
     fields = channel_fields(nc, LX, -1.0, 1.0, LZ, 2, 0.075, noise=0.1, seed=77)
     dm = Domain(nc, LX, -1.0, 1.0, LZ, projection=PROJ_EXACT, **CFG)
     of = po.Flow(list(nc), LX, -1.0, 1.0, LZ, wall_y=True, iacc=4, poisson=2, projection=1, **CFG)
+    op_ = po.Flow(list(nc), LX, -1.0, 1.0, LZ, wall_y=True, iacc=4, poisson=2, projection=1, **CFG)
     try:
-        for x in (dm, of):
-            x.set_state(*fields)
+        for x, st in ((dm, fields), (of, fields), (op_, ulp_perturbed(fields))):
+            x.set_state(*st)
             x.update_re(0)
             x.substep(0)
         dm.run(0, 1)
         of.step(1)
-        pt = pressure_tol(of.dom, 4)
-        _, bad = _compare("256x24x256 fp64c multi-tile:", dm.get_state(), of.get_state(), of, 1e-12, pt, pt)
+        op_.step(1)
+        ref = of.get_state()
+        pt, gt = _floor_tols(of, 4, ref, op_.get_state())
+        _, bad = _compare("256x24x256 fp64c multi-tile:", dm.get_state(), ref, of, 1e-12, pt, gt)
         assert not bad, bad
         assert np.all(dm.mass_check() < 1e-12)
     finally:

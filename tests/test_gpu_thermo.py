@@ -153,28 +153,39 @@ def test_thermo_mixed_vs_fp64_oracle(po):
 def test_thermo_surrogate_mode_vs_table(po):
     """The surrogate (trained on the table's own interpolation) in place of
     the table for the properties from h: 10 RK3 steps against the table mode
-    on the same state.  Reported per field; the temperature and properties
-    follow the surrogate's fit error (a few 1e-3 after a short training), and
-    the velocity departs far less."""
+    on the same state.  The properties can differ from the table's by the
+    surrogate's fit error over the enthalpies present (measured here), and
+    the velocity q = g / rho by the density's; each field is held to three
+    times the fit error that drives it.  Reported per field."""
     from chapsim2_amd.domain import PROJ_EXACT, Domain
 
-    dm, of, keep = _pair(po, (32, 24, 16), 17, mode=1, surrogate_steps=3000)
+    dm, of, keep = _pair(po, (32, 24, 16), 17, mode=1, surrogate_steps=6000)
     try:
+        pt, sur = keep
         tab, fields, rhoh, tw, t0 = _initial(po, (32, 24, 16), 17)
         ref = Domain((32, 24, 16), LX, -1.0, 1.0, LZ, projection=PROJ_EXACT, **CFG)
         ref.set_state(*fields)
-        ref.thermo_enable(keep[0], tw, 1.0, t0, 0)
+        ref.thermo_enable(pt, tw, 1.0, t0, 0)
         ref.thermo_set_rhoh(rhoh)
         for x in (dm, ref):
             x.update_re(0)
             x.substep(0)
             x.run(0, 10)
         a, b = dm.thermo_get(), ref.thermo_get()
+        hgrid = np.linspace(b["h"].min(), b["h"].max(), 2001)
+        fit, tbl = sur(hgrid), pt.from_h(hgrid)
+        fit_err = {k: rel(fit[k], tbl[k]) for k in ("t", "d", "m", "k")}
         qa, qb = dm.get_state(), ref.get_state()
         errs = {k: rel(a[k], b[k]) for k in ("t", "d", "m", "k")}
         errs.update({k: rel(x, y) for k, x, y in zip(("qx", "qy", "qz"), qa[:3], qb[:3])})
+        print("surrogate fit error over the state's h:", " ".join(f"{k} {v:.2e}" for k, v in fit_err.items()),
+              flush=True)
         print("surrogate vs table after 10 steps:", " ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
-        assert all(v < 2e-2 for v in errs.values()), errs
+        for k in ("t", "d", "m", "k"):
+            assert errs[k] < 3.0 * fit_err[k] + 1e-12, (k, errs[k], fit_err[k])
+        for k in ("qx", "qy", "qz"):
+            assert errs[k] < 3.0 * fit_err["d"] + 1e-12, (k, errs[k], fit_err["d"])
+        assert max(fit_err.values()) < 5e-2
         assert np.all(dm.mass_check() < 1e-12)
         ref.close()
     finally:

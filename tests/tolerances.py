@@ -8,7 +8,15 @@ number kappa = |lambda|_max / |lambda|_min,nonzero (1.9e4 at ny = 128 with
 tanh beta 1.5).  Two backward-stable FP64 solves of the same system may
 differ by kappa * eps relative, and the pressure is the running sum of one
 increment per projection, so pressure comparisons (mean removed) are held to
-4 * kappa * eps * (number of projections)."""
+4 * kappa * eps * (number of projections).
+
+That bound misses the other source of pressure round-off: each increment
+solves div(u*) / (alpha dt), and div(u*) is a difference of O(1) velocities
+that is O(dt) small, so its relative rounding is ~eps |u| ||D|| / |div u*|
+(~1e-12 at dt = 5e-4).  That floor is measured on the oracle itself: the
+same steps from an initial velocity perturbed by one ulp per value
+(ulp_perturbed); the pressure held to ten times what that moves, when that
+is larger than the kappa bound."""
 import numpy as np
 
 EPS64 = 2.0 ** -52
@@ -42,3 +50,16 @@ def pressure_tol(dom, n_proj):
     """4 * kappa(L_y) * eps per projection, summed over the projections
     whose increments make up the pressure."""
     return 4.0 * ly_kappa(dom) * EPS64 * n_proj
+
+
+def ulp_perturbed(fields, seed=12345):
+    """qx, qy, qz each multiplied by (1 + e eps), e in {-1, 0, 1} per value;
+    the pressure unchanged"""
+    rng = np.random.default_rng(seed)
+    out = [np.asfortranarray(f * (1.0 + EPS64 * rng.integers(-1, 2, f.shape))) for f in fields[:3]]
+    return out + [fields[3]]
+
+
+def rel_mean_removed(a, b):
+    a, b = a - a.mean(), b - b.mean()
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
