@@ -16,7 +16,7 @@ MEM_HOST, MEM_DEVICE = 0, 1
 EXPORTS = [
     "chs_abi_version", "chs_last_error", "chs_create", "chs_destroy", "chs_sync", "chs_op3d",
     "chs_poisson", "chs_set_state", "chs_get_state", "chs_update_re", "chs_substep", "chs_run",
-    "chs_momentum_rhs", "chs_mass_check", "chs_cfl", "chs_bulk", "chs_velocity_range",
+    "chs_momentum_rhs", "chs_mass_check", "chs_cfl", "chs_bulk", "chs_velocity_range", "chs_velocity_range_noalloc",
     "chs_divergence",
     "chs_batch_create", "chs_batch_destroy", "chs_batch_instance", "chs_batch_run",
     "chs_props_create", "chs_props_destroy", "chs_props_info", "chs_props_from_rhoh", "chs_props_from_h",
@@ -94,6 +94,7 @@ def load():
     L.chs_cfl.argtypes = [vp, dp]
     L.chs_bulk.argtypes = [vp, dp]
     L.chs_velocity_range.argtypes = [vp, dp]
+    L.chs_velocity_range_noalloc.argtypes = [vp, dp]
     L.chs_props_set_surrogate.argtypes = [vp, vp]
     L.chs_thermo_enable.argtypes = [vp, C.POINTER(ChsThermoDesc)]
     L.chs_thermo_set_rhoh.argtypes = [vp, vp, C.c_int]

@@ -605,6 +605,15 @@ int chs_velocity_range(chs_handle_t* H, double out[6]) {
   return CHS_OK;
 }
 
+int chs_velocity_range_noalloc(chs_handle_t* H, double out[6]) {
+  FLOW_OR_FAIL(H);
+  if (!out) return fail(CHS_EINVAL, "invalid argument");
+  long bad = 0;
+  RET(F.velocity_range_noalloc(out, &bad));
+  if (bad > 0) return fail(CHS_ENAN, "velocity has non-finite values");
+  return CHS_OK;
+}
+
 int chs_mass_check(chs_handle_t* H, double out[3]) {
   FLOW_OR_FAIL(H);
   RET(F.mass_check(out));

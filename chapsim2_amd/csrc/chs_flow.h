@@ -76,6 +76,9 @@ class Flow {
   // max and min of qx, qy, qz (out[2c], out[2c+1]) and the count of
   // non-finite values among them
   std::string velocity_range(double out[6], long* nonfinite);
+  // the same with the block partials in the library's static per-device
+  // array (range_scratch) and the host copy on the stack: no allocation
+  std::string velocity_range_noalloc(double out[6], long* nonfinite);
   std::string cell_velocity();      // P2C interpolations of qx, qy, qz into w64.qxix, qyiy, qziz
 
   // ---- variable properties (heated channel, isothermal y walls) ----
@@ -124,6 +127,9 @@ class Flow {
   RhsWork<float> w32;
   double *partial = nullptr, *bulk = nullptr;
   long npartial = 0;  // entries of partial (at least kMaxPartials)
+  // device address of this device's static range partials (3 kMaxPartials
+  // doubles, shared by every handle on the device)
+  double* range_scratch = nullptr;
   double* dyw_dev = nullptr;
   double* ymc_dev = nullptr;  // ds/dy at cell centres (stretched y only; made on first use)
 

@@ -21,9 +21,45 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace chs {
 
+// ---------------------------------------------------------------------------
+// Static block partials of the allocation-free velocity range
+// (chs_velocity_range_noalloc): 3 x kMaxPartials doubles (12 KiB) per device,
+// held by each device the library runs on until the process ends.  In host
+// code the array's name is not a device address: kernels and copies take the
+// address hipGetSymbolAddress returns for the current device, looked up once
+// per device when a handle's flow solver is created.  Every handle on a
+// device shares the array, so calls that overlap on one device race; each
+// call overwrites all of it before reading it back.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[3 * kMaxPartials];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device (looked up on the first
+// call for that device).
+static std::string range_scratch_address(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return std::string("hipGetDevice: ") + hipGetErrorString(e);
+  if (dev < 0 || dev >= kMaxScratchDevices) return "device index beyond the static scratch table";
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    double* p = nullptr;
+    e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return "";
+}
 
 #define FCHK(x)                                                             \
   do {                                                                      \
@@ -554,6 +590,7 @@ std::string Flow::create(Handle* H, std::unique_ptr<Flow>* out) {
     FCHK(hipMemcpy(f.dyw_dev, f.dyw.data(), sizeof(double) * ny, hipMemcpyHostToDevice));
     f.owned.push_back(f.dyw_dev);
   }
+  SCHK(range_scratch_address(&f.range_scratch));
   // side streams (default priority) and the events that fork and join them
   for (hipStream_t* st : {&f.side[0], &f.side[1]}) FCHK(hipStreamCreate(st));
   for (hipEvent_t* ev : {&f.ev_fork, &f.ev_join[0], &f.ev_join[1]})
@@ -1003,6 +1040,30 @@ std::string Flow::velocity_range(double out[6], long* nonfinite) {
   for (int c = 0; c < 3; ++c) {
     hipLaunchKernelGGL(k_range, dim3(nb), dim3(256), 0, H->stream, q[c]->p, q[c]->size(), partial);
     FCHK(hipMemcpyAsync(h.data(), partial, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, H->stream));
+    FCHK(hipStreamSynchronize(H->stream));
+    double mx = -1.0e20, mn = 1.0e20;
+    for (int b = 0; b < nb; ++b) {
+      if (h[b] > mx) mx = h[b];
+      if (h[nb + b] < mn) mn = h[nb + b];
+      bad += (long)h[2 * nb + b];
+    }
+    out[2 * c] = mx;
+    out[2 * c + 1] = mn;
+  }
+  *nonfinite = bad;
+  return "";
+}
+
+std::string Flow::velocity_range_noalloc(double out[6], long* nonfinite) {
+  if (!range_scratch) return "velocity range: static scratch not set up";
+  constexpr int nb = kMaxPartials;
+  double h[3 * nb];
+  const Field<double>* q[3] = {&qx, &qy, &qz};
+  long bad = 0;
+  for (int c = 0; c < 3; ++c) {
+    // every block writes its three slots, so nothing is cleared first
+    hipLaunchKernelGGL(k_range, dim3(nb), dim3(256), 0, H->stream, q[c]->p, q[c]->size(), range_scratch);
+    FCHK(hipMemcpyAsync(h, range_scratch, sizeof(h), hipMemcpyDeviceToHost, H->stream));
     FCHK(hipStreamSynchronize(H->stream));
     double mx = -1.0e20, mn = 1.0e20;
     for (int b = 0; b < nb; ++b) {

@@ -157,6 +157,17 @@ int chs_bulk(chs_handle_t *h, double out[2]);
  * blown up. */
 int chs_velocity_range(chs_handle_t *h, double out[6]);
 
+/* chs_velocity_range with no allocation of any kind on the device (no
+ * counterpart in the reference): for callers that cannot allocate, such as
+ * tight stepping loops and harnesses that forbid hipMalloc.  The block
+ * partials of the reduction live in a static device array of the library
+ * (3 x 512 doubles per device, kept until the process ends) and come back
+ * through a host stack buffer; the field size is not limited by it.  Same
+ * results and return codes as chs_velocity_range (CHS_ENAN with out filled
+ * when a value is NaN or infinite).  Every handle on one device shares the
+ * array: calls that overlap on one device race. */
+int chs_velocity_range_noalloc(chs_handle_t *h, double out[6]);
+
 /* Divergence of the current velocity, (nc1,nc2,nc3).  Replaces
  * Get_divergence_flow (src/eq_continuity.f90:91-161). */
 int chs_divergence(chs_handle_t *h, double *div, int mem);
