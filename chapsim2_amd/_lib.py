@@ -25,7 +25,7 @@ EXPORTS = [
     "chs_inspect_coeff", "chs_inspect_op1d", "chs_inspect_projection", "chs_profile_begin", "chs_profile_end", "chs_tune_report",
     "chs_op3d_f32", "chs_poisson_f32", "chs_tune_force", "chs_tune_set",
     "chs_props_set_surrogate", "chs_thermo_enable", "chs_thermo_set_rhoh", "chs_thermo_set_mode", "chs_thermo_get",
-    "chs_energy_rhs",
+    "chs_energy_rhs", "chs_mlp_set_matrix_cores",
 ]
 
 
@@ -101,6 +101,7 @@ def load():
     L.chs_thermo_set_mode.argtypes = [vp, C.c_int]
     L.chs_thermo_get.argtypes = [vp] + [vp] * 9 + [C.c_int]
     L.chs_energy_rhs.argtypes = [vp, C.c_int, vp, C.c_int]
+    L.chs_mlp_set_matrix_cores.argtypes = [vp, C.c_int]
     L.chs_divergence.argtypes = [vp, vp, C.c_int]
     L.chs_inspect_coeff.restype = C.c_double
     L.chs_inspect_coeff.argtypes = [C.c_int] * 6

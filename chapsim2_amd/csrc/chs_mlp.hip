@@ -261,6 +261,7 @@ struct chs_mlp {
   float *lossp = nullptr, *tin = nullptr;
   int step = 0;
   uint32_t seed = 1;
+  bool matrix_cores = true;  // property refresh on the matrix cores (k_mlp_from_dh_mfma)
   void* io = nullptr;
   size_t io_bytes = 0;
   std::vector<void*> owned;
@@ -309,6 +310,113 @@ int reduce_grads(chs_mlp* m, int B, int nblk, float* loss) {
 }  // namespace
 
 namespace {
+
+// ---- matrix-core form of the network (inference) ----
+// v_mfma_f32_16x16x16_bf16: D(16x16) = A(16x16) B(16x16) + C, one wave; lane
+// l holds A[l%16][4(l/16)+0..3], B[4(l/16)+0..3][l%16] and
+// D[4(l/16)+0..3][l%16].  The 64x64 layer is 4 (output blocks) x 4 (input
+// blocks) of these per 16 cells: A = a block of W2 (rows: output units,
+// columns: input units), B = the BF16 first-layer activations (rows: input
+// units, columns: cells), D = the second-layer pre-activations.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// tanh x = 1 - 2 / (1 + e^{2x}) with the hardware exponential and a fast
+// divide (absolute error ~1e-7, below the BF16 rounding of the activations)
+__device__ __forceinline__ float tanh_fast(float x) { return 1.0f - __fdividef(2.0f, 1.0f + __expf(2.0f * x)); }
+
+// out[0..4] (normalised) of the 16 cells whose normalised inputs the lanes
+// with l%16 = c hold in xv; every lane ends with the outputs of cell l%16.
+// wa: this lane's 16 A operands (W2 blocks), sp: the FP32 parameters in LDS.
+__device__ __forceinline__ void mlp16_mfma(const bf16x4 wa[16], const float* __restrict__ sp, float xv, float out[NO]) {
+  const int q = (threadIdx.x & 63) >> 4;
+  bf16x4 bv[4];
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int u = 16 * kk + 4 * q + r;
+      bv[kk][r] = (short)f2bf(tanh_fast(sp[OW1 + u] * xv + sp[OB1 + u]));
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < NO; ++o) out[o] = 0.0f;
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[4 * mt + kk], bv[kk], acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int u = 16 * mt + 4 * q + r;
+      const float a2 = tanh_fast(acc[r] + sp[OB2 + u]);
+#pragma unroll
+      for (int o = 0; o < NO; ++o) out[o] += sp[OW3 + o * H2 + u] * a2;
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < NO; ++o) {
+    out[o] += __shfl_xor(out[o], 16);
+    out[o] += __shfl_xor(out[o], 32);
+    out[o] += sp[OB3 + o];
+  }
+}
+
+// k_mlp_from_dh on the matrix cores: 64 cells per wave (one per lane for the
+// clamp and the bisection), the network evaluated for them in four passes of
+// 16 cells, twice (h from the table, then h = rho*h / rho).
+__global__ void __launch_bounds__(256) k_mlp_from_dh_mfma(const uint16_t* __restrict__ p16, Norm nm,
+                                                          const double* __restrict__ tab, int nt, double dhmin,
+                                                          double dhmax, const double* __restrict__ rhoh, long N,
+                                                          double* __restrict__ oh, double* __restrict__ od,
+                                                          double* __restrict__ om, double* __restrict__ ok,
+                                                          double* __restrict__ ot, int* __restrict__ flag) {
+  __shared__ float sp[NP];
+  stage_params(p16, sp);
+  const int lane = threadIdx.x & 63;
+  const int ci = lane & 15, q = lane >> 4;
+  bf16x4 wa[16];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+      wa[4 * mt + kk] = *reinterpret_cast<const bf16x4*>(p16 + OW2 + (16 * mt + ci) * H1 + 16 * kk + 4 * q);
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = c < N;
+  double v = live ? rhoh[c] : dhmin;
+  if (live && v < dhmin) {
+    v = dhmin;
+    atomicAdd(flag, 1);
+  } else if (live && v > dhmax) {
+    v = dhmax;
+    atomicAdd(flag, 1);
+  }
+  const double* crh = tab + (long)CHS_PC_RH * nt;
+  const double* ch = tab + (long)CHS_PC_H * nt;
+  int i1, i2;
+  bisect(crh, nt, v, &i1, &i2);
+  const double w1 = (crh[i2] - v) / (crh[i2] - crh[i1]), w2 = 1.0 - w1;
+  double h = w1 * ch[i1] + w2 * ch[i2];
+  float mine[NO];
+  for (int pass = 0; pass < 2; ++pass) {
+    const float x = (float)((h - nm.xc) / nm.xs);
+#pragma unroll 1
+    for (int p = 0; p < 4; ++p) {
+      float out[NO];
+      mlp16_mfma(wa, sp, __shfl(x, 16 * p + ci), out);
+      if (q == p)
+#pragma unroll
+        for (int o = 0; o < NO; ++o) mine[o] = out[o];
+    }
+    if (pass == 0) h = v / ((double)mine[1] * nm.sd[1] + nm.mu[1]);
+  }
+  if (!live) return;
+  if (oh) oh[c] = h;
+  if (ot) ot[c] = (double)mine[0] * nm.sd[0] + nm.mu[0];
+  if (od) od[c] = (double)mine[1] * nm.sd[1] + nm.mu[1];
+  if (om) om[c] = (double)mine[2] * nm.sd[2] + nm.mu[2];
+  if (ok) ok[c] = (double)mine[3] * nm.sd[3] + nm.mu[3];
+}
 
 // Properties from rho*h with the surrogate (flow solver's refresh, mode
 // CHS_PROPS_MLP): clamp to [dhmin, dhmax] (counted), bisection on the
@@ -371,8 +479,12 @@ namespace chs {
 std::string mlp_refresh_dh(chs_mlp* m, const chs_props* p, const double* rhoh, long n, double* h, double* d,
                            double* mu, double* k, double* t, int* flag, hipStream_t s) {
   if (n <= 0) return "";
-  hipLaunchKernelGGL(k_mlp_from_dh, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->p16, m->nm, p->dev, p->n,
-                     p->dhmin, p->dhmax, rhoh, n, h, d, mu, k, t, flag);
+  if (m->matrix_cores)
+    hipLaunchKernelGGL(k_mlp_from_dh_mfma, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->p16, m->nm, p->dev,
+                       p->n, p->dhmin, p->dhmax, rhoh, n, h, d, mu, k, t, flag);
+  else
+    hipLaunchKernelGGL(k_mlp_from_dh, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->p16, m->nm, p->dev,
+                       p->n, p->dhmin, p->dhmax, rhoh, n, h, d, mu, k, t, flag);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("surrogate kernel: ") + hipGetErrorString(e);
   return "";
@@ -515,6 +627,12 @@ int chs_mlp_train(chs_mlp_t* m, int steps, int batch, float lr, float* loss) {
                        lr, b1, b2, eps, c1, c2);
   }
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) return fail(CHS_EHIP, "training");
+  return CHS_OK;
+}
+
+int chs_mlp_set_matrix_cores(chs_mlp_t* m, int on) {
+  if (!m) return fail(CHS_EINVAL, "null handle");
+  m->matrix_cores = on != 0;
   return CHS_OK;
 }
 

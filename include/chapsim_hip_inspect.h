@@ -76,6 +76,13 @@ int chs_tune_force(const char *prefix, int variant);
  * earlier run chose.  No counterpart in the reference. */
 int chs_tune_set(const char *key, int choice);
 
+/* Selects how the surrogate evaluates the network in the property refresh
+ * (chs_props_from_rhoh and the flow solver's refresh in mode
+ * CHS_PROPS_MLP): on the matrix cores (on != 0, the default: BF16
+ * activations, FP32 accumulation) or with vector FMAs on FP32 activations.
+ * For tests that compare the two; no counterpart in the reference. */
+int chs_mlp_set_matrix_cores(chs_mlp_t *m, int on);
+
 #ifdef __cplusplus
 }
 #endif
