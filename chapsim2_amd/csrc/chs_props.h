@@ -24,6 +24,8 @@ struct chs_props {
   size_t io_bytes = 0;
   hipStream_t stream = nullptr;
   chs_mlp* surrogate = nullptr;  // used by CHS_PROPS_MLP (not owned)
+  size_t lds_block = 65536;      // LDS bytes a workgroup may use on the table's device
+  int cus = 0;                   // compute units of that device
   ~chs_props() {
     if (dev) (void)hipFree(dev);
     if (flag) (void)hipFree(flag);

@@ -26,7 +26,8 @@ namespace {
 
 enum { CT = 0, CD, CM, CK, CB, CCP, CH, CRH };
 constexpr double kMinP = 1.0e-20;  // src/modules.f90:141
-constexpr int kPropsLds = 2048;    // rows staged in LDS (3 columns, 48 KB)
+constexpr int kPropsLds = 2048;    // rows of the one column k_props_from_h stages (x3, 48 KB)
+constexpr int kPropsBlock = 1024;  // threads of the persistent from-rho*h kernel
 
 int fail(int code, const std::string& msg) {
   chs::set_error(msg);
@@ -59,31 +60,17 @@ __device__ __forceinline__ void bisect_dev(const double* c, int n, double v, int
   *i2o = i2;
 }
 
-template <bool LDS>
-__global__ void __launch_bounds__(256) k_props_from_dh(const double* __restrict__ tab, int n, double dhmin,
-                                                       double dhmax, const double* __restrict__ rhoh_in, long N,
-                                                       double* __restrict__ oh, double* __restrict__ od,
-                                                       double* __restrict__ om, double* __restrict__ ok,
-                                                       double* __restrict__ ot, double* __restrict__ ob,
-                                                       double* __restrict__ ocp, int* __restrict__ flag) {
-  extern __shared__ double sh[];
-  const double* crh = tab + (long)CRH * n;
-  const double* ch = tab + (long)CH * n;
-  const double* cd = tab + (long)CD * n;
-  if (LDS) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      sh[i] = crh[i];
-      sh[n + i] = ch[i];
-      sh[2 * n + i] = cd[i];
-    }
-    __syncthreads();
-    crh = sh;
-    ch = sh + n;
-    cd = sh + 2 * n;
-  }
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
-  double v = rhoh_in[c];
+// One cell of ftp_refresh_thermal_properties_from_DH: clamp rho*h to the
+// table, h from the rho*h column, d from h, h = rho*h / d, then every
+// property from that h (src/thermo_info.f90 ftp_refresh_thermal_properties_
+// from_DH).  crh, ch and cd may point into LDS; the other columns are read
+// from tab.
+__device__ __forceinline__ void props_dh_cell(const double* crh, const double* ch, const double* cd,
+                                              const double* __restrict__ tab, int n, double dhmin, double dhmax,
+                                              double v, long c, double* __restrict__ oh, double* __restrict__ od,
+                                              double* __restrict__ om, double* __restrict__ ok,
+                                              double* __restrict__ ot, double* __restrict__ ob,
+                                              double* __restrict__ ocp, int* __restrict__ flag) {
   if (v < dhmin) {
     v = dhmin;
     atomicAdd(flag, 1);
@@ -113,6 +100,45 @@ __global__ void __launch_bounds__(256) k_props_from_dh(const double* __restrict_
   if (ot) ot[c] = interp(CT);
   if (ob) ob[c] = interp(CB);
   if (ocp) ocp[c] = interp(CCP);
+}
+
+// Table in global memory (any size): one thread per cell.
+__global__ void __launch_bounds__(256) k_props_from_dh(const double* __restrict__ tab, int n, double dhmin,
+                                                       double dhmax, const double* __restrict__ rhoh_in, long N,
+                                                       double* __restrict__ oh, double* __restrict__ od,
+                                                       double* __restrict__ om, double* __restrict__ ok,
+                                                       double* __restrict__ ot, double* __restrict__ ob,
+                                                       double* __restrict__ ocp, int* __restrict__ flag) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  props_dh_cell(tab + (long)CRH * n, tab + (long)CH * n, tab + (long)CD * n, tab, n, dhmin, dhmax, rhoh_in[c], c,
+                oh, od, om, ok, ot, ob, ocp, flag);
+}
+
+// The bisection columns (rho*h and h, and d when NC is 3: NC n doubles)
+// staged in LDS once per workgroup; a grid of about two workgroups per CU
+// then walks the cells with a grid stride, so the table crosses L2 once per
+// workgroup instead of once per few hundred cells.  Every thread reaches the
+// barrier: the staging loop runs before any thread leaves.
+template <int NC>
+__global__ void __launch_bounds__(kPropsBlock) k_props_from_dh_lds(
+    const double* __restrict__ tab, int n, double dhmin, double dhmax, const double* __restrict__ rhoh_in, long N,
+    double* __restrict__ oh, double* __restrict__ od, double* __restrict__ om, double* __restrict__ ok,
+    double* __restrict__ ot, double* __restrict__ ob, double* __restrict__ ocp, int* __restrict__ flag) {
+  extern __shared__ double sh[];
+  const double* crh = tab + (long)CRH * n;
+  const double* ch = tab + (long)CH * n;
+  const double* cd = tab + (long)CD * n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    sh[i] = crh[i];
+    sh[n + i] = ch[i];
+    if (NC == 3) sh[2 * n + i] = cd[i];
+  }
+  __syncthreads();
+  const double* sd = NC == 3 ? sh + 2 * n : cd;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += stride)
+    props_dh_cell(sh, sh + n, sd, tab, n, dhmin, dhmax, rhoh_in[c], c, oh, od, om, ok, ot, ob, ocp, flag);
 }
 
 // ftp_refresh_thermal_properties_from_H per cell (no clamping, as the
@@ -148,6 +174,32 @@ __global__ void __launch_bounds__(256) k_props_from_h(const double* __restrict__
 
 #pragma clang fp contract(on)
 
+// Launches the from-rho*h kernel for n cells: the persistent LDS form with as
+// many bisection columns as fit a workgroup's LDS (three or two), else the
+// global form.
+// o holds the seven outputs (h, d, m, k, t, b, cp), each possibly null.
+hipError_t launch_from_dh(const chs_props* p, const double* rhoh, long n, double* const o[7], int* flag,
+                          hipStream_t s) {
+  const size_t col = sizeof(double) * (size_t)p->n;
+  const int nc = 3 * col <= p->lds_block ? 3 : 2 * col <= p->lds_block ? 2 : 0;
+  if (nc) {
+    const long want = (n + kPropsBlock - 1) / kPropsBlock;
+    const long cap = 2L * (p->cus > 0 ? p->cus : 256);
+    const unsigned grid = (unsigned)(want < cap ? want : cap);
+    if (nc == 3)
+      hipLaunchKernelGGL(k_props_from_dh_lds<3>, dim3(grid), dim3(kPropsBlock), 3 * col, s, p->dev, p->n, p->dhmin,
+                         p->dhmax, rhoh, n, o[0], o[1], o[2], o[3], o[4], o[5], o[6], flag);
+    else
+      hipLaunchKernelGGL(k_props_from_dh_lds<2>, dim3(grid), dim3(kPropsBlock), 2 * col, s, p->dev, p->n, p->dhmin,
+                         p->dhmax, rhoh, n, o[0], o[1], o[2], o[3], o[4], o[5], o[6], flag);
+  } else {
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_props_from_dh, dim3(grid), dim3(256), 0, s, p->dev, p->n, p->dhmin, p->dhmax, rhoh, n,
+                       o[0], o[1], o[2], o[3], o[4], o[5], o[6], flag);
+  }
+  return hipGetLastError();
+}
+
 }  // namespace
 
 namespace chs {
@@ -159,14 +211,8 @@ std::string props_refresh_dh(const chs_props* p, int mode, const double* rhoh, l
     return mlp_refresh_dh(p->surrogate, p, rhoh, n, h, d, m, k, t, flag, s);
   }
   if (mode != CHS_PROPS_TABLE) return "properties: unknown mode";
-  const unsigned grid = (unsigned)((n + 255) / 256);
-  if (p->n <= kPropsLds)
-    hipLaunchKernelGGL(k_props_from_dh<true>, dim3(grid), dim3(256), sizeof(double) * 3 * (size_t)p->n, s, p->dev,
-                       p->n, p->dhmin, p->dhmax, rhoh, n, h, d, m, k, t, (double*)nullptr, (double*)nullptr, flag);
-  else
-    hipLaunchKernelGGL(k_props_from_dh<false>, dim3(grid), dim3(256), 0, s, p->dev, p->n, p->dhmin, p->dhmax, rhoh,
-                       n, h, d, m, k, t, (double*)nullptr, (double*)nullptr, flag);
-  const hipError_t e = hipGetLastError();
+  double* const o[7] = {h, d, m, k, t, nullptr, nullptr};
+  const hipError_t e = launch_from_dh(p, rhoh, n, o, flag, s);
   if (e != hipSuccess) return std::string("property kernel: ") + hipGetErrorString(e);
   return "";
 }
@@ -245,6 +291,12 @@ int chs_props_create(const double* table, int nrows, double ref_t0, chs_props_t*
     delete p;
     return fail(CHS_EHIP, "no HIP device");
   }
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+      p->lds_block = (size_t)v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) p->cus = v;
+  }
   std::vector<double> flat(8 * (size_t)nrows);
   for (int c = 0; c < 8; ++c) std::memcpy(flat.data() + (size_t)c * nrows, p->col[c].data(), sizeof(double) * nrows);
   if (hipStreamCreate(&p->stream) != hipSuccess || hipMalloc(&p->dev, sizeof(double) * flat.size()) != hipSuccess ||
@@ -308,15 +360,9 @@ int chs_props_from_rhoh(chs_props_t* p, const double* rhoh, long n, double* h, d
     const std::string e = chs::mlp_refresh_dh(p->surrogate, p, din, n, dout[0], dout[1], dout[2], dout[3], dout[4],
                                               p->flag, p->stream);
     if (!e.empty()) return fail(CHS_EHIP, e);
-  } else if (p->n <= kPropsLds) {
-    const size_t shm = sizeof(double) * 3 * (size_t)p->n;
-    hipLaunchKernelGGL(k_props_from_dh<true>, dim3(grid), dim3(256), shm, p->stream, p->dev, p->n, p->dhmin,
-                       p->dhmax, din, n, dout[0], dout[1], dout[2], dout[3], dout[4], dout[5], dout[6], p->flag);
-  } else {
-    hipLaunchKernelGGL(k_props_from_dh<false>, dim3(grid), dim3(256), 0, p->stream, p->dev, p->n, p->dhmin, p->dhmax,
-                       din, n, dout[0], dout[1], dout[2], dout[3], dout[4], dout[5], dout[6], p->flag);
+  } else if (launch_from_dh(p, din, n, dout, p->flag, p->stream) != hipSuccess) {
+    return fail(CHS_EHIP, "property kernel launch");
   }
-  if (hipGetLastError() != hipSuccess) return fail(CHS_EHIP, "property kernel launch");
   if (mem == CHS_MEM_HOST)
     for (int q = 0; q < 7; ++q)
       if (outs[q] && hipMemcpyAsync(outs[q], dout[q], sizeof(double) * n, hipMemcpyDeviceToHost, p->stream) !=

@@ -66,3 +66,34 @@ def test_props_large_batch_device_memory():
             assert np.array_equal(o.cpu().numpy()[sub], got[("h", "d", "m", "k", "t", "b", "cp")[j]])
     finally:
         pt.close()
+
+
+@pytest.mark.parametrize("rows", [700, 4096, 8000, 12000])
+def test_props_table_sizes_across_lds_forms(po, rows):
+    """The from-rho*h kernel stages three bisection columns in LDS, or two,
+    or none, by table size against the workgroup LDS limit (160 KiB on
+    gfx950: 4096 rows stage three columns, 8000 two, 12000 none); every form
+    gives the oracle's bits, on more cells than one persistent grid covers."""
+    import torch
+
+    from chapsim2_amd.initial import HEATED_H_RANGE, heated_channel_table
+    from chapsim2_amd.props import NAMES, PropertyTable
+
+    lds = getattr(torch.cuda.get_device_properties(0), "shared_memory_per_block", None)
+    print(f"rows {rows}: {8 * rows} bytes per column, workgroup LDS {lds}")
+    tab = heated_channel_table(rows, *HEATED_H_RANGE)
+    op = po.Props(tab, -1.0)
+    pt = PropertyTable(tab, -1.0)
+    try:
+        lo, hi, _ = pt.info()
+        assert lo == op.dhmin and hi == op.dhmax
+        rng = np.random.default_rng(rows)
+        x = rng.uniform(lo, hi, 600_000)
+        x[:4] = [lo, hi, op.column("rhoh")[rows // 3], op.column("rhoh")[rows - 2]]
+        got = pt.from_rhoh(x)
+        res, flags = op.from_dh(x)
+        assert not flags.any()
+        for j, n in enumerate(NAMES):
+            assert np.array_equal(got[n], res[:, j]), (rows, n, np.abs(got[n] - res[:, j]).max())
+    finally:
+        pt.close()
