@@ -46,6 +46,10 @@ struct DevPlan {
   // (win_ok false when a recipe reaches outside that window)
   T* swin = nullptr;
   bool win_ok = false;
+  // every chunk's coefficient rows are those of chunk 0 (equal chunks of a
+  // periodic line), and whether any row carries an output metric other than 1
+  bool co_uni = false;
+  bool has_met = false;
   ~DevPlan();
 };
 

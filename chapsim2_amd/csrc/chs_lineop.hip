@@ -43,6 +43,7 @@
 // forward sweep so the correction is applied in the same backward pass.
 #include "chs_lineop_dev.h"
 #include "chs_tune.h"
+#include <cstring>
 
 #include <cmath>
 #include <cstdint>
@@ -158,6 +159,13 @@ std::string upload_plan(const LinePlan& p, DevPlan<T>* dp) {
         if (!p.metric.empty()) c[CO_MET] = p.metric[r];
       }
     }
+    const size_t per = (size_t)CH * kCo;
+    dp->co_uni = true;
+    for (int q = 1; q < p.part_P && dp->co_uni; ++q)
+      dp->co_uni = std::memcmp(&co[0], &co[(size_t)q * per], sizeof(double) * per) == 0;
+    dp->has_met = false;
+    for (size_t k = CO_MET; k < co.size(); k += kCo)
+      if (co[k] != 1.0) dp->has_met = true;
     if (!(e = up(co, &dp->co)).empty()) return e;
   }
   return "";
@@ -508,6 +516,30 @@ static size_t max_lds_per_block() {
   return cached[dev];
 }
 
+// Compute units and LDS bytes per CU of the current device, queried once
+// per device (persistent launches size their grids by them).
+static void device_cus_lds(int* cus, size_t* lds_cu) {
+  static int ccus[64] = {0};
+  static size_t clds[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    *cus = 256;
+    *lds_cu = 65536;
+    return;
+  }
+  if (!ccus[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    ccus[dev] = v;
+    v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || v <= 0)
+      v = (int)max_lds_per_block();
+    clds[dev] = (size_t)v;
+  }
+  *cus = ccus[dev];
+  *lds_cu = clds[dev];
+}
+
 template <typename T, int DIR, int IN, int CH = kChunk, int NLW = 64>
 static void dispatch_yz(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
   switch (op) {
@@ -629,6 +661,8 @@ static void part_args(const DevPlan<T>& D, KArgs<T>* a) {
   a->swin = D.swin;
   a->wrap = (p.ibc[0] == BC_PERIODIC);
   a->co = D.co;
+  a->co_uni = D.co_uni ? 1 : 0;
+  a->met = D.has_met ? 1 : 0;
 }
 
 template <typename T>
@@ -670,11 +704,16 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
       //  3: lines of kChunkLong-row chunks only: the lane-per-line kernel
       //  4: lines of kChunk-row chunks only: 16 whole lines in LDS, four
       //     chunks per wave (nothing else in LDS), four workgroups per CU
+      //  5: lines of kChunk-row chunks read 16 bytes at a time, one or two
+      //     input arrays: persistent workgroups over 32-line tiles, the next
+      //     tile's input loaded while the current one is solved and stored
       const size_t lds = max_lds_per_block();
       const int lsu = nmax | 1;  // odd row stride
       const size_t shm64 = sizeof(T) * ((size_t)64 * lsu + (size_t)p.part_P * CHp * kCo + P2);
-      const size_t shm32 = sizeof(T) * ((size_t)32 * lsu + P2);
-      const size_t shm16 = sizeof(T) * (size_t)16 * lsu;
+      // (+ one chunk's rows: the 32- and 16-line tiles keep the rows of equal
+      // chunks in LDS)
+      const size_t shm32 = sizeof(T) * ((size_t)32 * lsu + P2 + (size_t)CHp * kCo);
+      const size_t shm16 = sizeof(T) * ((size_t)16 * lsu + (size_t)CHp * kCo);
       int ls = lds_pos_host(nmax) + 1;
       ls += ((8 - ls) % 32 + 32) % 32;  // row stride = 8 (mod 32) doubles: neighbouring lines in other banks
       // lines per workgroup: at most 256 threads and 48 KiB of LDS (long
@@ -693,6 +732,23 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
           ka.ls = lsu;
           dim3 g((unsigned)((ka.nlines + nlw - 1) / nlw)), b((unsigned)(nlw * p.part_P));
           if (!launch_x_tile<T>(p.op, CHp, nlw, g, b, shm, s, ka)) return false;
+        } else if (v == 5) {
+          // persistent pipelined 32-line tiles; coefficient rows and inverse in LDS
+          if (CHp != kChunk) return false;
+          const size_t shm = sizeof(T) * ((size_t)32 * lsu + (size_t)p.part_P * CHp * kCo + P2);
+          if (shm > lds) return false;
+          ka.ls = lsu;
+          int cus = 0;
+          size_t lds_cu = 0;
+          device_cus_lds(&cus, &lds_cu);
+          const long per_cu = shm <= lds_cu ? (long)(lds_cu / shm) : 1;
+          const long ntile = (ka.nlines + 31) / 32;
+          long grid = (long)cus * per_cu;
+          if (grid > ntile) grid = ntile;
+          const long rounds = (ntile + grid - 1) / grid;
+          grid = (ntile + rounds - 1) / rounds;  // the same number of rounds on fewer workgroups
+          if (!launch_x_pipe<T>(p.op, 32, dim3((unsigned)grid), dim3((unsigned)(32 * p.part_P)), shm, s, ka))
+            return false;
         } else if (v == 2) {
           if (nl < 1) return false;
           ka.ls = ls;
@@ -706,7 +762,7 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
         }
         return hipGetLastError() == hipSuccess;
       };
-      const int nvar = 5;
+      const int nvar = 6;
       int v = -1;
       if (scratch) {
         KArgs<T> ts = a;

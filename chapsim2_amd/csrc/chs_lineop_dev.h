@@ -43,6 +43,8 @@ struct KArgs {
   const T* swin;  // 4 x (CH + 4) boundary-row window weights (CH = the plan's chunk rows)
   int wrap;       // periodic line: windows wrap around instead of clamping
   const T* co;    // P x CH coefficient rows of kCo entries (see upload_plan)
+  int co_uni;     // every chunk's rows equal chunk 0's
+  int met;        // some row's output metric differs from 1
 };
 
 // bulk offsets (p1,p2,p3,p4) per operator kind, relative to the output row
@@ -256,7 +258,7 @@ template <typename T, int CH, class EX>
 __device__ __forceinline__ void chunk_couple(const KArgs<T>& P, int q, const T* rinv, const T* c, EX&& ex,
                                              T (&y)[CH]) {
   const int Pn = P.P, M = 2 * Pn;
-  const int qm = (q + Pn - 1) % Pn, qp = (q + 1) % Pn;
+  const int qm = q == 0 ? Pn - 1 : q - 1, qp = q == Pn - 1 ? 0 : q + 1;
   const T* rL = rinv + (2 * qm + 1) * M;
   const T* rF = rinv + (2 * qp) * M;
   T uL = T(0), uF = T(0);
@@ -274,7 +276,7 @@ __device__ __forceinline__ void chunk_couple(const KArgs<T>& P, int q, const T* 
 // Row i of a chunk (line row r): metric scaling and Neumann overrides.
 template <typename T>
 __device__ __forceinline__ T chunk_post(const KArgs<T>& P, const T* c, int i, int r, T val, const T (&fb4)[4]) {
-  val = val * c[i * kCo + CO_MET];
+  if (P.met) val = val * c[i * kCo + CO_MET];
   if (r == 0 && P.neu_lo) val = fb4[0];
   if (r == P.n_out - 1 && P.neu_hi) val = fb4[1];
   return val;
@@ -302,6 +304,12 @@ inline int input_kind(const KArgs<T>& a) { return a.D ? 2 : ((a.B || a.Cm) ? 1 :
 // there is no such kernel: 64 FP64 lines of long chunks):
 template <typename T>
 bool launch_x_tile(int op, int ch, int nlw, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a);
+// Persistent, software-pipelined x tiles (k_line_xp): 32 whole lines per
+// tile, a grid of g workgroups looping over the tiles, one input array or
+// two (A and one of B, Cm), 16-byte accesses, chunks of kChunk rows (false
+// when the operator is not of that form):
+template <typename T>
+bool launch_x_pipe(int op, int nlw, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a);
 // Two operators of one plan geometry on one input (k_line_xu2; pair 0 =
 // MIDP_P2C + D1_P2C, 1 = MIDP_C2P + D1_C2P, plans a and a2), whole lines in
 // LDS as above (nlw 64 or 32; chunks of kChunk rows only):

@@ -214,6 +214,20 @@ __device__ __forceinline__ void store_tile(const KArgs<T>& P, const T* tile, lon
   }
 }
 
+// The (CH + 4)-row window of a chunk starting at line row s0 - 2 from a
+// staged line: interior chunks read it straight, the edge chunks through
+// window_index (wrapped on periodic lines, clamped otherwise).
+template <typename T, int CH>
+__device__ __forceinline__ void read_window(const T* row, int s0, int n_in, int wrap, T (&f)[CH + 4]) {
+  if (s0 >= 2 && s0 + CH + 1 < n_in) {
+#pragma unroll
+    for (int t = 0; t < CH + 4; ++t) f[t] = row[s0 - 2 + t];
+  } else {
+#pragma unroll
+    for (int t = 0; t < CH + 4; ++t) f[t] = row[window_index(s0 - 2 + t, n_in, wrap)];
+  }
+}
+
 // The rest of a staged x chunk after its RHS rows y[]: chunk-local solve,
 // exchange of first/last values through the start of the tile (NLW x 2P,
 // once every window has been read), spike correction, then the chunk's rows
@@ -298,22 +312,131 @@ __global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_
     const int idx = tid + u * nthr;
     if (idx < nst) sco[idx] = cv[u];
   }
+  // equal chunks (co_uni): the one set of chunk rows in LDS after the
+  // inverse, read by every chunk instead of its own rows through the cache
+  T* scu = sco + nst;
+  if (!kCoLds && P.co_uni)
+    for (int idx = tid; idx < CH * kCo; idx += nthr) scu[idx] = P.co[idx];
   __syncthreads();
   const T* c = (kCoLds ? sco : P.co) + q * CH * kCo;
   const T* rinv = kRinvLds ? sco + nco : P.rinv;
   const long L = L0 + lane;
   const bool live = L < P.nlines;
-  T* row = tile + lane * LS;
+  const T* row = tile + lane * LS;
   const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
   T f[CH + 4];
-#pragma unroll
-  for (int t = 0; t < CH + 4; ++t) f[t] = row[window_index(s0 - 2 + t, n_in, P.wrap)];
+  read_window<T, CH>(row, s0, n_in, P.wrap, f);
   T fb4[4] = {T(0), T(0), T(0), T(0)};
   if (P.fbc && live && (q == 0 || q == Pn - 1))
     for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
   T y[CH];
   chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
-  xu_finish<T, CH, NLW, IN == 0 && sizeof(T) == 8>(P, c, rinv, tile, lane, q, s0, len, L0, lmax, y, fb4);
+  if (!kCoLds && P.co_uni)
+    xu_finish<T, CH, NLW, IN == 0 && sizeof(T) == 8>(P, scu, rinv, tile, lane, q, s0, len, L0, lmax, y, fb4);
+  else
+    xu_finish<T, CH, NLW, IN == 0 && sizeof(T) == 8>(P, c, rinv, tile, lane, q, s0, len, L0, lmax, y, fb4);
+}
+
+// Persistent, software-pipelined form of k_line_xu for lines of 16-row
+// chunks read and written 16 bytes per access.  A workgroup of NLW x P
+// threads loops over tiles of NLW lines (tile = blockIdx.x, + gridDim.x,
+// ...); while one tile is solved and stored, the raw input of its next tile
+// is already in flight into registers (JM groups of 16 bytes per thread and
+// input array), so the loads overlap the arithmetic and the stores instead of
+// forming a phase of their own.  IN = 0 (A) or 1 (A and one of B, Cm).  The
+// chunk rows and the reduced-system inverse are copied into LDS after the
+// tile once, for the workgroup's life.  The global loads a tile's solve
+// waits for (the boundary values fbc) are issued before the next tile's
+// prefetch, so waiting for them does not wait for the prefetch.
+template <typename T, int OPK, int IN, int NLW>
+__global__ void __launch_bounds__(NLW * kMaxChunks) k_line_xp(KArgs<T> P) {
+  constexpr int CH = kChunk;
+  constexpr int G = 16 / sizeof(T);
+  constexpr int JM = CH / G;  // groups per thread and array: ceil(n_in / (G P)) <= JM (host-checked)
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* tile = reinterpret_cast<T*>(smem_raw);
+  const int LS = P.ls, Pn = P.P;
+  T* sco = tile + (long)NLW * LS;
+  const int nco = Pn * CH * kCo;
+  const int nst = nco + (P.rinv ? 4 * Pn * Pn : 0);
+  const int tid = threadIdx.x, nthr = NLW * Pn;
+  for (int idx = tid; idx < nst; idx += nthr) sco[idx] = idx < nco ? P.co[idx] : P.rinv[idx - nco];
+  const int lane = tid & (NLW - 1);
+  const int q = tid / NLW;
+  const int n_in = P.n_in, n_out = P.n_out, ng = n_in / G;
+  const long ntile = (P.nlines + NLW - 1) / NLW;
+  const T* S = IN ? (P.B ? P.B : P.Cm) : nullptr;
+  // this thread's groups e = tid + j nthr of a tile: line lj, first row kj
+  int lj[JM], kj[JM];
+  {
+    const int dl = nthr / ng, dk = nthr - dl * ng;
+    int l = tid / ng, k = tid - l * ng;
+#pragma unroll
+    for (int j = 0; j < JM; ++j) {
+      lj[j] = l;
+      kj[j] = k * G;
+      k += dk;
+      l += dl;
+      if (k >= ng) { k -= ng; ++l; }
+    }
+  }
+  Grp<T, G> ra[JM], rb[JM];
+  // every load of a tile is issued (clamped to the tile's last line, so each
+  // address is valid); groups past the tile are not written
+  auto issue = [&](long t) {
+    const long L0 = t * NLW;
+    const int lmax = (int)((P.nlines - 1 - L0) < NLW - 1 ? (P.nlines - 1 - L0) : NLW - 1);
+    const T* Ab = P.A + L0 * P.e1;
+    const T* Sb = IN ? S + L0 * P.e1 : nullptr;
+#pragma unroll
+    for (int j = 0; j < JM; ++j) {
+      const int lc = lj[j] < lmax ? lj[j] : lmax;
+      const unsigned o = (unsigned)((lc * P.e1 + kj[j]) * (int)sizeof(T));
+      ra[j] = ld_grp<T, G>(Ab, o);
+      if (IN) rb[j] = ld_grp<T, G>(Sb, o);
+    }
+  };
+  long t = blockIdx.x;
+  if (t < ntile) issue(t);
+  __syncthreads();  // coefficient rows and inverse in LDS
+  const T* c = sco + q * CH * kCo;
+  const T* rinv = sco + nco;
+  const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
+  T* row = tile + lane * LS;
+  for (; t < ntile; t += gridDim.x) {
+    const long L0 = t * NLW;
+    const int lmax = (int)((P.nlines - 1 - L0) < NLW - 1 ? (P.nlines - 1 - L0) : NLW - 1);
+    // the tile, input transform applied (sA (A + cB B) or sA A Cm)
+#pragma unroll
+    for (int j = 0; j < JM; ++j) {
+      if (lj[j] < NLW) {
+#pragma unroll
+        for (int cc = 0; cc < G; ++cc) {
+          T v = ra[j].v[cc];
+          if (IN && P.B) v = v + P.cB * rb[j].v[cc];
+          v = P.sA * v;
+          if (IN && !P.B) v = v * rb[j].v[cc];
+          tile[lj[j] * LS + kj[j] + cc] = v;
+        }
+      }
+    }
+    const long L = L0 + lane;
+    const bool live = L < P.nlines;
+    T fb4[4] = {T(0), T(0), T(0), T(0)};
+    if (P.fbc && live && (q == 0 || q == Pn - 1))
+      for (int m = 0; m < 4; ++m) fb4[m] = P.fbc[4 * L + m];
+    __syncthreads();
+    // the next tile's input (the current one again on the last pass: every
+    // wave issues the same number of loads on every pass)
+    const long tn = t + gridDim.x < ntile ? t + gridDim.x : t;
+    issue(tn);
+    T f[CH + 4];
+    read_window<T, CH>(row, s0, n_in, P.wrap, f);
+    T y[CH];
+    chunk_rhs<T, OPK>(P, q, len, f, fb4, y);
+    xu_finish<T, CH, NLW, IN == 0 && sizeof(T) == 8>(P, c, rinv, tile, lane, q, s0, len, L0, lmax, y, fb4);
+    __syncthreads();  // the stores have read the tile
+  }
 }
 
 // Two operators of one line plan geometry on one input (PAIR 0 = MIDP_P2C +
@@ -374,8 +497,7 @@ __global__ void __launch_bounds__(NLW * kMaxChunks) __attribute__((amdgpu_waves_
   const T* row = tile + lane * LS;
   const int s0 = (q * n_out) / Pn, len = ((q + 1) * n_out) / Pn - s0;
   T f[CH + 4];
-#pragma unroll
-  for (int t = 0; t < CH + 4; ++t) f[t] = row[window_index(s0 - 2 + t, n_in, P.wrap)];
+  read_window<T, CH>(row, s0, n_in, P.wrap, f);
   T fbP[4] = {T(0), T(0), T(0), T(0)}, fbQ[4] = {T(0), T(0), T(0), T(0)};
   if (live && (q == 0 || q == Pn - 1)) {
     if (P.fbc)
@@ -434,6 +556,28 @@ bool launch_x_tile(int op, int ch, int nlw, dim3 g, dim3 b, size_t shm, hipStrea
   return false;
 }
 
+template <typename T, int IN, int NLW>
+static void pipe_ops(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
+  switch (op) {
+    case MIDP_C2P: hipLaunchKernelGGL((k_line_xp<T, MIDP_C2P, IN, NLW>), g, b, shm, s, a); break;
+    case MIDP_P2C: hipLaunchKernelGGL((k_line_xp<T, MIDP_P2C, IN, NLW>), g, b, shm, s, a); break;
+    case D1_C2C: hipLaunchKernelGGL((k_line_xp<T, D1_C2C, IN, NLW>), g, b, shm, s, a); break;
+    case D1_P2P: hipLaunchKernelGGL((k_line_xp<T, D1_P2P, IN, NLW>), g, b, shm, s, a); break;
+    case D1_C2P: hipLaunchKernelGGL((k_line_xp<T, D1_C2P, IN, NLW>), g, b, shm, s, a); break;
+    case D1_P2C: hipLaunchKernelGGL((k_line_xp<T, D1_P2C, IN, NLW>), g, b, shm, s, a); break;
+  }
+}
+
+template <typename T>
+bool launch_x_pipe(int op, int nlw, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
+  constexpr int G = 16 / sizeof(T);
+  if (nlw != 32 || !a.vec || a.D || (a.B && a.Cm)) return false;
+  if (a.n_in / G > (kChunk / G) * a.P) return false;  // more groups per thread than the kernel holds
+  if (a.B || a.Cm) pipe_ops<T, 1, 32>(op, g, b, shm, s, a);
+  else pipe_ops<T, 0, 32>(op, g, b, shm, s, a);
+  return true;
+}
+
 template <typename T, int CH>
 static void skew_ops(int op, dim3 g, dim3 b, size_t shm, hipStream_t s, const KArgs<T>& a) {
   switch (op) {
@@ -475,6 +619,8 @@ template bool launch_x_tile2<double>(int, int, int, dim3, dim3, size_t, hipStrea
                                      const KArgs<double>&);
 template bool launch_x_tile2<float>(int, int, int, dim3, dim3, size_t, hipStream_t, const KArgs<float>&,
                                     const KArgs<float>&);
+template bool launch_x_pipe<double>(int, int, dim3, dim3, size_t, hipStream_t, const KArgs<double>&);
+template bool launch_x_pipe<float>(int, int, dim3, dim3, size_t, hipStream_t, const KArgs<float>&);
 template void launch_x_skew<double>(int, int, dim3, dim3, size_t, hipStream_t, const KArgs<double>&);
 template void launch_x_skew<float>(int, int, dim3, dim3, size_t, hipStream_t, const KArgs<float>&);
 

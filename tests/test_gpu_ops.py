@@ -264,16 +264,19 @@ def test_poisson_fp32_vs_oracle(chan, po):
     assert 1e-9 < err < 1e-5, err
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 def test_x_launch_shapes_vs_oracle(po, variant):
     """Every x-line launch shape, forced through chs_tune_force (0: 64 whole
     lines in LDS, 1: 32 whole lines in LDS with two chunks per wave, 2: skewed
     rows, 3: lane-per-line, long lines only, 4: 16 whole lines in LDS with
-    four chunks per wave, 16-row chunks only; a shape that does not apply
-    falls back to the next that does): all operators on lines of 250 rows
-    (16-row chunks), 300 and 512 rows (32-row chunks), CP6 and the explicit
-    CD4, then the momentum right-hand side (merged operators reading two to
-    four arrays along x), against the oracle."""
+    four chunks per wave, 16-row chunks only, 5: persistent workgroups over
+    32-line tiles with the next tile's input prefetched, one or two input
+    arrays read 16 bytes at a time; a shape that does not apply falls back to
+    the next that does): all operators on lines of 250 rows (16-row chunks),
+    300 and 512 rows (32-row chunks), CP6 and the explicit CD4, then the
+    momentum right-hand side (merged operators reading two to four arrays
+    along x), against the oracle.  Shape 5 also runs 256-row lines on 9 000
+    lines (282 tiles: more tiles than workgroups, a last tile of 8 lines)."""
     from chapsim2_amd import _lib
     from chapsim2_amd import operations as ops
     from chapsim2_amd.domain import Domain
@@ -281,7 +284,10 @@ def test_x_launch_shapes_vs_oracle(po, variant):
     lib = _lib.load()
     _lib.check(lib.chs_tune_force(b"line_x/", variant))
     try:
-        for nc in ((250, 8, 12), (300, 8, 12), (512, 8, 8)):
+        grids = ((250, 8, 12), (300, 8, 12), (512, 8, 8))
+        if variant == 5:
+            grids = grids + ((256, 90, 100),)
+        for nc in grids:
             dm = Domain(nc, LX, -1.0, 1.0, LZ, istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1)
             odm = po.Dom(list(nc), [1, 0, 1], LX, -1.0, 1.0, LZ, 2, 0.075)
             rng = np.random.default_rng(41 + variant)
@@ -294,6 +300,23 @@ def test_x_launch_shapes_vs_oracle(po, variant):
                         ops.op3d(dm, 0, op, fi, out, acc, [1, 1], None)
                         err = np.abs(out - ref).max() / np.abs(ref).max()
                         assert err < RTOL, (variant, nc, op, acc, err)
+                if variant == 5 and nc[0] == 256:
+                    # the FP32 kernels of the same shape (four 16-byte groups per thread)
+                    import ctypes as C
+
+                    import torch
+
+                    for op in range(6):
+                        fi = np.asfortranarray(rng.standard_normal(nc))
+                        ref = odm.op3d(0, op, 4, [1, 1], fi, list(nc), None)
+                        ti = torch.from_numpy(np.ascontiguousarray(fi.T)).float().cuda()
+                        to = torch.zeros(nc[::-1], dtype=torch.float32, device="cuda")
+                        _lib.check(dm.lib.chs_op3d_f32(dm.h, 0, op, 4, _lib.ivec([1, 1]), C.c_void_p(ti.data_ptr()),
+                                                       _lib.ivec(list(nc)), C.c_void_p(to.data_ptr()),
+                                                       _lib.ivec(list(nc)), None))
+                        got = to.double().cpu().numpy().T
+                        err = np.abs(got - ref).max() / np.abs(ref).max()
+                        assert 1e-9 < err < 2e-6, (variant, "f32", op, err)
             finally:
                 dm.close()
         from chapsim2_amd.initial import channel_fields
