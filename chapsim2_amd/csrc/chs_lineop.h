@@ -50,6 +50,9 @@ struct DevPlan {
   // periodic line), and whether any row carries an output metric other than 1
   bool co_uni = false;
   bool has_met = false;
+  // the four boundary rows are the bulk stencil on the wrapped window
+  // (periodic lines), so the kernels need not form them apart
+  bool ends_bulk = false;
   ~DevPlan();
 };
 

@@ -129,6 +129,29 @@ std::string upload_plan(const LinePlan& p, DevPlan<T>* dp) {
       }
     }
     dp->win_ok = ok;
+    // periodic lines without boundary values: are the four boundary rows the
+    // bulk stencil applied to the wrapped line (up to rounding of the probed
+    // weights)?
+    dp->ends_bulk = false;
+    if (ok && wrap && !p.has_fbc && p.n_in == p.n_out) {
+      bool same = true;
+      const int rows_out[4] = {0, 1, p.n_out - 2, p.n_out - 1};
+      for (int q = 0; q < 4 && same; ++q) {
+        std::vector<double> wr(p.n_in, 0.0), wb(p.n_in, 0.0);
+        for (int t = 0; t < p.rows[q].nterm; ++t) wr[p.rows[q].idx[t]] += p.rows[q].w[t];
+        const int r = rows_out[q];
+        auto wi = [&](int k) { return ((k % p.n_in) + p.n_in) % p.n_in; };
+        wb[wi(r + p.p1)] += p.c1;
+        wb[wi(r + p.p2)] += p.sgn * p.c1;
+        wb[wi(r + p.p3)] += p.c2;
+        wb[wi(r + p.p4)] += p.sgn * p.c2;
+        double mag = 0.0;
+        for (double v : wb) mag = std::max(mag, std::fabs(v));
+        for (int k = 0; k < p.n_in && same; ++k)
+          if (std::fabs(wr[k] - wb[k]) > 1e-14 * mag) same = false;
+      }
+      dp->ends_bulk = same;
+    }
     if (ok && !(e = up(sw, &dp->swin)).empty()) return e;
   }
   // chunk coefficient rows: CH = part_ch rows per chunk, kCo entries per row,
@@ -663,6 +686,7 @@ static void part_args(const DevPlan<T>& D, KArgs<T>* a) {
   a->co = D.co;
   a->co_uni = D.co_uni ? 1 : 0;
   a->met = D.has_met ? 1 : 0;
+  a->ends = D.ends_bulk ? 0 : 1;
 }
 
 template <typename T>

@@ -45,6 +45,7 @@ struct KArgs {
   const T* co;    // P x CH coefficient rows of kCo entries (see upload_plan)
   int co_uni;     // every chunk's rows equal chunk 0's
   int met;        // some row's output metric differs from 1
+  int ends;       // the boundary rows have recipes of their own (else the bulk stencil on the wrapped window)
 };
 
 // bulk offsets (p1,p2,p3,p4) per operator kind, relative to the output row
@@ -81,7 +82,7 @@ __device__ __forceinline__ void chunk_rhs(const KArgs<T>& P, int q, int len, con
       for (int m = 0; m < 4; ++m) v = v + R.u[m] * fb4[m];
     return v * P.dd;
   };
-  const bool first = q == 0, last = q == P.P - 1;
+  const bool first = q == 0 && P.ends, last = q == P.P - 1 && P.ends;
   T v0 = T(0), v1 = T(0), v2 = T(0), v3 = T(0);
   if (first) {
     v0 = special(0);

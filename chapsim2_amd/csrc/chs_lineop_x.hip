@@ -222,9 +222,18 @@ __device__ __forceinline__ void read_window(const T* row, int s0, int n_in, int 
   if (s0 >= 2 && s0 + CH + 1 < n_in) {
 #pragma unroll
     for (int t = 0; t < CH + 4; ++t) f[t] = row[s0 - 2 + t];
+  } else if (wrap) {
+#pragma unroll
+    for (int t = 0; t < CH + 4; ++t) {
+      const int k = s0 - 2 + t;
+      f[t] = row[k < 0 ? k + n_in : (k >= n_in ? k - n_in : k)];
+    }
   } else {
 #pragma unroll
-    for (int t = 0; t < CH + 4; ++t) f[t] = row[window_index(s0 - 2 + t, n_in, wrap)];
+    for (int t = 0; t < CH + 4; ++t) {
+      const int k = s0 - 2 + t;
+      f[t] = row[k < 0 ? 0 : (k > n_in - 1 ? n_in - 1 : k)];
+    }
   }
 }
 
