@@ -88,9 +88,22 @@ template <typename T, int G>
 __device__ __forceinline__ Grp<T, G> ld_grp(const T* base, unsigned boff) {
   return *reinterpret_cast<const Grp<T, G>*>(reinterpret_cast<const char*>(base) + boff);
 }
+// Stores of finished output are non-temporal: each value is written once
+// and read by a later kernel, long after this tile's lines have left the
+// cache (FP64 x operator 35.6 -> 34.0 us, FP32 23.8 -> 20.5 us at
+// 256x128x256; non-temporal loads of the input were slower).
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 template <typename T, int G>
 __device__ __forceinline__ void st_grp(T* base, unsigned boff, const Grp<T, G>& g) {
-  *reinterpret_cast<Grp<T, G>*>(reinterpret_cast<char*>(base) + boff) = g;
+  char* a = reinterpret_cast<char*>(base) + boff;
+  if constexpr (sizeof(T) * G == 16) {
+    u32x4 v;
+    __builtin_memcpy(&v, &g, 16);
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(a));
+  } else {
+#pragma unroll
+    for (int c = 0; c < G; ++c) __builtin_nontemporal_store(g.v[c], reinterpret_cast<T*>(a) + c);
+  }
 }
 
 // Stage NLW whole x lines (input transform applied) into tile rows of
