@@ -726,8 +726,9 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
       //     through the vector cache): half the LDS, two workgroups per CU
       //  2: 256/P lines with skewed rows
       //  3: lines of kChunkLong-row chunks only: the lane-per-line kernel
-      //  4: lines of kChunk-row chunks only: 16 whole lines in LDS, four
-      //     chunks per wave (nothing else in LDS), four workgroups per CU
+      //  4: 16 whole lines in LDS, four chunks per wave (nothing else in
+      //     LDS but the rows of equal chunks): four workgroups per CU at 256
+      //     rows, two FP64 workgroups per CU at 512 rows
       //  5: lines of kChunk-row chunks read 16 bytes at a time, one or two
       //     input arrays: persistent workgroups over 32-line tiles, the next
       //     tile's input loaded while the current one is solved and stored
@@ -749,7 +750,6 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
       auto run = [&](int v, KArgs<T> ka) -> bool {
         ka.vec = x_vec_ok(ka);
         if (v == 0 || v == 1 || v == 4) {
-          if (v == 4 && CHp != kChunk) return false;
           const int nlw = v == 0 ? 64 : (v == 1 ? 32 : 16);
           const size_t shm = v == 0 ? shm64 : (v == 1 ? shm32 : shm16);
           if (shm > lds) return false;

@@ -565,9 +565,15 @@ bool launch_x_tile(int op, int ch, int nlw, dim3 g, dim3 b, size_t shm, hipStrea
     else tile_in<T, kChunk, 16>(op, g, b, shm, s, a);
     return true;
   }
-  if (ch != kChunkLong || nlw == 16) return false;
+  if (ch != kChunkLong) return false;
   if (nlw == 32) {
     tile_in<T, kChunkLong, 32>(op, g, b, shm, s, a);
+    return true;
+  }
+  if (nlw == 16) {
+    // 16 lines of long chunks: an FP64 tile of 512-row lines takes 66 KB, so
+    // two workgroups share a CU where one 32-line tile fills it
+    tile_in<T, kChunkLong, 16>(op, g, b, shm, s, a);
     return true;
   }
   // 64 lines of long chunks: FP32 only (an FP64 tile exceeds the LDS)
