@@ -210,21 +210,27 @@ __global__ void __launch_bounds__(128) k_bwd(const uint16_t* __restrict__ p16, c
   if (threadIdx.x == 0) lossp[blockIdx.x] = red[0];
 }
 
-// G[r][c] = sum_b U[r][b] * V[c][b] (V null: V = 1) in a fixed order
-__global__ void k_outer(const float* __restrict__ U, int R, const float* __restrict__ V, int C, int B,
-                        float* __restrict__ G) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= R * C) return;
-  const int r = t / C, c = t - r * C;
+// G[r][c] = sum_b U[r][b] * V[c][b] (V null: V = 1) in a fixed order: one
+// workgroup of 256 threads per gradient entry (r, c) = blockIdx.x; thread t
+// sums b = t, t + 256, ... (coalesced across the workgroup), then a fixed
+// tree in LDS adds the 256 partial sums.
+constexpr int kOuterThreads = 256;
+__global__ void __launch_bounds__(kOuterThreads) k_outer(const float* __restrict__ U, int R, const float* __restrict__ V,
+                                                         int C, int B, float* __restrict__ G) {
+  __shared__ float red[kOuterThreads];
+  const int e = blockIdx.x;
+  const int r = e / C, c = e - r * C;
   const float* u = U + (long)r * B;
   const float* v = V ? V + (long)c * B : nullptr;
-  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  int b = 0;
-  for (; b + 3 < B; b += 4)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] += u[b + q] * (v ? v[b + q] : 1.0f);
-  for (; b < B; ++b) acc[0] += u[b] * (v ? v[b] : 1.0f);
-  G[t] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  float acc = 0.0f;
+  for (int b = threadIdx.x; b < B; b += kOuterThreads) acc += u[b] * (v ? v[b] : 1.0f);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = kOuterThreads / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) G[e] = red[0];
 }
 
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -286,7 +292,7 @@ bool dalloc(chs_mlp* m, T** p, size_t n) {
 int reduce_grads(chs_mlp* m, int B, int nblk, float* loss) {
   hipStream_t s = m->stream;
   auto outer = [&](const float* U, int R, const float* V, int C, float* G) {
-    hipLaunchKernelGGL(k_outer, dim3((R * C + 127) / 128), dim3(128), 0, s, U, R, V, C, B, G);
+    hipLaunchKernelGGL(k_outer, dim3(R * C), dim3(kOuterThreads), 0, s, U, R, V, C, B, G);
   };
   outer(m->dz1T, H1, m->xb, 1, m->g32 + OW1);
   outer(m->dz1T, H1, nullptr, 1, m->g32 + OB1);

@@ -155,10 +155,14 @@ def test_thermo_surrogate_mode_vs_table(po):
     the table for the properties from h: 10 RK3 steps against the table mode
     on the same state.  The properties start off by the surrogate's fit
     error over the enthalpies present (measured here) and the velocity
-    q = g / rho by the density's; over the steps the differences feed back
-    through rho, h and g (4-7 times the fit error after 10 steps when
-    measured), so each field is held to ten times the fit error that drives
-    it.  Reported per field."""
+    q = g / rho by the density's.  The refresh takes h = rho*h / rho, so a
+    density error moves h by the same relative amount and every property
+    with it (through dp/dh); over the steps the differences feed back
+    through rho, h and g (4-12 times the driving fit error after 10 steps
+    when measured).  Each property is held to ten times the larger of its
+    own fit error and the density's, the velocities (relative to the
+    largest velocity component) to ten times the density's.  Reported per
+    field."""
     from chapsim2_amd.domain import PROJ_EXACT, Domain
 
     dm, of, keep = _pair(po, (32, 24, 16), 17, mode=1, surrogate_steps=6000)
@@ -179,12 +183,18 @@ def test_thermo_surrogate_mode_vs_table(po):
         fit_err = {k: rel(fit[k], tbl[k]) for k in ("t", "d", "m", "k")}
         qa, qb = dm.get_state(), ref.get_state()
         errs = {k: rel(a[k], b[k]) for k in ("t", "d", "m", "k")}
-        errs.update({k: rel(x, y) for k, x, y in zip(("qx", "qy", "qz"), qa[:3], qb[:3])})
+        # velocity differences relative to the velocity scale (the largest
+        # component, qx): a density error changes q = g / rho by the same
+        # relative amount of the local velocity, which for qy and qz is far
+        # below their own small maxima
+        vscale = max(np.abs(x).max() for x in qb[:3])
+        errs.update({k: np.abs(x - y).max() / vscale for k, x, y in zip(("qx", "qy", "qz"), qa[:3], qb[:3])})
         print("surrogate fit error over the state's h:", " ".join(f"{k} {v:.2e}" for k, v in fit_err.items()),
               flush=True)
         print("surrogate vs table after 10 steps:", " ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
         for k in ("t", "d", "m", "k"):
-            assert errs[k] < 10.0 * fit_err[k] + 1e-12, (k, errs[k], fit_err[k])
+            drive = max(fit_err[k], fit_err["d"])
+            assert errs[k] < 10.0 * drive + 1e-12, (k, errs[k], fit_err[k], fit_err["d"])
         for k in ("qx", "qy", "qz"):
             assert errs[k] < 10.0 * fit_err["d"] + 1e-12, (k, errs[k], fit_err["d"])
         assert max(fit_err.values()) < 5e-2
