@@ -530,7 +530,7 @@ void line_rhs_host(const LineGeom& g, Op op, int acc, const int ibc[2], const do
 }
 
 std::string make_line_plan(const LineGeom& g, Op op, int acc, const int ibc0[2], bool has_fbc, int n_in, int n_out,
-                           LinePlan* P) {
+                           LinePlan* P, int chunk_rows) {
   if (n_out < 8 || n_in < 8) return "line too short (need at least 8 points)";
   if (acc < CD2 || acc > CP6) return "invalid accuracy";
   for (int s = 0; s < 2; ++s)
@@ -629,6 +629,8 @@ std::string make_line_plan(const LineGeom& g, Op op, int acc, const int ibc0[2],
   {
     const int n = n_out;
     p.part_ch = (n <= kChunk * kMaxChunks) ? kChunk : kChunkLong;
+    if (chunk_rows == kChunkLong) p.part_ch = kChunkLong;
+    else if (chunk_rows != 0 && chunk_rows != kChunk) return "chunk rows must be kChunk or kChunkLong";
     int P = (n + p.part_ch - 1) / p.part_ch;
     if (P < 1) P = 1;
     p.part_P = P;

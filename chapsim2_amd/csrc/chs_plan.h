@@ -88,8 +88,10 @@ struct LinePlan {
 };
 
 // Build a plan; returns empty string on success or an error message.
+// chunk_rows: rows per chunk of the partitioned evaluation (kChunk or
+// kChunkLong); 0 takes kChunk up to kChunk * kMaxChunks rows, else kChunkLong.
 std::string make_line_plan(const LineGeom& g, Op op, int acc, const int ibc0[2], bool has_fbc,
-                           int n_in, int n_out, LinePlan* out);
+                           int n_in, int n_out, LinePlan* out, int chunk_rows = 0);
 
 // Evaluate the explicit RHS of one line on the host (reference row formulas,
 // src/basics_operations2.f90:1980-2719), before the 1/h scaling.

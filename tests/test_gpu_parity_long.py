@@ -330,3 +330,54 @@ def test_fp32_x_operators_512_rows_vs_oracle(po, op):
         assert 1e-9 < err < tol
     finally:
         dm.close()
+
+
+@pytest.mark.parametrize("ny", [128, 150])
+@pytest.mark.parametrize("op", range(6))
+def test_fp32_y_operators_long_chunks_vs_oracle(po, ny, op):
+    """FP32 y lines of 129-160 output rows run on 32-row chunks (5 instead of
+    9 chunks of 16); every operator against the FP64 oracle, walls with the
+    operator's boundary kinds (Neumann for cell-centred inputs, Dirichlet for
+    node inputs, zero wall values).  A relative perturbation eps of input,
+    coefficients and arithmetic moves an output by at most
+    eps * ||M||_inf * max|f| with ||M||_inf the largest absolute row sum of
+    the operator's matrix (assembled from the oracle's impulse responses);
+    the error is held to 4 * EPS32 * ||M||_inf * max|f| / max|ref|."""
+    import ctypes as C
+
+    import torch
+
+    from chapsim2_amd import _lib
+    from chapsim2_amd.domain import Domain
+
+    nc = (16, ny, 8)
+    cin = op in (0, 2, 4)  # MIDP_C2P, D1_C2C, D1_C2P read cell-centred values
+    cout = op in (1, 2, 5)
+    bc = [5, 5] if cin else [4, 4]
+    n_in = ny if cin else ny + 1
+    n_out = ny if cout else ny + 1
+    ei, eo = [nc[0], n_in, nc[2]], [nc[0], n_out, nc[2]]
+    dm = Domain(nc, LX, -1.0, 1.0, LZ, **CFG)
+    odm = po.Dom(list(nc), [1, 0, 1], LX, -1.0, 1.0, LZ, 2, 0.075)
+    try:
+        rng = np.random.default_rng(100 + op + ny)
+        s = np.linspace(-1.0, 1.0, n_in)
+        ph = rng.uniform(0.0, 2 * np.pi, (nc[0], 1, nc[2]))
+        fi = np.asfortranarray(sum(np.cos(k * np.pi * s[None, :, None] + ph) / k for k in (1, 3, 7, 19)))
+        fbc = np.zeros((nc[0], 4, nc[2]), order="F")
+        ref = odm.op3d(1, op, 4, bc, fi, eo, fbc)
+        ti = torch.from_numpy(np.ascontiguousarray(fi.T)).float().cuda()
+        to = torch.zeros(eo[::-1], dtype=torch.float32, device="cuda")
+        tb = torch.zeros(nc[2], 4, nc[0], dtype=torch.float32, device="cuda")
+        _lib.check(dm.lib.chs_op3d_f32(dm.h, 1, op, 4, _lib.ivec(bc), C.c_void_p(ti.data_ptr()), _lib.ivec(ei),
+                                       C.c_void_p(to.data_ptr()), _lib.ivec(eo), C.c_void_p(tb.data_ptr())))
+        got = to.double().cpu().numpy().T
+        err = rel(got, ref)
+        M = np.stack([odm.op1d(1, op, 4, bc, np.eye(n_in)[j], n_out, np.zeros(4)) for j in range(n_in)], axis=1)
+        mnorm = float(np.abs(M).sum(axis=1).max())
+        tol = 4 * EPS32 * mnorm * np.abs(fi).max() / np.abs(ref).max()
+        print(f"y op {op}, {n_out} rows, FP32 vs FP64 oracle: {err:.2e} (||M||_inf {mnorm:.1f}, "
+              f"tolerance {tol:.2e})", flush=True)
+        assert 1e-9 < err < tol
+    finally:
+        dm.close()
