@@ -26,6 +26,7 @@ struct chs_props {
   chs_mlp* surrogate = nullptr;  // used by CHS_PROPS_MLP (not owned)
   size_t lds_block = 65536;      // LDS bytes a workgroup may use on the table's device
   int cus = 0;                   // compute units of that device
+  bool h_rising = false;         // the h column strictly increases (bisections on h may be skipped)
   ~chs_props() {
     if (dev) (void)hipFree(dev);
     if (flag) (void)hipFree(flag);

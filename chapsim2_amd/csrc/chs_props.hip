@@ -26,6 +26,7 @@ namespace {
 
 enum { CT = 0, CD, CM, CK, CB, CCP, CH, CRH };
 constexpr double kMinP = 1.0e-20;  // src/modules.f90:141
+constexpr double kNodeGap = 1.0e-10;  // sqrt(kMinP)
 constexpr int kPropsLds = 2048;    // rows of the one column k_props_from_h stages (x3, 48 KB)
 constexpr int kPropsBlock = 1024;  // threads of the persistent from-rho*h kernel
 
@@ -66,7 +67,8 @@ __device__ __forceinline__ void bisect_dev(const double* c, int n, double v, int
 // from_DH).  crh, ch and cd may point into LDS; the other columns are read
 // from tab.
 __device__ __forceinline__ void props_dh_cell(const double* crh, const double* ch, const double* cd,
-                                              const double* __restrict__ tab, int n, double dhmin, double dhmax,
+                                              const double* __restrict__ tab, int n, int hrise, double dhmin,
+                                              double dhmax,
                                               double v, long c, double* __restrict__ oh, double* __restrict__ od,
                                               double* __restrict__ om, double* __restrict__ ok,
                                               double* __restrict__ ot, double* __restrict__ ob,
@@ -83,13 +85,17 @@ __device__ __forceinline__ void props_dh_cell(const double* crh, const double* c
   double w1 = (crh[i2] - v) / (crh[i2] - crh[i1]);
   double w2 = 1.0 - w1;
   double h = w1 * ch[i1] + w2 * ch[i2];
-  // properties from h (first pass needs only d)
-  bisect_dev(ch, n, h, &i1, &i2);
+  // properties from h (first pass needs only d).  On a strictly increasing
+  // h column the bisection for a value h with ch[k] + kNodeGap < h <
+  // ch[k+1] ends in (k, k+1): every sign product it tests is then at least
+  // (h - ch[k])^2 > kMinP.  So where h lies so inside the interval already
+  // found, that interval is the bisection's result and the search is skipped.
+  if (!(hrise && ch[i1] + kNodeGap < h && h < ch[i2])) bisect_dev(ch, n, h, &i1, &i2);
   w1 = (ch[i2] - h) / (ch[i2] - ch[i1]);
   w2 = 1.0 - w1;
   const double d1 = w1 * cd[i1] + w2 * cd[i2];
   h = v / d1;
-  bisect_dev(ch, n, h, &i1, &i2);
+  if (!(hrise && ch[i1] + kNodeGap < h && h < ch[i2])) bisect_dev(ch, n, h, &i1, &i2);
   w1 = (ch[i2] - h) / (ch[i2] - ch[i1]);
   w2 = 1.0 - w1;
   auto interp = [&](int col) { return w1 * tab[(long)col * n + i1] + w2 * tab[(long)col * n + i2]; };
@@ -103,7 +109,7 @@ __device__ __forceinline__ void props_dh_cell(const double* crh, const double* c
 }
 
 // Table in global memory (any size): one thread per cell.
-__global__ void __launch_bounds__(256) k_props_from_dh(const double* __restrict__ tab, int n, double dhmin,
+__global__ void __launch_bounds__(256) k_props_from_dh(const double* __restrict__ tab, int n, int hrise, double dhmin,
                                                        double dhmax, const double* __restrict__ rhoh_in, long N,
                                                        double* __restrict__ oh, double* __restrict__ od,
                                                        double* __restrict__ om, double* __restrict__ ok,
@@ -111,7 +117,7 @@ __global__ void __launch_bounds__(256) k_props_from_dh(const double* __restrict_
                                                        double* __restrict__ ocp, int* __restrict__ flag) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
-  props_dh_cell(tab + (long)CRH * n, tab + (long)CH * n, tab + (long)CD * n, tab, n, dhmin, dhmax, rhoh_in[c], c,
+  props_dh_cell(tab + (long)CRH * n, tab + (long)CH * n, tab + (long)CD * n, tab, n, hrise, dhmin, dhmax, rhoh_in[c], c,
                 oh, od, om, ok, ot, ob, ocp, flag);
 }
 
@@ -122,7 +128,8 @@ __global__ void __launch_bounds__(256) k_props_from_dh(const double* __restrict_
 // barrier: the staging loop runs before any thread leaves.
 template <int NC>
 __global__ void __launch_bounds__(kPropsBlock) k_props_from_dh_lds(
-    const double* __restrict__ tab, int n, double dhmin, double dhmax, const double* __restrict__ rhoh_in, long N,
+    const double* __restrict__ tab, int n, int hrise, double dhmin, double dhmax, const double* __restrict__ rhoh_in,
+    long N,
     double* __restrict__ oh, double* __restrict__ od, double* __restrict__ om, double* __restrict__ ok,
     double* __restrict__ ot, double* __restrict__ ob, double* __restrict__ ocp, int* __restrict__ flag) {
   extern __shared__ double sh[];
@@ -138,7 +145,7 @@ __global__ void __launch_bounds__(kPropsBlock) k_props_from_dh_lds(
   const double* sd = NC == 3 ? sh + 2 * n : cd;
   const long stride = (long)gridDim.x * blockDim.x;
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += stride)
-    props_dh_cell(sh, sh + n, sd, tab, n, dhmin, dhmax, rhoh_in[c], c, oh, od, om, ok, ot, ob, ocp, flag);
+    props_dh_cell(sh, sh + n, sd, tab, n, hrise, dhmin, dhmax, rhoh_in[c], c, oh, od, om, ok, ot, ob, ocp, flag);
 }
 
 // ftp_refresh_thermal_properties_from_H per cell (no clamping, as the
@@ -181,20 +188,21 @@ __global__ void __launch_bounds__(256) k_props_from_h(const double* __restrict__
 hipError_t launch_from_dh(const chs_props* p, const double* rhoh, long n, double* const o[7], int* flag,
                           hipStream_t s) {
   const size_t col = sizeof(double) * (size_t)p->n;
+  const int hrise = p->h_rising ? 1 : 0;
   const int nc = 3 * col <= p->lds_block ? 3 : 2 * col <= p->lds_block ? 2 : 0;
   if (nc) {
     const long want = (n + kPropsBlock - 1) / kPropsBlock;
     const long cap = 2L * (p->cus > 0 ? p->cus : 256);
     const unsigned grid = (unsigned)(want < cap ? want : cap);
     if (nc == 3)
-      hipLaunchKernelGGL(k_props_from_dh_lds<3>, dim3(grid), dim3(kPropsBlock), 3 * col, s, p->dev, p->n, p->dhmin,
+      hipLaunchKernelGGL(k_props_from_dh_lds<3>, dim3(grid), dim3(kPropsBlock), 3 * col, s, p->dev, p->n, hrise, p->dhmin,
                          p->dhmax, rhoh, n, o[0], o[1], o[2], o[3], o[4], o[5], o[6], flag);
     else
-      hipLaunchKernelGGL(k_props_from_dh_lds<2>, dim3(grid), dim3(kPropsBlock), 2 * col, s, p->dev, p->n, p->dhmin,
+      hipLaunchKernelGGL(k_props_from_dh_lds<2>, dim3(grid), dim3(kPropsBlock), 2 * col, s, p->dev, p->n, hrise, p->dhmin,
                          p->dhmax, rhoh, n, o[0], o[1], o[2], o[3], o[4], o[5], o[6], flag);
   } else {
     const unsigned grid = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_props_from_dh, dim3(grid), dim3(256), 0, s, p->dev, p->n, p->dhmin, p->dhmax, rhoh, n,
+    hipLaunchKernelGGL(k_props_from_dh, dim3(grid), dim3(256), 0, s, p->dev, p->n, hrise, p->dhmin, p->dhmax, rhoh, n,
                        o[0], o[1], o[2], o[3], o[4], o[5], o[6], flag);
   }
   return hipGetLastError();
@@ -286,6 +294,9 @@ int chs_props_create(const double* table, int nrows, double ref_t0, chs_props_t*
   }
   p->dhmin = p->col[CRH][imin];
   p->dhmax = p->col[CRH][imax];
+  p->h_rising = true;
+  for (int i = 1; i < nrows; ++i)
+    if (!(p->col[CH][i] > p->col[CH][i - 1])) p->h_rising = false;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
     delete p;
