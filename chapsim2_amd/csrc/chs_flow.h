@@ -56,6 +56,19 @@ struct RhsWork {
   bool ready = false;
 };
 
+// The energy right-hand side's operands in the precision T its operators
+// run in: h, k and T at the cell centres (FP64: the Flow's own fields; FP32:
+// rounded copies made at the start of each energy sub-step), the x, y, z
+// parts of the right-hand side, h, k and dT/dx at the faces, and the wall
+// rows (nx,4,nz) of h, k, T and of the conductive flux k dT/dy.
+template <typename T>
+struct EnergyWork {
+  Field<T> hent, kcond, temp;
+  Field<T> ex, ey, ez;
+  Field<T> hpcc, hcpc, hccp, kpcc, kcpc, kccp, tpcc, tcpc, tccp;
+  T *fby_h = nullptr, *fby_k = nullptr, *fby_t = nullptr, *fby_q = nullptr;
+};
+
 class Flow {
  public:
   static std::string create(Handle* H, std::unique_ptr<Flow>* out);
@@ -99,9 +112,15 @@ class Flow {
   const chs_props* props = nullptr;
   double wall_prop[2][7] = {};  // h, d, m, k, t, b, cp of each y wall
   double prandtl = 1.0, rprren = 0.0;
-  Field<double> rhoh, hent, temp, kcond, dens, eres, erhs0, ex, ey, ez;
-  Field<double> hpcc, hcpc, hccp, kpcc, kcpc, kccp, tpcc, tcpc, tccp, dface[3];
-  double *fby_h = nullptr, *fby_k = nullptr, *fby_d = nullptr, *fby_t = nullptr, *fby_q = nullptr;
+  Field<double> rhoh, hent, temp, kcond, dens, eres, erhs0, dface[3];
+  double* fby_d = nullptr;
+  // the energy right-hand side's work: FP64, and FP32 in the mixed form
+  // (there the energy right-hand side is FP32 like the momentum one)
+  EnergyWork<double> e64;
+  EnergyWork<float> e32;
+  // mixed form: the FP32 copies of g (w32) were made by the energy sub-step
+  // of this sub-step, so the momentum right-hand side need not remake them
+  bool g32_fresh = false;
   int* props_flag = nullptr;
   std::string copy_rhs(double* mx, double* my, double* mz, int mem);
 
@@ -173,7 +192,14 @@ class Flow {
   template <typename T> std::string thermo_first_level(RhsWork<T>& w);
   template <typename T> std::string alloc_thermo_work(RhsWork<T>& w);
   std::string refresh_props();
-  std::string energy_terms(int isub);  // ex, ey, ez: the x, y, z parts of the energy right-hand side
+  // the x, y, z parts of the energy right-hand side into e.ex, e.ey, e.ez
+  // (the FP32 form first rounds g, h, k and T into w and e)
+  std::string energy_terms();
+  template <typename T> std::string energy_terms_t(RhsWork<T>& w, EnergyWork<T>& e);
+  // rhs = dt (tg cur + tz rhs0), rhs0 = cur, with cur = ex + ey + ez; into
+  // res and / or added to rho h
+  std::string energy_combine(int isub, double* res, double* rhoh_acc);
+  template <typename T> std::string alloc_energy_work(EnergyWork<T>& e, bool own_centres);
   template <typename T> std::string alloc_work(RhsWork<T>& w, bool alias_state);
   template <typename T> std::string alloc(Field<T>& f, int e1, int e2, int e3);
   std::vector<void*> owned;

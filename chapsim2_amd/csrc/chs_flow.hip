@@ -288,13 +288,14 @@ __global__ void k_g2q(double* __restrict__ q, const double* __restrict__ g, cons
 // energy: rhs = ex + (ey + ez) (the pencil order of src/eq_energy.f90:386-391),
 // Calculate_energy_fractional_step (:148-176): res = dt (tg rhs + tz rhs0),
 // rhs0 = rhs; with update, rho*h += res (Solve_energy_eq, :440)
-__global__ void k_energy_combine(const double* __restrict__ ex, const double* __restrict__ ey,
-                                 const double* __restrict__ ez, double* __restrict__ r0, double* __restrict__ res,
-                                 double* __restrict__ rhoh, long n, double tg, double tz, double dt) {
+template <typename TI>
+__global__ void k_energy_combine(const TI* __restrict__ ex, const TI* __restrict__ ey, const TI* __restrict__ ez,
+                                 double* __restrict__ r0, double* __restrict__ res, double* __restrict__ rhoh, long n,
+                                 double tg, double tz, double dt) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double yz = ey[i] + ez[i];
-  const double cur = ex[i] + yz;
+  double yz = (double)ey[i] + (double)ez[i];
+  const double cur = (double)ex[i] + yz;
   double tot = tg * cur + tz * r0[i];
   r0[i] = cur;
   const double rv = dt * tot;
@@ -936,12 +937,15 @@ std::string Flow::momentum_rhs(int isub, bool fuse) {
     q32_fresh = true;
   }
   if (thermo) {
-    // g and mu change every sub-step (projection, property refresh)
+    // g and mu change every sub-step (projection, property refresh); the
+    // energy sub-step before this right-hand side has rounded g already
     hipStream_t s = H->stream;
     for (auto pr2 : {std::make_pair(&w32.gx, &w64.gx), std::make_pair(&w32.gy, &w64.gy),
                      std::make_pair(&w32.gz, &w64.gz), std::make_pair(&w32.visc, &w64.visc)})
-      hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr2.second->size()), dim3(256), 0, s, pr2.first->p,
-                         pr2.second->p, pr2.second->size());
+      if (!g32_fresh || pr2.first == &w32.visc)
+        hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr2.second->size()), dim3(256), 0, s, pr2.first->p,
+                           pr2.second->p, pr2.second->size());
+    g32_fresh = false;
   }
   const std::string r = momentum_rhs_t<float>(w32, isub, fuse);
   if (fuse) q32_fresh = false;  // the state was updated
@@ -1175,6 +1179,44 @@ std::string Flow::alloc_thermo_work(RhsWork<T>& w) {
   return "";
 }
 
+// wall rows (nx,4,nz) of wall property col (h, d, m, k, t, b, cp): rows 1,2
+// the two walls, rows 3,4 the same again (rows34) or zero
+template <typename T>
+static std::string upload_wall_rows(T** q, int nx, int nz, const double wp[2][7], int col, bool rows34,
+                                    std::vector<void*>& owned) {
+  const long nf = (long)nx * 4 * nz;
+  std::vector<T> hb(nf);
+  for (long t = 0; t < nf; ++t) {
+    const long r = (t / nx) % 4;
+    hb[t] = (!rows34 && r >= 2) ? T(0) : T(wp[r % 2][col]);
+  }
+  FCHK(hipMalloc(q, sizeof(T) * nf));
+  owned.push_back(*q);
+  FCHK(hipMemcpy(*q, hb.data(), sizeof(T) * nf, hipMemcpyHostToDevice));
+  return "";
+}
+
+// The energy right-hand side's work in precision T; own_centres allocates
+// h, k, T at the cell centres (the FP32 copies), else the caller sets them.
+// Wall rows: h and k of the walls in rows 1,2 and again in rows 3,4; T of
+// the walls in rows 1,2 and zero in rows 3,4 (get_fbcy_iTh,
+// src/bc_general.f90:784-805); the wall flux is scratch.
+template <typename T>
+std::string Flow::alloc_energy_work(EnergyWork<T>& e, bool own_centres) {
+  if (own_centres)
+    for (Field<T>* f : {&e.hent, &e.kcond, &e.temp}) SCHK(alloc(*f, nx, ny, nz));
+  for (Field<T>* f : {&e.ex, &e.ey, &e.ez}) SCHK(alloc(*f, nx, ny, nz));
+  SCHK(alloc(e.hpcc, px, ny, nz)); SCHK(alloc(e.hcpc, nx, py, nz)); SCHK(alloc(e.hccp, nx, ny, pz));
+  SCHK(alloc(e.kpcc, px, ny, nz)); SCHK(alloc(e.kcpc, nx, py, nz)); SCHK(alloc(e.kccp, nx, ny, pz));
+  SCHK(alloc(e.tpcc, px, ny, nz)); SCHK(alloc(e.tcpc, nx, py, nz)); SCHK(alloc(e.tccp, nx, ny, pz));
+  SCHK(upload_wall_rows(&e.fby_h, nx, nz, wall_prop, 0, true, owned));
+  SCHK(upload_wall_rows(&e.fby_k, nx, nz, wall_prop, 3, true, owned));
+  SCHK(upload_wall_rows(&e.fby_t, nx, nz, wall_prop, 4, false, owned));
+  FCHK(hipMalloc(&e.fby_q, sizeof(T) * (size_t)nx * 4 * nz));
+  owned.push_back(e.fby_q);
+  return "";
+}
+
 // interpolated g of the convective products and the viscosity interpolations
 // (src/eq_momentum2.f90:576-703), issued on the three lanes of the
 // first-level operators
@@ -1231,35 +1273,15 @@ std::string Flow::thermo_enable(const chs_thermo_desc_t& d) {
   double p0[7];
   props_from_t_host(props, d.t_init, p0);
   // FP64 fields
-  for (Field<double>* f : {&rhoh, &hent, &temp, &kcond, &dens, &eres, &erhs0, &ex, &ey, &ez})
-    SCHK(alloc(*f, nx, ny, nz));
-  SCHK(alloc(hpcc, px, ny, nz)); SCHK(alloc(hcpc, nx, py, nz)); SCHK(alloc(hccp, nx, ny, pz));
-  SCHK(alloc(kpcc, px, ny, nz)); SCHK(alloc(kcpc, nx, py, nz)); SCHK(alloc(kccp, nx, ny, pz));
-  SCHK(alloc(tpcc, px, ny, nz)); SCHK(alloc(tcpc, nx, py, nz)); SCHK(alloc(tccp, nx, ny, pz));
+  for (Field<double>* f : {&rhoh, &hent, &temp, &kcond, &dens, &eres, &erhs0}) SCHK(alloc(*f, nx, ny, nz));
   SCHK(alloc(dface[0], px, ny, nz)); SCHK(alloc(dface[1], nx, py, nz)); SCHK(alloc(dface[2], nx, ny, pz));
   SCHK(alloc_thermo_work<double>(w64));
   if (mixed) SCHK(alloc_thermo_work<float>(w32));
-  // wall arrays (nx,4,nz): h, k, d of the walls (rows 3,4 repeat rows 1,2);
-  // T of the walls in rows 1,2 and zero in rows 3,4 (get_fbcy_iTh,
-  // src/bc_general.f90:784-805); the wall flux scratch
-  const long nf = (long)nx * 4 * nz;
-  std::vector<double> hb(nf);
-  auto wall = [&](double** q, int col, bool rows34) -> std::string {
-    for (long t = 0; t < nf; ++t) {
-      const long r = (t / nx) % 4;
-      hb[t] = (!rows34 && r >= 2) ? 0.0 : wall_prop[r % 2][col];
-    }
-    FCHK(hipMalloc(q, sizeof(double) * nf));
-    owned.push_back(*q);
-    FCHK(hipMemcpy(*q, hb.data(), sizeof(double) * nf, hipMemcpyHostToDevice));
-    return "";
-  };
-  SCHK(wall(&fby_h, 0, true));
-  SCHK(wall(&fby_k, 3, true));
-  SCHK(wall(&fby_d, 1, true));
-  SCHK(wall(&fby_t, 4, false));
-  FCHK(hipMalloc(&fby_q, sizeof(double) * nf));
-  owned.push_back(fby_q);
+  e64.hent = hent; e64.kcond = kcond; e64.temp = temp;
+  SCHK(alloc_energy_work(e64, false));
+  if (mixed) SCHK(alloc_energy_work(e32, true));
+  // the density's wall rows (nx,4,nz; rows 3,4 repeat rows 1,2)
+  SCHK(upload_wall_rows(&fby_d, nx, nz, wall_prop, 1, true, owned));
   FCHK(hipMalloc(&props_flag, sizeof(int)));
   owned.push_back(props_flag);
   FCHK(hipMemset(props_flag, 0, sizeof(int)));
@@ -1336,25 +1358,46 @@ std::string Flow::convert_qg(bool to_g) {
   return "";
 }
 
-std::string Flow::energy_rhs(int isub) {
-  SCHK(energy_terms(isub));
+std::string Flow::energy_combine(int isub, double* res, double* rhoh_acc) {
   const long n = rhoh.size();
-  hipLaunchKernelGGL(k_energy_combine, grid1(n), dim3(256), 0, H->stream, ex.p, ey.p, ez.p, erhs0.p, eres.p,
-                     (double*)nullptr, n, tg[isub], tz[isub], dt);
+  if (mixed)
+    hipLaunchKernelGGL(k_energy_combine<float>, grid1(n), dim3(256), 0, H->stream, e32.ex.p, e32.ey.p, e32.ez.p,
+                       erhs0.p, res, rhoh_acc, n, tg[isub], tz[isub], dt);
+  else
+    hipLaunchKernelGGL(k_energy_combine<double>, grid1(n), dim3(256), 0, H->stream, e64.ex.p, e64.ey.p, e64.ez.p,
+                       erhs0.p, res, rhoh_acc, n, tg[isub], tz[isub], dt);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return std::string("energy_rhs: ") + hipGetErrorString(e);
+  if (e != hipSuccess) return std::string("energy_combine: ") + hipGetErrorString(e);
   return "";
 }
 
-std::string Flow::energy_terms(int isub) {
-  (void)isub;
+std::string Flow::energy_rhs(int isub) {
+  SCHK(energy_terms());
+  return energy_combine(isub, eres.p, nullptr);
+}
+
+std::string Flow::energy_terms() {
+  set_lanes();
+  if (!mixed) return energy_terms_t<double>(w64, e64);
+  // rounded copies of g (kept for this sub-step's momentum right-hand side)
+  // and of the cell-centre h, k, T of the last property refresh
+  hipStream_t s = H->stream;
+  for (auto pr2 : {std::make_pair(&w32.gx, &w64.gx), std::make_pair(&w32.gy, &w64.gy),
+                   std::make_pair(&w32.gz, &w64.gz), std::make_pair(&e32.hent, &hent),
+                   std::make_pair(&e32.kcond, &kcond), std::make_pair(&e32.temp, &temp)})
+    hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr2.second->size()), dim3(256), 0, s, pr2.first->p,
+                       pr2.second->p, pr2.second->size());
+  return energy_terms_t<float>(w32, e32);
+}
+
+template <typename T>
+std::string Flow::energy_terms_t(RhsWork<T>& w, EnergyWork<T>& e) {
   // Compute_energy_rhs (src/eq_energy.f90:178-396): convection -d(g_i h_i),
   // diffusion rPrRen d(k_i dT/dx_i); x on lane 0, y on lane 1, z on lane 2.
   // The y flux takes its own wall rows as wall values (extract_dirichlet_
   // fbcy) before the outer derivative: the reference guards that with
   // is_fbcx_velo_required (:356), which is false on this channel and would
   // pass MAXP wall values (wall rows ~1e19); the y-wall test is followed.
-  set_lanes();
   int ftp[3][2], econv[3][2], edifu[3][2];
   for (int d = 0; d < 3; ++d)
     for (int s2 = 0; s2 < 2; ++s2) ftp[d][s2] = (ibc[d][VT][s2] == BC_NEUMANN) ? BC_DIRICHLET : ibc[d][VT][s2];
@@ -1362,33 +1405,32 @@ std::string Flow::energy_terms(int isub) {
     prod_bc(ibc[d][d], ftp[d], econv[d]);
     tau_bc(ftp[d], ibc[d][VT], edifu[d]);
   }
-  const double m1 = -1.0, r = rprren;
-  auto o = [&](int lane, int dir, int opk, const int* bc, const Field<double>& A, Field<double>& out,
-               const double* fbc, double sA, const Field<double>* Cm, double sOut, int acc) {
-    return op<double>(w64, dir, opk, bc, A, out, fbc, sA, nullptr, 1.0, Cm, sOut, acc, nullptr, 0.0, nullptr, 0.0,
-                      lane);
+  const T m1 = T(-1), r = T(rprren), one = T(1);
+  auto o = [&](int lane, int dir, int opk, const int* bc, const Field<T>& A, Field<T>& out, const T* fbc, T sA,
+               const Field<T>* Cm, T sOut, int acc) {
+    return op<T>(w, dir, opk, bc, A, out, fbc, sA, nullptr, one, Cm, sOut, acc, nullptr, T(0), nullptr, T(0), lane);
   };
   SCHK(fork());
   // x
-  SCHK(o(0, 0, MIDP_C2P, ftp[0], hent, hpcc, nullptr, 1.0, nullptr, 1.0, 0));
-  SCHK(o(0, 0, MIDP_C2P, ftp[0], kcond, kpcc, nullptr, 1.0, nullptr, 1.0, 0));
-  SCHK(o(0, 0, D1_P2C, econv[0], w64.gx, ex, nullptr, m1, &hpcc, 1.0, 0));
-  SCHK(o(0, 0, D1_C2P, ibc[0][VT], temp, tpcc, nullptr, 1.0, nullptr, 1.0, 0));
-  SCHK(o(0, 0, D1_P2C, edifu[0], tpcc, ex, nullptr, 1.0, &kpcc, r, 1));
+  SCHK(o(0, 0, MIDP_C2P, ftp[0], e.hent, e.hpcc, nullptr, one, nullptr, one, 0));
+  SCHK(o(0, 0, MIDP_C2P, ftp[0], e.kcond, e.kpcc, nullptr, one, nullptr, one, 0));
+  SCHK(o(0, 0, D1_P2C, econv[0], w.gx, e.ex, nullptr, m1, &e.hpcc, one, 0));
+  SCHK(o(0, 0, D1_C2P, ibc[0][VT], e.temp, e.tpcc, nullptr, one, nullptr, one, 0));
+  SCHK(o(0, 0, D1_P2C, edifu[0], e.tpcc, e.ex, nullptr, one, &e.kpcc, r, 1));
   // y
-  SCHK(o(1, 1, MIDP_C2P, ftp[1], hent, hcpc, fby_h, 1.0, nullptr, 1.0, 0));
-  SCHK(o(1, 1, MIDP_C2P, ftp[1], kcond, kcpc, fby_k, 1.0, nullptr, 1.0, 0));
-  SCHK(o(1, 1, D1_P2C, econv[1], w64.gy, ey, nullptr, m1, &hcpc, 1.0, 0));
-  SCHK(o(1, 1, D1_C2P, ibc[1][VT], temp, tcpc, fby_t, 1.0, nullptr, 1.0, 0));
-  hipLaunchKernelGGL(k_wall_rows4<double>, grid1((long)nx * nz), dim3(256), 0, lane_stream(1), fby_q, tcpc.p,
-                     (const double*)kcpc.p, nx, py, nz);
-  SCHK(o(1, 1, D1_P2C, edifu[1], tcpc, ey, fby_q, 1.0, &kcpc, r, 1));
+  SCHK(o(1, 1, MIDP_C2P, ftp[1], e.hent, e.hcpc, e.fby_h, one, nullptr, one, 0));
+  SCHK(o(1, 1, MIDP_C2P, ftp[1], e.kcond, e.kcpc, e.fby_k, one, nullptr, one, 0));
+  SCHK(o(1, 1, D1_P2C, econv[1], w.gy, e.ey, nullptr, m1, &e.hcpc, one, 0));
+  SCHK(o(1, 1, D1_C2P, ibc[1][VT], e.temp, e.tcpc, e.fby_t, one, nullptr, one, 0));
+  hipLaunchKernelGGL(k_wall_rows4<T>, grid1((long)nx * nz), dim3(256), 0, lane_stream(1), e.fby_q, e.tcpc.p,
+                     (const T*)e.kcpc.p, nx, py, nz);
+  SCHK(o(1, 1, D1_P2C, edifu[1], e.tcpc, e.ey, e.fby_q, one, &e.kcpc, r, 1));
   // z
-  SCHK(o(2, 2, MIDP_C2P, ftp[2], hent, hccp, nullptr, 1.0, nullptr, 1.0, 0));
-  SCHK(o(2, 2, MIDP_C2P, ftp[2], kcond, kccp, nullptr, 1.0, nullptr, 1.0, 0));
-  SCHK(o(2, 2, D1_P2C, econv[2], w64.gz, ez, nullptr, m1, &hccp, 1.0, 0));
-  SCHK(o(2, 2, D1_C2P, ibc[2][VT], temp, tccp, nullptr, 1.0, nullptr, 1.0, 0));
-  SCHK(o(2, 2, D1_P2C, edifu[2], tccp, ez, nullptr, 1.0, &kccp, r, 1));
+  SCHK(o(2, 2, MIDP_C2P, ftp[2], e.hent, e.hccp, nullptr, one, nullptr, one, 0));
+  SCHK(o(2, 2, MIDP_C2P, ftp[2], e.kcond, e.kccp, nullptr, one, nullptr, one, 0));
+  SCHK(o(2, 2, D1_P2C, econv[2], w.gz, e.ez, nullptr, m1, &e.hccp, one, 0));
+  SCHK(o(2, 2, D1_C2P, ibc[2][VT], e.temp, e.tccp, nullptr, one, nullptr, one, 0));
+  SCHK(o(2, 2, D1_P2C, edifu[2], e.tccp, e.ez, nullptr, one, &e.kccp, r, 1));
   SCHK(join());
   return "";
 }
@@ -1396,14 +1438,10 @@ std::string Flow::energy_terms(int isub) {
 std::string Flow::energy_substep(int isub) {
   // Solve_energy_eq, src/eq_energy.f90:398-458 (the isub 3 density history
   // shift only feeds Calculate_drhodt, which is zero)
-  SCHK(energy_terms(isub));
-  const long n = rhoh.size();
-  hipLaunchKernelGGL(k_energy_combine, grid1(n), dim3(256), 0, H->stream, ex.p, ey.p, ez.p, erhs0.p,
-                     (double*)nullptr, rhoh.p, n, tg[isub], tz[isub], dt);
-  SCHK(refresh_props());
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return std::string("energy_substep: ") + hipGetErrorString(e);
-  return "";
+  SCHK(energy_terms());
+  g32_fresh = mixed;
+  SCHK(energy_combine(isub, nullptr, rhoh.p));
+  return refresh_props();
 }
 
 }  // namespace chs

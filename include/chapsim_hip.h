@@ -266,8 +266,8 @@ int chs_mlp_eval(chs_mlp_t *m, const double *h, long n, double *T, double *rho, 
  * (src/eq_momentum2.f90:576-703 and the thermo branches of :798-1507,
  * 1799-1932; convert_primary_conservative, src/para_conversion.f90:5-227).
  * The initial sub-step (isub 0) is momentum only, as at src/chapsim.f90:135.
- * FP64 and mixed precision (CHS_PREC_MIXED: the momentum right-hand side in
- * FP32; energy, properties and projection FP64). */
+ * FP64 and mixed precision (CHS_PREC_MIXED: the momentum and energy
+ * right-hand sides in FP32; the state, properties and projection FP64). */
 typedef struct chs_thermo_desc {
   chs_props_t *props;  /* property table (kept by the caller while the handle lives) */
   int mode;            /* CHS_PROPS_TABLE or CHS_PROPS_MLP */

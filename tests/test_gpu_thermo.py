@@ -130,7 +130,8 @@ def test_thermo_steps_vs_oracle(po, nc):
 
 def test_thermo_mixed_vs_fp64_oracle(po):
     """Mixed precision (FP32 momentum right-hand side with the variable-
-    property terms; energy, properties and projection FP64): 10 RK3 steps
+    property terms, FP32 energy right-hand side; rho h, properties and
+    projection FP64): 10 RK3 steps
     within 1e-6 of the FP64 oracle in velocity and in the thermal fields."""
     from chapsim2_amd.domain import PREC_MIXED
 
