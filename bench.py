@@ -210,8 +210,9 @@ def band_limited(shape, seed, kfrac=1.0 / 3.0):
 def ops_bench(args):
     """Isolated operator benchmark: every compact operator (CP6, channel
     boundary kinds), the second derivative d2 = 1der_P2C after 1der_C2P
-    along each axis, and the FFT(x,z) + TDMA(y) Poisson solve of a zero-mean
-    right-hand side, each in FP64 and FP32 on device-resident seeded
+    along each axis, the FFT(x,z) + TDMA(y) Poisson solve of a zero-mean
+    right-hand side and the 3-D FFT library's channel solve (poisson_010, on a
+    uniform y mesh of the same grid), each in FP64 and FP32 on device-resident seeded
     band-limited inputs.  Per kernel: average time from HIP events on the
     solver's stream, algorithmic bandwidth (arrays read + written; five
     read+write passes for the Poisson solve) and the FP32 result's error
@@ -222,7 +223,7 @@ def ops_bench(args):
     import torch
 
     from chapsim2_amd import _lib
-    from chapsim2_amd.domain import Domain
+    from chapsim2_amd.domain import FFT_2DECOMP_3DFFT, Domain
 
     nc = tuple(args.nc)
     dm = Domain(nc, 4 * np.pi, -1.0, 1.0, 4 * np.pi / 3, istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1)
@@ -320,40 +321,52 @@ def ops_bench(args):
     # Poisson solve of a zero-mean cell-centred right-hand side
     rhs = field(ext(0, "c"))
     rhs -= rhs.mean()
-    pois = {"op": "solve_fft_poisson"}
-    outs = {}
-    for dt, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
-        src = rhs.to(dt)
-        work = src.clone()
-        fn = (lambda: _lib.check(L.chs_poisson(dm.h, vp(work.data_ptr()), _lib.MEM_DEVICE))) if tag == "f64" \
-            else (lambda: _lib.check(L.chs_poisson_f32(dm.h, vp(work.data_ptr()))))
-        for _ in range(2):
-            work.copy_(src)
-            torch.cuda.synchronize()
-            fn()
-        tot_ms, tot_b = 0.0, 0.0
-        for _ in range(args.steps):
-            work.copy_(src)
-            torch.cuda.synchronize()
-            _lib.check(L.chs_profile_begin(dm.h, 3))
-            fn()
-            ms, nl, byt = C.c_double(), C.c_longlong(), C.c_double()
-            _lib.check(L.chs_profile_end(dm.h, C.byref(ms), C.byref(nl), C.byref(byt)))
-            tot_ms += ms.value
-            tot_b += byt.value
-        t = tot_ms / args.steps
-        pois[f"{tag}_us"] = round(t * 1e3, 2)
-        pois[f"{tag}_GBps"] = round(tot_b / args.steps / (t * 1e-3) / 1e9, 1)
-        outs[tag] = work
-    pois["f32_err_vs_f64"] = rel(outs["f32"], outs["f64"])
-    del outs
+
+    def time_poisson(dmp, name):
+        row = {"op": name}
+        outs = {}
+        for dt, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+            src = rhs.to(dt)
+            work = src.clone()
+            fn = (lambda: _lib.check(L.chs_poisson(dmp.h, vp(work.data_ptr()), _lib.MEM_DEVICE))) if tag == "f64" \
+                else (lambda: _lib.check(L.chs_poisson_f32(dmp.h, vp(work.data_ptr()))))
+            for _ in range(2):
+                work.copy_(src)
+                torch.cuda.synchronize()
+                fn()
+            tot_ms, tot_b = 0.0, 0.0
+            for _ in range(args.steps):
+                work.copy_(src)
+                torch.cuda.synchronize()
+                _lib.check(L.chs_profile_begin(dmp.h, 3))
+                fn()
+                ms, nl, byt = C.c_double(), C.c_longlong(), C.c_double()
+                _lib.check(L.chs_profile_end(dmp.h, C.byref(ms), C.byref(nl), C.byref(byt)))
+                tot_ms += ms.value
+                tot_b += byt.value
+            t = tot_ms / args.steps
+            row[f"{tag}_us"] = round(t * 1e3, 2)
+            row[f"{tag}_GBps"] = round(tot_b / args.steps / (t * 1e-3) / 1e9, 1)
+            outs[tag] = work
+        row["f32_err_vs_f64"] = rel(outs["f32"], outs["f64"])
+        return row
+
+    pois = time_poisson(dm, "solve_fft_poisson")
     dm.close()
+    # the 3-D FFT library's channel path (poisson_010) on a uniform y mesh of
+    # the same grid (the tanh mesh takes FISHPACK, as in the reference)
+    pois010 = None
+    if nc[1] % 2 == 0:
+        dmu = Domain(nc, 4 * np.pi, -1.0, 1.0, 4 * np.pi / 3, istret=0, dt=5e-4, ren=2800.0, idriven=1,
+                     poisson=FFT_2DECOMP_3DFFT)
+        pois010 = time_poisson(dmu, "poisson_010 (uniform y)")
+        dmu.close()
     # launch-shape choices the autotuner made for these kernels (key=choice
     # and the time of each candidate)
     buf = C.create_string_buffer(1 << 16)
     L.chs_tune_report(buf, len(buf))
     tuned = buf.value.decode().splitlines()
-    print(json.dumps({"ops_bench": res, "d2": d2, "poisson": pois, "grid": list(nc),
+    print(json.dumps({"ops_bench": res, "d2": d2, "poisson": pois, "poisson_010": pois010, "grid": list(nc),
                       "inputs": "seeded band-limited random fields (modes |k_i| <= N_i/3, |k|^-5/3 spectrum, "
                                 f"seed 20261202, generated in {gen_s:.1f} s); zero-mean Poisson right-hand side",
                       "applications_each": args.steps, "autotuned": tuned}))

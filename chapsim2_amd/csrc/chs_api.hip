@@ -151,9 +151,13 @@ int chs_create(const chs_desc_t* desc, chs_handle_t** out) {
   }
   h.mesh.build(desc->nc, desc->periodic_y != 0, desc->lxx, desc->lyb, desc->lyt, desc->lzz, desc->istret,
                desc->rstret);
-  if (!h.mesh.periodic[1] && desc->poisson != CHS_POISSON_FISHPACK) {
+  if (!h.mesh.periodic[1] && desc->poisson != CHS_POISSON_FISHPACK && desc->poisson != CHS_POISSON_3DFFT) {
     delete H;
-    return fail(CHS_EUNSUP, "walls in y need the FFT(x,z)+TDMA(y) Poisson path");
+    return fail(CHS_EUNSUP, "unknown Poisson path");
+  }
+  if (!h.mesh.periodic[1] && desc->poisson == CHS_POISSON_3DFFT && desc->projection == CHS_PROJ_EXACT) {
+    delete H;
+    return fail(CHS_EUNSUP, "the exact projection replaces the Poisson solve: use it with the FISHPACK set-up");
   }
   if (h.mesh.periodic[1] && desc->poisson != CHS_POISSON_3DFFT) {
     delete H;

@@ -17,9 +17,10 @@ class Poisson {
  public:
   // kind: 2 = FFT(x,z)+TDMA(y) with the 2nd-order y operator of
   // src/poisson_fishpack.f90:213-350 (y walls or periodic y is rejected).
-  // kind 3 = triply periodic 3-D FFT path of poisson_000
-  // (src/poisson_1stderivcomp.f90:1016-1195) with the modified wavenumbers of
-  // the iacc staggered scheme.
+  // kind 3 = 3-D FFT path with the modified wavenumbers of the iacc
+  // staggered scheme: triply periodic, poisson_000
+  // (src/poisson_1stderivcomp.f90:1016-1195); y walls on a uniform mesh,
+  // poisson_010 (:1457-1816, istret = 0; a cosine transform along y).
   static std::string create(const Mesh& m, int kind, int iacc, std::unique_ptr<Poisson>* out);
   ~Poisson();
   // In-place solve on a device array of (nx, ny, nz), Fortran order.
@@ -32,12 +33,15 @@ class Poisson {
   std::string setup_exact(const ExactOps& o);
   std::string solve_exact(double* rhs, hipStream_t s, double in_scale = 1.0);
   std::string solve_exact(float* rhs, hipStream_t s, float in_scale = 1.0f);
-  bool has_exact() const { return V64 != nullptr; }
+  bool has_exact() const { return V64 != nullptr && !dct_y; }
   int nx = 0, ny = 0, nz = 0, nxh = 0;
   int kind = 2;
+  // kind 3 with y walls: the exact-solve tables hold the cosine transform
+  bool dct_y = false;
 
  private:
   static std::string create_3d(const Mesh& m, int iacc, std::unique_ptr<Poisson>* out);
+  static std::string create_010(const Mesh& m, int iacc, std::unique_ptr<Poisson>* out);
   // times the launch-shape variants of this solver's FP64 transforms once
   // per shape (the choices are cached for the process)
   std::string tune(bool ydir);

@@ -1203,6 +1203,7 @@ static std::string upv(const std::vector<double>& v, T** d) {
 
 std::string Poisson::create(const Mesh& m, int kind, int iacc, std::unique_ptr<Poisson>* out) {
   if (kind != 2 && kind != 3) return "poisson: unknown kind";
+  if (kind == 3 && !m.periodic[1]) return create_010(m, iacc, out);
   if (kind == 3) return create_3d(m, iacc, out);
   std::unique_ptr<Poisson> p(new Poisson());
   p->kind = 2;
@@ -1303,6 +1304,91 @@ std::string Poisson::create(const Mesh& m, int kind, int iacc, std::unique_ptr<P
   p->work_bytes = sizeof(double) * 2 * (size_t)p->nxh * ny * p->nz;
   PCHK(hipMalloc(&p->work, p->work_bytes));
   if (!(e = p->tune(false)).empty()) return e;
+  *out = std::move(p);
+  return "";
+}
+
+// Squared modified wavenumber of the staggered first derivative of accuracy
+// iacc (interior periodic coefficients alpha, a, b per unit spacing) at
+// phase w on n cells over length len (waves, src/poisson_1stderivcomp.f90:
+// 2405-2480):  (n/len * (2a sin(w/2) + 2b sin(3w/2)) / (1 + 2 alpha cos w))^2.
+static double mod_k2(int iacc, double w, int n, double len) {
+  const double al = coeff(F_D1C2P, false, 3, 1, BC_PERIODIC, iacc);
+  const double ar = coeff(F_D1C2P, true, 3, 1, BC_PERIODIC, iacc);
+  const double br = coeff(F_D1C2P, true, 3, 2, BC_PERIODIC, iacc);
+  const double wp = (2.0 * ar * std::sin(w * 0.5) + 2.0 * br * std::sin(1.5 * w)) / (1.0 + 2.0 * al * std::cos(w));
+  const double kk = double(n) * wp / len;
+  return kk * kk;
+}
+
+// Channel path of the 3-D FFT library (poisson_010,
+// src/poisson_1stderivcomp.f90:1457-1816, istret = 0: x and z periodic, y
+// walls on a uniform mesh).  The reference reorders y, transforms in 3-D,
+// turns the y transform into a cosine transform by its post-processing
+// (:1557-1578), divides by -(kx'^2 + ky'^2 + kz'^2) with the walled
+// wavenumbers ky' at w = pi j / ny (:2465-2479), and undoes every step; its
+// half-cell rotations in x and z are applied before and undone after a
+// real division, so they cancel.  Per (kx, kz) mode that is
+//   phi = C^-1 diag(1 / -(kx'^2 + ky'^2 + kz'^2)) C r
+// with C the cosine transform (DCT-II) along y, which is the exact solve's
+// pipeline (FFT x, FFT z, two dense products along y, inverse FFTs) with
+// the cosine basis in place of the eigenvectors of D_y G_y:
+//   V[j][k] = cos(pi k (2j+1) / (2 ny)), Vi = diag(1/ny, 2/ny, ...) V^T,
+// eigenvalue -ky'^2(k), and the same split into even (k even) and odd
+// (k odd) halves, whose first-half blocks are inverted by their own
+// orthogonality (sum over j < ny/2 of a product = ny/4, ny/2 for k = 0).
+// The (0, 0, 0) mode is set to zero as the reference does for kxyz < 1e-16.
+std::string Poisson::create_010(const Mesh& m, int iacc, std::unique_ptr<Poisson>* out) {
+  if (!m.periodic[0] || !m.periodic[2]) return "poisson: the channel 3-D FFT path needs periodic x and z";
+  if (m.stretched)
+    return "poisson: the 3-D FFT path on a stretched channel (3FMD mapping with the pentadiagonal y solve) "
+           "is not built";
+  const int ny = m.nc[1];
+  if (ny % 2) return "poisson: the channel 3-D FFT path needs an even ny (the reference's y reorder)";
+  std::unique_ptr<Poisson> p;
+  std::string e = create(m, 2, iacc, &p);  // the FFT(x, z) tables of the FISHPACK path
+  if (!e.empty()) return e;
+  p->kind = 3;
+  p->dct_y = true;
+  const double pi = 2.0 * std::asin(1.0);
+  ExactOps o;
+  o.nx = p->nx; o.ny = ny; o.nz = p->nz; o.nxh = p->nxh;
+  o.lx.resize(o.nxh);
+  for (int k = 0; k < o.nxh; ++k) o.lx[k] = -mod_k2(iacc, 2.0 * pi * k / o.nx, o.nx, m.lxx);
+  o.lz.resize(o.nz);
+  for (int k = 0; k < o.nz; ++k) {
+    const int kk = k <= o.nz / 2 ? k : o.nz - k;
+    o.lz[k] = -mod_k2(iacc, 2.0 * pi * kk / o.nz, o.nz, m.lzz);
+  }
+  const double ly = m.lyt - m.lyb;
+  o.lam.resize(ny);
+  for (int k = 0; k < ny; ++k) o.lam[k] = k == 0 ? 0.0 : -mod_k2(iacc, pi * k / ny, ny, ly);
+  auto cb = [&](int j, int k) { return std::cos(pi * double((long)k * (2 * j + 1) % (4 * ny)) / (2.0 * ny)); };
+  o.V.resize((size_t)ny * ny);
+  o.Vi.resize((size_t)ny * ny);
+  for (int j = 0; j < ny; ++j)
+    for (int k = 0; k < ny; ++k) {
+      o.V[(size_t)j * ny + k] = cb(j, k);
+      o.Vi[(size_t)k * ny + j] = cb(j, k) * (k == 0 ? 1.0 : 2.0) / ny;
+    }
+  o.null_idx = 0;
+  const int h = ny / 2;
+  o.Ve.resize((size_t)h * h); o.Vie.resize((size_t)h * h); o.lame.resize(h);
+  o.Vo.resize((size_t)h * h); o.Vio.resize((size_t)h * h); o.lamo.resize(h);
+  for (int q = 0; q < h; ++q) {
+    o.lame[q] = o.lam[2 * q];
+    o.lamo[q] = o.lam[2 * q + 1];
+    for (int j = 0; j < h; ++j) {
+      const double ce = cb(j, 2 * q), co = cb(j, 2 * q + 1);
+      o.Ve[(size_t)j * h + q] = ce;
+      o.Vo[(size_t)j * h + q] = co;
+      o.Vie[(size_t)q * h + j] = ce / (q == 0 ? 0.5 * ny : 0.25 * ny);
+      o.Vio[(size_t)q * h + j] = co / (0.25 * ny);
+    }
+  }
+  o.null_even = 0;
+  o.sym = true;
+  if (!(e = p->setup_exact(o)).empty()) return e;
   *out = std::move(p);
   return "";
 }
@@ -1430,6 +1516,7 @@ std::string Poisson::solve_3d_t(T* rhs, hipStream_t s, T in_scale) {
 
 template <typename T>
 std::string Poisson::solve_t(T* rhs, hipStream_t s, T in_scale) {
+  if (kind == 3 && dct_y) return solve_exact_t<T>(rhs, s, in_scale);
   if (kind == 3) return solve_3d_t<T>(rhs, s, in_scale);
   Radices rx{}, rz{};
   rx.n = (int)rad_x.size();

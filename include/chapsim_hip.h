@@ -56,8 +56,9 @@ enum { CHS_PREC_FP64 = 0, CHS_PREC_MIXED = 1 };
 /* Poisson path, src/modules.f90:169-173 */
 enum { CHS_POISSON_FISHPACK = 2, CHS_POISSON_3DFFT = 3 };
 /* pressure correction of a sub-step:
- *  REFERENCE - one solve with the 2nd-order operator of the FISHPACK path
- *              (src/poisson_fishpack.f90), as the reference does;
+ *  REFERENCE - one solve with the configured Poisson path (the 2nd-order
+ *              operator of src/poisson_fishpack.f90, or the 3-D FFT path
+ *              with modified wavenumbers), as the reference does;
  *  EXACT     - inverts the composed operator D*G of the configured compact
  *              schemes (FFT in x and z, eigen-basis of the wall-normal block),
  *              so one correction leaves max|div u| at round-off.  No
@@ -104,8 +105,9 @@ int chs_op3d(chs_handle_t *h, int dir, int op, int iacc, const int ibc[2], const
 
 /* Pressure Poisson solve in place on a (nc1,nc2,nc3) array.  Replaces
  * solve_fft_poisson (src/poisson_interface.f90:119-149) ->
- * fishpack_fft_simple (src/poisson_fishpack.f90:354-490) or poisson_000
- * (src/poisson_1stderivcomp.f90:1016-1195). */
+ * fishpack_fft_simple (src/poisson_fishpack.f90:354-490), poisson_000
+ * (src/poisson_1stderivcomp.f90:1016-1195, triply periodic) or poisson_010
+ * (:1457-1816, y walls on a uniform mesh, even nc2; CHS_POISSON_3DFFT). */
 int chs_poisson(chs_handle_t *h, double *rhs, int mem);
 
 /* Device-resident state exchange (the reference keeps qx, qy, qz, pres in

@@ -524,7 +524,8 @@ orc_flow *orc_flow_create(const orc_flow_cfg *cfg) {
   }
   if (cfg->poisson == 3) {
     /* waves, src/poisson_1stderivcomp.f90:2405-2480 (bcx = bcy = bcz = 0,
-     * istret = 0), coefficients of :190-200 */
+     * istret = 0), coefficients of :190-200; with y walls (bcy = 1,
+     * poisson_010) the y wavenumbers follow :2465-2479 below */
     double pi = 2.0 * asin(1.0);
     double alc = orc_coeff(ORC_FAM_D1C2P, 0, 3, 1, BPER, cfg->iacc);
     double ar = orc_coeff(ORC_FAM_D1C2P, 1, 3, 1, BPER, cfg->iacc);
@@ -544,6 +545,18 @@ orc_flow *orc_flow_create(const orc_flow_cfg *cfg) {
         t[i - 1] = kk * kk;
       }
       for (int i = n / 2 + 2; i <= n; ++i) t[i - 1] = t[n - i + 1];
+      if (d == 1 && !dm->periodic[1]) {
+        /* bcy = 1 (:2465-2479): w = pi (j-1) / nym over all ny modes, nym =
+         * the y cell count (np_geo(2) - 1, :150-156), yk2(1) = 0 */
+        for (int j = 1; j <= n; ++j) {
+          double w = pi * (double)(j - 1) / (double)n;
+          double wp = ac * 2.0 * h * sin(w * 0.5) + bc * 2.0 * h * sin(1.5 * w);
+          wp = wp / (1.0 + 2.0 * alc * cos(w));
+          double kk = (double)n * wp / len[d];
+          t[j - 1] = kk * kk;
+        }
+        t[0] = 0.0;
+      }
       *k2[d] = t;
     }
   }
@@ -924,10 +937,155 @@ static void poisson_000(orc_flow *f, double *rhs) {
   free(c);
 }
 
+/* poisson_010, src/poisson_1stderivcomp.f90:1457-1816, istret = 0: x and z
+ * periodic, y walls (bcy = 1) on a uniform y mesh, ny = ny_global - 1 = the
+ * y cell count (decomp_2d_init on np = nc + 1 for the walled direction,
+ * src/domain_decomposition.f90:150, src/geometry.f90:435-444; :818-826).
+ * Steps, in the reference's order (1-based indices in the comments):
+ *  1. y reorder (:1490-1500): b(j) = r(2j-1) for j <= ny/2, b(j) = r(2ny-2j+2)
+ *     above;
+ *  2. decomp_2d_fft_3d with PHYSICAL_IN_Z: r2c along z (modes 0..nz/2),
+ *     c2c along y, c2c along x (e^{-i...}); / (nx ny nz) (:1503-1511);
+ *  3. z post-processing: c * (bz - i az), az/bz = sin/cos(pi (k-1)/nz)
+ *     (abxyz :2343-2401, bcz = 0) (:1524-1536);
+ *  4. x post-processing: c * (bx - i ax), then -c for i > nx/2+1 (:1539-1552);
+ *  5. y post-processing (:1557-1578): X(1) = V(1); X(j) = 1/2 [e^{-i th_j}
+ *     V(j) + e^{+i th_j} V(ny-j+2)], th_j = pi (j-1) / (2 ny) (ay, by with
+ *     bcy = 1);
+ *  6. istret = 0 division (:1591-1612): the real part by -Re(kxyz), the
+ *     imaginary part by -Im(kxyz), kxyz = (1,1) (xk2 + yk2 + zk2); zero
+ *     where both are below 1e-16;
+ *  7. backward y post-processing (:1719-1740): V(1) = X(1); V(j) =
+ *     (by + i ay)(X(j) - i X(ny-j+2)) written out as in the reference;
+ *  8. x, then z post-processing back (:1757-1786): c * (b + i a), the x sign
+ *     flip again;
+ *  9. inverse transform c2c x, c2c y, c2r z (e^{+i...}, unnormalised), and
+ *     the y un-reorder r(2j-1) = b(j), r(2j) = b(ny-j+1) (:1792-1808).
+ * The c2r along z rebuilds modes nz/2+1..nz-1 as conjugates of the stored
+ * ones and takes the real part, as a c2r transform does. */
+static void poisson_010(orc_flow *f, double *rhs) {
+  const int nx = f->nx, ny = f->ny, nz = f->nz, nzh = nz / 2 + 1;
+  const double pi = 2.0 * asin(1.0);
+  const long nsp = (long)nx * ny * nzh;
+  cpx *c = (cpx *)malloc(sizeof(cpx) * (size_t)nsp);
+  double *b = (double *)malloc(sizeof(double) * (size_t)nx * ny * nz);
+  /* 1. reorder */
+  for (int k = 0; k < nz; ++k)
+    for (int i = 0; i < nx; ++i) {
+      for (int j = 1; j <= ny / 2; ++j)
+        b[i + (long)nx * ((j - 1) + (long)ny * k)] = rhs[i + (long)nx * ((2 * (j - 1)) + (long)ny * k)];
+      for (int j = ny / 2 + 1; j <= ny; ++j)
+        b[i + (long)nx * ((j - 1) + (long)ny * k)] = rhs[i + (long)nx * ((2 * ny - 2 * j + 1) + (long)ny * k)];
+    }
+  int nmax = nx > ny ? nx : ny;
+  nmax = nmax > nz ? nmax : nz;
+  cpx *line = (cpx *)malloc(sizeof(cpx) * (size_t)nmax);
+  /* 2. r2c z, c2c y, c2c x; c indexed (i, j, k) with k < nzh */
+  for (long L = 0; L < (long)nx * ny; ++L) {
+    for (int k = 0; k < nz; ++k) { line[k].re = b[L + (long)nx * ny * k]; line[k].im = 0.0; }
+    cfft(line, nz, -1);
+    for (int k = 0; k < nzh; ++k) c[L + (long)nx * ny * k] = line[k];
+  }
+  for (long L = 0; L < (long)nx * nzh; ++L) {
+    long i = L % nx, k = L / nx;
+    for (int j = 0; j < ny; ++j) line[j] = c[i + (long)nx * (j + (long)ny * k)];
+    cfft(line, ny, -1);
+    for (int j = 0; j < ny; ++j) c[i + (long)nx * (j + (long)ny * k)] = line[j];
+  }
+  for (long L = 0; L < (long)ny * nzh; ++L) cfft(c + L * nx, nx, -1);
+  const double sc = 1.0 / (double)nx / (double)ny / (double)nz;
+  for (long t = 0; t < nsp; ++t) { c[t].re *= sc; c[t].im *= sc; }
+  /* 3. z, 4. x post-processing */
+  for (long t = 0; t < nsp; ++t) {
+    long i = t % nx, k = t / ((long)nx * ny);
+    double az = sin((double)k * pi / (double)nz), bz = cos((double)k * pi / (double)nz);
+    double t1 = c[t].re, t2 = c[t].im;
+    c[t].re = t1 * bz + t2 * az;
+    c[t].im = t2 * bz - t1 * az;
+    double ax = sin((double)i * pi / (double)nx), bx = cos((double)i * pi / (double)nx);
+    t1 = c[t].re; t2 = c[t].im;
+    c[t].re = t1 * bx + t2 * ax;
+    c[t].im = t2 * bx - t1 * ax;
+    if (i + 1 > nx / 2 + 1) { c[t].re = -c[t].re; c[t].im = -c[t].im; }
+  }
+  /* 5. y post-processing, 6. division, 7. backward y post-processing */
+  cpx *v = (cpx *)malloc(sizeof(cpx) * (size_t)ny);
+  cpx *x = (cpx *)malloc(sizeof(cpx) * (size_t)ny);
+  for (long L = 0; L < (long)nx * nzh; ++L) {
+    long i = L % nx, k = L / nx;
+    for (int j = 0; j < ny; ++j) v[j] = c[i + (long)nx * (j + (long)ny * k)];
+    x[0] = v[0];
+    for (int j = 2; j <= ny; ++j) {
+      double ay = sin((double)(j - 1) * pi * 0.5 / (double)ny), by = cos((double)(j - 1) * pi * 0.5 / (double)ny);
+      double t1 = v[j - 1].re, t2 = v[j - 1].im, t3 = v[ny - j + 1].re, t4 = v[ny - j + 1].im;
+      double x1 = t1 * by, x2 = t1 * ay, x3 = t2 * by, x4 = t2 * ay;
+      double x5 = t3 * by, x6 = t3 * ay, x7 = t4 * by, x8 = t4 * ay;
+      x[j - 1].re = 0.5 * (x1 + x4 + x5 - x8);
+      x[j - 1].im = 0.5 * (-x2 + x3 + x6 + x7);
+    }
+    for (int j = 0; j < ny; ++j) {
+      double kk = f->kx2[i] + f->ky2[j] + f->kz2[k];
+      if (fabs(kk) < 1.0e-16) { x[j].re = 0.0; x[j].im = 0.0; }
+      else { x[j].re = x[j].re / (-kk); x[j].im = x[j].im / (-kk); }
+    }
+    v[0] = x[0];
+    for (int j = 2; j <= ny; ++j) {
+      double ay = sin((double)(j - 1) * pi * 0.5 / (double)ny), by = cos((double)(j - 1) * pi * 0.5 / (double)ny);
+      double t1 = x[j - 1].re, t2 = x[j - 1].im, t3 = x[ny - j + 1].re, t4 = x[ny - j + 1].im;
+      double x1 = t1 * by, x2 = t1 * ay, x3 = t2 * by, x4 = t2 * ay;
+      double x5 = t3 * by, x6 = t3 * ay, x7 = t4 * by, x8 = t4 * ay;
+      v[j - 1].re = x1 - x4 + x6 + x7;
+      v[j - 1].im = -(-x2 - x3 + x5 - x8);
+    }
+    for (int j = 0; j < ny; ++j) c[i + (long)nx * (j + (long)ny * k)] = v[j];
+  }
+  free(v);
+  free(x);
+  /* 8. x, z post-processing back */
+  for (long t = 0; t < nsp; ++t) {
+    long i = t % nx, k = t / ((long)nx * ny);
+    double ax = sin((double)i * pi / (double)nx), bx = cos((double)i * pi / (double)nx);
+    double t1 = c[t].re, t2 = c[t].im;
+    c[t].re = t1 * bx - t2 * ax;
+    c[t].im = t2 * bx + t1 * ax;
+    if (i + 1 > nx / 2 + 1) { c[t].re = -c[t].re; c[t].im = -c[t].im; }
+    double az = sin((double)k * pi / (double)nz), bz = cos((double)k * pi / (double)nz);
+    t1 = c[t].re; t2 = c[t].im;
+    c[t].re = t1 * bz - t2 * az;
+    c[t].im = t2 * bz + t1 * az;
+  }
+  /* 9. c2c x, c2c y, c2r z, un-reorder */
+  for (long L = 0; L < (long)ny * nzh; ++L) cfft(c + L * nx, nx, 1);
+  for (long L = 0; L < (long)nx * nzh; ++L) {
+    long i = L % nx, k = L / nx;
+    for (int j = 0; j < ny; ++j) line[j] = c[i + (long)nx * (j + (long)ny * k)];
+    cfft(line, ny, 1);
+    for (int j = 0; j < ny; ++j) c[i + (long)nx * (j + (long)ny * k)] = line[j];
+  }
+  for (long L = 0; L < (long)nx * ny; ++L) {
+    for (int k = 0; k < nzh; ++k) line[k] = c[L + (long)nx * ny * k];
+    line[0].im = 0.0;
+    if (nz % 2 == 0) line[nz / 2].im = 0.0;
+    for (int k = nzh; k < nz; ++k) { line[k].re = line[nz - k].re; line[k].im = -line[nz - k].im; }
+    cfft(line, nz, 1);
+    for (int k = 0; k < nz; ++k) b[L + (long)nx * ny * k] = line[k].re;
+  }
+  for (int k = 0; k < nz; ++k)
+    for (int i = 0; i < nx; ++i)
+      for (int j = 1; j <= ny / 2; ++j) {
+        rhs[i + (long)nx * ((2 * j - 2) + (long)ny * k)] = b[i + (long)nx * ((j - 1) + (long)ny * k)];
+        rhs[i + (long)nx * ((2 * j - 1) + (long)ny * k)] = b[i + (long)nx * ((ny - j) + (long)ny * k)];
+      }
+  free(line);
+  free(b);
+  free(c);
+}
+
 void orc_poisson_solve(orc_flow *f, double *rhs) {
   if (f->cfg.projection == 1) poisson_exact(f, rhs);
   else if (f->cfg.poisson == 2) poisson_fishpack(f, rhs);
-  else if (f->cfg.poisson == 3) poisson_000(f, rhs);
+  else if (f->cfg.poisson == 3 && f->dm.periodic[1]) poisson_000(f, rhs);
+  else if (f->cfg.poisson == 3) poisson_010(f, rhs);
 }
 
 /* ---------------------------------------------------------------------- */

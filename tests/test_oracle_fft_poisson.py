@@ -170,6 +170,27 @@ def test_poisson_000_single_mode(po):
     assert np.abs(phi0).max() < 1e-14
 
 
+@pytest.mark.parametrize("iacc,nc", [(4, (16, 12, 10)), (4, (12, 16, 15)), (1, (10, 8, 12)), (3, (8, 20, 9))])
+def test_poisson_010_is_poisson_000_of_the_mirrored_channel(po, iacc, nc):
+    """poisson_010 (uniform channel, y walls): the right-hand side mirrored
+    across a wall (2 ny cells over 2 ly, periodic) and solved by the pinned
+    triply periodic poisson_000 gives, on the first ny rows, the channel
+    solution. The half-cell mirror is the cosine series of the reference's
+    reordered transform, and both divide by the same modified wavenumbers,
+    so a slip in the reorder, the y post-processing, the z half spectrum or
+    the walled y wavenumbers breaks the equality (odd nz included)."""
+    nx, ny, nz = nc
+    lx, lz = 2.0, 1.5
+    ch = po.Flow(list(nc), lx, -1.0, 1.0, lz, istret=0, wall_y=True, iacc=iacc, poisson=3)
+    box = po.Flow([nx, 2 * ny, nz], lx, -1.0, 3.0, lz, istret=0, wall_y=False, iacc=iacc, poisson=3)
+    rng = np.random.default_rng(sum(nc) + iacc)
+    r = rng.standard_normal(nc)
+    p = ch.poisson(np.asfortranarray(r))
+    q = box.poisson(np.asfortranarray(np.concatenate([r, r[:, ::-1, :]], axis=1)))
+    assert np.abs(q[:, :ny, :] - q[:, ::-1, :][:, :ny, :]).max() < 1e-12 * np.abs(q).max()
+    assert np.abs(p - q[:, :ny, :]).max() < 1e-12 * np.abs(q).max(), np.abs(p - q[:, :ny, :]).max()
+
+
 def test_tgv_early_viscous_decay(po):
     """Taylor-Green vortex (src/initialisation.f90:892-975): for t << 1 the
     kinetic energy decays at the viscous rate 2 nu |k|^2 = 6/Re."""
