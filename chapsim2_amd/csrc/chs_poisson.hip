@@ -1027,17 +1027,20 @@ __global__ void __launch_bounds__(256) k_ymix_fused(const T* __restrict__ Vie, c
 // stages in one pass.  v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16]
 // and B[l/16][l%16], and register v of lane l holds D[4v + l/16][l%16].
 // Workgroup = 8 waves; wave w owns rows 16*(w%4) .. +16 of half w/4 (even,
-// odd) and reads that row block of Vi (then of V) into registers from
-// L2 before each phase.  It loops over tiles of 64 columns: fold into LDS, D = Vi B (16
-// K-steps x 4 column blocks), scale by 1/(lam + lx + lz), back to LDS, D = V Y,
-// back to LDS, unfold into the output (in place: a tile is read before it is
+// odd) and reads that row block of Vi (then of V) from L2 one K-step ahead
+// of its products, so that the kernel fits 128 registers and two workgroups
+// share a CU (one's loads and barriers overlap the other's products; holding
+// the whole row block in registers allowed one workgroup per CU and was
+// slower).  Per tile of 64 columns: fold into LDS, D = Vi B (16 K-steps x 4
+// column blocks), scale by 1/(lam + lx + lz), back to LDS, D = V Y, back to
+// LDS, unfold into the output (in place: a tile is read before it is
 // written, and no other workgroup touches it).
 typedef double chs_d4 __attribute__((ext_vector_type(4)));
 constexpr int kMmC = 64;            // columns per tile
 constexpr int kMmW = kMmC + 1;      // LDS row stride
 constexpr int kMmWaves = 8;
 
-__global__ void __launch_bounds__(64 * kMmWaves) k_ymix_mfma(const double* __restrict__ Vie,
+__global__ void __launch_bounds__(64 * kMmWaves, 4) k_ymix_mfma(const double* __restrict__ Vie,
                                                              const double* __restrict__ Vio,
                                                              const double* __restrict__ Ve,
                                                              const double* __restrict__ Vo, double* W, int ny,
@@ -1051,14 +1054,20 @@ __global__ void __launch_bounds__(64 * kMmWaves) k_ymix_mfma(const double* __res
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int half = w >> 2, rb = w & 3;
   const int li = lane & 15, lk = lane >> 4;
-  // row block of a matrix for this wave: a[ks] = M[16 rb + li][4 ks + lk]
-  // (read from L2 before each phase)
-  auto load_block = [&](const double* M, double (&a)[16]) {
-    const int i = 16 * rb + li;
-#pragma unroll
+  // D += M[16 rb + li][4 ks + lk] B[4 ks + lk][column] over the 16 K-steps
+  // and the 4 column blocks, M read from L2 (zero outside h x h)
+  const int ia = 16 * rb + li;
+  auto products = [&](const double* M, chs_d4 (&acc)[4]) {
+    const double* arow = M + (long)(ia < h ? ia : 0) * h + lk;
+#pragma unroll 4
     for (int ks = 0; ks < 16; ++ks) {
-      const int k = 4 * ks + lk;
-      a[ks] = (i < h && k < h) ? M[(long)i * h + k] : 0.0;
+      const double av = (ia < h && 4 * ks + lk < h) ? arow[4 * ks] : 0.0;
+      const double* brow = Bs + half * 64 * kMmW + (4 * ks + lk) * kMmW + li;
+      double bv[4];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) bv[cb] = brow[16 * cb];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[cb], acc[cb], 0, 0, 0);
     }
   };
   const double* Mi = half ? Vio : Vie;
@@ -1066,9 +1075,8 @@ __global__ void __launch_bounds__(64 * kMmWaves) k_ymix_mfma(const double* __res
   const double* lam = half ? lamo : lame;
   double* Bh = Bs + half * 64 * kMmW;
   const long ntile = (ncol + kMmC - 1) / kMmC;
-  // the fold's inputs (rows j and ny-1-j of column g0 + tid % 64, j = tid / 64
-  // + 8u) of a workgroup's next tile are loaded while it finishes the current
-  // one (clamped to valid rows and columns, so every address is in range)
+  // the fold's inputs: rows j and ny-1-j of column g0 + tid % 64, j = tid / 64
+  // + 8u (clamped to valid rows and columns, so every address is in range)
   double pa[8], pb[8];
   auto fold_load = [&](long g0n) {
     const long g = g0n + (tid & 63);
@@ -1083,9 +1091,9 @@ __global__ void __launch_bounds__(64 * kMmWaves) k_ymix_mfma(const double* __res
       pb[u] = W[base + (long)ncr * (ny - 1 - jc)];
     }
   };
-  if ((long)blockIdx.x < ntile) fold_load((long)blockIdx.x * kMmC);
   for (long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
     const long g0 = tile * kMmC;
+    fold_load(g0);
     // ---- fold: 64 columns x 64 row pairs, 8 per thread
     {
       const int c = tid & 63;
@@ -1100,21 +1108,10 @@ __global__ void __launch_bounds__(64 * kMmWaves) k_ymix_mfma(const double* __res
     }
     __syncthreads();
     // ---- D = Vi B over this wave's 16 rows and the 64 columns
-    double a1[16];
-    load_block(Mi, a1);
     chs_d4 acc[4];
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) acc[cb] = chs_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 2
-    for (int ks = 0; ks < 16; ++ks) {
-      const double* brow = Bh + (4 * ks + lk) * kMmW + li;
-      double bv[4];
-#pragma unroll
-      for (int cb = 0; cb < 4; ++cb) bv[cb] = brow[16 * cb];
-#pragma unroll
-      for (int cb = 0; cb < 4; ++cb)
-        acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[ks], bv[cb], acc[cb], 0, 0, 0);
-    }
+    products(Mi, acc);
     __syncthreads();  // every wave has read the folded block
     // ---- scale by 1/(lam + lx + lz) (null mode to 0) and store Y
 #pragma unroll
@@ -1137,23 +1134,9 @@ __global__ void __launch_bounds__(64 * kMmWaves) k_ymix_mfma(const double* __res
     }
     __syncthreads();
     // ---- D = V Y
-    double a2[16];
-    load_block(Mv, a2);
-    // (issued after every other load of this tile, so waiting for those does
-    // not wait for these)
-    if (tile + gridDim.x < ntile) fold_load((tile + gridDim.x) * kMmC);
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) acc[cb] = chs_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 2
-    for (int ks = 0; ks < 16; ++ks) {
-      const double* brow = Bh + (4 * ks + lk) * kMmW + li;
-      double bv[4];
-#pragma unroll
-      for (int cb = 0; cb < 4; ++cb) bv[cb] = brow[16 * cb];
-#pragma unroll
-      for (int cb = 0; cb < 4; ++cb)
-        acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[ks], bv[cb], acc[cb], 0, 0, 0);
-    }
+    products(Mv, acc);
     __syncthreads();  // every wave has read Y
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb)
@@ -1592,15 +1575,9 @@ std::string Poisson::solve_exact_t(T* rhs, hipStream_t s, T in_scale) {
   const int ncr = 2 * nxh;
   const long ncol = (long)ncr * nz;
   if (Ve64 && dbl && ny / 2 <= 64) {
-    // one workgroup per CU (the kernel's registers allow one per CU), each
-    // looping over its tiles with the next tile's loads overlapping the
-    // current one's products
+    // one workgroup per tile of 64 columns (two share a CU)
     const long ntile = (ncol + kMmC - 1) / kMmC;
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
-      cus = 256;
-    const unsigned nwg = (unsigned)(ntile < cus ? ntile : cus);
+    const unsigned nwg = (unsigned)ntile;
     hipLaunchKernelGGL(k_ymix_mfma, dim3(nwg), dim3(64 * kMmWaves), 0, s, Vie64, Vio64, Ve64, Vo64,
                        reinterpret_cast<double*>(Wr), ny, ncr, ncol, lame64, lamo64,
                        reinterpret_cast<const double*>(lxv), reinterpret_cast<const double*>(lzv), null_even);

@@ -259,10 +259,10 @@ def test_batched_channels_vs_oracle(po, i):
 
 
 def test_exact_projection_multi_tile_ymix(po):
-    """(256, 24, 256): 1032 y-mixing tiles of 64 spectrum columns for at most
-    256 persistent workgroups, so every workgroup loops over several tiles
-    and prefetches the next one; the initial step and one RK3 step against
-    the oracle's per-mode dense solve."""
+    """(256, 24, 256): 1032 y-mixing tiles of 64 spectrum columns (more
+    workgroups than the CUs hold at once, and ny / 2 = 12 rows of the
+    64-row blocks in use); the initial step and one RK3 step against the
+    oracle's per-mode dense solve."""
     from chapsim2_amd.domain import PROJ_EXACT, Domain
     from chapsim2_amd.initial import channel_fields
 
