@@ -58,7 +58,7 @@ struct RhsWork {
 
 // The energy right-hand side's operands in the precision T its operators
 // run in: h, k and T at the cell centres (FP64: the Flow's own fields; FP32:
-// rounded copies made at the start of each energy sub-step), the x, y, z
+// rounded copies, stored by the property refresh), the x, y, z
 // parts of the right-hand side, h, k and dT/dx at the faces, and the wall
 // rows (nx,4,nz) of h, k, T and of the conductive flux k dT/dy.
 template <typename T>
@@ -121,6 +121,9 @@ class Flow {
   // mixed form: the FP32 copies of g (w32) were made by the energy sub-step
   // of this sub-step, so the momentum right-hand side need not remake them
   bool g32_fresh = false;
+  // mixed form: the FP32 copies of h, k, T (e32) and mu (w32) equal the
+  // rounded FP64 fields (the last property refresh stored them)
+  bool props32_fresh = false;
   int* props_flag = nullptr;
   std::string copy_rhs(double* mx, double* my, double* mz, int mem);
 

@@ -942,7 +942,7 @@ std::string Flow::momentum_rhs(int isub, bool fuse) {
     hipStream_t s = H->stream;
     for (auto pr2 : {std::make_pair(&w32.gx, &w64.gx), std::make_pair(&w32.gy, &w64.gy),
                      std::make_pair(&w32.gz, &w64.gz), std::make_pair(&w32.visc, &w64.visc)})
-      if (!g32_fresh || pr2.first == &w32.visc)
+      if (pr2.first == &w32.visc ? !props32_fresh : !g32_fresh)
         hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr2.second->size()), dim3(256), 0, s, pr2.first->p,
                            pr2.second->p, pr2.second->size());
     g32_fresh = false;
@@ -1294,6 +1294,7 @@ std::string Flow::thermo_enable(const chs_thermo_desc_t& d) {
   hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(256), 0, s, hent.p, p0[0], n);
   hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(256), 0, s, kcond.p, p0[3], n);
   hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(256), 0, s, temp.p, p0[4], n);
+  props32_fresh = false;  // the fills left the FP32 roundings unset
   thermo = true;
   update_re(0);
   SCHK(convert_qg(true));
@@ -1303,9 +1304,22 @@ std::string Flow::thermo_enable(const chs_thermo_desc_t& d) {
 
 std::string Flow::refresh_props() {
   // Update_thermal_properties, src/eq_energy.f90:14-47 (dDensm1 = dDens
-  // after the refresh, so Calculate_drhodt is zero and is not formed)
-  return props_refresh_dh(props, props_mode, rhoh.p, rhoh.size(), hent.p, dens.p, w64.visc.p, kcond.p, temp.p,
-                          props_flag, H->stream);
+  // after the refresh, so Calculate_drhodt is zero and is not formed).  The
+  // mixed form also gets the FP32 roundings of h, mu, k, T that its energy
+  // and momentum right-hand sides read: stored by the table kernel itself,
+  // rounded after the surrogate.
+  float* const r32[4] = {e32.hent.p, w32.visc.p, e32.kcond.p, e32.temp.p};
+  SCHK(props_refresh_dh(props, props_mode, rhoh.p, rhoh.size(), hent.p, dens.p, w64.visc.p, kcond.p, temp.p,
+                        props_flag, H->stream, mixed ? r32 : nullptr));
+  if (mixed && props_mode != CHS_PROPS_TABLE)
+    for (auto pr2 : {std::make_pair(&e32.hent, &hent), std::make_pair(&w32.visc, &w64.visc),
+                     std::make_pair(&e32.kcond, &kcond), std::make_pair(&e32.temp, &temp)})
+      hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr2.second->size()), dim3(256), 0, H->stream, pr2.first->p,
+                         pr2.second->p, pr2.second->size());
+  props32_fresh = mixed;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("refresh_props: ") + hipGetErrorString(e);
+  return "";
 }
 
 std::string Flow::thermo_set_rhoh(const double* r, int mem, long* clamped) {
@@ -1380,13 +1394,14 @@ std::string Flow::energy_terms() {
   set_lanes();
   if (!mixed) return energy_terms_t<double>(w64, e64);
   // rounded copies of g (kept for this sub-step's momentum right-hand side)
-  // and of the cell-centre h, k, T of the last property refresh
+  // and, unless the last property refresh stored them, of h, k, T
   hipStream_t s = H->stream;
   for (auto pr2 : {std::make_pair(&w32.gx, &w64.gx), std::make_pair(&w32.gy, &w64.gy),
                    std::make_pair(&w32.gz, &w64.gz), std::make_pair(&e32.hent, &hent),
                    std::make_pair(&e32.kcond, &kcond), std::make_pair(&e32.temp, &temp)})
-    hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr2.second->size()), dim3(256), 0, s, pr2.first->p,
-                       pr2.second->p, pr2.second->size());
+    if (!props32_fresh || pr2.first == &w32.gx || pr2.first == &w32.gy || pr2.first == &w32.gz)
+      hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr2.second->size()), dim3(256), 0, s, pr2.first->p,
+                         pr2.second->p, pr2.second->size());
   return energy_terms_t<float>(w32, e32);
 }
 

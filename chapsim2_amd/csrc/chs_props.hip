@@ -72,7 +72,9 @@ __device__ __forceinline__ void props_dh_cell(const double* crh, const double* c
                                               double v, long c, double* __restrict__ oh, double* __restrict__ od,
                                               double* __restrict__ om, double* __restrict__ ok,
                                               double* __restrict__ ot, double* __restrict__ ob,
-                                              double* __restrict__ ocp, int* __restrict__ flag) {
+                                              double* __restrict__ ocp, int* __restrict__ flag,
+                                              float* __restrict__ fh, float* __restrict__ fm,
+                                              float* __restrict__ fk, float* __restrict__ ft) {
   if (v < dhmin) {
     v = dhmin;
     atomicAdd(flag, 1);
@@ -101,9 +103,22 @@ __device__ __forceinline__ void props_dh_cell(const double* crh, const double* c
   auto interp = [&](int col) { return w1 * tab[(long)col * n + i1] + w2 * tab[(long)col * n + i2]; };
   if (oh) oh[c] = h;
   if (od) od[c] = w1 * cd[i1] + w2 * cd[i2];
-  if (om) om[c] = interp(CM);
-  if (ok) ok[c] = interp(CK);
-  if (ot) ot[c] = interp(CT);
+  if (om || fm) {
+    const double m = interp(CM);
+    if (om) om[c] = m;
+    if (fm) fm[c] = (float)m;
+  }
+  if (ok || fk) {
+    const double k = interp(CK);
+    if (ok) ok[c] = k;
+    if (fk) fk[c] = (float)k;
+  }
+  if (ot || ft) {
+    const double t = interp(CT);
+    if (ot) ot[c] = t;
+    if (ft) ft[c] = (float)t;
+  }
+  if (fh) fh[c] = (float)h;
   if (ob) ob[c] = interp(CB);
   if (ocp) ocp[c] = interp(CCP);
 }
@@ -114,11 +129,13 @@ __global__ void __launch_bounds__(256) k_props_from_dh(const double* __restrict_
                                                        double* __restrict__ oh, double* __restrict__ od,
                                                        double* __restrict__ om, double* __restrict__ ok,
                                                        double* __restrict__ ot, double* __restrict__ ob,
-                                                       double* __restrict__ ocp, int* __restrict__ flag) {
+                                                       double* __restrict__ ocp, int* __restrict__ flag,
+                                                       float* __restrict__ fh, float* __restrict__ fm,
+                                                       float* __restrict__ fk, float* __restrict__ ft) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
   props_dh_cell(tab + (long)CRH * n, tab + (long)CH * n, tab + (long)CD * n, tab, n, hrise, dhmin, dhmax, rhoh_in[c], c,
-                oh, od, om, ok, ot, ob, ocp, flag);
+                oh, od, om, ok, ot, ob, ocp, flag, fh, fm, fk, ft);
 }
 
 // The bisection columns (rho*h and h, and d when NC is 3: NC n doubles)
@@ -131,7 +148,8 @@ __global__ void __launch_bounds__(kPropsBlock) k_props_from_dh_lds(
     const double* __restrict__ tab, int n, int hrise, double dhmin, double dhmax, const double* __restrict__ rhoh_in,
     long N,
     double* __restrict__ oh, double* __restrict__ od, double* __restrict__ om, double* __restrict__ ok,
-    double* __restrict__ ot, double* __restrict__ ob, double* __restrict__ ocp, int* __restrict__ flag) {
+    double* __restrict__ ot, double* __restrict__ ob, double* __restrict__ ocp, int* __restrict__ flag,
+    float* __restrict__ fh, float* __restrict__ fm, float* __restrict__ fk, float* __restrict__ ft) {
   extern __shared__ double sh[];
   const double* crh = tab + (long)CRH * n;
   const double* ch = tab + (long)CH * n;
@@ -145,7 +163,8 @@ __global__ void __launch_bounds__(kPropsBlock) k_props_from_dh_lds(
   const double* sd = NC == 3 ? sh + 2 * n : cd;
   const long stride = (long)gridDim.x * blockDim.x;
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += stride)
-    props_dh_cell(sh, sh + n, sd, tab, n, hrise, dhmin, dhmax, rhoh_in[c], c, oh, od, om, ok, ot, ob, ocp, flag);
+    props_dh_cell(sh, sh + n, sd, tab, n, hrise, dhmin, dhmax, rhoh_in[c], c, oh, od, om, ok, ot, ob, ocp, flag, fh,
+                  fm, fk, ft);
 }
 
 // ftp_refresh_thermal_properties_from_H per cell (no clamping, as the
@@ -184,9 +203,12 @@ __global__ void __launch_bounds__(256) k_props_from_h(const double* __restrict__
 // Launches the from-rho*h kernel for n cells: the persistent LDS form with as
 // many bisection columns as fit a workgroup's LDS (three or two), else the
 // global form.
-// o holds the seven outputs (h, d, m, k, t, b, cp), each possibly null.
+// o holds the seven outputs (h, d, m, k, t, b, cp), each possibly null; f
+// the FP32 roundings of h, m, k, t (or null).
 hipError_t launch_from_dh(const chs_props* p, const double* rhoh, long n, double* const o[7], int* flag,
-                          hipStream_t s) {
+                          hipStream_t s, float* const* f = nullptr) {
+  float* const fz[4] = {nullptr, nullptr, nullptr, nullptr};
+  if (!f) f = fz;
   const size_t col = sizeof(double) * (size_t)p->n;
   const int hrise = p->h_rising ? 1 : 0;
   const int nc = 3 * col <= p->lds_block ? 3 : 2 * col <= p->lds_block ? 2 : 0;
@@ -196,14 +218,14 @@ hipError_t launch_from_dh(const chs_props* p, const double* rhoh, long n, double
     const unsigned grid = (unsigned)(want < cap ? want : cap);
     if (nc == 3)
       hipLaunchKernelGGL(k_props_from_dh_lds<3>, dim3(grid), dim3(kPropsBlock), 3 * col, s, p->dev, p->n, hrise, p->dhmin,
-                         p->dhmax, rhoh, n, o[0], o[1], o[2], o[3], o[4], o[5], o[6], flag);
+                         p->dhmax, rhoh, n, o[0], o[1], o[2], o[3], o[4], o[5], o[6], flag, f[0], f[1], f[2], f[3]);
     else
       hipLaunchKernelGGL(k_props_from_dh_lds<2>, dim3(grid), dim3(kPropsBlock), 2 * col, s, p->dev, p->n, hrise, p->dhmin,
-                         p->dhmax, rhoh, n, o[0], o[1], o[2], o[3], o[4], o[5], o[6], flag);
+                         p->dhmax, rhoh, n, o[0], o[1], o[2], o[3], o[4], o[5], o[6], flag, f[0], f[1], f[2], f[3]);
   } else {
     const unsigned grid = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_props_from_dh, dim3(grid), dim3(256), 0, s, p->dev, p->n, hrise, p->dhmin, p->dhmax, rhoh, n,
-                       o[0], o[1], o[2], o[3], o[4], o[5], o[6], flag);
+                       o[0], o[1], o[2], o[3], o[4], o[5], o[6], flag, f[0], f[1], f[2], f[3]);
   }
   return hipGetLastError();
 }
@@ -212,7 +234,7 @@ hipError_t launch_from_dh(const chs_props* p, const double* rhoh, long n, double
 
 namespace chs {
 std::string props_refresh_dh(const chs_props* p, int mode, const double* rhoh, long n, double* h, double* d,
-                             double* m, double* k, double* t, int* flag, hipStream_t s) {
+                             double* m, double* k, double* t, int* flag, hipStream_t s, float* const* r32) {
   if (n <= 0) return "";
   if (mode == CHS_PROPS_MLP) {
     if (!p->surrogate) return "properties: no surrogate attached to the table";
@@ -220,7 +242,7 @@ std::string props_refresh_dh(const chs_props* p, int mode, const double* rhoh, l
   }
   if (mode != CHS_PROPS_TABLE) return "properties: unknown mode";
   double* const o[7] = {h, d, m, k, t, nullptr, nullptr};
-  const hipError_t e = launch_from_dh(p, rhoh, n, o, flag, s);
+  const hipError_t e = launch_from_dh(p, rhoh, n, o, flag, s, r32);
   if (e != hipSuccess) return std::string("property kernel: ") + hipGetErrorString(e);
   return "";
 }
