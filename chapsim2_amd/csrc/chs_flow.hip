@@ -143,26 +143,32 @@ __global__ void k_combine(T* __restrict__ ax, const T* __restrict__ ay, const T*
 // sum the block partials in a fixed order; bulk = sum / vol.  One block of
 // 1024 threads; each thread first sums its strided share with 8 loads in
 // flight.
+// One workgroup: 32 partials per thread loaded at once (one memory round
+// trip for up to 32 768 block partials), summed per thread, across each wave
+// by lane exchanges, then the 16 wave sums by one wave.
 __global__ void __launch_bounds__(1024) k_finish_sum(const double* __restrict__ partial, int np, double vol,
                                                      double* __restrict__ bulk) {
-  __shared__ double red[1024];
+  __shared__ double red[16];
   double s = 0.0;
   int i = threadIdx.x;
-  for (; i + 7 * 1024 < np; i += 8 * 1024) {
-    double v[8];
+  for (; i + 31 * 1024 < np; i += 32 * 1024) {
+    double v[32];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = partial[i + u * 1024];
+    for (int u = 0; u < 32; ++u) v[u] = partial[i + u * 1024];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) s = s + v[u];
+    for (int u = 0; u < 32; ++u) s = s + v[u];
   }
   for (; i < np; i += 1024) s = s + partial[i];
-  red[threadIdx.x] = s;
+#pragma unroll
+  for (int k = 32; k > 0; k >>= 1) s = s + __shfl_xor(s, k);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + k];
-    __syncthreads();
+  if (threadIdx.x < 64) {
+    double t = threadIdx.x < 16 ? red[threadIdx.x] : 0.0;
+#pragma unroll
+    for (int k = 8; k > 0; k >>= 1) t = t + __shfl_xor(t, k);
+    if (threadIdx.x == 0) bulk[0] = t / vol;
   }
-  if (threadIdx.x == 0) bulk[0] = red[0] / vol;
 }
 
 // q = q + (rhs - bulk) in FP64; y walls of qy set from fby (enforce_velo_from_fbc)
