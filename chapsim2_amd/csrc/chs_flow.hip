@@ -140,10 +140,8 @@ __global__ void k_combine(T* __restrict__ ax, const T* __restrict__ ay, const T*
   }
 }
 
-// sum the block partials in a fixed order; bulk = sum / vol.  One block of
-// 1024 threads; each thread first sums its strided share with 8 loads in
-// flight.
-// One workgroup: 32 partials per thread loaded at once (one memory round
+// Sum the block partials in a fixed order; bulk = sum / vol.  One workgroup
+// of 1024 threads: 32 partials per thread loaded at once (one memory round
 // trip for up to 32 768 block partials), summed per thread, across each wave
 // by lane exchanges, then the 16 wave sums by one wave.
 __global__ void __launch_bounds__(1024) k_finish_sum(const double* __restrict__ partial, int np, double vol,
