@@ -41,9 +41,11 @@ def test_poisson_010_vs_oracle(po, nc, iacc):
         dm.close()
 
 
-def test_poisson_010_fp32_vs_oracle(po):
-    """FP32 solve (chs_poisson_f32, the FP32 even/odd mixing) against the
-    oracle's FP64 solve: FP32 rounding, above FP64 round-off."""
+@pytest.mark.parametrize("nc", [(64, 48, 40), (32, 160, 16)])
+def test_poisson_010_fp32_vs_oracle(po, nc):
+    """FP32 solve (chs_poisson_f32: the one-pass FP32 even/odd mixing for
+    ny / 2 <= 64, the two-pass one above) against the oracle's FP64 solve:
+    FP32 rounding, above FP64 round-off."""
     import ctypes as C
 
     import torch
@@ -51,7 +53,6 @@ def test_poisson_010_fp32_vs_oracle(po):
     from chapsim2_amd import _lib
     from chapsim2_amd.domain import FFT_2DECOMP_3DFFT, Domain
 
-    nc = (64, 48, 40)
     dm = Domain(nc, LX, -1.0, 1.0, LZ, iacc=4, poisson=FFT_2DECOMP_3DFFT, **CFG)
     try:
         of = po.Flow(list(nc), LX, -1.0, 1.0, LZ, wall_y=True, iacc=4, poisson=3, **CFG)
