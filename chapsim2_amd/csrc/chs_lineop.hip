@@ -961,21 +961,47 @@ std::string launch_line_op2(const DevPlan<T>& DA, const DevPlan<T>& DB, const Ex
     *passes = v == 0 ? 1 : 2;
     return "";
   }
-  if (check_only) {
-    *passes = 1;
-    return "";
-  }
   const long span = (dir == 1) ? (long)ein.n[0] * (ein.n[1] > eout.n[1] ? ein.n[1] : eout.n[1]) *
                                      (64 / ein.n[0] + 2)
                                : 64L;
   if (span * (long)sizeof(T) >= (1L << 32)) return "";
   dim3 g((unsigned)((a.nlines + 63) / 64)), bb((unsigned)(64 * Pn));
   const size_t shm = sizeof(T) * 2 * (cset + P2);
-  if (dir == 1) dispatch_yz2<T, 1>(pair, CHp, g, bb, shm, s, a, b);
-  else dispatch_yz2<T, 2>(pair, CHp, g, bb, shm, s, a, b);
-  hipError_t he = hipGetLastError();
-  if (he != hipSuccess) return std::string("line op launch: ") + hipGetErrorString(he);
-  *passes = 1;
+  // y/z lines: one pass, or (when timed faster, once per shape; FP32 pairs
+  // of 32-row chunks hold both windows and spill) the two operators one by one
+  auto run = [&](int v, const KArgs<T>& ka, const KArgs<T>& kb, T* oa, T* ob, T* scr) -> bool {
+    if (v == 0) {
+      if (dir == 1) dispatch_yz2<T, 1>(pair, CHp, g, bb, shm, s, ka, kb);
+      else dispatch_yz2<T, 2>(pair, CHp, g, bb, shm, s, ka, kb);
+      return hipGetLastError() == hipSuccess;
+    }
+    LineOut<T> la = oM, lb = oG;
+    la.out = oa;
+    lb.out = ob;
+    la.accumulate = lb.accumulate = ka.accumulate;
+    return launch_line_op<T>(M, ein, eout, li, fM, la, scr, s).empty() &&
+           launch_line_op<T>(G, ein, eout, li, fG, lb, scr, s).empty();
+  };
+  int v = -1;
+  if (scratch) {
+    const std::string key = std::string("line_yz2/") + (sizeof(T) == 8 ? "f64/" : "f32/") + std::to_string(dir) +
+                            "/" + std::to_string(pair) + "/" + std::to_string(pm.n_in) + "/" +
+                            std::to_string(pm.n_out) + "/" + std::to_string(a.nlines);
+    v = tuned_choice(key);
+    if (v < 0 && check_only) {
+      KArgs<T> ta = a, tb = b;
+      ta.out = tb.out = scratch;  // the candidates write the scratch array, never the outputs
+      ta.accumulate = tb.accumulate = 0;
+      v = tune(key, 2, s, [&](int vv) { return run(vv, ta, tb, scratch, scratch, scratch); });
+    }
+  }
+  if (v < 0) v = 0;
+  if (check_only) {
+    *passes = v == 0 ? 1 : 2;
+    return "";
+  }
+  if (!run(v, a, b, oM.out, oG.out, scratch)) return std::string("line op launch: ") + hipGetErrorString(hipGetLastError());
+  *passes = v == 0 ? 1 : 2;
   return "";
 }
 
