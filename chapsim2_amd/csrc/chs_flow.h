@@ -130,7 +130,7 @@ class Flow {
   Handle* H = nullptr;
   int nx, ny, nz, px, py, pz;
   bool wall_y = true;
-  bool mixed = false;   // momentum right-hand side in FP32 (CHS_PREC_MIXED)
+  bool mixed = false;   // momentum and energy right-hand sides in FP32 (CHS_PREC_MIXED)
   // mixed form: the FP32 copies of qx, qy, qz, pr (w32) equal the rounded
   // state.  A sub-step leaves them so (its corrections and pressure update
   // store both); set_state clears it, and so does the start of a graph
