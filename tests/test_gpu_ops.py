@@ -338,6 +338,54 @@ def test_x_launch_shapes_vs_oracle(po, variant):
         _lib.check(lib.chs_tune_force(b"line_x/", -1))
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_yz_pair_shapes_vs_oracle(po, variant):
+    """The y/z operator pairs (interpolation and derivative of one input),
+    forced through chs_tune_force (0: one pass over 64 lines per workgroup,
+    1: the two operators one by one, 2: one pass over 32 lines per
+    workgroup): the FP64 momentum right-hand side, whose first level runs
+    every pair, against the oracle on channels with 24- and 129-row y lines
+    and 16- and 256-row z lines (1e-12); the mixed form's FP32 pairs (16- and
+    32-row chunks) against the forced one-pass 64-line shape (1e-6)."""
+    from chapsim2_amd import _lib
+    from chapsim2_amd.domain import PREC_MIXED, Domain
+    from chapsim2_amd.initial import channel_fields
+
+    lib = _lib.load()
+    cfg = dict(istret=2, rstret=0.075, dt=5e-4, ren=2800.0, idriven=1)
+    try:
+        for nc in ((40, 24, 16), (24, 129, 256)):
+            print(f"pairs variant {variant} nc={nc}", flush=True)
+            fields = channel_fields(nc, LX, -1.0, 1.0, LZ, 2, 0.075, noise=0.1, seed=17)
+            of = po.Flow(list(nc), LX, -1.0, 1.0, LZ, wall_y=True, iacc=4, poisson=2, **cfg)
+            of.set_state(*fields)
+            ref = of.momentum_rhs(1)
+            _lib.check(lib.chs_tune_force(b"line_yz2/", variant))
+            dm = Domain(nc, LX, -1.0, 1.0, LZ, **cfg)
+            try:
+                dm.set_state(*fields)
+                got = dm.momentum_rhs(1)
+            finally:
+                dm.close()
+            for name, g, r in zip(("mx", "my", "mz"), got, ref):
+                err = np.abs(g - r).max() / np.abs(r).max()
+                assert err < 1e-12, (variant, nc, name, err)
+            rhs = []
+            for v in (0, variant):
+                _lib.check(lib.chs_tune_force(b"line_yz2/", v))
+                dm = Domain(nc, LX, -1.0, 1.0, LZ, precision=PREC_MIXED, **cfg)
+                try:
+                    dm.set_state(*fields)
+                    rhs.append(dm.momentum_rhs(1))
+                finally:
+                    dm.close()
+            for name, a, b in zip(("mx", "my", "mz"), *rhs):
+                err = np.abs(a - b).max() / np.abs(b).max()
+                assert err < 1e-6, (variant, nc, name, err)
+    finally:
+        _lib.check(lib.chs_tune_force(b"line_yz2/", -1))
+
+
 @pytest.mark.parametrize("variant", [0, 1])
 def test_yz_launch_shapes_vs_oracle(po, variant):
     """Both y/z launch shapes, forced through chs_tune_force (0: 64 lines per
