@@ -1038,8 +1038,10 @@ std::string launch_line_op2(const DevPlan<T>& DA, const DevPlan<T>& DB, const Ex
     return launch_line_op<T>(M, ein, eout, li, fM, la, scr, s).empty() &&
            launch_line_op<T>(G, ein, eout, li, fG, lb, scr, s).empty();
   };
+  // (FP64 pairs keep the 64-line pass: their candidates differ by 1-5 us
+  // timed alone, and the overlapped step measured faster with the pass)
   int v = -1;
-  if (scratch) {
+  if (scratch && sizeof(T) == 4) {
     const std::string key = std::string("line_yz2/") + (sizeof(T) == 8 ? "f64/" : "f32/") + std::to_string(dir) +
                             "/" + std::to_string(pair) + "/" + std::to_string(pm.n_in) + "/" +
                             std::to_string(pm.n_out) + "/" + std::to_string(a.nlines);

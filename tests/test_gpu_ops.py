@@ -340,13 +340,14 @@ def test_x_launch_shapes_vs_oracle(po, variant):
 
 @pytest.mark.parametrize("variant", [0, 1, 2])
 def test_yz_pair_shapes_vs_oracle(po, variant):
-    """The y/z operator pairs (interpolation and derivative of one input),
-    forced through chs_tune_force (0: one pass over 64 lines per workgroup,
-    1: the two operators one by one, 2: one pass over 32 lines per
-    workgroup): the FP64 momentum right-hand side, whose first level runs
-    every pair, against the oracle on channels with 24- and 129-row y lines
-    and 16- and 256-row z lines (1e-12); the mixed form's FP32 pairs (16- and
-    32-row chunks) against the forced one-pass 64-line shape (1e-6)."""
+    """The FP32 y/z operator pairs (interpolation and derivative of one
+    input), forced through chs_tune_force (0: one pass over 64 lines per
+    workgroup, 1: the two operators one by one, 2: one pass over 32 lines
+    per workgroup; FP64 pairs always take the 64-line pass): on channels
+    with 24- and 129-row y lines and 16- and 256-row z lines, the FP64
+    momentum right-hand side, whose first level runs every pair, against
+    the oracle (1e-12), and the mixed form's (16- and 32-row chunks) in
+    each forced shape against the 64-line pass (1e-6)."""
     from chapsim2_amd import _lib
     from chapsim2_amd.domain import PREC_MIXED, Domain
     from chapsim2_amd.initial import channel_fields
