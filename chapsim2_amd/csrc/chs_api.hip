@@ -77,22 +77,12 @@ std::string Handle::plan(int dir, int op, int iacc, const int ibc[2], bool has_f
     return "";
   }
   LinePlan lp;
-  const int chunk = plan_chunk_rows(sizeof(T), n_out);
-  std::string e = make_line_plan(mesh.line_geom(dir), Op(op), iacc, ibc, has_fbc, n_in, n_out, &lp, chunk);
+  std::string e = make_line_plan(mesh.line_geom(dir), Op(op), iacc, ibc, has_fbc, n_in, n_out, &lp,
+                                 plan_chunk_rows(sizeof(T), n_out));
   if (!e.empty()) return e;
   std::unique_ptr<DevPlan<T>> dp(new DevPlan<T>());
   e = upload_plan<T>(lp, dp.get());
   if (!e.empty()) return e;
-  if (chunk == kChunkLong && dir != 0) {
-    // the same operator on kChunk-row chunks: the y/z launches time both
-    LinePlan l16;
-    e = make_line_plan(mesh.line_geom(dir), Op(op), iacc, ibc, has_fbc, n_in, n_out, &l16, 0);
-    if (!e.empty()) return e;
-    std::unique_ptr<DevPlan<T>> d16(new DevPlan<T>());
-    e = upload_plan<T>(l16, d16.get());
-    if (!e.empty()) return e;
-    dp->alt = d16.release();
-  }
   *out = dp.get();
   mp[key] = std::move(dp);
   return "";

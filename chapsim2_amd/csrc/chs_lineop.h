@@ -53,9 +53,6 @@ struct DevPlan {
   // the four boundary rows are the bulk stencil on the wrapped window
   // (periodic lines), so the kernels need not form them apart
   bool ends_bulk = false;
-  // y/z plans of kChunkLong-row chunks: the same operator on kChunk-row
-  // chunks (owned), a launch candidate for y and z lines
-  DevPlan<T>* alt = nullptr;
   ~DevPlan();
 };
 

@@ -62,7 +62,6 @@ DevPlan<T>::~DevPlan() {
   for (void* p : {(void*)rows, (void*)a, (void*)c, (void*)d, (void*)pw, (void*)x1, (void*)metric, (void*)rinv,
                   (void*)swin, (void*)co})
     if (p) (void)hipFree(p);
-  delete alt;
 }
 
 template <typename T>
@@ -808,56 +807,9 @@ std::string launch_line_op(const DevPlan<T>& D, const Ext3& ein, const Ext3& eou
       for (int w = 0; w < nvar && !done; ++w) done = run(w, a);
       if (!done) return std::string("line op: no x-line launch shape fits: ") + hipGetErrorString(hipGetLastError());
     } else if (CHp == kChunkLong) {
-      // y and z lines of kChunkLong-row chunks: the long-chunk kernel (v = 0),
-      // or the plan's kChunk-row twin over 64 (v = 1) or 32 (v = 2) lines
-      // per workgroup, timed once per shape when a scratch output is at hand
       if (!lanes_ok) return "line op: lines too far apart for 32-bit offsets";
-      const DevPlan<T>* D16 = (D.alt && partitioned(*D.alt) && D.alt->host.part_ch == kChunk &&
-                               (!D.alt->host.solve || D.alt->rinv))
-                                  ? D.alt
-                                  : nullptr;
-      KArgs<T> b16 = a;
-      size_t shm16 = 0;
-      if (D16) {
-        const std::string e = line_args(*D16, ein, eout, in, fbc, out, scratch, &b16);
-        if (!e.empty()) return e;
-        part_args(*D16, &b16);
-        const int P16 = D16->host.part_P;
-        shm16 = sizeof(T) * ((size_t)P16 * kChunk * kCo + 4 * (size_t)P16 * P16);
-      }
-      auto run = [&](int v, const KArgs<T>& ka, const KArgs<T>& kb) -> bool {
-        if (v == 0) {
-          if (dir == 1) dispatch_long<T, 1>(p.op, gl, bl, shm_l, s, ka);
-          else dispatch_long<T, 2>(p.op, gl, bl, shm_l, s, ka);
-          return hipGetLastError() == hipSuccess;
-        }
-        if (!D16 || (v == 2 && input_kind(kb) != 0 && sizeof(T) != 4)) return false;
-        const int nlw = v == 1 ? 64 : 32;
-        dim3 g((unsigned)((kb.nlines + nlw - 1) / nlw)), b((unsigned)(nlw * D16->host.part_P));
-        if (v == 1) {
-          if (dir == 1) dispatch_part<T, 1, 64>(p.op, g, b, shm16, s, kb);
-          else dispatch_part<T, 2, 64>(p.op, g, b, shm16, s, kb);
-        } else {
-          if (dir == 1) dispatch_part<T, 1, 32>(p.op, g, b, shm16, s, kb);
-          else dispatch_part<T, 2, 32>(p.op, g, b, shm16, s, kb);
-        }
-        return hipGetLastError() == hipSuccess;
-      };
-      int v = -1;
-      if (scratch) {
-        KArgs<T> ts = a, tb = b16;
-        ts.out = tb.out = scratch;  // the candidates write the scratch array, never the output
-        ts.accumulate = tb.accumulate = 0;
-        ts.out32 = tb.out32 = nullptr;
-        const std::string key = std::string("line_yzl/") + (sizeof(T) == 8 ? "f64/" : "f32/") +
-                                std::to_string(dir) + "/" + std::to_string(p.op) + "/" + std::to_string(p.n_in) +
-                                "/" + std::to_string(p.n_out) + "/" + std::to_string(a.nlines) + "/" +
-                                std::to_string(input_kind(a));
-        v = tune(key, 3, s, [&](int vv) { return run(vv, ts, tb); });
-      }
-      // (a forced shape that does not apply falls back to the long-chunk kernel)
-      if (!run(v < 0 ? 0 : v, a, b16) && (v <= 0 || !run(0, a, b16)))
-        return std::string("line op launch: ") + hipGetErrorString(hipGetLastError());
+      if (dir == 1) dispatch_long<T, 1>(p.op, gl, bl, shm_l, s, a);
+      else dispatch_long<T, 2>(p.op, gl, bl, shm_l, s, a);
     } else {
       // y and z lines of 16-row chunks: 64 or (single-input operators, and
       // every FP32 operator) 32 lines per workgroup, timed once per shape
