@@ -254,16 +254,17 @@ __global__ void __launch_bounds__(kThreads) k_fft_s(cx<T>* __restrict__ W, int n
 }
 
 // ---------------------------------------------------------------------------
-// Two-pass transforms for N = A * B in {64, 128, 256, 512}.  With n = a + A b
+// Two-pass transforms for N = A * B (A, B <= 32).  With n = a + A b
 // and k = k2 + B k1 (a, k1 < A; b, k2 < B):
 //   X[k2 + B k1] = sum_a W_A^(a k1) [ W_N^(a k2) sum_b x[a + A b] W_B^(b k2) ].
 // A threads per line: thread a loads x[a + A b] for all b and does the
 // B-point DFT in registers, applies W_N^(a k2) (table, index a k2 < N), and
 // writes to LDS (rows of A + 1 entries, so the A-point reads of pass 2 hit
 // distinct banks); after one barrier, thread a does the A-point DFTs for
-// k2 = a, a + A, ... in registers.  The register DFTs are radix-2 with
-// compile-time twiddles W_32^m = (-i)^(m/8) W_32^(m%8) from the first
-// octant below, exactly symmetric.
+// k2 = a, a + A, ... < B in registers.  The register DFTs of power-of-two
+// length are radix-2 with compile-time twiddles W_32^m = (-i)^(m/8)
+// W_32^(m%8) from the first octant below, exactly symmetric; other lengths
+// (3, 5, 6, 9, 10, 12, 15) are direct sums over the N-point twiddle table.
 __device__ constexpr double kOctC[8] = {1.0, 0.9807852804032304, 0.9238795325112867, 0.8314696123025452,
                                         0.7071067811865476, 0.5555702330196022, 0.3826834323650898,
                                         0.19509032201612825};
@@ -324,6 +325,31 @@ __device__ __forceinline__ void dft_reg(cx<T> (&v)[R]) {
   }
 }
 
+// In-register DFT of R points: radix-2 for powers of two, else the direct
+// sum X[k] = sum_j v[j] W_R^(j k) with W_R^m = twN[m S] read from the
+// transform's N-point table (S = N / R).
+template <typename T, int R, bool FWD>
+__device__ __forceinline__ void dft_any(cx<T> (&v)[R], const cx<T>* __restrict__ twN, int S) {
+  if constexpr ((R & (R - 1)) == 0) {
+    dft_reg<T, R, FWD>(v);
+  } else {
+    cx<T> out[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      cx<T> acc = v[0];
+#pragma unroll
+      for (int j = 1; j < R; ++j) {
+        cx<T> w = twN[((j * k) % R) * S];
+        if (!FWD) w.im = -w.im;
+        acc = cadd(acc, cmul(v[j], w));
+      }
+      out[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = out[k];
+  }
+}
+
 // Pass-1 twiddle v[k2] *= W_N^(a k2) (table of exp(-2 pi i q / N)).
 template <typename T, int B, bool FWD>
 __device__ __forceinline__ void twiddle_rows(cx<T> (&v)[B], int a, const cx<T>* __restrict__ twN) {
@@ -339,6 +365,7 @@ template <int A, int B>
 struct Fft2 {
   static constexpr int N = A * B;
   static constexpr int LSL = B * (A + 1) + 1;  // LDS entries per line
+  static constexpr int G = (B + A - 1) / A;    // pass-2 columns per thread
 };
 
 // Strided complex transform (as k_fft_s) in two passes; LPW consecutive kx
@@ -358,19 +385,20 @@ __global__ void __launch_bounds__(LPW * A) k_fft2_s(cx<T>* __restrict__ W, int n
   cx<T> v[B];
 #pragma unroll
   for (int b = 0; b < B; ++b) v[b] = W[base + ls * (a + A * b)];
-  dft_reg<T, B, FWD>(v);
+  dft_any<T, B, FWD>(v, twN, A);
   twiddle_rows<T, B, FWD>(v, a, twN);
   cx<T>* lb = buf + l * F::LSL;
 #pragma unroll
   for (int k2 = 0; k2 < B; ++k2) lb[k2 * (A + 1) + a] = v[k2];
   __syncthreads();
 #pragma unroll
-  for (int g = 0; g < B / A; ++g) {
+  for (int g = 0; g < F::G; ++g) {
     const int k2 = a + A * g;
+    if (F::G * A != B && k2 >= B) break;
     cx<T> u[A];
 #pragma unroll
     for (int j = 0; j < A; ++j) u[j] = lb[k2 * (A + 1) + j];
-    dft_reg<T, A, FWD>(u);
+    dft_any<T, A, FWD>(u, twN, B);
     if (ok) {
 #pragma unroll
       for (int k1 = 0; k1 < A; ++k1) W[base + ls * (k2 + B * k1)] = u[k1];
@@ -398,28 +426,29 @@ __global__ void __launch_bounds__(LPW * A) k_fft2_x_fwd(const T* __restrict__ in
     const int n = a + A * b;
     v[b] = {src[2 * n] * in_scale, src[2 * n + 1] * in_scale};
   }
-  dft_reg<T, B, true>(v);
+  dft_any<T, B, true>(v, twN, A);
   twiddle_rows<T, B, true>(v, a, twN);
   cx<T>* lb = buf + l * F::LSL;
 #pragma unroll
   for (int k2 = 0; k2 < B; ++k2) lb[k2 * (A + 1) + a] = v[k2];
   __syncthreads();
-  cx<T> z[B];  // z[g * A + k1] = Z[(a + A g) + B k1]
+  cx<T> z[F::G * A];  // z[g * A + k1] = Z[(a + A g) + B k1]
 #pragma unroll
-  for (int g = 0; g < B / A; ++g) {
+  for (int g = 0; g < F::G; ++g) {
     const int k2 = a + A * g;
     cx<T> u[A];
 #pragma unroll
-    for (int j = 0; j < A; ++j) u[j] = lb[k2 * (A + 1) + j];
-    dft_reg<T, A, true>(u);
+    for (int j = 0; j < A; ++j) u[j] = (k2 < B) ? lb[k2 * (A + 1) + j] : cx<T>{T(0), T(0)};
+    dft_any<T, A, true>(u, twN, B);
 #pragma unroll
     for (int k1 = 0; k1 < A; ++k1) z[g * A + k1] = u[k1];
   }
   __syncthreads();  // pass-2 reads done: the line buffer takes Z in natural order
 #pragma unroll
-  for (int g = 0; g < B / A; ++g)
+  for (int g = 0; g < F::G; ++g)
 #pragma unroll
-    for (int k1 = 0; k1 < A; ++k1) lb[(a + A * g) + B * k1] = z[g * A + k1];
+    for (int k1 = 0; k1 < A; ++k1)
+      if (a + A * g < B) lb[(a + A * g) + B * k1] = z[g * A + k1];
   __syncthreads();
   if (!ok) return;
   cx<T>* dst = W + L * nxh;
@@ -463,7 +492,7 @@ __global__ void __launch_bounds__(LPW * A) k_fft2_x_inv(const cx<T>* __restrict_
   cx<T> v[B];
 #pragma unroll
   for (int b = 0; b < B; ++b) v[b] = lb[a + A * b];
-  dft_reg<T, B, false>(v);
+  dft_any<T, B, false>(v, twN, A);
   twiddle_rows<T, B, false>(v, a, twN);
   __syncthreads();  // every thread has read its pass-1 input
 #pragma unroll
@@ -471,12 +500,13 @@ __global__ void __launch_bounds__(LPW * A) k_fft2_x_inv(const cx<T>* __restrict_
   __syncthreads();
   T* dst = out + L * (2L * N);
 #pragma unroll
-  for (int g = 0; g < B / A; ++g) {
+  for (int g = 0; g < F::G; ++g) {
     const int k2 = a + A * g;
+    if (F::G * A != B && k2 >= B) break;
     cx<T> u[A];
 #pragma unroll
     for (int j = 0; j < A; ++j) u[j] = lb[k2 * (A + 1) + j];
-    dft_reg<T, A, false>(u);
+    dft_any<T, A, false>(u, twN, B);
     if (ok) {
 #pragma unroll
       for (int k1 = 0; k1 < A; ++k1) {
@@ -488,9 +518,10 @@ __global__ void __launch_bounds__(LPW * A) k_fft2_x_inv(const cx<T>* __restrict_
   }
 }
 
-// Launchers: the two-pass kernels for N in {64, 128, 256, 512}, the general
-// Stockham kernels otherwise.  The lines per workgroup of a two-pass kernel
-// (variant v: 4, 8 or 16) are chosen by timing the variants once per shape
+// Launchers: the two-pass kernels for N in {36, 40, 50, 64, 72, 80, 90, 96,
+// 100, 128, 144, 180, 192, 256, 512}, the general Stockham kernels otherwise.
+// The lines per workgroup of a two-pass kernel (variant v: 4, 8 or 16) or the
+// general kernel (v = 3) are chosen by timing the variants once per shape
 // (chs_tune.h); untuned shapes use 16 (8 for N = 512).
 template <typename T, int A, int B>
 static size_t fft2_lds(int lpw) { return sizeof(cx<T>) * (size_t)lpw * Fft2<A, B>::LSL; }
@@ -529,8 +560,19 @@ template <typename T>
 static bool fft2_s_variant(int v, cx<T>* W, int nxh, int n, long ls, int no, long os, const cx<T>* tw, T sgn,
                            hipStream_t s) {
   switch (n) {
+    case 36: return fft2_s_lpw<T, 6, 6>(v, W, nxh, ls, no, os, tw, sgn, s);
+    case 40: return fft2_s_lpw<T, 5, 8>(v, W, nxh, ls, no, os, tw, sgn, s);
+    case 50: return fft2_s_lpw<T, 5, 10>(v, W, nxh, ls, no, os, tw, sgn, s);
     case 64: return fft2_s_lpw<T, 8, 8>(v, W, nxh, ls, no, os, tw, sgn, s);
+    case 72: return fft2_s_lpw<T, 8, 9>(v, W, nxh, ls, no, os, tw, sgn, s);
+    case 80: return fft2_s_lpw<T, 8, 10>(v, W, nxh, ls, no, os, tw, sgn, s);
+    case 90: return fft2_s_lpw<T, 9, 10>(v, W, nxh, ls, no, os, tw, sgn, s);
+    case 96: return fft2_s_lpw<T, 8, 12>(v, W, nxh, ls, no, os, tw, sgn, s);
+    case 100: return fft2_s_lpw<T, 10, 10>(v, W, nxh, ls, no, os, tw, sgn, s);
     case 128: return fft2_s_lpw<T, 8, 16>(v, W, nxh, ls, no, os, tw, sgn, s);
+    case 144: return fft2_s_lpw<T, 12, 12>(v, W, nxh, ls, no, os, tw, sgn, s);
+    case 180: return fft2_s_lpw<T, 12, 15>(v, W, nxh, ls, no, os, tw, sgn, s);
+    case 192: return fft2_s_lpw<T, 12, 16>(v, W, nxh, ls, no, os, tw, sgn, s);
     case 256: return fft2_s_lpw<T, 16, 16>(v, W, nxh, ls, no, os, tw, sgn, s);
     case 512: return fft2_s_lpw<T, 16, 32>(v, W, nxh, ls, no, os, tw, sgn, s);
   }
@@ -564,20 +606,29 @@ template <typename T>
 static bool fft2_x_variant(int v, bool fwd, const T* in, cx<T>* W, T* out, int nx, long nlines, const cx<T>* tw,
                            const cx<T>* rot, T sc, hipStream_t s) {
   switch (nx / 2) {
+    case 36: return fft2_x_lpw<T, 6, 6>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 40: return fft2_x_lpw<T, 5, 8>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 50: return fft2_x_lpw<T, 5, 10>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
     case 64: return fft2_x_lpw<T, 8, 8>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 72: return fft2_x_lpw<T, 8, 9>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 80: return fft2_x_lpw<T, 8, 10>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 90: return fft2_x_lpw<T, 9, 10>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 96: return fft2_x_lpw<T, 8, 12>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 100: return fft2_x_lpw<T, 10, 10>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
     case 128: return fft2_x_lpw<T, 8, 16>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 144: return fft2_x_lpw<T, 12, 12>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 180: return fft2_x_lpw<T, 12, 15>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
+    case 192: return fft2_x_lpw<T, 12, 16>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
     case 256: return fft2_x_lpw<T, 16, 16>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
     case 512: return fft2_x_lpw<T, 16, 32>(v, fwd, in, W, out, nlines, tw, rot, sc, s);
   }
   return false;
 }
 
+// the general Stockham kernels (any 2,3,5-smooth length of one tile)
 template <typename T>
-static void launch_fft_s(cx<T>* W, int nxh, int n, long ls, int no, long os, const Radices& rp, const cx<T>* tw,
-                         T sgn, hipStream_t s) {
-  int v = tuned_choice(fft_key(sgn < T(0) ? "fft_s+" : "fft_s-", sizeof(T), n, nxh, no));
-  if (v < 0) v = fft2_default_variant(n);
-  if (fft2_s_variant<T>(v, W, nxh, n, ls, no, os, tw, sgn, s)) return;
+static void fft_s_general(cx<T>* W, int nxh, int n, long ls, int no, long os, const Radices& rp, const cx<T>* tw,
+                          T sgn, hipStream_t s) {
   const int lpb = kTileC / n;
   dim3 g((unsigned)(((nxh + lpb - 1) / lpb) * no));
   hipLaunchKernelGGL(k_fft_s<T>, g, dim3(kThreads), sizeof(cx<T>) * (size_t)lpb * n, s, W, nxh, n, ls, no, os, rp,
@@ -585,12 +636,9 @@ static void launch_fft_s(cx<T>* W, int nxh, int n, long ls, int no, long os, con
 }
 
 template <typename T>
-static void launch_fft_x(bool fwd, const T* in, cx<T>* W, T* out, int nx, long nlines, const Radices& rp,
-                         const cx<T>* tw, const cx<T>* rot, T sc, hipStream_t s) {
+static void fft_x_general(bool fwd, const T* in, cx<T>* W, T* out, int nx, long nlines, const Radices& rp,
+                          const cx<T>* tw, const cx<T>* rot, T sc, hipStream_t s) {
   const int N = nx / 2;
-  int v = tuned_choice(fft_key(fwd ? "fft_x+" : "fft_x-", sizeof(T), N, nlines, 1));
-  if (v < 0) v = fft2_default_variant(N);
-  if (fft2_x_variant<T>(v, fwd, in, W, out, nx, nlines, tw, rot, sc, s)) return;
   const int lpb = kTileC / N;
   dim3 g((unsigned)((nlines + lpb - 1) / lpb));
   const size_t sh = sizeof(cx<T>) * (size_t)lpb * N;
@@ -600,25 +648,59 @@ static void launch_fft_x(bool fwd, const T* in, cx<T>* W, T* out, int nx, long n
     hipLaunchKernelGGL(k_fft_x_inv<T>, g, dim3(kThreads), sh, s, W, out, nx, nlines, rp, tw, rot, sc);
 }
 
+// variant 3 (timed with the two-pass ones) is the general kernel
+template <typename T>
+static void launch_fft_s(cx<T>* W, int nxh, int n, long ls, int no, long os, const Radices& rp, const cx<T>* tw,
+                         T sgn, hipStream_t s) {
+  int v = forced_or_tuned(fft_key(sgn < T(0) ? "fft_s+" : "fft_s-", sizeof(T), n, nxh, no), 4);
+  if (v < 0) v = fft2_default_variant(n);
+  if (v < 3 && fft2_s_variant<T>(v, W, nxh, n, ls, no, os, tw, sgn, s)) return;
+  fft_s_general<T>(W, nxh, n, ls, no, os, rp, tw, sgn, s);
+}
+
+template <typename T>
+static void launch_fft_x(bool fwd, const T* in, cx<T>* W, T* out, int nx, long nlines, const Radices& rp,
+                         const cx<T>* tw, const cx<T>* rot, T sc, hipStream_t s) {
+  const int N = nx / 2;
+  int v = forced_or_tuned(fft_key(fwd ? "fft_x+" : "fft_x-", sizeof(T), N, nlines, 1), 4);
+  if (v < 0) v = fft2_default_variant(N);
+  if (v < 3 && fft2_x_variant<T>(v, fwd, in, W, out, nx, nlines, tw, rot, sc, s)) return;
+  fft_x_general<T>(fwd, in, W, out, nx, nlines, rp, tw, rot, sc, s);
+}
+
 // Times the two-pass variants of every transform this solver launches (on
 // `tmp`, an array as large as the work array, and the work array itself;
 // their contents are overwritten by every solve).
 template <typename T>
 static void tune_transforms(int nx, int ny, int nz, int nxh, bool ydir, cx<T>* work, cx<T>* tmp, const cx<T>* twx,
-                            const cx<T>* twy, const cx<T>* twz, const cx<T>* rot) {
+                            const cx<T>* twy, const cx<T>* twz, const cx<T>* rot, const Radices& rx,
+                            const Radices& ry, const Radices& rz) {
   hipStream_t s = nullptr;
   const long nlx = (long)ny * nz;
   T* real = reinterpret_cast<T*>(tmp);
+  auto ok = [] { return hipGetLastError() == hipSuccess; };
   for (int fwd = 1; fwd >= 0; --fwd) {
-    tune(fft_key(fwd ? "fft_x+" : "fft_x-", sizeof(T), nx / 2, nlx, 1), 3, s, [&](int v) {
+    tune(fft_key(fwd ? "fft_x+" : "fft_x-", sizeof(T), nx / 2, nlx, 1), 4, s, [&](int v) {
+      if (v == 3) {
+        fft_x_general<T>(fwd != 0, real, work, real, nx, nlx, rx, twx, rot, T(1), s);
+        return ok();
+      }
       return fft2_x_variant<T>(v, fwd != 0, real, work, real, nx, nlx, twx, rot, T(1), s);
     });
     const T sgn = fwd ? T(-1) : T(1);
-    tune(fft_key(fwd ? "fft_s+" : "fft_s-", sizeof(T), nz, nxh, ny), 3, s, [&](int v) {
+    tune(fft_key(fwd ? "fft_s+" : "fft_s-", sizeof(T), nz, nxh, ny), 4, s, [&](int v) {
+      if (v == 3) {
+        fft_s_general<T>(work, nxh, nz, (long)nxh * ny, ny, (long)nxh, rz, twz, sgn, s);
+        return ok();
+      }
       return fft2_s_variant<T>(v, work, nxh, nz, (long)nxh * ny, ny, (long)nxh, twz, sgn, s);
     });
     if (ydir)
-      tune(fft_key(fwd ? "fft_s+" : "fft_s-", sizeof(T), ny, nxh, nz), 3, s, [&](int v) {
+      tune(fft_key(fwd ? "fft_s+" : "fft_s-", sizeof(T), ny, nxh, nz), 4, s, [&](int v) {
+        if (v == 3) {
+          fft_s_general<T>(work, nxh, ny, (long)nxh, nz, (long)nxh * ny, ry, twy, sgn, s);
+          return ok();
+        }
         return fft2_s_variant<T>(v, work, nxh, ny, (long)nxh, nz, (long)nxh * ny, twy, sgn, s);
       });
   }
@@ -1454,9 +1536,17 @@ std::string Poisson::create_3d(const Mesh& m, int iacc, std::unique_ptr<Poisson>
 std::string Poisson::tune(bool ydir) {
   void* tmp = nullptr;
   PCHK(hipMalloc(&tmp, work_bytes));
+  Radices rx{}, ry{}, rz{};
+  rx.n = (int)rad_x.size();
+  for (int q = 0; q < rx.n; ++q) rx.r[q] = rad_x[q];
+  ry.n = (int)rad_y.size();
+  for (int q = 0; q < ry.n; ++q) ry.r[q] = rad_y[q];
+  rz.n = (int)rad_z.size();
+  for (int q = 0; q < rz.n; ++q) rz.r[q] = rad_z[q];
   tune_transforms<double>(nx, ny, nz, nxh, ydir, reinterpret_cast<cx<double>*>(work), reinterpret_cast<cx<double>*>(tmp),
                           reinterpret_cast<const cx<double>*>(tw_x64), reinterpret_cast<const cx<double>*>(tw_y64),
-                          reinterpret_cast<const cx<double>*>(tw_z64), reinterpret_cast<const cx<double>*>(rot64));
+                          reinterpret_cast<const cx<double>*>(tw_z64), reinterpret_cast<const cx<double>*>(rot64), rx,
+                          ry, rz);
   const hipError_t sync = hipDeviceSynchronize();
   (void)hipFree(tmp);
   if (sync != hipSuccess) return std::string("poisson tuning: ") + hipGetErrorString(sync);

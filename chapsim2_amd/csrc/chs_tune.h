@@ -12,6 +12,9 @@ namespace chs {
 
 // Cached choice for `key`, or -1 when the shape has not been tuned.
 int tuned_choice(const std::string& key);
+// The variant forced for `key` (chs_tune_force; -1 when none applies or it
+// is not below nvar), else the cached choice (-1 when untuned).
+int forced_or_tuned(const std::string& key, int nvar);
 
 // Times launch(v) for v = 0 .. nvar-1 on stream s (one warm-up, best of three
 // timed launches; launch returns false for a variant that does not apply) and

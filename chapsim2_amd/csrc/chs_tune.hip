@@ -37,6 +37,15 @@ int tuned_choice(const std::string& key) {
   return it == g_cache.end() ? -1 : it->second.choice;
 }
 
+int forced_or_tuned(const std::string& key, int nvar) {
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_force_variant >= 0 && key.compare(0, g_force_prefix.size(), g_force_prefix) == 0)
+      return g_force_variant < nvar ? g_force_variant : -1;
+  }
+  return tuned_choice(key);
+}
+
 int tune(const std::string& key, int nvar, hipStream_t s, const std::function<bool(int)>& launch) {
   {
     std::lock_guard<std::mutex> lk(g_mu);

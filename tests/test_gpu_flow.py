@@ -69,6 +69,39 @@ def test_poisson_transform_lengths(po, nc):
         dm.close()
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_poisson_mixed_radix_lengths_vs_oracle(po, variant):
+    """FISHPACK-form Poisson solve where nx/2 and nz take the two-pass
+    lengths with factors 3 and 5 (36, 40, 50, 72, 80, 90, 96, 100, 144, 180,
+    192: register DFTs of 5, 6, 9, 10, 12 and 15 points as direct sums), each
+    transform forced to one launch shape through chs_tune_force (0, 1, 2:
+    the two-pass kernels with 4, 8, 16 lines per workgroup; 3: the general
+    Stockham kernel), against the oracle's FFTPACK-form solve."""
+    from chapsim2_amd import _lib
+    from chapsim2_amd.domain import Domain
+    from chapsim2_amd.poisson import solve_fft_poisson
+
+    lib = _lib.load()
+    _lib.check(lib.chs_tune_force(b"fft_", variant))
+    try:
+        for nc in ((144, 8, 180), (72, 8, 192), (200, 8, 90), (80, 8, 100), (100, 8, 50), (192, 8, 40),
+                   (180, 8, 144), (288, 8, 72), (160, 8, 36), (360, 8, 96), (384, 8, 80)):
+            print(f"mixed-radix poisson variant {variant} nc={nc}", flush=True)
+            dm = Domain(nc, LX, -1.0, 1.0, LZ, **CFG)
+            try:
+                of = po.Flow(list(nc), LX, -1.0, 1.0, LZ, wall_y=True, iacc=4, poisson=2, **CFG)
+                rng = np.random.default_rng(sum(nc) + variant)
+                rhs = np.asfortranarray(rng.standard_normal(nc))
+                ref = of.poisson(rhs)
+                out = np.array(rhs, order="F")
+                solve_fft_poisson(out, dm)
+                assert rel(out, ref) < 1e-12, (variant, nc, rel(out, ref))
+            finally:
+                dm.close()
+    finally:
+        _lib.check(lib.chs_tune_force(b"fft_", -1))
+
+
 def test_momentum_rhs_vs_oracle(pair):
     dm, of = pair
     for isub in (0, 1):
