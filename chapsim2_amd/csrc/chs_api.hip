@@ -475,10 +475,13 @@ static std::string batch_capture(chs_batch* b) {
   for (size_t i = 0; i < b->inst.size() && err.empty(); ++i) {
     chs::Handle& h = b->inst[i]->h;
     if (hipStreamWaitEvent(h.stream, b->fork, 0) != hipSuccess) { err = "fork wait"; break; }
-    // a replay may follow a set_state: the captured step converts the state
-    // it starts from
-    h.flow->q32_fresh = false;
+    // the captured step starts from fresh FP32 copies of the state (a replay
+    // follows a step, which leaves them so, or chs_batch_run's rounding of
+    // an instance whose state was set); the flag is restored after capture
+    const bool fresh = h.flow->q32_fresh;
+    h.flow->q32_fresh = true;
     for (int isub = 1; isub <= 3 && err.empty(); ++isub) err = h.flow->substep(isub);
+    h.flow->q32_fresh = fresh;
     if (err.empty() && hipEventRecord(b->join[i], h.stream) != hipSuccess) err = "join record";
     if (err.empty() && hipStreamWaitEvent(b->main, b->join[i], 0) != hipSuccess) err = "join wait";
   }
@@ -580,7 +583,17 @@ int chs_batch_run(chs_batch_t* b, int iter0, int nsteps, int use_graph) {
       const std::string e = batch_capture(b);
       if (!e.empty()) return fail(CHS_EHIP, e);
     }
+    // mixed instances whose FP32 copies of the state are stale (a set_state
+    // since the last step) are rounded before the replay
+    for (chs_handle_t* h : b->inst) {
+      chs::Flow& F = *h->h.flow;
+      if (F.mixed && !F.q32_fresh) {
+        RET(F.refresh_q32());
+        if (hipStreamSynchronize(h->h.stream) != hipSuccess) return fail(CHS_EHIP, "synchronise");
+      }
+    }
     if (hipGraphLaunch(b->exec, b->main) != hipSuccess) return fail(CHS_EHIP, "graph launch");
+    for (chs_handle_t* h : b->inst) h->h.flow->q32_fresh = h->h.flow->mixed;
   }
   if (hipStreamSynchronize(b->main) != hipSuccess) return fail(CHS_EHIP, "synchronise");
   for (chs_handle_t* h : b->inst)

@@ -133,9 +133,11 @@ class Flow {
   bool mixed = false;   // momentum and energy right-hand sides in FP32 (CHS_PREC_MIXED)
   // mixed form: the FP32 copies of qx, qy, qz, pr (w32) equal the rounded
   // state.  A sub-step leaves them so (its corrections and pressure update
-  // store both); set_state clears it, and so does the start of a graph
-  // capture, so that a captured step converts the state it is replayed on.
+  // store both); set_state clears it.  The batched form captures its steps
+  // as if the copies were fresh and, before a replay, rounds the state of
+  // any instance whose copies are not (refresh_q32).
   bool q32_fresh = false;
+  std::string refresh_q32();  // w32.qx, qy, qz, pr = the rounded state (mixed form)
   double rre = 1.0, dt = 0.0, sigma2p = 1.0;
   double tg[4], tz[4], ta[4];
   int ibc[3][5][2];     // [dir][u,v,w,p,T][side]

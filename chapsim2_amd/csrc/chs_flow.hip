@@ -927,19 +927,25 @@ std::string Flow::momentum_rhs_t(RhsWork<T>& w, int isub, bool fuse) {
   return "";
 }
 
+std::string Flow::refresh_q32() {
+  if (!mixed) return "";
+  hipStream_t s = H->stream;
+  hipLaunchKernelGGL((k_cvt<float, double>), grid1(qx.size()), dim3(256), 0, s, w32.qx.p, qx.p, qx.size());
+  hipLaunchKernelGGL((k_cvt<float, double>), grid1(qy.size()), dim3(256), 0, s, w32.qy.p, qy.p, qy.size());
+  hipLaunchKernelGGL((k_cvt<float, double>), grid1(qz.size()), dim3(256), 0, s, w32.qz.p, qz.p, qz.size());
+  hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr.size()), dim3(256), 0, s, w32.pr.p, pr.p, pr.size());
+  q32_fresh = true;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("refresh_q32: ") + hipGetErrorString(e);
+  return "";
+}
+
 std::string Flow::momentum_rhs(int isub, bool fuse) {
   set_lanes();
   if (!mixed) return momentum_rhs_t<double>(w64, isub, fuse);
   // rounded copies of the state for the FP32 right-hand side, unless the
   // last sub-step left them
-  if (!q32_fresh) {
-    hipStream_t s = H->stream;
-    hipLaunchKernelGGL((k_cvt<float, double>), grid1(qx.size()), dim3(256), 0, s, w32.qx.p, qx.p, qx.size());
-    hipLaunchKernelGGL((k_cvt<float, double>), grid1(qy.size()), dim3(256), 0, s, w32.qy.p, qy.p, qy.size());
-    hipLaunchKernelGGL((k_cvt<float, double>), grid1(qz.size()), dim3(256), 0, s, w32.qz.p, qz.p, qz.size());
-    hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr.size()), dim3(256), 0, s, w32.pr.p, pr.p, pr.size());
-    q32_fresh = true;
-  }
+  if (!q32_fresh) SCHK(refresh_q32());
   if (thermo) {
     // g and mu change every sub-step (projection, property refresh); the
     // energy sub-step before this right-hand side has rounded g already
