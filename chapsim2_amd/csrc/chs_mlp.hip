@@ -327,9 +327,19 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// tanh x = 1 - 2 / (1 + e^{2x}) with the hardware exponential and a fast
-// divide (absolute error ~1e-7, below the BF16 rounding of the activations)
-__device__ __forceinline__ float tanh_fast(float x) { return 1.0f - __fdividef(2.0f, 1.0f + __expf(2.0f * x)); }
+// tanh x = 1 - 2 / (1 + 2^{2 log2(e) x}) with the hardware base-2
+// exponential and reciprocal (v_exp_f32, v_rcp_f32, each ~1 ulp: absolute
+// error ~3e-7, below the BF16 rounding of the activations).  Five
+// instructions; a written divide (also __fdividef) compiles to the
+// correctly rounded IEEE sequence of fifteen.  Saturates to +-1 for large
+// |x| (exp2 overflow gives a zero reciprocal).
+__device__ __forceinline__ float tanh_fast(float x) {
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(2.8853900817779268f * x));
+}
+
+// BF16 of a finite float, rounded to nearest even by the hardware
+// conversion (v_cvt_pk_bf16_f32; the same bits as f2bf for finite values)
+__device__ __forceinline__ short bf16_rne(float x) { return __builtin_bit_cast(short, (__bf16)x); }
 
 // out[0..4] (normalised) of the 16 cells whose normalised inputs the lanes
 // with l%16 = c hold in xv; every lane ends with the outputs of cell l%16.
@@ -342,7 +352,7 @@ __device__ __forceinline__ void mlp16_mfma(const bf16x4 wa[16], const float* __r
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int u = 16 * kk + 4 * q + r;
-      bv[kk][r] = (short)f2bf(tanh_fast(sp[OW1 + u] * xv + sp[OB1 + u]));
+      bv[kk][r] = bf16_rne(tanh_fast(sp[OW1 + u] * xv + sp[OB1 + u]));
     }
   }
 #pragma unroll
