@@ -1316,16 +1316,11 @@ std::string Flow::refresh_props() {
   // Update_thermal_properties, src/eq_energy.f90:14-47 (dDensm1 = dDens
   // after the refresh, so Calculate_drhodt is zero and is not formed).  The
   // mixed form also gets the FP32 roundings of h, mu, k, T that its energy
-  // and momentum right-hand sides read: stored by the table kernel itself,
-  // rounded after the surrogate.
+  // and momentum right-hand sides read, stored by the refresh kernel itself
+  // (table or surrogate).
   float* const r32[4] = {e32.hent.p, w32.visc.p, e32.kcond.p, e32.temp.p};
   SCHK(props_refresh_dh(props, props_mode, rhoh.p, rhoh.size(), hent.p, dens.p, w64.visc.p, kcond.p, temp.p,
                         props_flag, H->stream, mixed ? r32 : nullptr));
-  if (mixed && props_mode != CHS_PROPS_TABLE)
-    for (auto pr2 : {std::make_pair(&e32.hent, &hent), std::make_pair(&w32.visc, &w64.visc),
-                     std::make_pair(&e32.kcond, &kcond), std::make_pair(&e32.temp, &temp)})
-      hipLaunchKernelGGL((k_cvt<float, double>), grid1(pr2.second->size()), dim3(256), 0, H->stream, pr2.first->p,
-                         pr2.second->p, pr2.second->size());
   props32_fresh = mixed;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("refresh_props: ") + hipGetErrorString(e);

@@ -386,7 +386,9 @@ __global__ void __launch_bounds__(256) k_mlp_from_dh_mfma(const uint16_t* __rest
                                                           double dhmax, const double* __restrict__ rhoh, long N,
                                                           double* __restrict__ oh, double* __restrict__ od,
                                                           double* __restrict__ om, double* __restrict__ ok,
-                                                          double* __restrict__ ot, int* __restrict__ flag) {
+                                                          double* __restrict__ ot, int* __restrict__ flag,
+                                                          float* __restrict__ fh, float* __restrict__ fm,
+                                                          float* __restrict__ fk, float* __restrict__ ft) {
   __shared__ float sp[NP];
   stage_params(p16, sp);
   const int lane = threadIdx.x & 63;
@@ -427,11 +429,17 @@ __global__ void __launch_bounds__(256) k_mlp_from_dh_mfma(const uint16_t* __rest
     if (pass == 0) h = v / ((double)mine[1] * nm.sd[1] + nm.mu[1]);
   }
   if (!live) return;
+  const double vt = (double)mine[0] * nm.sd[0] + nm.mu[0], vm = (double)mine[2] * nm.sd[2] + nm.mu[2],
+               vk = (double)mine[3] * nm.sd[3] + nm.mu[3];
   if (oh) oh[c] = h;
-  if (ot) ot[c] = (double)mine[0] * nm.sd[0] + nm.mu[0];
+  if (ot) ot[c] = vt;
   if (od) od[c] = (double)mine[1] * nm.sd[1] + nm.mu[1];
-  if (om) om[c] = (double)mine[2] * nm.sd[2] + nm.mu[2];
-  if (ok) ok[c] = (double)mine[3] * nm.sd[3] + nm.mu[3];
+  if (om) om[c] = vm;
+  if (ok) ok[c] = vk;
+  if (fh) fh[c] = (float)h;
+  if (fm) fm[c] = (float)vm;
+  if (fk) fk[c] = (float)vk;
+  if (ft) ft[c] = (float)vt;
 }
 
 // Properties from rho*h with the surrogate (flow solver's refresh, mode
@@ -439,13 +447,16 @@ __global__ void __launch_bounds__(256) k_mlp_from_dh_mfma(const uint16_t* __rest
 // table's rho*h column and linear interpolation for the first h (as
 // ftp_refresh_thermal_properties_from_DH, src/input_thermo.f90:393-446),
 // then the network for the properties from h, h = rho*h / rho, and the
-// network again.  One thread per cell; parameters in LDS as in k_eval.
+// network again; fh, fm, fk, ft (each possibly null) receive h, mu, kappa
+// and T rounded to FP32.  One thread per cell; parameters in LDS as in k_eval.
 __global__ void __launch_bounds__(256) k_mlp_from_dh(const uint16_t* __restrict__ p16, Norm nm,
                                                      const double* __restrict__ tab, int nt, double dhmin,
                                                      double dhmax, const double* __restrict__ rhoh, long N,
                                                      double* __restrict__ oh, double* __restrict__ od,
                                                      double* __restrict__ om, double* __restrict__ ok,
-                                                     double* __restrict__ ot, int* __restrict__ flag) {
+                                                     double* __restrict__ ot, int* __restrict__ flag,
+                                                     float* __restrict__ fh, float* __restrict__ fm,
+                                                     float* __restrict__ fk, float* __restrict__ ft) {
   __shared__ float sp[NP];
   stage_params(p16, sp);
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -482,25 +493,33 @@ __global__ void __launch_bounds__(256) k_mlp_from_dh(const uint16_t* __restrict_
     }
     if (pass == 0) h = v / ((double)out[1] * nm.sd[1] + nm.mu[1]);
   }
+  const double vt = (double)out[0] * nm.sd[0] + nm.mu[0], vm = (double)out[2] * nm.sd[2] + nm.mu[2],
+               vk = (double)out[3] * nm.sd[3] + nm.mu[3];
   if (oh) oh[c] = h;
-  if (ot) ot[c] = (double)out[0] * nm.sd[0] + nm.mu[0];
+  if (ot) ot[c] = vt;
   if (od) od[c] = (double)out[1] * nm.sd[1] + nm.mu[1];
-  if (om) om[c] = (double)out[2] * nm.sd[2] + nm.mu[2];
-  if (ok) ok[c] = (double)out[3] * nm.sd[3] + nm.mu[3];
+  if (om) om[c] = vm;
+  if (ok) ok[c] = vk;
+  if (fh) fh[c] = (float)h;
+  if (fm) fm[c] = (float)vm;
+  if (fk) fk[c] = (float)vk;
+  if (ft) ft[c] = (float)vt;
 }
 
 }  // namespace
 
 namespace chs {
 std::string mlp_refresh_dh(chs_mlp* m, const chs_props* p, const double* rhoh, long n, double* h, double* d,
-                           double* mu, double* k, double* t, int* flag, hipStream_t s) {
+                           double* mu, double* k, double* t, int* flag, hipStream_t s, float* const* r32) {
   if (n <= 0) return "";
+  float* const f[4] = {r32 ? r32[0] : nullptr, r32 ? r32[1] : nullptr, r32 ? r32[2] : nullptr,
+                       r32 ? r32[3] : nullptr};
   if (m->matrix_cores)
     hipLaunchKernelGGL(k_mlp_from_dh_mfma, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->p16, m->nm, p->dev,
-                       p->n, p->dhmin, p->dhmax, rhoh, n, h, d, mu, k, t, flag);
+                       p->n, p->dhmin, p->dhmax, rhoh, n, h, d, mu, k, t, flag, f[0], f[1], f[2], f[3]);
   else
     hipLaunchKernelGGL(k_mlp_from_dh, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->p16, m->nm, p->dev,
-                       p->n, p->dhmin, p->dhmax, rhoh, n, h, d, mu, k, t, flag);
+                       p->n, p->dhmin, p->dhmax, rhoh, n, h, d, mu, k, t, flag, f[0], f[1], f[2], f[3]);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("surrogate kernel: ") + hipGetErrorString(e);
   return "";

@@ -41,11 +41,12 @@ namespace chs {
 // surrogate the same clamp and bisection on rho*h for the first h and the
 // MLP for the properties from h (twice, as the table path).  Outputs may be
 // null; the out-of-range count is added to *flag (device int).  With r32,
-// the table path also stores h, m, k, t rounded to FP32 into r32[0..3]
-// (each possibly null); the surrogate path leaves r32 to the caller.
+// both paths also store h, m, k, t rounded to FP32 into r32[0..3] (each
+// possibly null)
 std::string props_refresh_dh(const chs_props* p, int mode, const double* rhoh, long n, double* h, double* d,
                              double* m, double* k, double* t, int* flag, hipStream_t s,
                              float* const* r32 = nullptr);
 std::string mlp_refresh_dh(chs_mlp* m, const chs_props* p, const double* rhoh, long n, double* h, double* d,
-                           double* mu, double* k, double* t, int* flag, hipStream_t s);
+                           double* mu, double* k, double* t, int* flag, hipStream_t s,
+                           float* const* r32 = nullptr);
 }  // namespace chs

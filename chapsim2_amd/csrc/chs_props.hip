@@ -238,7 +238,7 @@ std::string props_refresh_dh(const chs_props* p, int mode, const double* rhoh, l
   if (n <= 0) return "";
   if (mode == CHS_PROPS_MLP) {
     if (!p->surrogate) return "properties: no surrogate attached to the table";
-    return mlp_refresh_dh(p->surrogate, p, rhoh, n, h, d, m, k, t, flag, s);
+    return mlp_refresh_dh(p->surrogate, p, rhoh, n, h, d, m, k, t, flag, s, r32);
   }
   if (mode != CHS_PROPS_TABLE) return "properties: unknown mode";
   double* const o[7] = {h, d, m, k, t, nullptr, nullptr};

@@ -205,6 +205,50 @@ def test_thermo_surrogate_mode_vs_table(po):
         dm.close()
 
 
+@pytest.mark.parametrize("matrix_cores,tol", [(0, 3e-5), (1, 1e-3)])
+def test_thermo_surrogate_mixed_vs_fp64(po, matrix_cores, tol):
+    """The surrogate mode in the mixed form, whose refresh kernel stores the
+    FP32 copies of h, mu, kappa and T that the FP32 right-hand sides read,
+    against the surrogate mode in FP64 with the same trained network: the
+    two differ by the FP32 right-hand sides, h by ~1e-7 after 10 RK3 steps.
+    With FP32 activations (the vector kernel) the network is smooth in h
+    and the properties and velocities stay within 3e-5; with BF16
+    activations (the matrix cores) a change of h that small can move an
+    activation across a BF16 rounding boundary, a jump of up to ~1e-4 in a
+    property (6e-5 measured), so the bound there is 1e-3.  Both lie above
+    FP64 round-off; a swapped or missing FP32 copy puts the runs O(1)
+    apart."""
+    from chapsim2_amd import _lib
+    from chapsim2_amd.domain import PREC_MIXED, PROJ_EXACT, Domain
+
+    dm, of, keep = _pair(po, (32, 24, 16), 19, precision=PREC_MIXED, mode=1, surrogate_steps=1500)
+    try:
+        pt, sur = keep
+        _lib.check(_lib.load().chs_mlp_set_matrix_cores(sur.h, matrix_cores))
+        tab, fields, rhoh, tw, t0 = _initial(po, (32, 24, 16), 19)
+        dm.thermo_set_rhoh(rhoh)  # initial properties and g = rho q again, with this kernel
+        ref = Domain((32, 24, 16), LX, -1.0, 1.0, LZ, projection=PROJ_EXACT, **CFG)
+        ref.set_state(*fields)
+        ref.thermo_enable(pt, tw, 1.0, t0, 1)
+        ref.thermo_set_rhoh(rhoh)
+        for x in (dm, ref):
+            x.update_re(0)
+            x.substep(0)
+            x.run(0, 10)
+        a, b = dm.thermo_get(), ref.thermo_get()
+        errs = {k: rel(a[k], b[k]) for k in ("h", "t", "d", "m", "k")}
+        errs.update({k: rel(x, y) for k, x, y in zip(("qx", "qy", "qz"), dm.get_state()[:3], ref.get_state()[:3])})
+        print(f"surrogate mode (matrix cores {matrix_cores}), mixed vs FP64 after 10 steps:",
+              " ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+        assert max(errs.values()) < tol, errs
+        assert 1e-12 < max(errs[k] for k in ("qx", "qy", "qz")), errs
+        assert np.all(dm.mass_check() < 1e-12)
+        ref.close()
+    finally:
+        _lib.check(_lib.load().chs_mlp_set_matrix_cores(keep[1].h, 1))
+        dm.close()
+
+
 def test_thermo_full_size_vs_oracle(po):
     """256x128x256 heated channel: the initial sub-step and one RK3 step
     against the oracle at the benchmark's size."""
